@@ -1,0 +1,591 @@
+// ge_abi.hip -- the extern "C" entry points of include/ge_physics.h:
+// context lifecycle, device buffers, state transfer and the frame driver that
+// orders the stage kernels as updateScene() does (globalLogic.cpp:1155-1372).
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ge_internal.hpp"
+
+namespace ge {
+uint32_t radix_hist_entries(uint32_t cap);
+}
+
+using namespace ge;
+
+namespace {
+
+template <typename T>
+hipError_t dalloc(T *&p, size_t count) {
+    p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * count);
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int alloc_state(ge_ctx *c, State &s, uint32_t cap, bool sph) {
+    float **fs[] = {&s.x, &s.y, &s.vx, &s.vy, &s.ax, &s.ay, &s.pvx, &s.pvy, &s.m};
+    for (float **f : fs) GE_HIP_CHECK(c, dalloc(*f, cap));
+    GE_HIP_CHECK(c, dalloc(s.flags, cap));
+    GE_HIP_CHECK(c, dalloc(s.id, cap));
+    if (sph) {
+        float **gs[] = {&s.sph_mass, &s.rest_dens, &s.stiff, &s.visc, &s.cohesion, &s.dens, &s.press,
+                        &s.press_tmp, &s.pred_dens, &s.ppx, &s.ppy, &s.pfx, &s.pfy};
+        for (float **f : gs) GE_HIP_CHECK(c, dalloc(*f, cap));
+    }
+    return GE_OK;
+}
+
+void free_state(State &s) {
+    float **fs[] = {&s.x, &s.y, &s.vx, &s.vy, &s.ax, &s.ay, &s.pvx, &s.pvy, &s.m, &s.sph_mass, &s.rest_dens,
+                    &s.stiff, &s.visc, &s.cohesion, &s.dens, &s.press, &s.press_tmp, &s.pred_dens, &s.ppx,
+                    &s.ppy, &s.pfx, &s.pfy};
+    for (float **f : fs) dfree(*f);
+    dfree(s.flags);
+    dfree(s.id);
+}
+
+void free_all(ge_ctx *c) {
+    Buffers &b = c->b;
+    free_state(b.st[0]);
+    free_state(b.st[1]);
+    dfree(b.pxym); dfree(b.sort_a); dfree(b.sort_b); dfree(b.keys); dfree(b.perm); dfree(b.hist);
+    dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.bb); dfree(b.interactions);
+    TopNodes &T = b.top;
+    dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
+    dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset);
+    dfree(b.nodes); dfree(b.node_geom); dfree(b.sphf);
+    for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
+        dfree(g->keys); dfree(g->keys_alt); dfree(g->cell_start); dfree(g->cell_end); dfree(g->frame);
+    }
+}
+
+int alloc_all(ge_ctx *c) {
+    Buffers &b = c->b;
+    const uint32_t cap = c->cap;
+    if (int r = alloc_state(c, b.st[0], cap, c->sph)) return r;
+    if (int r = alloc_state(c, b.st[1], cap, c->sph)) return r;
+    GE_HIP_CHECK(c, dalloc(b.pxym, cap));
+    GE_HIP_CHECK(c, dalloc(b.sort_a, cap));
+    GE_HIP_CHECK(c, dalloc(b.sort_b, cap));
+    GE_HIP_CHECK(c, dalloc(b.keys, cap));
+    GE_HIP_CHECK(c, dalloc(b.perm, cap));
+    b.hist_cap = radix_hist_entries(cap);
+    GE_HIP_CHECK(c, dalloc(b.hist, b.hist_cap));
+    GE_HIP_CHECK(c, dalloc(b.partials, 4 * 4096));
+    GE_HIP_CHECK(c, dalloc(b.partials2, 4 * 4096));
+    GE_HIP_CHECK(c, dalloc(b.counters, kCntSlots));
+    GE_HIP_CHECK(c, hipMemset(b.counters, 0, sizeof(uint32_t) * kCntSlots));
+    GE_HIP_CHECK(c, dalloc(b.bb, 8));
+    GE_HIP_CHECK(c, dalloc(b.interactions, 1));
+    TopNodes &T = b.top;
+    T.cap = 2u * cap + 1024u;
+    GE_HIP_CHECK(c, dalloc(T.px, T.cap)); GE_HIP_CHECK(c, dalloc(T.py, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.size, T.cap)); GE_HIP_CHECK(c, dalloc(T.start, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.end, T.cap)); GE_HIP_CHECK(c, dalloc(T.parent, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.child, 4ull * T.cap)); GE_HIP_CHECK(c, dalloc(T.kind, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.count, T.cap)); GE_HIP_CHECK(c, dalloc(T.pending, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.accum, T.cap)); GE_HIP_CHECK(c, dalloc(T.pending2, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.offset, T.cap));
+    // Quadtree::root reserves 4N nodes (quadtree.cpp:94)
+    b.node_cap = 4u * cap + 1024u;
+    GE_HIP_CHECK(c, dalloc(b.nodes, b.node_cap));
+    GE_HIP_CHECK(c, dalloc(b.node_geom, b.node_cap));
+    if (c->sph) GE_HIP_CHECK(c, dalloc(b.sphf, 2ull * cap));
+    return GE_OK;
+}
+
+int fail(ge_ctx *c, int code, const char *msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int check_params(ge_ctx *c, const ge_step_params *p) {
+    if (!p) return fail(c, GE_E_INVALID, "params is NULL");
+    if (p->periodic) return fail(c, GE_E_UNSUPPORTED, "periodic boundary is not implemented (infiniteDomain only)");
+    if (p->temperature) return fail(c, GE_E_UNSUPPORTED, "temperature transfer is not implemented");
+    if (p->sph_ground) return fail(c, GE_E_UNSUPPORTED, "SPH ground walls are not implemented");
+    if (p->sph && !c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    if (p->sph && !(p->sph_cell > 0.0f && p->sph_h > 0.0f)) return fail(c, GE_E_INVALID, "sph_h and sph_cell must be > 0");
+    return GE_OK;
+}
+
+int check_device_errors(ge_ctx *c) {
+    uint32_t e = 0;
+    GE_HIP_CHECK(c, hipMemcpyAsync(&e, c->b.counters + kCntError, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (e) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s%s)", e,
+                      (e & kErrNodeCap) ? "node capacity " : "", (e & kErrDepth) ? "tree depth " : "",
+                      (e & kErrGridRange) ? "SPH grid range " : "", (e & kErrNeighborCap) ? "neighbour cap" : "");
+        c->err = buf;
+        uint32_t z = 0;
+        (void)hipMemcpy(c->b.counters + kCntError, &z, sizeof z, hipMemcpyHostToDevice);
+        return GE_E_DEVICE;
+    }
+    return GE_OK;
+}
+
+// Records a hipEvent pair around one phase when profiling is on.
+struct PhaseScope {
+    ge_ctx *c;
+    ge_ctx::PhaseEvent ev{};
+    bool on;
+    PhaseScope(ge_ctx *ctx, int phase) : c(ctx), on(ctx->prof) {
+        if (!on) return;
+        ev.phase = phase;
+        if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) {
+            on = false;
+            return;
+        }
+        (void)hipEventRecord(ev.a, c->stream);
+    }
+    ~PhaseScope() {
+        if (!on) return;
+        (void)hipEventRecord(ev.b, c->stream);
+        c->prof_events.push_back(ev);
+    }
+};
+
+bool use_direct(const ge_ctx *c, const ge_step_params *p) {
+    // globalLogic.cpp:1319-1348: naive flag, or N < 5000 on either code path
+    return p->naive || (int64_t)c->n < (int64_t)p->direct_below;
+}
+
+// One frame of updateScene() (globalLogic.cpp:1155-1372).
+int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) {
+    if (dt_start != 0.0f) {
+        // integrateStart + boundingBox, then computeMortonKeys, the sort and the tree
+        { PhaseScope s(c, GE_PHASE_INTEGRATE_BBOX); launch_integrate_start_bbox(c, dt_start, p->sph, p->sph_max_vel, 1); }
+        { PhaseScope s(c, GE_PHASE_KEYS); launch_morton_keys(c, c->b.bb); }
+        { PhaseScope s(c, GE_PHASE_SORT); radix_sort_packed(c, c->b.sort_a, c->b.sort_b, c->n, kIdxBits, kIdxBits + 36); }
+        { PhaseScope s(c, GE_PHASE_GATHER); launch_gather(c); }
+        if (!p->naive) {
+            PhaseScope s(c, GE_PHASE_TREE);
+            launch_tree_build(c, c->b.bb);
+            c->have_tree = true;
+        }
+    }
+    if (p->dt == 0.0f) return GE_OK;
+    if (p->sph) {
+        PhaseScope s(c, GE_PHASE_SPH_GRID);
+        if (int r = sph_new_grid(c, p)) return r;
+    }
+    const float end_kick = p->sph ? 0.0f : p->dt * 0.5f;  // fused integrateEnd when no SPH follows
+    if (p->G != 0.0f) {
+        PhaseScope s(c, GE_PHASE_GRAVITY);
+        if (use_direct(c, p) || !c->have_tree)
+            launch_direct(c, p, end_kick);
+        else
+            launch_bh(c, p, end_kick, count_inter);
+    } else {
+        launch_zero_acc(c);
+        if (!p->sph) {
+            PhaseScope s(c, GE_PHASE_INTEGRATE_END);
+            launch_integrate_end(c, p->dt);
+        }
+    }
+    if (p->sph) {
+        PhaseScope s(c, GE_PHASE_SPH);
+        if (int r = sph_pcisph(c, p, p->dt)) return r;
+    }
+    return GE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ge_version(void) { return "galaxy-step-mi355x 0.1 (gfx950)"; }
+
+void ge_default_config(ge_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->capacity = 1u << 20;
+    cfg->device = 0;
+    cfg->enable_sph = 0;
+}
+
+void ge_default_params(ge_step_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->G = 6.674e-11f;       // parameters.h:114 (times gravityMultiplier 1)
+    p->theta = 0.8f;         // parameters.h:117
+    p->softening = 1.5f;     // parameters.h:116
+    p->dt_start = 0.045f;    // fixedDeltaTime (parameters.h:129)
+    p->dt = 0.045f;
+    p->naive = 0;
+    p->direct_below = 5000;  // globalLogic.cpp:1323
+    p->sph = 0;
+    p->sph_h = 3.0f;         // SPH.h:17
+    p->sph_cell = 3.0f;      // neighborSearch.h:604
+    p->sph_unit_mass = 0.03f;  // parameters.h:311
+    p->viscosity = 0.3f;
+    p->cohesion_coef = 1.0f;
+    p->delta = 19000.0f;
+    p->stiff_mult = 1.0f;
+    p->sph_max_vel = 250.0f;   // parameters.h:118
+    p->compensated = 1;
+}
+
+int ge_create(ge_ctx **out, const ge_config *cfg) {
+    if (!out || !cfg) return GE_E_INVALID;
+    *out = nullptr;
+    if (cfg->capacity == 0 || cfg->capacity > kMaxParticles) return GE_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GE_E_NODEVICE;
+    if (cfg->device < 0 || cfg->device >= ndev) return GE_E_NODEVICE;
+    ge_ctx *c = new (std::nothrow) ge_ctx();
+    if (!c) return GE_E_NOMEM;
+    c->device = cfg->device;
+    c->cap = cfg->capacity;
+    c->sph = cfg->enable_sph != 0;
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return GE_E_HIP;
+    }
+    if (int r = alloc_all(c)) {
+        free_all(c);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return r == GE_E_HIP ? GE_E_NOMEM : r;
+    }
+    *out = c;
+    return GE_OK;
+}
+
+int ge_destroy(ge_ctx *c) {
+    if (!c) return GE_E_INVALID;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &e : c->prof_events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    free_all(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GE_OK;
+}
+
+const char *ge_last_error(const ge_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int ge_count(const ge_ctx *c, uint32_t *n) {
+    if (!c || !n) return GE_E_INVALID;
+    *n = c->n;
+    return GE_OK;
+}
+
+int ge_sync(ge_ctx *c) {
+    if (!c) return GE_E_INVALID;
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return check_device_errors(c);
+}
+
+int ge_upload(ge_ctx *c, uint32_t n, const ge_particles *p) {
+    if (!c || !p) return GE_E_INVALID;
+    if (n > c->cap) return fail(c, GE_E_CAPACITY, "particle count above capacity");
+    if (!p->x || !p->y || !p->mass) return fail(c, GE_E_INVALID, "x, y and mass are required");
+    (void)hipSetDevice(c->device);
+    c->b.cur = 0;
+    State &s = c->b.st[0];
+    c->n = n;
+    c->have_tree = false;
+    c->have_grid = false;
+    if (n == 0) return GE_OK;
+    const size_t fb = sizeof(float) * n;
+    auto put = [&](float *dst, const float *src, float fill) -> int {
+        if (!dst) return GE_OK;
+        if (src) {
+            GE_HIP_CHECK(c, hipMemcpyAsync(dst, src, fb, hipMemcpyHostToDevice, c->stream));
+        } else if (fill == 0.0f) {
+            GE_HIP_CHECK(c, hipMemsetAsync(dst, 0, fb, c->stream));
+        } else {
+            std::vector<float> tmp(n, fill);
+            GE_HIP_CHECK(c, hipMemcpyAsync(dst, tmp.data(), fb, hipMemcpyHostToDevice, c->stream));
+            GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        }
+        return GE_OK;
+    };
+    int r = 0;
+    if ((r = put(s.x, p->x, 0.f)) || (r = put(s.y, p->y, 0.f)) || (r = put(s.vx, p->vx, 0.f)) ||
+        (r = put(s.vy, p->vy, 0.f)) || (r = put(s.ax, p->ax, 0.f)) || (r = put(s.ay, p->ay, 0.f)) ||
+        (r = put(s.pvx, p->pvx, 0.f)) || (r = put(s.pvy, p->pvy, 0.f)) || (r = put(s.m, p->mass, 0.f)))
+        return r;
+    if (p->flags) {
+        GE_HIP_CHECK(c, hipMemcpyAsync(s.flags, p->flags, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+    } else {
+        GE_HIP_CHECK(c, hipMemsetAsync(s.flags, 0, sizeof(uint32_t) * n, c->stream));
+    }
+    if (p->id) {
+        GE_HIP_CHECK(c, hipMemcpyAsync(s.id, p->id, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+    } else {
+        std::vector<uint32_t> ids(n);
+        for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+        GE_HIP_CHECK(c, hipMemcpyAsync(s.id, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+        GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    }
+    if (c->sph) {
+        if ((r = put(s.sph_mass, p->sph_mass, 1.f)) || (r = put(s.rest_dens, p->rest_dens, 0.f)) ||
+            (r = put(s.stiff, p->stiff, 0.f)) || (r = put(s.visc, p->visc, 0.f)) ||
+            (r = put(s.cohesion, p->cohesion, 0.f)) || (r = put(s.dens, p->dens, 0.f)) ||
+            (r = put(s.press, p->press, 0.f)) || (r = put(s.press_tmp, p->press_tmp, 0.f)) ||
+            (r = put(s.pred_dens, p->pred_dens, 0.f)) || (r = put(s.ppx, p->ppx, 0.f)) ||
+            (r = put(s.ppy, p->ppy, 0.f)) || (r = put(s.pfx, p->pfx, 0.f)) || (r = put(s.pfy, p->pfy, 0.f)))
+            return r;
+    }
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+int ge_download(ge_ctx *c, const ge_particles *p) {
+    if (!c || !p) return GE_E_INVALID;
+    (void)hipSetDevice(c->device);
+    const uint32_t n = c->n;
+    if (n == 0) return GE_OK;
+    const State &s = c->b.st[c->b.cur];
+    const size_t fb = sizeof(float) * n;
+    auto get = [&](float *dst, const float *src) -> int {
+        if (dst && src) GE_HIP_CHECK(c, hipMemcpyAsync(dst, src, fb, hipMemcpyDeviceToHost, c->stream));
+        return GE_OK;
+    };
+    int r = 0;
+    if ((r = get(p->x, s.x)) || (r = get(p->y, s.y)) || (r = get(p->vx, s.vx)) || (r = get(p->vy, s.vy)) ||
+        (r = get(p->ax, s.ax)) || (r = get(p->ay, s.ay)) || (r = get(p->pvx, s.pvx)) || (r = get(p->pvy, s.pvy)) ||
+        (r = get(p->mass, s.m)))
+        return r;
+    if (p->flags) GE_HIP_CHECK(c, hipMemcpyAsync(p->flags, s.flags, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    if (p->id) GE_HIP_CHECK(c, hipMemcpyAsync(p->id, s.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    if (c->sph) {
+        if ((r = get(p->sph_mass, s.sph_mass)) || (r = get(p->rest_dens, s.rest_dens)) || (r = get(p->stiff, s.stiff)) ||
+            (r = get(p->visc, s.visc)) || (r = get(p->cohesion, s.cohesion)) || (r = get(p->dens, s.dens)) ||
+            (r = get(p->press, s.press)) || (r = get(p->press_tmp, s.press_tmp)) ||
+            (r = get(p->pred_dens, s.pred_dens)) || (r = get(p->ppx, s.ppx)) || (r = get(p->ppy, s.ppy)) ||
+            (r = get(p->pfx, s.pfx)) || (r = get(p->pfy, s.pfy)))
+            return r;
+    }
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return check_device_errors(c);
+}
+
+int ge_step(ge_ctx *c, const ge_step_params *p, int32_t nsteps) {
+    if (!c) return GE_E_INVALID;
+    if (int r = check_params(c, p)) return r;
+    if (nsteps < 0) return fail(c, GE_E_INVALID, "nsteps < 0");
+    if (c->n == 0 || nsteps == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    for (int32_t k = 0; k < nsteps; ++k) {
+        const float dt0 = (k == 0) ? p->dt_start : p->dt;
+        if (int r = frame(c, p, dt0, false)) return r;
+    }
+    GE_HIP_CHECK(c, hipGetLastError());
+    return GE_OK;
+}
+
+int ge_bounding_box(ge_ctx *c, float out_bb[3]) {
+    if (!c || !out_bb) return GE_E_INVALID;
+    if (c->n == 0) return fail(c, GE_E_INVALID, "no particles");
+    (void)hipSetDevice(c->device);
+    launch_integrate_start_bbox(c, 0.0f, 0, 0.0f, 0);
+    float h[4];
+    GE_HIP_CHECK(c, hipMemcpyAsync(h, c->b.bb, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    out_bb[0] = h[0]; out_bb[1] = h[1]; out_bb[2] = h[2];
+    return GE_OK;
+}
+
+int ge_compute_morton_keys(ge_ctx *c, const float bb[3]) {
+    if (!c || !bb) return GE_E_INVALID;
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    float h[4] = {bb[0], bb[1], bb[2], 0.0f};
+    GE_HIP_CHECK(c, hipMemcpyAsync(c->b.bb, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    launch_morton_keys(c, c->b.bb);
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+int ge_sort_by_morton_key(ge_ctx *c) {
+    if (!c) return GE_E_INVALID;
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    radix_sort_packed(c, c->b.sort_a, c->b.sort_b, c->n, kIdxBits, kIdxBits + 36);
+    launch_gather(c);
+    c->have_tree = false;
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+int ge_build_quadtree(ge_ctx *c, const float bb[3]) {
+    if (!c || !bb) return GE_E_INVALID;
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    float h[4] = {bb[0], bb[1], bb[2], 0.0f};
+    GE_HIP_CHECK(c, hipMemcpyAsync(c->b.bb, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    launch_tree_build(c, c->b.bb);
+    c->have_tree = true;
+    return ge_sync(c);
+}
+
+int ge_compute_gravity(ge_ctx *c, const ge_step_params *p) {
+    if (!c) return GE_E_INVALID;
+    if (int r = check_params(c, p)) return r;
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    if (p->G == 0.0f) {
+        launch_zero_acc(c);
+    } else if (use_direct(c, p) || !c->have_tree) {
+        launch_direct(c, p, 0.0f);
+    } else {
+        launch_bh(c, p, 0.0f, true);
+    }
+    return ge_sync(c);
+}
+
+int ge_integrate_start(ge_ctx *c, const ge_step_params *p) {
+    if (!c) return GE_E_INVALID;
+    if (int r = check_params(c, p)) return r;
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    launch_integrate_start_bbox(c, p->dt_start, p->sph, p->sph_max_vel, 1);
+    return ge_sync(c);
+}
+
+int ge_integrate_end(ge_ctx *c, const ge_step_params *p) {
+    if (!c) return GE_E_INVALID;
+    if (int r = check_params(c, p)) return r;
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    launch_integrate_end(c, p->dt);
+    return ge_sync(c);
+}
+
+int ge_sph_new_grid(ge_ctx *c, const ge_step_params *p) {
+    if (!c) return GE_E_INVALID;
+    if (int r = check_params(c, p)) return r;
+    if (!c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    // the grid frame uses the current bounding box
+    launch_integrate_start_bbox(c, 0.0f, 0, 0.0f, 0);
+    if (int r = sph_new_grid(c, p)) return r;
+    return ge_sync(c);
+}
+
+int ge_sph_pcisph(ge_ctx *c, const ge_step_params *p) {
+    if (!c) return GE_E_INVALID;
+    if (int r = check_params(c, p)) return r;
+    if (!c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    if (!c->have_grid) return fail(c, GE_E_INVALID, "ge_sph_new_grid must run first");
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    if (int r = sph_pcisph(c, p, 0.0f)) return r;
+    return ge_sync(c);
+}
+
+int ge_get_keys(ge_ctx *c, uint64_t *keys) {
+    if (!c || !keys) return GE_E_INVALID;
+    if (c->n == 0) return GE_OK;
+    GE_HIP_CHECK(c, hipMemcpyAsync(keys, c->b.keys, sizeof(uint64_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+int ge_get_perm(ge_ctx *c, uint32_t *perm) {
+    if (!c || !perm) return GE_E_INVALID;
+    if (c->n == 0) return GE_OK;
+    GE_HIP_CHECK(c, hipMemcpyAsync(perm, c->b.perm, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+int ge_get_node_count(ge_ctx *c, uint32_t *count) {
+    if (!c || !count) return GE_E_INVALID;
+    *count = 0;
+    if (!c->have_tree || c->n == 0) return GE_OK;
+    GE_HIP_CHECK(c, hipMemcpyAsync(count, c->b.counters + kCntNodeTotal, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return check_device_errors(c);
+}
+
+int ge_get_nodes(ge_ctx *c, ge_node *out, uint32_t cap) {
+    if (!c || !out) return GE_E_INVALID;
+    uint32_t nn = 0;
+    if (int r = ge_get_node_count(c, &nn)) return r;
+    if (nn > cap) return fail(c, GE_E_CAPACITY, "node buffer too small");
+    if (!c->debug) return fail(c, GE_E_INVALID, "node geometry not kept");
+    std::vector<NodeRec> rec(nn);
+    std::vector<float4> geo(nn);
+    if (nn) {
+        GE_HIP_CHECK(c, hipMemcpy(rec.data(), c->b.nodes, sizeof(NodeRec) * nn, hipMemcpyDeviceToHost));
+        GE_HIP_CHECK(c, hipMemcpy(geo.data(), c->b.node_geom, sizeof(float4) * nn, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < nn; ++i) {
+        ge_node &o = out[i];
+        o.posx = geo[i].x; o.posy = geo[i].y; o.size = geo[i].z;
+        o.comx = rec[i].comx; o.comy = rec[i].comy; o.mass = rec[i].mass;
+        o.start = rec[i].start; o.end = rec[i].end; o.next = rec[i].next;
+        uint32_t ch = i + 1;
+        for (int q = 0; q < 4; ++q) {
+            if (rec[i].child_mask & (1u << q)) {
+                o.child[q] = ch;
+                ch += (ch < nn ? rec[ch].next : 0u) + 1;
+            } else {
+                o.child[q] = kNone;
+            }
+        }
+    }
+    return GE_OK;
+}
+
+int ge_get_interactions(ge_ctx *c, uint64_t *count) {
+    if (!c || !count) return GE_E_INVALID;
+    GE_HIP_CHECK(c, hipMemcpyAsync(count, c->b.interactions, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+int ge_profile_enable(ge_ctx *c, int enable) {
+    if (!c) return GE_E_INVALID;
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (auto &e : c->prof_events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    c->prof_events.clear();
+    c->prof = enable != 0;
+    return GE_OK;
+}
+
+int ge_profile_read(ge_ctx *c, double ms[GE_NPHASES], uint32_t launches[GE_NPHASES]) {
+    if (!c || !ms || !launches) return GE_E_INVALID;
+    for (int k = 0; k < GE_NPHASES; ++k) { ms[k] = 0.0; launches[k] = 0; }
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    for (auto &e : c->prof_events) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, e.a, e.b) == hipSuccess && e.phase >= 0 && e.phase < GE_NPHASES) {
+            ms[e.phase] += t;
+            launches[e.phase] += 1;
+        }
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    c->prof_events.clear();
+    return GE_OK;
+}
+
+int ge_get_neighbors(ge_ctx *c, int32_t which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total) {
+    if (!c || !off || !total || (which != 0 && which != 1)) return GE_E_INVALID;
+    if (!c->sph || !c->have_grid) return fail(c, GE_E_INVALID, "no SPH grid");
+    (void)hipSetDevice(c->device);
+    return sph_neighbors(c, which, off, idx, cap, total);
+}
+
+}  // extern "C"

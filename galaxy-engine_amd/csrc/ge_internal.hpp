@@ -1,0 +1,183 @@
+// ge_internal.hpp -- device-side layout and launchers shared by the HIP
+// translation units of libge_physics.so.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ge_physics.h"
+
+namespace ge {
+
+constexpr int kBlock = 256;            // threads per workgroup for streaming kernels
+constexpr int kIdxBits = 28;           // low bits of a packed sort word hold the slot
+constexpr uint64_t kIdxMask = (1ull << kIdxBits) - 1;
+constexpr uint32_t kMaxParticles = 1u << kIdxBits;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Tree build: levels built one launch each, then whole subtrees per thread.
+constexpr int kTopLevels = 12;         // launches of the level-synchronous phase
+constexpr uint32_t kSubtreeMax = 32;   // nodes with at most this many particles finish in-thread
+constexpr int kMaxDepth = 48;          // in-thread recursion depth below a subtree root
+
+// device error flag bits
+enum : uint32_t {
+    kErrNodeCap = 1u,    // final node array too small
+    kErrDepth = 2u,      // in-thread recursion deeper than kMaxDepth
+    kErrGridRange = 4u,  // SPH grid cannot index the coordinates
+    kErrNeighborCap = 8u,
+};
+
+// counters block (device, uint32 each)
+enum CounterSlot : int {
+    kCntError = 0,
+    kCntBboxTicket = 1,
+    kCntPredTicket = 2,
+    kCntNodeTotal = 3,      // final node count (root subtree size)
+    kCntTopTotal = 4,       // number of top nodes
+    kCntLevel0 = 8,         // kCntLevel0 + L: nodes allocated on level L
+    kCntLevelBase0 = 8 + 64,  // first top index of level L
+    kCntSlots = 8 + 128,
+};
+
+// One buffer set of particle state (structure of arrays).
+struct State {
+    float *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr;
+    float *ax = nullptr, *ay = nullptr, *pvx = nullptr, *pvy = nullptr, *m = nullptr;
+    uint32_t *flags = nullptr, *id = nullptr;
+    float *sph_mass = nullptr, *rest_dens = nullptr, *stiff = nullptr, *visc = nullptr;
+    float *cohesion = nullptr, *dens = nullptr, *press = nullptr, *press_tmp = nullptr;
+    float *pred_dens = nullptr, *ppx = nullptr, *ppy = nullptr, *pfx = nullptr, *pfy = nullptr;
+};
+
+// Node record read by the traversal: 32 bytes, one scalar load per visit.
+struct alignas(16) NodeRec {
+    float comx, comy, mass, size_sq;
+    uint32_t next, start, end, child_mask;
+};
+
+// Top part of the tree, built level by level (structure of arrays).
+struct TopNodes {
+    float *px = nullptr, *py = nullptr, *size = nullptr;
+    uint32_t *start = nullptr, *end = nullptr, *parent = nullptr;
+    uint32_t *child = nullptr;     // 4 per node
+    uint32_t *kind = nullptr;      // 0 internal, 1 leaf, 2 in-thread subtree
+    uint32_t *count = nullptr;     // nodes in the subtree (final after the level phase)
+    uint32_t *pending = nullptr;   // children still to report their count
+    uint32_t *accum = nullptr;     // sum of reported child counts
+    uint32_t *pending2 = nullptr;  // children still to report their mass
+    uint32_t *offset = nullptr;    // pre-order index in the final array
+    uint32_t cap = 0;
+};
+
+// SPH uniform grid (dense cell table over the occupied cell range).
+struct Grid {
+    uint64_t *keys = nullptr;      // packed (cell << 28 | slot), sorted
+    uint64_t *keys_alt = nullptr;
+    uint32_t *cell_start = nullptr;  // per cell: first sorted position
+    uint32_t *cell_end = nullptr;    // per cell: one past the last
+    int32_t *frame = nullptr;        // device: {cx0, cy0, W, H} of the table
+    uint64_t cells_cap = 0;
+};
+
+struct Buffers {
+    State st[2];
+    int cur = 0;
+    float4 *pxym = nullptr;        // sorted (x, y, m, 0) for the traversal
+    uint64_t *sort_a = nullptr, *sort_b = nullptr;  // packed sort words
+    uint64_t *keys = nullptr;      // 36-bit Morton key per slot (pre-sort order)
+    uint32_t *perm = nullptr;      // last permutation
+    uint32_t *hist = nullptr;      // radix histogram table
+    uint32_t hist_cap = 0;
+    float *partials = nullptr;     // bbox partial min/max, 4 per block
+    float *partials2 = nullptr;    // second reduction (predPos)
+    uint32_t *counters = nullptr;  // CounterSlot
+    float *bb = nullptr;           // device bbox {x, y, size, _} and predPos box at +4
+    uint64_t *interactions = nullptr;
+    TopNodes top;
+    NodeRec *nodes = nullptr;
+    float4 *node_geom = nullptr;   // debug: (posx, posy, size, 0) per final node
+    uint32_t node_cap = 0;
+    float *sphf = nullptr;         // SPH force scratch (x, y interleaved), 2 per particle
+    Grid grid_pos, grid_pred;
+};
+
+}  // namespace ge
+
+struct ge_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cap = 0;
+    uint32_t n = 0;
+    bool sph = false;
+    bool debug = true;             // keep node geometry for ge_get_nodes
+    bool have_tree = false;
+    bool have_grid = false;
+    ge::Buffers b;
+    std::string err;
+    // phase timing (ge_profile_enable)
+    bool prof = false;
+    struct PhaseEvent { int phase; hipEvent_t a, b; };
+    std::vector<PhaseEvent> prof_events;
+};
+
+namespace ge {
+
+// ---- launchers (each defined next to its kernels) ----
+// sort.hip
+void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n,
+                       int lo_bit, int hi_bit);
+// step.hip
+void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate);
+void launch_morton_keys(ge_ctx *c, const float *bb_dev);
+void launch_gather(ge_ctx *c);
+void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_dt, bool count);
+void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
+void launch_zero_acc(ge_ctx *c);
+void launch_integrate_end(ge_ctx *c, float dt);
+// tree.hip
+void launch_tree_build(ge_ctx *c, const float *bb_dev);
+// sph.hip
+int sph_new_grid(ge_ctx *c, const ge_step_params *p);
+int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
+int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total);
+
+inline uint32_t blocks_for(uint64_t n, int per_block) {
+    uint64_t b = (n + per_block - 1) / per_block;
+    return b == 0 ? 1u : (uint32_t)b;
+}
+
+// glm::min / glm::max scalar forms used by the reference bbox
+__device__ __forceinline__ float glm_min(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float glm_max(float a, float b) { return (a < b) ? b : a; }
+
+// XCD-aware block order: consecutive logical blocks (neighbouring Morton
+// ranges) are placed on the same XCD so its L2 sees one spatial region.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
+    constexpr uint32_t kXcd = 8;
+    uint32_t per = nblocks / kXcd;
+    uint32_t full = per * kXcd;
+    if (bid >= full) return bid;  // tail blocks keep their id
+    return (bid % kXcd) * per + bid / kXcd;
+}
+
+__device__ __forceinline__ uint32_t atomic_load_u32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float atomic_load_f32(const float *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace ge
+
+#define GE_HIP_CHECK(c, expr)                                                  \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            (c)->err = std::string(#expr) + ": " + hipGetErrorString(e_);      \
+            return GE_E_HIP;                                                   \
+        }                                                                      \
+    } while (0)

@@ -1,0 +1,414 @@
+// ge_step.hip -- per-frame streaming kernels of the 2D step:
+//   integrateStart fused with the bounding-box reduction, Morton keys, the
+//   Morton-order gather, the wave-coherent Barnes-Hut walk (with the
+//   integrateEnd kick fused in), the direct sum, and integrateEnd.
+#include <cfloat>
+
+#include "ge_internal.hpp"
+
+namespace ge {
+namespace {
+
+constexpr int kWave = 64;
+constexpr uint32_t kBboxMaxBlocks = 2048;
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = glm_min(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = glm_max(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Block-wide min/max of four values; result valid in thread 0.
+__device__ __forceinline__ void block_minmax(float &a, float &b, float &c, float &d) {
+    __shared__ float red[4][kBlock / kWave];
+    a = wave_min(a); b = wave_min(b); c = wave_max(c); d = wave_max(d);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[0][w] = a; red[1][w] = b; red[2][w] = c; red[3][w] = d; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / kWave; ++k) {
+            a = glm_min(a, red[0][k]); b = glm_min(b, red[1][k]);
+            c = glm_max(c, red[2][k]); d = glm_max(d, red[3][k]);
+        }
+    }
+}
+
+// Physics::integrateStart (physics.cpp:1069-1112) followed by the
+// boundingBox() reduction (globalLogic.cpp:1083-1099) over the drifted
+// positions.  The last workgroup to finish turns the partials into {x, y, size}.
+__global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, uint32_t n, float dt, int sph_clamp,
+                                                           float max_vel, int do_integrate,
+                                                           float *__restrict__ partials,
+                                                           uint32_t *__restrict__ ticket,
+                                                           float *__restrict__ bb) {
+#pragma clang fp contract(off)
+    const float half_dt = dt * 0.5f;
+    const float max_vsq = max_vel * max_vel;
+    float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        float x = s.x[i], y = s.y[i];
+        if (do_integrate && !(s.flags[i] & GE_FLAG_PINNED)) {
+            float pvx = s.vx[i], pvy = s.vy[i];
+            float vx = pvx + s.ax[i] * half_dt;
+            float vy = pvy + s.ay[i] * half_dt;
+            if (sph_clamp) {
+                float vsq = vx * vx + vy * vy;
+                if (vsq > max_vsq) {
+                    float psq = pvx * pvx + pvy * pvy;
+                    if (psq > 0.00001f) {
+                        float inv = max_vel / sqrtf(psq);
+                        pvx = pvx * inv;
+                        pvy = pvy * inv;
+                    }
+                    float inv = max_vel / sqrtf(vsq);
+                    vx = vx * inv;
+                    vy = vy * inv;
+                }
+            }
+            s.pvx[i] = pvx; s.pvy[i] = pvy;
+            s.vx[i] = vx; s.vy[i] = vy;
+            x = x + vx * dt;
+            y = y + vy * dt;
+            s.x[i] = x; s.y[i] = y;
+        }
+        mnx = glm_min(mnx, x); mny = glm_min(mny, y);
+        mxx = glm_max(mxx, x); mxy = glm_max(mxy, y);
+    }
+    block_minmax(mnx, mny, mxx, mxy);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        float4 v = make_float4(mnx, mny, mxx, mxy);
+        reinterpret_cast<float4 *>(partials)[blockIdx.x] = v;
+        __threadfence();
+        uint32_t t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    mnx = FLT_MAX; mny = FLT_MAX; mxx = -FLT_MAX; mxy = -FLT_MAX;
+    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) {
+        const float *pp = partials + 4 * k;
+        mnx = glm_min(mnx, atomic_load_f32(pp + 0));
+        mny = glm_min(mny, atomic_load_f32(pp + 1));
+        mxx = glm_max(mxx, atomic_load_f32(pp + 2));
+        mxy = glm_max(mxy, atomic_load_f32(pp + 3));
+    }
+    block_minmax(mnx, mny, mxx, mxy);
+    if (threadIdx.x == 0) {
+        float size = glm_max(mxx - mnx, mxy - mny);
+        float cx = (mnx + mxx) * 0.5f;
+        float cy = (mny + mxy) * 0.5f;
+        float half = size * 0.5f;
+        bb[0] = cx - half;
+        bb[1] = cy - half;
+        bb[2] = size;
+        bb[3] = 0.0f;
+        *ticket = 0;  // ready for the next frame
+    }
+}
+
+// Morton::scaleToGrid / spreadBits / morton2D / computeMortonKeys
+// (morton.cpp:3-38).  Writes the 36-bit key per slot and the packed sort word.
+__device__ __forceinline__ uint64_t scale_to_grid(float pos, float lo, float hi) {
+#pragma clang fp contract(off)
+    if (hi <= lo) return 0;
+    float c = (pos < lo) ? lo : ((hi < pos) ? hi : pos);
+    float norm = (c - lo) / (hi - lo);
+    uint64_t s = (uint64_t)(norm * 262144.0f);
+    return s < 262143u ? s : 262143u;
+}
+
+__device__ __forceinline__ uint64_t spread_bits(uint64_t v) {
+    v &= 0x3FFFFull;
+    v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+    v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+    v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton_keys(const float *__restrict__ x,
+                                                        const float *__restrict__ y, uint32_t n,
+                                                        const float *__restrict__ bb,
+                                                        uint64_t *__restrict__ keys,
+                                                        uint64_t *__restrict__ packed) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float bx = bb[0], by = bb[1], bs = bb[2];
+    const float ext = bs < 1e-6f ? 1e-6f : bs;
+    const float maxx = bx + ext, maxy = by + ext;
+    uint64_t ix = scale_to_grid(x[i], bx, maxx);
+    uint64_t iy = scale_to_grid(y[i], by, maxy);
+    uint64_t k = spread_bits(ix) | (spread_bits(iy) << 1);
+    keys[i] = k;
+    packed[i] = (k << kIdxBits) | i;
+}
+
+// Morton::sortParticlesByMortonKey's gather (morton.cpp:56-62) for every
+// SoA field, plus the packed (x, y, m) stream the traversal reads.
+__global__ __launch_bounds__(kBlock) void k_gather(State src, State dst, uint32_t n, int sph,
+                                                   const uint64_t *__restrict__ sorted,
+                                                   uint32_t *__restrict__ perm,
+                                                   float4 *__restrict__ pxym) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = (uint32_t)(sorted[i] & kIdxMask);
+    perm[i] = j;
+    float x = src.x[j], y = src.y[j], m = src.m[j];
+    dst.x[i] = x; dst.y[i] = y; dst.m[i] = m;
+    dst.vx[i] = src.vx[j]; dst.vy[i] = src.vy[j];
+    dst.ax[i] = src.ax[j]; dst.ay[i] = src.ay[j];
+    dst.pvx[i] = src.pvx[j]; dst.pvy[i] = src.pvy[j];
+    dst.flags[i] = src.flags[j]; dst.id[i] = src.id[j];
+    pxym[i] = make_float4(x, y, m, 0.0f);
+    if (sph) {
+        dst.sph_mass[i] = src.sph_mass[j]; dst.rest_dens[i] = src.rest_dens[j];
+        dst.stiff[i] = src.stiff[j]; dst.visc[i] = src.visc[j];
+        dst.cohesion[i] = src.cohesion[j]; dst.dens[i] = src.dens[j];
+        dst.press[i] = src.press[j]; dst.press_tmp[i] = src.press_tmp[j];
+        dst.pred_dens[i] = src.pred_dens[j];
+        dst.ppx[i] = src.ppx[j]; dst.ppy[i] = src.ppy[j];
+        dst.pfx[i] = src.pfx[j]; dst.pfy[i] = src.pfy[j];
+    }
+}
+
+__device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
+    float y = v - comp;
+    float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// Barnes-Hut accelerations: the per-particle pre-order walk of
+// Physics::calculateForceFromGrid (physics.cpp:91-205), executed by a whole
+// wave in lockstep.  Every lane walks its own reference path; the wave visits
+// the union of the 64 paths.  A lane that accepted node k stays muted until
+// the walk leaves k's subtree (index k + next + 1).  Node reads are
+// wave-uniform (scalar loads).  The acceptance test uses the reference's
+// unfused arithmetic so the accepted-node sets match the CPU path.
+template <bool KAHAN, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
+                                               const NodeRec *__restrict__ nodes,
+                                               const uint32_t *__restrict__ counters, uint32_t n,
+                                               float G, float theta_sq, float soft_sq,
+                                               float *__restrict__ ax, float *__restrict__ ay,
+                                               float *__restrict__ vx, float *__restrict__ vy,
+                                               const uint32_t *__restrict__ flags, float half_dt_end,
+                                               unsigned long long *__restrict__ inter) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * kBlock + threadIdx.x;
+    const bool live = i < n;
+    const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // a tree that raised an error flag may hold unwritten nodes: do not walk it
+    const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth);
+    const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
+    uint32_t skip_until = live ? 0u : kNone;
+    float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
+    uint32_t cnt = 0;
+    uint32_t idx = 0;
+    while (idx < node_count) {
+        const NodeRec nd = nodes[idx];
+        const uint32_t next = nd.next;
+        // index after this subtree, saturated so the walk always moves forward
+        const uint32_t after = (next >= node_count - idx) ? node_count : idx + next + 1;
+        if (nd.mass <= 0.0f) {  // empty node: skip the subtree (physics.cpp:107-110)
+            idx = after;
+            continue;
+        }
+        const bool active = idx >= skip_until;
+        float dx, dy, dist_sq;
+        bool accept;
+        {
+#pragma clang fp contract(off)
+            dx = nd.comx - p.x;
+            dy = nd.comy - p.y;
+            dist_sq = dx * dx + dy * dy + soft_sq;
+            accept = nd.size_sq < theta_sq * dist_sq;
+        }
+        if (active && accept) {
+            const float inv = __builtin_amdgcn_rsqf(dist_sq);
+            const float f = G * nd.mass * (inv * inv * inv);
+            if (KAHAN) {
+                kahan_add(fx, cx, dx * f);
+                kahan_add(fy, cy, dy * f);
+            } else {
+                fx = fmaf(dx, f, fx);
+                fy = fmaf(dy, f, fy);
+            }
+            if (COUNT) ++cnt;
+            skip_until = after;
+        }
+        const bool leaf = next == 0;
+        const bool open_leaf = active && !accept && leaf;
+        if (__ballot(open_leaf)) {
+            // leaf direct sum over [start, end), skipping i == j (physics.cpp:163-196)
+            const uint32_t jend = min(nd.end, n);
+            for (uint32_t j = nd.start; j < jend; ++j) {
+                const float4 q = pxym[j];
+                if (open_leaf && j != i) {
+                    const float px = q.x - p.x, py = q.y - p.y;
+                    const float d2 = px * px + py * py + soft_sq;
+                    const float inv = __builtin_amdgcn_rsqf(d2);
+                    const float f = G * q.z * (inv * inv * inv);
+                    if (KAHAN) {
+                        kahan_add(fx, cx, px * f);
+                        kahan_add(fy, cy, py * f);
+                    } else {
+                        fx = fmaf(px, f, fx);
+                        fy = fmaf(py, f, fy);
+                    }
+                    if (COUNT) ++cnt;
+                }
+            }
+        }
+        const bool descend = active && !accept && !leaf;
+        idx = __ballot(descend) ? idx + 1u : after;
+    }
+    if (COUNT) {
+        uint64_t c = cnt;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+        if ((threadIdx.x & 63) == 0) atomicAdd(inter, (unsigned long long)c);
+    }
+    if (!live) return;
+    const float accx = KAHAN ? fx - cx : fx;
+    const float accy = KAHAN ? fy - cy : fy;
+    ax[i] = accx;
+    ay[i] = accy;
+    if (half_dt_end != 0.0f && !(flags[i] & GE_FLAG_PINNED)) {
+#pragma clang fp contract(off)
+        vx[i] = vx[i] + accx * half_dt_end;
+        vy[i] = vy[i] + accy * half_dt_end;
+    }
+}
+
+// Physics::naiveGravity (physics.cpp:392-434): O(N^2) with the softening
+// added unsquared; sources staged through LDS one tile at a time.
+__global__ __launch_bounds__(kBlock) void k_direct(const float4 *__restrict__ pxym, uint32_t n, float G,
+                                                   float eps, float *__restrict__ ax,
+                                                   float *__restrict__ ay, float *__restrict__ vx,
+                                                   float *__restrict__ vy,
+                                                   const uint32_t *__restrict__ flags,
+                                                   float half_dt_end) {
+    __shared__ float4 tile[kBlock];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const float4 p = i < n ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float tx = 0.f, ty = 0.f;
+    for (uint32_t base = 0; base < n; base += kBlock) {
+        const uint32_t j = base + threadIdx.x;
+        tile[threadIdx.x] = j < n ? pxym[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        const uint32_t lim = min((uint32_t)kBlock, n - base);
+        for (uint32_t k = 0; k < lim; ++k) {
+            const float4 q = tile[k];
+            const float dx = q.x - p.x, dy = q.y - p.y;
+            const float d2 = dx * dx + dy * dy + eps;
+            const float inv = __builtin_amdgcn_rsqf(d2);
+            const float f = G * q.z * (inv * inv * inv);
+            tx = fmaf(dx, f, tx);
+            ty = fmaf(dy, f, ty);
+        }
+        __syncthreads();
+    }
+    if (i >= n) return;
+    ax[i] = tx;
+    ay[i] = ty;
+    if (half_dt_end != 0.0f && !(flags[i] & GE_FLAG_PINNED)) {
+#pragma clang fp contract(off)
+        vx[i] = vx[i] + tx * half_dt_end;
+        vy[i] = vy[i] + ty * half_dt_end;
+    }
+}
+
+__global__ void k_fill_u64(uint64_t *__restrict__ p, uint64_t v) { *p = v; }
+
+__global__ __launch_bounds__(kBlock) void k_zero2(float *__restrict__ a, float *__restrict__ b, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) { a[i] = 0.f; b[i] = 0.f; }
+}
+
+// Physics::integrateEnd (physics.cpp:1114-1125)
+__global__ __launch_bounds__(kBlock) void k_integrate_end(State s, uint32_t n, float half_dt) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n || (s.flags[i] & GE_FLAG_PINNED)) return;
+    s.vx[i] = s.vx[i] + s.ax[i] * half_dt;
+    s.vy[i] = s.vy[i] + s.ay[i] * half_dt;
+}
+
+}  // namespace
+
+void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate) {
+    uint32_t nb = blocks_for(c->n, kBlock);
+    if (nb > kBboxMaxBlocks) nb = kBboxMaxBlocks;
+    hipLaunchKernelGGL(k_integrate_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->b.st[c->b.cur], c->n, dt,
+                       sph_clamp, max_vel, do_integrate, c->b.partials, c->b.counters + kCntBboxTicket,
+                       c->b.bb);
+}
+
+void launch_morton_keys(ge_ctx *c, const float *bb_dev) {
+    const State &s = c->b.st[c->b.cur];
+    hipLaunchKernelGGL(k_morton_keys, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y,
+                       c->n, bb_dev, c->b.keys, c->b.sort_a);
+}
+
+void launch_gather(ge_ctx *c) {
+    const int cur = c->b.cur;
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.st[cur],
+                       c->b.st[1 - cur], c->n, (int)c->sph, c->b.sort_a, c->b.perm, c->b.pxym);
+    c->b.cur = 1 - cur;
+}
+
+void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
+    const State &s = c->b.st[c->b.cur];
+    const float theta_sq = p->theta * p->theta;
+    const float soft_sq = p->softening * p->softening;
+    const uint32_t nb = blocks_for(c->n, kBlock);
+    if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, c->stream, c->b.interactions, 0ull);
+    auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
+    if (p->compensated) {
+        if (count)
+            hipLaunchKernelGGL((k_bh<true, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
+                               c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
+                               end_kick_half_dt, inter);
+        else
+            hipLaunchKernelGGL((k_bh<true, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
+                               c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
+                               end_kick_half_dt, inter);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_bh<false, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
+                               c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
+                               end_kick_half_dt, inter);
+        else
+            hipLaunchKernelGGL((k_bh<false, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
+                               c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
+                               end_kick_half_dt, inter);
+    }
+}
+
+void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
+    const State &s = c->b.st[c->b.cur];
+    hipLaunchKernelGGL(k_direct, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.pxym, c->n,
+                       p->G, p->softening, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt);
+}
+
+void launch_zero_acc(ge_ctx *c) {
+    const State &s = c->b.st[c->b.cur];
+    hipLaunchKernelGGL(k_zero2, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, s.ax, s.ay, c->n);
+}
+
+void launch_integrate_end(ge_ctx *c, float dt) {
+    hipLaunchKernelGGL(k_integrate_end, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream,
+                       c->b.st[c->b.cur], c->n, dt * 0.5f);
+}
+
+}  // namespace ge

@@ -1,0 +1,195 @@
+/*
+ * ge_physics.h -- C ABI of the MI355X (gfx950) implementation of Galaxy
+ * Engine's 2D per-frame physics step.
+ *
+ * The reference exposes this path as C++ member functions that work in place
+ * on std::vector<ParticlePhysics> and a global node vector (citations are
+ * path:line under the reference tree, GE/ = GalaxyEngine/).  Each entry point
+ * below names the reference function it replaces.  Conventions:
+ *   - every call returns an int status: GE_OK (0) or a negative GE_E_* code;
+ *     ge_last_error() describes the last failure of a context;
+ *   - the context owns all device memory; callers own every host array;
+ *   - particle state lives in HBM between calls, stored structure-of-arrays;
+ *   - calls are ordered on the context's stream; ge_sync() waits for them;
+ *   - one context per host thread (contexts are independent).
+ *   - After any sort (ge_sort_by_morton_key, ge_step) particles are stored in
+ *     Morton order, as the reference leaves them (morton.cpp:61-62); the id
+ *     array travels with them so callers can map results back.
+ */
+#ifndef GE_PHYSICS_H
+#define GE_PHYSICS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GE_OK 0
+#define GE_E_INVALID (-1)     /* bad argument */
+#define GE_E_NOMEM (-2)       /* device or host allocation failed */
+#define GE_E_HIP (-3)         /* HIP runtime error */
+#define GE_E_UNSUPPORTED (-4) /* flag this build does not implement */
+#define GE_E_CAPACITY (-5)    /* particle count above the context capacity */
+#define GE_E_DEVICE (-6)      /* a kernel raised its error flag (see message) */
+#define GE_E_NODEVICE (-7)    /* no usable gfx950 device */
+#define GE_E_RANGE (-8)       /* coordinates outside what a grid can index */
+
+/* per-particle flag bits; the reference keeps these in ParticleRendering
+ * (GE/include/Particles/particle.h:75-140) */
+#define GE_FLAG_PINNED 1u
+#define GE_FLAG_SPH 2u
+#define GE_FLAG_DRAWN 4u
+#define GE_FLAG_DARK 8u
+
+typedef struct ge_ctx ge_ctx;
+
+typedef struct {
+    uint32_t capacity;  /* maximum particle count */
+    int32_t device;     /* HIP device ordinal */
+    int32_t enable_sph; /* allocate SPH state (needed for ge_step with sph=1) */
+    uint32_t reserved[5];
+} ge_config;
+
+/* Structure-of-arrays view of the ParticlePhysics fields on this path
+ * (GE/include/Particles/particle.h:5-73).  Upload reads the non-NULL arrays
+ * (NULL: zero, or id = index, sph_mass = 1); download writes them. */
+typedef struct {
+    float *x, *y;         /* pos */
+    float *vx, *vy;       /* vel */
+    float *ax, *ay;       /* acc */
+    float *pvx, *pvy;     /* prevVel */
+    float *mass;          /* mass */
+    uint32_t *flags;      /* GE_FLAG_* */
+    uint32_t *id;         /* id */
+    float *sph_mass, *rest_dens, *stiff, *visc, *cohesion, *dens;
+    float *press, *press_tmp, *pred_dens;
+    float *ppx, *ppy;     /* predPos */
+    float *pfx, *pfy;     /* pressF */
+} ge_particles;
+
+/* The physics fields of UpdateVariables on this path
+ * (GE/include/parameters.h:114-133,311-318). */
+typedef struct {
+    float G;              /* myVar.G */
+    float theta;          /* myVar.theta */
+    float softening;      /* myVar.softening */
+    float dt_start;       /* timeFactor seen by integrateStart (previous frame) */
+    float dt;             /* timeFactor for the rest of the frame */
+    int32_t naive;        /* myVar.naive: always use the direct sum */
+    int32_t direct_below; /* direct sum when N < this (globalLogic.cpp:1323: 5000) */
+    int32_t sph;          /* myVar.isSPHEnabled */
+    float sph_h;          /* SPH::radiusMultiplier (SPH.h:17) */
+    float sph_cell;       /* NeighborSearchV2AVX2::cellSize (neighborSearch.h:604) */
+    float sph_unit_mass;  /* myVar.mass */
+    float viscosity;      /* myVar.viscosity */
+    float cohesion_coef;  /* myVar.cohesionCoefficient */
+    float delta;          /* myVar.delta */
+    float stiff_mult;     /* myVar.stiffMultiplier */
+    float sph_max_vel;    /* myVar.sphMaxVel */
+    int32_t periodic;     /* isPeriodicBoundaryEnabled && !infiniteDomain: unsupported */
+    int32_t temperature;  /* isTempEnabled: unsupported */
+    int32_t sph_ground;   /* sphGround: unsupported */
+    int32_t compensated;  /* Kahan-compensated force accumulation (1 = on) */
+    uint32_t reserved[6];
+} ge_step_params;
+
+/* Quadtree node as laid out by the reference (GE/include/Physics/quadtree.h:
+ * 31-112) in DFS pre-order; child[q] uses q&1 -> +x, q&2 -> +y. */
+typedef struct {
+    float posx, posy, comx, comy, size, mass;
+    uint32_t start, end, next;
+    uint32_t child[4];
+} ge_node;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+void ge_default_config(ge_config *cfg);
+/* reference defaults: parameters.h:114-133 and 311-318, SPH.h:17,
+ * neighborSearch.h:604, globalLogic.cpp:1323 */
+void ge_default_params(ge_step_params *p);
+int ge_create(ge_ctx **out, const ge_config *cfg);
+int ge_destroy(ge_ctx *ctx);
+const char *ge_last_error(const ge_ctx *ctx);
+const char *ge_version(void);
+
+/* ---- state transfer ----------------------------------------------------- */
+/* replaces the caller filling myParam.pParticles (globalLogic.cpp:3) */
+int ge_upload(ge_ctx *ctx, uint32_t n, const ge_particles *p);
+/* replaces reading myParam.pParticles back; order is the device order */
+int ge_download(ge_ctx *ctx, const ge_particles *p);
+int ge_count(const ge_ctx *ctx, uint32_t *n);
+int ge_sync(ge_ctx *ctx);
+
+/* ---- one frame ---------------------------------------------------------- */
+/* updateScene()'s 2D physics ordering (GE/src/globalLogic.cpp:1155-1372):
+ * integrateStart, boundingBox, computeMortonKeys, sortParticlesByMortonKey,
+ * Quadtree, acc = 0, calculateForceFromGrid / naiveGravity, pcisphSolver,
+ * integrateEnd.  nsteps frames; the first uses dt_start for integrateStart. */
+int ge_step(ge_ctx *ctx, const ge_step_params *p, int32_t nsteps);
+
+/* ---- stage entry points (one per reference function) -------------------- */
+/* glm::vec3 boundingBox() -- GE/src/globalLogic.cpp:1083-1099 */
+int ge_bounding_box(ge_ctx *ctx, float out_bb[3]);
+/* Morton::computeMortonKeys -- GE/include/Physics/morton.h:17,
+ * GE/src/Physics/morton.cpp:27-38 */
+int ge_compute_morton_keys(ge_ctx *ctx, const float bb[3]);
+/* Morton::sortParticlesByMortonKey -- GE/include/Physics/morton.h:19-20,
+ * GE/src/Physics/morton.cpp:40-63 (stable by (key, index)) */
+int ge_sort_by_morton_key(ge_ctx *ctx);
+/* Quadtree::Quadtree / Quadtree::root -- GE/include/Physics/quadtree.h:118-128,
+ * GE/src/Physics/quadtree.cpp:5-107 */
+int ge_build_quadtree(ge_ctx *ctx, const float bb[3]);
+/* acc = 0 (globalLogic.cpp:1311-1314), then Physics::calculateForceFromGrid
+ * (physics.cpp:91-205) or naiveGravity (physics.cpp:392-434) by the reference's
+ * dispatch (globalLogic.cpp:1319-1348) */
+int ge_compute_gravity(ge_ctx *ctx, const ge_step_params *p);
+/* Physics::integrateStart / integrateEnd -- GE/include/Physics/physics.h:66-68,
+ * GE/src/Physics/physics.cpp:1069-1125 */
+int ge_integrate_start(ge_ctx *ctx, const ge_step_params *p);
+int ge_integrate_end(ge_ctx *ctx, const ge_step_params *p);
+/* NeighborSearchV2AVX2::newGridAVX2 -- GE/include/Particles/neighborSearch.h:636 */
+int ge_sph_new_grid(ge_ctx *ctx, const ge_step_params *p);
+/* SPH::pcisphSolver -- GE/include/Physics/SPH.h:623-632 */
+int ge_sph_pcisph(ge_ctx *ctx, const ge_step_params *p);
+
+/* ---- parity hooks ------------------------------------------------------- */
+/* Morton keys of the last ge_compute_morton_keys, indexed by particle slot */
+int ge_get_keys(ge_ctx *ctx, uint64_t *keys);
+/* permutation applied by the last sort: slot i took particle perm[i] */
+int ge_get_perm(ge_ctx *ctx, uint32_t *perm);
+/* node count and nodes of the last tree (reference layout, pre-order) */
+int ge_get_node_count(ge_ctx *ctx, uint32_t *count);
+int ge_get_nodes(ge_ctx *ctx, ge_node *nodes, uint32_t cap);
+/* BH interaction count (accepted nodes + leaf partners) of the last gravity */
+int ge_get_interactions(ge_ctx *ctx, uint64_t *count);
+/* SPH candidate sets of the current grid (queryNeighborsAVX2,
+ * neighborSearch.h:722-777) queried at pos (which=0) or predPos (which=1):
+ * CSR offsets off[n+1] and particle slots idx[] (ascending per particle).
+ * *total receives the number of entries; idx may be NULL to size it. */
+int ge_get_neighbors(ge_ctx *ctx, int32_t which, int64_t *off, uint32_t *idx,
+                     int64_t cap, int64_t *total);
+
+/* ---- tracing (no counterpart in the reference, which has only an FPS
+ * counter, GE/src/UI/UI.cpp:1317-1324) -------------------------------------- */
+/* phases timed with hipEvents on the context stream */
+#define GE_PHASE_INTEGRATE_BBOX 0 /* integrateStart + boundingBox */
+#define GE_PHASE_KEYS 1           /* computeMortonKeys */
+#define GE_PHASE_SORT 2           /* radix sort of the keys */
+#define GE_PHASE_GATHER 3         /* permutation gather of the state */
+#define GE_PHASE_TREE 4           /* quadtree build */
+#define GE_PHASE_GRAVITY 5        /* BH walk or direct sum (+ fused kick) */
+#define GE_PHASE_SPH_GRID 6       /* SPH grid builds */
+#define GE_PHASE_SPH 7            /* PCISPH passes (+ kick) */
+#define GE_PHASE_INTEGRATE_END 8  /* stand-alone integrateEnd */
+#define GE_NPHASES 9
+/* enable (1) or disable (0) phase timing; clears the accumulators */
+int ge_profile_enable(ge_ctx *ctx, int enable);
+/* synchronises, then returns accumulated milliseconds and launch counts per
+ * phase since the last enable/read, and clears them */
+int ge_profile_read(ge_ctx *ctx, double ms[GE_NPHASES], uint32_t launches[GE_NPHASES]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GE_PHYSICS_H */

@@ -1,0 +1,687 @@
+/*
+ * ge_oracle.c -- CPU restatement of Galaxy Engine's 2D physics step.
+ *
+ * TEST INFRASTRUCTURE ONLY (checker and timed CPU baseline); the product
+ * library never links it.  Build: oracle/Makefile (-O2 -mavx2 -mfma
+ * -ffp-contract=off -fopenmp, the reference's flags from CMakeLists.txt:56-62
+ * with contraction pinned off).
+ *
+ * Parity pins (what checks each function independently of this file) are
+ * listed in tests/test_oracle.py; functions and the reference lines they
+ * follow are cited inline.  GE/ = GalaxyEngine/.
+ */
+#include "ge_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define U32MAX 0xFFFFFFFFu
+
+/* glm::min / glm::max scalar forms (glm 1.0.1 func_common.inl) */
+static inline float fmin_glm(float a, float b) { return (b < a) ? b : a; }
+static inline float fmax_glm(float a, float b) { return (a < b) ? b : a; }
+
+/* ---------------------------------------------------------------- bbox --- */
+/* GE/src/globalLogic.cpp:1083-1099 */
+void geo_bbox(int32_t n, const float *x, const float *y, float out[3]) {
+    float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
+    for (int32_t i = 0; i < n; ++i) {
+        mnx = fmin_glm(mnx, x[i]);
+        mny = fmin_glm(mny, y[i]);
+        mxx = fmax_glm(mxx, x[i]);
+        mxy = fmax_glm(mxy, y[i]);
+    }
+    float size = fmax_glm(mxx - mnx, mxy - mny);
+    float cx = (mnx + mxx) * 0.5f;
+    float cy = (mny + mxy) * 0.5f;
+    float half = size * 0.5f;
+    out[0] = cx - half;
+    out[1] = cy - half;
+    out[2] = size;
+}
+
+/* -------------------------------------------------------------- Morton --- */
+/* GE/src/Physics/morton.cpp:3-10 */
+uint64_t geo_scale_to_grid(float pos, float lo, float hi) {
+    if (hi <= lo) return 0;
+    float c = (pos < lo) ? lo : ((hi < pos) ? hi : pos); /* std::clamp */
+    float norm = (c - lo) / (hi - lo);
+    uint64_t s = (uint64_t)(norm * 262144.0f);
+    return s < 262143u ? s : 262143u;
+}
+
+/* GE/src/Physics/morton.cpp:12-20: 18-bit magic-mask spread */
+uint64_t geo_spread_bits(uint64_t v) {
+    v &= 0x3FFFFull;
+    v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+    v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+    v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+}
+
+/* GE/src/Physics/morton.cpp:22-25: x in even bits, y in odd bits */
+uint64_t geo_morton2d(uint64_t ix, uint64_t iy) {
+    return geo_spread_bits(ix) | (geo_spread_bits(iy) << 1);
+}
+
+/* GE/src/Physics/morton.cpp:27-38 */
+void geo_morton_keys(int32_t n, const float *x, const float *y, const float bb[3], uint64_t *keys) {
+    float ext = bb[2] < 1e-6f ? 1e-6f : bb[2]; /* std::max(posSize.z, 1e-6f) */
+    float maxx = bb[0] + ext;
+    float maxy = bb[1] + ext;
+    for (int32_t i = 0; i < n; ++i) {
+        uint64_t ix = geo_scale_to_grid(x[i], bb[0], maxx);
+        uint64_t iy = geo_scale_to_grid(y[i], bb[1], maxy);
+        keys[i] = geo_morton2d(ix, iy);
+    }
+}
+
+/* stable sort by (key, index): LSD radix over 4 x 9-bit digits of the 36-bit key */
+void geo_sort_perm(int32_t n, const uint64_t *keys, uint32_t *perm) {
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    uint32_t *a = perm, *b = tmp;
+    for (int32_t i = 0; i < n; ++i) a[i] = (uint32_t)i;
+    /* keys may use more than 36 bits for generic callers: sort all 64 bits */
+    for (int shift = 0; shift < 64; shift += 8) {
+        uint32_t cnt[257];
+        memset(cnt, 0, sizeof(cnt));
+        int any = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            uint32_t d = (uint32_t)(keys[a[i]] >> shift) & 0xFFu;
+            cnt[d + 1]++;
+            any |= (int)(d != 0);
+        }
+        if (!any) continue;
+        for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+        for (int32_t i = 0; i < n; ++i) {
+            uint32_t d = (uint32_t)(keys[a[i]] >> shift) & 0xFFu;
+            b[cnt[d]++] = a[i];
+        }
+        uint32_t *t = a; a = b; b = t;
+    }
+    if (a != perm) memcpy(perm, a, sizeof(uint32_t) * (size_t)n);
+    free(tmp);
+}
+
+/* ------------------------------------------------------------ quadtree --- */
+typedef struct {
+    geo_node *nodes;
+    int64_t count, cap;
+    const float *x, *y, *m;
+} tree_builder;
+
+/* std::partition_point over [first,last) with predicate v[i] < mid: the
+ * libstdc++ halving search (quadtree.cpp:42-65 calls it three times). */
+static uint32_t part_point(const float *v, uint32_t first, uint32_t last, float mid) {
+    uint32_t len = last - first;
+    while (len > 0) {
+        uint32_t half = len >> 1;
+        uint32_t middle = first + half;
+        if (v[middle] < mid) {
+            first = middle + 1;
+            len = len - half - 1;
+        } else {
+            len = half;
+        }
+    }
+    return first;
+}
+
+static int64_t tb_alloc(tree_builder *t) {
+    if (t->count == t->cap) {
+        int64_t nc = t->cap * 2 + 16;
+        geo_node *nn = (geo_node *)realloc(t->nodes, sizeof(geo_node) * (size_t)nc);
+        if (!nn) return -1;
+        t->nodes = nn;
+        t->cap = nc;
+    }
+    return t->count++;
+}
+
+/* Node::Node (GE/src/Physics/quadtree.cpp:5-26), subGridMaker (:28-88),
+ * computeLeafMass / computeInternalMass / calculateNextNeighbor
+ * (GE/include/Physics/quadtree.h:53-111). */
+static int tb_node(tree_builder *t, int64_t idx, float px, float py, float size,
+                   uint32_t s, uint32_t e) {
+    geo_node nd;
+    nd.posx = px; nd.posy = py; nd.size = size;
+    nd.start = s; nd.end = e; nd.next = 0;
+    nd.mass = 0.0f; nd.comx = 0.0f; nd.comy = 0.0f;
+    nd.child[0] = nd.child[1] = nd.child[2] = nd.child[3] = U32MAX;
+    uint32_t cnt = e - s;
+    if (((cnt <= 4 && size <= 2.0f) || cnt == 1) || size <= 0.000001f) {
+        /* computeLeafMass, quadtree.h:53-67 */
+        float gm = 0.0f, cx = 0.0f, cy = 0.0f;
+        for (uint32_t i = s; i < e; ++i) {
+            gm = gm + t->m[i];
+            cx = cx + t->x[i] * t->m[i];
+            cy = cy + t->y[i] * t->m[i];
+        }
+        if (gm > 0) { cx = cx / gm; cy = cy / gm; }
+        nd.mass = gm; nd.comx = cx; nd.comy = cy;
+        t->nodes[idx] = nd;
+        return 0;
+    }
+    /* subGridMaker, quadtree.cpp:28-88 */
+    float half = size * 0.5f;
+    float midx = px + half, midy = py + half;
+    uint32_t b[5];
+    b[0] = s; b[4] = e;
+    b[2] = part_point(t->y, b[0], b[4], midy);
+    b[1] = part_point(t->x, b[0], b[2], midx);
+    b[3] = part_point(t->x, b[2], b[4], midx);
+    t->nodes[idx] = nd; /* parent slot holds a placeholder while children build */
+    for (int q = 0; q < 4; ++q) {
+        if (b[q + 1] > b[q]) {
+            float nx = px + ((q & 1) ? half : 0.0f);
+            float ny = py + ((q & 2) ? half : 0.0f);
+            int64_t ci = tb_alloc(t);
+            if (ci < 0) return -1;
+            if (tb_node(t, ci, nx, ny, half, b[q], b[q + 1]) != 0) return -1;
+            nd.child[q] = (uint32_t)ci;
+        }
+    }
+    /* computeInternalMass: loops subGrids[lr][ud], i.e. q = 0, 2, 1, 3 */
+    static const int order[4] = {0, 2, 1, 3};
+    float gm = 0.0f, cx = 0.0f, cy = 0.0f;
+    uint32_t next = 0;
+    for (int k = 0; k < 4; ++k) {
+        uint32_t c = nd.child[order[k]];
+        if (c == U32MAX) continue;
+        const geo_node *ch = &t->nodes[c];
+        gm = gm + ch->mass;
+        cx = cx + ch->comx * ch->mass;
+        cy = cy + ch->comy * ch->mass;
+        next += 1 + ch->next;
+    }
+    if (gm > 0) { cx = cx / gm; cy = cy / gm; }
+    nd.mass = gm; nd.comx = cx; nd.comy = cy; nd.next = next;
+    t->nodes[idx] = nd;
+    return 0;
+}
+
+static int64_t tree_build_internal(int32_t n, const float *x, const float *y, const float *m,
+                                   const float bb[3], geo_node **out) {
+    tree_builder t;
+    t.cap = 4 * (int64_t)n + 16; /* Quadtree::root reserves 4N (quadtree.cpp:94) */
+    t.count = 0;
+    t.nodes = (geo_node *)malloc(sizeof(geo_node) * (size_t)t.cap);
+    t.x = x; t.y = y; t.m = m;
+    if (!t.nodes) return -1;
+    int64_t r = tb_alloc(&t);
+    if (r < 0 || tb_node(&t, r, bb[0], bb[1], bb[2], 0, (uint32_t)n) != 0) {
+        free(t.nodes);
+        return -1;
+    }
+    *out = t.nodes;
+    return t.count;
+}
+
+int32_t geo_build_tree(int32_t n, const float *x, const float *y, const float *m,
+                       const float bb[3], geo_node *nodes, int32_t cap) {
+    geo_node *tmp = NULL;
+    int64_t c = tree_build_internal(n, x, y, m, bb, &tmp);
+    if (c < 0) return -1;
+    int64_t k = c < cap ? c : cap;
+    if (nodes && k > 0) memcpy(nodes, tmp, sizeof(geo_node) * (size_t)k);
+    free(tmp);
+    return (int32_t)c;
+}
+
+/* ------------------------------------------------------------- gravity --- */
+/* Physics::calculateForceFromGrid, GE/src/Physics/physics.cpp:91-205
+ * (temperature and periodic branches off: isTempEnabled=false,
+ * infiniteDomain=true). */
+void geo_bh_accel(int32_t n, const float *x, const float *y, const float *m,
+                  const geo_node *nodes, int32_t nn, float G, float theta, float eps,
+                  float *ax, float *ay, uint64_t *interactions) {
+    const float thetaSq = theta * theta;
+    const float softSq = eps * eps;
+    uint64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
+    for (int32_t i = 0; i < n; ++i) {
+        float fx = 0.0f, fy = 0.0f;
+        const float pmass = m[i];
+        const float xi = x[i], yi = y[i];
+        uint32_t idx = 0;
+        uint64_t cnt = 0;
+        while (idx < (uint32_t)nn) {
+            const geo_node *g = &nodes[idx];
+            float gmass = g->mass;
+            if (gmass <= 0.0f) { idx += g->next + 1; continue; }
+            float dx = g->comx - xi;
+            float dy = g->comy - yi;
+            float distSq = dx * dx + dy * dy + softSq;
+            int leaf = g->next == 0; /* all subGrids empty */
+            float sizeSq = g->size * g->size;
+            if (sizeSq < thetaSq * distSq) {
+                float inv = 1.0f / sqrtf(distSq);
+                float inv2 = inv * inv;
+                float inv3 = inv2 * inv;
+                float fm = G * pmass * gmass * inv3;
+                fx = fx + dx * fm;
+                fy = fy + dy * fm;
+                cnt++;
+                idx += g->next + 1;
+            } else if (leaf) {
+                for (uint32_t j = g->start; j < g->end; ++j) {
+                    if ((uint32_t)i == j) continue;
+                    float px = x[j] - xi;
+                    float py = y[j] - yi;
+                    float d2 = px * px + py * py + softSq;
+                    float inv = 1.0f / sqrtf(d2);
+                    float inv2 = inv * inv;
+                    float inv3 = inv2 * inv;
+                    float fm = G * pmass * m[j] * inv3;
+                    fx = fx + px * fm;
+                    fy = fy + py * fm;
+                    cnt++;
+                }
+                idx += g->next + 1;
+            } else {
+                ++idx;
+            }
+        }
+        ax[i] = fx / m[i];
+        ay[i] = fy / m[i];
+        total += cnt;
+    }
+    if (interactions) *interactions = total;
+}
+
+/* Physics::naiveGravity, GE/src/Physics/physics.cpp:392-434 */
+void geo_direct_accel(int32_t n, const float *x, const float *y, const float *m,
+                      float G, float eps, float *ax, float *ay) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int32_t i = 0; i < n; ++i) {
+        float pix = x[i], piy = y[i];
+        float tx = 0.0f, ty = 0.0f;
+        for (int32_t j = 0; j < n; ++j) {
+            float dx = x[j] - pix;
+            float dy = y[j] - piy;
+            float distSq = dx * dx + dy * dy + eps; /* softening not squared */
+            float inv = 1.0f / sqrtf(distSq);
+            float inv3 = inv * inv * inv;
+            float factor = G * m[j] * inv3;
+            tx = tx + dx * factor;
+            ty = ty + dy * factor;
+        }
+        ax[i] = tx;
+        ay[i] = ty;
+    }
+}
+
+void geo_direct_fp64(int32_t n, const float *x, const float *y, const float *m,
+                     double G, double eps, int32_t nt, const uint32_t *targets,
+                     double *ax, double *ay) {
+    const double e2 = eps * eps;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int32_t t = 0; t < nt; ++t) {
+        uint32_t i = targets ? targets[t] : (uint32_t)t;
+        double xi = x[i], yi = y[i];
+        double sx = 0.0, sy = 0.0;
+        for (int32_t j = 0; j < n; ++j) {
+            if ((uint32_t)j == i) continue;
+            double dx = (double)x[j] - xi;
+            double dy = (double)y[j] - yi;
+            double r2 = dx * dx + dy * dy + e2;
+            double inv = 1.0 / sqrt(r2);
+            double f = (double)m[j] * inv * inv * inv;
+            sx += dx * f;
+            sy += dy * f;
+        }
+        ax[t] = G * sx;
+        ay[t] = G * sy;
+    }
+}
+
+/* ---------------------------------------------------------- integrator --- */
+/* Physics::integrateStart, GE/src/Physics/physics.cpp:1069-1112 */
+void geo_integrate_start(geo_state *s, float dt, int32_t sph, float sph_max_vel) {
+    const float halfDt = dt * 0.5f;
+    const float maxVSq = sph_max_vel * sph_max_vel;
+    for (int32_t i = 0; i < s->n; ++i) {
+        if (s->flags[i] & GEO_PINNED) continue;
+        float pvx = s->vx[i], pvy = s->vy[i];
+        float vx = s->vx[i] + s->ax[i] * halfDt;
+        float vy = s->vy[i] + s->ay[i] * halfDt;
+        if (sph) {
+            float vSq = vx * vx + vy * vy;
+            if (vSq > maxVSq) {
+                float pSq = pvx * pvx + pvy * pvy;
+                if (pSq > 0.00001f) {
+                    float inv = sph_max_vel / sqrtf(pSq);
+                    pvx = pvx * inv;
+                    pvy = pvy * inv;
+                }
+                float inv = sph_max_vel / sqrtf(vSq);
+                vx = vx * inv;
+                vy = vy * inv;
+            }
+        }
+        s->pvx[i] = pvx; s->pvy[i] = pvy;
+        s->vx[i] = vx; s->vy[i] = vy;
+        s->x[i] = s->x[i] + vx * dt;
+        s->y[i] = s->y[i] + vy * dt;
+    }
+}
+
+/* Physics::integrateEnd, GE/src/Physics/physics.cpp:1114-1125 */
+void geo_integrate_end(geo_state *s, float dt) {
+    const float h = dt * 0.5f;
+    for (int32_t i = 0; i < s->n; ++i) {
+        if (s->flags[i] & GEO_PINNED) continue;
+        s->vx[i] = s->vx[i] + s->ax[i] * h;
+        s->vy[i] = s->vy[i] + s->ay[i] * h;
+    }
+}
+
+/* ----------------------------------------------------------------- SPH --- */
+typedef struct { int32_t cx, cy; uint32_t i; } cell_entry;
+
+static int cell_cmp(const void *a, const void *b) {
+    const cell_entry *p = (const cell_entry *)a, *q = (const cell_entry *)b;
+    if (p->cx != q->cx) return p->cx < q->cx ? -1 : 1;
+    if (p->cy != q->cy) return p->cy < q->cy ? -1 : 1;
+    return p->i < q->i ? -1 : (p->i > q->i);
+}
+
+/* NeighborSearchV2AVX2::posToCellCoord, neighborSearch.h:631-635 */
+static inline int32_t cell_of(float p, float cell) { return (int32_t)floorf(p / cell); }
+
+typedef struct { cell_entry *e; int32_t n; } cell_grid;
+
+/* newGridAVX2 (neighborSearch.h:636-720), restated as a (cx, cy, index)
+ * ordered table; the candidate SET is the same as the hashed table's. */
+static cell_grid grid_build(int32_t n, const float *x, const float *y, float cell) {
+    cell_grid g;
+    g.n = n;
+    g.e = (cell_entry *)malloc(sizeof(cell_entry) * (size_t)(n > 0 ? n : 1));
+    for (int32_t i = 0; i < n; ++i) {
+        g.e[i].cx = cell_of(x[i], cell);
+        g.e[i].cy = cell_of(y[i], cell);
+        g.e[i].i = (uint32_t)i;
+    }
+    qsort(g.e, (size_t)n, sizeof(cell_entry), cell_cmp);
+    return g;
+}
+
+static int32_t grid_lower(const cell_grid *g, int32_t cx, int32_t cy) {
+    int32_t lo = 0, hi = g->n;
+    while (lo < hi) {
+        int32_t mid = lo + (hi - lo) / 2;
+        const cell_entry *e = &g->e[mid];
+        if (e->cx < cx || (e->cx == cx && e->cy < cy)) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* queryNeighborsAVX2 (neighborSearch.h:722-777): dx outer, dy inner, every
+ * particle whose cell coordinates match; no distance filter; self included. */
+static int32_t grid_query(const cell_grid *g, float qx, float qy, float cell,
+                          uint32_t *buf, int32_t cap) {
+    int32_t cx = cell_of(qx, cell), cy = cell_of(qy, cell);
+    int32_t k = 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            int32_t nx = cx + dx, ny = cy + dy;
+            int32_t p = grid_lower(g, nx, ny);
+            while (p < g->n && g->e[p].cx == nx && g->e[p].cy == ny) {
+                if (k < cap) buf[k] = g->e[p].i;
+                ++k;
+                ++p;
+            }
+        }
+    }
+    return k;
+}
+
+static int u32_cmp(const void *a, const void *b) {
+    uint32_t p = *(const uint32_t *)a, q = *(const uint32_t *)b;
+    return p < q ? -1 : (p > q);
+}
+
+int64_t geo_sph_candidates(int32_t n, const float *x, const float *y,
+                           int32_t nq, const float *qx, const float *qy, float cell,
+                           int64_t *off, uint32_t *idx, int64_t cap) {
+    cell_grid g = grid_build(n, x, y, cell);
+    int64_t total = 0;
+    uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    for (int32_t q = 0; q < nq; ++q) {
+        off[q] = total;
+        int32_t k = grid_query(&g, qx[q], qy[q], cell, buf, n);
+        qsort(buf, (size_t)k, sizeof(uint32_t), u32_cmp);
+        if (idx && total + k <= cap) memcpy(idx + total, buf, sizeof(uint32_t) * (size_t)k);
+        total += k;
+    }
+    off[nq] = total;
+    free(buf);
+    free(g.e);
+    return total <= cap ? total : -1;
+}
+
+#define PI_F 3.14159265358979323846f /* raylib PI (a float) */
+
+/* SPH.h:38-56: constants computed through ::pow(double) with float PI, then
+ * rounded to float at the assignment. */
+void geo_sph_constants(float h, float out[3]) {
+    out[0] = (float)(((double)PI_F * pow((double)h, 4.0)) / 6.0);  /* W volume */
+    out[1] = (float)(-45.0 / ((double)PI_F * pow((double)h, 6.0))); /* spiky scale */
+    out[2] = (float)(45.0 / ((double)PI_F * pow((double)h, 6.0)));  /* laplacian */
+}
+
+/* SPH::spikyKernelDerivative, SPH.h:45-50: float scale times double pow */
+static inline float spiky_grad(float r, float h, float scale) {
+    if (r >= h) return 0.0f;
+    return (float)((double)scale * pow((double)(h - r), 2.0));
+}
+
+/* SPH::smoothingKernelCohesion, SPH.h:58-64 */
+static inline float coh_kernel(float r, float h) {
+    if (r >= h) return 0.0f;
+    float q = r / h;
+    return (1.0f - q) * (0.5f - q) * (0.5f - q) * 30.0f / (PI_F * h * h);
+}
+
+/* SPH::PCISPH with maxIter = 1 (SPH.cpp:152-281) and computeViscCohesionForces
+ * (SPH.cpp:98-150); scatter form in index order. */
+int32_t geo_sph_pcisph(geo_state *s, const geo_params *p) {
+    const int32_t n = s->n;
+    const float h = p->sph_h, h2 = h * h;
+    float kc[3];
+    geo_sph_constants(h, kc);
+    const float volume = kc[0], spiky = kc[1], lap = kc[2];
+    const float coeff = 0.5f * p->dt * p->dt;
+    float *fx = (float *)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
+    float *fy = (float *)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
+    uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    cell_grid g = grid_build(n, s->x, s->y, p->sph_cell);
+    const uint32_t *fl = s->flags;
+#define IS_SPH(k) ((fl[k] & GEO_SPH) != 0)
+#define DRAWN(k) ((fl[k] & GEO_DRAWN) != 0)
+#define PINNED(k) ((fl[k] & GEO_PINNED) != 0)
+
+    /* viscosity + cohesion, SPH.cpp:98-150 */
+    for (int32_t i = 0; i < n; ++i) {
+        if (!IS_SPH(i) || PINNED(i) || DRAWN(i)) continue;
+        int32_t k = grid_query(&g, s->x[i], s->y[i], p->sph_cell, buf, n);
+        for (int32_t t = 0; t < k; ++t) {
+            uint32_t j = buf[t];
+            if (!IS_SPH(j) || DRAWN(j)) continue;
+            if ((int32_t)j == i) continue;
+            float dx = s->x[j] - s->x[i], dy = s->y[j] - s->y[i];
+            float rSq = dx * dx + dy * dy;
+            if (rSq >= h2) continue;
+            float r = sqrtf(rSq < 1e-6f ? 1e-6f : rSq); /* std::max(rSq, 1e-6f) */
+            float nrx = dx / r, nry = dy / r;
+            float mJ = s->sph_mass[j] * p->sph_unit_mass;
+            float lapW = lap; /* smoothingKernelLaplacian: constant for r < h */
+            float dj = s->dens[j] < 0.001f ? 0.001f : s->dens[j]; /* std::max(dens, 0.001f) */
+            float vfx = p->viscosity * s->visc[j] * mJ / dj * lapW * (s->vx[j] - s->vx[i]);
+            float vfy = p->viscosity * s->visc[j] * mJ / dj * lapW * (s->vy[j] - s->vy[i]);
+            float cohCoef = p->cohesion_coef * s->cohesion[i];
+            float cf = coh_kernel(r, h);
+            float cfx = cohCoef * mJ * cf * nrx;
+            float cfy = cohCoef * mJ * cf * nry;
+            fx[i] = fx[i] + (vfx + cfx);
+            fy[i] = fy[i] + (vfy + cfy);
+            fx[j] = fx[j] - (vfx + cfx);
+            fy[j] = fy[j] - (vfy + cfy);
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        s->press[i] = 0.0f;
+        s->pfx[i] = 0.0f;
+        s->pfy[i] = 0.0f;
+    }
+    /* predicted positions, SPH.cpp:173-181 */
+    for (int32_t i = 0; i < n; ++i) {
+        if (!IS_SPH(i) || DRAWN(i)) continue;
+        float dxp = (fx[i] / s->sph_mass[i]) * coeff;
+        float dyp = (fy[i] / s->sph_mass[i]) * coeff;
+        s->ppx[i] = s->x[i] + dxp;
+        s->ppy[i] = s->y[i] + dyp;
+    }
+    /* predicted density and pressure, SPH.cpp:183-219 */
+    for (int32_t i = 0; i < n; ++i) {
+        if (!IS_SPH(i) || DRAWN(i)) continue;
+        float pd = 0.0f;
+        int32_t k = grid_query(&g, s->ppx[i], s->ppy[i], p->sph_cell, buf, n);
+        for (int32_t t = 0; t < k; ++t) {
+            uint32_t j = buf[t];
+            if (!IS_SPH(j) || DRAWN(j)) continue;
+            float dx = s->ppx[i] - s->ppx[j], dy = s->ppy[i] - s->ppy[j];
+            float rrSq = dx * dx + dy * dy;
+            if (rrSq >= h * h) continue;
+            float rr = sqrtf(rrSq);
+            float mJ = s->sph_mass[j] * p->sph_unit_mass;
+            float rho0 = 0.5f * (s->rest_dens[i] + s->rest_dens[j]);
+            float w = (h - rr) * (h - rr) / volume;
+            pd = pd + mJ * w / rho0;
+        }
+        s->pred_dens[i] = pd;
+        float err = pd - s->rest_dens[i];
+        float pt = p->delta * err;
+        if (pt < 0.0f) pt = 0.0f;
+        s->press_tmp[i] = pt;
+        s->press[i] = s->press[i] + pt * s->stiff[i] * p->stiff_mult;
+    }
+    /* pressure force, SPH.cpp:221-264 */
+    for (int32_t i = 0; i < n; ++i) {
+        if (!IS_SPH(i) || DRAWN(i)) continue;
+        int32_t k = grid_query(&g, s->ppx[i], s->ppy[i], p->sph_cell, buf, n);
+        for (int32_t t = 0; t < k; ++t) {
+            uint32_t j = buf[t];
+            if ((int32_t)j == i) continue;
+            if (!IS_SPH(j) || DRAWN(j)) continue;
+            float dx = s->ppx[i] - s->ppx[j], dy = s->ppy[i] - s->ppy[j];
+            float rr = sqrtf(dx * dx + dy * dy);
+            if (rr < 1e-5f || rr >= h) continue;
+            float gradW = spiky_grad(rr, h, spiky);
+            float nx = dx / rr, ny = dy / rr;
+            float avgP = 0.5f * (s->press[i] + s->press[j]);
+            float avgD = 0.5f * (s->pred_dens[i] + s->pred_dens[j]);
+            float mi = s->sph_mass[i] * p->sph_unit_mass;
+            float mj = s->sph_mass[j] * p->sph_unit_mass;
+            float mag = -(mi + mj) * avgP / (avgD < 0.01f ? 0.01f : avgD);
+            float smax = s->sph_mass[i] < s->sph_mass[j] ? s->sph_mass[j] : s->sph_mass[i];
+            float smin = s->sph_mass[j] < s->sph_mass[i] ? s->sph_mass[j] : s->sph_mass[i];
+            float ratio = smax / smin;
+            float sc = 8.0f / ratio;
+            if (1.0f < sc) sc = 1.0f;
+            mag = mag * sc;
+            float pfx = mag * gradW * nx, pfy = mag * gradW * ny;
+            fx[i] = fx[i] + pfx;
+            fy[i] = fy[i] + pfy;
+            fx[j] = fx[j] - pfx;
+            fy[j] = fy[j] - pfy;
+        }
+    }
+    /* pressF and acceleration, SPH.cpp:271-280 */
+    for (int32_t i = 0; i < n; ++i) {
+        s->pfx[i] = fx[i] / s->sph_mass[i];
+        s->pfy[i] = fy[i] / s->sph_mass[i];
+        if (!PINNED(i)) {
+            s->ax[i] = s->ax[i] + s->pfx[i];
+            s->ay[i] = s->ay[i] + s->pfy[i];
+        }
+    }
+#undef IS_SPH
+#undef DRAWN
+#undef PINNED
+    free(g.e);
+    free(buf);
+    free(fx);
+    free(fy);
+    return 0;
+}
+
+/* ---------------------------------------------------------- full step --- */
+static void permute_f(float *a, const uint32_t *perm, int32_t n, float *tmp) {
+    if (!a) return;
+    for (int32_t i = 0; i < n; ++i) tmp[i] = a[perm[i]];
+    memcpy(a, tmp, sizeof(float) * (size_t)n);
+}
+static void permute_u(uint32_t *a, const uint32_t *perm, int32_t n, uint32_t *tmp) {
+    if (!a) return;
+    for (int32_t i = 0; i < n; ++i) tmp[i] = a[perm[i]];
+    memcpy(a, tmp, sizeof(uint32_t) * (size_t)n);
+}
+
+/* Morton::sortParticlesByMortonKey (morton.cpp:40-63): gather every field */
+static void permute_state(geo_state *s, const uint32_t *perm) {
+    int32_t n = s->n;
+    float *t = (float *)malloc(sizeof(float) * (size_t)n);
+    float *fs[] = {s->x, s->y, s->vx, s->vy, s->ax, s->ay, s->pvx, s->pvy, s->m,
+                   s->sph_mass, s->rest_dens, s->stiff, s->visc, s->cohesion, s->dens,
+                   s->press, s->press_tmp, s->pred_dens, s->ppx, s->ppy, s->pfx, s->pfy};
+    for (size_t k = 0; k < sizeof(fs) / sizeof(fs[0]); ++k) permute_f(fs[k], perm, n, t);
+    permute_u(s->flags, perm, n, (uint32_t *)t);
+    permute_u(s->id, perm, n, (uint32_t *)t);
+    free(t);
+}
+
+/* updateScene() 2D physics ordering, GE/src/globalLogic.cpp:1155-1372 */
+int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps) {
+    const int32_t n = s->n;
+    if (n <= 0) return 0;
+    uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
+    uint32_t *perm = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
+    for (int32_t step = 0; step < nsteps; ++step) {
+        float dt0 = step == 0 ? p->dt_start : p->dt;
+        geo_node *nodes = NULL;
+        int64_t nn = 0;
+        if (dt0 != 0.0f) {
+            geo_integrate_start(s, dt0, p->sph, p->sph_max_vel);
+            float bb[3];
+            geo_bbox(n, s->x, s->y, bb);
+            geo_morton_keys(n, s->x, s->y, bb, keys);
+            geo_sort_perm(n, keys, perm);
+            permute_state(s, perm);
+            if (!p->naive) {
+                nn = tree_build_internal(n, s->x, s->y, s->m, bb, &nodes);
+                if (nn < 0) { free(keys); free(perm); return -2; }
+            }
+        }
+        if (p->dt != 0.0f) {
+            for (int32_t i = 0; i < n; ++i) { s->ax[i] = 0.0f; s->ay[i] = 0.0f; }
+            if (p->G != 0.0f) {
+                if (p->naive || n < p->direct_below || !nodes)
+                    geo_direct_accel(n, s->x, s->y, s->m, p->G, p->softening, s->ax, s->ay);
+                else
+                    geo_bh_accel(n, s->x, s->y, s->m, nodes, (int32_t)nn, p->G, p->theta,
+                                 p->softening, s->ax, s->ay, NULL);
+            }
+            if (p->sph) geo_sph_pcisph(s, p);
+            geo_integrate_end(s, p->dt);
+        }
+        free(nodes);
+    }
+    free(keys);
+    free(perm);
+    return 0;
+}

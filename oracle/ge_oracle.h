@@ -1,0 +1,121 @@
+/*
+ * ge_oracle.h -- CPU restatement of Galaxy Engine's 2D per-frame physics step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path and the timed CPU baseline ("port").  Only tests/, the smoke entry and
+ * the bench's cpu_baseline leg load it.  The product library never links it.
+ *
+ * Every function restates the reference algorithm in plain C with the same
+ * floating-point expression order.  It is compiled with -ffp-contract=off so
+ * that no a*b+c is fused (the contraction mode is pinned; see the A-12 note in
+ * the design notes).  Citations are path:line under the reference tree, with
+ * GE/ = GalaxyEngine/.
+ *
+ * Differences from the reference that are deliberate and documented:
+ *  - the Morton sort is stable by (key, original index); the reference uses
+ *    std::sort, whose order inside equal-key runs is unspecified
+ *    (GE/src/Physics/morton.cpp:51);
+ *  - the BH leaf body follows the scalar variant (GE/src/Physics/physics.cpp:
+ *    91-205).  The AVX2 variant's 8-wide leaf body only runs for leaves with at
+ *    least 8 particles, which the leaf rule only allows below size 1e-6;
+ *  - the direct sum uses the exact 1/sqrtf of naiveGravity
+ *    (physics.cpp:392-434), not the vendor-specific approximate rsqrt of
+ *    naiveGravityAVX2 (physics.cpp:436-540);
+ *  - SPH neighbour candidates are enumerated cell by cell in particle-index
+ *    order, not in the reference's hash-bucket order (summation order only);
+ *  - the SPH cell size is a parameter (the reference fixes it at 3.0,
+ *    GE/include/Particles/neighborSearch.h:604).
+ */
+#ifndef GE_ORACLE_H
+#define GE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* per-particle flag bits (the reference keeps them in ParticleRendering,
+ * GE/include/Particles/particle.h:75-140) */
+#define GEO_PINNED 1u
+#define GEO_SPH 2u
+#define GEO_DRAWN 4u
+#define GEO_DARK 8u
+
+typedef struct {
+    float posx, posy;   /* lower-left corner (Node::pos) */
+    float comx, comy;   /* Node::centerOfMass */
+    float size;         /* Node::size (side length) */
+    float mass;         /* Node::gridMass */
+    uint32_t start, end;
+    uint32_t next;      /* number of descendants (pre-order skip) */
+    uint32_t child[4];  /* child node index per quadrant q (q&1 -> +x, q&2 -> +y), or UINT32_MAX */
+} geo_node;
+
+typedef struct {
+    int32_t n;
+    float *x, *y, *vx, *vy, *ax, *ay, *pvx, *pvy, *m;
+    uint32_t *flags, *id;
+    /* SPH fields (may be NULL when SPH is off) */
+    float *sph_mass, *rest_dens, *stiff, *visc, *cohesion, *dens;
+    float *press, *press_tmp, *pred_dens, *ppx, *ppy, *pfx, *pfy;
+} geo_state;
+
+typedef struct {
+    float G, theta, softening;
+    float dt_start;        /* timeFactor used by integrateStart (previous frame's) */
+    float dt;              /* timeFactor used by the rest of the step */
+    int32_t naive;         /* myVar.naive: always direct sum */
+    int32_t direct_below;  /* direct sum when N < this (reference: 5000) */
+    int32_t sph;           /* myVar.isSPHEnabled */
+    float sph_h;           /* SPH::radiusMultiplier */
+    float sph_cell;        /* neighbour-grid cell size */
+    float sph_unit_mass;   /* myVar.mass */
+    float viscosity, cohesion_coef, delta, stiff_mult, sph_max_vel;
+} geo_params;
+
+/* bounding box: GE/src/globalLogic.cpp:1083-1099 -> {x, y, size} */
+void geo_bbox(int32_t n, const float *x, const float *y, float out[3]);
+/* Morton: GE/src/Physics/morton.cpp:3-38 */
+uint64_t geo_scale_to_grid(float pos, float lo, float hi);
+uint64_t geo_spread_bits(uint64_t v);
+uint64_t geo_morton2d(uint64_t ix, uint64_t iy);
+void geo_morton_keys(int32_t n, const float *x, const float *y, const float bb[3], uint64_t *keys);
+/* stable (key, index) sort -> perm[i] = source index of sorted slot i */
+void geo_sort_perm(int32_t n, const uint64_t *keys, uint32_t *perm);
+/* quadtree: GE/src/Physics/quadtree.cpp:5-107; returns node count or -1 if cap exceeded */
+int32_t geo_build_tree(int32_t n, const float *x, const float *y, const float *m,
+                       const float bb[3], geo_node *nodes, int32_t cap);
+/* BH accelerations: GE/src/Physics/physics.cpp:91-205 */
+void geo_bh_accel(int32_t n, const float *x, const float *y, const float *m,
+                  const geo_node *nodes, int32_t nn, float G, float theta, float eps,
+                  float *ax, float *ay, uint64_t *interactions);
+/* direct sum: GE/src/Physics/physics.cpp:392-434 (epsilon NOT squared) */
+void geo_direct_accel(int32_t n, const float *x, const float *y, const float *m,
+                      float G, float eps, float *ax, float *ay);
+/* FP64 direct sum with Plummer r^2+eps^2, j != i, for nt targets */
+void geo_direct_fp64(int32_t n, const float *x, const float *y, const float *m,
+                     double G, double eps, int32_t nt, const uint32_t *targets,
+                     double *ax, double *ay);
+/* integrator: GE/src/Physics/physics.cpp:1069-1125 */
+void geo_integrate_start(geo_state *s, float dt, int32_t sph, float sph_max_vel);
+void geo_integrate_end(geo_state *s, float dt);
+/* SPH neighbour candidates (AVX2 grid semantics, neighborSearch.h:636-777):
+ * all particles whose cell of (x,y) is within one cell (3x3) of cell(qx,qy).
+ * Writes CSR: off[n+1], idx[] (each list sorted ascending). Returns total or
+ * -1 when cap is too small. */
+int64_t geo_sph_candidates(int32_t n, const float *x, const float *y,
+                           int32_t nq, const float *qx, const float *qy, float cell,
+                           int64_t *off, uint32_t *idx, int64_t cap);
+/* PCISPH (one iteration): GE/src/Physics/SPH.cpp:98-281 */
+int32_t geo_sph_pcisph(geo_state *s, const geo_params *p);
+/* SPH kernel constants computed as the reference does (double pow, float PI) */
+void geo_sph_constants(float h, float out[3]); /* {W volume, spiky scale, laplacian scale} */
+/* one full frame: GE/src/globalLogic.cpp:1155-1372 for the 2D path.
+ * Permutes every state array into Morton order.  Returns 0 or negative. */
+int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""ctypes loader for the CPU restatement (libge_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke entry and the bench's
+cpu_baseline leg, never by the product library or its Python binding.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libge_oracle.so")
+
+GEO_PINNED, GEO_SPH, GEO_DRAWN, GEO_DARK = 1, 2, 4, 8
+
+NODE_DTYPE = np.dtype([("posx", "<f4"), ("posy", "<f4"), ("comx", "<f4"), ("comy", "<f4"),
+                       ("size", "<f4"), ("mass", "<f4"), ("start", "<u4"), ("end", "<u4"),
+                       ("next", "<u4"), ("child", "<u4", (4,))])
+
+STATE_F = ["x", "y", "vx", "vy", "ax", "ay", "pvx", "pvy", "mass"]
+STATE_SPH = ["sph_mass", "rest_dens", "stiff", "visc", "cohesion", "dens", "press", "press_tmp",
+             "pred_dens", "ppx", "ppy", "pfx", "pfy"]
+
+
+class _State(C.Structure):
+    _fields_ = ([("n", C.c_int32)] + [(f, C.POINTER(C.c_float)) for f in
+                                      ["x", "y", "vx", "vy", "ax", "ay", "pvx", "pvy", "m"]]
+                + [("flags", C.POINTER(C.c_uint32)), ("id", C.POINTER(C.c_uint32))]
+                + [(f, C.POINTER(C.c_float)) for f in STATE_SPH])
+
+
+class _Params(C.Structure):
+    _fields_ = [("G", C.c_float), ("theta", C.c_float), ("softening", C.c_float),
+                ("dt_start", C.c_float), ("dt", C.c_float), ("naive", C.c_int32),
+                ("direct_below", C.c_int32), ("sph", C.c_int32), ("sph_h", C.c_float),
+                ("sph_cell", C.c_float), ("sph_unit_mass", C.c_float), ("viscosity", C.c_float),
+                ("cohesion_coef", C.c_float), ("delta", C.c_float), ("stiff_mult", C.c_float),
+                ("sph_max_vel", C.c_float)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        build()
+    lib = C.CDLL(_LIB)
+    f32p = C.POINTER(C.c_float)
+    u32p = C.POINTER(C.c_uint32)
+    u64p = C.POINTER(C.c_uint64)
+    f64p = C.POINTER(C.c_double)
+    i64p = C.POINTER(C.c_int64)
+    lib.geo_bbox.argtypes = [C.c_int32, f32p, f32p, f32p]
+    lib.geo_scale_to_grid.restype = C.c_uint64
+    lib.geo_scale_to_grid.argtypes = [C.c_float, C.c_float, C.c_float]
+    lib.geo_spread_bits.restype = C.c_uint64
+    lib.geo_spread_bits.argtypes = [C.c_uint64]
+    lib.geo_morton2d.restype = C.c_uint64
+    lib.geo_morton2d.argtypes = [C.c_uint64, C.c_uint64]
+    lib.geo_morton_keys.argtypes = [C.c_int32, f32p, f32p, f32p, u64p]
+    lib.geo_sort_perm.argtypes = [C.c_int32, u64p, u32p]
+    lib.geo_build_tree.restype = C.c_int32
+    lib.geo_build_tree.argtypes = [C.c_int32, f32p, f32p, f32p, f32p, C.c_void_p, C.c_int32]
+    lib.geo_bh_accel.argtypes = [C.c_int32, f32p, f32p, f32p, C.c_void_p, C.c_int32, C.c_float,
+                                 C.c_float, C.c_float, f32p, f32p, u64p]
+    lib.geo_direct_accel.argtypes = [C.c_int32, f32p, f32p, f32p, C.c_float, C.c_float, f32p, f32p]
+    lib.geo_direct_fp64.argtypes = [C.c_int32, f32p, f32p, f32p, C.c_double, C.c_double, C.c_int32,
+                                    u32p, f64p, f64p]
+    lib.geo_sph_candidates.restype = C.c_int64
+    lib.geo_sph_candidates.argtypes = [C.c_int32, f32p, f32p, C.c_int32, f32p, f32p, C.c_float,
+                                       i64p, u32p, C.c_int64]
+    lib.geo_sph_constants.argtypes = [C.c_float, f32p]
+    lib.geo_step.restype = C.c_int32
+    lib.geo_step.argtypes = [C.POINTER(_State), C.POINTER(_Params), C.c_int32]
+    lib.geo_sph_pcisph.restype = C.c_int32
+    lib.geo_sph_pcisph.argtypes = [C.POINTER(_State), C.POINTER(_Params)]
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def bbox(x, y):
+    x, y = f32(x), f32(y)
+    out = np.zeros(3, np.float32)
+    load().geo_bbox(len(x), _f(x), _f(y), _f(out))
+    return out
+
+
+def scale_to_grid(p, lo, hi):
+    return load().geo_scale_to_grid(float(np.float32(p)), float(np.float32(lo)), float(np.float32(hi)))
+
+
+def morton2d(ix, iy):
+    return load().geo_morton2d(int(ix), int(iy))
+
+
+def morton_keys(x, y, bb):
+    x, y, bb = f32(x), f32(y), f32(bb)
+    k = np.zeros(len(x), np.uint64)
+    load().geo_morton_keys(len(x), _f(x), _f(y), _f(bb), k.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return k
+
+
+def sort_perm(keys):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    p = np.zeros(len(keys), np.uint32)
+    load().geo_sort_perm(len(keys), keys.ctypes.data_as(C.POINTER(C.c_uint64)), _u(p))
+    return p
+
+
+def build_tree(x, y, m, bb):
+    x, y, m, bb = f32(x), f32(y), f32(m), f32(bb)
+    lib = load()
+    cap = 4 * len(x) + 16
+    nodes = np.zeros(cap, NODE_DTYPE)
+    c = lib.geo_build_tree(len(x), _f(x), _f(y), _f(m), _f(bb), C.c_void_p(nodes.ctypes.data), cap)
+    if c < 0:
+        raise RuntimeError("tree build failed")
+    if c > cap:
+        nodes = np.zeros(c, NODE_DTYPE)
+        lib.geo_build_tree(len(x), _f(x), _f(y), _f(m), _f(bb), C.c_void_p(nodes.ctypes.data), c)
+    return nodes[:c].copy()
+
+
+def bh_accel(x, y, m, nodes, G, theta, eps):
+    x, y, m = f32(x), f32(y), f32(m)
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    ax = np.zeros(len(x), np.float32)
+    ay = np.zeros(len(x), np.float32)
+    cnt = C.c_uint64()
+    load().geo_bh_accel(len(x), _f(x), _f(y), _f(m), C.c_void_p(nodes.ctypes.data), len(nodes),
+                        G, theta, eps, _f(ax), _f(ay), C.byref(cnt))
+    return ax, ay, cnt.value
+
+
+def direct_accel(x, y, m, G, eps):
+    x, y, m = f32(x), f32(y), f32(m)
+    ax = np.zeros(len(x), np.float32)
+    ay = np.zeros(len(x), np.float32)
+    load().geo_direct_accel(len(x), _f(x), _f(y), _f(m), G, eps, _f(ax), _f(ay))
+    return ax, ay
+
+
+def direct_fp64(x, y, m, G, eps, targets=None):
+    x, y, m = f32(x), f32(y), f32(m)
+    if targets is None:
+        targets = np.arange(len(x), dtype=np.uint32)
+    targets = np.ascontiguousarray(targets, dtype=np.uint32)
+    ax = np.zeros(len(targets), np.float64)
+    ay = np.zeros(len(targets), np.float64)
+    load().geo_direct_fp64(len(x), _f(x), _f(y), _f(m), G, eps, len(targets), _u(targets),
+                           ax.ctypes.data_as(C.POINTER(C.c_double)),
+                           ay.ctypes.data_as(C.POINTER(C.c_double)))
+    return ax, ay
+
+
+def sph_candidates(x, y, qx, qy, cell):
+    x, y, qx, qy = f32(x), f32(y), f32(qx), f32(qy)
+    off = np.zeros(len(qx) + 1, np.int64)
+    i64p = C.POINTER(C.c_int64)
+    lib = load()
+    tot = lib.geo_sph_candidates(len(x), _f(x), _f(y), len(qx), _f(qx), _f(qy), cell,
+                                 off.ctypes.data_as(i64p), None, 0)
+    if tot < 0:
+        tot = int(off[-1])
+    idx = np.zeros(max(tot, 1), np.uint32)
+    lib.geo_sph_candidates(len(x), _f(x), _f(y), len(qx), _f(qx), _f(qy), cell,
+                           off.ctypes.data_as(i64p), _u(idx), tot)
+    return off, idx[:tot]
+
+
+def sph_constants(h):
+    out = np.zeros(3, np.float32)
+    load().geo_sph_constants(h, _f(out))
+    return out
+
+
+class State:
+    """Host particle state for the oracle step; arrays are permuted in place."""
+
+    def __init__(self, st, sph=False):
+        n = len(st["x"])
+        self.n = n
+        self.a = {}
+        for f in STATE_F:
+            self.a[f] = f32(st[f]).copy() if f in st and st[f] is not None else np.zeros(n, np.float32)
+        self.a["flags"] = np.ascontiguousarray(st.get("flags", np.zeros(n)), dtype=np.uint32).copy()
+        self.a["id"] = np.ascontiguousarray(st.get("id", np.arange(n)), dtype=np.uint32).copy()
+        self.sph = sph
+        if sph:
+            for f in STATE_SPH:
+                dflt = np.ones(n, np.float32) if f == "sph_mass" else np.zeros(n, np.float32)
+                self.a[f] = f32(st[f]).copy() if f in st and st[f] is not None else dflt
+        self._s = _State()
+        self._s.n = n
+        for f in STATE_F:
+            setattr(self._s, "m" if f == "mass" else f, _f(self.a[f]))
+        self._s.flags = _u(self.a["flags"])
+        self._s.id = _u(self.a["id"])
+        if sph:
+            for f in STATE_SPH:
+                setattr(self._s, f, _f(self.a[f]))
+
+    def step(self, prm, nsteps=1):
+        p = _Params()
+        d = dict(G=1.0, theta=0.8, softening=1.0, dt=0.01, naive=0, direct_below=5000, sph=0,
+                 sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3, cohesion_coef=1.0,
+                 delta=19000.0, stiff_mult=1.0, sph_max_vel=250.0)
+        d.update({k: v for k, v in prm.items() if k in d or k == "dt_start"})
+        d.setdefault("dt_start", d["dt"])
+        for k, v in d.items():
+            setattr(p, k, v)
+        rc = load().geo_step(C.byref(self._s), C.byref(p), int(nsteps))
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed {rc}")
+
+    def __getitem__(self, k):
+        return self.a[k]

@@ -1,0 +1,86 @@
+"""CPU checks of the C-ABI boundary: the shared library loads, exports every
+entry point include/ge_physics.h declares, its struct layouts match the
+ctypes mirror, and the defaults equal the reference's parameters."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ge_physics.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(ge_[a-z_0-9]+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import galaxy_step
+    lib = galaxy_step.load_library()
+    names = declared_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(galaxy_step.lib.EXPORTS)
+
+
+def test_struct_layout_matches_header(tmp_path):
+    import galaxy_step.lib as L
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ge_physics.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(ge_config), '
+                   'sizeof(ge_particles), sizeof(ge_step_params), sizeof(ge_node), '
+                   'offsetof(ge_step_params, compensated));return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    want = [C.sizeof(L.GeConfig), C.sizeof(L.GeParticles), C.sizeof(L.StepParams),
+            C.sizeof(L.GeNode), L.StepParams.compensated.offset]
+    assert [int(v) for v in out] == want
+
+
+def test_default_params_are_reference_defaults():
+    import galaxy_step
+    p = galaxy_step.default_params()
+    # GE/include/parameters.h:114-133,311-318; SPH.h:17; neighborSearch.h:604
+    assert p.theta == pytest.approx(0.8) and p.softening == pytest.approx(1.5)
+    assert p.G == pytest.approx(6.674e-11)
+    assert p.dt == pytest.approx(0.045) and p.direct_below == 5000
+    assert p.sph_h == 3.0 and p.sph_cell == 3.0 and p.sph_unit_mass == pytest.approx(0.03)
+    assert p.viscosity == pytest.approx(0.3) and p.delta == 19000.0
+    assert p.sph_max_vel == 250.0 and p.periodic == 0 and p.temperature == 0
+
+
+def test_create_reports_status_without_crashing():
+    import galaxy_step
+    from galaxy_step.lib import GeConfig
+    lib = galaxy_step.load_library()
+    cfg = GeConfig()
+    lib.ge_default_config(C.byref(cfg))
+    h = C.c_void_p()
+    assert lib.ge_create(C.byref(h), None) == -1  # NULL config -> GE_E_INVALID
+    cfg.capacity = 0
+    assert lib.ge_create(C.byref(h), C.byref(cfg)) == -1
+    cfg.capacity = 1000
+    rc = lib.ge_create(C.byref(h), C.byref(cfg))
+    assert rc in (0, -7)  # GE_OK on a GPU host, GE_E_NODEVICE otherwise
+    if rc == 0:
+        assert lib.ge_destroy(h) == 0
+    assert lib.ge_version().decode().startswith("galaxy-step")
+
+
+def test_config_generators_shapes():
+    from galaxy_step import configs
+    st, p = configs.config0()
+    assert len(st["x"]) == 20000 and st["mass"][-1] == 5e4
+    assert p["theta"] == 0.8 and p["dt"] == 0.01
+    st, p = configs.config2(n_target=2000, n_impactor=500)
+    assert len(st["x"]) == 2500 and p["sph"] == 1
+    n, s = configs.config3_sizes()
+    assert len(n) == 512 and n.min() >= 1000 and n.max() <= 131072
+    reqs = configs.config4_stream(50, window_s=2.0)
+    assert all(2000 <= r[1] <= 500000 and 10 <= r[2] <= 200 for r in reqs)

@@ -1,0 +1,214 @@
+"""GPU parity of the gravity path through the C ABI against the oracle.
+
+Integer and index work is bit-exact: bounding box, Morton keys, sort
+permutation, and the whole quadtree (ranges, pre-order skip counts, node
+geometry, and masses / centres of mass, which follow the reference's rounding
+order).  Accelerations use v_rsq_f32 and fused multiply-adds, so they are
+compared with a tolerance: per particle |a_gpu - a_ref| <= 2e-5 |a_ref| +
+1e-6 max|a_ref|, and the accepted-interaction counts must match exactly
+(same tree, same unfused acceptance test).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from galaxy_step import Context, default_params, configs
+
+pytestmark = pytest.mark.gpu
+
+ACC_RTOL = 2e-5
+ACC_ATOL_FRAC = 1e-6
+
+
+def _params(**kw):
+    p = default_params()
+    p.G, p.theta, p.softening, p.dt, p.dt_start = 1.0, 0.8, 1.0, 0.01, 0.01
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _oracle_sorted(st):
+    x, y = st["x"], st["y"]
+    bb = O.bbox(x, y)
+    keys = O.morton_keys(x, y, bb)
+    perm = O.sort_perm(keys)
+    return bb, keys, perm
+
+
+def _assert_acc_close(ax, ay, rx, ry):
+    mag = np.hypot(rx, ry)
+    tol = ACC_RTOL * mag + ACC_ATOL_FRAC * mag.max()
+    err = np.hypot(ax - rx, ay - ry)
+    bad = np.nonzero(err > tol)[0]
+    assert bad.size == 0, (bad[:10], err[bad[:10]], mag[bad[:10]])
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    st, prm = configs.config0()
+    return st, prm
+
+
+def test_bbox_keys_perm_bitexact(cfg0):
+    st, _ = cfg0
+    n = len(st["x"])
+    bb_ref, keys_ref, perm_ref = _oracle_sorted(st)
+    with Context(n) as c:
+        c.upload(st)
+        bb = c.bounding_box()
+        np.testing.assert_array_equal(bb, bb_ref)
+        c.compute_morton_keys(bb)
+        np.testing.assert_array_equal(c.keys(), keys_ref)
+        c.sort_by_morton_key()
+        np.testing.assert_array_equal(c.perm(), perm_ref)
+        d = c.download()
+        np.testing.assert_array_equal(d["x"], st["x"][perm_ref])
+        np.testing.assert_array_equal(d["id"], st["id"][perm_ref])
+        np.testing.assert_array_equal(d["mass"], st["mass"][perm_ref])
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 257, 4999, 5000, 20000])
+def test_tree_bitexact_sizes(n):
+    st, _ = configs.config0(seed=11, n=max(n, 2))
+    st = {k: v[:n] for k, v in st.items()}
+    bb, keys, perm = _oracle_sorted(st)
+    xs, ys, ms = st["x"][perm], st["y"][perm], st["mass"][perm]
+    ref = O.build_tree(xs, ys, ms, bb)
+    with Context(n) as c:
+        c.upload(st)
+        c.compute_morton_keys(bb)
+        c.sort_by_morton_key()
+        c.build_quadtree(bb)
+        got = c.nodes()
+    assert len(got) == len(ref)
+    for f in ("start", "end", "next", "child", "posx", "posy", "size", "mass", "comx", "comy"):
+        np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+
+
+def _run_gravity(st, prm_kw, theta):
+    n = len(st["x"])
+    bb, keys, perm = _oracle_sorted(st)
+    xs, ys, ms = st["x"][perm], st["y"][perm], st["mass"][perm]
+    with Context(n) as c:
+        c.upload(st)
+        c.compute_morton_keys(bb)
+        c.sort_by_morton_key()
+        c.build_quadtree(bb)
+        p = _params(theta=theta, **prm_kw)
+        c.compute_gravity(p)
+        d = c.download(["ax", "ay"])
+        inter = c.interactions()
+    return xs, ys, ms, bb, d, inter
+
+
+@pytest.mark.parametrize("theta", [0.5, 0.8, 1.0])
+@pytest.mark.parametrize("compensated", [0, 1])
+def test_bh_accel_matches_oracle(cfg0, theta, compensated):
+    st, _ = cfg0
+    xs, ys, ms, bb, d, inter = _run_gravity(st, dict(compensated=compensated), theta)
+    nodes = O.build_tree(xs, ys, ms, bb)
+    rx, ry, cnt = O.bh_accel(xs, ys, ms, nodes, 1.0, theta, 1.0)
+    assert inter == cnt
+    _assert_acc_close(d["ax"], d["ay"], rx, ry)
+
+
+def test_bh_accel_million_two_disks():
+    st, prm = configs.config1(n=1_000_000)
+    xs, ys, ms, bb, d, inter = _run_gravity(st, {}, 0.7)
+    nodes = O.build_tree(xs, ys, ms, bb)
+    rng = np.random.default_rng(0)
+    t = np.sort(rng.choice(len(xs), 4096, replace=False)).astype(np.uint32)
+    # oracle forces on the sample (full walk per target)
+    rx, ry, _ = O.bh_accel(xs, ys, ms, nodes, 1.0, 0.7, 1.0)
+    _assert_acc_close(d["ax"][t], d["ay"][t], rx[t], ry[t])
+    # FP64 direct-sum error of the reference algorithm at theta = 0.7
+    fx, fy = O.direct_fp64(xs, ys, ms, 1.0, 1.0, t)
+    e = np.hypot(d["ax"][t] - fx, d["ay"][t] - fy) / np.hypot(fx, fy)
+    rms = math.sqrt(np.mean(e ** 2))
+    print(f"BH theta=0.7 RMS rel force error vs FP64 direct: {rms:.3e}, interactions/particle "
+          f"{inter / len(xs):.1f}")
+    assert rms < 2e-2
+
+
+@pytest.mark.parametrize("n", [3, 100, 1000, 4999])
+def test_direct_sum_small_n(n):
+    st, _ = configs.config0(seed=5, n=max(n, 3))
+    st = {k: v[:n] for k, v in st.items()}
+    bb, keys, perm = _oracle_sorted(st)
+    xs, ys, ms = st["x"][perm], st["y"][perm], st["mass"][perm]
+    with Context(n) as c:
+        c.upload(st)
+        c.compute_morton_keys(bb)
+        c.sort_by_morton_key()
+        c.build_quadtree(bb)
+        c.compute_gravity(_params())  # N < 5000 -> direct sum, softening unsquared
+        d = c.download(["ax", "ay"])
+    rx, ry = O.direct_accel(xs, ys, ms, 1.0, 1.0)
+    _assert_acc_close(d["ax"], d["ay"], rx, ry)
+
+
+def _by_id(d):
+    o = np.argsort(d["id"])
+    return {k: v[o] for k, v in d.items()}
+
+
+@pytest.mark.parametrize("n,steps", [(20000, 10), (3000, 10)])
+def test_step_trajectory_matches_oracle(n, steps):
+    st, prm = configs.config0(seed=7, n=n)
+    p = _params(theta=prm["theta"], dt=prm["dt"], dt_start=prm["dt"])
+    s = O.State(st)
+    s.step(dict(prm, dt_start=prm["dt"]), nsteps=steps)
+    ref = _by_id({k: s[k].copy() for k in ("x", "y", "vx", "vy", "ax", "ay", "id")})
+    with Context(n) as c:
+        c.upload(st)
+        c.step(p, steps)
+        c.sync()
+        got = _by_id(c.download(["x", "y", "vx", "vy", "ax", "ay", "id"]))
+    ext = float(np.ptp(ref["x"]))
+    assert np.abs(got["x"] - ref["x"]).max() < 1e-4 * ext
+    assert np.abs(got["y"] - ref["y"]).max() < 1e-4 * ext
+    vs = float(np.abs(ref["vx"]).max())
+    assert np.abs(got["vx"] - ref["vx"]).max() < 1e-3 * vs
+
+
+def test_upload_edge_cases():
+    with Context(10) as c:
+        c.upload({"x": np.zeros(0), "y": np.zeros(0), "mass": np.zeros(0)})
+        c.step(_params(), 3)
+        assert c.count() == 0
+        # one particle: no force, drifts with its velocity
+        c.upload({"x": np.array([1.0]), "y": np.array([2.0]), "vx": np.array([1.0]),
+                  "mass": np.array([1.0])})
+        c.step(_params(dt=0.5, dt_start=0.5), 2)
+        d = c.download(["x", "y", "ax"])
+        assert d["x"][0] == pytest.approx(2.0) and d["ax"][0] == 0.0
+        with pytest.raises(Exception):
+            c.upload({"x": np.zeros(11), "y": np.zeros(11), "mass": np.ones(11)})
+
+
+def test_coincident_particles():
+    # all particles at one point: the root is a leaf (size 0 <= 1e-6)
+    n = 6000
+    st = {"x": np.full(n, 3.0), "y": np.full(n, -2.0), "mass": np.ones(n)}
+    with Context(n) as c:
+        c.upload(st)
+        bb = c.bounding_box()
+        c.compute_morton_keys(bb)
+        c.sort_by_morton_key()
+        c.build_quadtree(bb)
+        assert c.node_count() == 1
+        c.compute_gravity(_params())
+        d = c.download(["ax", "ay"])
+    assert np.all(d["ax"] == 0) and np.all(d["ay"] == 0)
+
+
+def test_unsupported_flags_rejected():
+    with Context(100) as c:
+        c.upload({"x": np.arange(100.0), "y": np.zeros(100), "mass": np.ones(100)})
+        with pytest.raises(Exception):
+            c.step(_params(periodic=1), 1)
+        with pytest.raises(Exception):
+            c.step(_params(temperature=1), 1)

@@ -1,0 +1,318 @@
+"""Pins the CPU restatement (oracle/) before it is trusted as the checker.
+
+The reference ships no tests or fixtures for this path, and cannot be built
+here (its headers need glm, which is absent).  Each oracle function is pinned
+by something independent of its own code:
+
+  geo_morton2d / geo_scale_to_grid / geo_bbox -> tests/golden/kat.json
+      (explicit bit interleave, float32 numpy evaluation)
+  geo_sort_perm   -> numpy stable argsort
+  geo_build_tree  -> the hand-checked tree in kat.json, plus structural
+      invariants (ranges partition, next = subtree size - 1, pre-order,
+      leaf rule, children inside parent squares) and FP64 mass / COM sums
+  geo_bh_accel    -> FP64 direct sum: theta -> 0 reduces BH to the direct sum,
+      and the two-body closed form; theta = 0.7 error bound
+  geo_direct_accel-> closed forms with the unsquared softening, momentum
+      conservation (Newton's third law)
+  geo_direct_fp64 -> two-body closed form
+  geo_integrate_* -> kick-drift-kick on a harmonic force: closed-form one-step
+      update
+  geo_sph_candidates -> O(N^2) brute force over cell coordinates
+  geo_sph_constants  -> closed forms in kat.json; W integrates to 1
+  geo_sph_pcisph  -> symmetry: pair forces cancel (momentum conserved) for
+      equal masses; density of an isolated particle equals its self term.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+KAT = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kat.json")))
+
+
+def test_morton_kat():
+    for ix, iy, k in KAT["morton"]:
+        assert O.morton2d(ix, iy) == k
+
+
+def test_scale_to_grid_kat():
+    for p, lo, hi, v in KAT["scale_to_grid"]:
+        assert O.scale_to_grid(p, lo, hi) == v, (p, lo, hi)
+
+
+def test_bbox_kat():
+    for case in KAT["bbox"]:
+        bb = O.bbox(case["x"], case["y"])
+        np.testing.assert_array_equal(bb, np.array(case["bb"], np.float32))
+
+
+def test_morton_keys_matches_kat_map():
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-100, 100, 500).astype(np.float32)
+    y = rng.uniform(-30, 70, 500).astype(np.float32)
+    bb = O.bbox(x, y)
+    keys = O.morton_keys(x, y, bb)
+    ext = np.float32(max(bb[2], np.float32(1e-6)))
+    hx, hy = np.float32(bb[0] + ext), np.float32(bb[1] + ext)
+    for i in range(0, 500, 7):
+        ix = O.scale_to_grid(x[i], bb[0], hx)
+        iy = O.scale_to_grid(y[i], bb[1], hy)
+        k = 0
+        for b in range(18):
+            k |= ((ix >> b) & 1) << (2 * b) | ((iy >> b) & 1) << (2 * b + 1)
+        assert int(keys[i]) == k
+    # the particle on the max edge clamps to 262143 (morton.cpp:9)
+    j = int(np.argmax(x))
+    assert O.scale_to_grid(x[j], bb[0], hx) <= 262143
+
+
+def test_sort_perm_is_stable_argsort():
+    rng = np.random.default_rng(1)
+    keys = rng.integers(0, 50, 3000).astype(np.uint64)  # many ties
+    p = O.sort_perm(keys)
+    np.testing.assert_array_equal(p, np.argsort(keys, kind="stable").astype(np.uint32))
+
+
+def test_tree_hand_checked():
+    t = KAT["tree"]
+    pts = np.array(t["points"], np.float32)
+    m = np.ones(len(pts), np.float32)
+    nodes = O.build_tree(pts[:, 0], pts[:, 1], m, np.array(t["bb"], np.float32))
+    got = [(int(n["start"]), int(n["end"]), int(n["next"]), float(n["size"])) for n in nodes]
+    assert got == [tuple(v) for v in t["nodes"]]
+
+
+def _check_tree(nodes, x, y, m):
+    nn = len(nodes)
+    n = len(x)
+    assert nodes[0]["start"] == 0 and nodes[0]["end"] == n
+    assert nodes[0]["next"] == nn - 1
+    for i in range(nn):
+        nd = nodes[i]
+        cnt = int(nd["end"] - nd["start"])
+        assert cnt >= 1
+        is_leaf = ((cnt <= 4 and nd["size"] <= 2.0) or cnt == 1) or nd["size"] <= 1e-6
+        ch = [c for c in nd["child"] if c != 0xFFFFFFFF]
+        if is_leaf:
+            assert nd["next"] == 0 and not ch
+            mass = m[nd["start"]:nd["end"]].astype(np.float64).sum()
+            assert abs(nd["mass"] - mass) <= 1e-5 * mass + 1e-30
+        else:
+            assert ch, i
+            # children are consecutive in pre-order and tile the parent range
+            assert ch[0] == i + 1
+            s = nd["start"]
+            nxt = i + 1
+            tot = 0
+            for c in ch:
+                assert c == nxt
+                assert nodes[c]["start"] == s
+                s = nodes[c]["end"]
+                tot += 1 + int(nodes[c]["next"])
+                nxt = c + int(nodes[c]["next"]) + 1
+                assert nodes[c]["size"] == np.float32(nd["size"] * np.float32(0.5))
+            assert s == nd["end"]
+            assert tot == nd["next"]
+        # COM is the mass-weighted mean of the particles in range (FP64 check)
+        sl = slice(int(nd["start"]), int(nd["end"]))
+        mm = m[sl].astype(np.float64)
+        if mm.sum() > 0:
+            cx = (x[sl].astype(np.float64) * mm).sum() / mm.sum()
+            cy = (y[sl].astype(np.float64) * mm).sum() / mm.sum()
+            scale = max(1.0, abs(cx), abs(cy))
+            assert abs(nd["comx"] - cx) <= 1e-4 * scale and abs(nd["comy"] - cy) <= 1e-4 * scale
+
+
+def _sorted_cloud(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 50, n).astype(np.float32)
+    y = rng.normal(0, 20, n).astype(np.float32)
+    m = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    bb = O.bbox(x, y)
+    p = O.sort_perm(O.morton_keys(x, y, bb))
+    return x[p], y[p], m[p], bb
+
+
+def test_tree_invariants():
+    x, y, m, bb = _sorted_cloud(3000, 2)
+    nodes = O.build_tree(x, y, m, bb)
+    _check_tree(nodes, x, y, m)
+    # particles lie inside their leaf's square (closed on the lower edges)
+    for nd in nodes:
+        if nd["next"] == 0:
+            sl = slice(int(nd["start"]), int(nd["end"]))
+            assert np.all(x[sl] >= nd["posx"] - 1e-3) and np.all(x[sl] <= nd["posx"] + nd["size"] + 1e-3)
+            assert np.all(y[sl] >= nd["posy"] - 1e-3) and np.all(y[sl] <= nd["posy"] + nd["size"] + 1e-3)
+
+
+def test_tree_duplicates_and_single():
+    x = np.array([1.0, 1.0, 1.0, 1.0, 1.0, 1.0], np.float32)
+    y = np.array([2.0] * 6, np.float32)
+    m = np.ones(6, np.float32)
+    nodes = O.build_tree(x, y, m, O.bbox(x, y))
+    assert len(nodes) == 1 and nodes[0]["next"] == 0  # size 0 <= 1e-6 -> leaf
+    nodes = O.build_tree(x[:1], y[:1], m[:1], O.bbox(x[:1], y[:1]))
+    assert len(nodes) == 1
+
+
+def test_bh_two_body_closed_form():
+    x = np.array([0.0, 3.0], np.float32)
+    y = np.array([0.0, 4.0], np.float32)
+    m = np.array([2.0, 5.0], np.float32)
+    bb = O.bbox(x, y)
+    nodes = O.build_tree(x, y, m, bb)
+    ax, ay, cnt = O.bh_accel(x, y, m, nodes, 1.0, 0.5, 1.0)
+    r2 = 25.0 + 1.0
+    f = 1.0 / r2 ** 1.5
+    np.testing.assert_allclose([ax[0], ay[0]], [5 * 3 * f, 5 * 4 * f], rtol=1e-6)
+    np.testing.assert_allclose([ax[1], ay[1]], [-2 * 3 * f, -2 * 4 * f], rtol=1e-6)
+    dx, dy = O.direct_fp64(x, y, m, 1.0, 1.0)
+    np.testing.assert_allclose(ax, dx, rtol=1e-6)
+
+
+def test_bh_theta_zero_is_direct_sum():
+    x, y, m, bb = _sorted_cloud(1500, 3)
+    nodes = O.build_tree(x, y, m, bb)
+    ax, ay, cnt = O.bh_accel(x, y, m, nodes, 1.0, 0.0, 1.0)
+    assert cnt == 1500 * 1499
+    rx, ry = O.direct_fp64(x, y, m, 1.0, 1.0)
+    err = np.hypot(ax - rx, ay - ry) / np.hypot(rx, ry)
+    assert err.max() < 1e-4
+
+
+def test_bh_theta_error_bound():
+    from galaxy_step import configs
+    st, _ = configs.config0(n=5000)
+    x, y, m = st["x"], st["y"], st["mass"]
+    bb = O.bbox(x, y)
+    p = O.sort_perm(O.morton_keys(x, y, bb))
+    x, y, m = x[p], y[p], m[p]
+    nodes = O.build_tree(x, y, m, bb)
+    rx, ry = O.direct_fp64(x, y, m, 1.0, 1.0)
+    prev = None
+    prev_rms = 0.0
+    for th in (0.3, 0.7, 1.0):
+        ax, ay, cnt = O.bh_accel(x, y, m, nodes, 1.0, th, 1.0)
+        rms = math.sqrt(np.mean((np.hypot(ax - rx, ay - ry) / np.hypot(rx, ry)) ** 2))
+        assert rms < 0.05
+        assert rms > prev_rms  # monopole truncation error grows with theta
+        prev_rms = rms
+        if prev is not None:
+            assert cnt < prev  # a larger opening angle accepts more nodes
+        prev = cnt
+
+
+def test_direct_unsquared_softening_and_momentum():
+    x = np.array([0.0, 2.0], np.float32)
+    y = np.array([0.0, 0.0], np.float32)
+    m = np.array([1.0, 3.0], np.float32)
+    ax, ay = O.direct_accel(x, y, m, 1.0, 0.5)  # eps added unsquared: r^2 + 0.5
+    f = 1.0 / (4.0 + 0.5) ** 1.5
+    np.testing.assert_allclose(ax, [3 * 2 * f, -1 * 2 * f], rtol=1e-6)
+    rng = np.random.default_rng(5)
+    x = rng.normal(0, 10, 700).astype(np.float32)
+    y = rng.normal(0, 10, 700).astype(np.float32)
+    m = rng.uniform(0.5, 2, 700).astype(np.float32)
+    ax, ay = O.direct_accel(x, y, m, 1.0, 1.0)
+    px = (m.astype(np.float64) * ax).sum() / (m * np.abs(ax)).sum()
+    assert abs(px) < 1e-5
+
+
+def test_integrator_kdk_closed_form():
+    st = {"x": np.array([1.0, 2.0]), "y": np.array([0.0, 1.0]), "vx": np.array([0.5, 0.0]),
+          "vy": np.array([0.0, -1.0]), "mass": np.array([1.0, 1.0]),
+          "ax": np.array([0.25, -1.0]), "ay": np.array([0.0, 2.0]),
+          "flags": np.array([0, O.GEO_PINNED])}
+    s = O.State(st)
+    # G = 0 -> acc zeroed; one frame = kick(dt/2, old acc), drift(dt), kick(dt/2, 0)
+    s.step(dict(G=0.0, dt=0.1, naive=1))
+    ids = s["id"].tolist()
+    i0, i1 = ids.index(0), ids.index(1)
+    vx = np.float32(np.float32(0.5) + np.float32(np.float32(0.25) * np.float32(0.05)))
+    assert s["vx"][i0] == vx
+    assert s["x"][i0] == np.float32(np.float32(1.0) + np.float32(vx * np.float32(0.1)))
+    assert s["pvx"][i0] == np.float32(0.5)
+    # pinned particle untouched
+    assert s["x"][i1] == np.float32(2.0) and s["vy"][i1] == np.float32(-1.0)
+
+
+def test_sph_candidates_brute_force():
+    rng = np.random.default_rng(6)
+    n = 800
+    x = rng.uniform(-20, 20, n).astype(np.float32)
+    y = rng.uniform(-15, 15, n).astype(np.float32)
+    qx = (x + rng.normal(0, 0.5, n)).astype(np.float32)
+    qy = (y + rng.normal(0, 0.5, n)).astype(np.float32)
+    cell = 3.0
+    off, idx = O.sph_candidates(x, y, qx, qy, cell)
+    cx = np.floor(x / np.float32(cell)).astype(np.int64)
+    cy = np.floor(y / np.float32(cell)).astype(np.int64)
+    qcx = np.floor(qx / np.float32(cell)).astype(np.int64)
+    qcy = np.floor(qy / np.float32(cell)).astype(np.int64)
+    for i in range(0, n, 13):
+        want = np.nonzero((np.abs(cx - qcx[i]) <= 1) & (np.abs(cy - qcy[i]) <= 1))[0]
+        np.testing.assert_array_equal(idx[off[i]:off[i + 1]], want.astype(np.uint32))
+
+
+def test_sph_constants_closed_form():
+    for c in KAT["sph_constants"]:
+        k = O.sph_constants(c["h"])
+        np.testing.assert_allclose(k, [c["volume"], c["spiky"], c["lap"]], rtol=2e-7)
+    # W = (h - r)^2 / volume integrates to 1 over the disk of radius h
+    h = 3.0
+    vol = O.sph_constants(h)[0]
+    r = np.linspace(0, h, 200001)
+    w = (h - r) ** 2 / vol * 2 * math.pi * r
+    assert abs(np.trapezoid(w, r) - 1.0) < 1e-5
+
+
+def _sph_state(n, seed, spacing=1.2):
+    rng = np.random.default_rng(seed)
+    side = int(math.ceil(math.sqrt(n)))
+    gx, gy = np.meshgrid(np.arange(side), np.arange(side))
+    x = (gx.ravel()[:n] * spacing + rng.normal(0, 0.2, n)).astype(np.float32)
+    y = (gy.ravel()[:n] * spacing + rng.normal(0, 0.2, n)).astype(np.float32)
+    st = {"x": x, "y": y, "vx": rng.normal(0, 1, n), "vy": rng.normal(0, 1, n),
+          "mass": np.ones(n), "flags": np.full(n, O.GEO_SPH), "sph_mass": np.ones(n),
+          "rest_dens": np.full(n, 0.008), "stiff": np.ones(n), "visc": np.ones(n),
+          "cohesion": np.ones(n), "dens": np.zeros(n)}
+    return st
+
+
+def test_sph_momentum_conserved():
+    st = _sph_state(900, 7)
+    s = O.State(st, sph=True)
+    s.step(dict(G=0.0, dt=0.002, sph=1, sph_h=3.0, sph_cell=3.0), nsteps=1)
+    fx = s["pfx"].astype(np.float64)  # sphMass = 1 -> pressF is the force
+    fy = s["pfy"].astype(np.float64)
+    scale = np.abs(fx).sum() + np.abs(fy).sum()
+    assert scale > 0
+    assert abs(fx.sum()) < 1e-4 * scale and abs(fy.sum()) < 1e-4 * scale
+
+
+def test_sph_isolated_density_is_self_term():
+    st = _sph_state(4, 8)
+    st["x"] = np.array([0.0, 100.0, 200.0, 300.0], np.float32)
+    st["y"] = np.zeros(4, np.float32)
+    s = O.State(st, sph=True)
+    s.step(dict(G=0.0, dt=0.002, sph=1, sph_h=3.0, sph_cell=3.0), nsteps=1)
+    vol = O.sph_constants(3.0)[0]
+    want = np.float32(np.float32(np.float32(0.03) * np.float32(np.float32(9.0) / vol)) / np.float32(0.008))
+    np.testing.assert_allclose(s["pred_dens"], np.full(4, want), rtol=1e-6)
+    assert np.all(s["pfx"] == 0) and np.all(s["pfy"] == 0)
+
+
+def test_full_step_permutes_consistently():
+    from galaxy_step import configs
+    st, prm = configs.config0(n=6000)
+    s = O.State(st)
+    s.step(dict(prm, theta=0.8), nsteps=2)
+    ids = s["id"]
+    assert sorted(ids.tolist()) == list(range(6000))
+    # the stored order is the Morton order of the current positions' frame
+    assert np.all(np.isfinite(s["x"])) and np.all(np.isfinite(s["vx"]))
+    np.testing.assert_array_equal(s["mass"], st["mass"][ids])
