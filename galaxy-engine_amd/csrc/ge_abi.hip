@@ -59,7 +59,7 @@ void free_all(ge_ctx *c) {
     dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.bb); dfree(b.interactions);
     TopNodes &T = b.top;
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
-    dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset);
+    dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset); dfree(T.broot);
     dfree(b.nodes); dfree(b.node_geom); dfree(b.sphf);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
         dfree(g->keys); dfree(g->keys_alt); dfree(g->cell_start); dfree(g->cell_end); dfree(g->frame);
@@ -93,6 +93,11 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(T.count, T.cap)); GE_HIP_CHECK(c, dalloc(T.pending, T.cap));
     GE_HIP_CHECK(c, dalloc(T.accum, T.cap)); GE_HIP_CHECK(c, dalloc(T.pending2, T.cap));
     GE_HIP_CHECK(c, dalloc(T.offset, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.broot, T.cap));
+    // zeroed top nodes read as "internal" (never emitted) until a build writes them
+    for (uint32_t *u : {T.start, T.end, T.parent, T.kind, T.count, T.pending, T.accum, T.pending2, T.offset, T.broot})
+        GE_HIP_CHECK(c, hipMemset(u, 0, sizeof(uint32_t) * T.cap));
+    GE_HIP_CHECK(c, hipMemset(T.child, 0xFF, sizeof(uint32_t) * 4ull * T.cap));
     // Quadtree::root reserves 4N nodes (quadtree.cpp:94)
     b.node_cap = 4u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(b.nodes, b.node_cap));
@@ -121,10 +126,11 @@ int check_device_errors(ge_ctx *c) {
     GE_HIP_CHECK(c, hipMemcpyAsync(&e, c->b.counters + kCntError, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     if (e) {
-        char buf[160];
-        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s%s)", e,
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s%s%s)", e,
                       (e & kErrNodeCap) ? "node capacity " : "", (e & kErrDepth) ? "tree depth " : "",
-                      (e & kErrGridRange) ? "SPH grid range " : "", (e & kErrNeighborCap) ? "neighbour cap" : "");
+                      (e & kErrGridRange) ? "SPH grid range " : "", (e & kErrNeighborCap) ? "neighbour cap " : "",
+                      (e & kErrFrontier) ? "tree frontier" : "");
         c->err = buf;
         uint32_t z = 0;
         (void)hipMemcpy(c->b.counters + kCntError, &z, sizeof z, hipMemcpyHostToDevice);

@@ -18,10 +18,13 @@ constexpr uint64_t kIdxMask = (1ull << kIdxBits) - 1;
 constexpr uint32_t kMaxParticles = 1u << kIdxBits;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-// Tree build: levels built one launch each, then whole subtrees per thread.
-constexpr int kTopLevels = 12;         // launches of the level-synchronous phase
-constexpr uint32_t kSubtreeMax = 32;   // nodes with at most this many particles finish in-thread
-constexpr int kMaxDepth = 48;          // in-thread recursion depth below a subtree root
+// Tree build: one launch per level for nodes above kBlockSubMax particles,
+// then one workgroup per subtree of at most kBlockSubMax particles, then one
+// wave per subtree of at most 64 particles.
+constexpr int kTopLevels = 16;         // launches of the grid-wide level phase
+constexpr uint32_t kBlockSubMax = 4096;
+constexpr int kMaxDepth = 48;          // wave-subtree recursion depth (LDS stack)
+constexpr int kBlockLevelsMax = 160;   // local levels a workgroup subtree may take
 
 // device error flag bits
 enum : uint32_t {
@@ -29,6 +32,7 @@ enum : uint32_t {
     kErrDepth = 2u,      // in-thread recursion deeper than kMaxDepth
     kErrGridRange = 4u,  // SPH grid cannot index the coordinates
     kErrNeighborCap = 8u,
+    kErrFrontier = 16u,  // workgroup subtree frontier or level bound exceeded
 };
 
 // counters block (device, uint32 each)
@@ -38,6 +42,8 @@ enum CounterSlot : int {
     kCntPredTicket = 2,
     kCntNodeTotal = 3,      // final node count (root subtree size)
     kCntTopTotal = 4,       // number of top nodes
+    kCntTopAlloc = 5,       // next free top node (workgroup-subtree phase)
+    kCntBlockRoots = 6,     // number of workgroup-subtree roots
     kCntLevel0 = 8,         // kCntLevel0 + L: nodes allocated on level L
     kCntLevelBase0 = 8 + 64,  // first top index of level L
     kCntSlots = 8 + 128,
@@ -70,6 +76,7 @@ struct TopNodes {
     uint32_t *accum = nullptr;     // sum of reported child counts
     uint32_t *pending2 = nullptr;  // children still to report their mass
     uint32_t *offset = nullptr;    // pre-order index in the final array
+    uint32_t *broot = nullptr;     // roots handed to the workgroup-subtree phase
     uint32_t cap = 0;
 };
 

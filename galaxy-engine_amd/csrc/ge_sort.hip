@@ -1,7 +1,7 @@
 // ge_sort.hip -- stable LSD radix sort of packed 64-bit sort words
 // (key << 28 | slot), used for the Morton order (morton.cpp:40-63) and the
 // SPH cell order (neighborSearch.h:636-720).  Three kernels per 8-bit digit:
-// per-tile digit counts, one-workgroup scan of the (digit, tile) table, and a
+// per-tile digit counts, one-workgroup scan of the (tile, digit) table, and a
 // stable scatter ranked with wave ballots.  Because the slot sits in the low
 // bits and LSD passes are stable, ties keep ascending slot order.
 #include "ge_internal.hpp"
@@ -17,8 +17,7 @@ constexpr int kScanThreads = 1024;
 
 __global__ __launch_bounds__(kSortThreads) void k_digit_count(const uint64_t *__restrict__ in,
                                                               uint32_t n, int shift, uint32_t dmask,
-                                                              uint32_t *__restrict__ hist,
-                                                              uint32_t nb) {
+                                                              uint32_t *__restrict__ hist) {
     __shared__ uint32_t cnt[256];
     const uint32_t t = threadIdx.x;
     cnt[t] = 0;
@@ -33,33 +32,63 @@ __global__ __launch_bounds__(kSortThreads) void k_digit_count(const uint64_t *__
         }
     }
     __syncthreads();
-    // digit-major layout so one linear exclusive scan yields scatter offsets
-    hist[(uint64_t)t * nb + blockIdx.x] = cnt[t];
+    hist[(uint64_t)blockIdx.x * 256 + t] = cnt[t];  // tile-major table
 }
 
-// Exclusive scan of len entries by one workgroup: each thread owns a
-// contiguous chunk; chunk sums are scanned across the workgroup.
-__global__ __launch_bounds__(kScanThreads) void k_scan_table(uint32_t *__restrict__ data, uint32_t len) {
-    __shared__ uint32_t sums[kScanThreads];
-    const uint32_t t = threadIdx.x;
-    const uint32_t chunk = (len + kScanThreads - 1) / kScanThreads;
-    const uint32_t lo = t * chunk;
-    const uint32_t hi = min(len, lo + chunk);
-    uint32_t s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += data[i];
-    sums[t] = s;
+// Turns the (tile, digit) count table into scatter offsets: offset of digit d
+// in tile b = sum of all counts of smaller digits + counts of d in tiles < b.
+// Thread (g, d) owns digit d over a quarter g of the tiles; every pass reads
+// the table with coalesced, independent loads.
+__global__ __launch_bounds__(kScanThreads) void k_scan_table(uint32_t *__restrict__ hist, uint32_t nb) {
+    __shared__ uint32_t part[4][256];
+    __shared__ uint32_t dsum[256];
+    const uint32_t d = threadIdx.x & 255u;
+    const uint32_t g = threadIdx.x >> 8;
+    const uint32_t b0 = (uint32_t)((uint64_t)nb * g / 4), b1 = (uint32_t)((uint64_t)nb * (g + 1) / 4);
+    uint32_t sum = 0;
+    uint32_t b = b0;
+    for (; b + 8 <= b1; b += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = hist[(uint64_t)(b + k) * 256 + d];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sum += v[k];
+    }
+    for (; b < b1; ++b) sum += hist[(uint64_t)b * 256 + d];
+    part[g][d] = sum;
     __syncthreads();
-    // Hillis-Steele inclusive scan over the chunk sums
-    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
-        uint32_t v = t >= off ? sums[t - off] : 0u;
+    if (g == 0) {
+        const uint32_t t0 = part[0][d], t1 = part[1][d], t2 = part[2][d], t3 = part[3][d];
+        part[0][d] = 0;
+        part[1][d] = t0;
+        part[2][d] = t0 + t1;
+        part[3][d] = t0 + t1 + t2;
+        dsum[d] = t0 + t1 + t2 + t3;
+    }
+    __syncthreads();
+    // exclusive scan of the 256 digit totals
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        uint32_t v = 0;
+        if (g == 0 && d >= off) v = dsum[d - off];
         __syncthreads();
-        sums[t] += v;
+        if (g == 0) dsum[d] += v;
         __syncthreads();
     }
-    uint32_t run = t == 0 ? 0u : sums[t - 1];
-    for (uint32_t i = lo; i < hi; ++i) {
-        uint32_t v = data[i];
-        data[i] = run;
+    uint32_t run = (d == 0 ? 0u : dsum[d - 1]) + part[g][d];
+    b = b0;
+    for (; b + 8 <= b1; b += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = hist[(uint64_t)(b + k) * 256 + d];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            hist[(uint64_t)(b + k) * 256 + d] = run;
+            run += v[k];
+        }
+    }
+    for (; b < b1; ++b) {
+        const uint32_t v = hist[(uint64_t)b * 256 + d];
+        hist[(uint64_t)b * 256 + d] = run;
         run += v;
     }
 }
@@ -67,8 +96,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_table(uint32_t *__restric
 __global__ __launch_bounds__(kSortThreads) void k_scatter(const uint64_t *__restrict__ in,
                                                           uint64_t *__restrict__ out, uint32_t n,
                                                           int shift, int bits,
-                                                          const uint32_t *__restrict__ offs,
-                                                          uint32_t nb) {
+                                                          const uint32_t *__restrict__ offs) {
     __shared__ uint32_t wave_cnt[kSortWaves][256];
     __shared__ uint32_t blk_off[256];
     const uint32_t t = threadIdx.x;
@@ -77,7 +105,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const uint64_t *__rest
     const uint32_t dmask = (1u << bits) - 1u;
 #pragma unroll
     for (int k = 0; k < kSortWaves; ++k) wave_cnt[k][t] = 0;
-    blk_off[t] = (t <= dmask) ? offs[(uint64_t)t * nb + blockIdx.x] : 0u;
+    blk_off[t] = (t <= dmask) ? offs[(uint64_t)blockIdx.x * 256 + t] : 0u;
     __syncthreads();
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -138,11 +166,10 @@ void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n, i
         int bits = hi_bit - shift < 8 ? hi_bit - shift : 8;
         uint32_t dmask = (1u << bits) - 1u;
         hipLaunchKernelGGL(k_digit_count, dim3(nb), dim3(kSortThreads), 0, c->stream, data, n, shift,
-                           dmask, c->b.hist, nb);
-        hipLaunchKernelGGL(k_scan_table, dim3(1), dim3(kScanThreads), 0, c->stream, c->b.hist,
-                           (uint32_t)(256u * nb));
+                           dmask, c->b.hist);
+        hipLaunchKernelGGL(k_scan_table, dim3(1), dim3(kScanThreads), 0, c->stream, c->b.hist, nb);
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kSortThreads), 0, c->stream, data, alt, n, shift,
-                           bits, c->b.hist, nb);
+                           bits, c->b.hist);
         uint64_t *t = data;
         data = alt;
         alt = t;

@@ -10,7 +10,7 @@ namespace ge {
 namespace {
 
 constexpr int kWave = 64;
-constexpr uint32_t kBboxMaxBlocks = 2048;
+constexpr uint32_t kBboxMaxBlocks = 512;
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
