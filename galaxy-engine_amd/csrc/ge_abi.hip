@@ -59,7 +59,8 @@ void free_all(ge_ctx *c) {
     dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.bb); dfree(b.interactions);
     TopNodes &T = b.top;
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
-    dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset); dfree(T.broot);
+    dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
+    dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
     dfree(b.nodes); dfree(b.node_geom); dfree(b.sphf);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
         dfree(g->keys); dfree(g->keys_alt); dfree(g->cell_start); dfree(g->cell_end); dfree(g->frame);
@@ -94,8 +95,13 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(T.accum, T.cap)); GE_HIP_CHECK(c, dalloc(T.pending2, T.cap));
     GE_HIP_CHECK(c, dalloc(T.offset, T.cap));
     GE_HIP_CHECK(c, dalloc(T.broot, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.depth, T.cap));
+    GE_HIP_CHECK(c, dalloc(b.start_count, (size_t)cap + 1));
+    GE_HIP_CHECK(c, dalloc(b.start_level, (size_t)cap + 1));
+    GE_HIP_CHECK(c, dalloc(b.scan_part, (size_t)cap / 1024 + 64));
     // zeroed top nodes read as "internal" (never emitted) until a build writes them
-    for (uint32_t *u : {T.start, T.end, T.parent, T.kind, T.count, T.pending, T.accum, T.pending2, T.offset, T.broot})
+    for (uint32_t *u : {T.start, T.end, T.parent, T.kind, T.count, T.pending, T.accum, T.pending2, T.offset, T.broot,
+                        T.depth})
         GE_HIP_CHECK(c, hipMemset(u, 0, sizeof(uint32_t) * T.cap));
     GE_HIP_CHECK(c, hipMemset(T.child, 0xFF, sizeof(uint32_t) * 4ull * T.cap));
     // Quadtree::root reserves 4N nodes (quadtree.cpp:94)

@@ -77,6 +77,7 @@ struct TopNodes {
     uint32_t *pending2 = nullptr;  // children still to report their mass
     uint32_t *offset = nullptr;    // pre-order index in the final array
     uint32_t *broot = nullptr;     // roots handed to the workgroup-subtree phase
+    uint32_t *depth = nullptr;     // tree level of the node
     uint32_t cap = 0;
 };
 
@@ -105,6 +106,9 @@ struct Buffers {
     float *bb = nullptr;           // device bbox {x, y, size, _} and predPos box at +4
     uint64_t *interactions = nullptr;
     TopNodes top;
+    uint32_t *start_count = nullptr;  // per particle: nodes whose range starts there (then its prefix)
+    uint32_t *start_level = nullptr;  // per particle: shallowest level of such a node
+    uint32_t *scan_part = nullptr;    // block sums of the node-start scan
     NodeRec *nodes = nullptr;
     float4 *node_geom = nullptr;   // debug: (posx, posy, size, 0) per final node
     uint32_t node_cap = 0;

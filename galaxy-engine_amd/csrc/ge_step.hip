@@ -10,7 +10,7 @@ namespace ge {
 namespace {
 
 constexpr int kWave = 64;
-constexpr uint32_t kBboxMaxBlocks = 512;
+constexpr uint32_t kBboxMaxBlocks = 1024;
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -40,12 +40,11 @@ __device__ __forceinline__ void block_minmax(float &a, float &b, float &c, float
 
 // Physics::integrateStart (physics.cpp:1069-1112) followed by the
 // boundingBox() reduction (globalLogic.cpp:1083-1099) over the drifted
-// positions.  The last workgroup to finish turns the partials into {x, y, size}.
+// positions: per-workgroup partial min/max here, the final box in
+// k_bbox_final (a second launch instead of a device-scope fence).
 __global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, uint32_t n, float dt, int sph_clamp,
                                                            float max_vel, int do_integrate,
-                                                           float *__restrict__ partials,
-                                                           uint32_t *__restrict__ ticket,
-                                                           float *__restrict__ bb) {
+                                                           float *__restrict__ partials) {
 #pragma clang fp contract(off)
     const float half_dt = dt * 0.5f;
     const float max_vsq = max_vel * max_vel;
@@ -80,24 +79,17 @@ __global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, uint32_t n, 
         mxx = glm_max(mxx, x); mxy = glm_max(mxy, y);
     }
     block_minmax(mnx, mny, mxx, mxy);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        float4 v = make_float4(mnx, mny, mxx, mxy);
-        reinterpret_cast<float4 *>(partials)[blockIdx.x] = v;
-        __threadfence();
-        uint32_t t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    mnx = FLT_MAX; mny = FLT_MAX; mxx = -FLT_MAX; mxy = -FLT_MAX;
-    for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) {
-        const float *pp = partials + 4 * k;
-        mnx = glm_min(mnx, atomic_load_f32(pp + 0));
-        mny = glm_min(mny, atomic_load_f32(pp + 1));
-        mxx = glm_max(mxx, atomic_load_f32(pp + 2));
-        mxy = glm_max(mxy, atomic_load_f32(pp + 3));
+    if (threadIdx.x == 0) reinterpret_cast<float4 *>(partials)[blockIdx.x] = make_float4(mnx, mny, mxx, mxy);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const float *__restrict__ partials, uint32_t nparts,
+                                                       float *__restrict__ bb) {
+#pragma clang fp contract(off)
+    float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
+    for (uint32_t k = threadIdx.x; k < nparts; k += kBlock) {
+        const float4 v = reinterpret_cast<const float4 *>(partials)[k];
+        mnx = glm_min(mnx, v.x); mny = glm_min(mny, v.y);
+        mxx = glm_max(mxx, v.z); mxy = glm_max(mxy, v.w);
     }
     block_minmax(mnx, mny, mxx, mxy);
     if (threadIdx.x == 0) {
@@ -109,7 +101,6 @@ __global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, uint32_t n, 
         bb[1] = cy - half;
         bb[2] = size;
         bb[3] = 0.0f;
-        *ticket = 0;  // ready for the next frame
     }
 }
 
@@ -350,8 +341,8 @@ void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_v
     uint32_t nb = blocks_for(c->n, kBlock);
     if (nb > kBboxMaxBlocks) nb = kBboxMaxBlocks;
     hipLaunchKernelGGL(k_integrate_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->b.st[c->b.cur], c->n, dt,
-                       sph_clamp, max_vel, do_integrate, c->b.partials, c->b.counters + kCntBboxTicket,
-                       c->b.bb);
+                       sph_clamp, max_vel, do_integrate, c->b.partials);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, c->stream, c->b.partials, nb, c->b.bb);
 }
 
 void launch_morton_keys(ge_ctx *c, const float *bb_dev) {

@@ -5,27 +5,31 @@
 // `next` skip counts, and the same mass / centre-of-mass rounding order
 // (quadtree.h:53-111).
 //
-// Device schedule (no host round trips, fixed launch count):
-//   1. k_tree_init: the root.
+// Pre-order without parent links: nodes of nested Morton ranges appear in DFS
+// pre-order sorted by (start particle, level).  Every node registers at its
+// start particle i: C[i] += 1 and F[i] = min(F[i], level).  After one
+// exclusive scan P of C, node (level l, range [a, b)) has pre-order index
+// P[a] + (l - F[a]) and next = P[b] - P[a] - (l - F[a]) - 1.  No kernel waits
+// on another workgroup: workgroups only meet at launch boundaries.
+//
+// Device schedule (fixed launch count, no host round trips):
+//   1. k_tree_clear: C, F and the root.
 //   2. kTopLevels launches of k_tree_level, one wave per pending node of
-//      level L.  A node with more than kBlockSubMax particles is split with a
+//      level L.  A node above kBlockSubMax particles is split with a
 //      speculative binary search: each round the wave evaluates the predicate
 //      at the 63 probes of the next six levels of partition_point's decision
-//      tree, then follows the decision path; the result is exactly the
-//      sequential search's.  Its children go to level L+1.  Smaller nodes are
-//      handed on (step 3) or, at <= 64 particles, finished as wave subtrees.
+//      tree and follows the decision path, exactly the sequential result.
 //   3. k_tree_blocks: one workgroup per subtree of at most kBlockSubMax
-//      particles runs that subtree's level loop itself, frontier in LDS and a
+//      particles runs that subtree's level loop, frontier in LDS and a
 //      workgroup barrier per level instead of a launch per level.
-//   A "wave subtree" (<= 64 particles) is replayed by one wave from ballot
-//   masks: one particle per lane, every partition point a bit search on a
-//   64-bit mask, the recursion stack in LDS.  Subtree node counts climb to the
-//   parents with the last-child-finishes pattern (integer atomics).
-//   4. k_tree_offsets: each top node walks to the root summing earlier
-//      siblings' subtree sizes: its pre-order index.
-//   5. k_tree_emit: leaves and wave subtrees write their nodes at their
-//      pre-order indices; masses climb to the internal nodes, again
-//      last-child-finishes, summed in the reference's child order.
+//   A node of at most 64 particles is a "wave subtree": one particle per
+//   lane, all of its levels built at once from ballot masks (each partition
+//   point is a bit search on a 64-bit mask).
+//   4. k_scan_* : the exclusive scan of C.
+//   5. k_tree_emit: every node's record at its pre-order index; wave
+//      subtrees and leaves also get their masses.
+//   6. k_tree_mass: one workgroup finishes the masses of the internal nodes
+//      above the wave subtrees, deepest first (children q = 0, 2, 1, 3).
 #include "ge_internal.hpp"
 
 namespace ge {
@@ -38,6 +42,7 @@ constexpr int kTreeWaves = kTreeBlock / 64;
 constexpr uint32_t kTreeGridMax = 8192;
 constexpr uint32_t kWaveSubMax = 64;
 constexpr uint32_t kFrontMax = 1024;  // LDS frontier of a workgroup subtree
+constexpr int kWaveLevels = 64;       // levels of a wave subtree
 
 __device__ __forceinline__ bool leaf_rule(uint32_t cnt, float size) {
     // quadtree.cpp:16-17
@@ -50,22 +55,6 @@ __device__ __forceinline__ float readlane_f(float v, uint32_t lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// std::partition_point with predicate v[i] < mid (lower_bound halving)
-__device__ __forceinline__ uint32_t part_point(const float *__restrict__ v, uint32_t first, uint32_t last,
-                                               float mid) {
-    uint32_t len = last - first;
-    while (len > 0) {
-        uint32_t half = len >> 1;
-        uint32_t middle = first + half;
-        if (v[middle] < mid) {
-            first = middle + 1;
-            len = len - half - 1;
-        } else {
-            len = half;
-        }
-    }
-    return first;
-}
 
 // The same halving search over predicate bits held in a 64-bit mask
 // (bit k = predicate of particle k of a wave subtree).
@@ -127,176 +116,236 @@ struct TreeOut {
 
 __device__ __forceinline__ void write_node(const TreeOut &o, uint32_t idx, float cx, float cy, float gm,
                                            float px, float py, float size, uint32_t next, uint32_t s,
-                                           uint32_t e, uint32_t mask) {
+                                           uint32_t e, uint32_t mask, bool geom = true) {
     if (idx >= o.cap) return;  // kErrNodeCap is raised by the caller
     NodeRec r;
     r.comx = cx; r.comy = cy; r.mass = gm; r.size_sq = size * size;
     r.next = next; r.start = s; r.end = e; r.child_mask = mask;
     o.nodes[idx] = r;
-    if (o.debug) o.geom[idx] = make_float4(px, py, size, 0.0f);
+    if (geom && o.debug) o.geom[idx] = make_float4(px, py, size, 0.0f);
 }
 
-// ---------------------------------------------------------------------------
-// Wave subtree: particles [s0, s0 + cnt) with cnt <= 64, one per lane.  The
-// recursion of Node::Node is replayed uniformly by the wave; its stack lives
-// in LDS.  EMIT=false counts nodes; EMIT=true writes them from index out0 and
-// returns the root's mass and centre of mass.
-struct WFrame {
-    float px, py, size;
-    uint32_t b[5];
-    uint32_t q, out, nacc;
-    float cm[4], cx[4], cy[4];
+struct WLevel {
+    uint64_t starts, covered, leafs, qb0, qb1;
 };
 
+__device__ __forceinline__ uint64_t rfl64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ WLevel load_level(const WLevel *lev, int L) {
+    WLevel r;
+    r.starts = rfl64(lev[L].starts);
+    r.covered = rfl64(lev[L].covered);
+    r.leafs = rfl64(lev[L].leafs);
+    r.qb0 = rfl64(lev[L].qb0);
+    r.qb1 = rfl64(lev[L].qb1);
+    return r;
+}
+
+// bits strictly above `lane`
+__device__ __forceinline__ uint64_t above(uint32_t lane) { return lane >= 63 ? 0ull : (~0ull << (lane + 1)); }
+
+// end of the node starting at `lane` on a level: the next node start or the
+// first particle no longer covered
+__device__ __forceinline__ uint32_t node_end(const WLevel &lv, uint32_t lane) {
+    const uint64_t stop = (lv.starts | ~lv.covered) & above(lane);
+    return stop ? (uint32_t)__builtin_ctzll(stop) : 64u;
+}
+
+// lane of the child with quadrant q inside [a, b) on the next level, or 64
+__device__ __forceinline__ uint32_t child_lane(const WLevel &nx, uint32_t a, uint32_t b, int q) {
+    const uint64_t range = (b >= 64 ? ~0ull : ((1ull << b) - 1ull)) & (~0ull << a);
+    uint64_t sel = nx.starts & range;
+    sel &= (q & 1) ? nx.qb0 : ~nx.qb0;
+    sel &= (q & 2) ? nx.qb1 : ~nx.qb1;
+    return sel ? (uint32_t)__builtin_ctzll(sel) : 64u;
+}
+
+__device__ __forceinline__ float shfl_f(float v, uint32_t src) { return __shfl(v, (int)src, 64); }
+
+
+struct StartTable {
+    uint32_t *count;  // C, then its exclusive prefix P (n + 1 entries)
+    uint32_t *level;  // F
+};
+
+__device__ __forceinline__ void register_start(const StartTable &st, uint32_t a, uint32_t level) {
+    atomicAdd(&st.count[a], 1u);
+    atomicMin(&st.level[a], level);
+}
+
+// Wave subtree: particles [s0, s0 + cnt), cnt <= 64, one per lane, built
+// level-parallel.  At each level every lane carries the range and corner of
+// the node holding its particle; the node's three partition points are bit
+// searches on two ballot masks (x < mid, y < mid).  EMIT=false registers the
+// subtree's nodes in the start table; EMIT=true (after the scan) writes them
+// and returns the root's mass and centre of mass.  Per level, four 64-bit
+// masks kept in LDS describe the nodes: starts, covered particles, leaf
+// starts and the two quadrant bits of each start.
 template <bool EMIT>
 __device__ uint32_t wave_subtree(const float *__restrict__ x, const float *__restrict__ y,
-                                 const float *__restrict__ m, float px, float py, float size, uint32_t s0,
-                                 uint32_t cnt, uint32_t out0, const TreeOut &o, WFrame *stk,
-                                 uint32_t *err, float &rgm, float &rcx, float &rcy) {
+                                 const float *__restrict__ m, float px0, float py0, float size0, uint32_t s0,
+                                 uint32_t cnt, uint32_t depth0, const StartTable &st, const TreeOut &o,
+                                 WLevel *lev, uint32_t *err, float &rgm, float &rcx, float &rcy) {
 #pragma clang fp contract(off)
     const uint32_t lane = lane_id();
     const bool have = lane < cnt;
     const float lx = have ? x[s0 + lane] : 0.0f;
     const float ly = have ? y[s0 + lane] : 0.0f;
-    const float lm = (EMIT && have) ? m[s0 + lane] : 0.0f;
-    const uint64_t valid = __ballot(have);
-    uint32_t counter = out0;
 
-    // leaf mass over relative range [a, b): computeLeafMass order
-    auto leaf_mass = [&](uint32_t a, uint32_t b, float &gm, float &cx, float &cy) {
-        gm = 0.0f; cx = 0.0f; cy = 0.0f;
-        for (uint32_t i = a; i < b; ++i) {
-            const float mi = readlane_f(lm, i);
-            gm = gm + mi;
-            cx = cx + readlane_f(lx, i) * mi;
-            cy = cy + readlane_f(ly, i) * mi;
+    // ---- pass A: top-down, every node of a level at once
+    bool act = have;
+    uint32_t a = 0, b = cnt;
+    float npx = px0, npy = py0, size = size0;
+    uint32_t C = 0, F = 0xFFu, leaf_end = 0;
+    uint64_t qb0 = 0, qb1 = 0;
+    uint32_t total = 0;
+    int nlev = 0;
+    for (; nlev < kWaveLevels; ++nlev) {
+        const uint64_t actm = __ballot(act);
+        if (actm == 0) break;
+        const bool is_start = act && lane == a;
+        const bool leaf = act && leaf_rule(b - a, size);
+        const uint64_t startm = __ballot(is_start);
+        const uint64_t leafm = __ballot(is_start && leaf);
+        total += (uint32_t)__popcll(startm);
+        if (is_start) {
+            C += 1;
+            if (F == 0xFFu) F = (uint32_t)nlev;
+            if (leaf) leaf_end = b;
         }
-        if (gm > 0) { cx = cx / gm; cy = cy / gm; }
-    };
-    auto split = [&](float fpx, float fpy, float fsize, uint32_t a, uint32_t b, uint32_t *bb) {
-        const float half = fsize * 0.5f;
-        const float midx = fpx + half, midy = fpy + half;
-        const uint64_t mx = __ballot(have && lx < midx) & valid;
-        const uint64_t my = __ballot(have && ly < midy) & valid;
-        bb[0] = a;
-        bb[4] = b;
-        bb[2] = mask_part_point(my, a, b);
-        bb[1] = mask_part_point(mx, a, bb[2]);
-        bb[3] = mask_part_point(mx, bb[2], b);
-    };
+        if (EMIT && lane == 0) {
+            WLevel &lv = lev[nlev];
+            lv.starts = startm; lv.covered = actm; lv.leafs = leafm; lv.qb0 = qb0; lv.qb1 = qb1;
+        }
+        const float half = size * 0.5f;
+        const bool split = act && !leaf;
+        const float midx = npx + half, midy = npy + half;
+        const uint64_t MX = __ballot(split && lx < midx);
+        const uint64_t MY = __ballot(split && ly < midy);
+        uint32_t q = 0;
+        if (split) {
+            const uint32_t b2 = mask_part_point(MY, a, b);
+            const uint32_t b1 = mask_part_point(MX, a, b2);
+            const uint32_t b3 = mask_part_point(MX, b2, b);
+            q = (lane >= b1 ? 1u : 0u) + (lane >= b2 ? 1u : 0u) + (lane >= b3 ? 1u : 0u);
+            const uint32_t na = q == 0 ? a : (q == 1 ? b1 : (q == 2 ? b2 : b3));
+            const uint32_t nb = q == 0 ? b1 : (q == 1 ? b2 : (q == 2 ? b3 : b));
+            a = na;
+            b = nb;
+            npx = npx + ((q & 1) ? half : 0.0f);
+            npy = npy + ((q & 2) ? half : 0.0f);
+        }
+        act = split;
+        qb0 = __ballot(split && lane == a && (q & 1u));
+        qb1 = __ballot(split && lane == a && (q & 2u));
+        size = half;
+    }
+    if (nlev == kWaveLevels && __ballot(act)) {
+        if (lane == 0) atomicOr(err, kErrDepth);
+    }
+    if (!EMIT) {
+        if (C > 0) {
+            atomicAdd(&st.count[s0 + lane], C);
+            atomicMin(&st.level[s0 + lane], depth0 + F);
+        }
+        return total;
+    }
 
-    if (leaf_rule(cnt, size)) {
-        if (EMIT) {
-            leaf_mass(0, cnt, rgm, rcx, rcy);
-            if (lane == 0) write_node(o, counter, rcx, rcy, rgm, px, py, size, 0, s0, s0 + cnt, 0);
-        }
-        return 1;
-    }
-    // Every lane writes the same (uniform) values into the LDS frames, so each
-    // lane only ever reads back its own writes: no cross-lane ordering needed.
-    int sp = 0;
-    {
-        WFrame &f = stk[0];
-        uint32_t bb[5];
-        split(px, py, size, 0, cnt, bb);
-        f.px = px; f.py = py; f.size = size; f.q = 0; f.out = counter; f.nacc = 0;
-        for (int k = 0; k < 5; ++k) f.b[k] = bb[k];
-        ++counter;
-    }
-    while (sp >= 0) {
-        WFrame &f = stk[sp];
-        uint32_t q = f.q;
-        while (q < 4 && !(f.b[q + 1] > f.b[q])) ++q;
-        if (q < 4) {
-            const float half = f.size * 0.5f;
-            const float cpx = f.px + ((q & 1) ? half : 0.0f);
-            const float cpy = f.py + ((q & 2) ? half : 0.0f);
-            const uint32_t a = f.b[q], b = f.b[q + 1];
-            const uint32_t cout = counter++;
-            f.q = q + 1;
-            if (leaf_rule(b - a, half)) {
-                f.nacc += 1;
-                if (EMIT) {
-                    float gm, cx, cy;
-                    leaf_mass(a, b, gm, cx, cy);
-                    if (lane == 0) write_node(o, cout, cx, cy, gm, cpx, cpy, half, 0, s0 + a, s0 + b, 0);
-                    f.cm[q] = gm; f.cx[q] = cx; f.cy[q] = cy;
-                }
-            } else {
-                if (sp + 1 >= kMaxDepth) {
-                    if (lane == 0) atomicOr(err, kErrDepth);
-                    return counter - out0;
-                }
-                uint32_t bb[5];
-                split(cpx, cpy, half, a, b, bb);
-                WFrame &c = stk[sp + 1];
-                c.px = cpx; c.py = cpy; c.size = half; c.q = 0; c.out = cout; c.nacc = 0;
-                for (int k = 0; k < 5; ++k) c.b[k] = bb[k];
-                ++sp;
+    // after the scan: P = exclusive prefix at this lane's particle, Fg = its
+    // shallowest start level (valid where this lane starts a node)
+    const uint32_t P = have ? st.count[s0 + lane] : 0u;
+    const uint32_t Fg = have ? st.level[s0 + lane] : 0u;
+
+    // ---- pass B (debug geometry): corners top-down from the parent's start lane
+    if (o.debug) {
+        float gpx = px0, gpy = py0, gsize = size0;
+        for (int L = 0; L < nlev; ++L) {
+            const WLevel lv = load_level(lev, L);
+            const bool is_start = (lv.starts >> lane) & 1ull;
+            float ppx = gpx, ppy = gpy;
+            if (L > 0) {
+                const WLevel pv = load_level(lev, L - 1);
+                const uint64_t below = pv.starts & (lane >= 63 ? ~0ull : ((2ull << lane) - 1ull));
+                const uint32_t plane = below ? 63u - (uint32_t)__builtin_clzll(below) : lane;
+                ppx = shfl_f(gpx, plane);  // every lane takes part in the shuffle
+                ppy = shfl_f(gpy, plane);
             }
-        } else {
-            // computeInternalMass (children q = 0, 2, 1, 3) and next
-            const uint32_t next = f.nacc;
-            if (EMIT) {
-                float gm = 0.0f, cx = 0.0f, cy = 0.0f;
-                uint32_t mask = 0;
-                const int order[4] = {0, 2, 1, 3};
-                for (int k = 0; k < 4; ++k) {
-                    const int qq = order[k];
-                    if (!(f.b[qq + 1] > f.b[qq])) continue;
+            if (is_start) {
+                if (L > 0) {
+                    const uint32_t qq = (uint32_t)((lv.qb0 >> lane) & 1ull) | ((uint32_t)((lv.qb1 >> lane) & 1ull) << 1);
+                    gpx = ppx + ((qq & 1) ? gsize : 0.0f);
+                    gpy = ppy + ((qq & 2) ? gsize : 0.0f);
+                }
+                const uint32_t idx = P + (depth0 + (uint32_t)L - Fg);
+                if (idx < o.cap) o.geom[idx] = make_float4(gpx, gpy, gsize, 0.0f);
+            }
+            gsize = gsize * 0.5f;  // size of the next level's nodes
+        }
+    }
+
+    // ---- leaf masses: computeLeafMass order, every leaf at once
+    const float lm = have ? m[s0 + lane] : 0.0f;
+    float lgm = 0.0f, lcx = 0.0f, lcy = 0.0f;
+    for (uint32_t j = 0; j < cnt; ++j) {
+        const float mj = readlane_f(lm, j);
+        const float xj = readlane_f(lx, j);
+        const float yj = readlane_f(ly, j);
+        if (j >= lane && j < leaf_end) {
+            lgm = lgm + mj;
+            lcx = lcx + xj * mj;
+            lcy = lcy + yj * mj;
+        }
+    }
+    if (lgm > 0) { lcx = lcx / lgm; lcy = lcy / lgm; }
+
+    // ---- pass C: bottom-up masses (computeInternalMass, q = 0, 2, 1, 3), records
+    float M = 0.0f, X = 0.0f, Y = 0.0f;
+    float lsize = size0;
+    for (int L = 0; L < nlev - 1; ++L) lsize = lsize * 0.5f;
+    for (int L = nlev - 1; L >= 0; --L) {
+        const WLevel lv = load_level(lev, L);
+        const bool is_start = (lv.starts >> lane) & 1ull;
+        const bool is_leaf = (lv.leafs >> lane) & 1ull;
+        const uint32_t e = node_end(lv, lane);
+        float gm = 0.0f, cx = 0.0f, cy = 0.0f;
+        uint32_t mask = 0;
+        if (L + 1 < nlev) {
+            const WLevel nx = load_level(lev, L + 1);
+            const int order[4] = {0, 2, 1, 3};
+            for (int k = 0; k < 4; ++k) {
+                const int qq = order[k];
+                const uint32_t cl = child_lane(nx, lane, e, qq);
+                // every lane takes part in the shuffles
+                const float cm = shfl_f(M, cl & 63u);
+                const float ccx = shfl_f(X, cl & 63u);
+                const float ccy = shfl_f(Y, cl & 63u);
+                if (is_start && !is_leaf && cl < 64u) {
                     mask |= 1u << qq;
-                    const float cm = f.cm[qq];
                     gm = gm + cm;
-                    cx = cx + f.cx[qq] * cm;
-                    cy = cy + f.cy[qq] * cm;
-                }
-                if (gm > 0) { cx = cx / gm; cy = cy / gm; }
-                if (lane == 0) write_node(o, f.out, cx, cy, gm, f.px, f.py, f.size, next, s0 + f.b[0], s0 + f.b[4], mask);
-                if (sp == 0) {
-                    rgm = gm; rcx = cx; rcy = cy;
-                } else {
-                    WFrame &p = stk[sp - 1];
-                    const uint32_t pq = p.q - 1;  // the quadrant this frame fills
-                    p.cm[pq] = gm; p.cx[pq] = cx; p.cy[pq] = cy;
+                    cx = cx + ccx * cm;
+                    cy = cy + ccy * cm;
                 }
             }
-            if (sp > 0) stk[sp - 1].nacc += 1 + next;
-            --sp;
+            if (gm > 0) { cx = cx / gm; cy = cy / gm; }
         }
+        if (is_leaf) { gm = lgm; cx = lcx; cy = lcy; }
+        if (is_start) {
+            const uint32_t depth = depth0 + (uint32_t)L - Fg;
+            const uint32_t idx = P + depth;
+            const uint32_t next = (st.count[s0 + e] - P) - depth - 1u;
+            write_node(o, idx, cx, cy, gm, 0.0f, 0.0f, lsize, next, s0 + lane, s0 + e, mask, false);
+            M = gm; X = cx; Y = cy;
+        }
+        lsize = lsize * 2.0f;
     }
-    return counter - out0;
-}
-
-// ---------------------------------------------------------------------------
-__global__ void k_tree_init(TopNodes T, const float *__restrict__ bb, uint32_t n,
-                            uint32_t *__restrict__ counters) {
-    const uint32_t t = threadIdx.x;
-    if (t < kCntSlots - kCntLevel0) counters[kCntLevel0 + t] = 0;
-    if (t == 0) {
-        counters[kCntError] &= ~(kErrNodeCap | kErrDepth | kErrFrontier);
-        counters[kCntBlockRoots] = 0;
-        counters[kCntTopAlloc] = 0;
-        T.px[0] = bb[0]; T.py[0] = bb[1]; T.size[0] = bb[2];
-        T.start[0] = 0; T.end[0] = n; T.parent[0] = kNone; T.kind[0] = kKindPending;
-    }
-    __syncthreads();
-    if (t == 0) {
-        counters[kCntLevel0 + 0] = 1;
-        counters[kCntLevelBase0 + 0] = 0;
-    }
-}
-
-// Report a finished subtree's node count to its ancestors.
-__device__ void propagate_count(TopNodes &T, uint32_t node, uint32_t count) {
-    uint32_t p = T.parent[node];
-    for (int hops = 0; p != kNone && p < T.cap && hops < 4096; ++hops) {
-        __hip_atomic_fetch_add(&T.accum[p], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t old = __hip_atomic_fetch_sub(&T.pending[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old != 1u) return;
-        count = 1u + __hip_atomic_load(&T.accum[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&T.count[p], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        p = T.parent[p];
-    }
+    rgm = readlane_f(M, 0);
+    rcx = readlane_f(X, 0);
+    rcy = readlane_f(Y, 0);
+    return total;
 }
 
 // Split a node with the whole wave: subGridMaker's three searches
@@ -313,13 +362,16 @@ __device__ __forceinline__ void wave_split(const float *__restrict__ x, const fl
     b[3] = wave_part_point(x, b[2], e, midx);
 }
 
-// Lane 0: record children of an internal node at indices first.. (q order).
-__device__ void write_children(TopNodes &T, uint32_t node, float px, float py, float size, const uint32_t b[5],
-                               uint32_t first, uint32_t k, uint32_t *out_list) {
+__device__ __forceinline__ uint32_t child_count(const uint32_t b[5]) {
+    uint32_t k = 0;
+    for (int q = 0; q < 4; ++q) k += (b[q + 1] > b[q]) ? 1u : 0u;
+    return k;
+}
+
+// Lane 0: an internal node and its children at indices first.. (q order).
+__device__ void write_children(TopNodes &T, uint32_t node, float px, float py, float size, uint32_t depth,
+                               const uint32_t b[5], uint32_t first, uint32_t *out_list) {
     T.kind[node] = kKindInternal;
-    T.pending[node] = k;
-    T.pending2[node] = k;
-    T.accum[node] = 0;
     const float half = size * 0.5f;
     uint32_t c = first, j = 0;
     for (int q = 0; q < 4; ++q) {
@@ -330,6 +382,7 @@ __device__ void write_children(TopNodes &T, uint32_t node, float px, float py, f
             T.start[c] = b[q];
             T.end[c] = b[q + 1];
             T.parent[c] = node;
+            T.depth[c] = depth + 1;
             T.kind[c] = kKindPending;
             T.child[4 * node + q] = c;
             if (out_list) out_list[j++] = c;
@@ -340,36 +393,24 @@ __device__ void write_children(TopNodes &T, uint32_t node, float px, float py, f
     }
 }
 
-__device__ __forceinline__ uint32_t child_count(const uint32_t b[5]) {
-    uint32_t k = 0;
-    for (int q = 0; q < 4; ++q) k += (b[q + 1] > b[q]) ? 1u : 0u;
-    return k;
-}
-
-// Leaf or wave subtree: finish the node now.  Returns false if the node needs
-// splitting.  Uniform across the wave.
-__device__ bool finish_small(TopNodes &T, const float *__restrict__ x, const float *__restrict__ y,
-                             uint32_t node, float px, float py, float size, uint32_t s, uint32_t e, WFrame *stk,
-                             uint32_t *err) {
+// Leaf or wave subtree: finish the node now (registering its nodes).
+// Returns false if the node needs splitting.  Uniform across the wave.
+__device__ bool finish_small(TopNodes &T, const StartTable &st, const float *__restrict__ x,
+                             const float *__restrict__ y, uint32_t node, float px, float py, float size,
+                             uint32_t s, uint32_t e, uint32_t depth, uint32_t *err) {
     const uint32_t c = e - s;
-    const uint32_t lane = lane_id();
     if (leaf_rule(c, size)) {
-        if (lane == 0) {
+        if (lane_id() == 0) {
             T.kind[node] = kKindLeaf;
-            T.count[node] = 1;
-            propagate_count(T, node, 1);
+            register_start(st, s, depth);
         }
         return true;
     }
     if (c <= kWaveSubMax) {
         const TreeOut none{nullptr, nullptr, 0, 0};
         float d0, d1, d2;
-        const uint32_t k = wave_subtree<false>(x, y, nullptr, px, py, size, s, c, 0, none, stk, err, d0, d1, d2);
-        if (lane == 0) {
-            T.kind[node] = kKindWaveSub;
-            T.count[node] = k;
-            propagate_count(T, node, k);
-        }
+        wave_subtree<false>(x, y, nullptr, px, py, size, s, c, depth, st, none, nullptr, err, d0, d1, d2);
+        if (lane_id() == 0) T.kind[node] = kKindWaveSub;
         return true;
     }
     return false;
@@ -381,12 +422,35 @@ __device__ __forceinline__ void push_block_root(TopNodes &T, uint32_t *counters,
     else atomicOr(&counters[kCntError], kErrFrontier);
 }
 
-__global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, const float *__restrict__ x,
+__global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTable st, const float *__restrict__ bb,
+                                                           uint32_t n, uint32_t *__restrict__ counters) {
+    const uint32_t i = blockIdx.x * kTreeBlock + threadIdx.x;
+    if (i <= n) {
+        st.count[i] = 0;
+        st.level[i] = 0xFFFFFFFFu;
+    }
+    if (blockIdx.x == 0) {
+        const uint32_t t = threadIdx.x;
+        if (t < kCntSlots - kCntLevel0) counters[kCntLevel0 + t] = 0;
+        if (t == 0) {
+            counters[kCntError] &= ~(kErrNodeCap | kErrDepth | kErrFrontier);
+            counters[kCntBlockRoots] = 0;
+            counters[kCntTopAlloc] = 0;
+            T.px[0] = bb[0]; T.py[0] = bb[1]; T.size[0] = bb[2];
+            T.start[0] = 0; T.end[0] = n; T.parent[0] = kNone; T.kind[0] = kKindPending; T.depth[0] = 0;
+        }
+        __syncthreads();
+        if (t == 0) {
+            counters[kCntLevel0 + 0] = 1;
+            counters[kCntLevelBase0 + 0] = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTable st, const float *__restrict__ x,
                                                            const float *__restrict__ y,
                                                            uint32_t *__restrict__ counters, int L,
                                                            int last_level) {
-    __shared__ WFrame stk_all[kTreeWaves][kMaxDepth];
-    WFrame *stk = stk_all[threadIdx.x >> 6];
     const uint32_t lane = lane_id();
     const uint32_t base = counters[kCntLevelBase0 + L];
     uint32_t cnt = counters[kCntLevel0 + L];
@@ -405,7 +469,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, const flo
         if (T.kind[node] == kKindDead) continue;
         const float px = T.px[node], py = T.py[node], size = T.size[node];
         const uint32_t s = T.start[node], e = T.end[node];
-        if (finish_small(T, x, y, node, px, py, size, s, e, stk, counters + kCntError)) continue;
+        if (finish_small(T, st, x, y, node, px, py, size, s, e, (uint32_t)L, counters + kCntError)) continue;
         if (e - s <= kBlockSubMax || L == last_level) {
             if (lane == 0) push_block_root(T, counters, node);
             continue;
@@ -421,7 +485,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, const flo
                 for (uint32_t cc = first; cc < T.cap && cc < first + k; ++cc) T.kind[cc] = kKindDead;
                 push_block_root(T, counters, node);
             } else {
-                write_children(T, node, px, py, size, b, first, k, nullptr);
+                register_start(st, s, (uint32_t)L);
+                write_children(T, node, px, py, size, (uint32_t)L, b, first, nullptr);
             }
         }
     }
@@ -429,13 +494,11 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, const flo
 
 // One workgroup per subtree root handed over by the level phase: the level
 // loop of that subtree with its frontier in LDS.
-__global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, const float *__restrict__ x,
+__global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTable st, const float *__restrict__ x,
                                                             const float *__restrict__ y,
                                                             uint32_t *__restrict__ counters) {
-    __shared__ WFrame stk_all[kTreeWaves][kMaxDepth];
     __shared__ uint32_t front[2][kFrontMax];
     __shared__ uint32_t fcount[2];
-    WFrame *stk = stk_all[threadIdx.x >> 6];
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t nroots = min(counters[kCntBlockRoots], T.cap);
@@ -454,8 +517,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, const fl
             for (uint32_t i = w; i < nf; i += kTreeWaves) {
                 const uint32_t node = front[cur][i];
                 const float px = T.px[node], py = T.py[node], size = T.size[node];
-                const uint32_t s = T.start[node], e = T.end[node];
-                if (finish_small(T, x, y, node, px, py, size, s, e, stk, counters + kCntError)) continue;
+                const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
+                if (finish_small(T, st, x, y, node, px, py, size, s, e, depth, counters + kCntError)) continue;
                 uint32_t b[5];
                 wave_split(x, y, px, py, size, s, e, b);
                 if (lane == 0) {
@@ -465,7 +528,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, const fl
                     if (first + k > T.cap || slot + k > kFrontMax) {
                         atomicOr(&counters[kCntError], kErrFrontier);
                     } else {
-                        write_children(T, node, px, py, size, b, first, k, &front[cur ^ 1][slot]);
+                        register_start(st, s, depth);
+                        write_children(T, node, px, py, size, depth, b, first, &front[cur ^ 1][slot]);
                     }
                 }
             }
@@ -479,110 +543,198 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, const fl
     }
 }
 
-__global__ __launch_bounds__(kTreeBlock) void k_tree_offsets(TopNodes T, uint32_t *__restrict__ counters,
-                                                             uint32_t node_cap) {
-    const uint32_t total = min(counters[kCntTopAlloc], T.cap);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const uint32_t nn = T.count[0];
-        counters[kCntTopTotal] = total;
-        counters[kCntNodeTotal] = nn <= node_cap ? nn : node_cap;
-        if (nn > node_cap) atomicOr(&counters[kCntError], kErrNodeCap);
+// ---- exclusive scan of the start counts over n + 1 entries
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t &block_total) {
+    // wave inclusive scan, then across the 16 waves
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)incl, d, 64);
+        if (lane >= (uint32_t)d) incl += u;
     }
-    for (uint32_t t = blockIdx.x * kTreeBlock + threadIdx.x; t < total; t += gridDim.x * kTreeBlock) {
-        if (T.kind[t] == kKindDead) continue;
-        uint32_t o = 0, nd = t;
-        int hops = 0;
-        while (T.parent[nd] != kNone) {
-            const uint32_t p = T.parent[nd];
-            if (p >= T.cap || ++hops > 4096) {  // a corrupt chain: flag it, never loop
-                atomicOr(&counters[kCntError], kErrFrontier);
-                break;
-            }
-            o += 1;
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t ch = T.child[4 * p + q];
-                if (ch == nd) break;
-                if (ch != kNone && ch < T.cap) o += T.count[ch];
-            }
-            nd = p;
+    if (lane == 63) sh[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < kScanThreads / 64; ++k) {
+            const uint32_t t = sh[k];
+            sh[k] = run;
+            run += t;
         }
-        T.offset[t] = o;
+        sh[kScanThreads / 64] = run;
+    }
+    __syncthreads();
+    block_total = sh[kScanThreads / 64];
+    const uint32_t r = sh[w] + incl - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(const uint32_t *__restrict__ c, uint32_t len,
+                                                            uint32_t *__restrict__ part) {
+    __shared__ uint32_t sh[kScanThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v = 0;
+    for (int k = 0; k < kScanItems; ++k) v += (base + k < len) ? c[base + k] : 0u;
+    uint32_t tot;
+    block_excl_scan(v, sh, tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_parts(uint32_t *__restrict__ part, uint32_t nparts,
+                                                             uint32_t *__restrict__ counters, uint32_t node_cap,
+                                                             uint32_t top_cap) {
+    __shared__ uint32_t sh[kScanThreads / 64 + 1];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nparts; b0 += kScanThreads) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < nparts ? part[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, sh, tot);
+        if (i < nparts) part[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        // carry = total number of nodes
+        counters[kCntNodeTotal] = carry <= node_cap ? carry : node_cap;
+        if (carry > node_cap) counters[kCntError] |= kErrNodeCap;
+        counters[kCntTopTotal] = min(counters[kCntTopAlloc], top_cap);
     }
 }
 
-// Report a finished node's mass to its ancestors; the last child to report
-// computes the parent as computeInternalMass does (children q = 0, 2, 1, 3).
-__device__ void propagate_mass(TopNodes &T, const TreeOut &o, uint32_t node) {
-#pragma clang fp contract(off)
-    uint32_t p = T.parent[node];
-    for (int hops = 0; p != kNone && p < T.cap && hops < 4096; ++hops) {
-        __threadfence();
-        uint32_t old = __hip_atomic_fetch_sub(&T.pending2[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old != 1u) return;
-        __threadfence();
-        float gm = 0.0f, cx = 0.0f, cy = 0.0f;
-        uint32_t mask = 0;
-        const int order[4] = {0, 2, 1, 3};
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t ch = T.child[4 * p + order[k]];
-            if (ch == kNone || ch >= T.cap) continue;
-            mask |= 1u << order[k];
-            const uint32_t oc = T.offset[ch];
-            if (oc >= o.cap) continue;
-            const float *rec = reinterpret_cast<const float *>(&o.nodes[oc]);
-            const float cm = atomic_load_f32(rec + 2);
-            const float ccx = atomic_load_f32(rec + 0);
-            const float ccy = atomic_load_f32(rec + 1);
-            gm = gm + cm;
-            cx = cx + ccx * cm;
-            cy = cy + ccy * cm;
-        }
-        if (gm > 0) { cx = cx / gm; cy = cy / gm; }
-        write_node(o, T.offset[p], cx, cy, gm, T.px[p], T.py[p], T.size[p], T.count[p] - 1u, T.start[p],
-                   T.end[p], mask);
-        p = T.parent[p];
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t *__restrict__ c, uint32_t len,
+                                                             const uint32_t *__restrict__ part) {
+    __shared__ uint32_t sh[kScanThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < len) ? c[base + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan(sum, sh, tot) + part[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < len) c[base + k] = run;
+        run += v[k];
     }
 }
 
-__global__ __launch_bounds__(kTreeBlock) void k_tree_emit(TopNodes T, const float *__restrict__ x,
+// Every top node's record at its pre-order index; leaves and wave subtrees
+// complete (with masses), internal nodes without masses yet.
+__global__ __launch_bounds__(kTreeBlock) void k_tree_emit(TopNodes T, StartTable st, const float *__restrict__ x,
                                                           const float *__restrict__ y,
                                                           const float *__restrict__ m,
                                                           uint32_t *__restrict__ counters, TreeOut o,
                                                           uint32_t n) {
 #pragma clang fp contract(off)
-    __shared__ WFrame stk_all[kTreeWaves][kMaxDepth];
-    WFrame *stk = stk_all[threadIdx.x >> 6];
+    __shared__ WLevel lev_all[kTreeWaves][kWaveLevels];
+    WLevel *lev = lev_all[threadIdx.x >> 6];
     const uint32_t lane = lane_id();
     const uint32_t total = counters[kCntTopTotal];
     const uint32_t wave = (blockIdx.x * kTreeBlock + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * kTreeBlock) >> 6;
     for (uint32_t t = wave; t < total; t += nwaves) {
         const uint32_t kind = T.kind[t];
-        if (kind != kKindLeaf && kind != kKindWaveSub) continue;
-        const uint32_t out = T.offset[t];
-        const uint32_t s = T.start[t], e = T.end[t];
+        if (kind != kKindLeaf && kind != kKindWaveSub && kind != kKindInternal) continue;
+        const uint32_t s = T.start[t], e = T.end[t], depth = T.depth[t];
         if (e > n || s >= e || (kind == kKindWaveSub && e - s > kWaveSubMax)) {
             if (lane == 0) atomicOr(&counters[kCntError], kErrFrontier);
             continue;
         }
-        if (lane == 0 && out + T.count[t] > o.cap) atomicOr(&counters[kCntError], kErrNodeCap);
-        float gm, cx, cy;
+        const uint32_t Pa = st.count[s], Fa = st.level[s], Pb = st.count[e];
+        const uint32_t d = depth - Fa;
+        const uint32_t idx = Pa + d;
+        const uint32_t next = (Pb - Pa) - d - 1u;
+        if (lane == 0) T.offset[t] = idx;
         if (kind == kKindWaveSub) {
-            wave_subtree<true>(x, y, m, T.px[t], T.py[t], T.size[t], s, e - s, out, o, stk,
+            float gm, cx, cy;
+            wave_subtree<true>(x, y, m, T.px[t], T.py[t], T.size[t], s, e - s, depth, st, o, lev,
                                counters + kCntError, gm, cx, cy);
         } else if (lane == 0) {
-            // computeLeafMass (quadtree.h:53-67)
-            gm = 0.0f; cx = 0.0f; cy = 0.0f;
-            for (uint32_t i = s; i < e; ++i) {
-                const float mi = m[i];
-                gm = gm + mi;
-                cx = cx + x[i] * mi;
-                cy = cy + y[i] * mi;
+            float gm = 0.0f, cx = 0.0f, cy = 0.0f;
+            uint32_t mask = 0;
+            if (kind == kKindLeaf) {
+                // computeLeafMass (quadtree.h:53-67)
+                for (uint32_t i = s; i < e; ++i) {
+                    const float mi = m[i];
+                    gm = gm + mi;
+                    cx = cx + x[i] * mi;
+                    cy = cy + y[i] * mi;
+                }
+                if (gm > 0) { cx = cx / gm; cy = cy / gm; }
+            } else {
+                for (int q = 0; q < 4; ++q) mask |= (T.child[4 * t + q] != kNone) ? (1u << q) : 0u;
+            }
+            write_node(o, idx, cx, cy, gm, T.px[t], T.py[t], T.size[t], next, s, e, mask);
+        }
+    }
+}
+
+// Masses of the internal top nodes, deepest first, by one workgroup: nodes
+// of one depth are independent; barriers order the depths.
+constexpr int kMassThreads = 1024;
+constexpr int kMaxTopDepth = 256;
+
+__global__ __launch_bounds__(kMassThreads) void k_tree_mass(TopNodes T, uint32_t *__restrict__ counters, TreeOut o) {
+#pragma clang fp contract(off)
+    __shared__ uint32_t hist[kMaxTopDepth + 1];
+    __shared__ uint32_t cursor[kMaxTopDepth];
+    const uint32_t total = counters[kCntTopTotal];
+    for (uint32_t d = threadIdx.x; d <= kMaxTopDepth; d += kMassThreads) hist[d] = 0;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < total; t += kMassThreads) {
+        if (T.kind[t] != kKindInternal) continue;
+        const uint32_t d = T.depth[t];
+        if (d < kMaxTopDepth) atomicAdd(&hist[d], 1u);
+        else atomicOr(&counters[kCntError], kErrFrontier);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int d = 0; d < kMaxTopDepth; ++d) {
+            const uint32_t v = hist[d];
+            hist[d] = run;
+            cursor[d] = run;
+            run += v;
+        }
+        hist[kMaxTopDepth] = run;
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < total; t += kMassThreads) {
+        if (T.kind[t] != kKindInternal) continue;
+        const uint32_t d = T.depth[t];
+        if (d < kMaxTopDepth) T.broot[atomicAdd(&cursor[d], 1u)] = t;
+    }
+    __syncthreads();
+    for (int d = kMaxTopDepth - 1; d >= 0; --d) {
+        const uint32_t lo = hist[d], hi = hist[d + 1];
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += kMassThreads) {
+            const uint32_t t = T.broot[i];
+            float gm = 0.0f, cx = 0.0f, cy = 0.0f;
+            const int order[4] = {0, 2, 1, 3};
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t ch = T.child[4 * t + order[k]];
+                if (ch == kNone || ch >= T.cap) continue;
+                const uint32_t oc = T.offset[ch];
+                if (oc >= o.cap) continue;
+                const NodeRec r = o.nodes[oc];
+                gm = gm + r.mass;
+                cx = cx + r.comx * r.mass;
+                cy = cy + r.comy * r.mass;
             }
             if (gm > 0) { cx = cx / gm; cy = cy / gm; }
-            write_node(o, out, cx, cy, gm, T.px[t], T.py[t], T.size[t], 0, s, e, 0);
+            const uint32_t ot = T.offset[t];
+            if (ot < o.cap) {
+                const float sz = T.size[t];
+                reinterpret_cast<float4 *>(o.nodes)[2 * ot] = make_float4(cx, cy, gm, sz * sz);
+            }
         }
-        if (lane == 0) propagate_mass(T, o, t);
+        __syncthreads();
     }
 }
 
@@ -599,7 +751,9 @@ void launch_tree_build(ge_ctx *c, const float *bb_dev) {
     const State &s = c->b.st[c->b.cur];
     TopNodes &T = c->b.top;
     const uint64_t n = c->n;
-    hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(kTreeBlock), 0, c->stream, T, bb_dev, c->n, c->b.counters);
+    const StartTable st{c->b.start_count, c->b.start_level};
+    hipLaunchKernelGGL(k_tree_clear, dim3(blocks_for(n + 1, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, T, st, bb_dev,
+                       c->n, c->b.counters);
     // level L+1 only holds children of nodes above kBlockSubMax particles
     const uint64_t big_children = 4 * (n / (kBlockSubMax + 1)) + 4;
     for (int L = 0; L < kTopLevels; ++L) {
@@ -607,20 +761,26 @@ void launch_tree_build(ge_ctx *c, const float *bb_dev) {
         for (int k = 0; k < L && maxn < n; ++k) maxn *= 4;
         if (L > 0 && maxn > big_children) maxn = big_children;
         if (maxn > n) maxn = n;
-        hipLaunchKernelGGL(k_tree_level, dim3(wave_grid(maxn)), dim3(kTreeBlock), 0, c->stream, T, s.x, s.y,
+        hipLaunchKernelGGL(k_tree_level, dim3(wave_grid(maxn)), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y,
                            c->b.counters, L, kTopLevels - 1);
     }
     // roots of workgroup subtrees each hold more than 64 particles
     uint64_t roots = n / (kWaveSubMax + 1) + 1;
     if (roots > 4096) roots = 4096;
-    hipLaunchKernelGGL(k_tree_blocks, dim3((uint32_t)roots), dim3(kTreeBlock), 0, c->stream, T, s.x, s.y,
+    hipLaunchKernelGGL(k_tree_blocks, dim3((uint32_t)roots), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y,
                        c->b.counters);
-    uint32_t tg = (uint32_t)((T.cap + kTreeBlock - 1) / kTreeBlock);
-    if (tg > 1024) tg = 1024;
-    hipLaunchKernelGGL(k_tree_offsets, dim3(tg), dim3(kTreeBlock), 0, c->stream, T, c->b.counters, c->b.node_cap);
+    const uint32_t len = c->n + 1;
+    const uint32_t parts = (uint32_t)((len + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_sums, dim3(parts), dim3(kScanThreads), 0, c->stream, st.count, len, c->b.scan_part);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanThreads), 0, c->stream, c->b.scan_part, parts, c->b.counters,
+                       c->b.node_cap, T.cap);
+    hipLaunchKernelGGL(k_scan_apply, dim3(parts), dim3(kScanThreads), 0, c->stream, st.count, len, c->b.scan_part);
     TreeOut o{c->b.nodes, c->b.node_geom, c->b.node_cap, c->debug ? 1 : 0};
-    hipLaunchKernelGGL(k_tree_emit, dim3(wave_grid(T.cap)), dim3(kTreeBlock), 0, c->stream, T, s.x, s.y, s.m,
+    uint64_t top_bound = 8 * (n / (kWaveSubMax + 1)) + 64;
+    if (top_bound > T.cap) top_bound = T.cap;
+    hipLaunchKernelGGL(k_tree_emit, dim3(wave_grid(top_bound)), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y, s.m,
                        c->b.counters, o, c->n);
+    hipLaunchKernelGGL(k_tree_mass, dim3(1), dim3(kMassThreads), 0, c->stream, T, c->b.counters, o);
 }
 
 }  // namespace ge

@@ -84,8 +84,18 @@ def test_tree_bitexact_sizes(n):
         c.build_quadtree(bb)
         got = c.nodes()
     assert len(got) == len(ref)
-    for f in ("start", "end", "next", "child", "posx", "posy", "size", "mass", "comx", "comy"):
-        np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+    _assert_nodes_equal(got, ref)
+
+
+def _assert_nodes_equal(got, ref, fields=("start", "end", "next", "child", "posx", "posy", "size",
+                                          "mass", "comx", "comy")):
+    for f in fields:
+        a, b = got[f], ref[f]
+        eq = np.all(a == b, axis=tuple(range(1, a.ndim))) if a.ndim > 1 else (a == b)
+        if not np.all(eq):
+            i = int(np.nonzero(~eq)[0][0])
+            raise AssertionError(f"field {f}: first mismatch at node {i} of {len(ref)}:\n"
+                                 f" got {got[i]}\n want {ref[i]}")
 
 
 def test_tree_rebuilt_every_frame_matches_oracle():
@@ -102,8 +112,7 @@ def test_tree_rebuilt_every_frame_matches_oracle():
             ref = O.build_tree(d["x"], d["y"], d["mass"], bb)
             got = c.nodes()
             assert len(got) == len(ref)
-            for f in ("start", "end", "next", "child", "mass", "comx", "comy"):
-                np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+            _assert_nodes_equal(got, ref, ("start", "end", "next", "child", "mass", "comx", "comy"))
 
 
 def _run_gravity(st, prm_kw, theta):
