@@ -61,9 +61,10 @@ void free_all(ge_ctx *c) {
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
     dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
-    dfree(b.nodes); dfree(b.node_geom); dfree(b.sphf);
+    dfree(b.nodes); dfree(b.node_geom); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
-        dfree(g->keys); dfree(g->keys_alt); dfree(g->cell_start); dfree(g->cell_end); dfree(g->frame);
+        dfree(g->keys); dfree(g->keys_alt); dfree(g->first); dfree(g->ord); dfree(g->frame);
+        dfree(g->pk0); dfree(g->pk1); dfree(g->pkf);
     }
 }
 
@@ -83,7 +84,7 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.partials2, 4 * 4096));
     GE_HIP_CHECK(c, dalloc(b.counters, kCntSlots));
     GE_HIP_CHECK(c, hipMemset(b.counters, 0, sizeof(uint32_t) * kCntSlots));
-    GE_HIP_CHECK(c, dalloc(b.bb, 8));
+    GE_HIP_CHECK(c, dalloc(b.bb, 16));
     GE_HIP_CHECK(c, dalloc(b.interactions, 1));
     TopNodes &T = b.top;
     T.cap = 2u * cap + 1024u;
@@ -108,7 +109,26 @@ int alloc_all(ge_ctx *c) {
     b.node_cap = 4u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(b.nodes, b.node_cap));
     GE_HIP_CHECK(c, dalloc(b.node_geom, b.node_cap));
-    if (c->sph) GE_HIP_CHECK(c, dalloc(b.sphf, 2ull * cap));
+    if (c->sph) {
+        GE_HIP_CHECK(c, dalloc(b.sphf, 2ull * cap));
+        GE_HIP_CHECK(c, dalloc(b.sph_pp, 4ull * cap));
+        GE_HIP_CHECK(c, dalloc(b.sph_pd, 2ull * cap));
+        for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
+            // dense cell table: at least 4 cells per particle of capacity, at most 2^24
+            uint64_t cells = 4ull * cap + 4096;
+            if (cells < (1ull << 20)) cells = 1ull << 20;
+            if (cells > (1ull << kCellBits) - 16) cells = (1ull << kCellBits) - 16;  // below the sentinel key
+            g->cells_cap = (uint32_t)cells;
+            GE_HIP_CHECK(c, dalloc(g->keys, cap));
+            GE_HIP_CHECK(c, dalloc(g->keys_alt, cap));
+            GE_HIP_CHECK(c, dalloc(g->first, (size_t)g->cells_cap + 1));
+            GE_HIP_CHECK(c, dalloc(g->ord, cap));
+            GE_HIP_CHECK(c, dalloc(g->frame, 8));
+            GE_HIP_CHECK(c, dalloc(g->pk0, cap));
+            GE_HIP_CHECK(c, dalloc(g->pk1, cap));
+            GE_HIP_CHECK(c, dalloc(g->pkf, cap));
+        }
+    }
     return GE_OK;
 }
 

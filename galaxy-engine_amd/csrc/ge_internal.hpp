@@ -81,15 +81,21 @@ struct TopNodes {
     uint32_t cap = 0;
 };
 
-// SPH uniform grid (dense cell table over the occupied cell range).
+// SPH uniform grid: particles sorted by cell over the occupied cell range;
+// first[c] = number of particles in cells before c (W*H + 1 entries), so the
+// particles of cells c0..c1 of one row are positions [first[c0], first[c1+1]).
 struct Grid {
     uint64_t *keys = nullptr;      // packed (cell << 28 | slot), sorted
     uint64_t *keys_alt = nullptr;
-    uint32_t *cell_start = nullptr;  // per cell: first sorted position
-    uint32_t *cell_end = nullptr;    // per cell: one past the last
-    int32_t *frame = nullptr;        // device: {cx0, cy0, W, H} of the table
-    uint64_t cells_cap = 0;
+    uint32_t *first = nullptr;     // prefix cell table
+    uint32_t *ord = nullptr;       // slot of the k-th particle in cell order
+    int32_t *frame = nullptr;      // device: {cx0, cy0, W, H, valid} of the table
+    float4 *pk0 = nullptr;         // per cell-order position: packed fields
+    float4 *pk1 = nullptr;
+    uint32_t *pkf = nullptr;       // flags in cell order
+    uint32_t cells_cap = 0;
 };
+constexpr int kCellBits = 24;      // cell keys below 2^24
 
 struct Buffers {
     State st[2];
@@ -113,6 +119,8 @@ struct Buffers {
     float4 *node_geom = nullptr;   // debug: (posx, posy, size, 0) per final node
     uint32_t node_cap = 0;
     float *sphf = nullptr;         // SPH force scratch (x, y interleaved), 2 per particle
+    float *sph_pp = nullptr;       // grid-A order (predX, predY, restDens, sphMass)
+    float *sph_pd = nullptr;       // grid-A order (press, predDens)
     Grid grid_pos, grid_pred;
 };
 
@@ -127,6 +135,7 @@ struct ge_ctx {
     bool debug = true;             // keep node geometry for ge_get_nodes
     bool have_tree = false;
     bool have_grid = false;
+    float sph_cell = 3.0f;         // cell size grid A was built with
     ge::Buffers b;
     std::string err;
     // phase timing (ge_profile_enable)

@@ -1,9 +1,552 @@
-// ge_sph.hip -- SPH uniform grid and PCISPH passes (placeholder until the
-// SPH kernels land).
+// ge_sph.hip -- SPH neighbour grid and the PCISPH passes of SPH::pcisphSolver
+// (GE/include/Physics/SPH.h:623-632, GE/src/Physics/SPH.cpp:98-281) with the
+// candidate semantics of NeighborSearchV2AVX2 (GE/include/Particles/
+// neighborSearch.h:636-777): a particle's candidates are all particles whose
+// grid cell of pos is within one cell (3 x 3) of the query point's cell.
+//
+// Uniform grid instead of the reference's 16384-slot hash: particles are
+// radix-sorted by cell over the occupied cell range and first[c] (prefix
+// cell table) turns the three cells of one row into one contiguous range.
+// Particle fields are packed in cell order, so a neighbour range is read
+// contiguously.  The candidate SET is the hash's: the hash only groups cells,
+// its queries compare exact cell coordinates.
+//
+// The reference scatters every pair force to both particles with float
+// atomics.  Here each particle gathers both halves itself, so results are
+// deterministic:
+//   viscosity/cohesion (queries at pos, a symmetric relation): own visits
+//     (+F(i,j)) and the neighbours' visits of i (-F(j,i)) from one range;
+//   pressure (queries at predPos, not symmetric): own visits from grid A
+//     around cell(predPos_i), neighbours' visits from grid B (cells of
+//     predPos of visiting particles) around cell(pos_i).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
 #include "ge_internal.hpp"
 
 namespace ge {
-int sph_new_grid(ge_ctx *c, const ge_step_params *) { c->err = "SPH not built yet"; return GE_E_UNSUPPORTED; }
-int sph_pcisph(ge_ctx *c, const ge_step_params *, float) { c->err = "SPH not built yet"; return GE_E_UNSUPPORTED; }
-int sph_neighbors(ge_ctx *c, int, int64_t *, uint32_t *, int64_t, int64_t *) { c->err = "SPH not built yet"; return GE_E_UNSUPPORTED; }
+namespace {
+
+constexpr float kPiF = 3.14159265358979323846f;  // raylib PI, a float
+constexpr uint32_t kSentinelCell = (1u << kCellBits) - 1u;
+constexpr uint32_t kGridMaxBlocks = 1024;
+
+struct SphConst {
+    float h, h2, cell, unit, visc, coh, delta, stiff_mult, coeff;
+    float vol, spiky, lap;  // kernel constants (SPH.h:38-56)
+    float half_dt;          // fused integrateEnd (0 = none)
+};
+
+struct GridView {
+    const uint32_t *first;
+    const uint32_t *ord;
+    const int32_t *frame;  // cx0, cy0, W, H, valid
+    const float4 *pk0, *pk1;
+    const uint32_t *pkf;
+};
+
+__device__ __forceinline__ bool is_visitor(uint32_t f) { return (f & GE_FLAG_SPH) && !(f & GE_FLAG_DRAWN); }
+
+// NeighborSearchV2AVX2::posToCellCoord (neighborSearch.h:631-635), with the
+// float clamped so far-away points cannot overflow the int conversion
+__device__ __forceinline__ int cell_of(float p, float cell) {
+    float c = floorf(p / cell);
+    c = fminf(fmaxf(c, -1073741824.0f), 1073741824.0f);
+    return (int)c;
+}
+
+// particles of cells (qx-1..qx+1, row) as one range of cell-order positions
+__device__ __forceinline__ bool row_range(const int32_t *fr, const uint32_t *first, int qx, int row, uint32_t &lo,
+                                          uint32_t &hi) {
+    if (!fr[4]) return false;
+    const int64_t r = (int64_t)row - fr[1];
+    if (r < 0 || r >= fr[3]) return false;
+    int64_t c0 = (int64_t)qx - 1 - fr[0], c1 = (int64_t)qx + 1 - fr[0];
+    if (c1 < 0 || c0 >= fr[2]) return false;
+    if (c0 < 0) c0 = 0;
+    if (c1 >= fr[2]) c1 = fr[2] - 1;
+    const uint64_t base = (uint64_t)r * (uint64_t)fr[2];
+    lo = first[base + (uint64_t)c0];
+    hi = first[base + (uint64_t)c1 + 1];
+    return lo < hi;
+}
+
+// SPH::smoothingKernelCohesion (SPH.h:58-64)
+__device__ __forceinline__ float coh_kernel(float r, float h) {
+#pragma clang fp contract(off)
+    if (r >= h) return 0.0f;
+    const float q = r / h;
+    return (1.0f - q) * (0.5f - q) * (0.5f - q) * 30.0f / (kPiF * h * h);
+}
+
+// SPH::spikyKernelDerivative (SPH.h:45-50): float scale times ::pow(double)
+__device__ __forceinline__ float spiky_grad(float r, float h, float scale) {
+    if (r >= h) return 0.0f;
+    const double d = (double)(h - r);
+    return (float)((double)scale * (d * d));
+}
+
+// min/max of (x, y) over the included particles, per workgroup
+__global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ x, const float *__restrict__ y,
+                                                   const uint32_t *__restrict__ flags, uint32_t n, int visitors_only,
+                                                   float4 *__restrict__ partials) {
+    __shared__ float red[4][kBlock / 64];
+    float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        if (visitors_only && !is_visitor(flags[i])) continue;
+        const float px = x[i], py = y[i];
+        mnx = glm_min(mnx, px); mny = glm_min(mny, py);
+        mxx = glm_max(mxx, px); mxy = glm_max(mxy, py);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mnx = glm_min(mnx, __shfl_xor(mnx, o, 64)); mny = glm_min(mny, __shfl_xor(mny, o, 64));
+        mxx = glm_max(mxx, __shfl_xor(mxx, o, 64)); mxy = glm_max(mxy, __shfl_xor(mxy, o, 64));
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[0][w] = mnx; red[1][w] = mny; red[2][w] = mxx; red[3][w] = mxy; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) {
+            mnx = glm_min(mnx, red[0][k]); mny = glm_min(mny, red[1][k]);
+            mxx = glm_max(mxx, red[2][k]); mxy = glm_max(mxy, red[3][k]);
+        }
+        partials[blockIdx.x] = make_float4(mnx, mny, mxx, mxy);
+    }
+}
+
+// Grid frame from the extremes: the occupied cell range.
+__global__ void k_frame(const float4 *__restrict__ partials, uint32_t nparts, const float *__restrict__ ext_in,
+                        float cell, uint32_t cells_cap, int32_t *__restrict__ frame, uint32_t *__restrict__ counters) {
+    if (threadIdx.x != 0) return;
+    float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
+    if (ext_in) {
+        mnx = ext_in[0]; mny = ext_in[1]; mxx = ext_in[2]; mxy = ext_in[3];
+    } else {
+        for (uint32_t k = 0; k < nparts; ++k) {
+            const float4 v = partials[k];
+            mnx = glm_min(mnx, v.x); mny = glm_min(mny, v.y);
+            mxx = glm_max(mxx, v.z); mxy = glm_max(mxy, v.w);
+        }
+    }
+    if (!(mnx <= mxx) || !(mny <= mxy)) {  // no particle in the grid
+        frame[0] = 0; frame[1] = 0; frame[2] = 0; frame[3] = 0; frame[4] = 1;
+        return;
+    }
+    const float lim = 536870912.0f;  // keep cell coordinates well inside int
+    const float fx0 = floorf(mnx / cell), fy0 = floorf(mny / cell);
+    const float fx1 = floorf(mxx / cell), fy1 = floorf(mxy / cell);
+    bool ok = fabsf(fx0) < lim && fabsf(fy0) < lim && fabsf(fx1) < lim && fabsf(fy1) < lim;
+    const int64_t W = ok ? (int64_t)fx1 - (int64_t)fx0 + 1 : 0;
+    const int64_t H = ok ? (int64_t)fy1 - (int64_t)fy0 + 1 : 0;
+    ok = ok && W > 0 && H > 0 && W * H <= (int64_t)cells_cap;
+    if (!ok) {
+        atomicOr(&counters[kCntError], kErrGridRange);
+        frame[0] = 0; frame[1] = 0; frame[2] = 0; frame[3] = 0; frame[4] = 0;
+        return;
+    }
+    frame[0] = (int32_t)fx0; frame[1] = (int32_t)fy0; frame[2] = (int32_t)W; frame[3] = (int32_t)H; frame[4] = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_keys(const float *__restrict__ x, const float *__restrict__ y,
+                                                      const uint32_t *__restrict__ flags, uint32_t n, int visitors_only,
+                                                      float cell, const int32_t *__restrict__ frame,
+                                                      uint64_t *__restrict__ packed) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t key = kSentinelCell;
+    if (frame[4] && (!visitors_only || is_visitor(flags[i]))) {
+        const int cx = cell_of(x[i], cell), cy = cell_of(y[i], cell);
+        key = (uint64_t)(cy - frame[1]) * (uint64_t)frame[2] + (uint64_t)(cx - frame[0]);
+    }
+    packed[i] = (key << kIdxBits) | i;
+}
+
+// first[c] for every cell c in [0, W*H]: the first cell-order position whose
+// key is >= c.  Position i fills the cells between its predecessor's key and
+// its own, so every entry has exactly one writer.
+__global__ __launch_bounds__(kBlock) void k_cell_table(const uint64_t *__restrict__ sorted, uint32_t n,
+                                                       const int32_t *__restrict__ frame, uint32_t *__restrict__ first,
+                                                       uint32_t *__restrict__ ord) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i > n) return;
+    const int64_t ncells = frame[4] ? (int64_t)frame[2] * frame[3] : 0;
+    if (i < n) ord[i] = (uint32_t)(sorted[i] & kIdxMask);
+    const int64_t ki = i < n ? (int64_t)(sorted[i] >> kIdxBits) : ncells;
+    const int64_t kp = i == 0 ? -1 : (int64_t)(sorted[i - 1] >> kIdxBits);
+    const int64_t lo = (kp < ncells ? kp : ncells) + 1;
+    const int64_t hi = ki < ncells ? ki : ncells;
+    for (int64_t c = lo; c <= hi; ++c) first[c] = i;
+}
+
+// cell-order packs of grid A: (x, y, vx, vy), (sphMass, visc, cohesion, dens), flags
+__global__ __launch_bounds__(kBlock) void k_pack_a(State s, uint32_t n, const uint32_t *__restrict__ ord,
+                                                   float4 *__restrict__ pk0, float4 *__restrict__ pk1,
+                                                   uint32_t *__restrict__ pkf) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = ord[k];
+    pk0[k] = make_float4(s.x[i], s.y[i], s.vx[i], s.vy[i]);
+    pk1[k] = make_float4(s.sph_mass[i], s.visc[i], s.cohesion[i], s.dens[i]);
+    pkf[k] = s.flags[i];
+}
+
+// SPH::computeViscCohesionForces (SPH.cpp:98-150) in gather form, then the
+// predicted position (SPH.cpp:173-181).  pp[k] = (predX, predY, restDens, sphMass).
+__global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A, SphConst K,
+                                                 float2 *__restrict__ sphf, float4 *__restrict__ pp) {
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = A.ord[k];
+    const uint32_t fi = A.pkf[k];
+    const float4 own = A.pk0[k];
+    const float4 oq = A.pk1[k];
+    if (!is_visitor(fi)) {
+        sphf[i] = make_float2(0.0f, 0.0f);
+        pp[k] = make_float4(s.ppx[i], s.ppy[i], s.rest_dens[i], oq.x);
+        return;
+    }
+    const bool fwd = !(fi & GE_FLAG_PINNED);
+    const float mI = oq.x * K.unit;
+    const float di = oq.w < 0.001f ? 0.001f : oq.w;
+    const int cx = cell_of(own.x, K.cell), cy = cell_of(own.y, K.cell);
+    float sx = 0.0f, sy = 0.0f;
+    for (int dr = -1; dr <= 1; ++dr) {
+        uint32_t lo, hi;
+        if (!row_range(A.frame, A.first, cx, cy + dr, lo, hi)) continue;
+        for (uint32_t kk = lo; kk < hi; ++kk) {
+            if (kk == k) continue;
+            const uint32_t fj = A.pkf[kk];
+            if (!is_visitor(fj)) continue;
+            const float4 pj = A.pk0[kk];
+            const float dx = pj.x - own.x, dy = pj.y - own.y;
+            const float rsq = dx * dx + dy * dy;
+            if (rsq >= K.h2) continue;
+            const float r = sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
+            const float4 qj = A.pk1[kk];
+            const float wc = coh_kernel(r, K.h);
+            if (fwd) {  // i's visit of j
+                const float mJ = qj.x * K.unit;
+                const float dj = qj.w < 0.001f ? 0.001f : qj.w;
+                const float vfx = K.visc * qj.y * mJ / dj * K.lap * (pj.z - own.z);
+                const float vfy = K.visc * qj.y * mJ / dj * K.lap * (pj.w - own.w);
+                const float cc = K.coh * oq.z;
+                const float cfx = cc * mJ * wc * (dx / r);
+                const float cfy = cc * mJ * wc * (dy / r);
+                sx = sx + (vfx + cfx);
+                sy = sy + (vfy + cfy);
+            }
+            if (!(fj & GE_FLAG_PINNED)) {  // j's visit of i, scattered back to i
+                const float ex = own.x - pj.x, ey = own.y - pj.y;
+                const float vfx = K.visc * oq.y * mI / di * K.lap * (own.z - pj.z);
+                const float vfy = K.visc * oq.y * mI / di * K.lap * (own.w - pj.w);
+                const float cc = K.coh * qj.z;
+                const float cfx = cc * mI * wc * (ex / r);
+                const float cfy = cc * mI * wc * (ey / r);
+                sx = sx - (vfx + cfx);
+                sy = sy - (vfy + cfy);
+            }
+        }
+    }
+    sphf[i] = make_float2(sx, sy);
+    const float px = own.x + (sx / oq.x) * K.coeff;
+    const float py = own.y + (sy / oq.x) * K.coeff;
+    s.ppx[i] = px;
+    s.ppy[i] = py;
+    pp[k] = make_float4(px, py, s.rest_dens[i], oq.x);
+}
+
+// predicted density and pressure (SPH.cpp:183-219); pd[k] = (press, predDens)
+__global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridView A, const float4 *__restrict__ pp,
+                                                    SphConst K, float2 *__restrict__ pd) {
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = A.ord[k];
+    const uint32_t fi = A.pkf[k];
+    if (!is_visitor(fi)) {
+        s.press[i] = 0.0f;  // press is cleared for every particle (SPH.cpp:162-165)
+        pd[k] = make_float2(0.0f, s.pred_dens[i]);
+        return;
+    }
+    const float4 me = pp[k];
+    const int cx = cell_of(me.x, K.cell), cy = cell_of(me.y, K.cell);
+    float dens = 0.0f;
+    for (int dr = -1; dr <= 1; ++dr) {
+        uint32_t lo, hi;
+        if (!row_range(A.frame, A.first, cx, cy + dr, lo, hi)) continue;
+        for (uint32_t kk = lo; kk < hi; ++kk) {
+            if (!is_visitor(A.pkf[kk])) continue;
+            const float4 pj = pp[kk];
+            const float dx = me.x - pj.x, dy = me.y - pj.y;
+            const float rrsq = dx * dx + dy * dy;
+            if (rrsq >= K.h * K.h) continue;
+            const float rr = sqrtf(rrsq);
+            const float mJ = pj.w * K.unit;
+            const float rho0 = 0.5f * (me.z + pj.z);
+            const float w = (K.h - rr) * (K.h - rr) / K.vol;
+            dens = dens + mJ * w / rho0;
+        }
+    }
+    float pt = K.delta * (dens - me.z);
+    if (pt < 0.0f) pt = 0.0f;
+    const float pr = 0.0f + pt * s.stiff[i] * K.stiff_mult;
+    s.pred_dens[i] = dens;
+    s.press_tmp[i] = pt;
+    s.press[i] = pr;
+    pd[k] = make_float2(pr, dens);
+}
+
+// grid B packs: (predX, predY, press, predDens), (sphMass, -, -, -)
+__global__ __launch_bounds__(kBlock) void k_pack_b(State s, uint32_t n, const uint32_t *__restrict__ ord,
+                                                   float4 *__restrict__ pk0, float4 *__restrict__ pk1) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = ord[k];
+    pk0[k] = make_float4(s.ppx[i], s.ppy[i], s.press[i], s.pred_dens[i]);
+    pk1[k] = make_float4(s.sph_mass[i], 0.0f, 0.0f, 0.0f);
+}
+
+// pressure force of one visit (SPH.cpp:233-256); dx, dy = predPos_visitor - predPos_other
+__device__ __forceinline__ void pressure_pair(float dx, float dy, float rr, float mi, float mj, float pi_, float pj_,
+                                              float di, float dj, const SphConst &K, float &fx, float &fy) {
+#pragma clang fp contract(off)
+    const float gw = spiky_grad(rr, K.h, K.spiky);
+    const float nx = dx / rr, ny = dy / rr;
+    const float avgP = 0.5f * (pi_ + pj_);
+    const float avgD = 0.5f * (di + dj);
+    float mag = -(mi * K.unit + mj * K.unit) * avgP / (avgD < 0.01f ? 0.01f : avgD);
+    const float smax = mi < mj ? mj : mi;
+    const float smin = mj < mi ? mj : mi;
+    const float sc8 = 8.0f / (smax / smin);
+    mag = mag * (sc8 < 1.0f ? sc8 : 1.0f);
+    fx = mag * gw * nx;
+    fy = mag * gw * ny;
+}
+
+// pressure forces (SPH.cpp:221-264), pressF and acc (SPH.cpp:271-280), then
+// the integrateEnd kick (physics.cpp:1114-1125) when the frame asks for it
+__global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridView A, const float4 *__restrict__ pp,
+                                                     const float2 *__restrict__ pd, GridView B, SphConst K,
+                                                     const float2 *__restrict__ sphf) {
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = A.ord[k];
+    const uint32_t fi = A.pkf[k];
+    float2 f = sphf[i];
+    if (is_visitor(fi)) {
+        const float4 me = pp[k];
+        const float2 mp = pd[k];
+        // i's visits: candidates by pos cell around cell(predPos_i)
+        const int qx = cell_of(me.x, K.cell), qy = cell_of(me.y, K.cell);
+        for (int dr = -1; dr <= 1; ++dr) {
+            uint32_t lo, hi;
+            if (!row_range(A.frame, A.first, qx, qy + dr, lo, hi)) continue;
+            for (uint32_t kk = lo; kk < hi; ++kk) {
+                if (kk == k || !is_visitor(A.pkf[kk])) continue;
+                const float4 pj = pp[kk];
+                const float dx = me.x - pj.x, dy = me.y - pj.y;
+                const float rr = sqrtf(dx * dx + dy * dy);
+                if (rr < 1e-5f || rr >= K.h) continue;
+                const float2 q = pd[kk];
+                float fx, fy;
+                pressure_pair(dx, dy, rr, me.w, pj.w, mp.x, q.x, mp.y, q.y, K, fx, fy);
+                f.x = f.x + fx;
+                f.y = f.y + fy;
+            }
+        }
+        // visits of i by others: predPos cells around cell(pos_i)
+        const float4 mypos = A.pk0[k];
+        const int px = cell_of(mypos.x, K.cell), py = cell_of(mypos.y, K.cell);
+        for (int dr = -1; dr <= 1; ++dr) {
+            uint32_t lo, hi;
+            if (!row_range(B.frame, B.first, px, py + dr, lo, hi)) continue;
+            for (uint32_t kb = lo; kb < hi; ++kb) {
+                if (B.ord[kb] == i) continue;
+                const float4 pj = B.pk0[kb];
+                const float dx = pj.x - me.x, dy = pj.y - me.y;
+                const float rr = sqrtf(dx * dx + dy * dy);
+                if (rr < 1e-5f || rr >= K.h) continue;
+                const float mj = B.pk1[kb].x;
+                float fx, fy;
+                pressure_pair(dx, dy, rr, mj, me.w, pj.z, mp.x, pj.w, mp.y, K, fx, fy);
+                f.x = f.x - fx;
+                f.y = f.y - fy;
+            }
+        }
+    }
+    const float sm = s.sph_mass[i];
+    const float pfx = f.x / sm, pfy = f.y / sm;
+    s.pfx[i] = pfx;
+    s.pfy[i] = pfy;
+    if (!(fi & GE_FLAG_PINNED)) {
+        const float ax = s.ax[i] + pfx, ay = s.ay[i] + pfy;
+        s.ax[i] = ax;
+        s.ay[i] = ay;
+        if (K.half_dt != 0.0f) {
+            s.vx[i] = s.vx[i] + ax * K.half_dt;
+            s.vy[i] = s.vy[i] + ay * K.half_dt;
+        }
+    }
+}
+
+// ---- candidate lists for the parity hook
+__global__ __launch_bounds__(kBlock) void k_cand_count(const float *__restrict__ qx, const float *__restrict__ qy,
+                                                       uint32_t n, float cell, const int32_t *__restrict__ frame,
+                                                       const uint32_t *__restrict__ first, uint32_t *__restrict__ cnt) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int cx = cell_of(qx[i], cell), cy = cell_of(qy[i], cell);
+    uint32_t c = 0;
+    for (int dr = -1; dr <= 1; ++dr) {
+        uint32_t lo, hi;
+        if (row_range(frame, first, cx, cy + dr, lo, hi)) c += hi - lo;
+    }
+    cnt[i] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cand_fill(const float *__restrict__ qx, const float *__restrict__ qy,
+                                                      uint32_t n, float cell, const int32_t *__restrict__ frame,
+                                                      const uint32_t *__restrict__ first, const uint32_t *__restrict__ ord,
+                                                      const int64_t *__restrict__ off, uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int cx = cell_of(qx[i], cell), cy = cell_of(qy[i], cell);
+    int64_t w = off[i];
+    for (int dr = -1; dr <= 1; ++dr) {
+        uint32_t lo, hi;
+        if (!row_range(frame, first, cx, cy + dr, lo, hi)) continue;
+        for (uint32_t kk = lo; kk < hi; ++kk) out[w++] = ord[kk];
+    }
+}
+
+SphConst make_const(const ge_step_params *p, float half_dt) {
+    SphConst K;
+    K.h = p->sph_h;
+    K.h2 = p->sph_h * p->sph_h;
+    K.cell = p->sph_cell;
+    K.unit = p->sph_unit_mass;
+    K.visc = p->viscosity;
+    K.coh = p->cohesion_coef;
+    K.delta = p->delta;
+    K.stiff_mult = p->stiff_mult;
+    K.coeff = 0.5f * p->dt * p->dt;  // predictionCoeff (SPH.cpp:156)
+    // SPH.h:38-56: ::pow in double with the float PI, rounded to float
+    K.vol = (float)(((double)kPiF * std::pow((double)K.h, 4.0)) / 6.0);
+    K.spiky = (float)(-45.0 / ((double)kPiF * std::pow((double)K.h, 6.0)));
+    K.lap = (float)(45.0 / ((double)kPiF * std::pow((double)K.h, 6.0)));
+    K.half_dt = half_dt;
+    return K;
+}
+
+GridView view(const Grid &g) { return GridView{g.first, g.ord, g.frame, g.pk0, g.pk1, g.pkf}; }
+
+uint32_t grid_blocks(uint32_t n) {
+    uint32_t b = blocks_for(n, kBlock);
+    return b > kGridMaxBlocks ? kGridMaxBlocks : b;
+}
+
+// sort by cell and build first[] / ord[] for one grid
+void build_grid(ge_ctx *c, Grid &g) {
+    const uint32_t n = c->n;
+    radix_sort_packed(c, g.keys, g.keys_alt, n, kIdxBits, kIdxBits + kCellBits);
+    hipLaunchKernelGGL(k_cell_table, dim3(blocks_for((uint64_t)n + 1, kBlock)), dim3(kBlock), 0, c->stream, g.keys, n,
+                       g.frame, g.first, g.ord);
+}
+
+}  // namespace
+
+// NeighborSearchV2AVX2::newGridAVX2 (neighborSearch.h:636-720): grid A over
+// pos, using the extremes stored with the frame's bounding box.
+int sph_new_grid(ge_ctx *c, const ge_step_params *p) {
+    Grid &g = c->b.grid_pos;
+    const State &s = c->b.st[c->b.cur];
+    const uint32_t n = c->n;
+    hipLaunchKernelGGL(k_frame, dim3(1), dim3(64), 0, c->stream, nullptr, 0u, c->b.bb + 4, p->sph_cell, g.cells_cap,
+                       g.frame, c->b.counters);
+    hipLaunchKernelGGL(k_cell_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, s.flags, n, 0,
+                       p->sph_cell, g.frame, g.keys);
+    build_grid(c, g);
+    hipLaunchKernelGGL(k_pack_a, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s, n, g.ord, g.pk0, g.pk1,
+                       g.pkf);
+    c->have_grid = true;
+    c->sph_cell = p->sph_cell;
+    return GE_OK;
+}
+
+// SPH::PCISPH (SPH.cpp:152-281), one iteration
+int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt) {
+    Grid &A = c->b.grid_pos;
+    Grid &B = c->b.grid_pred;
+    const State &s = c->b.st[c->b.cur];
+    const uint32_t n = c->n;
+    const SphConst K = make_const(p, end_kick_dt * 0.5f);
+    const uint32_t nb = blocks_for(n, kBlock);
+    float2 *sphf = reinterpret_cast<float2 *>(c->b.sphf);
+    // pp (predX, predY, restDens, sphMass) and pd (press, predDens) in grid-A order
+    float4 *ppA = reinterpret_cast<float4 *>(c->b.sph_pp);
+    float2 *pdA = reinterpret_cast<float2 *>(c->b.sph_pd);
+    hipLaunchKernelGGL(k_visc, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view(A), K, sphf, ppA);
+    // grid B over the predicted positions of visiting particles
+    const uint32_t mb = grid_blocks(n);
+    hipLaunchKernelGGL(k_minmax, dim3(mb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.flags, n, 1,
+                       reinterpret_cast<float4 *>(c->b.partials2));
+    hipLaunchKernelGGL(k_frame, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<const float4 *>(c->b.partials2), mb,
+                       nullptr, p->sph_cell, B.cells_cap, B.frame, c->b.counters);
+    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.flags, n, 1, p->sph_cell,
+                       B.frame, B.keys);
+    build_grid(c, B);
+    hipLaunchKernelGGL(k_density, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view(A), ppA, K, pdA);
+    hipLaunchKernelGGL(k_pack_b, dim3(nb), dim3(kBlock), 0, c->stream, s, n, B.ord, B.pk0, B.pk1);
+    hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view(A), ppA, pdA, view(B), K, sphf);
+    return GE_OK;
+}
+
+// queryNeighborsAVX2 candidate sets of grid A at pos (which 0) or predPos (1)
+int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total) {
+    const Grid &g = c->b.grid_pos;
+    const State &s = c->b.st[c->b.cur];
+    const uint32_t n = c->n;
+    const float *qx = which ? s.ppx : s.x;
+    const float *qy = which ? s.ppy : s.y;
+    uint32_t *dcnt = nullptr;
+    int64_t *doff = nullptr;
+    uint32_t *dout = nullptr;
+    // the cell size of the grid is the one it was built with
+    float cell = c->sph_cell;
+    GE_HIP_CHECK(c, hipMalloc(&dcnt, sizeof(uint32_t) * (n + 1)));
+    hipLaunchKernelGGL(k_cand_count, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, qx, qy, n, cell, g.frame,
+                       g.first, dcnt);
+    std::vector<uint32_t> hc(n);
+    GE_HIP_CHECK(c, hipMemcpyAsync(hc.data(), dcnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    off[0] = 0;
+    for (uint32_t i = 0; i < n; ++i) off[i + 1] = off[i] + hc[i];
+    *total = off[n];
+    if (!idx) {
+        (void)hipFree(dcnt);
+        return GE_OK;
+    }
+    if (cap < off[n]) {
+        (void)hipFree(dcnt);
+        c->err = "neighbour buffer too small";
+        return GE_E_CAPACITY;
+    }
+    GE_HIP_CHECK(c, hipMalloc(&doff, sizeof(int64_t) * (n + 1)));
+    GE_HIP_CHECK(c, hipMalloc(&dout, sizeof(uint32_t) * (off[n] > 0 ? off[n] : 1)));
+    GE_HIP_CHECK(c, hipMemcpyAsync(doff, off, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_cand_fill, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, qx, qy, n, cell, g.frame,
+                       g.first, g.ord, doff, dout);
+    GE_HIP_CHECK(c, hipMemcpyAsync(idx, dout, sizeof(uint32_t) * off[n], hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(dcnt);
+    (void)hipFree(doff);
+    (void)hipFree(dout);
+    for (uint32_t i = 0; i < n; ++i) std::sort(idx + off[i], idx + off[i + 1]);
+    return GE_OK;
+}
+
 }  // namespace ge

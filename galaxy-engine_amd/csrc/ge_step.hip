@@ -101,6 +101,8 @@ __global__ __launch_bounds__(kBlock) void k_bbox_final(const float *__restrict__
         bb[1] = cy - half;
         bb[2] = size;
         bb[3] = 0.0f;
+        // the exact extremes, for the SPH grid frame
+        bb[4] = mnx; bb[5] = mny; bb[6] = mxx; bb[7] = mxy;
     }
 }
 
