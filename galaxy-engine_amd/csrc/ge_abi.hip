@@ -61,7 +61,7 @@ void free_all(ge_ctx *c) {
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
     dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
-    dfree(b.nodes); dfree(b.node_geom); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
+    dfree(b.nodes); dfree(b.node_geom); dfree(b.node_quad); dfree(b.moments); dfree(b.moment_part); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
         dfree(g->keys); dfree(g->keys_alt); dfree(g->first); dfree(g->ord); dfree(g->frame);
         dfree(g->pk0); dfree(g->pk1); dfree(g->pkf);
@@ -109,6 +109,9 @@ int alloc_all(ge_ctx *c) {
     b.node_cap = 4u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(b.nodes, b.node_cap));
     GE_HIP_CHECK(c, dalloc(b.node_geom, b.node_cap));
+    GE_HIP_CHECK(c, dalloc(b.node_quad, b.node_cap));
+    GE_HIP_CHECK(c, dalloc(b.moments, 6ull * ((size_t)cap + 1)));
+    GE_HIP_CHECK(c, dalloc(b.moment_part, 6ull * ((size_t)cap / 1024 + 64)));
     if (c->sph) {
         GE_HIP_CHECK(c, dalloc(b.sphf, 2ull * cap));
         GE_HIP_CHECK(c, dalloc(b.sph_pp, 4ull * cap));
@@ -144,6 +147,8 @@ int check_params(ge_ctx *c, const ge_step_params *p) {
     if (p->sph_ground) return fail(c, GE_E_UNSUPPORTED, "SPH ground walls are not implemented");
     if (p->sph && !c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
     if (p->sph && !(p->sph_cell > 0.0f && p->sph_h > 0.0f)) return fail(c, GE_E_INVALID, "sph_h and sph_cell must be > 0");
+    if (p->accuracy != 0 && p->accuracy != 1) return fail(c, GE_E_INVALID, "accuracy must be 0 or 1");
+    if (p->accuracy && !(p->theta > 0.0f)) return fail(c, GE_E_INVALID, "accuracy mode needs theta > 0");
     return GE_OK;
 }
 
@@ -213,10 +218,12 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
     const float end_kick = p->sph ? 0.0f : p->dt * 0.5f;  // fused integrateEnd when no SPH follows
     if (p->G != 0.0f) {
         PhaseScope s(c, GE_PHASE_GRAVITY);
-        if (use_direct(c, p) || !c->have_tree)
+        if (use_direct(c, p) || !c->have_tree) {
             launch_direct(c, p, end_kick);
-        else
+        } else {
+            if (p->accuracy) launch_quad_moments(c, p->theta);
             launch_bh(c, p, end_kick, count_inter);
+        }
     } else {
         launch_zero_acc(c);
         if (!p->sph) {
@@ -476,6 +483,7 @@ int ge_compute_gravity(ge_ctx *c, const ge_step_params *p) {
     } else if (use_direct(c, p) || !c->have_tree) {
         launch_direct(c, p, 0.0f);
     } else {
+        if (p->accuracy) launch_quad_moments(c, p->theta);
         launch_bh(c, p, 0.0f, true);
     }
     return ge_sync(c);

@@ -116,7 +116,10 @@ struct Buffers {
     uint32_t *start_level = nullptr;  // per particle: shallowest level of such a node
     uint32_t *scan_part = nullptr;    // block sums of the node-start scan
     NodeRec *nodes = nullptr;
-    float4 *node_geom = nullptr;   // debug: (posx, posy, size, 0) per final node
+    float4 *node_geom = nullptr;   // (posx, posy, size, 0) per final node
+    float4 *node_quad = nullptr;   // accuracy mode: (Qxx, Qxy, Qyy, opening radius)
+    double *moments = nullptr;     // accuracy mode: 6 prefix moments per particle (n + 1)
+    double *moment_part = nullptr; // block sums of the moment scan
     uint32_t node_cap = 0;
     float *sphf = nullptr;         // SPH force scratch (x, y interleaved), 2 per particle
     float *sph_pp = nullptr;       // grid-A order (predX, predY, restDens, sphMass)
@@ -155,6 +158,7 @@ void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_v
 void launch_morton_keys(ge_ctx *c, const float *bb_dev);
 void launch_gather(ge_ctx *c);
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_dt, bool count);
+void launch_quad_moments(ge_ctx *c, float theta);
 void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
 void launch_zero_acc(ge_ctx *c);
 void launch_integrate_end(ge_ctx *c, float dt);

@@ -187,9 +187,15 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // the walk leaves k's subtree (index k + next + 1).  Node reads are
 // wave-uniform (scalar loads).  The acceptance test uses the reference's
 // unfused arithmetic so the accepted-node sets match the CPU path.
-template <bool KAHAN, bool COUNT>
+//
+// QUAD (accuracy mode, no counterpart in the reference): the node is accepted
+// when |d| exceeds its opening radius (size/theta + COM offset) and adds its
+// quadrupole term  -G Q d / R^5 + 5/2 G (d^T Q d - eps^2 tr Q) d / R^7,
+// R^2 = |d|^2 + eps^2, to the softened monopole.
+template <bool KAHAN, bool COUNT, bool QUAD>
 __global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
                                                const NodeRec *__restrict__ nodes,
+                                               const float4 *__restrict__ quad,
                                                const uint32_t *__restrict__ counters, uint32_t n,
                                                float G, float theta_sq, float soft_sq,
                                                float *__restrict__ ax, float *__restrict__ ay,
@@ -219,7 +225,15 @@ __global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
         const bool active = idx >= skip_until;
         float dx, dy, dist_sq;
         bool accept;
-        {
+        float4 qd;
+        if (QUAD) {
+            qd = quad[idx];
+            dx = nd.comx - p.x;
+            dy = nd.comy - p.y;
+            const float d2 = dx * dx + dy * dy;
+            dist_sq = d2 + soft_sq;
+            accept = d2 > qd.w * qd.w;
+        } else {
 #pragma clang fp contract(off)
             dx = nd.comx - p.x;
             dy = nd.comy - p.y;
@@ -228,13 +242,28 @@ __global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
         }
         if (active && accept) {
             const float inv = __builtin_amdgcn_rsqf(dist_sq);
-            const float f = G * nd.mass * (inv * inv * inv);
-            if (KAHAN) {
-                kahan_add(fx, cx, dx * f);
-                kahan_add(fy, cy, dy * f);
+            const float inv2 = inv * inv;
+            const float inv3 = inv2 * inv;
+            float tx, ty;
+            if (QUAD) {
+                const float inv5 = inv3 * inv2, inv7 = inv5 * inv2;
+                const float qdx = qd.x * dx + qd.y * dy, qdy = qd.y * dx + qd.z * dy;
+                const float dqd = dx * qdx + dy * qdy - soft_sq * (qd.x + qd.z);
+                const float mono = nd.mass * inv3;
+                const float rad = 2.5f * dqd * inv7;
+                tx = G * ((mono + rad) * dx - qdx * inv5);
+                ty = G * ((mono + rad) * dy - qdy * inv5);
             } else {
-                fx = fmaf(dx, f, fx);
-                fy = fmaf(dy, f, fy);
+                const float f = G * nd.mass * inv3;
+                tx = dx * f;
+                ty = dy * f;
+            }
+            if (KAHAN) {
+                kahan_add(fx, cx, tx);
+                kahan_add(fy, cy, ty);
+            } else {
+                fx += tx;
+                fy += ty;
             }
             if (COUNT) ++cnt;
             skip_until = after;
@@ -360,32 +389,29 @@ void launch_gather(ge_ctx *c) {
     c->b.cur = 1 - cur;
 }
 
-void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
+template <bool KAHAN, bool COUNT, bool QUAD>
+void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const State &s = c->b.st[c->b.cur];
     const float theta_sq = p->theta * p->theta;
     const float soft_sq = p->softening * p->softening;
     const uint32_t nb = blocks_for(c->n, kBlock);
-    if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, c->stream, c->b.interactions, 0ull);
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
-    if (p->compensated) {
-        if (count)
-            hipLaunchKernelGGL((k_bh<true, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
-                               c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
-                               end_kick_half_dt, inter);
-        else
-            hipLaunchKernelGGL((k_bh<true, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
-                               c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
-                               end_kick_half_dt, inter);
-    } else {
-        if (count)
-            hipLaunchKernelGGL((k_bh<false, true>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
-                               c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
-                               end_kick_half_dt, inter);
-        else
-            hipLaunchKernelGGL((k_bh<false, false>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
-                               c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
-                               end_kick_half_dt, inter);
-    }
+    hipLaunchKernelGGL((k_bh<KAHAN, COUNT, QUAD>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
+                       c->b.node_quad, c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
+                       end_kick_half_dt, inter);
+}
+
+void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
+    if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, c->stream, c->b.interactions, 0ull);
+    const bool k = p->compensated != 0, q = p->accuracy != 0;
+    if (k && count && q) launch_bh_t<true, true, true>(c, p, end_kick_half_dt);
+    else if (k && count) launch_bh_t<true, true, false>(c, p, end_kick_half_dt);
+    else if (k && q) launch_bh_t<true, false, true>(c, p, end_kick_half_dt);
+    else if (k) launch_bh_t<true, false, false>(c, p, end_kick_half_dt);
+    else if (count && q) launch_bh_t<false, true, true>(c, p, end_kick_half_dt);
+    else if (count) launch_bh_t<false, true, false>(c, p, end_kick_half_dt);
+    else if (q) launch_bh_t<false, false, true>(c, p, end_kick_half_dt);
+    else launch_bh_t<false, false, false>(c, p, end_kick_half_dt);
 }
 
 void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {

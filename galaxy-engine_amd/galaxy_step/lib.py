@@ -56,8 +56,8 @@ class StepParams(C.Structure):
                 ("sph_cell", C.c_float), ("sph_unit_mass", C.c_float), ("viscosity", C.c_float),
                 ("cohesion_coef", C.c_float), ("delta", C.c_float), ("stiff_mult", C.c_float),
                 ("sph_max_vel", C.c_float), ("periodic", C.c_int32), ("temperature", C.c_int32),
-                ("sph_ground", C.c_int32), ("compensated", C.c_int32),
-                ("reserved", C.c_uint32 * 6)]
+                ("sph_ground", C.c_int32), ("compensated", C.c_int32), ("accuracy", C.c_int32),
+                ("reserved", C.c_uint32 * 5)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

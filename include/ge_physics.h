@@ -92,7 +92,11 @@ typedef struct {
     int32_t temperature;  /* isTempEnabled: unsupported */
     int32_t sph_ground;   /* sphGround: unsupported */
     int32_t compensated;  /* Kahan-compensated force accumulation (1 = on) */
-    uint32_t reserved[6];
+    int32_t accuracy;     /* 0: the reference's monopole walk and opening test;
+                             1: quadrupole moments and the offset-corrected
+                             opening test |d| > size/theta + |COM - centre|
+                             (no counterpart in the reference) */
+    uint32_t reserved[5];
 } ge_step_params;
 
 /* Quadtree node as laid out by the reference (GE/include/Physics/quadtree.h:

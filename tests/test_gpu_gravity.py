@@ -239,3 +239,43 @@ def test_unsupported_flags_rejected():
             c.step(_params(periodic=1), 1)
         with pytest.raises(Exception):
             c.step(_params(temperature=1), 1)
+
+
+def _fp64_errors(d, xs, ys, ms, t):
+    fx, fy = O.direct_fp64(xs, ys, ms, 1.0, 1.0, t)
+    err = np.hypot(d["ax"][t] - fx, d["ay"][t] - fy)
+    mag = np.hypot(fx, fy)
+    rel = err / mag
+    return math.sqrt(np.mean(rel ** 2)), math.sqrt(np.mean(err ** 2) / np.mean(mag ** 2))
+
+
+def test_accuracy_mode_two_disks_theta07():
+    # quadrupole moments + offset-corrected opening test, checked against the
+    # FP64 direct sum on 4096 sampled targets
+    st, _ = configs.config1(n=1_000_000)
+    xs, ys, ms, bb, d, inter = _run_gravity(st, dict(accuracy=1), 0.7)
+    t = np.sort(np.random.default_rng(0).choice(len(xs), 4096, replace=False)).astype(np.uint32)
+    rms_rel, rms_norm = _fp64_errors(d, xs, ys, ms, t)
+    _, _, _, _, d0, _ = _run_gravity(st, dict(accuracy=0), 0.7)
+    rms_rel0, rms_norm0 = _fp64_errors(d0, xs, ys, ms, t)
+    print(f"theta=0.7 accuracy mode: per-particle RMS {rms_rel:.3e}, force-normalised RMS {rms_norm:.3e}; "
+          f"reference walk: {rms_rel0:.3e}, {rms_norm0:.3e}; interactions/particle {inter / len(xs):.1f}")
+    assert rms_norm < 1e-3
+    assert rms_rel < 3e-3
+    assert rms_rel < 0.25 * rms_rel0
+
+
+def test_accuracy_mode_disk_per_particle():
+    st, _ = configs.config0()
+    xs, ys, ms, bb, d, inter = _run_gravity(st, dict(accuracy=1), 0.7)
+    t = np.arange(0, len(xs), 5, dtype=np.uint32)
+    rms_rel, rms_norm = _fp64_errors(d, xs, ys, ms, t)
+    assert rms_rel < 1e-3 and rms_norm < 1e-4
+
+
+def test_accuracy_mode_small_theta_is_direct():
+    st, _ = configs.config0(seed=21, n=6000)
+    xs, ys, ms, bb, d, inter = _run_gravity(st, dict(accuracy=1), 0.05)
+    t = np.arange(len(xs), dtype=np.uint32)
+    rms_rel, _ = _fp64_errors(d, xs, ys, ms, t)
+    assert rms_rel < 2e-5
