@@ -56,7 +56,9 @@ void free_all(ge_ctx *c) {
     free_state(b.st[0]);
     free_state(b.st[1]);
     dfree(b.pxym); dfree(b.sort_a); dfree(b.sort_b); dfree(b.keys); dfree(b.perm); dfree(b.hist);
-    dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.bb); dfree(b.interactions);
+    dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.interactions);
+    dfree(b.sys_off); dfree(b.sys_of); dfree(b.sys_bb); dfree(b.chunk_lo); dfree(b.chunk_first);
+    b.bb = nullptr;
     TopNodes &T = b.top;
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
     dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
@@ -80,11 +82,17 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.perm, cap));
     b.hist_cap = radix_hist_entries(cap);
     GE_HIP_CHECK(c, dalloc(b.hist, b.hist_cap));
-    GE_HIP_CHECK(c, dalloc(b.partials, 4 * 4096));
+    b.max_chunks = cap / kChunk + c->max_sys + 1;
+    GE_HIP_CHECK(c, dalloc(b.partials, 4ull * b.max_chunks));
     GE_HIP_CHECK(c, dalloc(b.partials2, 4 * 4096));
     GE_HIP_CHECK(c, dalloc(b.counters, kCntSlots));
     GE_HIP_CHECK(c, hipMemset(b.counters, 0, sizeof(uint32_t) * kCntSlots));
-    GE_HIP_CHECK(c, dalloc(b.bb, 16));
+    GE_HIP_CHECK(c, dalloc(b.sys_off, (size_t)c->max_sys + 1));
+    GE_HIP_CHECK(c, dalloc(b.sys_of, cap));
+    GE_HIP_CHECK(c, dalloc(b.sys_bb, 8ull * c->max_sys));
+    GE_HIP_CHECK(c, dalloc(b.chunk_lo, (size_t)b.max_chunks + 1));
+    GE_HIP_CHECK(c, dalloc(b.chunk_first, (size_t)c->max_sys + 1));
+    b.bb = b.sys_bb;  // system 0's box doubles as the single-scene box
     GE_HIP_CHECK(c, dalloc(b.interactions, 1));
     TopNodes &T = b.top;
     T.cap = 2u * cap + 1024u;
@@ -110,8 +118,8 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.nodes, b.node_cap));
     GE_HIP_CHECK(c, dalloc(b.node_geom, b.node_cap));
     GE_HIP_CHECK(c, dalloc(b.node_quad, b.node_cap));
-    GE_HIP_CHECK(c, dalloc(b.moments, 6ull * ((size_t)cap + 1)));
-    GE_HIP_CHECK(c, dalloc(b.moment_part, 6ull * ((size_t)cap / 1024 + 64)));
+    GE_HIP_CHECK(c, dalloc(b.moments, 6ull * ((size_t)cap + c->max_sys + 1)));
+    GE_HIP_CHECK(c, dalloc(b.moment_part, 6ull * ((size_t)cap / kChunk + c->max_sys + 2)));
     if (c->sph) {
         GE_HIP_CHECK(c, dalloc(b.sphf, 2ull * cap));
         GE_HIP_CHECK(c, dalloc(b.sph_pp, 4ull * cap));
@@ -135,6 +143,50 @@ int alloc_all(ge_ctx *c) {
     return GE_OK;
 }
 
+int bits_for(uint64_t v) {  // bits to hold values 0 .. v-1
+    int b = 0;
+    while ((1ull << b) < v) ++b;
+    return b;
+}
+
+// System offsets, the per-slot system table and the reduction chunks.
+int set_systems(ge_ctx *c, uint32_t nsys, const uint32_t *counts) {
+    std::vector<uint32_t> off(nsys + 1, 0);
+    uint32_t maxc = 1;
+    for (uint32_t s = 0; s < nsys; ++s) {
+        off[s + 1] = off[s] + counts[s];
+        if (counts[s] > maxc) maxc = counts[s];
+    }
+    const uint32_t n = off[nsys];
+    if (nsys == 1) {
+        c->idx_bits = kIdxBits;
+        c->sys_bits = 0;
+    } else {
+        c->idx_bits = bits_for(maxc);
+        c->sys_bits = bits_for(nsys);
+        if (c->idx_bits + c->sys_bits > kIdxBits) return GE_E_CAPACITY;
+    }
+    std::vector<uint32_t> sys_of(n > 0 ? n : 1);
+    std::vector<uint32_t> chunk_lo, chunk_first(nsys + 1);
+    for (uint32_t s = 0; s < nsys; ++s) {
+        chunk_first[s] = (uint32_t)chunk_lo.size();
+        for (uint32_t i = off[s]; i < off[s + 1]; ++i) sys_of[i] = s;
+        for (uint32_t a = off[s]; a < off[s + 1]; a += kChunk) chunk_lo.push_back(a);
+        if (counts[s] == 0) chunk_lo.push_back(off[s]);
+    }
+    chunk_first[nsys] = (uint32_t)chunk_lo.size();
+    if (chunk_lo.size() > c->b.max_chunks) return GE_E_CAPACITY;
+    c->nchunks = (uint32_t)chunk_lo.size();
+    chunk_lo.push_back(n);
+    c->nsys = nsys;
+    c->host_off = off;
+    GE_HIP_CHECK(c, hipMemcpy(c->b.sys_off, off.data(), sizeof(uint32_t) * (nsys + 1), hipMemcpyHostToDevice));
+    GE_HIP_CHECK(c, hipMemcpy(c->b.sys_of, sys_of.data(), sizeof(uint32_t) * sys_of.size(), hipMemcpyHostToDevice));
+    GE_HIP_CHECK(c, hipMemcpy(c->b.chunk_lo, chunk_lo.data(), sizeof(uint32_t) * chunk_lo.size(), hipMemcpyHostToDevice));
+    GE_HIP_CHECK(c, hipMemcpy(c->b.chunk_first, chunk_first.data(), sizeof(uint32_t) * (nsys + 1), hipMemcpyHostToDevice));
+    return GE_OK;
+}
+
 int fail(ge_ctx *c, int code, const char *msg) {
     if (c) c->err = msg;
     return code;
@@ -146,6 +198,7 @@ int check_params(ge_ctx *c, const ge_step_params *p) {
     if (p->temperature) return fail(c, GE_E_UNSUPPORTED, "temperature transfer is not implemented");
     if (p->sph_ground) return fail(c, GE_E_UNSUPPORTED, "SPH ground walls are not implemented");
     if (p->sph && !c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    if (p->sph && c->nsys > 1) return fail(c, GE_E_UNSUPPORTED, "SPH runs on single-system contexts");
     if (p->sph && !(p->sph_cell > 0.0f && p->sph_h > 0.0f)) return fail(c, GE_E_INVALID, "sph_h and sph_cell must be > 0");
     if (p->accuracy != 0 && p->accuracy != 1) return fail(c, GE_E_INVALID, "accuracy must be 0 or 1");
     if (p->accuracy && !(p->theta > 0.0f)) return fail(c, GE_E_INVALID, "accuracy mode needs theta > 0");
@@ -193,6 +246,7 @@ struct PhaseScope {
 
 bool use_direct(const ge_ctx *c, const ge_step_params *p) {
     // globalLogic.cpp:1319-1348: naive flag, or N < 5000 on either code path
+    if (c->nsys > 1) return false;  // the batched form always walks each system's tree
     return p->naive || (int64_t)c->n < (int64_t)p->direct_below;
 }
 
@@ -201,12 +255,12 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
     if (dt_start != 0.0f) {
         // integrateStart + boundingBox, then computeMortonKeys, the sort and the tree
         { PhaseScope s(c, GE_PHASE_INTEGRATE_BBOX); launch_integrate_start_bbox(c, dt_start, p->sph, p->sph_max_vel, 1); }
-        { PhaseScope s(c, GE_PHASE_KEYS); launch_morton_keys(c, c->b.bb); }
-        { PhaseScope s(c, GE_PHASE_SORT); radix_sort_packed(c, c->b.sort_a, c->b.sort_b, c->n, kIdxBits, kIdxBits + 36); }
+        { PhaseScope s(c, GE_PHASE_KEYS); launch_morton_keys(c); }
+        { PhaseScope s(c, GE_PHASE_SORT); launch_sort_keys(c); }
         { PhaseScope s(c, GE_PHASE_GATHER); launch_gather(c); }
         if (!p->naive) {
             PhaseScope s(c, GE_PHASE_TREE);
-            launch_tree_build(c, c->b.bb);
+            launch_tree_build(c);
             c->have_tree = true;
         }
     }
@@ -250,6 +304,7 @@ void ge_default_config(ge_config *cfg) {
     cfg->capacity = 1u << 20;
     cfg->device = 0;
     cfg->enable_sph = 0;
+    cfg->max_systems = 1;
 }
 
 void ge_default_params(ge_step_params *p) {
@@ -286,6 +341,7 @@ int ge_create(ge_ctx **out, const ge_config *cfg) {
     c->device = cfg->device;
     c->cap = cfg->capacity;
     c->sph = cfg->enable_sph != 0;
+    c->max_sys = cfg->max_systems ? cfg->max_systems : 1;
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return GE_E_HIP;
@@ -328,11 +384,23 @@ int ge_sync(ge_ctx *c) {
     return check_device_errors(c);
 }
 
-int ge_upload(ge_ctx *c, uint32_t n, const ge_particles *p) {
-    if (!c || !p) return GE_E_INVALID;
-    if (n > c->cap) return fail(c, GE_E_CAPACITY, "particle count above capacity");
-    if (!p->x || !p->y || !p->mass) return fail(c, GE_E_INVALID, "x, y and mass are required");
+int ge_upload(ge_ctx *c, uint32_t n, const ge_particles *p) { return ge_upload_systems(c, 1, &n, p); }
+
+int ge_upload_systems(ge_ctx *c, uint32_t nsys, const uint32_t *counts, const ge_particles *p) {
+    if (!c || !p || !counts || nsys == 0) return GE_E_INVALID;
+    if (nsys > c->max_sys) return fail(c, GE_E_CAPACITY, "more systems than max_systems");
+    uint64_t total = 0;
+    for (uint32_t s = 0; s < nsys; ++s) {
+        if (nsys > 1 && counts[s] == 0) return fail(c, GE_E_INVALID, "every system needs a particle");
+        total += counts[s];
+    }
+    if (total > c->cap) return fail(c, GE_E_CAPACITY, "particle count above capacity");
+    const uint32_t n = (uint32_t)total;
+    if (n > 0 && (!p->x || !p->y || !p->mass)) return fail(c, GE_E_INVALID, "x, y and mass are required");
     (void)hipSetDevice(c->device);
+    if (int r = set_systems(c, nsys, counts)) {
+        return fail(c, r, r == GE_E_CAPACITY ? "systems too many or too large for one packed sort word" : "HIP error");
+    }
     c->b.cur = 0;
     State &s = c->b.st[0];
     c->n = n;
@@ -428,9 +496,20 @@ int ge_step(ge_ctx *c, const ge_step_params *p, int32_t nsteps) {
     return GE_OK;
 }
 
+int ge_system_offsets(const ge_ctx *c, uint32_t *nsys, uint32_t *offsets, uint32_t cap) {
+    if (!c || !nsys) return GE_E_INVALID;
+    *nsys = c->nsys;
+    if (offsets) {
+        if (cap < c->nsys + 1) return GE_E_CAPACITY;
+        for (uint32_t s = 0; s <= c->nsys; ++s) offsets[s] = c->host_off.empty() ? 0 : c->host_off[s];
+    }
+    return GE_OK;
+}
+
 int ge_bounding_box(ge_ctx *c, float out_bb[3]) {
     if (!c || !out_bb) return GE_E_INVALID;
     if (c->n == 0) return fail(c, GE_E_INVALID, "no particles");
+    if (c->nsys != 1) return fail(c, GE_E_INVALID, "stage calls take a single-system context");
     (void)hipSetDevice(c->device);
     launch_integrate_start_bbox(c, 0.0f, 0, 0.0f, 0);
     float h[4];
@@ -443,10 +522,11 @@ int ge_bounding_box(ge_ctx *c, float out_bb[3]) {
 int ge_compute_morton_keys(ge_ctx *c, const float bb[3]) {
     if (!c || !bb) return GE_E_INVALID;
     if (c->n == 0) return GE_OK;
+    if (c->nsys != 1) return fail(c, GE_E_INVALID, "stage calls take a single-system context");
     (void)hipSetDevice(c->device);
     float h[4] = {bb[0], bb[1], bb[2], 0.0f};
     GE_HIP_CHECK(c, hipMemcpyAsync(c->b.bb, h, sizeof h, hipMemcpyHostToDevice, c->stream));
-    launch_morton_keys(c, c->b.bb);
+    launch_morton_keys(c);
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }
@@ -455,7 +535,7 @@ int ge_sort_by_morton_key(ge_ctx *c) {
     if (!c) return GE_E_INVALID;
     if (c->n == 0) return GE_OK;
     (void)hipSetDevice(c->device);
-    radix_sort_packed(c, c->b.sort_a, c->b.sort_b, c->n, kIdxBits, kIdxBits + 36);
+    launch_sort_keys(c);
     launch_gather(c);
     c->have_tree = false;
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -465,10 +545,11 @@ int ge_sort_by_morton_key(ge_ctx *c) {
 int ge_build_quadtree(ge_ctx *c, const float bb[3]) {
     if (!c || !bb) return GE_E_INVALID;
     if (c->n == 0) return GE_OK;
+    if (c->nsys != 1) return fail(c, GE_E_INVALID, "stage calls take a single-system context");
     (void)hipSetDevice(c->device);
     float h[4] = {bb[0], bb[1], bb[2], 0.0f};
     GE_HIP_CHECK(c, hipMemcpyAsync(c->b.bb, h, sizeof h, hipMemcpyHostToDevice, c->stream));
-    launch_tree_build(c, c->b.bb);
+    launch_tree_build(c);
     c->have_tree = true;
     return ge_sync(c);
 }

@@ -14,6 +14,8 @@ namespace ge {
 
 constexpr int kBlock = 256;            // threads per workgroup for streaming kernels
 constexpr int kIdxBits = 28;           // low bits of a packed sort word hold the slot
+constexpr int kMortonBits = 36;        // 18 bits per axis (morton.cpp:3-25)
+constexpr uint32_t kChunk = 8192;      // particles per bounding-box reduction chunk
 constexpr uint64_t kIdxMask = (1ull << kIdxBits) - 1;
 constexpr uint32_t kMaxParticles = 1u << kIdxBits;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -111,6 +113,13 @@ struct Buffers {
     uint32_t *counters = nullptr;  // CounterSlot
     float *bb = nullptr;           // device bbox {x, y, size, _} and predPos box at +4
     uint64_t *interactions = nullptr;
+    // systems (a single scene is one system)
+    uint32_t *sys_off = nullptr;   // nsys + 1 particle offsets
+    uint32_t *sys_of = nullptr;    // system of each particle slot
+    float *sys_bb = nullptr;       // 8 per system: {x, y, size, 0, minx, miny, maxx, maxy}
+    uint32_t *chunk_lo = nullptr;  // reduction chunks: first slot of each chunk (+ end)
+    uint32_t *chunk_first = nullptr;  // first chunk of each system (nsys + 1)
+    uint32_t max_chunks = 0;
     TopNodes top;
     uint32_t *start_count = nullptr;  // per particle: nodes whose range starts there (then its prefix)
     uint32_t *start_level = nullptr;  // per particle: shallowest level of such a node
@@ -134,6 +143,12 @@ struct ge_ctx {
     hipStream_t stream = nullptr;
     uint32_t cap = 0;
     uint32_t n = 0;
+    uint32_t max_sys = 1;
+    uint32_t nsys = 1;
+    uint32_t nchunks = 1;
+    int idx_bits = 28;             // low bits of a packed sort word: slot within the system
+    int sys_bits = 0;              // high bits: system index
+    std::vector<uint32_t> host_off;  // system offsets (host copy)
     bool sph = false;
     bool debug = true;             // keep node geometry for ge_get_nodes
     bool have_tree = false;
@@ -155,7 +170,8 @@ void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n,
                        int lo_bit, int hi_bit);
 // step.hip
 void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate);
-void launch_morton_keys(ge_ctx *c, const float *bb_dev);
+void launch_morton_keys(ge_ctx *c);
+void launch_sort_keys(ge_ctx *c);
 void launch_gather(ge_ctx *c);
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_dt, bool count);
 void launch_quad_moments(ge_ctx *c, float theta);
@@ -163,7 +179,7 @@ void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
 void launch_zero_acc(ge_ctx *c);
 void launch_integrate_end(ge_ctx *c, float dt);
 // tree.hip
-void launch_tree_build(ge_ctx *c, const float *bb_dev);
+void launch_tree_build(ge_ctx *c);
 // sph.hip
 int sph_new_grid(ge_ctx *c, const ge_step_params *p);
 int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt);

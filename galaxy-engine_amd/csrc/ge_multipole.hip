@@ -16,16 +16,22 @@ namespace ge {
 namespace {
 
 constexpr int kMomThreads = 1024;
-constexpr int kMomItems = 4;
-constexpr uint32_t kMomTile = kMomThreads * kMomItems;
+constexpr int kMomItems = kChunk / kMomThreads;  // one workgroup per reduction chunk
 
 struct Mom {
     double v[6];
 };
 
-__device__ __forceinline__ Mom mom_of(const float4 *__restrict__ pxym, uint64_t i, uint64_t n, double ox, double oy) {
+// raw moments are taken about the centre of the particle's system box
+__device__ __forceinline__ void box_centre(const float *__restrict__ sys_bb, uint32_t sy, double &ox, double &oy) {
+    const float *bb = sys_bb + 8ull * sy;
+    ox = (double)bb[0] + 0.5 * (double)bb[2];
+    oy = (double)bb[1] + 0.5 * (double)bb[2];
+}
+
+__device__ __forceinline__ Mom mom_of(const float4 *__restrict__ pxym, double ox, double oy, uint32_t i, uint32_t hi) {
     Mom r;
-    if (i < n) {
+    if (i < hi) {
         const float4 p = pxym[i];
         const double m = p.z, x = (double)p.x - ox, y = (double)p.y - oy;
         r.v[0] = m; r.v[1] = m * x; r.v[2] = m * y; r.v[3] = m * x * x; r.v[4] = m * x * y; r.v[5] = m * y * y;
@@ -35,7 +41,8 @@ __device__ __forceinline__ Mom mom_of(const float4 *__restrict__ pxym, uint64_t 
     return r;
 }
 
-// exclusive block scan of a 6-vector; returns the block total in tot
+// Exclusive workgroup scan of a 6-vector with a fixed addition tree: the
+// prefix of an item depends only on the items before it.  tot = group total.
 __device__ Mom block_scan6(Mom v, Mom &tot) {
     __shared__ double sh[kMomThreads / 64 + 1][6];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -45,6 +52,11 @@ __device__ Mom block_scan6(Mom v, Mom &tot) {
             const double u = __shfl_up(incl.v[c], d, 64);
             if (lane >= (uint32_t)d) incl.v[c] += u;
         }
+    }
+    Mom excl;
+    for (int c = 0; c < 6; ++c) {
+        const double u = __shfl_up(incl.v[c], 1, 64);
+        excl.v[c] = lane == 0 ? 0.0 : u;
     }
     if (lane == 63)
         for (int c = 0; c < 6; ++c) sh[w][c] = incl.v[c];
@@ -62,72 +74,83 @@ __device__ Mom block_scan6(Mom v, Mom &tot) {
     __syncthreads();
     Mom ex;
     for (int c = 0; c < 6; ++c) {
-        ex.v[c] = sh[w][c] + incl.v[c] - v.v[c];
+        ex.v[c] = w == 0 ? excl.v[c] : sh[w][c] + excl.v[c];
         tot.v[c] = sh[kMomThreads / 64][c];
     }
     __syncthreads();
     return ex;
 }
 
-__device__ __forceinline__ void box_centre(const float *__restrict__ bb, double &ox, double &oy) {
-    ox = (double)bb[0] + 0.5 * (double)bb[2];
-    oy = (double)bb[1] + 0.5 * (double)bb[2];
-}
-
-__global__ __launch_bounds__(kMomThreads) void k_mom_sums(const float4 *__restrict__ pxym, uint32_t n,
-                                                          const float *__restrict__ bb, double *__restrict__ part) {
-    double ox, oy;
-    box_centre(bb, ox, oy);
-    const uint64_t base = (uint64_t)blockIdx.x * kMomTile + (uint64_t)threadIdx.x * kMomItems;
-    Mom s;
-    for (int c = 0; c < 6; ++c) s.v[c] = 0.0;
-    for (int k = 0; k < kMomItems; ++k) {
-        const Mom m = mom_of(pxym, base + k, n, ox, oy);
-        for (int c = 0; c < 6; ++c) s.v[c] += m.v[c];
-    }
+// Per chunk (chunks never span systems and start at the system's first
+// particle + multiples of kChunk): thread t owns items lo + t*kMomItems ...
+__global__ __launch_bounds__(kMomThreads) void k_mom_sums(const float4 *__restrict__ pxym,
+                                                          const uint32_t *__restrict__ chunk_lo,
+                                                          const uint32_t *__restrict__ sys_of,
+                                                          const float *__restrict__ sys_bb, double *__restrict__ part) {
+    const uint32_t lo = chunk_lo[blockIdx.x], hi = chunk_lo[blockIdx.x + 1];
     Mom tot;
-    block_scan6(s, tot);
+    for (int c = 0; c < 6; ++c) tot.v[c] = 0.0;
+    if (lo < hi) {
+        double ox, oy;
+        box_centre(sys_bb, sys_of[lo], ox, oy);
+        const uint32_t base = lo + threadIdx.x * kMomItems;
+        Mom s;
+        for (int c = 0; c < 6; ++c) s.v[c] = 0.0;
+        for (int k = 0; k < kMomItems; ++k) {
+            const Mom m = mom_of(pxym, ox, oy, base + k, hi);
+            for (int c = 0; c < 6; ++c) s.v[c] += m.v[c];
+        }
+        block_scan6(s, tot);
+    }
     if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c) part[6 * blockIdx.x + c] = tot.v[c];
 }
 
-__global__ __launch_bounds__(kMomThreads) void k_mom_parts(double *__restrict__ part, uint32_t nparts) {
-    Mom carry;
-    for (int c = 0; c < 6; ++c) carry.v[c] = 0.0;
-    for (uint32_t b0 = 0; b0 < nparts; b0 += kMomThreads) {
-        const uint32_t i = b0 + threadIdx.x;
-        Mom v;
-        for (int c = 0; c < 6; ++c) v.v[c] = i < nparts ? part[6 * i + c] : 0.0;
-        Mom tot;
-        const Mom ex = block_scan6(v, tot);
-        if (i < nparts)
-            for (int c = 0; c < 6; ++c) part[6 * i + c] = carry.v[c] + ex.v[c];
-        for (int c = 0; c < 6; ++c) carry.v[c] += tot.v[c];
+// chunk offsets within each system, chunks in order (one thread per system)
+__global__ __launch_bounds__(kBlock) void k_mom_parts(double *__restrict__ part, const uint32_t *__restrict__ chunk_first,
+                                                      uint32_t nsys) {
+    for (uint32_t sy = blockIdx.x * kBlock + threadIdx.x; sy < nsys; sy += gridDim.x * kBlock) {
+        double run[6] = {0, 0, 0, 0, 0, 0};
+        for (uint32_t k = chunk_first[sy]; k < chunk_first[sy + 1]; ++k) {
+            for (int c = 0; c < 6; ++c) {
+                const double t = part[6 * k + c];
+                part[6 * k + c] = run[c];
+                run[c] += t;
+            }
+        }
     }
 }
 
-// moments[6 i + c] = sum over particles j < i (i = 0 .. n)
-__global__ __launch_bounds__(kMomThreads) void k_mom_apply(const float4 *__restrict__ pxym, uint32_t n,
-                                                           const float *__restrict__ bb,
+// mom[6 (i + s) + c] = sum over the particles of system s before slot i,
+// for i = off_s .. off_{s+1} (one extra entry per system)
+__global__ __launch_bounds__(kMomThreads) void k_mom_apply(const float4 *__restrict__ pxym,
+                                                           const uint32_t *__restrict__ chunk_lo,
+                                                           const uint32_t *__restrict__ sys_of,
+                                                           const uint32_t *__restrict__ sys_off,
+                                                           const float *__restrict__ sys_bb,
                                                            const double *__restrict__ part,
                                                            double *__restrict__ mom) {
+    const uint32_t lo = chunk_lo[blockIdx.x], hi = chunk_lo[blockIdx.x + 1];
+    if (lo >= hi) return;
+    const uint32_t sy = sys_of[lo];
     double ox, oy;
-    box_centre(bb, ox, oy);
-    const uint64_t base = (uint64_t)blockIdx.x * kMomTile + (uint64_t)threadIdx.x * kMomItems;
+    box_centre(sys_bb, sy, ox, oy);
+    const uint32_t base = lo + threadIdx.x * kMomItems;
     Mom item[kMomItems];
     Mom s;
     for (int c = 0; c < 6; ++c) s.v[c] = 0.0;
     for (int k = 0; k < kMomItems; ++k) {
-        item[k] = mom_of(pxym, base + k, n, ox, oy);
+        item[k] = mom_of(pxym, ox, oy, base + k, hi);
         for (int c = 0; c < 6; ++c) s.v[c] += item[k].v[c];
     }
     Mom tot;
     Mom run = block_scan6(s, tot);
     for (int c = 0; c < 6; ++c) run.v[c] += part[6 * blockIdx.x + c];
+    const bool last_chunk = hi == sys_off[sy + 1];
     for (int k = 0; k < kMomItems; ++k) {
-        const uint64_t i = base + k;
-        if (i <= n)
-            for (int c = 0; c < 6; ++c) mom[6 * i + c] = run.v[c];
+        const uint32_t i = base + k;
+        if (i < hi || (last_chunk && i == hi))
+            for (int c = 0; c < 6; ++c) mom[6ull * (i + sy) + c] = run.v[c];
         for (int c = 0; c < 6; ++c) run.v[c] += item[k].v[c];
     }
 }
@@ -135,16 +158,19 @@ __global__ __launch_bounds__(kMomThreads) void k_mom_apply(const float4 *__restr
 __global__ __launch_bounds__(kBlock) void k_node_quad(const NodeRec *__restrict__ nodes,
                                                       const float4 *__restrict__ geom,
                                                       const uint32_t *__restrict__ counters,
-                                                      const double *__restrict__ mom, const float *__restrict__ bb,
+                                                      const double *__restrict__ mom,
+                                                      const uint32_t *__restrict__ sys_of,
+                                                      const float *__restrict__ sys_bb,
                                                       float inv_theta, float4 *__restrict__ quad) {
     const uint32_t nn = counters[kCntNodeTotal];
-    double ox, oy;
-    box_centre(bb, ox, oy);
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < nn; k += gridDim.x * kBlock) {
         const NodeRec r = nodes[k];
+        const uint32_t sy = sys_of[r.start];
+        double ox, oy;
+        box_centre(sys_bb, sy, ox, oy);
         const float4 g = geom[k];
-        const double *a = mom + 6ull * r.start;
-        const double *b = mom + 6ull * r.end;
+        const double *a = mom + 6ull * (r.start + sy);
+        const double *b = mom + 6ull * (r.end + sy);
         const double M = b[0] - a[0], Sx = b[1] - a[1], Sy = b[2] - a[2];
         const double Sxx = b[3] - a[3], Sxy = b[4] - a[4], Syy = b[5] - a[5];
         const double cx = (double)r.comx - ox, cy = (double)r.comy - oy;
@@ -161,18 +187,16 @@ __global__ __launch_bounds__(kBlock) void k_node_quad(const NodeRec *__restrict_
 }  // namespace
 
 void launch_quad_moments(ge_ctx *c, float theta) {
-    const uint32_t n = c->n;
-    const uint32_t len = n + 1;
-    const uint32_t parts = (len + kMomTile - 1) / kMomTile;
-    hipLaunchKernelGGL(k_mom_sums, dim3(parts), dim3(kMomThreads), 0, c->stream, c->b.pxym, n, c->b.bb,
-                       c->b.moment_part);
-    hipLaunchKernelGGL(k_mom_parts, dim3(1), dim3(kMomThreads), 0, c->stream, c->b.moment_part, parts);
-    hipLaunchKernelGGL(k_mom_apply, dim3(parts), dim3(kMomThreads), 0, c->stream, c->b.pxym, n, c->b.bb,
-                       c->b.moment_part, c->b.moments);
+    hipLaunchKernelGGL(k_mom_sums, dim3(c->nchunks), dim3(kMomThreads), 0, c->stream, c->b.pxym, c->b.chunk_lo,
+                       c->b.sys_of, c->b.sys_bb, c->b.moment_part);
+    hipLaunchKernelGGL(k_mom_parts, dim3(blocks_for(c->nsys, kBlock)), dim3(kBlock), 0, c->stream, c->b.moment_part,
+                       c->b.chunk_first, c->nsys);
+    hipLaunchKernelGGL(k_mom_apply, dim3(c->nchunks), dim3(kMomThreads), 0, c->stream, c->b.pxym, c->b.chunk_lo,
+                       c->b.sys_of, c->b.sys_off, c->b.sys_bb, c->b.moment_part, c->b.moments);
     uint32_t nb = blocks_for(c->b.node_cap, kBlock);
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(k_node_quad, dim3(nb), dim3(kBlock), 0, c->stream, c->b.nodes, c->b.node_geom, c->b.counters,
-                       c->b.moments, c->b.bb, 1.0f / theta, c->b.node_quad);
+                       c->b.moments, c->b.sys_of, c->b.sys_bb, 1.0f / theta, c->b.node_quad);
 }
 
 }  // namespace ge

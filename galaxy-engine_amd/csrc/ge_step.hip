@@ -42,14 +42,16 @@ __device__ __forceinline__ void block_minmax(float &a, float &b, float &c, float
 // boundingBox() reduction (globalLogic.cpp:1083-1099) over the drifted
 // positions: per-workgroup partial min/max here, the final box in
 // k_bbox_final (a second launch instead of a device-scope fence).
-__global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, uint32_t n, float dt, int sph_clamp,
-                                                           float max_vel, int do_integrate,
+__global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, const uint32_t *__restrict__ chunk_lo, float dt,
+                                                           int sph_clamp, float max_vel, int do_integrate,
                                                            float *__restrict__ partials) {
 #pragma clang fp contract(off)
     const float half_dt = dt * 0.5f;
     const float max_vsq = max_vel * max_vel;
     float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    // one workgroup per chunk; a chunk never spans two systems
+    const uint32_t lo = chunk_lo[blockIdx.x], hi = chunk_lo[blockIdx.x + 1];
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         float x = s.x[i], y = s.y[i];
         if (do_integrate && !(s.flags[i] & GE_FLAG_PINNED)) {
             float pvx = s.vx[i], pvy = s.vy[i];
@@ -82,11 +84,15 @@ __global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, uint32_t n, 
     if (threadIdx.x == 0) reinterpret_cast<float4 *>(partials)[blockIdx.x] = make_float4(mnx, mny, mxx, mxy);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bbox_final(const float *__restrict__ partials, uint32_t nparts,
-                                                       float *__restrict__ bb) {
+// one workgroup per system: its chunks' partials -> its box
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const float *__restrict__ partials,
+                                                       const uint32_t *__restrict__ chunk_first,
+                                                       float *__restrict__ sys_bb) {
 #pragma clang fp contract(off)
+    float *bb = sys_bb + 8ull * blockIdx.x;
     float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
-    for (uint32_t k = threadIdx.x; k < nparts; k += kBlock) {
+    const uint32_t k0 = chunk_first[blockIdx.x], k1 = chunk_first[blockIdx.x + 1];
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
         const float4 v = reinterpret_cast<const float4 *>(partials)[k];
         mnx = glm_min(mnx, v.x); mny = glm_min(mny, v.y);
         mxx = glm_max(mxx, v.z); mxy = glm_max(mxy, v.w);
@@ -127,14 +133,25 @@ __device__ __forceinline__ uint64_t spread_bits(uint64_t v) {
     return v;
 }
 
+// packed sort word: system | key | slot within the system (the system part
+// is empty for a single scene), so one LSD sort orders each system by key
+struct PackFmt {
+    int idx_bits;
+    int sys_shift;  // 0 when there is one system
+};
+
 __global__ __launch_bounds__(kBlock) void k_morton_keys(const float *__restrict__ x,
                                                         const float *__restrict__ y, uint32_t n,
-                                                        const float *__restrict__ bb,
+                                                        const uint32_t *__restrict__ sys_of,
+                                                        const uint32_t *__restrict__ sys_off,
+                                                        const float *__restrict__ sys_bb, PackFmt fmt,
                                                         uint64_t *__restrict__ keys,
                                                         uint64_t *__restrict__ packed) {
 #pragma clang fp contract(off)
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
+    const uint32_t sy = sys_of[i];
+    const float *bb = sys_bb + 8ull * sy;
     const float bx = bb[0], by = bb[1], bs = bb[2];
     const float ext = bs < 1e-6f ? 1e-6f : bs;
     const float maxx = bx + ext, maxy = by + ext;
@@ -142,18 +159,22 @@ __global__ __launch_bounds__(kBlock) void k_morton_keys(const float *__restrict_
     uint64_t iy = scale_to_grid(y[i], by, maxy);
     uint64_t k = spread_bits(ix) | (spread_bits(iy) << 1);
     keys[i] = k;
-    packed[i] = (k << kIdxBits) | i;
+    const uint64_t local = i - sys_off[sy];
+    packed[i] = (fmt.sys_shift ? ((uint64_t)sy << fmt.sys_shift) : 0ull) | (k << fmt.idx_bits) | local;
 }
 
 // Morton::sortParticlesByMortonKey's gather (morton.cpp:56-62) for every
 // SoA field, plus the packed (x, y, m) stream the traversal reads.
 __global__ __launch_bounds__(kBlock) void k_gather(State src, State dst, uint32_t n, int sph,
                                                    const uint64_t *__restrict__ sorted,
+                                                   const uint32_t *__restrict__ sys_off, PackFmt fmt,
                                                    uint32_t *__restrict__ perm,
                                                    float4 *__restrict__ pxym) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const uint32_t j = (uint32_t)(sorted[i] & kIdxMask);
+    const uint64_t w = sorted[i];
+    const uint32_t sy = fmt.sys_shift ? (uint32_t)(w >> fmt.sys_shift) : 0u;
+    const uint32_t j = sys_off[sy] + (uint32_t)(w & ((1ull << fmt.idx_bits) - 1ull));
     perm[i] = j;
     float x = src.x[j], y = src.y[j], m = src.m[j];
     dst.x[i] = x; dst.y[i] = y; dst.m[i] = m;
@@ -192,10 +213,18 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // when |d| exceeds its opening radius (size/theta + COM offset) and adds its
 // quadrupole term  -G Q d / R^5 + 5/2 G (d^T Q d - eps^2 tr Q) d / R^7,
 // R^2 = |d|^2 + eps^2, to the softened monopole.
+//
+// Several systems: system s owns the node range [P[off_s], P[off_s+1]) (P is
+// the node-start prefix of the tree build); each lane is active only inside
+// its own system's range, and the wave walks from its lanes' first root to
+// their last end.
 template <bool KAHAN, bool COUNT, bool QUAD>
 __global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
                                                const NodeRec *__restrict__ nodes,
                                                const float4 *__restrict__ quad,
+                                               const uint32_t *__restrict__ start_prefix,
+                                               const uint32_t *__restrict__ sys_of,
+                                               const uint32_t *__restrict__ sys_off,
                                                const uint32_t *__restrict__ counters, uint32_t n,
                                                float G, float theta_sq, float soft_sq,
                                                float *__restrict__ ax, float *__restrict__ ay,
@@ -209,11 +238,24 @@ __global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
     // a tree that raised an error flag may hold unwritten nodes: do not walk it
     const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth);
     const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
-    uint32_t skip_until = live ? 0u : kNone;
+    uint32_t root = kNone, end = 0u;
+    if (live) {
+        const uint32_t sy = sys_of[i];
+        root = start_prefix[sys_off[sy]];
+        end = min(start_prefix[sys_off[sy + 1]], node_count);
+    }
+    uint32_t wlo = root, whi = end;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        wlo = min(wlo, (uint32_t)__shfl_xor((int)wlo, o, kWave));
+        whi = max(whi, (uint32_t)__shfl_xor((int)whi, o, kWave));
+    }
+    const uint32_t walk_end = __builtin_amdgcn_readfirstlane(whi);
+    uint32_t skip_until = root;
     float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
     uint32_t cnt = 0;
-    uint32_t idx = 0;
-    while (idx < node_count) {
+    uint32_t idx = __builtin_amdgcn_readfirstlane(wlo);
+    while (idx < walk_end) {
         const NodeRec nd = nodes[idx];
         const uint32_t next = nd.next;
         // index after this subtree, saturated so the walk always moves forward
@@ -222,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
             idx = after;
             continue;
         }
-        const bool active = idx >= skip_until;
+        const bool active = idx >= skip_until && idx < end;
         float dx, dy, dist_sq;
         bool accept;
         float4 qd;
@@ -369,23 +411,31 @@ __global__ __launch_bounds__(kBlock) void k_integrate_end(State s, uint32_t n, f
 }  // namespace
 
 void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate) {
-    uint32_t nb = blocks_for(c->n, kBlock);
-    if (nb > kBboxMaxBlocks) nb = kBboxMaxBlocks;
-    hipLaunchKernelGGL(k_integrate_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->b.st[c->b.cur], c->n, dt,
-                       sph_clamp, max_vel, do_integrate, c->b.partials);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, c->stream, c->b.partials, nb, c->b.bb);
+    hipLaunchKernelGGL(k_integrate_bbox, dim3(c->nchunks), dim3(kBlock), 0, c->stream, c->b.st[c->b.cur],
+                       c->b.chunk_lo, dt, sph_clamp, max_vel, do_integrate, c->b.partials);
+    hipLaunchKernelGGL(k_bbox_final, dim3(c->nsys), dim3(kBlock), 0, c->stream, c->b.partials, c->b.chunk_first,
+                       c->b.sys_bb);
 }
 
-void launch_morton_keys(ge_ctx *c, const float *bb_dev) {
+static PackFmt pack_fmt(const ge_ctx *c) {
+    return PackFmt{c->idx_bits, c->sys_bits ? c->idx_bits + kMortonBits : 0};
+}
+
+void launch_morton_keys(ge_ctx *c) {
     const State &s = c->b.st[c->b.cur];
-    hipLaunchKernelGGL(k_morton_keys, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y,
-                       c->n, bb_dev, c->b.keys, c->b.sort_a);
+    hipLaunchKernelGGL(k_morton_keys, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, c->n,
+                       c->b.sys_of, c->b.sys_off, c->b.sys_bb, pack_fmt(c), c->b.keys, c->b.sort_a);
+}
+
+void launch_sort_keys(ge_ctx *c) {
+    radix_sort_packed(c, c->b.sort_a, c->b.sort_b, c->n, c->idx_bits, c->idx_bits + kMortonBits + c->sys_bits);
 }
 
 void launch_gather(ge_ctx *c) {
     const int cur = c->b.cur;
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.st[cur],
-                       c->b.st[1 - cur], c->n, (int)c->sph, c->b.sort_a, c->b.perm, c->b.pxym);
+                       c->b.st[1 - cur], c->n, (int)c->sph, c->b.sort_a, c->b.sys_off, pack_fmt(c), c->b.perm,
+                       c->b.pxym);
     c->b.cur = 1 - cur;
 }
 
@@ -397,8 +447,8 @@ void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const uint32_t nb = blocks_for(c->n, kBlock);
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
     hipLaunchKernelGGL((k_bh<KAHAN, COUNT, QUAD>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
-                       c->b.node_quad, c->b.counters, c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
-                       end_kick_half_dt, inter);
+                       c->b.node_quad, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n, p->G,
+                       theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, inter);
 }
 
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {

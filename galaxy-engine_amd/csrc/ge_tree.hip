@@ -422,12 +422,19 @@ __device__ __forceinline__ void push_block_root(TopNodes &T, uint32_t *counters,
     else atomicOr(&counters[kCntError], kErrFrontier);
 }
 
-__global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTable st, const float *__restrict__ bb,
+__global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTable st, const float *__restrict__ sys_bb,
+                                                           const uint32_t *__restrict__ sys_off, uint32_t nsys,
                                                            uint32_t n, uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * kTreeBlock + threadIdx.x;
     if (i <= n) {
         st.count[i] = 0;
         st.level[i] = 0xFFFFFFFFu;
+    }
+    if (i < nsys) {  // the root of system i: Quadtree::root (quadtree.cpp:90-107)
+        const float *bb = sys_bb + 8ull * i;
+        T.px[i] = bb[0]; T.py[i] = bb[1]; T.size[i] = bb[2];
+        T.start[i] = sys_off[i]; T.end[i] = sys_off[i + 1];
+        T.parent[i] = kNone; T.kind[i] = kKindPending; T.depth[i] = 0;
     }
     if (blockIdx.x == 0) {
         const uint32_t t = threadIdx.x;
@@ -436,12 +443,10 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTabl
             counters[kCntError] &= ~(kErrNodeCap | kErrDepth | kErrFrontier);
             counters[kCntBlockRoots] = 0;
             counters[kCntTopAlloc] = 0;
-            T.px[0] = bb[0]; T.py[0] = bb[1]; T.size[0] = bb[2];
-            T.start[0] = 0; T.end[0] = n; T.parent[0] = kNone; T.kind[0] = kKindPending; T.depth[0] = 0;
         }
         __syncthreads();
         if (t == 0) {
-            counters[kCntLevel0 + 0] = 1;
+            counters[kCntLevel0 + 0] = nsys;
             counters[kCntLevelBase0 + 0] = 0;
         }
     }
@@ -747,17 +752,17 @@ uint32_t wave_grid(uint64_t waves) {
 
 }  // namespace
 
-void launch_tree_build(ge_ctx *c, const float *bb_dev) {
+void launch_tree_build(ge_ctx *c) {
     const State &s = c->b.st[c->b.cur];
     TopNodes &T = c->b.top;
     const uint64_t n = c->n;
     const StartTable st{c->b.start_count, c->b.start_level};
-    hipLaunchKernelGGL(k_tree_clear, dim3(blocks_for(n + 1, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, T, st, bb_dev,
-                       c->n, c->b.counters);
+    hipLaunchKernelGGL(k_tree_clear, dim3(blocks_for(n + 1, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, T, st,
+                       c->b.sys_bb, c->b.sys_off, c->nsys, c->n, c->b.counters);
     // level L+1 only holds children of nodes above kBlockSubMax particles
     const uint64_t big_children = 4 * (n / (kBlockSubMax + 1)) + 4;
     for (int L = 0; L < kTopLevels; ++L) {
-        uint64_t maxn = 1;
+        uint64_t maxn = c->nsys;
         for (int k = 0; k < L && maxn < n; ++k) maxn *= 4;
         if (L > 0 && maxn > big_children) maxn = big_children;
         if (maxn > n) maxn = n;
@@ -765,7 +770,7 @@ void launch_tree_build(ge_ctx *c, const float *bb_dev) {
                            c->b.counters, L, kTopLevels - 1);
     }
     // roots of workgroup subtrees each hold more than 64 particles
-    uint64_t roots = n / (kWaveSubMax + 1) + 1;
+    uint64_t roots = n / (kWaveSubMax + 1) + c->nsys;
     if (roots > 4096) roots = 4096;
     hipLaunchKernelGGL(k_tree_blocks, dim3((uint32_t)roots), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y,
                        c->b.counters);

@@ -26,6 +26,7 @@ EXPORTS = [
     "ge_compute_gravity", "ge_integrate_start", "ge_integrate_end", "ge_sph_new_grid",
     "ge_sph_pcisph", "ge_get_keys", "ge_get_perm", "ge_get_node_count", "ge_get_nodes",
     "ge_get_interactions", "ge_get_neighbors", "ge_profile_enable", "ge_profile_read",
+    "ge_upload_systems", "ge_system_offsets",
 ]
 PHASES = ["integrate_bbox", "keys", "sort", "gather", "tree", "gravity", "sph_grid", "sph",
           "integrate_end"]
@@ -41,7 +42,7 @@ PARTICLE_ORDER = ["x", "y", "vx", "vy", "ax", "ay", "pvx", "pvy", "mass", "flags
 
 class GeConfig(C.Structure):
     _fields_ = [("capacity", C.c_uint32), ("device", C.c_int32), ("enable_sph", C.c_int32),
-                ("reserved", C.c_uint32 * 5)]
+                ("max_systems", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class GeParticles(C.Structure):
@@ -130,6 +131,8 @@ def load_library():
                                        P(C.c_int64)]),
         "ge_profile_enable": (C.c_int, [vp, C.c_int]),
         "ge_profile_read": (C.c_int, [vp, P(C.c_double), P(C.c_uint32)]),
+        "ge_upload_systems": (C.c_int, [vp, C.c_uint32, P(C.c_uint32), P(GeParticles)]),
+        "ge_system_offsets": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32), C.c_uint32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -174,13 +177,14 @@ def _uptr(a):
 class Context:
     """One device context (ge_ctx)."""
 
-    def __init__(self, capacity, sph=False, device=0):
+    def __init__(self, capacity, sph=False, device=0, max_systems=1):
         self.lib = load_library()
         cfg = GeConfig()
         self.lib.ge_default_config(C.byref(cfg))
         cfg.capacity = int(capacity)
         cfg.device = int(device)
         cfg.enable_sph = 1 if sph else 0
+        cfg.max_systems = int(max_systems)
         h = C.c_void_p()
         rc = self.lib.ge_create(C.byref(h), C.byref(cfg))
         if rc != GE_OK:
@@ -211,8 +215,9 @@ class Context:
             raise GeError(rc, self.lib.ge_last_error(self.h).decode())
 
     # ---- state transfer
-    def upload(self, state):
-        """state: dict of numpy arrays (x, y, mass required)."""
+    def upload(self, state, counts=None):
+        """state: dict of numpy arrays (x, y, mass required).  counts: particles
+        per system when the arrays hold several systems back to back."""
         n = len(state["x"])
         keep = []
         gp = GeParticles()
@@ -223,7 +228,19 @@ class Context:
                 assert len(a) == n, f
                 keep.append(a)
                 setattr(gp, f, _uptr(a) if f in FIELDS_U else _fptr(a))
-        self._check(self.lib.ge_upload(self.h, n, C.byref(gp)))
+        if counts is None:
+            self._check(self.lib.ge_upload(self.h, n, C.byref(gp)))
+        else:
+            cnt = np.ascontiguousarray(counts, dtype=np.uint32)
+            assert int(cnt.sum()) == n
+            self._check(self.lib.ge_upload_systems(self.h, len(cnt), _uptr(cnt), C.byref(gp)))
+
+    def system_offsets(self):
+        ns = C.c_uint32()
+        self._check(self.lib.ge_system_offsets(self.h, C.byref(ns), None, 0))
+        off = np.zeros(ns.value + 1, np.uint32)
+        self._check(self.lib.ge_system_offsets(self.h, C.byref(ns), _uptr(off), len(off)))
+        return off
 
     def count(self):
         n = C.c_uint32()

@@ -46,10 +46,11 @@ extern "C" {
 typedef struct ge_ctx ge_ctx;
 
 typedef struct {
-    uint32_t capacity;  /* maximum particle count */
-    int32_t device;     /* HIP device ordinal */
-    int32_t enable_sph; /* allocate SPH state (needed for ge_step with sph=1) */
-    uint32_t reserved[5];
+    uint32_t capacity;    /* maximum particle count (all systems together) */
+    int32_t device;       /* HIP device ordinal */
+    int32_t enable_sph;   /* allocate SPH state (needed for ge_step with sph=1) */
+    uint32_t max_systems; /* independent systems one context can hold (>= 1) */
+    uint32_t reserved[4];
 } ge_config;
 
 /* Structure-of-arrays view of the ParticlePhysics fields on this path
@@ -123,6 +124,16 @@ int ge_upload(ge_ctx *ctx, uint32_t n, const ge_particles *p);
 /* replaces reading myParam.pParticles back; order is the device order */
 int ge_download(ge_ctx *ctx, const ge_particles *p);
 int ge_count(const ge_ctx *ctx, uint32_t *n);
+/* Batched form (no counterpart in the reference, which steps one scene):
+ * nsys independent systems, particles concatenated system by system
+ * (counts[s] >= 1 each).  Every stage then runs once for all systems; each
+ * system keeps its own bounding box, Morton order and quadtree, and its
+ * particles never interact with another system's.  A system's results are
+ * bit-identical to stepping it alone.  SPH and the direct-sum branch are
+ * single-system only. */
+int ge_upload_systems(ge_ctx *ctx, uint32_t nsys, const uint32_t *counts, const ge_particles *p);
+/* nsys + 1 offsets of the systems in the device order */
+int ge_system_offsets(const ge_ctx *ctx, uint32_t *nsys, uint32_t *offsets, uint32_t cap);
 int ge_sync(ge_ctx *ctx);
 
 /* ---- one frame ---------------------------------------------------------- */
