@@ -41,7 +41,23 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bound on the CPU-baseline sample (0 disables it)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a plain elementwise copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A copy kernel of known traffic for scaling FETCH_SIZE / WRITE_SIZE:
+    out = -in over `mib` MiB of float32 reads exactly mib MiB and writes mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "elementwise copy", "bytes_read": n * 4, "bytes_written": n * 4,
+                      "launches": 5}), flush=True)
 
 
 def traffic_from_profiles():
@@ -76,6 +92,9 @@ def cpu_baseline(st, prm, budget_s):
 
 def main():
     a = parse()
+    if a.calibrate_copy_mib:
+        calibrate_copy(a.calibrate_copy_mib)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
               "(independent systems shard with no communication; see DESIGN.md)", file=sys.stderr)
