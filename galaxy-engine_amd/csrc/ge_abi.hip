@@ -57,7 +57,7 @@ void free_all(ge_ctx *c) {
     free_state(b.st[1]);
     dfree(b.pxym); dfree(b.sort_a); dfree(b.sort_b); dfree(b.keys); dfree(b.perm); dfree(b.hist);
     dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.interactions);
-    dfree(b.sys_off); dfree(b.sys_of); dfree(b.sys_bb); dfree(b.chunk_lo); dfree(b.chunk_first);
+    dfree(b.sys_off); dfree(b.sys_of); dfree(b.sys_bb); dfree(b.chunk_lo); dfree(b.chunk_first); dfree(b.sys_src);
     b.bb = nullptr;
     TopNodes &T = b.top;
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
@@ -92,6 +92,7 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.sys_bb, 8ull * c->max_sys));
     GE_HIP_CHECK(c, dalloc(b.chunk_lo, (size_t)b.max_chunks + 1));
     GE_HIP_CHECK(c, dalloc(b.chunk_first, (size_t)c->max_sys + 1));
+    GE_HIP_CHECK(c, dalloc(b.sys_src, (size_t)c->max_sys + 1));
     b.bb = b.sys_bb;  // system 0's box doubles as the single-scene box
     GE_HIP_CHECK(c, dalloc(b.interactions, 1));
     TopNodes &T = b.top;
@@ -149,7 +150,8 @@ int bits_for(uint64_t v) {  // bits to hold values 0 .. v-1
     return b;
 }
 
-// System offsets, the per-slot system table and the reduction chunks.
+// System offsets, the per-slot system table and the reduction chunks.  Sets
+// c->n; waits for the context's stream first (earlier frames read the tables).
 int set_systems(ge_ctx *c, uint32_t nsys, const uint32_t *counts) {
     std::vector<uint32_t> off(nsys + 1, 0);
     uint32_t maxc = 1;
@@ -158,32 +160,109 @@ int set_systems(ge_ctx *c, uint32_t nsys, const uint32_t *counts) {
         if (counts[s] > maxc) maxc = counts[s];
     }
     const uint32_t n = off[nsys];
-    if (nsys == 1) {
-        c->idx_bits = kIdxBits;
-        c->sys_bits = 0;
-    } else {
-        c->idx_bits = bits_for(maxc);
-        c->sys_bits = bits_for(nsys);
-        if (c->idx_bits + c->sys_bits > kIdxBits) return GE_E_CAPACITY;
+    int idx_bits = kIdxBits, sys_bits = 0;
+    if (nsys > 1) {
+        idx_bits = bits_for(maxc);
+        sys_bits = bits_for(nsys);
+        if (idx_bits + sys_bits > kIdxBits) return GE_E_CAPACITY;
     }
-    std::vector<uint32_t> sys_of(n > 0 ? n : 1);
     std::vector<uint32_t> chunk_lo, chunk_first(nsys + 1);
     for (uint32_t s = 0; s < nsys; ++s) {
         chunk_first[s] = (uint32_t)chunk_lo.size();
-        for (uint32_t i = off[s]; i < off[s + 1]; ++i) sys_of[i] = s;
         for (uint32_t a = off[s]; a < off[s + 1]; a += kChunk) chunk_lo.push_back(a);
         if (counts[s] == 0) chunk_lo.push_back(off[s]);
     }
     chunk_first[nsys] = (uint32_t)chunk_lo.size();
     if (chunk_lo.size() > c->b.max_chunks) return GE_E_CAPACITY;
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    c->idx_bits = idx_bits;
+    c->sys_bits = sys_bits;
     c->nchunks = (uint32_t)chunk_lo.size();
     chunk_lo.push_back(n);
     c->nsys = nsys;
+    c->n = n;
     c->host_off = off;
     GE_HIP_CHECK(c, hipMemcpy(c->b.sys_off, off.data(), sizeof(uint32_t) * (nsys + 1), hipMemcpyHostToDevice));
-    GE_HIP_CHECK(c, hipMemcpy(c->b.sys_of, sys_of.data(), sizeof(uint32_t) * sys_of.size(), hipMemcpyHostToDevice));
     GE_HIP_CHECK(c, hipMemcpy(c->b.chunk_lo, chunk_lo.data(), sizeof(uint32_t) * chunk_lo.size(), hipMemcpyHostToDevice));
     GE_HIP_CHECK(c, hipMemcpy(c->b.chunk_first, chunk_first.data(), sizeof(uint32_t) * (nsys + 1), hipMemcpyHostToDevice));
+    if (n > 0) launch_sys_of(c);
+    return GE_OK;
+}
+
+// Writes host arrays p[0 .. cnt) into slots [at, at + cnt) of state s.  NULL
+// arrays give zero (sph_mass: 1; id: at-relative index 0 .. cnt-1).
+int put_fields(ge_ctx *c, State &s, uint32_t at, uint32_t cnt, const ge_particles *p) {
+    if (cnt == 0) return GE_OK;
+    const size_t fb = sizeof(float) * cnt;
+    auto put = [&](float *dst, const float *src, float fill) -> int {
+        if (!dst) return GE_OK;
+        if (src) {
+            GE_HIP_CHECK(c, hipMemcpyAsync(dst + at, src, fb, hipMemcpyHostToDevice, c->stream));
+        } else if (fill == 0.0f) {
+            GE_HIP_CHECK(c, hipMemsetAsync(dst + at, 0, fb, c->stream));
+        } else {
+            std::vector<float> tmp(cnt, fill);
+            GE_HIP_CHECK(c, hipMemcpyAsync(dst + at, tmp.data(), fb, hipMemcpyHostToDevice, c->stream));
+            GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        }
+        return GE_OK;
+    };
+    int r = 0;
+    if ((r = put(s.x, p->x, 0.f)) || (r = put(s.y, p->y, 0.f)) || (r = put(s.vx, p->vx, 0.f)) ||
+        (r = put(s.vy, p->vy, 0.f)) || (r = put(s.ax, p->ax, 0.f)) || (r = put(s.ay, p->ay, 0.f)) ||
+        (r = put(s.pvx, p->pvx, 0.f)) || (r = put(s.pvy, p->pvy, 0.f)) || (r = put(s.m, p->mass, 0.f)))
+        return r;
+    if (p->flags) {
+        GE_HIP_CHECK(c, hipMemcpyAsync(s.flags + at, p->flags, sizeof(uint32_t) * cnt, hipMemcpyHostToDevice, c->stream));
+    } else {
+        GE_HIP_CHECK(c, hipMemsetAsync(s.flags + at, 0, sizeof(uint32_t) * cnt, c->stream));
+    }
+    if (p->id) {
+        GE_HIP_CHECK(c, hipMemcpyAsync(s.id + at, p->id, sizeof(uint32_t) * cnt, hipMemcpyHostToDevice, c->stream));
+    } else {
+        std::vector<uint32_t> ids(cnt);
+        for (uint32_t i = 0; i < cnt; ++i) ids[i] = i;
+        GE_HIP_CHECK(c, hipMemcpyAsync(s.id + at, ids.data(), sizeof(uint32_t) * cnt, hipMemcpyHostToDevice, c->stream));
+        GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    }
+    if (c->sph) {
+        if ((r = put(s.sph_mass, p->sph_mass, 1.f)) || (r = put(s.rest_dens, p->rest_dens, 0.f)) ||
+            (r = put(s.stiff, p->stiff, 0.f)) || (r = put(s.visc, p->visc, 0.f)) ||
+            (r = put(s.cohesion, p->cohesion, 0.f)) || (r = put(s.dens, p->dens, 0.f)) ||
+            (r = put(s.press, p->press, 0.f)) || (r = put(s.press_tmp, p->press_tmp, 0.f)) ||
+            (r = put(s.pred_dens, p->pred_dens, 0.f)) || (r = put(s.ppx, p->ppx, 0.f)) ||
+            (r = put(s.ppy, p->ppy, 0.f)) || (r = put(s.pfx, p->pfx, 0.f)) || (r = put(s.pfy, p->pfy, 0.f)))
+            return r;
+    }
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+// Reads slots [at, at + cnt) of the current state into host arrays p[0 .. cnt).
+int get_fields(ge_ctx *c, uint32_t at, uint32_t cnt, const ge_particles *p) {
+    if (cnt == 0) return GE_OK;
+    const State &s = c->b.st[c->b.cur];
+    const size_t fb = sizeof(float) * cnt;
+    auto get = [&](float *dst, const float *src) -> int {
+        if (dst && src) GE_HIP_CHECK(c, hipMemcpyAsync(dst, src + at, fb, hipMemcpyDeviceToHost, c->stream));
+        return GE_OK;
+    };
+    int r = 0;
+    if ((r = get(p->x, s.x)) || (r = get(p->y, s.y)) || (r = get(p->vx, s.vx)) || (r = get(p->vy, s.vy)) ||
+        (r = get(p->ax, s.ax)) || (r = get(p->ay, s.ay)) || (r = get(p->pvx, s.pvx)) || (r = get(p->pvy, s.pvy)) ||
+        (r = get(p->mass, s.m)))
+        return r;
+    if (p->flags) GE_HIP_CHECK(c, hipMemcpyAsync(p->flags, s.flags + at, sizeof(uint32_t) * cnt, hipMemcpyDeviceToHost, c->stream));
+    if (p->id) GE_HIP_CHECK(c, hipMemcpyAsync(p->id, s.id + at, sizeof(uint32_t) * cnt, hipMemcpyDeviceToHost, c->stream));
+    if (c->sph) {
+        if ((r = get(p->sph_mass, s.sph_mass)) || (r = get(p->rest_dens, s.rest_dens)) || (r = get(p->stiff, s.stiff)) ||
+            (r = get(p->visc, s.visc)) || (r = get(p->cohesion, s.cohesion)) || (r = get(p->dens, s.dens)) ||
+            (r = get(p->press, s.press)) || (r = get(p->press_tmp, s.press_tmp)) ||
+            (r = get(p->pred_dens, s.pred_dens)) || (r = get(p->ppx, s.ppx)) || (r = get(p->ppy, s.ppy)) ||
+            (r = get(p->pfx, s.pfx)) || (r = get(p->pfy, s.pfy)))
+            return r;
+    }
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }
 
@@ -202,6 +281,9 @@ int check_params(ge_ctx *c, const ge_step_params *p) {
     if (p->sph && !(p->sph_cell > 0.0f && p->sph_h > 0.0f)) return fail(c, GE_E_INVALID, "sph_h and sph_cell must be > 0");
     if (p->accuracy != 0 && p->accuracy != 1) return fail(c, GE_E_INVALID, "accuracy must be 0 or 1");
     if (p->accuracy && !(p->theta > 0.0f)) return fail(c, GE_E_INVALID, "accuracy mode needs theta > 0");
+    if (p->leaf_capacity > 64) return fail(c, GE_E_INVALID, "leaf_capacity must be 0 (reference rule) or 1..64");
+    if (p->walk_block != 0 && p->walk_block != 64 && p->walk_block != 128 && p->walk_block != 256)
+        return fail(c, GE_E_INVALID, "walk_block must be 0, 64, 128 or 256");
     return GE_OK;
 }
 
@@ -252,6 +334,7 @@ bool use_direct(const ge_ctx *c, const ge_step_params *p) {
 
 // One frame of updateScene() (globalLogic.cpp:1155-1372).
 int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) {
+    c->leaf_cap = p->leaf_capacity;
     if (dt_start != 0.0f) {
         // integrateStart + boundingBox, then computeMortonKeys, the sort and the tree
         { PhaseScope s(c, GE_PHASE_INTEGRATE_BBOX); launch_integrate_start_bbox(c, dt_start, p->sph, p->sph_max_vel, 1); }
@@ -289,6 +372,54 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
         PhaseScope s(c, GE_PHASE_SPH);
         if (int r = sph_pcisph(c, p, p->dt)) return r;
     }
+    return GE_OK;
+}
+
+void drop_graph(ge_ctx *c) {
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    c->graph = ge_ctx::FrameGraph{};
+}
+
+bool graph_matches(const ge_ctx *c, const ge_step_params *p) {
+    const ge_ctx::FrameGraph &g = c->graph;
+    return g.exec && g.layout_version == c->layout_version && g.cur == c->b.cur && g.sort_a == c->b.sort_a &&
+           g.grid_a == c->b.grid_pos.keys && g.grid_b == c->b.grid_pred.keys &&
+           std::memcmp(&g.params, p, sizeof *p) == 0;
+}
+
+// Captures two frames at dt (the pair leaves every alternating buffer where it
+// found it, so the graph can be replayed back to back).  Capture runs no work.
+int capture_pair(ge_ctx *c, const ge_step_params *p) {
+    drop_graph(c);
+    ge_ctx::FrameGraph g;
+    g.params = *p;
+    g.layout_version = c->layout_version;
+    g.cur = c->b.cur;
+    g.sort_a = c->b.sort_a;
+    g.grid_a = c->b.grid_pos.keys;
+    g.grid_b = c->b.grid_pred.keys;
+    GE_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = frame(c, p, p->dt, false);
+    if (r == GE_OK) r = frame(c, p, p->dt, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (r != GE_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+    }
+    if (e != hipSuccess || !graph) return fail(c, GE_E_HIP, "hipStreamEndCapture failed");
+    const bool same = c->b.cur == g.cur && c->b.sort_a == g.sort_a && c->b.grid_pos.keys == g.grid_a &&
+                      c->b.grid_pred.keys == g.grid_b;
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(c, GE_E_HIP, "hipGraphInstantiate failed");
+    if (!same) {  // buffers did not return to their starting places: no replay
+        (void)hipGraphExecDestroy(exec);
+        return fail(c, GE_E_HIP, "frame pair does not restore the buffer layout");
+    }
+    g.exec = exec;
+    c->graph = g;
     return GE_OK;
 }
 
@@ -364,6 +495,7 @@ int ge_destroy(ge_ctx *c) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
+    drop_graph(c);
     free_all(c);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -395,59 +527,72 @@ int ge_upload_systems(ge_ctx *c, uint32_t nsys, const uint32_t *counts, const ge
         total += counts[s];
     }
     if (total > c->cap) return fail(c, GE_E_CAPACITY, "particle count above capacity");
-    const uint32_t n = (uint32_t)total;
-    if (n > 0 && (!p->x || !p->y || !p->mass)) return fail(c, GE_E_INVALID, "x, y and mass are required");
+    if (total > 0 && (!p->x || !p->y || !p->mass)) return fail(c, GE_E_INVALID, "x, y and mass are required");
     (void)hipSetDevice(c->device);
     if (int r = set_systems(c, nsys, counts)) {
         return fail(c, r, r == GE_E_CAPACITY ? "systems too many or too large for one packed sort word" : "HIP error");
     }
     c->b.cur = 0;
-    State &s = c->b.st[0];
-    c->n = n;
+    c->layout_version += 1;
     c->have_tree = false;
     c->have_grid = false;
-    if (n == 0) return GE_OK;
-    const size_t fb = sizeof(float) * n;
-    auto put = [&](float *dst, const float *src, float fill) -> int {
-        if (!dst) return GE_OK;
-        if (src) {
-            GE_HIP_CHECK(c, hipMemcpyAsync(dst, src, fb, hipMemcpyHostToDevice, c->stream));
-        } else if (fill == 0.0f) {
-            GE_HIP_CHECK(c, hipMemsetAsync(dst, 0, fb, c->stream));
-        } else {
-            std::vector<float> tmp(n, fill);
-            GE_HIP_CHECK(c, hipMemcpyAsync(dst, tmp.data(), fb, hipMemcpyHostToDevice, c->stream));
-            GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-        }
+    return put_fields(c, c->b.st[0], 0, c->n, p);
+}
+
+int ge_systems_update(ge_ctx *c, uint32_t nkeep, const uint32_t *keep, uint32_t nadd, const uint32_t *counts,
+                      const ge_particles *p) {
+    if (!c || (nkeep && !keep) || (nadd && (!counts || !p))) return GE_E_INVALID;
+    const uint32_t old_nsys = c->n ? c->nsys : 0;
+    const std::vector<uint32_t> old_off = c->host_off;
+    for (uint32_t t = 0; t < nkeep; ++t) {
+        if (keep[t] >= old_nsys || (t > 0 && keep[t] <= keep[t - 1]))
+            return fail(c, GE_E_INVALID, "keep[] must list existing systems in ascending order");
+    }
+    const uint32_t nsys = nkeep + nadd;
+    if (nsys == 0) {
+        c->n = 0;
+        c->nsys = 1;
+        c->host_off.assign(2, 0);
+        c->layout_version += 1;
+        c->have_tree = c->have_grid = false;
         return GE_OK;
-    };
-    int r = 0;
-    if ((r = put(s.x, p->x, 0.f)) || (r = put(s.y, p->y, 0.f)) || (r = put(s.vx, p->vx, 0.f)) ||
-        (r = put(s.vy, p->vy, 0.f)) || (r = put(s.ax, p->ax, 0.f)) || (r = put(s.ay, p->ay, 0.f)) ||
-        (r = put(s.pvx, p->pvx, 0.f)) || (r = put(s.pvy, p->pvy, 0.f)) || (r = put(s.m, p->mass, 0.f)))
-        return r;
-    if (p->flags) {
-        GE_HIP_CHECK(c, hipMemcpyAsync(s.flags, p->flags, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
-    } else {
-        GE_HIP_CHECK(c, hipMemsetAsync(s.flags, 0, sizeof(uint32_t) * n, c->stream));
     }
-    if (p->id) {
-        GE_HIP_CHECK(c, hipMemcpyAsync(s.id, p->id, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
-    } else {
-        std::vector<uint32_t> ids(n);
-        for (uint32_t i = 0; i < n; ++i) ids[i] = i;
-        GE_HIP_CHECK(c, hipMemcpyAsync(s.id, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
-        GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (nsys > c->max_sys) return fail(c, GE_E_CAPACITY, "more systems than max_systems");
+    if (c->sph && nsys > 1) return fail(c, GE_E_UNSUPPORTED, "SPH contexts hold one system");
+    std::vector<uint32_t> cnt(nsys), src(nkeep + 1, 0);
+    uint64_t total = 0;
+    bool moved = false;
+    uint32_t n_kept = 0;
+    for (uint32_t t = 0; t < nkeep; ++t) {
+        cnt[t] = old_off[keep[t] + 1] - old_off[keep[t]];
+        src[t] = old_off[keep[t]];
+        if (src[t] != n_kept) moved = true;
+        n_kept += cnt[t];
     }
-    if (c->sph) {
-        if ((r = put(s.sph_mass, p->sph_mass, 1.f)) || (r = put(s.rest_dens, p->rest_dens, 0.f)) ||
-            (r = put(s.stiff, p->stiff, 0.f)) || (r = put(s.visc, p->visc, 0.f)) ||
-            (r = put(s.cohesion, p->cohesion, 0.f)) || (r = put(s.dens, p->dens, 0.f)) ||
-            (r = put(s.press, p->press, 0.f)) || (r = put(s.press_tmp, p->press_tmp, 0.f)) ||
-            (r = put(s.pred_dens, p->pred_dens, 0.f)) || (r = put(s.ppx, p->ppx, 0.f)) ||
-            (r = put(s.ppy, p->ppy, 0.f)) || (r = put(s.pfx, p->pfx, 0.f)) || (r = put(s.pfy, p->pfy, 0.f)))
-            return r;
+    for (uint32_t a = 0; a < nadd; ++a) {
+        if (counts[a] == 0) return fail(c, GE_E_INVALID, "every system needs a particle");
+        cnt[nkeep + a] = counts[a];
     }
+    for (uint32_t t = 0; t < nsys; ++t) total += cnt[t];
+    if (total > c->cap) return fail(c, GE_E_CAPACITY, "particle count above capacity");
+    if (nadd && total > n_kept && (!p->x || !p->y || !p->mass)) return fail(c, GE_E_INVALID, "x, y and mass are required");
+    (void)hipSetDevice(c->device);
+    // the new layout must fit a packed sort word before anything moves
+    {
+        uint32_t maxc = 1;
+        for (uint32_t v : cnt) maxc = v > maxc ? v : maxc;
+        if (nsys > 1 && bits_for(maxc) + bits_for(nsys) > kIdxBits)
+            return fail(c, GE_E_CAPACITY, "systems too many or too large for one packed sort word");
+    }
+    if (int r = set_systems(c, nsys, cnt.data())) return fail(c, r, "HIP error");
+    if (moved) {
+        GE_HIP_CHECK(c, hipMemcpy(c->b.sys_src, src.data(), sizeof(uint32_t) * nkeep, hipMemcpyHostToDevice));
+        launch_compact(c, n_kept);
+    }
+    c->layout_version += 1;
+    c->have_tree = false;
+    c->have_grid = false;
+    if (nadd) return put_fields(c, c->b.st[c->b.cur], n_kept, (uint32_t)(total - n_kept), p);
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }
@@ -455,31 +600,32 @@ int ge_upload_systems(ge_ctx *c, uint32_t nsys, const uint32_t *counts, const ge
 int ge_download(ge_ctx *c, const ge_particles *p) {
     if (!c || !p) return GE_E_INVALID;
     (void)hipSetDevice(c->device);
-    const uint32_t n = c->n;
-    if (n == 0) return GE_OK;
-    const State &s = c->b.st[c->b.cur];
-    const size_t fb = sizeof(float) * n;
-    auto get = [&](float *dst, const float *src) -> int {
-        if (dst && src) GE_HIP_CHECK(c, hipMemcpyAsync(dst, src, fb, hipMemcpyDeviceToHost, c->stream));
-        return GE_OK;
-    };
-    int r = 0;
-    if ((r = get(p->x, s.x)) || (r = get(p->y, s.y)) || (r = get(p->vx, s.vx)) || (r = get(p->vy, s.vy)) ||
-        (r = get(p->ax, s.ax)) || (r = get(p->ay, s.ay)) || (r = get(p->pvx, s.pvx)) || (r = get(p->pvy, s.pvy)) ||
-        (r = get(p->mass, s.m)))
-        return r;
-    if (p->flags) GE_HIP_CHECK(c, hipMemcpyAsync(p->flags, s.flags, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
-    if (p->id) GE_HIP_CHECK(c, hipMemcpyAsync(p->id, s.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
-    if (c->sph) {
-        if ((r = get(p->sph_mass, s.sph_mass)) || (r = get(p->rest_dens, s.rest_dens)) || (r = get(p->stiff, s.stiff)) ||
-            (r = get(p->visc, s.visc)) || (r = get(p->cohesion, s.cohesion)) || (r = get(p->dens, s.dens)) ||
-            (r = get(p->press, s.press)) || (r = get(p->press_tmp, s.press_tmp)) ||
-            (r = get(p->pred_dens, s.pred_dens)) || (r = get(p->ppx, s.ppx)) || (r = get(p->ppy, s.ppy)) ||
-            (r = get(p->pfx, s.pfx)) || (r = get(p->pfy, s.pfy)))
-            return r;
-    }
-    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (int r = get_fields(c, 0, c->n, p)) return r;
     return check_device_errors(c);
+}
+
+int ge_download_system(ge_ctx *c, uint32_t sys, const ge_particles *p) {
+    if (!c || !p) return GE_E_INVALID;
+    if (c->n == 0 || sys >= c->nsys) return fail(c, GE_E_INVALID, "no such system");
+    (void)hipSetDevice(c->device);
+    if (int r = get_fields(c, c->host_off[sys], c->host_off[sys + 1] - c->host_off[sys], p)) return r;
+    return check_device_errors(c);
+}
+
+int ge_poll(ge_ctx *c, int32_t *done) {
+    if (!c || !done) return GE_E_INVALID;
+    (void)hipSetDevice(c->device);
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) {
+        *done = 1;
+        return GE_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return GE_OK;
+    }
+    c->err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+    return GE_E_HIP;
 }
 
 int ge_step(ge_ctx *c, const ge_step_params *p, int32_t nsteps) {
@@ -488,7 +634,19 @@ int ge_step(ge_ctx *c, const ge_step_params *p, int32_t nsteps) {
     if (nsteps < 0) return fail(c, GE_E_INVALID, "nsteps < 0");
     if (c->n == 0 || nsteps == 0) return GE_OK;
     (void)hipSetDevice(c->device);
-    for (int32_t k = 0; k < nsteps; ++k) {
+    int32_t k = 0;
+    if (p->use_graph && !c->prof && nsteps >= 3) {
+        // the first frame integrates with dt_start, so it runs eagerly
+        if (int r = frame(c, p, p->dt_start, false)) return r;
+        k = 1;
+        const int32_t pairs = (nsteps - 1) / 2;
+        if (!graph_matches(c, p)) {
+            if (int r = capture_pair(c, p)) return r;
+        }
+        for (int32_t q = 0; q < pairs; ++q) GE_HIP_CHECK(c, hipGraphLaunch(c->graph.exec, c->stream));
+        k += 2 * pairs;
+    }
+    for (; k < nsteps; ++k) {
         const float dt0 = (k == 0) ? p->dt_start : p->dt;
         if (int r = frame(c, p, dt0, false)) return r;
     }
@@ -549,6 +707,7 @@ int ge_build_quadtree(ge_ctx *c, const float bb[3]) {
     (void)hipSetDevice(c->device);
     float h[4] = {bb[0], bb[1], bb[2], 0.0f};
     GE_HIP_CHECK(c, hipMemcpyAsync(c->b.bb, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    c->leaf_cap = 0;  // the reference's leaf rule
     launch_tree_build(c);
     c->have_tree = true;
     return ge_sync(c);

@@ -119,6 +119,7 @@ struct Buffers {
     float *sys_bb = nullptr;       // 8 per system: {x, y, size, 0, minx, miny, maxx, maxy}
     uint32_t *chunk_lo = nullptr;  // reduction chunks: first slot of each chunk (+ end)
     uint32_t *chunk_first = nullptr;  // first chunk of each system (nsys + 1)
+    uint32_t *sys_src = nullptr;      // compaction: old first slot of each kept system
     uint32_t max_chunks = 0;
     TopNodes top;
     uint32_t *start_count = nullptr;  // per particle: nodes whose range starts there (then its prefix)
@@ -154,6 +155,17 @@ struct ge_ctx {
     bool have_tree = false;
     bool have_grid = false;
     float sph_cell = 3.0f;         // cell size grid A was built with
+    uint32_t leaf_cap = 0;         // leaf rule of the next tree build (0 = reference)
+    uint64_t layout_version = 0;   // bumped whenever the particle layout changes
+    // hipGraph of two frames (the state and sort buffers alternate, so a pair
+    // returns every buffer pointer to where it started)
+    struct FrameGraph {
+        hipGraphExec_t exec = nullptr;
+        ge_step_params params{};
+        uint64_t layout_version = 0;
+        int cur = -1;
+        const void *sort_a = nullptr, *grid_a = nullptr, *grid_b = nullptr;
+    } graph;
     ge::Buffers b;
     std::string err;
     // phase timing (ge_profile_enable)
@@ -178,6 +190,8 @@ void launch_quad_moments(ge_ctx *c, float theta);
 void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
 void launch_zero_acc(ge_ctx *c);
 void launch_integrate_end(ge_ctx *c, float dt);
+void launch_sys_of(ge_ctx *c);
+void launch_compact(ge_ctx *c, uint32_t n_kept);
 // tree.hip
 void launch_tree_build(ge_ctx *c);
 // sph.hip

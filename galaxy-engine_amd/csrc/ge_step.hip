@@ -10,7 +10,6 @@ namespace ge {
 namespace {
 
 constexpr int kWave = 64;
-constexpr uint32_t kBboxMaxBlocks = 1024;
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
@@ -218,8 +217,8 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // the node-start prefix of the tree build); each lane is active only inside
 // its own system's range, and the wave walks from its lanes' first root to
 // their last end.
-template <bool KAHAN, bool COUNT, bool QUAD>
-__global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
+template <int BS, bool KAHAN, bool COUNT, bool QUAD>
+__global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                                                const NodeRec *__restrict__ nodes,
                                                const float4 *__restrict__ quad,
                                                const uint32_t *__restrict__ start_prefix,
@@ -232,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_bh(const float4 *__restrict__ pxym,
                                                const uint32_t *__restrict__ flags, float half_dt_end,
                                                unsigned long long *__restrict__ inter) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t i = blk * kBlock + threadIdx.x;
+    const uint32_t i = blk * BS + threadIdx.x;
     const bool live = i < n;
     const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     // a tree that raised an error flag may hold unwritten nodes: do not walk it
@@ -394,6 +393,47 @@ __global__ __launch_bounds__(kBlock) void k_direct(const float4 *__restrict__ px
 
 __global__ void k_fill_u64(uint64_t *__restrict__ p, uint64_t v) { *p = v; }
 
+// sys_of[i] = the system holding slot i (binary search over the offsets)
+__global__ __launch_bounds__(kBlock) void k_sys_of(const uint32_t *__restrict__ sys_off, uint32_t nsys, uint32_t n,
+                                                   uint32_t *__restrict__ sys_of) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = nsys;  // sys_off[lo] <= i < sys_off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sys_off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    sys_of[i] = lo;
+}
+
+// Moves kept systems to their new slots: new slot i of system t takes old
+// slot src_off[t] + (i - sys_off[t]).  Every field moves, so the state is the
+// same as if the system had never shared the context.
+__global__ __launch_bounds__(kBlock) void k_compact(State src, State dst, uint32_t n, int sph,
+                                                    const uint32_t *__restrict__ sys_of,
+                                                    const uint32_t *__restrict__ sys_off,
+                                                    const uint32_t *__restrict__ src_off) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = sys_of[i];
+    const uint32_t j = src_off[t] + (i - sys_off[t]);
+    dst.x[i] = src.x[j]; dst.y[i] = src.y[j]; dst.m[i] = src.m[j];
+    dst.vx[i] = src.vx[j]; dst.vy[i] = src.vy[j];
+    dst.ax[i] = src.ax[j]; dst.ay[i] = src.ay[j];
+    dst.pvx[i] = src.pvx[j]; dst.pvy[i] = src.pvy[j];
+    dst.flags[i] = src.flags[j]; dst.id[i] = src.id[j];
+    if (sph) {
+        dst.sph_mass[i] = src.sph_mass[j]; dst.rest_dens[i] = src.rest_dens[j];
+        dst.stiff[i] = src.stiff[j]; dst.visc[i] = src.visc[j];
+        dst.cohesion[i] = src.cohesion[j]; dst.dens[i] = src.dens[j];
+        dst.press[i] = src.press[j]; dst.press_tmp[i] = src.press_tmp[j];
+        dst.pred_dens[i] = src.pred_dens[j];
+        dst.ppx[i] = src.ppx[j]; dst.ppy[i] = src.ppy[j];
+        dst.pfx[i] = src.pfx[j]; dst.pfy[i] = src.pfy[j];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero2(float *__restrict__ a, float *__restrict__ b, uint32_t n) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) { a[i] = 0.f; b[i] = 0.f; }
@@ -439,35 +479,56 @@ void launch_gather(ge_ctx *c) {
     c->b.cur = 1 - cur;
 }
 
-template <bool KAHAN, bool COUNT, bool QUAD>
+template <int BS, bool KAHAN, bool COUNT, bool QUAD>
 void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const State &s = c->b.st[c->b.cur];
     const float theta_sq = p->theta * p->theta;
     const float soft_sq = p->softening * p->softening;
-    const uint32_t nb = blocks_for(c->n, kBlock);
+    const uint32_t nb = blocks_for(c->n, BS);
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
-    hipLaunchKernelGGL((k_bh<KAHAN, COUNT, QUAD>), dim3(nb), dim3(kBlock), 0, c->stream, c->b.pxym, c->b.nodes,
+    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, QUAD>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym, c->b.nodes,
                        c->b.node_quad, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n, p->G,
                        theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, inter);
 }
 
+template <int BS>
+void launch_bh_bs(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
+    const bool k = p->compensated != 0, q = p->accuracy != 0;
+    if (k && count && q) launch_bh_t<BS, true, true, true>(c, p, end_kick_half_dt);
+    else if (k && count) launch_bh_t<BS, true, true, false>(c, p, end_kick_half_dt);
+    else if (k && q) launch_bh_t<BS, true, false, true>(c, p, end_kick_half_dt);
+    else if (k) launch_bh_t<BS, true, false, false>(c, p, end_kick_half_dt);
+    else if (count && q) launch_bh_t<BS, false, true, true>(c, p, end_kick_half_dt);
+    else if (count) launch_bh_t<BS, false, true, false>(c, p, end_kick_half_dt);
+    else if (q) launch_bh_t<BS, false, false, true>(c, p, end_kick_half_dt);
+    else launch_bh_t<BS, false, false, false>(c, p, end_kick_half_dt);
+}
+
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
     if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, c->stream, c->b.interactions, 0ull);
-    const bool k = p->compensated != 0, q = p->accuracy != 0;
-    if (k && count && q) launch_bh_t<true, true, true>(c, p, end_kick_half_dt);
-    else if (k && count) launch_bh_t<true, true, false>(c, p, end_kick_half_dt);
-    else if (k && q) launch_bh_t<true, false, true>(c, p, end_kick_half_dt);
-    else if (k) launch_bh_t<true, false, false>(c, p, end_kick_half_dt);
-    else if (count && q) launch_bh_t<false, true, true>(c, p, end_kick_half_dt);
-    else if (count) launch_bh_t<false, true, false>(c, p, end_kick_half_dt);
-    else if (q) launch_bh_t<false, false, true>(c, p, end_kick_half_dt);
-    else launch_bh_t<false, false, false>(c, p, end_kick_half_dt);
+    switch (p->walk_block) {
+    case 64: launch_bh_bs<64>(c, p, end_kick_half_dt, count); break;
+    case 128: launch_bh_bs<128>(c, p, end_kick_half_dt, count); break;
+    default: launch_bh_bs<256>(c, p, end_kick_half_dt, count); break;
+    }
 }
 
 void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const State &s = c->b.st[c->b.cur];
     hipLaunchKernelGGL(k_direct, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.pxym, c->n,
                        p->G, p->softening, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt);
+}
+
+void launch_sys_of(ge_ctx *c) {
+    hipLaunchKernelGGL(k_sys_of, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.sys_off, c->nsys,
+                       c->n, c->b.sys_of);
+}
+
+void launch_compact(ge_ctx *c, uint32_t n_kept) {
+    const int cur = c->b.cur;
+    hipLaunchKernelGGL(k_compact, dim3(blocks_for(n_kept, kBlock)), dim3(kBlock), 0, c->stream, c->b.st[cur],
+                       c->b.st[1 - cur], n_kept, (int)c->sph, c->b.sys_of, c->b.sys_off, c->b.sys_src);
+    c->b.cur = 1 - cur;
 }
 
 void launch_zero_acc(ge_ctx *c) {

@@ -44,8 +44,11 @@ constexpr uint32_t kWaveSubMax = 64;
 constexpr uint32_t kFrontMax = 1024;  // LDS frontier of a workgroup subtree
 constexpr int kWaveLevels = 64;       // levels of a wave subtree
 
-__device__ __forceinline__ bool leaf_rule(uint32_t cnt, float size) {
-    // quadtree.cpp:16-17
+// leaf_cap 0: the reference's rule (quadtree.cpp:16-17).  leaf_cap 1..64: a
+// node of at most leaf_cap particles is a leaf (autotuned trees; not in the
+// reference), with the same 1e-6 size floor so coincident particles end.
+__device__ __forceinline__ bool leaf_rule(uint32_t cnt, float size, uint32_t leaf_cap) {
+    if (leaf_cap) return cnt <= leaf_cap || size <= 0.000001f;
     return ((cnt <= 4 && size <= 2.0f) || cnt == 1) || size <= 0.000001f;
 }
 
@@ -168,8 +171,9 @@ __device__ __forceinline__ float shfl_f(float v, uint32_t src) { return __shfl(v
 
 
 struct StartTable {
-    uint32_t *count;  // C, then its exclusive prefix P (n + 1 entries)
-    uint32_t *level;  // F
+    uint32_t *count;    // C, then its exclusive prefix P (n + 1 entries)
+    uint32_t *level;    // F
+    uint32_t leaf_cap;  // leaf rule selector (see leaf_rule)
 };
 
 __device__ __forceinline__ void register_start(const StartTable &st, uint32_t a, uint32_t level) {
@@ -208,7 +212,7 @@ __device__ uint32_t wave_subtree(const float *__restrict__ x, const float *__res
         const uint64_t actm = __ballot(act);
         if (actm == 0) break;
         const bool is_start = act && lane == a;
-        const bool leaf = act && leaf_rule(b - a, size);
+        const bool leaf = act && leaf_rule(b - a, size, st.leaf_cap);
         const uint64_t startm = __ballot(is_start);
         const uint64_t leafm = __ballot(is_start && leaf);
         total += (uint32_t)__popcll(startm);
@@ -399,7 +403,7 @@ __device__ bool finish_small(TopNodes &T, const StartTable &st, const float *__r
                              const float *__restrict__ y, uint32_t node, float px, float py, float size,
                              uint32_t s, uint32_t e, uint32_t depth, uint32_t *err) {
     const uint32_t c = e - s;
-    if (leaf_rule(c, size)) {
+    if (leaf_rule(c, size, st.leaf_cap)) {
         if (lane_id() == 0) {
             T.kind[node] = kKindLeaf;
             register_start(st, s, depth);
@@ -756,7 +760,7 @@ void launch_tree_build(ge_ctx *c) {
     const State &s = c->b.st[c->b.cur];
     TopNodes &T = c->b.top;
     const uint64_t n = c->n;
-    const StartTable st{c->b.start_count, c->b.start_level};
+    const StartTable st{c->b.start_count, c->b.start_level, c->leaf_cap};
     hipLaunchKernelGGL(k_tree_clear, dim3(blocks_for(n + 1, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, T, st,
                        c->b.sys_bb, c->b.sys_off, c->nsys, c->n, c->b.counters);
     // level L+1 only holds children of nodes above kBlockSubMax particles

@@ -26,7 +26,7 @@ EXPORTS = [
     "ge_compute_gravity", "ge_integrate_start", "ge_integrate_end", "ge_sph_new_grid",
     "ge_sph_pcisph", "ge_get_keys", "ge_get_perm", "ge_get_node_count", "ge_get_nodes",
     "ge_get_interactions", "ge_get_neighbors", "ge_profile_enable", "ge_profile_read",
-    "ge_upload_systems", "ge_system_offsets",
+    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll",
 ]
 PHASES = ["integrate_bbox", "keys", "sort", "gather", "tree", "gravity", "sph_grid", "sph",
           "integrate_end"]
@@ -58,7 +58,8 @@ class StepParams(C.Structure):
                 ("cohesion_coef", C.c_float), ("delta", C.c_float), ("stiff_mult", C.c_float),
                 ("sph_max_vel", C.c_float), ("periodic", C.c_int32), ("temperature", C.c_int32),
                 ("sph_ground", C.c_int32), ("compensated", C.c_int32), ("accuracy", C.c_int32),
-                ("reserved", C.c_uint32 * 5)]
+                ("leaf_capacity", C.c_uint32), ("walk_block", C.c_uint32), ("use_graph", C.c_int32),
+                ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -133,6 +134,10 @@ def load_library():
         "ge_profile_read": (C.c_int, [vp, P(C.c_double), P(C.c_uint32)]),
         "ge_upload_systems": (C.c_int, [vp, C.c_uint32, P(C.c_uint32), P(GeParticles)]),
         "ge_system_offsets": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32), C.c_uint32]),
+        "ge_systems_update": (C.c_int, [vp, C.c_uint32, P(C.c_uint32), C.c_uint32, P(C.c_uint32),
+                                        P(GeParticles)]),
+        "ge_download_system": (C.c_int, [vp, C.c_uint32, P(GeParticles)]),
+        "ge_poll": (C.c_int, [vp, P(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -215,10 +220,8 @@ class Context:
             raise GeError(rc, self.lib.ge_last_error(self.h).decode())
 
     # ---- state transfer
-    def upload(self, state, counts=None):
-        """state: dict of numpy arrays (x, y, mass required).  counts: particles
-        per system when the arrays hold several systems back to back."""
-        n = len(state["x"])
+    @staticmethod
+    def _particles(state, n):
         keep = []
         gp = GeParticles()
         for f in PARTICLE_ORDER:
@@ -228,12 +231,52 @@ class Context:
                 assert len(a) == n, f
                 keep.append(a)
                 setattr(gp, f, _uptr(a) if f in FIELDS_U else _fptr(a))
+        return gp, keep
+
+    def upload(self, state, counts=None):
+        """state: dict of numpy arrays (x, y, mass required).  counts: particles
+        per system when the arrays hold several systems back to back."""
+        n = len(state["x"])
+        gp, keep = self._particles(state, n)
         if counts is None:
             self._check(self.lib.ge_upload(self.h, n, C.byref(gp)))
         else:
             cnt = np.ascontiguousarray(counts, dtype=np.uint32)
             assert int(cnt.sum()) == n
             self._check(self.lib.ge_upload_systems(self.h, len(cnt), _uptr(cnt), C.byref(gp)))
+
+    def systems_update(self, keep, state=None, counts=()):
+        """Keep systems `keep` (ascending current indices), then append the
+        systems of `state` (counts per system, concatenated)."""
+        kp = np.ascontiguousarray(keep, dtype=np.uint32)
+        cnt = np.ascontiguousarray(counts, dtype=np.uint32)
+        if state is not None and len(cnt):
+            gp, hold = self._particles(state, int(cnt.sum()))
+            gpp = C.byref(gp)
+        else:
+            gpp, hold = None, []
+        self._check(self.lib.ge_systems_update(self.h, len(kp), _uptr(kp) if len(kp) else None, len(cnt),
+                                               _uptr(cnt) if len(cnt) else None, gpp))
+        del hold
+
+    def download_system(self, sys, fields=None):
+        off = self.system_offsets()
+        n = int(off[sys + 1] - off[sys])
+        if fields is None:
+            fields = FIELDS_F + FIELDS_U + (FIELDS_SPH if self.sph else [])
+        out = {}
+        gp = GeParticles()
+        for f in fields:
+            a = np.empty(n, dtype=np.uint32 if f in FIELDS_U else np.float32)
+            out[f] = a
+            setattr(gp, f, _uptr(a) if f in FIELDS_U else _fptr(a))
+        self._check(self.lib.ge_download_system(self.h, int(sys), C.byref(gp)))
+        return out
+
+    def poll(self):
+        d = C.c_int32()
+        self._check(self.lib.ge_poll(self.h, C.byref(d)))
+        return bool(d.value)
 
     def system_offsets(self):
         ns = C.c_uint32()

@@ -97,7 +97,14 @@ typedef struct {
                              1: quadrupole moments and the offset-corrected
                              opening test |d| > size/theta + |COM - centre|
                              (no counterpart in the reference) */
-    uint32_t reserved[5];
+    uint32_t leaf_capacity; /* 0: the reference's leaf rule (quadtree.cpp:16-17);
+                               1..64: a node of at most this many particles is a
+                               leaf (autotuned trees; not in the reference) */
+    uint32_t walk_block;  /* threads per workgroup of the BH walk: 0 (= 256),
+                             64, 128 or 256 */
+    int32_t use_graph;    /* ge_step captures its frames in a hipGraph once and
+                             replays it while params and layout stay the same */
+    uint32_t reserved[2];
 } ge_step_params;
 
 /* Quadtree node as laid out by the reference (GE/include/Physics/quadtree.h:
@@ -134,7 +141,20 @@ int ge_count(const ge_ctx *ctx, uint32_t *n);
 int ge_upload_systems(ge_ctx *ctx, uint32_t nsys, const uint32_t *counts, const ge_particles *p);
 /* nsys + 1 offsets of the systems in the device order */
 int ge_system_offsets(const ge_ctx *ctx, uint32_t *nsys, uint32_t *offsets, uint32_t cap);
+/* Continuous batching (no counterpart in the reference): keep the systems
+ * keep[0 .. nkeep) (current indices, ascending), drop the rest, and append
+ * nadd new systems of counts[] particles read from p (concatenated; NULL id
+ * gives 0 .. count-1 over the appended block).  Kept systems stay resident
+ * in HBM and are moved on the device; the new order is the kept systems,
+ * then the new ones.  A kept system's later results are the same as if it
+ * had been stepped alone. */
+int ge_systems_update(ge_ctx *ctx, uint32_t nkeep, const uint32_t *keep, uint32_t nadd,
+                      const uint32_t *counts, const ge_particles *p);
+/* ge_download restricted to system sys (its particles in device order) */
+int ge_download_system(ge_ctx *ctx, uint32_t sys, const ge_particles *p);
 int ge_sync(ge_ctx *ctx);
+/* non-blocking: *done = 1 when every call queued on the context finished */
+int ge_poll(ge_ctx *ctx, int32_t *done);
 
 /* ---- one frame ---------------------------------------------------------- */
 /* updateScene()'s 2D physics ordering (GE/src/globalLogic.cpp:1155-1372):

@@ -112,6 +112,7 @@ typedef struct {
     geo_node *nodes;
     int64_t count, cap;
     const float *x, *y, *m;
+    uint32_t leaf_cap;
 } tree_builder;
 
 /* std::partition_point over [first,last) with predicate v[i] < mid: the
@@ -153,7 +154,11 @@ static int tb_node(tree_builder *t, int64_t idx, float px, float py, float size,
     nd.mass = 0.0f; nd.comx = 0.0f; nd.comy = 0.0f;
     nd.child[0] = nd.child[1] = nd.child[2] = nd.child[3] = U32MAX;
     uint32_t cnt = e - s;
-    if (((cnt <= 4 && size <= 2.0f) || cnt == 1) || size <= 0.000001f) {
+    /* leaf_cap 0: the reference rule (quadtree.cpp:16-17); otherwise a node of
+     * at most leaf_cap particles is a leaf (autotuned trees, same size floor) */
+    int is_leaf = t->leaf_cap ? (cnt <= t->leaf_cap || size <= 0.000001f)
+                              : (((cnt <= 4 && size <= 2.0f) || cnt == 1) || size <= 0.000001f);
+    if (is_leaf) {
         /* computeLeafMass, quadtree.h:53-67 */
         float gm = 0.0f, cx = 0.0f, cy = 0.0f;
         for (uint32_t i = s; i < e; ++i) {
@@ -205,8 +210,9 @@ static int tb_node(tree_builder *t, int64_t idx, float px, float py, float size,
 }
 
 static int64_t tree_build_internal(int32_t n, const float *x, const float *y, const float *m,
-                                   const float bb[3], geo_node **out) {
+                                   const float bb[3], uint32_t leaf_cap, geo_node **out) {
     tree_builder t;
+    t.leaf_cap = leaf_cap;
     t.cap = 4 * (int64_t)n + 16; /* Quadtree::root reserves 4N (quadtree.cpp:94) */
     t.count = 0;
     t.nodes = (geo_node *)malloc(sizeof(geo_node) * (size_t)t.cap);
@@ -223,8 +229,13 @@ static int64_t tree_build_internal(int32_t n, const float *x, const float *y, co
 
 int32_t geo_build_tree(int32_t n, const float *x, const float *y, const float *m,
                        const float bb[3], geo_node *nodes, int32_t cap) {
+    return geo_build_tree_cap(n, x, y, m, bb, 0, nodes, cap);
+}
+
+int32_t geo_build_tree_cap(int32_t n, const float *x, const float *y, const float *m,
+                           const float bb[3], uint32_t leaf_cap, geo_node *nodes, int32_t cap) {
     geo_node *tmp = NULL;
-    int64_t c = tree_build_internal(n, x, y, m, bb, &tmp);
+    int64_t c = tree_build_internal(n, x, y, m, bb, leaf_cap, &tmp);
     if (c < 0) return -1;
     int64_t k = c < cap ? c : cap;
     if (nodes && k > 0) memcpy(nodes, tmp, sizeof(geo_node) * (size_t)k);
@@ -663,7 +674,7 @@ int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps) {
             geo_sort_perm(n, keys, perm);
             permute_state(s, perm);
             if (!p->naive) {
-                nn = tree_build_internal(n, s->x, s->y, s->m, bb, &nodes);
+                nn = tree_build_internal(n, s->x, s->y, s->m, bb, 0, &nodes);
                 if (nn < 0) { free(keys); free(perm); return -2; }
             }
         }

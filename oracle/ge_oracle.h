@@ -86,6 +86,10 @@ void geo_sort_perm(int32_t n, const uint64_t *keys, uint32_t *perm);
 /* quadtree: GE/src/Physics/quadtree.cpp:5-107; returns node count or -1 if cap exceeded */
 int32_t geo_build_tree(int32_t n, const float *x, const float *y, const float *m,
                        const float bb[3], geo_node *nodes, int32_t cap);
+/* the same build with leaf rule "count <= leaf_cap || size <= 1e-6" when
+ * leaf_cap > 0 (autotuned trees; leaf_cap 0 is geo_build_tree) */
+int32_t geo_build_tree_cap(int32_t n, const float *x, const float *y, const float *m,
+                           const float bb[3], uint32_t leaf_cap, geo_node *nodes, int32_t cap);
 /* BH accelerations: GE/src/Physics/physics.cpp:91-205 */
 void geo_bh_accel(int32_t n, const float *x, const float *y, const float *m,
                   const geo_node *nodes, int32_t nn, float G, float theta, float eps,

@@ -69,6 +69,8 @@ def load():
     lib.geo_sort_perm.argtypes = [C.c_int32, u64p, u32p]
     lib.geo_build_tree.restype = C.c_int32
     lib.geo_build_tree.argtypes = [C.c_int32, f32p, f32p, f32p, f32p, C.c_void_p, C.c_int32]
+    lib.geo_build_tree_cap.restype = C.c_int32
+    lib.geo_build_tree_cap.argtypes = [C.c_int32, f32p, f32p, f32p, f32p, C.c_uint32, C.c_void_p, C.c_int32]
     lib.geo_bh_accel.argtypes = [C.c_int32, f32p, f32p, f32p, C.c_void_p, C.c_int32, C.c_float,
                                  C.c_float, C.c_float, f32p, f32p, u64p]
     lib.geo_direct_accel.argtypes = [C.c_int32, f32p, f32p, f32p, C.c_float, C.c_float, f32p, f32p]
@@ -127,17 +129,19 @@ def sort_perm(keys):
     return p
 
 
-def build_tree(x, y, m, bb):
+def build_tree(x, y, m, bb, leaf_cap=0):
     x, y, m, bb = f32(x), f32(y), f32(m), f32(bb)
     lib = load()
     cap = 4 * len(x) + 16
     nodes = np.zeros(cap, NODE_DTYPE)
-    c = lib.geo_build_tree(len(x), _f(x), _f(y), _f(m), _f(bb), C.c_void_p(nodes.ctypes.data), cap)
+    c = lib.geo_build_tree_cap(len(x), _f(x), _f(y), _f(m), _f(bb), int(leaf_cap),
+                               C.c_void_p(nodes.ctypes.data), cap)
     if c < 0:
         raise RuntimeError("tree build failed")
     if c > cap:
         nodes = np.zeros(c, NODE_DTYPE)
-        lib.geo_build_tree(len(x), _f(x), _f(y), _f(m), _f(bb), C.c_void_p(nodes.ctypes.data), c)
+        lib.geo_build_tree_cap(len(x), _f(x), _f(y), _f(m), _f(bb), int(leaf_cap),
+                               C.c_void_p(nodes.ctypes.data), c)
     return nodes[:c].copy()
 
 

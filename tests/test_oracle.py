@@ -86,7 +86,7 @@ def test_tree_hand_checked():
     assert got == [tuple(v) for v in t["nodes"]]
 
 
-def _check_tree(nodes, x, y, m):
+def _check_tree(nodes, x, y, m, leaf_cap=0):
     nn = len(nodes)
     n = len(x)
     assert nodes[0]["start"] == 0 and nodes[0]["end"] == n
@@ -95,7 +95,10 @@ def _check_tree(nodes, x, y, m):
         nd = nodes[i]
         cnt = int(nd["end"] - nd["start"])
         assert cnt >= 1
-        is_leaf = ((cnt <= 4 and nd["size"] <= 2.0) or cnt == 1) or nd["size"] <= 1e-6
+        if leaf_cap:
+            is_leaf = cnt <= leaf_cap or nd["size"] <= 1e-6
+        else:
+            is_leaf = ((cnt <= 4 and nd["size"] <= 2.0) or cnt == 1) or nd["size"] <= 1e-6
         ch = [c for c in nd["child"] if c != 0xFFFFFFFF]
         if is_leaf:
             assert nd["next"] == 0 and not ch
@@ -147,6 +150,32 @@ def test_tree_invariants():
             sl = slice(int(nd["start"]), int(nd["end"]))
             assert np.all(x[sl] >= nd["posx"] - 1e-3) and np.all(x[sl] <= nd["posx"] + nd["size"] + 1e-3)
             assert np.all(y[sl] >= nd["posy"] - 1e-3) and np.all(y[sl] <= nd["posy"] + nd["size"] + 1e-3)
+
+
+@pytest.mark.parametrize("cap", [8, 16, 32])
+def test_tree_leaf_capacity_is_pruned_full_tree(cap):
+    # a leaf-capacity tree splits exactly like the fully refined tree (same
+    # partition points and squares) and stops at nodes of <= cap particles
+    x, y, m, bb = _sorted_cloud(4000, 3)
+    full = O.build_tree(x, y, m, bb, leaf_cap=1)
+    t = O.build_tree(x, y, m, bb, leaf_cap=cap)
+    _check_tree(t, x, y, m, leaf_cap=cap)
+    geo = {(float(a["posx"]), float(a["posy"]), float(a["size"]), int(a["start"]), int(a["end"])) for a in full}
+    for nd in t:
+        key = (float(nd["posx"]), float(nd["posy"]), float(nd["size"]), int(nd["start"]), int(nd["end"]))
+        assert key in geo
+        cnt = int(nd["end"] - nd["start"])
+        if nd["next"] == 0:
+            assert cnt <= cap or nd["size"] <= 1e-6
+            # computeLeafMass: sequential float sums over the leaf's range
+            gm = np.float32(0)
+            for i in range(int(nd["start"]), int(nd["end"])):
+                gm = np.float32(gm + m[i])
+            assert nd["mass"] == gm
+        else:
+            assert cnt > cap
+    # every fully refined node of > cap particles survives
+    assert sum(1 for a in full if a["end"] - a["start"] > cap) == sum(1 for a in t if a["next"] > 0)
 
 
 def test_tree_duplicates_and_single():
