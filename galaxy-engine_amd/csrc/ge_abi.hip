@@ -16,10 +16,13 @@ using namespace ge;
 
 namespace {
 
+thread_local uint64_t t_alloc_bytes = 0;  // bytes allocated by this thread's current ge_create
+
 template <typename T>
 hipError_t dalloc(T *&p, size_t count) {
     p = nullptr;
     if (count == 0) count = 1;
+    t_alloc_bytes += sizeof(T) * count;
     return hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * count);
 }
 
@@ -477,7 +480,10 @@ int ge_create(ge_ctx **out, const ge_config *cfg) {
         delete c;
         return GE_E_HIP;
     }
-    if (int r = alloc_all(c)) {
+    t_alloc_bytes = 0;
+    const int ra = alloc_all(c);
+    c->device_bytes = t_alloc_bytes;
+    if (int r = ra) {
         free_all(c);
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -503,6 +509,12 @@ int ge_destroy(ge_ctx *c) {
 }
 
 const char *ge_last_error(const ge_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int ge_memory_bytes(const ge_ctx *c, uint64_t *bytes) {
+    if (!c || !bytes) return GE_E_INVALID;
+    *bytes = c->device_bytes;
+    return GE_OK;
+}
 
 int ge_count(const ge_ctx *c, uint32_t *n) {
     if (!c || !n) return GE_E_INVALID;

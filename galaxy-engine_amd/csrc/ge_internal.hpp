@@ -156,6 +156,7 @@ struct ge_ctx {
     bool have_grid = false;
     float sph_cell = 3.0f;         // cell size grid A was built with
     uint32_t leaf_cap = 0;         // leaf rule of the next tree build (0 = reference)
+    uint64_t device_bytes = 0;     // device memory the context allocated
     uint64_t layout_version = 0;   // bumped whenever the particle layout changes
     // hipGraph of two frames (the state and sort buffers alternate, so a pair
     // returns every buffer pointer to where it started)

@@ -138,3 +138,17 @@ def config4_stream(rate, window_s=60.0, seed=4, lo=2000, hi=500_000):
         kind = "sph" if rng.random() < 0.3 else "gravity"
         out.append((t, n, steps, kind, int(rng.integers(0, 2**31))))
     return out
+
+
+def config4_request(n, kind, seed):
+    """One served request: a config-0 disk of n particles (kind "gravity"),
+    or a config-2 impact scaled to n particles (kind "sph": 80% target, 20%
+    impactor, radii scaled by sqrt(n / 250,000))."""
+    if kind == "sph":
+        nt = int(round(0.8 * n))
+        k = math.sqrt(n / 250_000.0)
+        st, prm = config2(seed=seed, n_target=nt, n_impactor=n - nt, r_target=300.0 * k,
+                          r_impactor=120.0 * k)
+        return st, prm
+    st = _state(*exp_disk(_rng(seed), int(n) - 1, 100.0, 600.0, 5e4))
+    return st, dict(G=1.0, theta=0.7, softening=1.0, dt=0.01)

@@ -26,7 +26,7 @@ EXPORTS = [
     "ge_compute_gravity", "ge_integrate_start", "ge_integrate_end", "ge_sph_new_grid",
     "ge_sph_pcisph", "ge_get_keys", "ge_get_perm", "ge_get_node_count", "ge_get_nodes",
     "ge_get_interactions", "ge_get_neighbors", "ge_profile_enable", "ge_profile_read",
-    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll",
+    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes",
 ]
 PHASES = ["integrate_bbox", "keys", "sort", "gather", "tree", "gravity", "sph_grid", "sph",
           "integrate_end"]
@@ -138,6 +138,7 @@ def load_library():
                                         P(GeParticles)]),
         "ge_download_system": (C.c_int, [vp, C.c_uint32, P(GeParticles)]),
         "ge_poll": (C.c_int, [vp, P(C.c_int32)]),
+        "ge_memory_bytes": (C.c_int, [vp, P(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -277,6 +278,11 @@ class Context:
         d = C.c_int32()
         self._check(self.lib.ge_poll(self.h, C.byref(d)))
         return bool(d.value)
+
+    def memory_bytes(self):
+        b = C.c_uint64()
+        self._check(self.lib.ge_memory_bytes(self.h, C.byref(b)))
+        return b.value
 
     def system_offsets(self):
         ns = C.c_uint32()

@@ -131,6 +131,8 @@ int ge_upload(ge_ctx *ctx, uint32_t n, const ge_particles *p);
 /* replaces reading myParam.pParticles back; order is the device order */
 int ge_download(ge_ctx *ctx, const ge_particles *p);
 int ge_count(const ge_ctx *ctx, uint32_t *n);
+/* device memory held by the context (all buffers, fixed at ge_create) */
+int ge_memory_bytes(const ge_ctx *ctx, uint64_t *bytes);
 /* Batched form (no counterpart in the reference, which steps one scene):
  * nsys independent systems, particles concatenated system by system
  * (counts[s] >= 1 each).  Every stage then runs once for all systems; each

@@ -13,8 +13,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "ge_physics.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+SERVE_HEADER = os.path.join(ROOT, "include", "ge_serve.h")
+
+
+def declared_functions(header=HEADER):
+    text = open(header).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(ge_[a-z_0-9]+)\s*\(", text, re.M)))
 
 
@@ -26,6 +29,38 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(galaxy_step.lib.EXPORTS)
+
+
+def test_library_exports_serve_header():
+    from galaxy_step import serve
+    lib = serve._lib()
+    names = declared_functions(SERVE_HEADER)
+    assert set(names) == set(serve.SERVE_EXPORTS)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_serve_struct_layout_matches_header(tmp_path):
+    from galaxy_step import serve as S
+    src = tmp_path / "sz2.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ge_serve.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ge_tune_result), '
+                   'sizeof(ge_server_config), sizeof(ge_request), sizeof(ge_request_result), '
+                   'sizeof(ge_server_stats), offsetof(ge_request, out), offsetof(ge_server_stats, rounds));'
+                   'return 0;}\n')
+    exe = tmp_path / "sz2"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    want = [C.sizeof(S.TuneResult), C.sizeof(S.ServerConfig), C.sizeof(S.Request),
+            C.sizeof(S.RequestResult), C.sizeof(S.ServerStats), S.Request.out.offset,
+            S.ServerStats.rounds.offset]
+    assert [int(v) for v in out] == want
+
+
+def test_server_default_config():
+    from galaxy_step import serve
+    cfg = serve.default_server_config()
+    assert cfg.hbm_budget_bytes == 200 << 30 and cfg.batching == 1 and cfg.max_systems == 512
 
 
 def test_struct_layout_matches_header(tmp_path):
