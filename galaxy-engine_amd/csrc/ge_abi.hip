@@ -55,6 +55,9 @@ void free_state(State &s) {
 }
 
 void free_all(ge_ctx *c) {
+    if (c->diag) (void)hipFree(c->diag);
+    c->diag = nullptr;
+    c->diag_cap = 0;
     Buffers &b = c->b;
     free_state(b.st[0]);
     free_state(b.st[1]);

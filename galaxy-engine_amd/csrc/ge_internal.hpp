@@ -157,6 +157,8 @@ struct ge_ctx {
     float sph_cell = 3.0f;         // cell size grid A was built with
     uint32_t leaf_cap = 0;         // leaf rule of the next tree build (0 = reference)
     uint64_t device_bytes = 0;     // device memory the context allocated
+    double *diag = nullptr;        // FP64 diagnostics scratch (allocated on first use)
+    size_t diag_cap = 0;
     uint64_t layout_version = 0;   // bumped whenever the particle layout changes
     // hipGraph of two frames (the state and sort buffers alternate, so a pair
     // returns every buffer pointer to where it started)

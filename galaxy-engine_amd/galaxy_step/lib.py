@@ -26,7 +26,7 @@ EXPORTS = [
     "ge_compute_gravity", "ge_integrate_start", "ge_integrate_end", "ge_sph_new_grid",
     "ge_sph_pcisph", "ge_get_keys", "ge_get_perm", "ge_get_node_count", "ge_get_nodes",
     "ge_get_interactions", "ge_get_neighbors", "ge_profile_enable", "ge_profile_read",
-    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes",
+    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes", "ge_diagnostics", "ge_direct_fp64",
 ]
 PHASES = ["integrate_bbox", "keys", "sort", "gather", "tree", "gravity", "sph_grid", "sph",
           "integrate_end"]
@@ -48,6 +48,12 @@ class GeConfig(C.Structure):
 class GeParticles(C.Structure):
     _fields_ = [(f, C.POINTER(C.c_uint32) if f in FIELDS_U else C.POINTER(C.c_float))
                 for f in PARTICLE_ORDER]
+
+
+class GeDiag(C.Structure):
+    _fields_ = [("mass", C.c_double), ("px", C.c_double), ("py", C.c_double), ("lz", C.c_double),
+                ("kinetic", C.c_double), ("potential", C.c_double), ("comx", C.c_double),
+                ("comy", C.c_double)]
 
 
 class StepParams(C.Structure):
@@ -139,6 +145,9 @@ def load_library():
         "ge_download_system": (C.c_int, [vp, C.c_uint32, P(GeParticles)]),
         "ge_poll": (C.c_int, [vp, P(C.c_int32)]),
         "ge_memory_bytes": (C.c_int, [vp, P(C.c_uint64)]),
+        "ge_diagnostics": (C.c_int, [vp, C.c_uint32, P(StepParams), C.c_int32, P(GeDiag)]),
+        "ge_direct_fp64": (C.c_int, [vp, C.c_uint32, P(C.c_uint32), P(StepParams), P(C.c_double),
+                                     P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -278,6 +287,21 @@ class Context:
         d = C.c_int32()
         self._check(self.lib.ge_poll(self.h, C.byref(d)))
         return bool(d.value)
+
+    def diagnostics(self, sys=0, params=None, potential=False):
+        d = GeDiag()
+        self._check(self.lib.ge_diagnostics(self.h, int(sys), C.byref(params) if params is not None else None,
+                                            1 if potential else 0, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in d._fields_}
+
+    def direct_fp64(self, targets, params):
+        t = np.ascontiguousarray(targets, dtype=np.uint32)
+        ax = np.zeros(len(t), np.float64)
+        ay = np.zeros(len(t), np.float64)
+        dp = C.POINTER(C.c_double)
+        self._check(self.lib.ge_direct_fp64(self.h, len(t), _uptr(t), C.byref(params),
+                                            ax.ctypes.data_as(dp), ay.ctypes.data_as(dp)))
+        return ax, ay
 
     def memory_bytes(self):
         b = C.c_uint64()

@@ -207,6 +207,26 @@ int ge_get_interactions(ge_ctx *ctx, uint64_t *count);
 int ge_get_neighbors(ge_ctx *ctx, int32_t which, int64_t *off, uint32_t *idx,
                      int64_t cap, int64_t *total);
 
+/* ---- FP64 diagnostics (no counterpart in the reference, whose UI shows
+ * particle counts and FPS only, GE/src/UI/UI.cpp) ----------------------------- */
+typedef struct {
+    double mass;             /* sum m */
+    double px, py;           /* momentum, sum m v */
+    double lz;               /* angular momentum about the origin, sum m (x vy - y vx) */
+    double kinetic;          /* sum m |v|^2 / 2 */
+    double potential;        /* -G sum_{i<j} m_i m_j / sqrt(r^2 + eps^2) (with_potential only) */
+    double comx, comy;       /* centre of mass */
+} ge_diag;
+/* FP64 reductions over system sys of the current state; the potential is an
+ * O(N^2) FP64 direct sum with the walk's Plummer softening */
+int ge_diagnostics(ge_ctx *ctx, uint32_t sys, const ge_step_params *p, int32_t with_potential,
+                   ge_diag *out);
+/* FP64 direct-sum accelerations of the particles in slots targets[k] from
+ * every other particle of their system: G m_j d / (|d|^2 + eps^2)^1.5, the
+ * force the tree walk approximates (the reference to measure its error) */
+int ge_direct_fp64(ge_ctx *ctx, uint32_t ntargets, const uint32_t *targets,
+                   const ge_step_params *p, double *ax, double *ay);
+
 /* ---- tracing (no counterpart in the reference, which has only an FPS
  * counter, GE/src/UI/UI.cpp:1317-1324) -------------------------------------- */
 /* phases timed with hipEvents on the context stream */

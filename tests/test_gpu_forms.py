@@ -197,3 +197,41 @@ def test_systems_update_rejects_bad_requests():
         assert c.count() == 6
         c.systems_update([])
         assert c.count() == 0
+
+
+def test_direct_fp64_matches_oracle():
+    st, _ = configs.config0(seed=26, n=8000)
+    p = _params()
+    rng = np.random.default_rng(3)
+    with Context(8000) as c:
+        c.upload(st)
+        c.step(p, 1)
+        c.sync()
+        d = c.download(["x", "y", "mass"])
+        t = rng.choice(8000, 300, replace=False).astype(np.uint32)
+        gx, gy = c.direct_fp64(t, p)
+    rx, ry = O.direct_fp64(d["x"], d["y"], d["mass"], 1.0, 1.0, t)
+    mag = np.hypot(rx, ry)
+    assert np.all(np.hypot(gx - rx, gy - ry) <= 1e-11 * mag + 1e-14 * mag.max())
+
+
+def test_diagnostics_match_host_fp64():
+    st, _ = configs.config0(seed=27, n=3000)
+    p = _params()
+    with Context(3000) as c:
+        c.upload(st)
+        c.step(p, 2)
+        c.sync()
+        d = c.download(["x", "y", "vx", "vy", "mass"])
+        g = c.diagnostics(0, p, potential=True)
+    x, y, vx, vy, m = (d[k].astype(np.float64) for k in ("x", "y", "vx", "vy", "mass"))
+    want = {"mass": m.sum(), "px": (m * vx).sum(), "py": (m * vy).sum(), "lz": (m * (x * vy - y * vx)).sum(),
+            "kinetic": 0.5 * (m * (vx * vx + vy * vy)).sum(), "comx": (m * x).sum() / m.sum(),
+            "comy": (m * y).sum() / m.sum()}
+    dx = x[:, None] - x[None, :]
+    dy = y[:, None] - y[None, :]
+    inv = 1.0 / np.sqrt(dx * dx + dy * dy + 1.0)
+    np.fill_diagonal(inv, 0.0)
+    want["potential"] = -0.5 * float(m @ inv @ m)
+    for k, v in want.items():
+        assert g[k] == pytest.approx(v, rel=1e-10, abs=1e-9 * abs(want["kinetic"]) if k in ("px", "py", "lz") else 0), k
