@@ -69,7 +69,7 @@ void free_all(ge_ctx *c) {
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
     dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
-    dfree(b.nodes); dfree(b.node_geom); dfree(b.node_quad); dfree(b.moments); dfree(b.moment_part); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
+    dfree(b.nodes); dfree(b.node_geom); dfree(b.node_mp); dfree(b.mp_cnt); dfree(b.mp_wave_list); dfree(b.mp_big_list); dfree(b.mp_piece_owner); dfree(b.mp_piece_sum); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
         dfree(g->keys); dfree(g->keys_alt); dfree(g->first); dfree(g->ord); dfree(g->frame);
         dfree(g->pk0); dfree(g->pk1); dfree(g->pkf);
@@ -124,9 +124,6 @@ int alloc_all(ge_ctx *c) {
     b.node_cap = 4u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(b.nodes, b.node_cap));
     GE_HIP_CHECK(c, dalloc(b.node_geom, b.node_cap));
-    GE_HIP_CHECK(c, dalloc(b.node_quad, b.node_cap));
-    GE_HIP_CHECK(c, dalloc(b.moments, 6ull * ((size_t)cap + c->max_sys + 1)));
-    GE_HIP_CHECK(c, dalloc(b.moment_part, 6ull * ((size_t)cap / kChunk + c->max_sys + 2)));
     if (c->sph) {
         GE_HIP_CHECK(c, dalloc(b.sphf, 2ull * cap));
         GE_HIP_CHECK(c, dalloc(b.sph_pp, 4ull * cap));
@@ -285,7 +282,7 @@ int check_params(ge_ctx *c, const ge_step_params *p) {
     if (p->sph && !c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
     if (p->sph && c->nsys > 1) return fail(c, GE_E_UNSUPPORTED, "SPH runs on single-system contexts");
     if (p->sph && !(p->sph_cell > 0.0f && p->sph_h > 0.0f)) return fail(c, GE_E_INVALID, "sph_h and sph_cell must be > 0");
-    if (p->accuracy != 0 && p->accuracy != 1) return fail(c, GE_E_INVALID, "accuracy must be 0 or 1");
+    if (p->accuracy < 0 || p->accuracy > 2) return fail(c, GE_E_INVALID, "accuracy must be 0, 1 or 2");
     if (p->accuracy && !(p->theta > 0.0f)) return fail(c, GE_E_INVALID, "accuracy mode needs theta > 0");
     if (p->leaf_capacity > 64) return fail(c, GE_E_INVALID, "leaf_capacity must be 0 (reference rule) or 1..64");
     if (p->walk_block != 0 && p->walk_block != 64 && p->walk_block != 128 && p->walk_block != 256)
@@ -299,10 +296,10 @@ int check_device_errors(ge_ctx *c) {
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     if (e) {
         char buf[200];
-        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s%s%s)", e,
+        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s%s%s%s)", e,
                       (e & kErrNodeCap) ? "node capacity " : "", (e & kErrDepth) ? "tree depth " : "",
                       (e & kErrGridRange) ? "SPH grid range " : "", (e & kErrNeighborCap) ? "neighbour cap " : "",
-                      (e & kErrFrontier) ? "tree frontier" : "");
+                      (e & kErrFrontier) ? "tree frontier " : "", (e & kErrMultipole) ? "multipole lists" : "");
         c->err = buf;
         uint32_t z = 0;
         (void)hipMemcpy(c->b.counters + kCntError, &z, sizeof z, hipMemcpyHostToDevice);
@@ -364,7 +361,7 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
         if (use_direct(c, p) || !c->have_tree) {
             launch_direct(c, p, end_kick);
         } else {
-            if (p->accuracy) launch_quad_moments(c, p->theta);
+            if (p->accuracy) launch_multipoles(c, p->theta);
             launch_bh(c, p, end_kick, count_inter);
         }
     } else {
@@ -649,6 +646,9 @@ int ge_step(ge_ctx *c, const ge_step_params *p, int32_t nsteps) {
     if (nsteps < 0) return fail(c, GE_E_INVALID, "nsteps < 0");
     if (c->n == 0 || nsteps == 0) return GE_OK;
     (void)hipSetDevice(c->device);
+    if (p->accuracy) {
+        if (int r = ensure_multipole_buffers(c)) return r;
+    }
     int32_t k = 0;
     if (p->use_graph && !c->prof && nsteps >= 3) {
         // the first frame integrates with dt_start, so it runs eagerly
@@ -738,7 +738,10 @@ int ge_compute_gravity(ge_ctx *c, const ge_step_params *p) {
     } else if (use_direct(c, p) || !c->have_tree) {
         launch_direct(c, p, 0.0f);
     } else {
-        if (p->accuracy) launch_quad_moments(c, p->theta);
+        if (p->accuracy) {
+            if (int r = ensure_multipole_buffers(c)) return r;
+            launch_multipoles(c, p->theta);
+        }
         launch_bh(c, p, 0.0f, true);
     }
     return ge_sync(c);

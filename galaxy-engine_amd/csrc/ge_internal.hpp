@@ -35,6 +35,7 @@ enum : uint32_t {
     kErrGridRange = 4u,  // SPH grid cannot index the coordinates
     kErrNeighborCap = 8u,
     kErrFrontier = 16u,  // workgroup subtree frontier or level bound exceeded
+    kErrMultipole = 32u, // multipole work lists overflowed
 };
 
 // counters block (device, uint32 each)
@@ -127,9 +128,11 @@ struct Buffers {
     uint32_t *scan_part = nullptr;    // block sums of the node-start scan
     NodeRec *nodes = nullptr;
     float4 *node_geom = nullptr;   // (posx, posy, size, 0) per final node
-    float4 *node_quad = nullptr;   // accuracy mode: (Qxx, Qxy, Qyy, opening radius)
-    double *moments = nullptr;     // accuracy mode: 6 prefix moments per particle (n + 1)
-    double *moment_part = nullptr; // block sums of the moment scan
+    float4 *node_mp = nullptr;     // accuracy modes: 6 float4 multipole record per node
+    uint32_t *mp_cnt = nullptr;    // accuracy modes: list counters of the moment pass
+    uint32_t *mp_wave_list = nullptr, *mp_big_list = nullptr, *mp_piece_owner = nullptr;
+    double *mp_piece_sum = nullptr;
+    uint32_t mp_wave_cap = 0, mp_big_cap = 0, mp_piece_cap = 0;
     uint32_t node_cap = 0;
     float *sphf = nullptr;         // SPH force scratch (x, y interleaved), 2 per particle
     float *sph_pp = nullptr;       // grid-A order (predX, predY, restDens, sphMass)
@@ -189,7 +192,8 @@ void launch_morton_keys(ge_ctx *c);
 void launch_sort_keys(ge_ctx *c);
 void launch_gather(ge_ctx *c);
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_dt, bool count);
-void launch_quad_moments(ge_ctx *c, float theta);
+int ensure_multipole_buffers(ge_ctx *c);
+void launch_multipoles(ge_ctx *c, float theta);
 void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
 void launch_zero_acc(ge_ctx *c);
 void launch_integrate_end(ge_ctx *c, float dt);

@@ -1,202 +1,313 @@
-// ge_multipole.hip -- node quadrupole moments and opening radii for the
-// accuracy mode (ge_step_params.accuracy = 1).  This mode has no counterpart
-// in the reference, whose walk uses monopoles only (physics.cpp:250-259).
+// ge_multipole.hip -- node multipole moments and opening radii for the
+// accuracy modes (ge_step_params.accuracy = 1: quadrupole, 2: up to the
+// hexadecapole).  Neither has a counterpart in the reference, whose walk uses
+// monopoles only (physics.cpp:250-259).
 //
-// Quadrupole of a node about its centre of mass c, for the 1/r kernel of the
-// reference restricted to the plane:
-//   Q = sum_i m_i (3 s_i s_i^T - |s_i|^2 I),  s_i = x_i - c.
-// Over the Morton-sorted particles every node is a contiguous range, so its
-// raw moments (sum m, m x, m y, m x^2, m x y, m y^2, taken about the box
-// centre) are differences of FP64 prefix sums, and the central moments
-// follow in O(1) per node.  The opening radius is size/theta + |c - centre of
-// the node square| (Barnes 1994), compared against the plain distance.
+// Every node is a contiguous range of the Morton-sorted particles.  Its raw
+// moments S_ab = sum m (x - c)^a (y - c)^b (a + b <= 4) are summed in FP64
+// directly about the node's float centre of mass c, so the offsets are at
+// most the node's own size and no term cancels against a larger one:
+//   - a node of at most kMpThreadMax particles: one thread;
+//   - up to kMpWaveMax particles: one wave, lanes striding the range, then a
+//     fixed-order wave reduction;
+//   - larger: pieces of kMpWaveMax particles, one wave each, summed per node
+//     in piece order.
+// The lists of wave and piece nodes are filled with atomics, but every
+// node's sums have a fixed order, so the result does not depend on the
+// schedule.
+//
+// The expansion centre is the FP64 centre of mass of the node (rounded once
+// to float), so the dipole vanishes to rounding.  The opening radius is the
+// larger of size/theta + |COM - centre of the node square| and b/theta, b
+// the largest distance of the node's particles from the COM (Barnes 1994),
+// compared against the plain distance.  The second bound matters because the
+// reference's float partition points can leave a particle up to half a node
+// outside its node's square (when its Morton key and its float coordinate
+// disagree about a midline); a truncated series about such a node diverges
+// for targets closer than that particle.  Record per node, 6 float4:
+//   0: (comx, comy, opening radius, mass)
+//   1: (C2xx/2, C2xy/2, C2yy/2, tr C2/2)
+//   2: C3 / 6: (xxx, xxy, xyy, yyy)
+//   3: (T_x / 6, T_y / 6, W / 24, 0)        T_l = C3_iil,  W = C4_iijj
+//   4: C4 / 24: (xxxx, xxxy, xxyy, xyyy)
+//   5: (C4yyyy / 24, U_xx / 24, U_xy / 24, U_yy / 24)   U_lm = C4_iilm
+// with C_n the central moments.  The walk evaluates the Taylor expansion of
+// the softened kernel (|d|^2 + eps^2)^-1/2 to the order asked for.
 #include "ge_internal.hpp"
 
 namespace ge {
 namespace {
 
-constexpr int kMomThreads = 1024;
-constexpr int kMomItems = kChunk / kMomThreads;  // one workgroup per reduction chunk
+constexpr int kMono = 15;                 // monomials x^a y^b, a + b <= 4
+constexpr int kPiece = kMono + 1;         // per piece: the monomial sums and the largest offset
+constexpr uint32_t kMpThreadMax = 32;     // nodes summed by one thread
+constexpr uint32_t kMpWaveMax = 4096;     // nodes summed by one wave; pieces above
+constexpr int kMpBlock = 256;
 
-struct Mom {
-    double v[6];
-};
+// counters of the multipole pass (mp_cnt)
+enum : int { kMpWaveCount = 0, kMpBigCount = 1, kMpPieceCount = 2, kMpSlots = 4 };
 
-// raw moments are taken about the centre of the particle's system box
-__device__ __forceinline__ void box_centre(const float *__restrict__ sys_bb, uint32_t sy, double &ox, double &oy) {
-    const float *bb = sys_bb + 8ull * sy;
-    ox = (double)bb[0] + 0.5 * (double)bb[2];
-    oy = (double)bb[1] + 0.5 * (double)bb[2];
-}
+__host__ __device__ constexpr int mono(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
 
-__device__ __forceinline__ Mom mom_of(const float4 *__restrict__ pxym, double ox, double oy, uint32_t i, uint32_t hi) {
-    Mom r;
-    if (i < hi) {
-        const float4 p = pxym[i];
-        const double m = p.z, x = (double)p.x - ox, y = (double)p.y - oy;
-        r.v[0] = m; r.v[1] = m * x; r.v[2] = m * y; r.v[3] = m * x * x; r.v[4] = m * x * y; r.v[5] = m * y * y;
-    } else {
-        for (int c = 0; c < 6; ++c) r.v[c] = 0.0;
+// adds the monomials of particle p about (cx, cy) to S; r2 = largest squared offset
+__device__ __forceinline__ void mono_add(const float4 p, double cx, double cy, double *__restrict__ S, double &r2) {
+    const double m = p.z, x = (double)p.x - cx, y = (double)p.y - cy;
+    r2 = fmax(r2, x * x + y * y);
+    double xp[5], yp[5];
+    xp[0] = m;
+    yp[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) {
+        xp[k] = xp[k - 1] * x;
+        yp[k] = yp[k - 1] * y;
     }
-    return r;
+#pragma unroll
+    for (int a = 0; a <= 4; ++a)
+#pragma unroll
+        for (int b = 0; a + b <= 4; ++b) S[mono(a, b)] += xp[a] * yp[b];
 }
 
-// Exclusive workgroup scan of a 6-vector with a fixed addition tree: the
-// prefix of an item depends only on the items before it.  tot = group total.
-__device__ Mom block_scan6(Mom v, Mom &tot) {
-    __shared__ double sh[kMomThreads / 64 + 1][6];
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    Mom incl = v;
-    for (int d = 1; d < 64; d <<= 1) {
-        for (int c = 0; c < 6; ++c) {
-            const double u = __shfl_up(incl.v[c], d, 64);
-            if (lane >= (uint32_t)d) incl.v[c] += u;
+// fixed-order wave sum (xor butterfly) of S and wave max of r2: every lane
+// ends with the totals
+__device__ __forceinline__ void wave_sum(double *__restrict__ S, double &r2) {
+#pragma unroll
+    for (int c = 0; c < kMono; ++c) {
+        double v = S[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        S[c] = v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+}
+
+// T = S shifted by w: T_ab = sum_{i<=a, j<=b} C(a,i) C(b,j) (-w_x)^(a-i) (-w_y)^(b-j) S_ij
+__device__ __forceinline__ void mom_shift_add(const double *__restrict__ S, double wx, double wy, double *__restrict__ T) {
+    constexpr double binom[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
+    double px[5], py[5];
+    px[0] = py[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) {
+        px[k] = px[k - 1] * -wx;
+        py[k] = py[k - 1] * -wy;
+    }
+#pragma unroll
+    for (int a = 0; a <= 4; ++a)
+#pragma unroll
+        for (int b = 0; a + b <= 4; ++b) {
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i <= a; ++i)
+#pragma unroll
+                for (int j = 0; j <= b; ++j) t += binom[a][i] * binom[b][j] * px[a - i] * py[b - j] * S[mono(i, j)];
+            T[mono(a, b)] += t;
         }
-    }
-    Mom excl;
-    for (int c = 0; c < 6; ++c) {
-        const double u = __shfl_up(incl.v[c], 1, 64);
-        excl.v[c] = lane == 0 ? 0.0 : u;
-    }
-    if (lane == 63)
-        for (int c = 0; c < 6; ++c) sh[w][c] = incl.v[c];
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        const int c = threadIdx.x;
-        double run = 0.0;
-        for (int k = 0; k < kMomThreads / 64; ++k) {
-            const double t = sh[k][c];
-            sh[k][c] = run;
-            run += t;
-        }
-        sh[kMomThreads / 64][c] = run;
-    }
-    __syncthreads();
-    Mom ex;
-    for (int c = 0; c < 6; ++c) {
-        ex.v[c] = w == 0 ? excl.v[c] : sh[w][c] + excl.v[c];
-        tot.v[c] = sh[kMomThreads / 64][c];
-    }
-    __syncthreads();
-    return ex;
 }
 
-// Per chunk (chunks never span systems and start at the system's first
-// particle + multiples of kChunk): thread t owns items lo + t*kMomItems ...
-__global__ __launch_bounds__(kMomThreads) void k_mom_sums(const float4 *__restrict__ pxym,
-                                                          const uint32_t *__restrict__ chunk_lo,
-                                                          const uint32_t *__restrict__ sys_of,
-                                                          const float *__restrict__ sys_bb, double *__restrict__ part) {
-    const uint32_t lo = chunk_lo[blockIdx.x], hi = chunk_lo[blockIdx.x + 1];
-    Mom tot;
-    for (int c = 0; c < 6; ++c) tot.v[c] = 0.0;
-    if (lo < hi) {
-        double ox, oy;
-        box_centre(sys_bb, sys_of[lo], ox, oy);
-        const uint32_t base = lo + threadIdx.x * kMomItems;
-        Mom s;
-        for (int c = 0; c < 6; ++c) s.v[c] = 0.0;
-        for (int k = 0; k < kMomItems; ++k) {
-            const Mom m = mom_of(pxym, ox, oy, base + k, hi);
-            for (int c = 0; c < 6; ++c) s.v[c] += m.v[c];
-        }
-        block_scan6(s, tot);
-    }
-    if (threadIdx.x == 0)
-        for (int c = 0; c < 6; ++c) part[6 * blockIdx.x + c] = tot.v[c];
+// Central moments about the exact centre of mass from S (about the float
+// centre of mass r.comx, r.comy), packed into the node's record.
+__device__ void finish_record(uint32_t k, const NodeRec &r, const float4 g, const double *__restrict__ S, double r2,
+                              float inv_theta, float4 *__restrict__ mp) {
+    const double cx0 = r.comx, cy0 = r.comy;
+    const double M = S[mono(0, 0)];
+    const double dx = M > 0.0 ? S[mono(1, 0)] / M : 0.0, dy = M > 0.0 ? S[mono(0, 1)] / M : 0.0;
+    const float comx = (float)(cx0 + dx), comy = (float)(cy0 + dy);
+    double C[kMono];
+#pragma unroll
+    for (int c = 0; c < kMono; ++c) C[c] = 0.0;
+    mom_shift_add(S, (double)comx - cx0, (double)comy - cy0, C);
+    const float half = 0.5f * g.z;
+    const float ox = comx - (g.x + half), oy = comy - (g.y + half);
+    // largest particle offset from the exact COM, bounded through the float one
+    const double bmax = sqrt(r2) + sqrt(((double)comx - cx0) * ((double)comx - cx0) +
+                                        ((double)comy - cy0) * ((double)comy - cy0));
+    const float lim = fmaxf(g.z * inv_theta + sqrtf(ox * ox + oy * oy), (float)(bmax * (double)inv_theta) * 1.0000002f);
+    const double c2xx = C[mono(2, 0)], c2xy = C[mono(1, 1)], c2yy = C[mono(0, 2)];
+    const double c3xxx = C[mono(3, 0)], c3xxy = C[mono(2, 1)], c3xyy = C[mono(1, 2)], c3yyy = C[mono(0, 3)];
+    const double c4xxxx = C[mono(4, 0)], c4xxxy = C[mono(3, 1)], c4xxyy = C[mono(2, 2)];
+    const double c4xyyy = C[mono(1, 3)], c4yyyy = C[mono(0, 4)];
+    const double uxx = c4xxxx + c4xxyy, uxy = c4xxxy + c4xyyy, uyy = c4xxyy + c4yyyy;
+    float4 *o = mp + 6ull * k;
+    o[0] = make_float4(comx, comy, lim, (float)M);
+    o[1] = make_float4((float)(0.5 * c2xx), (float)(0.5 * c2xy), (float)(0.5 * c2yy), (float)(0.5 * (c2xx + c2yy)));
+    o[2] = make_float4((float)(c3xxx / 6.0), (float)(c3xxy / 6.0), (float)(c3xyy / 6.0), (float)(c3yyy / 6.0));
+    o[3] = make_float4((float)((c3xxx + c3xyy) / 6.0), (float)((c3xxy + c3yyy) / 6.0), (float)((uxx + uyy) / 24.0),
+                       0.0f);
+    o[4] = make_float4((float)(c4xxxx / 24.0), (float)(c4xxxy / 24.0), (float)(c4xxyy / 24.0), (float)(c4xyyy / 24.0));
+    o[5] = make_float4((float)(c4yyyy / 24.0), (float)(uxx / 24.0), (float)(uxy / 24.0), (float)(uyy / 24.0));
 }
 
-// chunk offsets within each system, chunks in order (one thread per system)
-__global__ __launch_bounds__(kBlock) void k_mom_parts(double *__restrict__ part, const uint32_t *__restrict__ chunk_first,
-                                                      uint32_t nsys) {
-    for (uint32_t sy = blockIdx.x * kBlock + threadIdx.x; sy < nsys; sy += gridDim.x * kBlock) {
-        double run[6] = {0, 0, 0, 0, 0, 0};
-        for (uint32_t k = chunk_first[sy]; k < chunk_first[sy + 1]; ++k) {
-            for (int c = 0; c < 6; ++c) {
-                const double t = part[6 * k + c];
-                part[6 * k + c] = run[c];
-                run[c] += t;
+__global__ void k_mp_reset(uint32_t *__restrict__ cnt) {
+    if (threadIdx.x < kMpSlots) cnt[threadIdx.x] = 0u;
+}
+
+// Thread per node: small nodes are finished here; the others are listed.
+__global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict__ nodes,
+                                                       const float4 *__restrict__ geom,
+                                                       const uint32_t *__restrict__ counters,
+                                                       const float4 *__restrict__ pxym, float inv_theta,
+                                                       uint32_t *__restrict__ cnt, uint32_t *__restrict__ wave_list,
+                                                       uint32_t wave_cap, uint32_t *__restrict__ big_list,
+                                                       uint32_t *__restrict__ big_first, uint32_t big_cap,
+                                                       uint32_t *__restrict__ piece_owner, uint32_t piece_cap,
+                                                       uint32_t *__restrict__ err, float4 *__restrict__ mp) {
+    const uint32_t nn = counters[kCntNodeTotal];
+    for (uint32_t k = blockIdx.x * kMpBlock + threadIdx.x; k < nn; k += gridDim.x * kMpBlock) {
+        const NodeRec r = nodes[k];
+        const uint32_t count = r.end > r.start ? r.end - r.start : 0u;
+        if (count <= kMpThreadMax) {
+            double S[kMono];
+#pragma unroll
+            for (int c = 0; c < kMono; ++c) S[c] = 0.0;
+            const double cx = r.comx, cy = r.comy;
+            double r2 = 0.0;
+            for (uint32_t j = r.start; j < r.end; ++j) mono_add(pxym[j], cx, cy, S, r2);
+            finish_record(k, r, geom[k], S, r2, inv_theta, mp);
+        } else if (count <= kMpWaveMax) {
+            const uint32_t w = atomicAdd(&cnt[kMpWaveCount], 1u);
+            if (w < wave_cap) wave_list[w] = k;
+            else atomicOr(err, kErrMultipole);
+        } else {
+            const uint32_t pieces = (count + kMpWaveMax - 1) / kMpWaveMax;
+            const uint32_t b = atomicAdd(&cnt[kMpBigCount], 1u);
+            const uint32_t f = atomicAdd(&cnt[kMpPieceCount], pieces);
+            if (b < big_cap && f + pieces <= piece_cap) {
+                big_list[b] = k;
+                big_first[b] = f;
+                for (uint32_t q = 0; q < pieces; ++q) piece_owner[f + q] = b;
+            } else {
+                atomicOr(err, kErrMultipole);
             }
         }
     }
 }
 
-// mom[6 (i + s) + c] = sum over the particles of system s before slot i,
-// for i = off_s .. off_{s+1} (one extra entry per system)
-__global__ __launch_bounds__(kMomThreads) void k_mom_apply(const float4 *__restrict__ pxym,
-                                                           const uint32_t *__restrict__ chunk_lo,
-                                                           const uint32_t *__restrict__ sys_of,
-                                                           const uint32_t *__restrict__ sys_off,
-                                                           const float *__restrict__ sys_bb,
-                                                           const double *__restrict__ part,
-                                                           double *__restrict__ mom) {
-    const uint32_t lo = chunk_lo[blockIdx.x], hi = chunk_lo[blockIdx.x + 1];
-    if (lo >= hi) return;
-    const uint32_t sy = sys_of[lo];
-    double ox, oy;
-    box_centre(sys_bb, sy, ox, oy);
-    const uint32_t base = lo + threadIdx.x * kMomItems;
-    Mom item[kMomItems];
-    Mom s;
-    for (int c = 0; c < 6; ++c) s.v[c] = 0.0;
-    for (int k = 0; k < kMomItems; ++k) {
-        item[k] = mom_of(pxym, ox, oy, base + k, hi);
-        for (int c = 0; c < 6; ++c) s.v[c] += item[k].v[c];
-    }
-    Mom tot;
-    Mom run = block_scan6(s, tot);
-    for (int c = 0; c < 6; ++c) run.v[c] += part[6 * blockIdx.x + c];
-    const bool last_chunk = hi == sys_off[sy + 1];
-    for (int k = 0; k < kMomItems; ++k) {
-        const uint32_t i = base + k;
-        if (i < hi || (last_chunk && i == hi))
-            for (int c = 0; c < 6; ++c) mom[6ull * (i + sy) + c] = run.v[c];
-        for (int c = 0; c < 6; ++c) run.v[c] += item[k].v[c];
+// Wave per listed node (kMpThreadMax < count <= kMpWaveMax).
+__global__ __launch_bounds__(kMpBlock) void k_mp_wave(const NodeRec *__restrict__ nodes,
+                                                      const float4 *__restrict__ geom,
+                                                      const float4 *__restrict__ pxym, float inv_theta,
+                                                      const uint32_t *__restrict__ cnt,
+                                                      const uint32_t *__restrict__ wave_list, uint32_t wave_cap,
+                                                      float4 *__restrict__ mp) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nw = min(cnt[kMpWaveCount], wave_cap);
+    const uint32_t waves = gridDim.x * (kMpBlock / 64);
+    for (uint32_t e = blockIdx.x * (kMpBlock / 64) + (threadIdx.x >> 6); e < nw; e += waves) {
+        const uint32_t k = wave_list[e];
+        const NodeRec r = nodes[k];
+        double S[kMono];
+#pragma unroll
+        for (int c = 0; c < kMono; ++c) S[c] = 0.0;
+        const double cx = r.comx, cy = r.comy;
+        double r2 = 0.0;
+        for (uint32_t j = r.start + lane; j < r.end; j += 64) mono_add(pxym[j], cx, cy, S, r2);
+        wave_sum(S, r2);
+        if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_node_quad(const NodeRec *__restrict__ nodes,
-                                                      const float4 *__restrict__ geom,
-                                                      const uint32_t *__restrict__ counters,
-                                                      const double *__restrict__ mom,
-                                                      const uint32_t *__restrict__ sys_of,
-                                                      const float *__restrict__ sys_bb,
-                                                      float inv_theta, float4 *__restrict__ quad) {
-    const uint32_t nn = counters[kCntNodeTotal];
-    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < nn; k += gridDim.x * kBlock) {
+// Wave per piece of a large node: sums about the node's float centre of mass.
+__global__ __launch_bounds__(kMpBlock) void k_mp_pieces(const NodeRec *__restrict__ nodes,
+                                                        const float4 *__restrict__ pxym,
+                                                        const uint32_t *__restrict__ cnt,
+                                                        const uint32_t *__restrict__ big_list,
+                                                        const uint32_t *__restrict__ big_first, uint32_t big_cap,
+                                                        const uint32_t *__restrict__ piece_owner, uint32_t piece_cap,
+                                                        double *__restrict__ piece_sum) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t np = min(cnt[kMpPieceCount], piece_cap);
+    const uint32_t nb = min(cnt[kMpBigCount], big_cap);
+    const uint32_t waves = gridDim.x * (kMpBlock / 64);
+    for (uint32_t e = blockIdx.x * (kMpBlock / 64) + (threadIdx.x >> 6); e < np; e += waves) {
+        const uint32_t b = piece_owner[e];
+        if (b >= nb) continue;
+        const uint32_t k = big_list[b];
         const NodeRec r = nodes[k];
-        const uint32_t sy = sys_of[r.start];
-        double ox, oy;
-        box_centre(sys_bb, sy, ox, oy);
-        const float4 g = geom[k];
-        const double *a = mom + 6ull * (r.start + sy);
-        const double *b = mom + 6ull * (r.end + sy);
-        const double M = b[0] - a[0], Sx = b[1] - a[1], Sy = b[2] - a[2];
-        const double Sxx = b[3] - a[3], Sxy = b[4] - a[4], Syy = b[5] - a[5];
-        const double cx = (double)r.comx - ox, cy = (double)r.comy - oy;
-        const double Cxx = Sxx - 2.0 * cx * Sx + cx * cx * M;
-        const double Cyy = Syy - 2.0 * cy * Sy + cy * cy * M;
-        const double Cxy = Sxy - cx * Sy - cy * Sx + cx * cy * M;
-        const float half = 0.5f * g.z;
-        const float ddx = r.comx - (g.x + half), ddy = r.comy - (g.y + half);
-        const float lim = g.z * inv_theta + sqrtf(ddx * ddx + ddy * ddy);
-        quad[k] = make_float4((float)(2.0 * Cxx - Cyy), (float)(3.0 * Cxy), (float)(2.0 * Cyy - Cxx), lim);
+        const uint32_t lo = r.start + (e - big_first[b]) * kMpWaveMax;
+        const uint32_t hi = min(lo + kMpWaveMax, r.end);
+        double S[kMono];
+#pragma unroll
+        for (int c = 0; c < kMono; ++c) S[c] = 0.0;
+        const double cx = r.comx, cy = r.comy;
+        double r2 = 0.0;
+        for (uint32_t j = lo + lane; j < hi; j += 64) mono_add(pxym[j], cx, cy, S, r2);
+        wave_sum(S, r2);
+        if (lane <= kMono) {
+            double v = r2;  // slot kMono: largest squared offset
+#pragma unroll
+            for (int c = 0; c < kMono; ++c)
+                if ((int)lane == c) v = S[c];
+            piece_sum[(size_t)kPiece * e + lane] = v;
+        }
+    }
+}
+
+// Thread per large node: its pieces in order.
+__global__ __launch_bounds__(kMpBlock) void k_mp_big(const NodeRec *__restrict__ nodes,
+                                                     const float4 *__restrict__ geom, float inv_theta,
+                                                     const uint32_t *__restrict__ cnt,
+                                                     const uint32_t *__restrict__ big_list,
+                                                     const uint32_t *__restrict__ big_first, uint32_t big_cap,
+                                                     uint32_t piece_cap, const double *__restrict__ piece_sum,
+                                                     float4 *__restrict__ mp) {
+    const uint32_t nb = min(cnt[kMpBigCount], big_cap);
+    const uint32_t np = cnt[kMpPieceCount];
+    if (np > piece_cap) return;  // the error flag is already raised
+    for (uint32_t b = blockIdx.x * kMpBlock + threadIdx.x; b < nb; b += gridDim.x * kMpBlock) {
+        const uint32_t k = big_list[b];
+        const NodeRec r = nodes[k];
+        const uint32_t pieces = (r.end - r.start + kMpWaveMax - 1) / kMpWaveMax;
+        double S[kMono];
+#pragma unroll
+        for (int c = 0; c < kMono; ++c) S[c] = 0.0;
+        double r2 = 0.0;
+        for (uint32_t q = 0; q < pieces; ++q) {
+            const double *ps = piece_sum + (size_t)kPiece * (big_first[b] + q);
+#pragma unroll
+            for (int c = 0; c < kMono; ++c) S[c] += ps[c];
+            r2 = fmax(r2, ps[kMono]);
+        }
+        finish_record(k, r, geom[k], S, r2, inv_theta, mp);
     }
 }
 
 }  // namespace
 
-void launch_quad_moments(ge_ctx *c, float theta) {
-    hipLaunchKernelGGL(k_mom_sums, dim3(c->nchunks), dim3(kMomThreads), 0, c->stream, c->b.pxym, c->b.chunk_lo,
-                       c->b.sys_of, c->b.sys_bb, c->b.moment_part);
-    hipLaunchKernelGGL(k_mom_parts, dim3(blocks_for(c->nsys, kBlock)), dim3(kBlock), 0, c->stream, c->b.moment_part,
-                       c->b.chunk_first, c->nsys);
-    hipLaunchKernelGGL(k_mom_apply, dim3(c->nchunks), dim3(kMomThreads), 0, c->stream, c->b.pxym, c->b.chunk_lo,
-                       c->b.sys_of, c->b.sys_off, c->b.sys_bb, c->b.moment_part, c->b.moments);
-    uint32_t nb = blocks_for(c->b.node_cap, kBlock);
-    if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(k_node_quad, dim3(nb), dim3(kBlock), 0, c->stream, c->b.nodes, c->b.node_geom, c->b.counters,
-                       c->b.moments, c->b.sys_of, c->b.sys_bb, 1.0f / theta, c->b.node_quad);
+// Device buffers of the accuracy modes, allocated on first use (never inside
+// a graph capture: ge_step calls this before capturing).  A node above
+// kMpWaveMax particles lies on a tree level holding at most n / kMpWaveMax
+// such nodes, so the pieces are bounded by 2 n / kMpWaveMax per level.
+int ensure_multipole_buffers(ge_ctx *c) {
+    Buffers &b = c->b;
+    if (b.node_mp) return GE_OK;
+    b.mp_wave_cap = b.node_cap;
+    b.mp_big_cap = (uint32_t)(((uint64_t)c->cap / kMpWaveMax + 1) * 64 + 1024);
+    b.mp_piece_cap = (uint32_t)(((uint64_t)c->cap / kMpWaveMax + 1) * 128 + 1024);
+    GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.node_mp), sizeof(float4) * 6ull * b.node_cap));
+    GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.mp_cnt), sizeof(uint32_t) * kMpSlots));
+    GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.mp_wave_list), sizeof(uint32_t) * b.mp_wave_cap));
+    GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.mp_big_list), sizeof(uint32_t) * 2ull * b.mp_big_cap));
+    GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.mp_piece_owner), sizeof(uint32_t) * b.mp_piece_cap));
+    GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.mp_piece_sum), sizeof(double) * kPiece * b.mp_piece_cap));
+    return GE_OK;
+}
+
+void launch_multipoles(ge_ctx *c, float theta) {
+    Buffers &b = c->b;
+    const float inv_theta = 1.0f / theta;
+    uint32_t *big_first = b.mp_big_list + b.mp_big_cap;
+    hipLaunchKernelGGL(k_mp_reset, dim3(1), dim3(64), 0, c->stream, b.mp_cnt);
+    uint32_t nb = blocks_for(b.node_cap, kMpBlock);
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_mp_small, dim3(nb), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.counters, b.pxym,
+                       inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.mp_big_list, big_first, b.mp_big_cap,
+                       b.mp_piece_owner, b.mp_piece_cap, b.counters + kCntError, b.node_mp);
+    hipLaunchKernelGGL(k_mp_wave, dim3(2048), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.pxym, inv_theta,
+                       b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.node_mp);
+    hipLaunchKernelGGL(k_mp_pieces, dim3(1024), dim3(kMpBlock), 0, c->stream, b.nodes, b.pxym, b.mp_cnt, b.mp_big_list,
+                       big_first, b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.mp_piece_sum);
+    hipLaunchKernelGGL(k_mp_big, dim3(64), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, inv_theta, b.mp_cnt,
+                       b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_cap, b.mp_piece_sum, b.node_mp);
 }
 
 }  // namespace ge

@@ -208,34 +208,39 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // wave-uniform (scalar loads).  The acceptance test uses the reference's
 // unfused arithmetic so the accepted-node sets match the CPU path.
 //
-// QUAD (accuracy mode, no counterpart in the reference): the node is accepted
-// when |d| exceeds its opening radius (size/theta + COM offset) and adds its
-// quadrupole term  -G Q d / R^5 + 5/2 G (d^T Q d - eps^2 tr Q) d / R^7,
-// R^2 = |d|^2 + eps^2, to the softened monopole.
+// ORDER 2 / 4 (accuracy modes, no counterpart in the reference): the node is
+// accepted when |d| exceeds its opening radius (size/theta + COM offset) and
+// contributes the Taylor expansion of the softened kernel about its FP64
+// centre of mass up to the quadrupole (2) or hexadecapole (4) term, with
+// R^2 = |d|^2 + eps^2 and D_k = (-1)^k (2k-1)!! R^-(2k+1):
+//   a = G [M D1 d + D3 d (d.A2.d) + D2 (tr A2 d + 2 A2 d)
+//          - D4 d (A3:ddd) - 3 D3 (A3:dd + (T.d) d) - 3 D2 T
+//          + D5 d (A4:dddd) + D4 (4 A4:ddd + 6 (U:dd) d) + D3 (3 W d + 12 U d)]
+// (A_n = C_n / n!, T, U, W traces of A3, A4; d = target - COM).
 //
 // Several systems: system s owns the node range [P[off_s], P[off_s+1]) (P is
 // the node-start prefix of the tree build); each lane is active only inside
 // its own system's range, and the wave walks from its lanes' first root to
 // their last end.
-template <int BS, bool KAHAN, bool COUNT, bool QUAD>
+template <int BS, bool KAHAN, bool COUNT, int ORDER>
 __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
-                                               const NodeRec *__restrict__ nodes,
-                                               const float4 *__restrict__ quad,
-                                               const uint32_t *__restrict__ start_prefix,
-                                               const uint32_t *__restrict__ sys_of,
-                                               const uint32_t *__restrict__ sys_off,
-                                               const uint32_t *__restrict__ counters, uint32_t n,
-                                               float G, float theta_sq, float soft_sq,
-                                               float *__restrict__ ax, float *__restrict__ ay,
-                                               float *__restrict__ vx, float *__restrict__ vy,
-                                               const uint32_t *__restrict__ flags, float half_dt_end,
-                                               unsigned long long *__restrict__ inter) {
+                                           const NodeRec *__restrict__ nodes,
+                                           const float4 *__restrict__ mp,
+                                           const uint32_t *__restrict__ start_prefix,
+                                           const uint32_t *__restrict__ sys_of,
+                                           const uint32_t *__restrict__ sys_off,
+                                           const uint32_t *__restrict__ counters, uint32_t n,
+                                           float G, float theta_sq, float soft_sq,
+                                           float *__restrict__ ax, float *__restrict__ ay,
+                                           float *__restrict__ vx, float *__restrict__ vy,
+                                           const uint32_t *__restrict__ flags, float half_dt_end,
+                                           unsigned long long *__restrict__ inter) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * BS + threadIdx.x;
     const bool live = i < n;
     const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     // a tree that raised an error flag may hold unwritten nodes: do not walk it
-    const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth);
+    const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | (ORDER > 0 ? kErrMultipole : 0u));
     const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
     uint32_t root = kNone, end = 0u;
     if (live) {
@@ -266,14 +271,14 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
         const bool active = idx >= skip_until && idx < end;
         float dx, dy, dist_sq;
         bool accept;
-        float4 qd;
-        if (QUAD) {
-            qd = quad[idx];
-            dx = nd.comx - p.x;
-            dy = nd.comy - p.y;
+        float4 h0;
+        if (ORDER > 0) {
+            h0 = mp[6ull * idx];
+            dx = h0.x - p.x;
+            dy = h0.y - p.y;
             const float d2 = dx * dx + dy * dy;
             dist_sq = d2 + soft_sq;
-            accept = d2 > qd.w * qd.w;
+            accept = d2 > h0.z * h0.z;
         } else {
 #pragma clang fp contract(off)
             dx = nd.comx - p.x;
@@ -286,14 +291,38 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
             const float inv2 = inv * inv;
             const float inv3 = inv2 * inv;
             float tx, ty;
-            if (QUAD) {
+            if (ORDER > 0) {
+                // d = target - COM
+                const float qx = -dx, qy = -dy;
                 const float inv5 = inv3 * inv2, inv7 = inv5 * inv2;
-                const float qdx = qd.x * dx + qd.y * dy, qdy = qd.y * dx + qd.z * dy;
-                const float dqd = dx * qdx + dy * qdy - soft_sq * (qd.x + qd.z);
-                const float mono = nd.mass * inv3;
-                const float rad = 2.5f * dqd * inv7;
-                tx = G * ((mono + rad) * dx - qdx * inv5);
-                ty = G * ((mono + rad) * dy - qdy * inv5);
+                const float D1 = -inv3, D2 = 3.0f * inv5, D3 = -15.0f * inv7;
+                const float4 h1 = mp[6ull * idx + 1];
+                const float aqx = h1.x * qx + h1.y * qy, aqy = h1.y * qx + h1.z * qy;
+                const float qaq = qx * aqx + qy * aqy;
+                float sx = h0.w * D1 * qx + D3 * qx * qaq + D2 * (h1.w * qx + 2.0f * aqx);
+                float sy = h0.w * D1 * qy + D3 * qy * qaq + D2 * (h1.w * qy + 2.0f * aqy);
+                if (ORDER >= 4) {
+                    const float inv9 = inv7 * inv2, inv11 = inv9 * inv2;
+                    const float D4 = 105.0f * inv9, D5 = -945.0f * inv11;
+                    const float4 h2 = mp[6ull * idx + 2], h3 = mp[6ull * idx + 3];
+                    const float4 h4 = mp[6ull * idx + 4], h5 = mp[6ull * idx + 5];
+                    const float qxx = qx * qx, qxy = qx * qy, qyy = qy * qy;
+                    const float bqx = h2.x * qxx + 2.0f * h2.y * qxy + h2.z * qyy;
+                    const float bqy = h2.y * qxx + 2.0f * h2.z * qxy + h2.w * qyy;
+                    const float bqqq = qx * bqx + qy * bqy;
+                    const float tq = h3.x * qx + h3.y * qy;
+                    sx -= D4 * qx * bqqq + 3.0f * D3 * (bqx + qx * tq) + 3.0f * D2 * h3.x;
+                    sy -= D4 * qy * bqqq + 3.0f * D3 * (bqy + qy * tq) + 3.0f * D2 * h3.y;
+                    const float eqx = h4.x * qxx * qx + 3.0f * h4.y * qxx * qy + 3.0f * h4.z * qx * qyy + h4.w * qyy * qy;
+                    const float eqy = h4.y * qxx * qx + 3.0f * h4.z * qxx * qy + 3.0f * h4.w * qx * qyy + h5.x * qyy * qy;
+                    const float eqqqq = qx * eqx + qy * eqy;
+                    const float uqx = h5.y * qx + h5.z * qy, uqy = h5.z * qx + h5.w * qy;
+                    const float uqq = qx * uqx + qy * uqy;
+                    sx += D5 * qx * eqqqq + D4 * (4.0f * eqx + 6.0f * qx * uqq) + D3 * (3.0f * h3.z * qx + 12.0f * uqx);
+                    sy += D5 * qy * eqqqq + D4 * (4.0f * eqy + 6.0f * qy * uqq) + D3 * (3.0f * h3.z * qy + 12.0f * uqy);
+                }
+                tx = G * sx;
+                ty = G * sy;
             } else {
                 const float f = G * nd.mass * inv3;
                 tx = dx * f;
@@ -309,7 +338,7 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
             if (COUNT) ++cnt;
             skip_until = after;
         }
-        const bool leaf = next == 0;
+    const bool leaf = next == 0;
         const bool open_leaf = active && !accept && leaf;
         if (__ballot(open_leaf)) {
             // leaf direct sum over [start, end), skipping i == j (physics.cpp:163-196)
@@ -479,29 +508,32 @@ void launch_gather(ge_ctx *c) {
     c->b.cur = 1 - cur;
 }
 
-template <int BS, bool KAHAN, bool COUNT, bool QUAD>
+template <int BS, bool KAHAN, bool COUNT, int ORDER>
 void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const State &s = c->b.st[c->b.cur];
     const float theta_sq = p->theta * p->theta;
     const float soft_sq = p->softening * p->softening;
     const uint32_t nb = blocks_for(c->n, BS);
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
-    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, QUAD>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym, c->b.nodes,
-                       c->b.node_quad, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n, p->G,
+    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym, c->b.nodes,
+                       c->b.node_mp, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n, p->G,
                        theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, inter);
+}
+
+template <int BS, int ORDER>
+void launch_bh_o(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
+    const bool k = p->compensated != 0;
+    if (k && count) launch_bh_t<BS, true, true, ORDER>(c, p, end_kick_half_dt);
+    else if (k) launch_bh_t<BS, true, false, ORDER>(c, p, end_kick_half_dt);
+    else if (count) launch_bh_t<BS, false, true, ORDER>(c, p, end_kick_half_dt);
+    else launch_bh_t<BS, false, false, ORDER>(c, p, end_kick_half_dt);
 }
 
 template <int BS>
 void launch_bh_bs(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
-    const bool k = p->compensated != 0, q = p->accuracy != 0;
-    if (k && count && q) launch_bh_t<BS, true, true, true>(c, p, end_kick_half_dt);
-    else if (k && count) launch_bh_t<BS, true, true, false>(c, p, end_kick_half_dt);
-    else if (k && q) launch_bh_t<BS, true, false, true>(c, p, end_kick_half_dt);
-    else if (k) launch_bh_t<BS, true, false, false>(c, p, end_kick_half_dt);
-    else if (count && q) launch_bh_t<BS, false, true, true>(c, p, end_kick_half_dt);
-    else if (count) launch_bh_t<BS, false, true, false>(c, p, end_kick_half_dt);
-    else if (q) launch_bh_t<BS, false, false, true>(c, p, end_kick_half_dt);
-    else launch_bh_t<BS, false, false, false>(c, p, end_kick_half_dt);
+    if (p->accuracy == 2) launch_bh_o<BS, 4>(c, p, end_kick_half_dt, count);
+    else if (p->accuracy == 1) launch_bh_o<BS, 2>(c, p, end_kick_half_dt, count);
+    else launch_bh_o<BS, 0>(c, p, end_kick_half_dt, count);
 }
 
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {

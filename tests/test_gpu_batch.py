@@ -43,7 +43,7 @@ def _split(d, off):
     return [{k: v[off[s]:off[s + 1]] for k, v in d.items()} for s in range(len(off) - 1)]
 
 
-@pytest.mark.parametrize("accuracy", [0, 1])
+@pytest.mark.parametrize("accuracy", [0, 1, 2])
 def test_batch_bitidentical_to_alone(accuracy):
     sizes = [7000, 1, 2, 130, 64, 65, 20000, 4097, 3]
     systems = _systems(sizes)

@@ -31,7 +31,7 @@ def _eq(a, b, msg=""):
         np.testing.assert_array_equal(a[f], b[f], err_msg=f"{msg} field {f}")
 
 
-@pytest.mark.parametrize("accuracy", [0, 1])
+@pytest.mark.parametrize("accuracy", [0, 1, 2])
 def test_graph_replay_bitidentical_to_eager(accuracy):
     st, _ = configs.config0(seed=21, n=20000)
     n = len(st["x"])

@@ -265,6 +265,39 @@ def test_accuracy_mode_two_disks_theta07():
     assert rms_rel < 0.25 * rms_rel0
 
 
+def test_hexadecapole_mode_two_disks_theta07():
+    # accuracy 2: moments to the hexadecapole; the per-particle RMS error vs
+    # the FP64 direct sum at theta = 0.7 must stay below 1e-3
+    st, _ = configs.config1(n=1_000_000)
+    xs, ys, ms, bb, d, inter = _run_gravity(st, dict(accuracy=2), 0.7)
+    t = np.sort(np.random.default_rng(1).choice(len(xs), 4096, replace=False)).astype(np.uint32)
+    rms_rel, rms_norm = _fp64_errors(d, xs, ys, ms, t)
+    print(f"theta=0.7 hexadecapole mode: per-particle RMS {rms_rel:.3e}, force-normalised RMS {rms_norm:.3e}")
+    assert rms_rel < 1e-3
+    assert rms_norm < 2e-4
+
+
+@pytest.mark.parametrize("accuracy", [1, 2])
+def test_multipole_modes_converge_to_direct(accuracy):
+    # theta -> 0 opens every node: the walk becomes the direct sum
+    st, _ = configs.config0(seed=31, n=5000)
+    xs, ys, ms, bb, d, inter = _run_gravity(st, dict(accuracy=accuracy), 0.02)
+    t = np.arange(len(xs), dtype=np.uint32)
+    rms_rel, _ = _fp64_errors(d, xs, ys, ms, t)
+    assert rms_rel < 2e-5
+
+
+def test_hexadecapole_beats_quadrupole_per_node_order():
+    # same tree and opening test: the higher order must cut the error
+    st, _ = configs.config0(seed=32, n=40000)
+    t = np.arange(0, 40000, 7, dtype=np.uint32)
+    errs = []
+    for acc in (0, 1, 2):
+        xs, ys, ms, bb, d, inter = _run_gravity(st, dict(accuracy=acc), 0.7)
+        errs.append(_fp64_errors(d, xs, ys, ms, t)[0])
+    assert errs[2] < 0.5 * errs[1] < 0.5 * errs[0], errs
+
+
 def test_accuracy_mode_disk_per_particle():
     st, _ = configs.config0()
     xs, ys, ms, bb, d, inter = _run_gravity(st, dict(accuracy=1), 0.7)
