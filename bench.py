@@ -1,17 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark: particle-updates/s of the 2D Barnes-Hut physics step on one
-MI355X, on the 1,000,000-particle two-disk collision at theta = 0.7
-(eps = 1, dt = 0.005, G = 1, synthetic seeded input).
+"""Benchmark of the 2D physics step on one MI355X.
+
+Headline (`value`): particle-updates/s of the 1,000,000-particle two-disk
+collision at theta = 0.7 (eps = 1, dt = 0.005, G = 1, seeded synthetic input)
+in the accuracy form: moments to the hexadecapole, Kahan-compensated FP32
+accumulation, leaf capacity and walk block size picked by the autotuner.  Its
+RMS force error against an FP64 direct sum (4096 sampled particles, on the
+device) is reported beside it, together with:
+
+  - the plain form: the reference's monopole walk and leaf rule on the same
+    input (the parity path), with its own error;
+  - the CPU baseline: the oracle's OpenMP C restatement of the reference step
+    on this host, on a bounded number of frames of the same workload;
+  - the batched form: 512 independent systems of 1,000 .. 131,072 particles
+    in one context, 100 steps, against the same systems solved one by one;
+  - the served form: Poisson request streams at 50, 200 and 800 requests/s
+    through the continuous-batching server, against the plain form of the
+    server (one request on the device at a time); p50/p99 time to
+    completion, solves/s and particle-updates/s.
 
 A step is one full frame through the C ABI (ge_step): integrateStart +
-bounding box, Morton keys, radix sort, gather, quadtree build, BH walk with
-the fused integrateEnd kick.  The state stays resident in HBM; the timed
-region covers exactly K frames between two device synchronisations.
-
-The JSON line also carries the roofline of the dominant kernel (the BH walk:
-VALU FP32 bound, 14 flops per interaction, interaction count measured by the
-kernel itself) and a CPU baseline: the oracle's OpenMP restatement of the
-reference step timed on this host on a bounded sample of the same workload.
+bounding box, Morton keys, radix sort, gather, quadtree build, node moments,
+tree walk with the fused integrateEnd kick.  State stays resident in HBM; the
+timed region covers exactly K frames between two device synchronisations,
+with hipEvent pairs around every phase on the context's stream, from which
+the roofline of the dominant kernel (the walk) is computed.
 """
 import argparse
 import json
@@ -25,7 +38,12 @@ for _p in (os.path.join(ROOT, "galaxy-engine_amd"), os.path.join(ROOT, "oracle")
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
-FLOPS_PER_INTERACTION = 14          # dx, dy, r^2+eps^2, rsqrt, r^-3, G*M*r^-3, two accumulates
+# Algorithmic FP32 flops per interaction of the walk (Kahan and opening tests
+# excluded): a leaf partner or a monopole node is dx, dy, r^2 + eps^2, rsqrt,
+# r^-3, G m r^-3 and two accumulates = 14; a quadrupole node 45 and a node to
+# the hexadecapole 160, counted from the expression the walk evaluates.
+FLOPS_PAIR = 14
+FLOPS_NODE = {0: 14, 1: 45, 2: 160}
 PEAK_FP32_TFLOPS = 157.3            # MI355X vector FP32
 PEAK_HBM_GBS = 8000.0
 
@@ -37,13 +55,23 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--theta", type=float, default=0.7)
-    ap.add_argument("--compensated", type=int, default=1)
+    ap.add_argument("--accuracy", type=int, default=2, help="0 reference walk, 1 quadrupole, 2 hexadecapole")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bound on the CPU-baseline sample (0 disables it)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-rates", type=str, default="50,200,800")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain elementwise copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def calibrate_copy(mib):
@@ -86,29 +114,42 @@ def cpu_baseline(st, prm, budget_s):
             break
     n = len(st["x"])
     return {"value": n * steps / el, "unit": "particle-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} full frames of the same {n}-particle workload (theta={prm['theta']}) "
-                      f"through the OpenMP C restatement of the reference step, {el:.1f} s"}
+            "sample": f"{steps} full frames of the same {n}-particle workload (theta={prm['theta']}, "
+                      f"reference walk) through the OpenMP C restatement of the reference step, {el:.1f} s"}
 
 
-def main():
-    a = parse()
-    if a.calibrate_copy_mib:
-        calibrate_copy(a.calibrate_copy_mib)
-        return
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(independent systems shard with no communication; see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
+def force_errors(c, p, n, samples=4096, seed=0):
+    """Per-particle and force-normalised RMS error of the current accelerations
+    against the FP64 direct sum on the device, on `samples` sampled particles."""
     import numpy as np
-    from galaxy_step import Context, configs, default_params
+    c.compute_gravity(p)
+    d = c.download(["ax", "ay"])
+    t = np.sort(np.random.default_rng(seed).choice(n, min(samples, n), replace=False)).astype(np.uint32)
+    rx, ry = c.direct_fp64(t, p)
+    e2 = (d["ax"][t] - rx) ** 2 + (d["ay"][t] - ry) ** 2
+    a2 = rx ** 2 + ry ** 2
+    return {"rms_per_particle": float(np.sqrt(np.mean(e2 / a2))),
+            "rms_force_normalised": float(np.sqrt(np.mean(e2) / np.mean(a2))),
+            "samples": int(len(t))}
 
-    st, prm = configs.config1(n=a.n, theta=a.theta)
+
+def single_system(a, st, prm, accuracy, tune):
+    """Times K frames of one form on the 1M system; returns a dict."""
+    import numpy as np
+    from galaxy_step import Context, default_params
+    from galaxy_step import serve as S
     n = len(st["x"])
     p = default_params(G=prm["G"], theta=a.theta, softening=prm["softening"], dt=prm["dt"],
-                       dt_start=prm["dt"], compensated=a.compensated)
-    print(f"[bench] n={n} theta={a.theta} warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+                       dt_start=prm["dt"], compensated=1, accuracy=accuracy, direct_below=0)
+    out = {}
     with Context(n) as c:
         c.upload(st)
+        if tune:
+            leaf, block, r = S.autotune(c, p, frames=3)
+            p.leaf_capacity, p.walk_block = leaf, block
+            out["autotune"] = {"leaf_capacity": leaf, "walk_block": block, "ms_per_frame": r.ms_per_frame,
+                               "reference_leaf_rule_ms": r.reference_ms,
+                               "table_ms": [[round(v, 4) for v in row] for row in r.table_ms]}
         c.step(p, a.warmup)
         c.sync()
         c.profile_enable(True)
@@ -118,46 +159,208 @@ def main():
         t1 = time.perf_counter()
         phases = c.profile_read()
         c.profile_enable(False)
-        # interaction count of the current tree (the walk counts itself)
-        c.compute_gravity(p)
-        inter = c.interactions()
-        nodes = c.node_count()
+        # the same frames replayed through a hipGraph
+        p.use_graph = 1
+        c.step(p, 3)
+        c.sync()
+        tg0 = time.perf_counter()
+        c.step(p, a.steps)
+        c.sync()
+        tg1 = time.perf_counter()
+        p.use_graph = 0
+        out["errors"] = force_errors(c, p, n)
+        nodes_acc, pairs = c.interaction_counts()
+        out["node_count"] = c.node_count()
     el = t1 - t0
-    ms_step = el * 1e3 / a.steps
-    value = n * a.steps / el
+    out["ms_per_step"] = el * 1e3 / a.steps
+    out["value"] = n * a.steps / el
+    out["graph_ms_per_step"] = (tg1 - tg0) * 1e3 / a.steps
+    out["phases"] = phases
     grav_ms, grav_launches = phases["gravity"]
-    bh_s = grav_ms * 1e-3 / max(grav_launches, 1)
-    achieved = FLOPS_PER_INTERACTION * inter / bh_s / 1e12
-    phase_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in phases.items() if v[1]}
-    print(f"[bench] {ms_step:.3f} ms/step, BH walk {bh_s * 1e3:.3f} ms, I/N={inter / n:.1f}, "
-          f"nodes={nodes}, phases(ms/launch)={phase_ms}", file=sys.stderr, flush=True)
+    out["walk_ms"] = grav_ms / max(grav_launches, 1)
+    flops = FLOPS_NODE[accuracy] * nodes_acc + FLOPS_PAIR * pairs
+    out["walk_tflops"] = flops / (out["walk_ms"] * 1e-3) / 1e12
+    out["interactions"] = {"nodes": nodes_acc, "pairs": pairs, "flops": flops,
+                           "per_particle": (nodes_acc + pairs) / n}
+    out["params"] = {"accuracy": accuracy, "leaf_capacity": p.leaf_capacity, "walk_block": p.walk_block}
+    return out
+
+
+def batched(a, accuracy):
+    """cfg3: the batched form against the same systems one by one."""
+    import numpy as np
+    from galaxy_step import Context, configs, default_params
+    sizes, scales = configs.config3_sizes(systems=a.batch_systems)
+    systems = [configs.config3_system(k, int(sizes[k]), float(scales[k]))[0] for k in range(len(sizes))]
+    keys = ("x", "y", "vx", "vy", "mass")
+    batch = {k: np.concatenate([s[k] for s in systems]) for k in keys}
+    total = int(sizes.sum())
+    p = default_params(G=1.0, theta=0.7, softening=1.0, dt=0.01, dt_start=0.01, compensated=1,
+                       accuracy=accuracy, direct_below=0, leaf_capacity=16, use_graph=1)
+    steps = a.batch_steps
+    log(f"batched: {len(sizes)} systems, {total} particles, {steps} steps")
+    with Context(total, max_systems=len(sizes)) as c:
+        c.upload(batch, counts=sizes)
+        c.step(p, 4)  # warm the code paths and the graph
+        c.sync()
+        t0 = time.perf_counter()
+        c.upload(batch, counts=sizes)
+        c.step(p, steps)
+        c.sync()
+        _ = c.download(["x", "y", "vx", "vy"])
+        t1 = time.perf_counter()
+        # per-system force error vs FP64 direct sum on up to 256 particles each
+        off = c.system_offsets()
+        c.compute_gravity(p)
+        d = c.download(["ax", "ay"])
+        rng = np.random.default_rng(3)
+        tgt, owner = [], []
+        for s in range(len(sizes)):
+            m = int(off[s + 1] - off[s])
+            pick = rng.choice(m, min(256, m), replace=False) + off[s]
+            tgt.append(pick)
+            owner.append(np.full(len(pick), s))
+        tgt = np.concatenate(tgt).astype(np.uint32)
+        owner = np.concatenate(owner)
+        rx, ry = c.direct_fp64(tgt, p)
+    e2 = (d["ax"][tgt] - rx) ** 2 + (d["ay"][tgt] - ry) ** 2
+    rel2 = e2 / (rx ** 2 + ry ** 2)
+    per_sys = np.sqrt(np.bincount(owner, rel2) / np.bincount(owner))
+    t_batch = t1 - t0
+    # plain form: every system alone on one context, same parameters
+    t0 = time.perf_counter()
+    with Context(int(sizes.max())) as c:
+        for s in systems:
+            c.upload(s)
+            c.step(p, steps)
+            c.sync()
+            _ = c.download(["x", "y", "vx", "vy"])
+    t_plain = time.perf_counter() - t0
+    ns = len(sizes)
+    return {"workload": f"{ns} systems, 1,000..131,072 particles (log-uniform), {total} particles, "
+                        f"{steps} steps, theta 0.7",
+            "batched_solves_per_s": ns / t_batch, "plain_solves_per_s": ns / t_plain,
+            "speedup": t_plain / t_batch,
+            "batched_particle_updates_per_s": total * steps / t_batch,
+            "plain_particle_updates_per_s": total * steps / t_plain,
+            "batched_s": t_batch, "plain_s": t_plain,
+            "force_error_rms_per_particle": {"median_system": float(np.median(per_sys)),
+                                             "max_system": float(per_sys.max())},
+            "params": {"accuracy": accuracy, "leaf_capacity": 16, "graphs": True}}
+
+
+def served(a, accuracy):
+    """cfg4: Poisson request streams through the server, batched and plain."""
+    import numpy as np
+    from galaxy_step import configs, default_params
+    from galaxy_step import serve as S
+    pg = default_params(G=1.0, theta=0.7, softening=1.0, dt=0.01, dt_start=0.01, compensated=1,
+                        accuracy=accuracy, direct_below=0, leaf_capacity=16)
+    _, sp = configs.config2(n_target=10, n_impactor=10)
+    ps = default_params(direct_below=0, compensated=1, accuracy=accuracy, leaf_capacity=16)
+    for k in ("G", "theta", "softening", "dt", "sph", "sph_h", "sph_cell", "sph_unit_mass", "viscosity",
+              "cohesion_coef", "delta", "stiff_mult", "sph_max_vel"):
+        setattr(ps, k, sp[k])
+    ps.dt_start = ps.dt
+    # one request mix (sizes log-uniform in [2,000, 500,000], steps uniform in
+    # [10, 200], 30 % gravity + SPH), replayed with each rate's arrivals
+    rng = np.random.default_rng(4)
+    mix = []
+    for k in range(a.serve_requests):
+        n = int(round(math.exp(rng.uniform(math.log(2000), math.log(500_000)))))
+        steps = int(rng.integers(10, 201))
+        kind = "sph" if rng.random() < 0.3 else "gravity"
+        st, _ = configs.config4_request(n, kind, seed=int(rng.integers(0, 2 ** 31)))
+        mix.append({"state": st, "steps": steps, "sph": int(kind == "sph")})
+    results = []
+    for rate in [float(r) for r in a.serve_rates.split(",")]:
+        arr = np.cumsum(np.random.default_rng(int(rate) * 7 + 1).exponential(1.0 / rate, len(mix)))
+        reqs = [dict(m, arrival_s=float(t)) for m, t in zip(mix, arr)]
+        row = {"rate_per_s": rate, "requests": len(reqs),
+               "sph_requests": sum(r["sph"] for r in reqs),
+               "particle_updates": int(sum(len(r["state"]["x"]) * r["steps"] for r in reqs))}
+        for form, batching in (("served", 1), ("plain", 0)):
+            cfg = S.default_server_config(batching=batching, sph_slots=4, sph_capacity=1 << 19,
+                                          batch_capacity=1 << 25, max_systems=512, round_frames=8, use_graph=1)
+            with S.Server(cfg, pg, ps) as srv:
+                _, res, stats = srv.run(reqs, ("x",))
+            bad = sum(1 for r in res if r["status"] != 0)
+            row[form] = {"p50_s": stats["p50_s"], "p99_s": stats["p99_s"], "mean_s": stats["mean_s"],
+                         "solves_per_s": stats["solves_per_s"], "particle_updates_per_s": stats["updates_per_s"],
+                         "wall_s": stats["wall_s"], "max_resident": stats["max_resident"],
+                         "device_bytes": stats["device_bytes"], "failed": bad}
+            log(f"served rate {rate}: {form} p50 {stats['p50_s']:.3f}s p99 {stats['p99_s']:.3f}s "
+                f"{stats['solves_per_s']:.1f} solves/s")
+        row["solves_speedup"] = row["served"]["solves_per_s"] / max(row["plain"]["solves_per_s"], 1e-12)
+        results.append(row)
+    return results
+
+
+def main():
+    a = parse()
+    if a.calibrate_copy_mib:
+        calibrate_copy(a.calibrate_copy_mib)
+        return
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              "(independent systems shard with no communication)", file=sys.stderr)
+        sys.exit(2)
+    from galaxy_step import configs
+
+    st, prm = configs.config1(n=a.n, theta=a.theta)
+    n = len(st["x"])
+    log(f"n={n} theta={a.theta} accuracy={a.accuracy} warmup={a.warmup} steps={a.steps}")
+    head = single_system(a, st, prm, a.accuracy, tune=True)
+    log(f"headline: {head['ms_per_step']:.3f} ms/step (graph {head['graph_ms_per_step']:.3f}), walk "
+        f"{head['walk_ms']:.3f} ms, errors {head['errors']}, tune {head.get('autotune', {}).get('leaf_capacity')}")
+    plain = single_system(a, st, prm, 0, tune=False) if a.accuracy != 0 else head
+    log(f"plain: {plain['ms_per_step']:.3f} ms/step, errors {plain['errors']}")
     cpu = None
     if not a.no_cpu and a.cpu_seconds > 0:
         cpu = cpu_baseline(st, dict(prm, theta=a.theta), a.cpu_seconds)
+        log(f"cpu: {cpu['value']:.3e} particle-updates/s")
+    bat = None if a.no_batch else batched(a, a.accuracy)
+    if bat:
+        log(f"batched: {bat['batched_solves_per_s']:.1f} solves/s vs plain {bat['plain_solves_per_s']:.1f} "
+            f"({bat['speedup']:.2f}x)")
+    srv = None if a.no_serve else served(a, a.accuracy)
+    phase_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in head["phases"].items() if v[1]}
+    achieved = head["walk_tflops"]
     out = {
-        "metric": "particle-updates/s, 1M-body BH step theta=0.7",
-        "value": value,
+        "metric": "particle-updates/s, 1M-body BH step theta=0.7; batched solves/s + p99 latency",
+        "value": head["value"],
         "unit": "particle-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded two-disk generator)",
-        "config": {"workload": "2D Barnes-Hut frame, two-disk collision",
-                   "particles": n, "theta": a.theta, "softening": prm["softening"],
-                   "dt": prm["dt"], "G": prm["G"], "compensated": bool(a.compensated),
+        "config": {"workload": "2D Barnes-Hut frame, two-disk collision (1,000,000 particles)",
+                   "particles": n, "theta": a.theta, "softening": prm["softening"], "dt": prm["dt"],
+                   "G": prm["G"], "compensated": True, "accuracy": head["params"]["accuracy"],
+                   "leaf_capacity": head["params"]["leaf_capacity"], "walk_block": head["params"]["walk_block"],
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": traffic_from_profiles(),
-                     "kernel": "k_bh (Barnes-Hut walk)", "kernel_ms": bh_s * 1e3,
-                     "interactions": inter, "flops_per_interaction": FLOPS_PER_INTERACTION},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic_from_profiles() if head["params"]["accuracy"] == 2 else None,
+                     "kernel": "k_bh (tree walk)", "kernel_ms": head["walk_ms"],
+                     "interactions": head["interactions"],
+                     "flops_per_node": FLOPS_NODE[head["params"]["accuracy"]], "flops_per_pair": FLOPS_PAIR},
+        "force_error_vs_fp64": head["errors"],
+        "graph_ms_per_step": head["graph_ms_per_step"],
+        "autotune": head.get("autotune"),
         "phase_ms": phase_ms,
+        "plain_form": {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
+                       "force_error_vs_fp64": plain["errors"], "walk_ms": plain["walk_ms"],
+                       "walk_tflops": plain["walk_tflops"],
+                       "note": "reference monopole walk, opening test and leaf rule (the parity path)"},
         "cpu_baseline": cpu,
+        "batched": bat,
+        "served": srv,
     }
     print(json.dumps(out), flush=True)
 

@@ -100,7 +100,7 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.chunk_first, (size_t)c->max_sys + 1));
     GE_HIP_CHECK(c, dalloc(b.sys_src, (size_t)c->max_sys + 1));
     b.bb = b.sys_bb;  // system 0's box doubles as the single-scene box
-    GE_HIP_CHECK(c, dalloc(b.interactions, 1));
+    GE_HIP_CHECK(c, dalloc(b.interactions, 2));  // all interactions, accepted nodes
     TopNodes &T = b.top;
     T.cap = 2u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(T.px, T.cap)); GE_HIP_CHECK(c, dalloc(T.py, T.cap));
@@ -357,11 +357,15 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
     }
     const float end_kick = p->sph ? 0.0f : p->dt * 0.5f;  // fused integrateEnd when no SPH follows
     if (p->G != 0.0f) {
-        PhaseScope s(c, GE_PHASE_GRAVITY);
         if (use_direct(c, p) || !c->have_tree) {
+            PhaseScope s(c, GE_PHASE_GRAVITY);
             launch_direct(c, p, end_kick);
         } else {
-            if (p->accuracy) launch_multipoles(c, p->theta);
+            if (p->accuracy) {
+                PhaseScope s(c, GE_PHASE_MULTIPOLE);
+                launch_multipoles(c, p->theta);
+            }
+            PhaseScope s(c, GE_PHASE_GRAVITY);
             launch_bh(c, p, end_kick, count_inter);
         }
     } else {
@@ -847,6 +851,17 @@ int ge_get_interactions(ge_ctx *c, uint64_t *count) {
     if (!c || !count) return GE_E_INVALID;
     GE_HIP_CHECK(c, hipMemcpyAsync(count, c->b.interactions, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+int ge_get_interaction_counts(ge_ctx *c, uint64_t *nodes, uint64_t *pairs) {
+    if (!c || !nodes || !pairs) return GE_E_INVALID;
+    (void)hipSetDevice(c->device);
+    uint64_t h[2] = {0, 0};
+    GE_HIP_CHECK(c, hipMemcpyAsync(h, c->b.interactions, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    *nodes = h[1];
+    *pairs = h[0] - h[1];
     return GE_OK;
 }
 

@@ -257,7 +257,7 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
     const uint32_t walk_end = __builtin_amdgcn_readfirstlane(whi);
     uint32_t skip_until = root;
     float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, cnt_nodes = 0;
     uint32_t idx = __builtin_amdgcn_readfirstlane(wlo);
     while (idx < walk_end) {
         const NodeRec nd = nodes[idx];
@@ -335,7 +335,10 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                 fx += tx;
                 fy += ty;
             }
-            if (COUNT) ++cnt;
+            if (COUNT) {
+                ++cnt;
+                ++cnt_nodes;
+            }
             skip_until = after;
         }
     const bool leaf = next == 0;
@@ -365,10 +368,16 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
         idx = __ballot(descend) ? idx + 1u : after;
     }
     if (COUNT) {
-        uint64_t c = cnt;
+        uint64_t c = cnt, cn = cnt_nodes;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
-        if ((threadIdx.x & 63) == 0) atomicAdd(inter, (unsigned long long)c);
+        for (int o = 32; o > 0; o >>= 1) {
+            c += __shfl_xor(c, o, kWave);
+            cn += __shfl_xor(cn, o, kWave);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(inter, (unsigned long long)c);
+            atomicAdd(inter + 1, (unsigned long long)cn);
+        }
     }
     if (!live) return;
     const float accx = KAHAN ? fx - cx : fx;
@@ -420,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_direct(const float4 *__restrict__ px
     }
 }
 
-__global__ void k_fill_u64(uint64_t *__restrict__ p, uint64_t v) { *p = v; }
+__global__ void k_fill_u64(uint64_t *__restrict__ p, uint64_t v) { p[threadIdx.x] = v; }
 
 // sys_of[i] = the system holding slot i (binary search over the offsets)
 __global__ __launch_bounds__(kBlock) void k_sys_of(const uint32_t *__restrict__ sys_off, uint32_t nsys, uint32_t n,
@@ -537,7 +546,7 @@ void launch_bh_bs(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bo
 }
 
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
-    if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, c->stream, c->b.interactions, 0ull);
+    if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(2), 0, c->stream, c->b.interactions, 0ull);
     switch (p->walk_block) {
     case 64: launch_bh_bs<64>(c, p, end_kick_half_dt, count); break;
     case 128: launch_bh_bs<128>(c, p, end_kick_half_dt, count); break;

@@ -26,10 +26,10 @@ EXPORTS = [
     "ge_compute_gravity", "ge_integrate_start", "ge_integrate_end", "ge_sph_new_grid",
     "ge_sph_pcisph", "ge_get_keys", "ge_get_perm", "ge_get_node_count", "ge_get_nodes",
     "ge_get_interactions", "ge_get_neighbors", "ge_profile_enable", "ge_profile_read",
-    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes", "ge_diagnostics", "ge_direct_fp64",
+    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes", "ge_diagnostics", "ge_direct_fp64", "ge_get_interaction_counts",
 ]
 PHASES = ["integrate_bbox", "keys", "sort", "gather", "tree", "gravity", "sph_grid", "sph",
-          "integrate_end"]
+          "integrate_end", "multipole"]
 
 FIELDS_F = ["x", "y", "vx", "vy", "ax", "ay", "pvx", "pvy", "mass"]
 FIELDS_U = ["flags", "id"]
@@ -134,6 +134,7 @@ def load_library():
         "ge_get_node_count": (C.c_int, [vp, P(C.c_uint32)]),
         "ge_get_nodes": (C.c_int, [vp, vp, C.c_uint32]),
         "ge_get_interactions": (C.c_int, [vp, P(C.c_uint64)]),
+        "ge_get_interaction_counts": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64)]),
         "ge_get_neighbors": (C.c_int, [vp, C.c_int32, P(C.c_int64), P(C.c_uint32), C.c_int64,
                                        P(C.c_int64)]),
         "ge_profile_enable": (C.c_int, [vp, C.c_int]),
@@ -397,6 +398,11 @@ class Context:
         c = C.c_uint64()
         self._check(self.lib.ge_get_interactions(self.h, C.byref(c)))
         return c.value
+
+    def interaction_counts(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.ge_get_interaction_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def profile_enable(self, on=True):
         self._check(self.lib.ge_profile_enable(self.h, 1 if on else 0))

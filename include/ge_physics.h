@@ -202,6 +202,8 @@ int ge_get_node_count(ge_ctx *ctx, uint32_t *count);
 int ge_get_nodes(ge_ctx *ctx, ge_node *nodes, uint32_t cap);
 /* BH interaction count (accepted nodes + leaf partners) of the last gravity */
 int ge_get_interactions(ge_ctx *ctx, uint64_t *count);
+/* the same count split into accepted nodes and leaf partners */
+int ge_get_interaction_counts(ge_ctx *ctx, uint64_t *nodes, uint64_t *pairs);
 /* SPH candidate sets of the current grid (queryNeighborsAVX2,
  * neighborSearch.h:722-777) queried at pos (which=0) or predPos (which=1):
  * CSR offsets off[n+1] and particle slots idx[] (ascending per particle).
@@ -241,7 +243,8 @@ int ge_direct_fp64(ge_ctx *ctx, uint32_t ntargets, const uint32_t *targets,
 #define GE_PHASE_SPH_GRID 6       /* SPH grid builds */
 #define GE_PHASE_SPH 7            /* PCISPH passes (+ kick) */
 #define GE_PHASE_INTEGRATE_END 8  /* stand-alone integrateEnd */
-#define GE_NPHASES 9
+#define GE_PHASE_MULTIPOLE 9      /* node moments of the accuracy modes */
+#define GE_NPHASES 10
 /* enable (1) or disable (0) phase timing; clears the accumulators */
 int ge_profile_enable(ge_ctx *ctx, int enable);
 /* synchronises, then returns accumulated milliseconds and launch counts per
