@@ -88,15 +88,20 @@ def calibrate_copy(mib):
                       "launches": 5}), flush=True)
 
 
-def traffic_from_profiles():
+def traffic_from_profiles(accuracy, block):
+    """HBM bytes per walk launch measured with rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a copy kernel) for this form and block size."""
     path = os.path.join(ROOT, "profiles", "bh_walk_traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            t = json.load(f)
     except (OSError, ValueError):
         return None
+    if t.get("accuracy") != accuracy:
+        return None
+    return t.get("bytes_per_launch_by_block", {}).get(str(block or 256))
 
 
 def cpu_baseline(st, prm, budget_s):
@@ -143,7 +148,9 @@ def single_system(a, st, prm, accuracy, tune):
                        dt_start=prm["dt"], compensated=1, accuracy=accuracy, direct_below=0)
     out = {}
     with Context(n) as c:
+        tu0 = time.perf_counter()
         c.upload(st)
+        out["upload_ms"] = (time.perf_counter() - tu0) * 1e3
         if tune:
             leaf, block, r = S.autotune(c, p, frames=3)
             p.leaf_capacity, p.walk_block = leaf, block
@@ -168,6 +175,9 @@ def single_system(a, st, prm, accuracy, tune):
         c.sync()
         tg1 = time.perf_counter()
         p.use_graph = 0
+        td0 = time.perf_counter()
+        _ = c.download(["x", "y", "vx", "vy", "mass", "id"])
+        out["download_ms"] = (time.perf_counter() - td0) * 1e3
         out["errors"] = force_errors(c, p, n)
         nodes_acc, pairs = c.interaction_counts()
         out["node_count"] = c.node_count()
@@ -346,12 +356,15 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": traffic_from_profiles() if head["params"]["accuracy"] == 2 else None,
+                     "traffic": traffic_from_profiles(head["params"]["accuracy"], head["params"]["walk_block"]),
                      "kernel": "k_bh (tree walk)", "kernel_ms": head["walk_ms"],
                      "interactions": head["interactions"],
                      "flops_per_node": FLOPS_NODE[head["params"]["accuracy"]], "flops_per_pair": FLOPS_PAIR},
         "force_error_vs_fp64": head["errors"],
         "graph_ms_per_step": head["graph_ms_per_step"],
+        "host_transfer": {"upload_ms": head["upload_ms"], "download_ms": head["download_ms"],
+                          "note": "one upload of all fields and one download of x, y, vx, vy, mass, id "
+                                  "(pageable host memory); frames never cross PCIe"},
         "autotune": head.get("autotune"),
         "phase_ms": phase_ms,
         "plain_form": {"value": plain["value"], "ms_per_step": plain["ms_per_step"],

@@ -119,3 +119,19 @@ def test_config_generators_shapes():
     assert len(n) == 512 and n.min() >= 1000 and n.max() <= 131072
     reqs = configs.config4_stream(50, window_s=2.0)
     assert all(2000 <= r[1] <= 500000 and 10 <= r[2] <= 200 for r in reqs)
+
+
+def _build_facade_program(tmp_path):
+    exe = tmp_path / "facade_frame"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "facade_frame.cpp"),
+                    "-L", os.path.join(ROOT, "galaxy-engine_amd"), "-lge_physics",
+                    "-Wl,-rpath," + os.path.join(ROOT, "galaxy-engine_amd"), "-o", str(exe)], check=True)
+    return exe
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    # the facade keeps the reference's call names over the C ABI; without a
+    # device the program must fail cleanly with GE_E_NODEVICE, not crash
+    exe = _build_facade_program(tmp_path)
+    assert exe.exists()
