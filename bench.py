@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-sph", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=120)
@@ -196,6 +197,36 @@ def single_system(a, st, prm, accuracy, tune):
     return out
 
 
+def sph_impact(a):
+    """cfg2: gravity + SPH planetary impact, 250,000 particles, h = cell = 4,
+    the reference's SPH constants, theta 0.7, dt 0.002 (parity form: the
+    reference's walk and leaf rule)."""
+    from galaxy_step import Context, configs, default_params
+    st, prm = configs.config2()
+    n = len(st["x"])
+    p = default_params(direct_below=0, compensated=1)
+    for k in ("G", "theta", "softening", "dt", "sph", "sph_h", "sph_cell", "sph_unit_mass", "viscosity",
+              "cohesion_coef", "delta", "stiff_mult", "sph_max_vel"):
+        setattr(p, k, prm[k])
+    p.dt_start = p.dt
+    steps = max(a.steps, 10)
+    with Context(n, sph=True) as c:
+        c.upload(st)
+        c.step(p, a.warmup)
+        c.sync()
+        c.profile_enable(True)
+        t0 = time.perf_counter()
+        c.step(p, steps)
+        c.sync()
+        t1 = time.perf_counter()
+        ph = c.profile_read()
+        c.profile_enable(False)
+    el = t1 - t0
+    return {"workload": f"gravity + SPH impact, {n} particles (200k + 50k), h = cell = 4, theta 0.7, dt 0.002",
+            "particle_updates_per_s": n * steps / el, "ms_per_step": el * 1e3 / steps, "steps": steps,
+            "phase_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items() if v[1]}}
+
+
 def batched(a, accuracy):
     """cfg3: the batched form against the same systems one by one."""
     import numpy as np
@@ -329,6 +360,9 @@ def main():
     if not a.no_cpu and a.cpu_seconds > 0:
         cpu = cpu_baseline(st, dict(prm, theta=a.theta), a.cpu_seconds)
         log(f"cpu: {cpu['value']:.3e} particle-updates/s")
+    sph = None if a.no_sph else sph_impact(a)
+    if sph:
+        log(f"sph impact: {sph['ms_per_step']:.3f} ms/step, phases {sph['phase_ms']}")
     bat = None if a.no_batch else batched(a, a.accuracy)
     if bat:
         log(f"batched: {bat['batched_solves_per_s']:.1f} solves/s vs plain {bat['plain_solves_per_s']:.1f} "
@@ -372,6 +406,7 @@ def main():
                        "walk_tflops": plain["walk_tflops"],
                        "note": "reference monopole walk, opening test and leaf rule (the parity path)"},
         "cpu_baseline": cpu,
+        "sph_impact": sph,
         "batched": bat,
         "served": srv,
     }

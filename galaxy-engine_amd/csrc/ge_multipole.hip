@@ -242,7 +242,8 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_pieces(const NodeRec *__restric
     }
 }
 
-// Thread per large node: its pieces in order.
+// Wave per large node: lane l sums pieces l, l + 64, ... in order, then a
+// fixed-order wave reduction.
 __global__ __launch_bounds__(kMpBlock) void k_mp_big(const NodeRec *__restrict__ nodes,
                                                      const float4 *__restrict__ geom, float inv_theta,
                                                      const uint32_t *__restrict__ cnt,
@@ -253,7 +254,9 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_big(const NodeRec *__restrict__
     const uint32_t nb = min(cnt[kMpBigCount], big_cap);
     const uint32_t np = cnt[kMpPieceCount];
     if (np > piece_cap) return;  // the error flag is already raised
-    for (uint32_t b = blockIdx.x * kMpBlock + threadIdx.x; b < nb; b += gridDim.x * kMpBlock) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (kMpBlock / 64);
+    for (uint32_t b = blockIdx.x * (kMpBlock / 64) + (threadIdx.x >> 6); b < nb; b += waves) {
         const uint32_t k = big_list[b];
         const NodeRec r = nodes[k];
         const uint32_t pieces = (r.end - r.start + kMpWaveMax - 1) / kMpWaveMax;
@@ -261,13 +264,14 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_big(const NodeRec *__restrict__
 #pragma unroll
         for (int c = 0; c < kMono; ++c) S[c] = 0.0;
         double r2 = 0.0;
-        for (uint32_t q = 0; q < pieces; ++q) {
+        for (uint32_t q = lane; q < pieces; q += 64) {
             const double *ps = piece_sum + (size_t)kPiece * (big_first[b] + q);
 #pragma unroll
             for (int c = 0; c < kMono; ++c) S[c] += ps[c];
             r2 = fmax(r2, ps[kMono]);
         }
-        finish_record(k, r, geom[k], S, r2, inv_theta, mp);
+        wave_sum(S, r2);
+        if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp);
     }
 }
 
@@ -306,7 +310,7 @@ void launch_multipoles(ge_ctx *c, float theta) {
                        b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.node_mp);
     hipLaunchKernelGGL(k_mp_pieces, dim3(1024), dim3(kMpBlock), 0, c->stream, b.nodes, b.pxym, b.mp_cnt, b.mp_big_list,
                        big_first, b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.mp_piece_sum);
-    hipLaunchKernelGGL(k_mp_big, dim3(64), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, inv_theta, b.mp_cnt,
+    hipLaunchKernelGGL(k_mp_big, dim3(256), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, inv_theta, b.mp_cnt,
                        b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_cap, b.mp_piece_sum, b.node_mp);
 }
 

@@ -7,9 +7,18 @@
  *
  * Every function restates the reference algorithm in plain C with the same
  * floating-point expression order.  It is compiled with -ffp-contract=off so
- * that no a*b+c is fused (the contraction mode is pinned; see the A-12 note in
- * the design notes).  Citations are path:line under the reference tree, with
+ * that no a*b+c is fused: the reference's build would fuse some of them
+ * depending on compiler and flags, so one mode is pinned here and in the HIP
+ * kernels.  Citations are path:line under the reference tree, with
  * GE/ = GalaxyEngine/.
+ *
+ * Pins: the golden KATs and invariant, closed-form and brute-force checks of
+ * tests/test_oracle.py.  The reference cannot be compiled here (its pch.h
+ * needs glm), so these parts are parity unpinned beyond invariants:
+ * geo_sph_pcisph's doubled pair sums, the dens = 0 viscosity of the first
+ * frame, the predDens / restDens equation of state and the double pow kernel
+ * constants (pinned by momentum conservation, the isolated-particle density
+ * and the kernel closed forms only).
  *
  * Differences from the reference that are deliberate and documented:
  *  - the Morton sort is stable by (key, original index); the reference uses
