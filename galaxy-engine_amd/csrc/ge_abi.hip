@@ -100,7 +100,7 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.chunk_first, (size_t)c->max_sys + 1));
     GE_HIP_CHECK(c, dalloc(b.sys_src, (size_t)c->max_sys + 1));
     b.bb = b.sys_bb;  // system 0's box doubles as the single-scene box
-    GE_HIP_CHECK(c, dalloc(b.interactions, 2));  // all interactions, accepted nodes
+    GE_HIP_CHECK(c, dalloc(b.interactions, 4));  // interactions, accepted nodes, wave visits, leaf trips
     TopNodes &T = b.top;
     T.cap = 2u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(T.px, T.cap)); GE_HIP_CHECK(c, dalloc(T.py, T.cap));
@@ -285,6 +285,9 @@ int check_params(ge_ctx *c, const ge_step_params *p) {
     if (p->accuracy < 0 || p->accuracy > 2) return fail(c, GE_E_INVALID, "accuracy must be 0, 1 or 2");
     if (p->accuracy && !(p->theta > 0.0f)) return fail(c, GE_E_INVALID, "accuracy mode needs theta > 0");
     if (p->leaf_capacity > 64) return fail(c, GE_E_INVALID, "leaf_capacity must be 0 (reference rule) or 1..64");
+    if (p->sph_search != 0 && p->sph_search != 1) return fail(c, GE_E_INVALID, "sph_search must be 0 or 1");
+    if (p->sph && p->sph_search == 1 && !(p->sph_search_radius > 0.0f && p->sph_search_radius <= 8.0f * p->sph_cell))
+        return fail(c, GE_E_INVALID, "sph_search_radius must be in (0, 8 cells]");
     if (p->walk_block != 0 && p->walk_block != 64 && p->walk_block != 128 && p->walk_block != 256)
         return fail(c, GE_E_INVALID, "walk_block must be 0, 64, 128 or 256");
     return GE_OK;
@@ -464,6 +467,8 @@ void ge_default_params(ge_step_params *p) {
     p->delta = 19000.0f;
     p->stiff_mult = 1.0f;
     p->sph_max_vel = 250.0f;   // parameters.h:118
+    p->sph_search = 0;
+    p->sph_search_radius = 3.5f;  // NeighborSearchV2::searchRadius (neighborSearch.h:441)
     p->compensated = 1;
 }
 
@@ -854,6 +859,14 @@ int ge_get_interactions(ge_ctx *c, uint64_t *count) {
     return GE_OK;
 }
 
+int ge_get_walk_stats(ge_ctx *c, uint64_t out[4]) {
+    if (!c || !out) return GE_E_INVALID;
+    (void)hipSetDevice(c->device);
+    GE_HIP_CHECK(c, hipMemcpyAsync(out, c->b.interactions, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
 int ge_get_interaction_counts(ge_ctx *c, uint64_t *nodes, uint64_t *pairs) {
     if (!c || !nodes || !pairs) return GE_E_INVALID;
     (void)hipSetDevice(c->device);
@@ -892,6 +905,17 @@ int ge_profile_read(ge_ctx *c, double ms[GE_NPHASES], uint32_t launches[GE_NPHAS
     }
     c->prof_events.clear();
     return GE_OK;
+}
+
+int ge_neighbor_counts(ge_ctx *c, const ge_step_params *p, uint32_t *counts) {
+    if (!c || !p || !counts) return GE_E_INVALID;
+    if (!c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    if (!(p->sph_cell > 0.0f && p->sph_search_radius > 0.0f && p->sph_search_radius <= 8.0f * p->sph_cell))
+        return fail(c, GE_E_INVALID, "sph_cell and sph_search_radius must be > 0 (radius at most 8 cells)");
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    launch_integrate_start_bbox(c, 0.0f, 0, 0.0f, 0);  // the grid frame uses the current extremes
+    return sph_neighbor_counts(c, p, counts);
 }
 
 int ge_get_neighbors(ge_ctx *c, int32_t which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total) {

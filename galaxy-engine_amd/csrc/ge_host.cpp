@@ -50,6 +50,74 @@ int bits_for(uint64_t v) {  // bits to hold values 0 .. v-1 (as the batched sort
 
 }  // namespace
 
+// ------------------------------------------------- array-of-structures --
+
+namespace {
+
+// Field k of ge_particles (declaration order) as a pointer slot.
+void **field_slot(ge_particles &p, int k) {
+    void **slots[GE_NFIELDS] = {
+        (void **)&p.x, (void **)&p.y, (void **)&p.vx, (void **)&p.vy, (void **)&p.ax, (void **)&p.ay,
+        (void **)&p.pvx, (void **)&p.pvy, (void **)&p.mass, (void **)&p.flags, (void **)&p.id,
+        (void **)&p.sph_mass, (void **)&p.rest_dens, (void **)&p.stiff, (void **)&p.visc, (void **)&p.cohesion,
+        (void **)&p.dens, (void **)&p.press, (void **)&p.press_tmp, (void **)&p.pred_dens, (void **)&p.ppx,
+        (void **)&p.ppy, (void **)&p.pfx, (void **)&p.pfy};
+    return slots[k];
+}
+
+constexpr int kFieldFlags = 9;
+
+bool layout_ok(const ge_aos_layout *l) {
+    if (!l || l->stride == 0) return false;
+    for (int k = 0; k < GE_NFIELDS; ++k)
+        if (l->off[k] >= 0 && (uint32_t)l->off[k] + 4u > l->stride) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" int ge_upload_aos(ge_ctx *ctx, uint32_t n, const void *records, const ge_aos_layout *layout,
+                             const uint32_t *flags) {
+    if (!ctx || (n && !records) || !layout_ok(layout)) return GE_E_INVALID;
+    // flattenParticles: one SoA column per present field
+    std::vector<std::vector<uint32_t>> cols(GE_NFIELDS);
+    ge_particles p{};
+    const unsigned char *base = static_cast<const unsigned char *>(records);
+    for (int k = 0; k < GE_NFIELDS; ++k) {
+        if (k == kFieldFlags || layout->off[k] < 0) continue;
+        cols[k].resize(n ? n : 1);
+        for (uint32_t i = 0; i < n; ++i)
+            std::memcpy(&cols[k][i], base + (size_t)i * layout->stride + layout->off[k], 4);
+        *field_slot(p, k) = cols[k].data();
+    }
+    p.flags = const_cast<uint32_t *>(flags);
+    return ge_upload(ctx, n, &p);
+}
+
+extern "C" int ge_download_aos(ge_ctx *ctx, void *records, const ge_aos_layout *layout, uint32_t *flags) {
+    uint32_t n = 0;
+    if (!ctx || !layout_ok(layout)) return GE_E_INVALID;
+    if (int r = ge_count(ctx, &n)) return r;
+    if (n && !records) return GE_E_INVALID;
+    std::vector<std::vector<uint32_t>> cols(GE_NFIELDS);
+    ge_particles p{};
+    for (int k = 0; k < GE_NFIELDS; ++k) {
+        if (k == kFieldFlags || layout->off[k] < 0) continue;
+        cols[k].resize(n ? n : 1);
+        *field_slot(p, k) = cols[k].data();
+    }
+    p.flags = flags;
+    if (int r = ge_download(ctx, &p)) return r;
+    // readFlattenBack: every present field back into its record
+    unsigned char *base = static_cast<unsigned char *>(records);
+    for (int k = 0; k < GE_NFIELDS; ++k) {
+        if (k == kFieldFlags || layout->off[k] < 0) continue;
+        for (uint32_t i = 0; i < n; ++i)
+            std::memcpy(base + (size_t)i * layout->stride + layout->off[k], &cols[k][i], 4);
+    }
+    return GE_OK;
+}
+
 // ------------------------------------------------------------------ tuner --
 
 extern "C" int ge_autotune(ge_ctx *ctx, const ge_step_params *base, int32_t frames, ge_tune_result *out) {

@@ -158,6 +158,8 @@ struct ge_ctx {
     bool have_tree = false;
     bool have_grid = false;
     float sph_cell = 3.0f;         // cell size grid A was built with
+    int sph_search = 0;            // search semantics grid A was built with
+    float sph_radius = 3.5f;       // and its search radius
     uint32_t leaf_cap = 0;         // leaf rule of the next tree build (0 = reference)
     uint64_t device_bytes = 0;     // device memory the context allocated
     double *diag = nullptr;        // FP64 diagnostics scratch (allocated on first use)
@@ -205,6 +207,7 @@ void launch_tree_build(ge_ctx *c);
 int sph_new_grid(ge_ctx *c, const ge_step_params *p);
 int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
 int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total);
+int sph_neighbor_counts(ge_ctx *c, const ge_step_params *p, uint32_t *counts);
 
 inline uint32_t blocks_for(uint64_t n, int per_block) {
     uint64_t b = (n + per_block - 1) / per_block;

@@ -4,6 +4,12 @@
 // neighborSearch.h:636-777): a particle's candidates are all particles whose
 // grid cell of pos is within one cell (3 x 3) of the query point's cell.
 //
+// Two search semantics (ge_step_params.sph_search): 0 is the AVX2 grid's
+// above; 1 is NeighborSearchV2, the reference's path without AVX2
+// (neighborSearch.h:439-599): cell = (int)(p * (1/cell)), ceil(radius/cell)
+// cells each way and a filter |p_j - q|^2 <= radius^2 on the grid position.
+// Both relations stay symmetric, so the gather forms below hold for both.
+//
 // Uniform grid instead of the reference's 16384-slot hash: particles are
 // radix-sorted by cell over the occupied cell range and first[c] (prefix
 // cell table) turns the three cells of one row into one contiguous range.
@@ -33,10 +39,19 @@ constexpr float kPiF = 3.14159265358979323846f;  // raylib PI, a float
 constexpr uint32_t kSentinelCell = (1u << kCellBits) - 1u;
 constexpr uint32_t kGridMaxBlocks = 1024;
 
+// candidate-set semantics of one grid
+struct Search {
+    int mode;        // 0 NeighborSearchV2AVX2, 1 NeighborSearchV2
+    float cell, inv; // cell size and the float 1/cell of NeighborSearchV2
+    float r2;        // searchRadius^2 (mode 1)
+    int R;           // cells searched on each side of the query cell
+};
+
 struct SphConst {
     float h, h2, cell, unit, visc, coh, delta, stiff_mult, coeff;
     float vol, spiky, lap;  // kernel constants (SPH.h:38-56)
     float half_dt;          // fused integrateEnd (0 = none)
+    Search S;
 };
 
 struct GridView {
@@ -49,21 +64,29 @@ struct GridView {
 
 __device__ __forceinline__ bool is_visitor(uint32_t f) { return (f & GE_FLAG_SPH) && !(f & GE_FLAG_DRAWN); }
 
-// NeighborSearchV2AVX2::posToCellCoord (neighborSearch.h:631-635), with the
-// float clamped so far-away points cannot overflow the int conversion
-__device__ __forceinline__ int cell_of(float p, float cell) {
-    float c = floorf(p / cell);
+// posToCellCoord: NeighborSearchV2AVX2 floor(p / cell) (neighborSearch.h:
+// 631-635) or NeighborSearchV2 (int)(p * invCellSize) (neighborSearch.h:
+// 472-474), with the float clamped so far-away points cannot overflow the int
+// conversion
+__device__ __forceinline__ int cell_coord(float p, const Search &S) {
+    float c = S.mode ? truncf(p * S.inv) : floorf(p / S.cell);
     c = fminf(fmaxf(c, -1073741824.0f), 1073741824.0f);
     return (int)c;
 }
 
-// particles of cells (qx-1..qx+1, row) as one range of cell-order positions
-__device__ __forceinline__ bool row_range(const int32_t *fr, const uint32_t *first, int qx, int row, uint32_t &lo,
-                                          uint32_t &hi) {
+// mode 1's distance filter; dx, dy = grid position - query point
+__device__ __forceinline__ bool within(const Search &S, float dx, float dy) {
+#pragma clang fp contract(off)
+    return !S.mode || dx * dx + dy * dy <= S.r2;
+}
+
+// particles of cells (qx-R..qx+R, row) as one range of cell-order positions
+__device__ __forceinline__ bool row_range(const int32_t *fr, const uint32_t *first, int qx, int row, int R,
+                                          uint32_t &lo, uint32_t &hi) {
     if (!fr[4]) return false;
     const int64_t r = (int64_t)row - fr[1];
     if (r < 0 || r >= fr[3]) return false;
-    int64_t c0 = (int64_t)qx - 1 - fr[0], c1 = (int64_t)qx + 1 - fr[0];
+    int64_t c0 = (int64_t)qx - R - fr[0], c1 = (int64_t)qx + R - fr[0];
     if (c1 < 0 || c0 >= fr[2]) return false;
     if (c0 < 0) c0 = 0;
     if (c1 >= fr[2]) c1 = fr[2] - 1;
@@ -118,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ x, 
 
 // Grid frame from the extremes: the occupied cell range.
 __global__ void k_frame(const float4 *__restrict__ partials, uint32_t nparts, const float *__restrict__ ext_in,
-                        float cell, uint32_t cells_cap, int32_t *__restrict__ frame, uint32_t *__restrict__ counters) {
+                        Search S, uint32_t cells_cap, int32_t *__restrict__ frame, uint32_t *__restrict__ counters) {
     if (threadIdx.x != 0) return;
     float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
     if (ext_in) {
@@ -135,8 +158,11 @@ __global__ void k_frame(const float4 *__restrict__ partials, uint32_t nparts, co
         return;
     }
     const float lim = 536870912.0f;  // keep cell coordinates well inside int
-    const float fx0 = floorf(mnx / cell), fy0 = floorf(mny / cell);
-    const float fx1 = floorf(mxx / cell), fy1 = floorf(mxy / cell);
+    // the cell function is monotone, so the extremes bound every cell
+    const float fx0 = S.mode ? truncf(mnx * S.inv) : floorf(mnx / S.cell);
+    const float fy0 = S.mode ? truncf(mny * S.inv) : floorf(mny / S.cell);
+    const float fx1 = S.mode ? truncf(mxx * S.inv) : floorf(mxx / S.cell);
+    const float fy1 = S.mode ? truncf(mxy * S.inv) : floorf(mxy / S.cell);
     bool ok = fabsf(fx0) < lim && fabsf(fy0) < lim && fabsf(fx1) < lim && fabsf(fy1) < lim;
     const int64_t W = ok ? (int64_t)fx1 - (int64_t)fx0 + 1 : 0;
     const int64_t H = ok ? (int64_t)fy1 - (int64_t)fy0 + 1 : 0;
@@ -151,13 +177,13 @@ __global__ void k_frame(const float4 *__restrict__ partials, uint32_t nparts, co
 
 __global__ __launch_bounds__(kBlock) void k_cell_keys(const float *__restrict__ x, const float *__restrict__ y,
                                                       const uint32_t *__restrict__ flags, uint32_t n, int visitors_only,
-                                                      float cell, const int32_t *__restrict__ frame,
+                                                      Search S, const int32_t *__restrict__ frame,
                                                       uint64_t *__restrict__ packed) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     uint64_t key = kSentinelCell;
     if (frame[4] && (!visitors_only || is_visitor(flags[i]))) {
-        const int cx = cell_of(x[i], cell), cy = cell_of(y[i], cell);
+        const int cx = cell_coord(x[i], S), cy = cell_coord(y[i], S);
         key = (uint64_t)(cy - frame[1]) * (uint64_t)frame[2] + (uint64_t)(cx - frame[0]);
     }
     packed[i] = (key << kIdxBits) | i;
@@ -211,17 +237,18 @@ __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A
     const bool fwd = !(fi & GE_FLAG_PINNED);
     const float mI = oq.x * K.unit;
     const float di = oq.w < 0.001f ? 0.001f : oq.w;
-    const int cx = cell_of(own.x, K.cell), cy = cell_of(own.y, K.cell);
+    const int cx = cell_coord(own.x, K.S), cy = cell_coord(own.y, K.S);
     float sx = 0.0f, sy = 0.0f;
-    for (int dr = -1; dr <= 1; ++dr) {
+    for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
         uint32_t lo, hi;
-        if (!row_range(A.frame, A.first, cx, cy + dr, lo, hi)) continue;
+        if (!row_range(A.frame, A.first, cx, cy + dr, K.S.R, lo, hi)) continue;
         for (uint32_t kk = lo; kk < hi; ++kk) {
             if (kk == k) continue;
             const uint32_t fj = A.pkf[kk];
             if (!is_visitor(fj)) continue;
             const float4 pj = A.pk0[kk];
             const float dx = pj.x - own.x, dy = pj.y - own.y;
+            if (!within(K.S, dx, dy)) continue;
             const float rsq = dx * dx + dy * dy;
             if (rsq >= K.h2) continue;
             const float r = sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
@@ -272,13 +299,17 @@ __global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridVie
         return;
     }
     const float4 me = pp[k];
-    const int cx = cell_of(me.x, K.cell), cy = cell_of(me.y, K.cell);
+    const int cx = cell_coord(me.x, K.S), cy = cell_coord(me.y, K.S);
     float dens = 0.0f;
-    for (int dr = -1; dr <= 1; ++dr) {
+    for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
         uint32_t lo, hi;
-        if (!row_range(A.frame, A.first, cx, cy + dr, lo, hi)) continue;
+        if (!row_range(A.frame, A.first, cx, cy + dr, K.S.R, lo, hi)) continue;
         for (uint32_t kk = lo; kk < hi; ++kk) {
             if (!is_visitor(A.pkf[kk])) continue;
+            if (K.S.mode) {  // filter on the candidate's grid position (pos)
+                const float4 gj = A.pk0[kk];
+                if (!within(K.S, gj.x - me.x, gj.y - me.y)) continue;
+            }
             const float4 pj = pp[kk];
             const float dx = me.x - pj.x, dy = me.y - pj.y;
             const float rrsq = dx * dx + dy * dy;
@@ -341,12 +372,16 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
         const float4 me = pp[k];
         const float2 mp = pd[k];
         // i's visits: candidates by pos cell around cell(predPos_i)
-        const int qx = cell_of(me.x, K.cell), qy = cell_of(me.y, K.cell);
-        for (int dr = -1; dr <= 1; ++dr) {
+        const int qx = cell_coord(me.x, K.S), qy = cell_coord(me.y, K.S);
+        for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
             uint32_t lo, hi;
-            if (!row_range(A.frame, A.first, qx, qy + dr, lo, hi)) continue;
+            if (!row_range(A.frame, A.first, qx, qy + dr, K.S.R, lo, hi)) continue;
             for (uint32_t kk = lo; kk < hi; ++kk) {
                 if (kk == k || !is_visitor(A.pkf[kk])) continue;
+                if (K.S.mode) {
+                    const float4 gj = A.pk0[kk];
+                    if (!within(K.S, gj.x - me.x, gj.y - me.y)) continue;
+                }
                 const float4 pj = pp[kk];
                 const float dx = me.x - pj.x, dy = me.y - pj.y;
                 const float rr = sqrtf(dx * dx + dy * dy);
@@ -360,13 +395,15 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
         }
         // visits of i by others: predPos cells around cell(pos_i)
         const float4 mypos = A.pk0[k];
-        const int px = cell_of(mypos.x, K.cell), py = cell_of(mypos.y, K.cell);
-        for (int dr = -1; dr <= 1; ++dr) {
+        const int px = cell_coord(mypos.x, K.S), py = cell_coord(mypos.y, K.S);
+        for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
             uint32_t lo, hi;
-            if (!row_range(B.frame, B.first, px, py + dr, lo, hi)) continue;
+            if (!row_range(B.frame, B.first, px, py + dr, K.S.R, lo, hi)) continue;
             for (uint32_t kb = lo; kb < hi; ++kb) {
                 if (B.ord[kb] == i) continue;
                 const float4 pj = B.pk0[kb];
+                // j's query at predPos_j meets i at pos_i
+                if (!within(K.S, mypos.x - pj.x, mypos.y - pj.y)) continue;
                 const float dx = pj.x - me.x, dy = pj.y - me.y;
                 const float rr = sqrtf(dx * dx + dy * dy);
                 if (rr < 1e-5f || rr >= K.h) continue;
@@ -393,34 +430,66 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
     }
 }
 
-// ---- candidate lists for the parity hook
+// ---- candidate lists for the parity hook; counts (cnt) and lists (out),
+// optionally skipping particles whose flags match skip_mask (count 0)
 __global__ __launch_bounds__(kBlock) void k_cand_count(const float *__restrict__ qx, const float *__restrict__ qy,
-                                                       uint32_t n, float cell, const int32_t *__restrict__ frame,
-                                                       const uint32_t *__restrict__ first, uint32_t *__restrict__ cnt) {
+                                                       uint32_t n, Search S, const int32_t *__restrict__ frame,
+                                                       const uint32_t *__restrict__ first,
+                                                       const float4 *__restrict__ pk0,
+                                                       const uint32_t *__restrict__ flags, uint32_t skip_mask,
+                                                       uint32_t *__restrict__ cnt) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int cx = cell_of(qx[i], cell), cy = cell_of(qy[i], cell);
+    if (flags && (flags[i] & skip_mask)) {
+        cnt[i] = 0;
+        return;
+    }
+    const float x = qx[i], y = qy[i];
+    const int cx = cell_coord(x, S), cy = cell_coord(y, S);
     uint32_t c = 0;
-    for (int dr = -1; dr <= 1; ++dr) {
+    for (int dr = -S.R; dr <= S.R; ++dr) {
         uint32_t lo, hi;
-        if (row_range(frame, first, cx, cy + dr, lo, hi)) c += hi - lo;
+        if (!row_range(frame, first, cx, cy + dr, S.R, lo, hi)) continue;
+        if (!S.mode) {
+            c += hi - lo;
+            continue;
+        }
+        for (uint32_t kk = lo; kk < hi; ++kk) {
+            const float4 g = pk0[kk];
+            c += within(S, g.x - x, g.y - y) ? 1u : 0u;
+        }
     }
     cnt[i] = c;
 }
 
 __global__ __launch_bounds__(kBlock) void k_cand_fill(const float *__restrict__ qx, const float *__restrict__ qy,
-                                                      uint32_t n, float cell, const int32_t *__restrict__ frame,
-                                                      const uint32_t *__restrict__ first, const uint32_t *__restrict__ ord,
+                                                      uint32_t n, Search S, const int32_t *__restrict__ frame,
+                                                      const uint32_t *__restrict__ first,
+                                                      const float4 *__restrict__ pk0, const uint32_t *__restrict__ ord,
                                                       const int64_t *__restrict__ off, uint32_t *__restrict__ out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int cx = cell_of(qx[i], cell), cy = cell_of(qy[i], cell);
+    const float x = qx[i], y = qy[i];
+    const int cx = cell_coord(x, S), cy = cell_coord(y, S);
     int64_t w = off[i];
-    for (int dr = -1; dr <= 1; ++dr) {
+    for (int dr = -S.R; dr <= S.R; ++dr) {
         uint32_t lo, hi;
-        if (!row_range(frame, first, cx, cy + dr, lo, hi)) continue;
-        for (uint32_t kk = lo; kk < hi; ++kk) out[w++] = ord[kk];
+        if (!row_range(frame, first, cx, cy + dr, S.R, lo, hi)) continue;
+        for (uint32_t kk = lo; kk < hi; ++kk) {
+            const float4 g = pk0[kk];
+            if (within(S, g.x - x, g.y - y)) out[w++] = ord[kk];
+        }
     }
+}
+
+Search make_search(int mode, float cell, float radius) {
+    Search S;
+    S.mode = mode;
+    S.cell = cell;
+    S.inv = 1.0f / cell;
+    S.r2 = radius * radius;
+    S.R = mode ? (int)std::ceil(radius * S.inv) : 1;  // neighborSearch.h:552
+    return S;
 }
 
 SphConst make_const(const ge_step_params *p, float half_dt) {
@@ -439,6 +508,7 @@ SphConst make_const(const ge_step_params *p, float half_dt) {
     K.spiky = (float)(-45.0 / ((double)kPiF * std::pow((double)K.h, 6.0)));
     K.lap = (float)(45.0 / ((double)kPiF * std::pow((double)K.h, 6.0)));
     K.half_dt = half_dt;
+    K.S = make_search(p->sph_search, p->sph_cell, p->sph_search_radius);
     return K;
 }
 
@@ -465,15 +535,18 @@ int sph_new_grid(ge_ctx *c, const ge_step_params *p) {
     Grid &g = c->b.grid_pos;
     const State &s = c->b.st[c->b.cur];
     const uint32_t n = c->n;
-    hipLaunchKernelGGL(k_frame, dim3(1), dim3(64), 0, c->stream, nullptr, 0u, c->b.bb + 4, p->sph_cell, g.cells_cap,
-                       g.frame, c->b.counters);
-    hipLaunchKernelGGL(k_cell_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, s.flags, n, 0,
-                       p->sph_cell, g.frame, g.keys);
+    const Search S = make_search(p->sph_search, p->sph_cell, p->sph_search_radius);
+    hipLaunchKernelGGL(k_frame, dim3(1), dim3(64), 0, c->stream, nullptr, 0u, c->b.bb + 4, S, g.cells_cap, g.frame,
+                       c->b.counters);
+    hipLaunchKernelGGL(k_cell_keys, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, s.flags, n, 0, S,
+                       g.frame, g.keys);
     build_grid(c, g);
     hipLaunchKernelGGL(k_pack_a, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s, n, g.ord, g.pk0, g.pk1,
                        g.pkf);
     c->have_grid = true;
     c->sph_cell = p->sph_cell;
+    c->sph_search = p->sph_search;
+    c->sph_radius = p->sph_search_radius;
     return GE_OK;
 }
 
@@ -495,9 +568,9 @@ int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt) {
     hipLaunchKernelGGL(k_minmax, dim3(mb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.flags, n, 1,
                        reinterpret_cast<float4 *>(c->b.partials2));
     hipLaunchKernelGGL(k_frame, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<const float4 *>(c->b.partials2), mb,
-                       nullptr, p->sph_cell, B.cells_cap, B.frame, c->b.counters);
-    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.flags, n, 1, p->sph_cell,
-                       B.frame, B.keys);
+                       nullptr, K.S, B.cells_cap, B.frame, c->b.counters);
+    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.flags, n, 1, K.S, B.frame,
+                       B.keys);
     build_grid(c, B);
     hipLaunchKernelGGL(k_density, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view(A), ppA, K, pdA);
     hipLaunchKernelGGL(k_pack_b, dim3(nb), dim3(kBlock), 0, c->stream, s, n, B.ord, B.pk0, B.pk1);
@@ -515,11 +588,11 @@ int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap
     uint32_t *dcnt = nullptr;
     int64_t *doff = nullptr;
     uint32_t *dout = nullptr;
-    // the cell size of the grid is the one it was built with
-    float cell = c->sph_cell;
+    // the semantics of the grid are the ones it was built with
+    const Search S = make_search(c->sph_search, c->sph_cell, c->sph_radius);
     GE_HIP_CHECK(c, hipMalloc(&dcnt, sizeof(uint32_t) * (n + 1)));
-    hipLaunchKernelGGL(k_cand_count, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, qx, qy, n, cell, g.frame,
-                       g.first, dcnt);
+    hipLaunchKernelGGL(k_cand_count, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, qx, qy, n, S, g.frame,
+                       g.first, g.pk0, nullptr, 0u, dcnt);
     std::vector<uint32_t> hc(n);
     GE_HIP_CHECK(c, hipMemcpyAsync(hc.data(), dcnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -538,14 +611,31 @@ int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap
     GE_HIP_CHECK(c, hipMalloc(&doff, sizeof(int64_t) * (n + 1)));
     GE_HIP_CHECK(c, hipMalloc(&dout, sizeof(uint32_t) * (off[n] > 0 ? off[n] : 1)));
     GE_HIP_CHECK(c, hipMemcpyAsync(doff, off, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_cand_fill, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, qx, qy, n, cell, g.frame,
-                       g.first, g.ord, doff, dout);
+    hipLaunchKernelGGL(k_cand_fill, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, qx, qy, n, S, g.frame,
+                       g.first, g.pk0, g.ord, doff, dout);
     GE_HIP_CHECK(c, hipMemcpyAsync(idx, dout, sizeof(uint32_t) * off[n], hipMemcpyDeviceToHost, c->stream));
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(dcnt);
     (void)hipFree(doff);
     (void)hipFree(dout);
     for (uint32_t i = 0; i < n; ++i) std::sort(idx + off[i], idx + off[i + 1]);
+    return GE_OK;
+}
+
+// NeighborSearchV2::newGrid + neighborAmount (neighborSearch.h:481-598)
+int sph_neighbor_counts(ge_ctx *c, const ge_step_params *p, uint32_t *counts) {
+    ge_step_params q = *p;
+    q.sph_search = 1;
+    if (int r = sph_new_grid(c, &q)) return r;
+    const Grid &g = c->b.grid_pos;
+    const State &s = c->b.st[c->b.cur];
+    const uint32_t n = c->n;
+    const Search S = make_search(1, q.sph_cell, q.sph_search_radius);
+    uint32_t *dcnt = reinterpret_cast<uint32_t *>(c->b.sphf);  // SPH force scratch: 2 floats per particle
+    hipLaunchKernelGGL(k_cand_count, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, n, S, g.frame,
+                       g.first, g.pk0, s.flags, GE_FLAG_PINNED | GE_FLAG_DRAWN | GE_FLAG_DARK, dcnt);
+    GE_HIP_CHECK(c, hipMemcpyAsync(counts, dcnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }
 

@@ -257,11 +257,12 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
     const uint32_t walk_end = __builtin_amdgcn_readfirstlane(whi);
     uint32_t skip_until = root;
     float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
-    uint32_t cnt = 0, cnt_nodes = 0;
+    uint32_t cnt = 0, cnt_nodes = 0, visits = 0, leaf_trips = 0;  // visits and trips: per wave
     uint32_t idx = __builtin_amdgcn_readfirstlane(wlo);
     while (idx < walk_end) {
         const NodeRec nd = nodes[idx];
         const uint32_t next = nd.next;
+        if (COUNT) ++visits;
         // index after this subtree, saturated so the walk always moves forward
         const uint32_t after = (next >= node_count - idx) ? node_count : idx + next + 1;
         if (nd.mass <= 0.0f) {  // empty node: skip the subtree (physics.cpp:107-110)
@@ -346,6 +347,7 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
         if (__ballot(open_leaf)) {
             // leaf direct sum over [start, end), skipping i == j (physics.cpp:163-196)
             const uint32_t jend = min(nd.end, n);
+            if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
             for (uint32_t j = nd.start; j < jend; ++j) {
                 const float4 q = pxym[j];
                 if (open_leaf && j != i) {
@@ -377,6 +379,8 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
         if ((threadIdx.x & 63) == 0) {
             atomicAdd(inter, (unsigned long long)c);
             atomicAdd(inter + 1, (unsigned long long)cn);
+            atomicAdd(inter + 2, (unsigned long long)visits);
+            atomicAdd(inter + 3, (unsigned long long)leaf_trips);
         }
     }
     if (!live) return;
@@ -546,7 +550,7 @@ void launch_bh_bs(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bo
 }
 
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
-    if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(2), 0, c->stream, c->b.interactions, 0ull);
+    if (count) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(4), 0, c->stream, c->b.interactions, 0ull);
     switch (p->walk_block) {
     case 64: launch_bh_bs<64>(c, p, end_kick_half_dt, count); break;
     case 128: launch_bh_bs<128>(c, p, end_kick_half_dt, count); break;

@@ -26,7 +26,7 @@ EXPORTS = [
     "ge_compute_gravity", "ge_integrate_start", "ge_integrate_end", "ge_sph_new_grid",
     "ge_sph_pcisph", "ge_get_keys", "ge_get_perm", "ge_get_node_count", "ge_get_nodes",
     "ge_get_interactions", "ge_get_neighbors", "ge_profile_enable", "ge_profile_read",
-    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes", "ge_diagnostics", "ge_direct_fp64", "ge_get_interaction_counts",
+    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes", "ge_diagnostics", "ge_direct_fp64", "ge_get_interaction_counts", "ge_get_walk_stats", "ge_upload_aos", "ge_download_aos", "ge_neighbor_counts",
 ]
 PHASES = ["integrate_bbox", "keys", "sort", "gather", "tree", "gravity", "sph_grid", "sph",
           "integrate_end", "multipole"]
@@ -56,6 +56,20 @@ class GeDiag(C.Structure):
                 ("comy", C.c_double)]
 
 
+class AosLayout(C.Structure):
+    _fields_ = [("stride", C.c_uint32), ("off", C.c_int32 * 24)]
+
+    @classmethod
+    def from_dtype(cls, dtype):
+        """Layout of a numpy structured dtype whose field names are ge_particles
+        field names (others are ignored)."""
+        lay = cls()
+        lay.stride = dtype.itemsize
+        for k, f in enumerate(PARTICLE_ORDER):
+            lay.off[k] = dtype.fields[f][1] if f in dtype.names else -1
+        return lay
+
+
 class StepParams(C.Structure):
     _fields_ = [("G", C.c_float), ("theta", C.c_float), ("softening", C.c_float),
                 ("dt_start", C.c_float), ("dt", C.c_float), ("naive", C.c_int32),
@@ -65,6 +79,7 @@ class StepParams(C.Structure):
                 ("sph_max_vel", C.c_float), ("periodic", C.c_int32), ("temperature", C.c_int32),
                 ("sph_ground", C.c_int32), ("compensated", C.c_int32), ("accuracy", C.c_int32),
                 ("leaf_capacity", C.c_uint32), ("walk_block", C.c_uint32), ("use_graph", C.c_int32),
+                ("sph_search", C.c_int32), ("sph_search_radius", C.c_float),
                 ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
@@ -135,6 +150,10 @@ def load_library():
         "ge_get_nodes": (C.c_int, [vp, vp, C.c_uint32]),
         "ge_get_interactions": (C.c_int, [vp, P(C.c_uint64)]),
         "ge_get_interaction_counts": (C.c_int, [vp, P(C.c_uint64), P(C.c_uint64)]),
+        "ge_get_walk_stats": (C.c_int, [vp, P(C.c_uint64)]),
+        "ge_upload_aos": (C.c_int, [vp, C.c_uint32, vp, P(AosLayout), P(C.c_uint32)]),
+        "ge_neighbor_counts": (C.c_int, [vp, P(StepParams), P(C.c_uint32)]),
+        "ge_download_aos": (C.c_int, [vp, vp, P(AosLayout), P(C.c_uint32)]),
         "ge_get_neighbors": (C.c_int, [vp, C.c_int32, P(C.c_int64), P(C.c_uint32), C.c_int64,
                                        P(C.c_int64)]),
         "ge_profile_enable": (C.c_int, [vp, C.c_int]),
@@ -309,6 +328,22 @@ class Context:
         self._check(self.lib.ge_memory_bytes(self.h, C.byref(b)))
         return b.value
 
+    def upload_aos(self, records, flags=None):
+        """records: numpy structured array (ge_particles field names)."""
+        lay = AosLayout.from_dtype(records.dtype)
+        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint32)
+        self._check(self.lib.ge_upload_aos(self.h, len(records), C.c_void_p(records.ctypes.data), C.byref(lay),
+                                           _uptr(fl) if fl is not None else None))
+
+    def download_aos(self, dtype, with_flags=False):
+        n = self.count()
+        rec = np.zeros(n, dtype=dtype)
+        fl = np.zeros(n, np.uint32) if with_flags else None
+        lay = AosLayout.from_dtype(rec.dtype)
+        self._check(self.lib.ge_download_aos(self.h, C.c_void_p(rec.ctypes.data), C.byref(lay),
+                                             _uptr(fl) if fl is not None else None))
+        return (rec, fl) if with_flags else rec
+
     def system_offsets(self):
         ns = C.c_uint32()
         self._check(self.lib.ge_system_offsets(self.h, C.byref(ns), None, 0))
@@ -404,6 +439,11 @@ class Context:
         self._check(self.lib.ge_get_interaction_counts(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def walk_stats(self):
+        a = (C.c_uint64 * 4)()
+        self._check(self.lib.ge_get_walk_stats(self.h, a))
+        return {"interactions": a[0], "nodes": a[1], "visits": a[2], "leaf_trips": a[3]}
+
     def profile_enable(self, on=True):
         self._check(self.lib.ge_profile_enable(self.h, 1 if on else 0))
 
@@ -413,6 +453,11 @@ class Context:
         cnt = (C.c_uint32 * len(PHASES))()
         self._check(self.lib.ge_profile_read(self.h, ms, cnt))
         return {ph: (ms[k], cnt[k]) for k, ph in enumerate(PHASES)}
+
+    def neighbor_counts(self, params):
+        out = np.zeros(self.count(), np.uint32)
+        self._check(self.lib.ge_neighbor_counts(self.h, C.byref(params), _uptr(out)))
+        return out
 
     def neighbors(self, which=0):
         n = self.count()

@@ -106,6 +106,13 @@ typedef struct {
                              64, 128 or 256 */
     int32_t use_graph;    /* ge_step captures its frames in a hipGraph once and
                              replays it while params and layout stay the same */
+    int32_t sph_search;   /* SPH candidate sets: 0 NeighborSearchV2AVX2
+                             (neighborSearch.h:601-797: floor cells, 3 x 3, no
+                             filter); 1 NeighborSearchV2, the reference's path
+                             without AVX2 (neighborSearch.h:439-599: truncated
+                             cells, ceil(radius / cell) cells each way,
+                             |p_j - q| <= sph_search_radius) */
+    float sph_search_radius; /* NeighborSearchV2::searchRadius (3.5) */
     uint32_t reserved[2];
 } ge_step_params;
 
@@ -160,6 +167,24 @@ int ge_sync(ge_ctx *ctx);
 /* non-blocking: *done = 1 when every call queued on the context finished */
 int ge_poll(ge_ctx *ctx, int32_t *done);
 
+/* Array-of-structures transfer: Physics::flattenParticles / readFlattenBack
+ * (GE/src/Physics/physics.cpp:354-390, 542-565) at the boundary.  The caller's
+ * records (e.g. std::vector<ParticlePhysics>, GE/include/Particles/particle.h:
+ * 5-73) are read or written in place through byte offsets; off[k] is the
+ * offset of the k-th ge_particles field (x, y, vx, ..., pfy: GE_NFIELDS in
+ * declaration order) or -1 when the record has no such field.  Flags live in
+ * the reference's ParticleRendering, so they come as their own array (NULL:
+ * zero on upload, skipped on download).  Download order is the device order,
+ * as the reference's vector is after sortParticlesByMortonKey. */
+#define GE_NFIELDS 24
+typedef struct {
+    uint32_t stride;          /* bytes from one record to the next */
+    int32_t off[GE_NFIELDS];  /* byte offset of each field, -1 = absent */
+} ge_aos_layout;
+int ge_upload_aos(ge_ctx *ctx, uint32_t n, const void *records, const ge_aos_layout *layout,
+                  const uint32_t *flags);
+int ge_download_aos(ge_ctx *ctx, void *records, const ge_aos_layout *layout, uint32_t *flags);
+
 /* ---- one frame ---------------------------------------------------------- */
 /* updateScene()'s 2D physics ordering (GE/src/globalLogic.cpp:1155-1372):
  * integrateStart, boundingBox, computeMortonKeys, sortParticlesByMortonKey,
@@ -204,12 +229,22 @@ int ge_get_nodes(ge_ctx *ctx, ge_node *nodes, uint32_t cap);
 int ge_get_interactions(ge_ctx *ctx, uint64_t *count);
 /* the same count split into accepted nodes and leaf partners */
 int ge_get_interaction_counts(ge_ctx *ctx, uint64_t *nodes, uint64_t *pairs);
+/* walk statistics of the last counted gravity call: {interactions, accepted
+ * nodes, node visits summed over waves, leaf partners looped over summed over
+ * waves}; lane utilisation = nodes / (64 visits), pairs / (64 leaf trips) */
+int ge_get_walk_stats(ge_ctx *ctx, uint64_t out[4]);
 /* SPH candidate sets of the current grid (queryNeighborsAVX2,
  * neighborSearch.h:722-777) queried at pos (which=0) or predPos (which=1):
  * CSR offsets off[n+1] and particle slots idx[] (ascending per particle).
  * *total receives the number of entries; idx may be NULL to size it. */
 int ge_get_neighbors(ge_ctx *ctx, int32_t which, int64_t *off, uint32_t *idx,
                      int64_t cap, int64_t *total);
+/* NeighborSearchV2::newGrid + neighborAmount (neighborSearch.h:481-598):
+ * rebuilds the SPH grid over all particles with the NeighborSearchV2
+ * semantics (p->sph_cell, p->sph_search_radius) and writes each particle's
+ * candidate count (self included) in device order; pinned, drawn and
+ * dark-matter particles get 0.  Needs a context with SPH state. */
+int ge_neighbor_counts(ge_ctx *ctx, const ge_step_params *p, uint32_t *counts);
 
 /* ---- FP64 diagnostics (no counterpart in the reference, whose UI shows
  * particle counts and FPS only, GE/src/UI/UI.cpp) ----------------------------- */

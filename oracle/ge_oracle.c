@@ -401,20 +401,44 @@ static int cell_cmp(const void *a, const void *b) {
     return p->i < q->i ? -1 : (p->i > q->i);
 }
 
-/* NeighborSearchV2AVX2::posToCellCoord, neighborSearch.h:631-635 */
-static inline int32_t cell_of(float p, float cell) { return (int32_t)floorf(p / cell); }
+/* search semantics: mode 0 = NeighborSearchV2AVX2, mode 1 = NeighborSearchV2 */
+typedef struct {
+    int32_t mode;
+    float cell, inv, r2;
+    int32_t R;  /* cells searched on each side of the query cell */
+} search_t;
 
-typedef struct { cell_entry *e; int32_t n; } cell_grid;
+static search_t make_search(int32_t mode, float cell, float radius) {
+    search_t S;
+    S.mode = mode;
+    S.cell = cell;
+    S.inv = 1.0f / cell;                  /* invCellSize (neighborSearch.h:443) */
+    S.r2 = radius * radius;               /* searchRadius^2 (neighborSearch.h:553) */
+    S.R = mode ? (int32_t)ceilf(radius * S.inv) : 1;  /* neighborSearch.h:552 */
+    return S;
+}
 
-/* newGridAVX2 (neighborSearch.h:636-720), restated as a (cx, cy, index)
- * ordered table; the candidate SET is the same as the hashed table's. */
-static cell_grid grid_build(int32_t n, const float *x, const float *y, float cell) {
+/* posToCellCoord: AVX2 floor(p / cell) (neighborSearch.h:631-635); V2
+ * (int)(p * invCellSize), a truncation (neighborSearch.h:472-474) */
+static inline int32_t cell_coord(float p, const search_t *S) {
+    if (S->mode) return (int32_t)(p * S->inv);
+    return (int32_t)floorf(p / S->cell);
+}
+
+typedef struct { cell_entry *e; int32_t n; const float *x, *y; search_t S; } cell_grid;
+
+/* newGridAVX2 (neighborSearch.h:636-720) or newGrid (:481-541), restated as
+ * a (cx, cy, index) ordered table; the candidate SET is the hashed table's. */
+static cell_grid grid_build(int32_t n, const float *x, const float *y, const search_t *S) {
     cell_grid g;
     g.n = n;
+    g.x = x;
+    g.y = y;
+    g.S = *S;
     g.e = (cell_entry *)malloc(sizeof(cell_entry) * (size_t)(n > 0 ? n : 1));
     for (int32_t i = 0; i < n; ++i) {
-        g.e[i].cx = cell_of(x[i], cell);
-        g.e[i].cy = cell_of(y[i], cell);
+        g.e[i].cx = cell_coord(x[i], S);
+        g.e[i].cy = cell_coord(y[i], S);
         g.e[i].i = (uint32_t)i;
     }
     qsort(g.e, (size_t)n, sizeof(cell_entry), cell_cmp);
@@ -432,18 +456,28 @@ static int32_t grid_lower(const cell_grid *g, int32_t cx, int32_t cy) {
 }
 
 /* queryNeighborsAVX2 (neighborSearch.h:722-777): dx outer, dy inner, every
- * particle whose cell coordinates match; no distance filter; self included. */
-static int32_t grid_query(const cell_grid *g, float qx, float qy, float cell,
-                          uint32_t *buf, int32_t cap) {
-    int32_t cx = cell_of(qx, cell), cy = cell_of(qy, cell);
+ * particle whose cell coordinates match; no distance filter; self included.
+ * NeighborSearchV2::queryNeighbors (neighborSearch.h:548-578): R cells each
+ * way and |p_j - q|^2 <= searchRadius^2 with dx = p_j.x - q.x. */
+static int32_t grid_query(const cell_grid *g, float qx, float qy, uint32_t *buf, int32_t cap) {
+    const search_t *S = &g->S;
+    int32_t cx = cell_coord(qx, S), cy = cell_coord(qy, S);
     int32_t k = 0;
-    for (int dx = -1; dx <= 1; ++dx) {
-        for (int dy = -1; dy <= 1; ++dy) {
+    for (int dx = -S->R; dx <= S->R; ++dx) {
+        for (int dy = -S->R; dy <= S->R; ++dy) {
             int32_t nx = cx + dx, ny = cy + dy;
             int32_t p = grid_lower(g, nx, ny);
             while (p < g->n && g->e[p].cx == nx && g->e[p].cy == ny) {
-                if (k < cap) buf[k] = g->e[p].i;
-                ++k;
+                const uint32_t j = g->e[p].i;
+                int take = 1;
+                if (S->mode) {
+                    float ddx = g->x[j] - qx, ddy = g->y[j] - qy;
+                    take = ddx * ddx + ddy * ddy <= S->r2;
+                }
+                if (take) {
+                    if (k < cap) buf[k] = j;
+                    ++k;
+                }
                 ++p;
             }
         }
@@ -459,12 +493,32 @@ static int u32_cmp(const void *a, const void *b) {
 int64_t geo_sph_candidates(int32_t n, const float *x, const float *y,
                            int32_t nq, const float *qx, const float *qy, float cell,
                            int64_t *off, uint32_t *idx, int64_t cap) {
-    cell_grid g = grid_build(n, x, y, cell);
+    return geo_sph_candidates_mode(n, x, y, nq, qx, qy, cell, 0, 0.0f, off, idx, cap);
+}
+
+void geo_neighbor_amount(int32_t n, const float *x, const float *y, const uint32_t *flags, float cell,
+                         float radius, uint32_t *counts) {
+    const search_t S = make_search(1, cell, radius);
+    cell_grid g = grid_build(n, x, y, &S);
+    uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    for (int32_t i = 0; i < n; ++i) {
+        const uint32_t f = flags ? flags[i] : 0u;
+        counts[i] = (f & (GEO_PINNED | GEO_DRAWN | GEO_DARK)) ? 0u : (uint32_t)grid_query(&g, x[i], y[i], buf, n);
+    }
+    free(buf);
+    free(g.e);
+}
+
+int64_t geo_sph_candidates_mode(int32_t n, const float *x, const float *y,
+                                int32_t nq, const float *qx, const float *qy, float cell,
+                                int32_t mode, float radius, int64_t *off, uint32_t *idx, int64_t cap) {
+    const search_t S = make_search(mode, cell, radius);
+    cell_grid g = grid_build(n, x, y, &S);
     int64_t total = 0;
     uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
     for (int32_t q = 0; q < nq; ++q) {
         off[q] = total;
-        int32_t k = grid_query(&g, qx[q], qy[q], cell, buf, n);
+        int32_t k = grid_query(&g, qx[q], qy[q], buf, n);
         qsort(buf, (size_t)k, sizeof(uint32_t), u32_cmp);
         if (idx && total + k <= cap) memcpy(idx + total, buf, sizeof(uint32_t) * (size_t)k);
         total += k;
@@ -510,7 +564,8 @@ int32_t geo_sph_pcisph(geo_state *s, const geo_params *p) {
     float *fx = (float *)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
     float *fy = (float *)calloc((size_t)(n > 0 ? n : 1), sizeof(float));
     uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
-    cell_grid g = grid_build(n, s->x, s->y, p->sph_cell);
+    const search_t S = make_search(p->sph_search, p->sph_cell, p->sph_search_radius);
+    cell_grid g = grid_build(n, s->x, s->y, &S);
     const uint32_t *fl = s->flags;
 #define IS_SPH(k) ((fl[k] & GEO_SPH) != 0)
 #define DRAWN(k) ((fl[k] & GEO_DRAWN) != 0)
@@ -519,7 +574,7 @@ int32_t geo_sph_pcisph(geo_state *s, const geo_params *p) {
     /* viscosity + cohesion, SPH.cpp:98-150 */
     for (int32_t i = 0; i < n; ++i) {
         if (!IS_SPH(i) || PINNED(i) || DRAWN(i)) continue;
-        int32_t k = grid_query(&g, s->x[i], s->y[i], p->sph_cell, buf, n);
+        int32_t k = grid_query(&g, s->x[i], s->y[i], buf, n);
         for (int32_t t = 0; t < k; ++t) {
             uint32_t j = buf[t];
             if (!IS_SPH(j) || DRAWN(j)) continue;
@@ -561,7 +616,7 @@ int32_t geo_sph_pcisph(geo_state *s, const geo_params *p) {
     for (int32_t i = 0; i < n; ++i) {
         if (!IS_SPH(i) || DRAWN(i)) continue;
         float pd = 0.0f;
-        int32_t k = grid_query(&g, s->ppx[i], s->ppy[i], p->sph_cell, buf, n);
+        int32_t k = grid_query(&g, s->ppx[i], s->ppy[i], buf, n);
         for (int32_t t = 0; t < k; ++t) {
             uint32_t j = buf[t];
             if (!IS_SPH(j) || DRAWN(j)) continue;
@@ -584,7 +639,7 @@ int32_t geo_sph_pcisph(geo_state *s, const geo_params *p) {
     /* pressure force, SPH.cpp:221-264 */
     for (int32_t i = 0; i < n; ++i) {
         if (!IS_SPH(i) || DRAWN(i)) continue;
-        int32_t k = grid_query(&g, s->ppx[i], s->ppy[i], p->sph_cell, buf, n);
+        int32_t k = grid_query(&g, s->ppx[i], s->ppy[i], buf, n);
         for (int32_t t = 0; t < k; ++t) {
             uint32_t j = buf[t];
             if ((int32_t)j == i) continue;

@@ -81,6 +81,8 @@ typedef struct {
     float sph_cell;        /* neighbour-grid cell size */
     float sph_unit_mass;   /* myVar.mass */
     float viscosity, cohesion_coef, delta, stiff_mult, sph_max_vel;
+    int32_t sph_search;       /* 0: NeighborSearchV2AVX2 semantics; 1: NeighborSearchV2 */
+    float sph_search_radius;  /* NeighborSearchV2::searchRadius (3.5) */
 } geo_params;
 
 /* bounding box: GE/src/globalLogic.cpp:1083-1099 -> {x, y, size} */
@@ -120,6 +122,19 @@ void geo_integrate_end(geo_state *s, float dt);
 int64_t geo_sph_candidates(int32_t n, const float *x, const float *y,
                            int32_t nq, const float *qx, const float *qy, float cell,
                            int64_t *off, uint32_t *idx, int64_t cap);
+/* The same with the search mode: mode 1 follows NeighborSearchV2 (the
+ * reference's fallback without AVX2, neighborSearch.h:439-599): cell =
+ * (int)(p * (1/cell)) (truncation), ceil(radius / cell) cells around the
+ * query cell, and |p_j - q|^2 <= radius^2 with p_j the grid position. */
+int64_t geo_sph_candidates_mode(int32_t n, const float *x, const float *y,
+                                int32_t nq, const float *qx, const float *qy, float cell,
+                                int32_t mode, float radius, int64_t *off, uint32_t *idx, int64_t cap);
+/* NeighborSearchV2::neighborAmount (neighborSearch.h:581-598): candidates
+ * (self included) of every particle over all particles at pos; pinned,
+ * drawn and dark-matter particles get 0 (the reference leaves them as they
+ * were). */
+void geo_neighbor_amount(int32_t n, const float *x, const float *y, const uint32_t *flags, float cell,
+                         float radius, uint32_t *counts);
 /* PCISPH (one iteration): GE/src/Physics/SPH.cpp:98-281 */
 int32_t geo_sph_pcisph(geo_state *s, const geo_params *p);
 /* SPH kernel constants computed as the reference does (double pow, float PI) */

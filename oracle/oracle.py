@@ -36,7 +36,7 @@ class _Params(C.Structure):
                 ("direct_below", C.c_int32), ("sph", C.c_int32), ("sph_h", C.c_float),
                 ("sph_cell", C.c_float), ("sph_unit_mass", C.c_float), ("viscosity", C.c_float),
                 ("cohesion_coef", C.c_float), ("delta", C.c_float), ("stiff_mult", C.c_float),
-                ("sph_max_vel", C.c_float)]
+                ("sph_max_vel", C.c_float), ("sph_search", C.c_int32), ("sph_search_radius", C.c_float)]
 
 
 _lib = None
@@ -69,6 +69,12 @@ def load():
     lib.geo_sort_perm.argtypes = [C.c_int32, u64p, u32p]
     lib.geo_build_tree.restype = C.c_int32
     lib.geo_build_tree.argtypes = [C.c_int32, f32p, f32p, f32p, f32p, C.c_void_p, C.c_int32]
+    lib.geo_sph_candidates_mode.restype = C.c_int64
+    lib.geo_sph_candidates_mode.argtypes = [C.c_int32, f32p, f32p, C.c_int32, f32p, f32p, C.c_float, C.c_int32,
+                                            C.c_float, C.POINTER(C.c_int64), C.POINTER(C.c_uint32), C.c_int64]
+    lib.geo_neighbor_amount.restype = None
+    lib.geo_neighbor_amount.argtypes = [C.c_int32, f32p, f32p, C.POINTER(C.c_uint32), C.c_float, C.c_float,
+                                        C.POINTER(C.c_uint32)]
     lib.geo_build_tree_cap.restype = C.c_int32
     lib.geo_build_tree_cap.argtypes = [C.c_int32, f32p, f32p, f32p, f32p, C.c_uint32, C.c_void_p, C.c_int32]
     lib.geo_bh_accel.argtypes = [C.c_int32, f32p, f32p, f32p, C.c_void_p, C.c_int32, C.c_float,
@@ -177,19 +183,29 @@ def direct_fp64(x, y, m, G, eps, targets=None):
     return ax, ay
 
 
-def sph_candidates(x, y, qx, qy, cell):
+def sph_candidates(x, y, qx, qy, cell, mode=0, radius=3.5):
+    """Candidate sets (CSR, ascending per query): mode 0 NeighborSearchV2AVX2,
+    mode 1 NeighborSearchV2 (truncated cells, radius filter)."""
     x, y, qx, qy = f32(x), f32(y), f32(qx), f32(qy)
     off = np.zeros(len(qx) + 1, np.int64)
     i64p = C.POINTER(C.c_int64)
     lib = load()
-    tot = lib.geo_sph_candidates(len(x), _f(x), _f(y), len(qx), _f(qx), _f(qy), cell,
-                                 off.ctypes.data_as(i64p), None, 0)
+    tot = lib.geo_sph_candidates_mode(len(x), _f(x), _f(y), len(qx), _f(qx), _f(qy), cell, mode, radius,
+                                      off.ctypes.data_as(i64p), None, 0)
     if tot < 0:
         tot = int(off[-1])
     idx = np.zeros(max(tot, 1), np.uint32)
-    lib.geo_sph_candidates(len(x), _f(x), _f(y), len(qx), _f(qx), _f(qy), cell,
-                           off.ctypes.data_as(i64p), _u(idx), tot)
+    lib.geo_sph_candidates_mode(len(x), _f(x), _f(y), len(qx), _f(qx), _f(qy), cell, mode, radius,
+                                off.ctypes.data_as(i64p), _u(idx), tot)
     return off, idx[:tot]
+
+
+def neighbor_amount(x, y, flags, cell=3.0, radius=3.5):
+    x, y = f32(x), f32(y)
+    fl = np.ascontiguousarray(flags, dtype=np.uint32)
+    out = np.zeros(len(x), np.uint32)
+    load().geo_neighbor_amount(len(x), _f(x), _f(y), _u(fl), cell, radius, _u(out))
+    return out
 
 
 def sph_constants(h):
@@ -228,7 +244,7 @@ class State:
         p = _Params()
         d = dict(G=1.0, theta=0.8, softening=1.0, dt=0.01, naive=0, direct_below=5000, sph=0,
                  sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3, cohesion_coef=1.0,
-                 delta=19000.0, stiff_mult=1.0, sph_max_vel=250.0)
+                 delta=19000.0, stiff_mult=1.0, sph_max_vel=250.0, sph_search=0, sph_search_radius=3.5)
         d.update({k: v for k, v in prm.items() if k in d or k == "dt_start"})
         d.setdefault("dt_start", d["dt"])
         for k, v in d.items():

@@ -235,3 +235,37 @@ def test_diagnostics_match_host_fp64():
     want["potential"] = -0.5 * float(m @ inv @ m)
     for k, v in want.items():
         assert g[k] == pytest.approx(v, rel=1e-10, abs=1e-9 * abs(want["kinetic"]) if k in ("px", "py", "lz") else 0), k
+
+
+# a record shaped like the reference's ParticlePhysics (particle.h:5-73);
+# field names that match ge_particles are transferred, the rest are not
+PARTICLE_PHYSICS = np.dtype([("x", "f4"), ("y", "f4"), ("vx", "f4"), ("vy", "f4"), ("ax", "f4"), ("ay", "f4"),
+                             ("ppx", "f4"), ("ppy", "f4"), ("pvx", "f4"), ("pvy", "f4"), ("predvx", "f4"),
+                             ("predvy", "f4"), ("pfx", "f4"), ("pfy", "f4"), ("morton", "u8"), ("mass", "f4"),
+                             ("press", "f4"), ("press_tmp", "f4"), ("dens", "f4"), ("pred_dens", "f4"),
+                             ("sph_mass", "f4"), ("rest_dens", "f4"), ("stiff", "f4"), ("visc", "f4"),
+                             ("cohesion", "f4"), ("temp", "f4"), ("ke", "f4"), ("prev_ke", "f4"), ("id", "u4"),
+                             ("neighbor_offset", "u4"), ("hot", "u1"), ("solid", "u1")], align=True)
+
+
+def test_aos_transfer_matches_soa():
+    st, _ = configs.config0(seed=28, n=7000)
+    n = len(st["x"])
+    rec = np.zeros(n, PARTICLE_PHYSICS)
+    for f in ("x", "y", "vx", "vy", "mass", "id"):
+        rec[f] = st[f]
+    rec["temp"] = 300.0
+    p = _params()
+    with Context(n) as a, Context(n) as b:
+        a.upload(st)
+        b.upload_aos(rec, flags=st["flags"])
+        a.step(p, 3)
+        b.step(p, 3)
+        a.sync()
+        b.sync()
+        da = a.download(FIELDS)
+        db, fl = b.download_aos(PARTICLE_PHYSICS, with_flags=True)
+    for f in FIELDS:
+        np.testing.assert_array_equal(db[f], da[f], err_msg=f)
+    assert np.all(db["temp"] == 0.0) and np.all(db["morton"] == 0)  # fields outside the path untouched
+    np.testing.assert_array_equal(fl, np.zeros(n, np.uint32))

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from galaxy_step import GE_FLAG_DRAWN, GE_FLAG_PINNED, GE_FLAG_SPH, Context, configs, default_params
+from galaxy_step import GE_FLAG_DARK, GE_FLAG_DRAWN, GE_FLAG_PINNED, GE_FLAG_SPH, Context, configs, default_params
 
 pytestmark = pytest.mark.gpu
 
@@ -164,3 +164,64 @@ def test_sph_requires_context_flag():
         c.upload({"x": np.arange(100.0), "y": np.zeros(100), "mass": np.ones(100)})
         with pytest.raises(Exception):
             c.step(_params(), 1)
+
+
+# ---- NeighborSearchV2 semantics (the reference's path without AVX2)
+
+@pytest.mark.parametrize("cell,radius", [(3.0, 3.5), (4.0, 4.5)])
+def test_neighbor_sets_v2_bitexact(cell, radius):
+    st = _sph_block(3000, 6)
+    st["x"] -= np.float32(40.0)  # negative coordinates: truncation differs from floor
+    st["x"][:4] -= np.float32(300.0)
+    p = _params(sph_cell=cell, sph_h=cell, sph_search=1, sph_search_radius=radius)
+    with Context(3000, sph=True) as c:
+        c.upload(st)
+        c.sph_new_grid(p)
+        off, idx = c.neighbors(0)
+        d = c.download(["x", "y"])
+        c.compute_gravity(p)
+        c.sph_pcisph(p)
+        off1, idx1 = c.neighbors(1)
+        d1 = c.download(["x", "y", "ppx", "ppy"])
+    ro, ri = O.sph_candidates(d["x"], d["y"], d["x"], d["y"], cell, mode=1, radius=radius)
+    np.testing.assert_array_equal(off, ro)
+    np.testing.assert_array_equal(idx, ri)
+    ro, ri = O.sph_candidates(d1["x"], d1["y"], d1["ppx"], d1["ppy"], cell, mode=1, radius=radius)
+    np.testing.assert_array_equal(off1, ro)
+    np.testing.assert_array_equal(idx1, ri)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_pcisph_v2_one_frame_matches_oracle(mixed):
+    n = 4000
+    st = _sph_block(n, 7)
+    st["x"] -= np.float32(30.0)
+    if mixed:
+        st = _mixed_flags(st, 8)
+    p = _params(sph_search=1, sph_search_radius=3.5)
+    s = O.State(st, sph=True)
+    s.step(dict(_oracle_prm(p), sph_search=1, sph_search_radius=3.5), nsteps=1)
+    ref = _by_id({k: s[k].copy() for k in ("x", "vx", "vy", "pred_dens", "press", "pfx", "pfy", "id")})
+    with Context(n, sph=True) as c:
+        c.upload(st)
+        c.step(p, 1)
+        c.sync()
+        got = _by_id(c.download(["x", "vx", "vy", "pred_dens", "press", "pfx", "pfy", "id"]))
+    np.testing.assert_array_equal(got["x"], ref["x"])
+    for f in ("pred_dens", "press", "pfx", "pfy", "vx", "vy"):
+        _close(got[f], ref[f], f)
+
+
+def test_neighbor_counts_match_oracle():
+    n = 5000
+    st = _sph_block(n, 9, spacing=1.1)
+    st["x"] -= np.float32(35.0)
+    st = _mixed_flags(st, 10)
+    st["flags"][::29] |= GE_FLAG_DARK
+    p = _params(sph_cell=3.0, sph_search_radius=3.5)
+    with Context(n, sph=True) as c:
+        c.upload(st)
+        got = c.neighbor_counts(p)
+        d = c.download(["x", "y", "flags"])
+    want = O.neighbor_amount(d["x"], d["y"], d["flags"], 3.0, 3.5)
+    np.testing.assert_array_equal(got, want)

@@ -345,3 +345,75 @@ def test_full_step_permutes_consistently():
     # the stored order is the Morton order of the current positions' frame
     assert np.all(np.isfinite(s["x"])) and np.all(np.isfinite(s["vx"]))
     np.testing.assert_array_equal(s["mass"], st["mass"][ids])
+
+
+def _v2_brute(x, y, qx, qy, cell, radius):
+    inv = np.float32(1.0) / np.float32(cell)
+    R = int(np.ceil(np.float32(radius) * inv))
+    cx = (x * inv).astype(np.int64)  # (int)(p * invCellSize): truncation toward zero
+    cy = (y * inv).astype(np.int64)
+    out = []
+    for q in range(len(qx)):
+        qcx, qcy = int(np.float32(qx[q]) * inv), int(np.float32(qy[q]) * inv)
+        dx = x - np.float32(qx[q])
+        dy = y - np.float32(qy[q])
+        d2 = dx * dx + dy * dy
+        ok = (np.abs(cx - qcx) <= R) & (np.abs(cy - qcy) <= R) & (d2 <= np.float32(radius) * np.float32(radius))
+        out.append(np.nonzero(ok)[0].astype(np.uint32))
+    return out
+
+
+@pytest.mark.parametrize("cell,radius", [(3.0, 3.5), (4.0, 4.5)])
+def test_sph_candidates_v2_brute_force(cell, radius):
+    # NeighborSearchV2 (neighborSearch.h:439-599): truncated cells around the
+    # origin (negative coordinates included), radius filter, self included
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-30, 30, 1500).astype(np.float32)
+    y = rng.uniform(-30, 30, 1500).astype(np.float32)
+    qx = (x + rng.normal(0, 0.5, 1500)).astype(np.float32)
+    qy = (y + rng.normal(0, 0.5, 1500)).astype(np.float32)
+    off, idx = O.sph_candidates(x, y, qx, qy, cell, mode=1, radius=radius)
+    want = _v2_brute(x, y, qx, qy, cell, radius)
+    for q in range(len(qx)):
+        np.testing.assert_array_equal(idx[off[q]:off[q + 1]], want[q])
+
+
+def test_neighbor_amount_brute_force():
+    rng = np.random.default_rng(13)
+    n = 1200
+    x = rng.uniform(-20, 20, n).astype(np.float32)
+    y = rng.uniform(-20, 20, n).astype(np.float32)
+    flags = np.zeros(n, np.uint32)
+    flags[::17] = 1   # pinned
+    flags[::23] |= 8  # dark matter
+    got = O.neighbor_amount(x, y, flags)
+    want = np.array([len(w) for w in _v2_brute(x, y, x, y, 3.0, 3.5)], np.uint32)
+    want[(flags & 9) != 0] = 0
+    np.testing.assert_array_equal(got, want)
+
+
+def test_pcisph_search_modes_agree_when_both_cover_h():
+    # with h <= cell (AVX2 3x3 stencil) and h <= search radius (V2 filter)
+    # both candidate sets contain every particle within h: same forces up to
+    # summation order
+    rng = np.random.default_rng(14)
+    n = 900
+    side = 30
+    gx, gy = np.meshgrid(np.arange(side), np.arange(side))
+    st = {"x": (gx.ravel()[:n] * 1.4 - 20 + rng.normal(0, 0.3, n)).astype(np.float32),
+          "y": (gy.ravel()[:n] * 1.4 - 20 + rng.normal(0, 0.3, n)).astype(np.float32),
+          "vx": rng.normal(0, 1, n).astype(np.float32), "vy": rng.normal(0, 1, n).astype(np.float32),
+          "mass": np.ones(n, np.float32), "flags": np.full(n, 2, np.uint32), "id": np.arange(n, dtype=np.uint32),
+          "sph_mass": np.ones(n, np.float32), "rest_dens": np.full(n, 0.008, np.float32),
+          "stiff": np.ones(n, np.float32), "visc": np.ones(n, np.float32), "cohesion": np.ones(n, np.float32),
+          "dens": np.zeros(n, np.float32)}
+    prm = dict(G=0.0, theta=0.7, softening=1.0, dt=0.002, dt_start=0.002, sph=1, sph_h=3.0, sph_cell=3.0)
+    res = []
+    for mode in (0, 1):
+        s = O.State(st, sph=True)
+        s.step(dict(prm, sph_search=mode, sph_search_radius=3.5), nsteps=1)
+        o = np.argsort(s["id"])
+        res.append({k: s[k][o].astype(np.float64) for k in ("pred_dens", "pfx", "pfy", "vx")})
+    for k in res[0]:
+        scale = np.abs(res[0][k]).max() or 1.0
+        assert np.abs(res[0][k] - res[1][k]).max() <= 1e-5 * scale, k
