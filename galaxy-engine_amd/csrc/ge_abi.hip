@@ -67,7 +67,8 @@ void free_all(ge_ctx *c) {
     b.bb = nullptr;
     TopNodes &T = b.top;
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
-    dfree(T.kind); dfree(T.count); dfree(T.pending); dfree(T.accum); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
+    dfree(T.kind); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
+    dfree(T.blist); dfree(T.blist_off); dfree(T.blist_cnt);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
     dfree(b.nodes); dfree(b.node_geom); dfree(b.node_mp); dfree(b.mp_cnt); dfree(b.mp_wave_list); dfree(b.mp_big_list); dfree(b.mp_piece_owner); dfree(b.mp_piece_sum); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
@@ -107,16 +108,18 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(T.size, T.cap)); GE_HIP_CHECK(c, dalloc(T.start, T.cap));
     GE_HIP_CHECK(c, dalloc(T.end, T.cap)); GE_HIP_CHECK(c, dalloc(T.parent, T.cap));
     GE_HIP_CHECK(c, dalloc(T.child, 4ull * T.cap)); GE_HIP_CHECK(c, dalloc(T.kind, T.cap));
-    GE_HIP_CHECK(c, dalloc(T.count, T.cap)); GE_HIP_CHECK(c, dalloc(T.pending, T.cap));
-    GE_HIP_CHECK(c, dalloc(T.accum, T.cap)); GE_HIP_CHECK(c, dalloc(T.pending2, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.pending2, T.cap));
     GE_HIP_CHECK(c, dalloc(T.offset, T.cap));
     GE_HIP_CHECK(c, dalloc(T.broot, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.blist, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.blist_off, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.blist_cnt, T.cap));
     GE_HIP_CHECK(c, dalloc(T.depth, T.cap));
     GE_HIP_CHECK(c, dalloc(b.start_count, (size_t)cap + 1));
     GE_HIP_CHECK(c, dalloc(b.start_level, (size_t)cap + 1));
     GE_HIP_CHECK(c, dalloc(b.scan_part, (size_t)cap / 1024 + 64));
     // zeroed top nodes read as "internal" (never emitted) until a build writes them
-    for (uint32_t *u : {T.start, T.end, T.parent, T.kind, T.count, T.pending, T.accum, T.pending2, T.offset, T.broot,
+    for (uint32_t *u : {T.start, T.end, T.parent, T.kind, T.pending2, T.offset, T.broot, T.blist_cnt,
                         T.depth})
         GE_HIP_CHECK(c, hipMemset(u, 0, sizeof(uint32_t) * T.cap));
     GE_HIP_CHECK(c, hipMemset(T.child, 0xFF, sizeof(uint32_t) * 4ull * T.cap));

@@ -47,9 +47,11 @@ enum CounterSlot : int {
     kCntTopTotal = 4,       // number of top nodes
     kCntTopAlloc = 5,       // next free top node (workgroup-subtree phase)
     kCntBlockRoots = 6,     // number of workgroup-subtree roots
+    kCntStamp = 7,          // tree builds so far (stamps nodes finished by the block mass pass)
     kCntLevel0 = 8,         // kCntLevel0 + L: nodes allocated on level L
     kCntLevelBase0 = 8 + 64,  // first top index of level L
-    kCntSlots = 8 + 128,
+    kCntBlistAlloc = 8 + 128, // entries of the per-subtree internal-node lists
+    kCntSlots = 8 + 136,
 };
 
 // One buffer set of particle state (structure of arrays).
@@ -73,13 +75,13 @@ struct TopNodes {
     float *px = nullptr, *py = nullptr, *size = nullptr;
     uint32_t *start = nullptr, *end = nullptr, *parent = nullptr;
     uint32_t *child = nullptr;     // 4 per node
-    uint32_t *kind = nullptr;      // 0 internal, 1 leaf, 2 in-thread subtree
-    uint32_t *count = nullptr;     // nodes in the subtree (final after the level phase)
-    uint32_t *pending = nullptr;   // children still to report their count
-    uint32_t *accum = nullptr;     // sum of reported child counts
-    uint32_t *pending2 = nullptr;  // children still to report their mass
+    uint32_t *kind = nullptr;      // 0 internal, 1 leaf, 2 wave subtree, 3 pending, 4 dead
+    uint32_t *pending2 = nullptr;  // build stamp of nodes whose mass the block pass computed
     uint32_t *offset = nullptr;    // pre-order index in the final array
     uint32_t *broot = nullptr;     // roots handed to the workgroup-subtree phase
+    uint32_t *blist = nullptr;     // internal nodes of each workgroup subtree, level by level
+    uint32_t *blist_off = nullptr; // per workgroup-subtree root: first entry in blist
+    uint32_t *blist_cnt = nullptr; // and its entry count (kNone: not listed)
     uint32_t *depth = nullptr;     // tree level of the node
     uint32_t cap = 0;
 };

@@ -28,8 +28,12 @@
 //   4. k_scan_* : the exclusive scan of C.
 //   5. k_tree_emit: every node's record at its pre-order index; wave
 //      subtrees and leaves also get their masses.
-//   6. k_tree_mass: one workgroup finishes the masses of the internal nodes
-//      above the wave subtrees, deepest first (children q = 0, 2, 1, 3).
+//   6. k_tree_block_mass: one workgroup per workgroup subtree finishes the
+//      masses of its internal nodes, deepest level first, from the list of
+//      internal nodes k_tree_blocks kept level by level.
+//   7. k_tree_mass: one workgroup finishes the rest (the nodes above the
+//      workgroup subtrees), deepest first.  Both sum children in the order
+//      q = 0, 2, 1, 3 of computeInternalMass.
 #include "ge_internal.hpp"
 
 namespace ge {
@@ -42,6 +46,7 @@ constexpr int kTreeWaves = kTreeBlock / 64;
 constexpr uint32_t kTreeGridMax = 8192;
 constexpr uint32_t kWaveSubMax = 64;
 constexpr uint32_t kFrontMax = 1024;  // LDS frontier of a workgroup subtree
+constexpr uint32_t kBlockListMax = 2048;  // internal nodes a workgroup subtree lists for the mass pass
 constexpr int kWaveLevels = 64;       // levels of a wave subtree
 
 // leaf_cap 0: the reference's rule (quadtree.cpp:16-17).  leaf_cap 1..64: a
@@ -447,6 +452,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTabl
             counters[kCntError] &= ~(kErrNodeCap | kErrDepth | kErrFrontier);
             counters[kCntBlockRoots] = 0;
             counters[kCntTopAlloc] = 0;
+            counters[kCntStamp] += 1u;
         }
         __syncthreads();
         if (t == 0) {
@@ -508,6 +514,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
                                                             uint32_t *__restrict__ counters) {
     __shared__ uint32_t front[2][kFrontMax];
     __shared__ uint32_t fcount[2];
+    __shared__ uint32_t blist[kBlockListMax];  // internal nodes, level by level
+    __shared__ uint32_t bcount, bbase;
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t nroots = min(counters[kCntBlockRoots], T.cap);
@@ -516,6 +524,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
             front[0][0] = T.broot[r];
             fcount[0] = 1;
             fcount[1] = 0;
+            bcount = 0;
         }
         __syncthreads();
         int cur = 0;
@@ -539,6 +548,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
                     } else {
                         register_start(st, s, depth);
                         write_children(T, node, px, py, size, depth, b, first, &front[cur ^ 1][slot]);
+                        const uint32_t li = atomicAdd(&bcount, 1u);
+                        if (li < kBlockListMax) blist[li] = node;
                     }
                 }
             }
@@ -548,6 +559,18 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
             __syncthreads();
         }
         if (lvl == kBlockLevelsMax && threadIdx.x == 0) atomicOr(&counters[kCntError], kErrFrontier);
+        // hand the internal-node list to the block mass pass (kNone: too long,
+        // the last mass pass takes these nodes instead)
+        const uint32_t nl = bcount;
+        if (threadIdx.x == 0) {
+            bbase = nl <= kBlockListMax ? atomicAdd(&counters[kCntBlistAlloc], nl) : kNone;
+            const bool fits = nl <= kBlockListMax && bbase + nl <= T.cap;
+            T.blist_off[r] = fits ? bbase : 0u;
+            T.blist_cnt[r] = fits ? nl : kNone;
+        }
+        __syncthreads();
+        if (nl <= kBlockListMax && bbase + nl <= T.cap)
+            for (uint32_t i = threadIdx.x; i < nl; i += kTreeBlock) T.blist[bbase + i] = blist[i];
         __syncthreads();
     }
 }
@@ -684,8 +707,71 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_emit(TopNodes T, StartTable
     }
 }
 
-// Masses of the internal top nodes, deepest first, by one workgroup: nodes
-// of one depth are independent; barriers order the depths.
+// computeInternalMass (quadtree.h:69-100) of top node t from its children's
+// final records: children q = 0, 2, 1, 3 (subGrids[lr][ud] loop order)
+__device__ __forceinline__ void internal_mass(const TopNodes &T, const TreeOut &o, uint32_t t) {
+#pragma clang fp contract(off)
+    float gm = 0.0f, cx = 0.0f, cy = 0.0f;
+    const int order[4] = {0, 2, 1, 3};
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t ch = T.child[4 * t + order[k]];
+        if (ch == kNone || ch >= T.cap) continue;
+        const uint32_t oc = T.offset[ch];
+        if (oc >= o.cap) continue;
+        const NodeRec r = o.nodes[oc];
+        gm = gm + r.mass;
+        cx = cx + r.comx * r.mass;
+        cy = cy + r.comy * r.mass;
+    }
+    if (gm > 0) { cx = cx / gm; cy = cy / gm; }
+    const uint32_t ot = T.offset[t];
+    if (ot < o.cap) {
+        const float sz = T.size[t];
+        reinterpret_cast<float4 *>(o.nodes)[2 * ot] = make_float4(cx, cy, gm, sz * sz);
+    }
+}
+
+// Masses of each workgroup subtree's internal nodes, deepest level first:
+// its list holds the levels in order, so each level is one range.
+__global__ __launch_bounds__(kTreeBlock) void k_tree_block_mass(TopNodes T, uint32_t *__restrict__ counters,
+                                                                TreeOut o) {
+    __shared__ uint32_t lstart[kBlockLevelsMax + 2];
+    __shared__ uint32_t maxl;
+    const uint32_t stamp = counters[kCntStamp];
+    const uint32_t nroots = min(counters[kCntBlockRoots], T.cap);
+    for (uint32_t r = blockIdx.x; r < nroots; r += gridDim.x) {
+        const uint32_t cnt = T.blist_cnt[r];
+        if (cnt == kNone || cnt == 0) continue;  // uniform across the workgroup
+        const uint32_t base = T.blist_off[r];
+        const uint32_t d0 = T.depth[T.blist[base]];
+        if (threadIdx.x == 0) maxl = T.depth[T.blist[base + cnt - 1]] - d0;
+        for (uint32_t l = threadIdx.x; l < kBlockLevelsMax + 2; l += kTreeBlock) lstart[l] = cnt;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < cnt; i += kTreeBlock) {
+            const uint32_t l = T.depth[T.blist[base + i]] - d0;
+            const uint32_t lp = i == 0 ? kNone : T.depth[T.blist[base + i - 1]] - d0;
+            if (l != lp && l <= kBlockLevelsMax) lstart[l] = i;
+        }
+        __syncthreads();
+        const uint32_t ml = maxl;
+        if (ml <= kBlockLevelsMax) {
+            for (int l = (int)ml; l >= 0; --l) {
+                const uint32_t lo = min(lstart[l], cnt), hi = l == (int)ml ? cnt : min(lstart[l + 1], cnt);
+                for (uint32_t i = lo + threadIdx.x; i < hi; i += kTreeBlock) {
+                    const uint32_t t = T.blist[base + i];
+                    internal_mass(T, o, t);
+                    T.pending2[t] = stamp;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Masses of the remaining internal top nodes (above the workgroup subtrees,
+// or in subtrees whose list overflowed), deepest first, by one workgroup:
+// nodes of one depth are independent; barriers order the depths.
 constexpr int kMassThreads = 1024;
 constexpr int kMaxTopDepth = 256;
 
@@ -693,14 +779,21 @@ __global__ __launch_bounds__(kMassThreads) void k_tree_mass(TopNodes T, uint32_t
 #pragma clang fp contract(off)
     __shared__ uint32_t hist[kMaxTopDepth + 1];
     __shared__ uint32_t cursor[kMaxTopDepth];
+    __shared__ uint32_t maxd;
     const uint32_t total = counters[kCntTopTotal];
+    const uint32_t stamp = counters[kCntStamp];
     for (uint32_t d = threadIdx.x; d <= kMaxTopDepth; d += kMassThreads) hist[d] = 0;
+    if (threadIdx.x == 0) maxd = 0;
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < total; t += kMassThreads) {
-        if (T.kind[t] != kKindInternal) continue;
+        if (T.kind[t] != kKindInternal || T.pending2[t] == stamp) continue;
         const uint32_t d = T.depth[t];
-        if (d < kMaxTopDepth) atomicAdd(&hist[d], 1u);
-        else atomicOr(&counters[kCntError], kErrFrontier);
+        if (d < kMaxTopDepth) {
+            atomicAdd(&hist[d], 1u);
+            atomicMax(&maxd, d);
+        } else {
+            atomicOr(&counters[kCntError], kErrFrontier);
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -715,34 +808,14 @@ __global__ __launch_bounds__(kMassThreads) void k_tree_mass(TopNodes T, uint32_t
     }
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < total; t += kMassThreads) {
-        if (T.kind[t] != kKindInternal) continue;
+        if (T.kind[t] != kKindInternal || T.pending2[t] == stamp) continue;
         const uint32_t d = T.depth[t];
         if (d < kMaxTopDepth) T.broot[atomicAdd(&cursor[d], 1u)] = t;
     }
     __syncthreads();
-    for (int d = kMaxTopDepth - 1; d >= 0; --d) {
+    for (int d = (int)maxd; d >= 0; --d) {
         const uint32_t lo = hist[d], hi = hist[d + 1];
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += kMassThreads) {
-            const uint32_t t = T.broot[i];
-            float gm = 0.0f, cx = 0.0f, cy = 0.0f;
-            const int order[4] = {0, 2, 1, 3};
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t ch = T.child[4 * t + order[k]];
-                if (ch == kNone || ch >= T.cap) continue;
-                const uint32_t oc = T.offset[ch];
-                if (oc >= o.cap) continue;
-                const NodeRec r = o.nodes[oc];
-                gm = gm + r.mass;
-                cx = cx + r.comx * r.mass;
-                cy = cy + r.comy * r.mass;
-            }
-            if (gm > 0) { cx = cx / gm; cy = cy / gm; }
-            const uint32_t ot = T.offset[t];
-            if (ot < o.cap) {
-                const float sz = T.size[t];
-                reinterpret_cast<float4 *>(o.nodes)[2 * ot] = make_float4(cx, cy, gm, sz * sz);
-            }
-        }
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += kMassThreads) internal_mass(T, o, T.broot[i]);
         __syncthreads();
     }
 }
@@ -789,6 +862,7 @@ void launch_tree_build(ge_ctx *c) {
     if (top_bound > T.cap) top_bound = T.cap;
     hipLaunchKernelGGL(k_tree_emit, dim3(wave_grid(top_bound)), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y, s.m,
                        c->b.counters, o, c->n);
+    hipLaunchKernelGGL(k_tree_block_mass, dim3((uint32_t)roots), dim3(kTreeBlock), 0, c->stream, T, c->b.counters, o);
     hipLaunchKernelGGL(k_tree_mass, dim3(1), dim3(kMassThreads), 0, c->stream, T, c->b.counters, o);
 }
 
