@@ -279,9 +279,9 @@ int fail(ge_ctx *c, int code, const char *msg) {
 
 int check_params(ge_ctx *c, const ge_step_params *p) {
     if (!p) return fail(c, GE_E_INVALID, "params is NULL");
-    if (p->periodic) return fail(c, GE_E_UNSUPPORTED, "periodic boundary is not implemented (infiniteDomain only)");
     if (p->temperature) return fail(c, GE_E_UNSUPPORTED, "temperature transfer is not implemented");
-    if (p->sph_ground) return fail(c, GE_E_UNSUPPORTED, "SPH ground walls are not implemented");
+    if ((p->periodic || p->sph_ground) && !(p->domain_w > 0.0f && p->domain_h > 0.0f))
+        return fail(c, GE_E_INVALID, "periodic and ground modes need domain_w, domain_h > 0");
     if (p->sph && !c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
     if (p->sph && c->nsys > 1) return fail(c, GE_E_UNSUPPORTED, "SPH runs on single-system contexts");
     if (p->sph && !(p->sph_cell > 0.0f && p->sph_h > 0.0f)) return fail(c, GE_E_INVALID, "sph_h and sph_cell must be > 0");
@@ -346,7 +346,10 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
     c->leaf_cap = p->leaf_capacity;
     if (dt_start != 0.0f) {
         // integrateStart + boundingBox, then computeMortonKeys, the sort and the tree
-        { PhaseScope s(c, GE_PHASE_INTEGRATE_BBOX); launch_integrate_start_bbox(c, dt_start, p->sph, p->sph_max_vel, 1); }
+        {
+            PhaseScope s(c, GE_PHASE_INTEGRATE_BBOX);
+            launch_integrate_start_bbox(c, dt_start, p->sph, p->sph_max_vel, 1, domain_of(p));
+        }
         { PhaseScope s(c, GE_PHASE_KEYS); launch_morton_keys(c); }
         { PhaseScope s(c, GE_PHASE_SORT); launch_sort_keys(c); }
         { PhaseScope s(c, GE_PHASE_GATHER); launch_gather(c); }
@@ -362,6 +365,7 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
         if (int r = sph_new_grid(c, p)) return r;
     }
     const float end_kick = p->sph ? 0.0f : p->dt * 0.5f;  // fused integrateEnd when no SPH follows
+    const bool ground = p->sph && p->sph_ground;           // groundModeBoundary runs before integrateEnd
     if (p->G != 0.0f) {
         if (use_direct(c, p) || !c->have_tree) {
             PhaseScope s(c, GE_PHASE_GRAVITY);
@@ -383,7 +387,11 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
     }
     if (p->sph) {
         PhaseScope s(c, GE_PHASE_SPH);
-        if (int r = sph_pcisph(c, p, p->dt)) return r;
+        if (int r = sph_pcisph(c, p, ground ? 0.0f : p->dt)) return r;
+        if (ground) {
+            launch_sph_ground(c, p);
+            launch_integrate_end(c, p->dt);
+        }
     }
     return GE_OK;
 }
@@ -472,6 +480,10 @@ void ge_default_params(ge_step_params *p) {
     p->sph_max_vel = 250.0f;   // parameters.h:118
     p->sph_search = 0;
     p->sph_search_radius = 3.5f;  // NeighborSearchV2::searchRadius (neighborSearch.h:441)
+    p->domain_w = 3840.0f;        // domainSize (parameters.h:97)
+    p->domain_h = 2160.0f;
+    p->vertical_gravity = 3.0f;   // parameters.h:316
+    p->boundary_friction = 0.0f;  // parameters.h:361
     p->compensated = 1;
 }
 
@@ -764,7 +776,7 @@ int ge_integrate_start(ge_ctx *c, const ge_step_params *p) {
     if (int r = check_params(c, p)) return r;
     if (c->n == 0) return GE_OK;
     (void)hipSetDevice(c->device);
-    launch_integrate_start_bbox(c, p->dt_start, p->sph, p->sph_max_vel, 1);
+    launch_integrate_start_bbox(c, p->dt_start, p->sph, p->sph_max_vel, 1, domain_of(p));
     return ge_sync(c);
 }
 
@@ -797,6 +809,7 @@ int ge_sph_pcisph(ge_ctx *c, const ge_step_params *p) {
     if (c->n == 0) return GE_OK;
     (void)hipSetDevice(c->device);
     if (int r = sph_pcisph(c, p, 0.0f)) return r;
+    if (p->sph_ground) launch_sph_ground(c, p);  // pcisphSolver (SPH.h:623-632)
     return ge_sync(c);
 }
 

@@ -54,6 +54,17 @@ enum CounterSlot : int {
     kCntSlots = 8 + 136,
 };
 
+// Periodic domain of the reference's isPeriodicBoundaryEnabled && !infiniteDomain
+// (w = 0: no wrap).
+struct Domain {
+    float w = 0.0f, h = 0.0f;
+};
+
+// d -= size * ((d > size/2) - (d < -size/2))  (physics.cpp:117-120, 169-172, 417-420)
+__device__ __forceinline__ float wrap_d(float d, float size, float half) {
+    return d - size * (float)((d > half) - (d < -half));
+}
+
 // One buffer set of particle state (structure of arrays).
 struct State {
     float *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr;
@@ -191,7 +202,10 @@ namespace ge {
 void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n,
                        int lo_bit, int hi_bit);
 // step.hip
-void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate);
+void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate,
+                                 Domain dom = Domain{});
+Domain domain_of(const ge_step_params *p);
+void launch_sph_ground(ge_ctx *c, const ge_step_params *p);
 void launch_morton_keys(ge_ctx *c);
 void launch_sort_keys(ge_ctx *c);
 void launch_gather(ge_ctx *c);

@@ -43,7 +43,7 @@ __device__ __forceinline__ void block_minmax(float &a, float &b, float &c, float
 // k_bbox_final (a second launch instead of a device-scope fence).
 __global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, const uint32_t *__restrict__ chunk_lo, float dt,
                                                            int sph_clamp, float max_vel, int do_integrate,
-                                                           float *__restrict__ partials) {
+                                                           Domain dom, float *__restrict__ partials) {
 #pragma clang fp contract(off)
     const float half_dt = dt * 0.5f;
     const float max_vsq = max_vel * max_vel;
@@ -74,6 +74,12 @@ __global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, const uint32
             s.vx[i] = vx; s.vy[i] = vy;
             x = x + vx * dt;
             y = y + vy * dt;
+            if (dom.w > 0.0f) {  // physics.cpp:1104-1110
+                if (x < 0.0f) x += dom.w;
+                else if (x >= dom.w) x -= dom.w;
+                if (y < 0.0f) y += dom.h;
+                else if (y >= dom.h) y -= dom.h;
+            }
             s.x[i] = x; s.y[i] = y;
         }
         mnx = glm_min(mnx, x); mny = glm_min(mny, y);
@@ -234,9 +240,11 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                                            float *__restrict__ ax, float *__restrict__ ay,
                                            float *__restrict__ vx, float *__restrict__ vy,
                                            const uint32_t *__restrict__ flags, float half_dt_end,
-                                           unsigned long long *__restrict__ inter) {
+                                           Domain dom, unsigned long long *__restrict__ inter) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * BS + threadIdx.x;
+    const bool per = dom.w > 0.0f;
+    const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
     const bool live = i < n;
     const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     // a tree that raised an error flag may hold unwritten nodes: do not walk it
@@ -277,6 +285,10 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
             h0 = mp[6ull * idx];
             dx = h0.x - p.x;
             dy = h0.y - p.y;
+            if (per) {
+                dx = wrap_d(dx, dom.w, hw);
+                dy = wrap_d(dy, dom.h, hh);
+            }
             const float d2 = dx * dx + dy * dy;
             dist_sq = d2 + soft_sq;
             accept = d2 > h0.z * h0.z;
@@ -284,6 +296,10 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
 #pragma clang fp contract(off)
             dx = nd.comx - p.x;
             dy = nd.comy - p.y;
+            if (per) {  // scalar variant (physics.cpp:117-120)
+                dx = wrap_d(dx, dom.w, hw);
+                dy = wrap_d(dy, dom.h, hh);
+            }
             dist_sq = dx * dx + dy * dy + soft_sq;
             accept = nd.size_sq < theta_sq * dist_sq;
         }
@@ -351,7 +367,11 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
             for (uint32_t j = nd.start; j < jend; ++j) {
                 const float4 q = pxym[j];
                 if (open_leaf && j != i) {
-                    const float px = q.x - p.x, py = q.y - p.y;
+                    float px = q.x - p.x, py = q.y - p.y;
+                    if (per) {  // physics.cpp:169-172
+                        px = wrap_d(px, dom.w, hw);
+                        py = wrap_d(py, dom.h, hh);
+                    }
                     const float d2 = px * px + py * py + soft_sq;
                     const float inv = __builtin_amdgcn_rsqf(d2);
                     const float f = G * q.z * (inv * inv * inv);
@@ -402,7 +422,9 @@ __global__ __launch_bounds__(kBlock) void k_direct(const float4 *__restrict__ px
                                                    float *__restrict__ ay, float *__restrict__ vx,
                                                    float *__restrict__ vy,
                                                    const uint32_t *__restrict__ flags,
-                                                   float half_dt_end) {
+                                                   float half_dt_end, Domain dom) {
+    const bool per = dom.w > 0.0f;
+    const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
     __shared__ float4 tile[kBlock];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const float4 p = i < n ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -414,7 +436,11 @@ __global__ __launch_bounds__(kBlock) void k_direct(const float4 *__restrict__ px
         const uint32_t lim = min((uint32_t)kBlock, n - base);
         for (uint32_t k = 0; k < lim; ++k) {
             const float4 q = tile[k];
-            const float dx = q.x - p.x, dy = q.y - p.y;
+            float dx = q.x - p.x, dy = q.y - p.y;
+            if (per) {  // physics.cpp:417-420
+                dx = wrap_d(dx, dom.w, hw);
+                dy = wrap_d(dy, dom.h, hh);
+            }
             const float d2 = dx * dx + dy * dy + eps;
             const float inv = __builtin_amdgcn_rsqf(d2);
             const float f = G * q.z * (inv * inv * inv);
@@ -481,6 +507,23 @@ __global__ __launch_bounds__(kBlock) void k_zero2(float *__restrict__ a, float *
     if (i < n) { a[i] = 0.f; b[i] = 0.f; }
 }
 
+// SPH::groundModeBoundary (SPH.cpp:283-322): vertical gravity and four
+// walls at distance radiusMultiplier (= h) from the domain edges
+__global__ __launch_bounds__(kBlock) void k_sph_ground(State s, uint32_t n, float radius, float w, float h,
+                                                       float vertical_gravity, float friction) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n || (s.flags[i] & GE_FLAG_PINNED)) return;
+    const float damp = -0.1f;  // SPH::boundDamping (SPH.h:18)
+    float x = s.x[i], y = s.y[i], vx = s.vx[i], vy = s.vy[i];
+    s.ay[i] = s.ay[i] + vertical_gravity;
+    if (x - radius < 0.0f) { vx *= damp; vy *= 1.0f - friction; x = radius; }
+    if (x + radius > w) { vx *= damp; vy *= 1.0f - friction; x = w - radius; }
+    if (y - radius < 0.0f) { vy *= damp; vx *= 1.0f - friction; y = radius; }
+    if (y + radius > h) { vy *= damp; vx *= 1.0f - friction; y = h - radius; }
+    s.x[i] = x; s.y[i] = y; s.vx[i] = vx; s.vy[i] = vy;
+}
+
 // Physics::integrateEnd (physics.cpp:1114-1125)
 __global__ __launch_bounds__(kBlock) void k_integrate_end(State s, uint32_t n, float half_dt) {
 #pragma clang fp contract(off)
@@ -492,9 +535,18 @@ __global__ __launch_bounds__(kBlock) void k_integrate_end(State s, uint32_t n, f
 
 }  // namespace
 
-void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate) {
+Domain domain_of(const ge_step_params *p) {
+    Domain d;
+    if (p->periodic) {
+        d.w = p->domain_w;
+        d.h = p->domain_h;
+    }
+    return d;
+}
+
+void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate, Domain dom) {
     hipLaunchKernelGGL(k_integrate_bbox, dim3(c->nchunks), dim3(kBlock), 0, c->stream, c->b.st[c->b.cur],
-                       c->b.chunk_lo, dt, sph_clamp, max_vel, do_integrate, c->b.partials);
+                       c->b.chunk_lo, dt, sph_clamp, max_vel, do_integrate, dom, c->b.partials);
     hipLaunchKernelGGL(k_bbox_final, dim3(c->nsys), dim3(kBlock), 0, c->stream, c->b.partials, c->b.chunk_first,
                        c->b.sys_bb);
 }
@@ -530,7 +582,7 @@ void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
     hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym, c->b.nodes,
                        c->b.node_mp, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n, p->G,
-                       theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, inter);
+                       theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p), inter);
 }
 
 template <int BS, int ORDER>
@@ -561,7 +613,7 @@ void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool 
 void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const State &s = c->b.st[c->b.cur];
     hipLaunchKernelGGL(k_direct, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.pxym, c->n,
-                       p->G, p->softening, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt);
+                       p->G, p->softening, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p));
 }
 
 void launch_sys_of(ge_ctx *c) {
@@ -579,6 +631,11 @@ void launch_compact(ge_ctx *c, uint32_t n_kept) {
 void launch_zero_acc(ge_ctx *c) {
     const State &s = c->b.st[c->b.cur];
     hipLaunchKernelGGL(k_zero2, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, s.ax, s.ay, c->n);
+}
+
+void launch_sph_ground(ge_ctx *c, const ge_step_params *p) {
+    hipLaunchKernelGGL(k_sph_ground, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.st[c->b.cur],
+                       c->n, p->sph_h, p->domain_w, p->domain_h, p->vertical_gravity, p->boundary_friction);
 }
 
 void launch_integrate_end(ge_ctx *c, float dt) {

@@ -80,7 +80,8 @@ class StepParams(C.Structure):
                 ("sph_ground", C.c_int32), ("compensated", C.c_int32), ("accuracy", C.c_int32),
                 ("leaf_capacity", C.c_uint32), ("walk_block", C.c_uint32), ("use_graph", C.c_int32),
                 ("sph_search", C.c_int32), ("sph_search_radius", C.c_float),
-                ("reserved", C.c_uint32 * 2)]
+                ("domain_w", C.c_float), ("domain_h", C.c_float), ("vertical_gravity", C.c_float),
+                ("boundary_friction", C.c_float), ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -89,9 +89,13 @@ typedef struct {
     float delta;          /* myVar.delta */
     float stiff_mult;     /* myVar.stiffMultiplier */
     float sph_max_vel;    /* myVar.sphMaxVel */
-    int32_t periodic;     /* isPeriodicBoundaryEnabled && !infiniteDomain: unsupported */
+    int32_t periodic;     /* isPeriodicBoundaryEnabled && !infiniteDomain: positions wrap
+                             into the domain (physics.cpp:1104-1110) and every
+                             separation takes the nearest image (physics.cpp:117-120,
+                             169-172, 417-420) */
     int32_t temperature;  /* isTempEnabled: unsupported */
-    int32_t sph_ground;   /* sphGround: unsupported */
+    int32_t sph_ground;   /* sphGround: SPH::groundModeBoundary after PCISPH
+                             (SPH.cpp:283-322) */
     int32_t compensated;  /* Kahan-compensated force accumulation (1 = on) */
     int32_t accuracy;     /* 0: the reference's monopole walk and opening test;
                              1: quadrupole, 2: moments up to the hexadecapole,
@@ -113,6 +117,9 @@ typedef struct {
                              cells, ceil(radius / cell) cells each way,
                              |p_j - q| <= sph_search_radius) */
     float sph_search_radius; /* NeighborSearchV2::searchRadius (3.5) */
+    float domain_w, domain_h; /* myVar.domainSize (parameters.h:97) */
+    float vertical_gravity;   /* myVar.verticalGravity (ground mode) */
+    float boundary_friction;  /* myVar.boundaryFriction (ground mode) */
     uint32_t reserved[2];
 } ge_step_params;
 

@@ -250,6 +250,20 @@ int32_t geo_build_tree_cap(int32_t n, const float *x, const float *y, const floa
 void geo_bh_accel(int32_t n, const float *x, const float *y, const float *m,
                   const geo_node *nodes, int32_t nn, float G, float theta, float eps,
                   float *ax, float *ay, uint64_t *interactions) {
+    const geo_domain none = {0.0f, 0.0f};
+    geo_bh_accel_dom(n, x, y, m, nodes, nn, G, theta, eps, &none, ax, ay, interactions);
+}
+
+/* d -= domain * ((d > half) - (d < -half)), physics.cpp:117-120 */
+static inline float wrap(float d, float size, float half) {
+    return d - size * (float)((d > half) - (d < -half));
+}
+
+void geo_bh_accel_dom(int32_t n, const float *x, const float *y, const float *m,
+                      const geo_node *nodes, int32_t nn, float G, float theta, float eps,
+                      const geo_domain *dom, float *ax, float *ay, uint64_t *interactions) {
+    const int per = dom->w > 0.0f;
+    const float hw = dom->w * 0.5f, hh = dom->h * 0.5f;
     const float thetaSq = theta * theta;
     const float softSq = eps * eps;
     uint64_t total = 0;
@@ -266,6 +280,7 @@ void geo_bh_accel(int32_t n, const float *x, const float *y, const float *m,
             if (gmass <= 0.0f) { idx += g->next + 1; continue; }
             float dx = g->comx - xi;
             float dy = g->comy - yi;
+            if (per) { dx = wrap(dx, dom->w, hw); dy = wrap(dy, dom->h, hh); }
             float distSq = dx * dx + dy * dy + softSq;
             int leaf = g->next == 0; /* all subGrids empty */
             float sizeSq = g->size * g->size;
@@ -283,6 +298,7 @@ void geo_bh_accel(int32_t n, const float *x, const float *y, const float *m,
                     if ((uint32_t)i == j) continue;
                     float px = x[j] - xi;
                     float py = y[j] - yi;
+                    if (per) { px = wrap(px, dom->w, hw); py = wrap(py, dom->h, hh); }
                     float d2 = px * px + py * py + softSq;
                     float inv = 1.0f / sqrtf(d2);
                     float inv2 = inv * inv;
@@ -307,6 +323,14 @@ void geo_bh_accel(int32_t n, const float *x, const float *y, const float *m,
 /* Physics::naiveGravity, GE/src/Physics/physics.cpp:392-434 */
 void geo_direct_accel(int32_t n, const float *x, const float *y, const float *m,
                       float G, float eps, float *ax, float *ay) {
+    const geo_domain none = {0.0f, 0.0f};
+    geo_direct_accel_dom(n, x, y, m, G, eps, &none, ax, ay);
+}
+
+void geo_direct_accel_dom(int32_t n, const float *x, const float *y, const float *m,
+                          float G, float eps, const geo_domain *dom, float *ax, float *ay) {
+    const int per = dom->w > 0.0f;
+    const float hw = dom->w * 0.5f, hh = dom->h * 0.5f;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int32_t i = 0; i < n; ++i) {
         float pix = x[i], piy = y[i];
@@ -314,6 +338,7 @@ void geo_direct_accel(int32_t n, const float *x, const float *y, const float *m,
         for (int32_t j = 0; j < n; ++j) {
             float dx = x[j] - pix;
             float dy = y[j] - piy;
+            if (per) { dx = wrap(dx, dom->w, hw); dy = wrap(dy, dom->h, hh); } /* physics.cpp:417-420 */
             float distSq = dx * dx + dy * dy + eps; /* softening not squared */
             float inv = 1.0f / sqrtf(distSq);
             float inv3 = inv * inv * inv;
@@ -353,6 +378,11 @@ void geo_direct_fp64(int32_t n, const float *x, const float *y, const float *m,
 /* ---------------------------------------------------------- integrator --- */
 /* Physics::integrateStart, GE/src/Physics/physics.cpp:1069-1112 */
 void geo_integrate_start(geo_state *s, float dt, int32_t sph, float sph_max_vel) {
+    const geo_domain none = {0.0f, 0.0f};
+    geo_integrate_start_dom(s, dt, sph, sph_max_vel, &none);
+}
+
+void geo_integrate_start_dom(geo_state *s, float dt, int32_t sph, float sph_max_vel, const geo_domain *dom) {
     const float halfDt = dt * 0.5f;
     const float maxVSq = sph_max_vel * sph_max_vel;
     for (int32_t i = 0; i < s->n; ++i) {
@@ -378,6 +408,12 @@ void geo_integrate_start(geo_state *s, float dt, int32_t sph, float sph_max_vel)
         s->vx[i] = vx; s->vy[i] = vy;
         s->x[i] = s->x[i] + vx * dt;
         s->y[i] = s->y[i] + vy * dt;
+        if (dom->w > 0.0f) { /* physics.cpp:1104-1110 */
+            if (s->x[i] < 0.0f) s->x[i] += dom->w;
+            else if (s->x[i] >= dom->w) s->x[i] -= dom->w;
+            if (s->y[i] < 0.0f) s->y[i] += dom->h;
+            else if (s->y[i] >= dom->h) s->y[i] -= dom->h;
+        }
     }
 }
 
@@ -686,6 +722,26 @@ int32_t geo_sph_pcisph(geo_state *s, const geo_params *p) {
     return 0;
 }
 
+void geo_sph_ground(geo_state *s, float radius, float w, float h, float vertical_gravity, float friction) {
+    const float damp = -0.1f; /* SPH::boundDamping, SPH.h:18 */
+    for (int32_t i = 0; i < s->n; ++i) {
+        if (s->flags[i] & GEO_PINNED) continue;
+        s->ay[i] += vertical_gravity;
+        if (s->x[i] - radius < 0.0f) { /* left */
+            s->vx[i] *= damp; s->vy[i] *= 1.0f - friction; s->x[i] = radius;
+        }
+        if (s->x[i] + radius > w) { /* right */
+            s->vx[i] *= damp; s->vy[i] *= 1.0f - friction; s->x[i] = w - radius;
+        }
+        if (s->y[i] - radius < 0.0f) { /* bottom */
+            s->vy[i] *= damp; s->vx[i] *= 1.0f - friction; s->y[i] = radius;
+        }
+        if (s->y[i] + radius > h) { /* top */
+            s->vy[i] *= damp; s->vx[i] *= 1.0f - friction; s->y[i] = h - radius;
+        }
+    }
+}
+
 /* ---------------------------------------------------------- full step --- */
 static void permute_f(float *a, const uint32_t *perm, int32_t n, float *tmp) {
     if (!a) return;
@@ -715,6 +771,8 @@ static void permute_state(geo_state *s, const uint32_t *perm) {
 int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps) {
     const int32_t n = s->n;
     if (n <= 0) return 0;
+    geo_domain dom = {0.0f, 0.0f};
+    if (p->periodic) { dom.w = p->domain_w; dom.h = p->domain_h; }
     uint64_t *keys = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n);
     uint32_t *perm = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)n);
     for (int32_t step = 0; step < nsteps; ++step) {
@@ -722,7 +780,7 @@ int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps) {
         geo_node *nodes = NULL;
         int64_t nn = 0;
         if (dt0 != 0.0f) {
-            geo_integrate_start(s, dt0, p->sph, p->sph_max_vel);
+            geo_integrate_start_dom(s, dt0, p->sph, p->sph_max_vel, &dom);
             float bb[3];
             geo_bbox(n, s->x, s->y, bb);
             geo_morton_keys(n, s->x, s->y, bb, keys);
@@ -737,12 +795,17 @@ int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps) {
             for (int32_t i = 0; i < n; ++i) { s->ax[i] = 0.0f; s->ay[i] = 0.0f; }
             if (p->G != 0.0f) {
                 if (p->naive || n < p->direct_below || !nodes)
-                    geo_direct_accel(n, s->x, s->y, s->m, p->G, p->softening, s->ax, s->ay);
+                    geo_direct_accel_dom(n, s->x, s->y, s->m, p->G, p->softening, &dom, s->ax, s->ay);
                 else
-                    geo_bh_accel(n, s->x, s->y, s->m, nodes, (int32_t)nn, p->G, p->theta,
-                                 p->softening, s->ax, s->ay, NULL);
+                    geo_bh_accel_dom(n, s->x, s->y, s->m, nodes, (int32_t)nn, p->G, p->theta,
+                                     p->softening, &dom, s->ax, s->ay, NULL);
             }
-            if (p->sph) geo_sph_pcisph(s, p);
+            if (p->sph) {
+                geo_sph_pcisph(s, p);
+                if (p->sph_ground)
+                    geo_sph_ground(s, p->sph_h, p->domain_w, p->domain_h, p->vertical_gravity,
+                                   p->boundary_friction);
+            }
             geo_integrate_end(s, p->dt);
         }
         free(nodes);

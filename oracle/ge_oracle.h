@@ -83,7 +83,15 @@ typedef struct {
     float viscosity, cohesion_coef, delta, stiff_mult, sph_max_vel;
     int32_t sph_search;       /* 0: NeighborSearchV2AVX2 semantics; 1: NeighborSearchV2 */
     float sph_search_radius;  /* NeighborSearchV2::searchRadius (3.5) */
+    int32_t periodic;         /* isPeriodicBoundaryEnabled && !infiniteDomain */
+    float domain_w, domain_h; /* myVar.domainSize */
+    int32_t sph_ground;       /* myVar.sphGround: SPH::groundModeBoundary after PCISPH */
+    float vertical_gravity;   /* myVar.verticalGravity */
+    float boundary_friction;  /* myVar.boundaryFriction */
 } geo_params;
+
+/* domain of the periodic wrap: width and height, or 0 for no wrap */
+typedef struct { float w, h; } geo_domain;
 
 /* bounding box: GE/src/globalLogic.cpp:1083-1099 -> {x, y, size} */
 void geo_bbox(int32_t n, const float *x, const float *y, float out[3]);
@@ -105,15 +113,26 @@ int32_t geo_build_tree_cap(int32_t n, const float *x, const float *y, const floa
 void geo_bh_accel(int32_t n, const float *x, const float *y, const float *m,
                   const geo_node *nodes, int32_t nn, float G, float theta, float eps,
                   float *ax, float *ay, uint64_t *interactions);
+/* the same with the periodic wrap of every separation (physics.cpp:117-120,
+ * 169-172) when dom->w > 0 */
+void geo_bh_accel_dom(int32_t n, const float *x, const float *y, const float *m,
+                      const geo_node *nodes, int32_t nn, float G, float theta, float eps,
+                      const geo_domain *dom, float *ax, float *ay, uint64_t *interactions);
 /* direct sum: GE/src/Physics/physics.cpp:392-434 (epsilon NOT squared) */
 void geo_direct_accel(int32_t n, const float *x, const float *y, const float *m,
                       float G, float eps, float *ax, float *ay);
+void geo_direct_accel_dom(int32_t n, const float *x, const float *y, const float *m,
+                          float G, float eps, const geo_domain *dom, float *ax, float *ay);
+/* SPH::groundModeBoundary, GE/src/Physics/SPH.cpp:283-322 */
+void geo_sph_ground(geo_state *s, float radius, float w, float h, float vertical_gravity, float friction);
 /* FP64 direct sum with Plummer r^2+eps^2, j != i, for nt targets */
 void geo_direct_fp64(int32_t n, const float *x, const float *y, const float *m,
                      double G, double eps, int32_t nt, const uint32_t *targets,
                      double *ax, double *ay);
 /* integrator: GE/src/Physics/physics.cpp:1069-1125 */
 void geo_integrate_start(geo_state *s, float dt, int32_t sph, float sph_max_vel);
+/* with the periodic position wrap (physics.cpp:1104-1110) when dom->w > 0 */
+void geo_integrate_start_dom(geo_state *s, float dt, int32_t sph, float sph_max_vel, const geo_domain *dom);
 void geo_integrate_end(geo_state *s, float dt);
 /* SPH neighbour candidates (AVX2 grid semantics, neighborSearch.h:636-777):
  * all particles whose cell of (x,y) is within one cell (3x3) of cell(qx,qy).

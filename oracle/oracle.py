@@ -36,7 +36,9 @@ class _Params(C.Structure):
                 ("direct_below", C.c_int32), ("sph", C.c_int32), ("sph_h", C.c_float),
                 ("sph_cell", C.c_float), ("sph_unit_mass", C.c_float), ("viscosity", C.c_float),
                 ("cohesion_coef", C.c_float), ("delta", C.c_float), ("stiff_mult", C.c_float),
-                ("sph_max_vel", C.c_float), ("sph_search", C.c_int32), ("sph_search_radius", C.c_float)]
+                ("sph_max_vel", C.c_float), ("sph_search", C.c_int32), ("sph_search_radius", C.c_float),
+                ("periodic", C.c_int32), ("domain_w", C.c_float), ("domain_h", C.c_float),
+                ("sph_ground", C.c_int32), ("vertical_gravity", C.c_float), ("boundary_friction", C.c_float)]
 
 
 _lib = None
@@ -244,7 +246,9 @@ class State:
         p = _Params()
         d = dict(G=1.0, theta=0.8, softening=1.0, dt=0.01, naive=0, direct_below=5000, sph=0,
                  sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3, cohesion_coef=1.0,
-                 delta=19000.0, stiff_mult=1.0, sph_max_vel=250.0, sph_search=0, sph_search_radius=3.5)
+                 delta=19000.0, stiff_mult=1.0, sph_max_vel=250.0, sph_search=0, sph_search_radius=3.5,
+                 periodic=0, domain_w=3840.0, domain_h=2160.0, sph_ground=0, vertical_gravity=3.0,
+                 boundary_friction=0.0)
         d.update({k: v for k, v in prm.items() if k in d or k == "dt_start"})
         d.setdefault("dt_start", d["dt"])
         for k, v in d.items():
@@ -255,3 +259,37 @@ class State:
 
     def __getitem__(self, k):
         return self.a[k]
+
+
+class _Domain(C.Structure):
+    _fields_ = [("w", C.c_float), ("h", C.c_float)]
+
+
+def direct_accel_periodic(x, y, m, G, eps, w, h):
+    """naiveGravity with the periodic wrap of each separation (physics.cpp:417-420)."""
+    x, y, m = f32(x), f32(y), f32(m)
+    ax = np.zeros(len(x), np.float32)
+    ay = np.zeros(len(x), np.float32)
+    lib = load()
+    lib.geo_direct_accel_dom.argtypes = [C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                         C.c_float, C.c_float, C.POINTER(_Domain), C.POINTER(C.c_float),
+                                         C.POINTER(C.c_float)]
+    d = _Domain(w, h)
+    lib.geo_direct_accel_dom(len(x), _f(x), _f(y), _f(m), G, eps, C.byref(d), _f(ax), _f(ay))
+    return ax, ay
+
+
+def bh_accel_periodic(x, y, m, nodes, G, theta, eps, w, h):
+    x, y, m = f32(x), f32(y), f32(m)
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    ax = np.zeros(len(x), np.float32)
+    ay = np.zeros(len(x), np.float32)
+    cnt = C.c_uint64()
+    lib = load()
+    lib.geo_bh_accel_dom.argtypes = [C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                     C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_float, C.POINTER(_Domain),
+                                     C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]
+    d = _Domain(w, h)
+    lib.geo_bh_accel_dom(len(x), _f(x), _f(y), _f(m), C.c_void_p(nodes.ctypes.data), len(nodes), G, theta, eps,
+                         C.byref(d), _f(ax), _f(ay), C.byref(cnt))
+    return ax, ay, cnt.value

@@ -236,9 +236,35 @@ def test_unsupported_flags_rejected():
     with Context(100) as c:
         c.upload({"x": np.arange(100.0), "y": np.zeros(100), "mass": np.ones(100)})
         with pytest.raises(Exception):
-            c.step(_params(periodic=1), 1)
-        with pytest.raises(Exception):
             c.step(_params(temperature=1), 1)
+        with pytest.raises(Exception):  # a periodic domain needs a size
+            c.step(_params(periodic=1, domain_w=0.0), 1)
+
+
+@pytest.mark.parametrize("n,direct_below", [(20000, 5000), (3000, 5000)])
+def test_periodic_trajectory_matches_oracle(n, direct_below):
+    # a disk near the corner of a periodic domain: particles and forces wrap
+    # across both seams (tree walk for 20000, the direct sum below 5000)
+    W, H = 900.0, 700.0
+    st, prm = configs.config0(seed=33, n=n, scale=60.0)
+    st = dict(st)
+    st["x"] = np.mod(st["x"], W).astype(np.float32)
+    st["y"] = np.mod(st["y"], H).astype(np.float32)
+    p = _params(theta=prm["theta"], dt=prm["dt"], dt_start=prm["dt"], periodic=1, domain_w=W, domain_h=H,
+                direct_below=direct_below)
+    s = O.State(st)
+    s.step(dict(prm, dt_start=prm["dt"], periodic=1, domain_w=W, domain_h=H, direct_below=direct_below), nsteps=5)
+    ref = _by_id({k: s[k].copy() for k in ("x", "y", "vx", "vy", "id")})
+    with Context(n) as c:
+        c.upload(st)
+        c.step(p, 5)
+        c.sync()
+        got = _by_id(c.download(["x", "y", "vx", "vy", "id"]))
+    assert np.all((got["x"] >= 0) & (got["x"] < W) & (got["y"] >= 0) & (got["y"] < H))
+    dx = np.abs(got["x"] - ref["x"])
+    dx = np.minimum(dx, W - dx)  # a particle on the seam may sit on either side
+    assert dx.max() < 1e-3, dx.max()
+    assert np.abs(got["vx"] - ref["vx"]).max() < 1e-3 * float(np.abs(ref["vx"]).max())
 
 
 def _fp64_errors(d, xs, ys, ms, t):

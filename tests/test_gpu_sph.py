@@ -225,3 +225,26 @@ def test_neighbor_counts_match_oracle():
         d = c.download(["x", "y", "flags"])
     want = O.neighbor_amount(d["x"], d["y"], d["flags"], 3.0, 3.5)
     np.testing.assert_array_equal(got, want)
+
+
+def test_ground_mode_matches_oracle():
+    # SPH::groundModeBoundary: vertical gravity and damped walls at h from the
+    # domain edges, between PCISPH and integrateEnd
+    n = 3000
+    st = _sph_block(n, 11)
+    st["y"] = (st["y"] + np.float32(1.0)).astype(np.float32)   # the bottom rows start inside the floor band
+    st["vy"] = (st["vy"] - np.float32(5.0)).astype(np.float32)
+    W, H = 200.0, 300.0
+    p = _params(sph_ground=1, domain_w=W, domain_h=H, vertical_gravity=3.0, boundary_friction=0.2)
+    s = O.State(st, sph=True)
+    s.step(dict(_oracle_prm(p), sph_ground=1, domain_w=W, domain_h=H, vertical_gravity=3.0,
+                boundary_friction=0.2), nsteps=2)
+    ref = _by_id({k: s[k].copy() for k in ("x", "y", "vx", "vy", "ay", "id")})
+    with Context(n, sph=True) as c:
+        c.upload(st)
+        c.step(p, 2)
+        c.sync()
+        got = _by_id(c.download(["x", "y", "vx", "vy", "ay", "id"]))
+    assert got["y"].min() >= np.float32(3.0)  # floor at radiusMultiplier = h
+    for f in ("x", "y", "vx", "vy", "ay"):
+        _close(got[f], ref[f], f, rtol=1e-4)

@@ -417,3 +417,52 @@ def test_pcisph_search_modes_agree_when_both_cover_h():
     for k in res[0]:
         scale = np.abs(res[0][k]).max() or 1.0
         assert np.abs(res[0][k] - res[1][k]).max() <= 1e-5 * scale, k
+
+
+def test_periodic_wrap_two_body_closed_form():
+    # physics.cpp:117-120 / 417-420: d -= W * ((d > W/2) - (d < -W/2)); two
+    # bodies 20 apart across the x = 0 / x = W seam pull toward each other
+    W, H = 1000.0, 600.0
+    x = np.array([10.0, W - 10.0], np.float32)
+    y = np.array([300.0, 300.0], np.float32)
+    m = np.array([1.0, 3.0], np.float32)
+    ax, ay = O.direct_accel_periodic(x, y, m, 1.0, 1.0, W, H)   # softening unsquared (A-2)
+    want = 3.0 * 20.0 / (400.0 + 1.0) ** 1.5
+    assert ax[0] == pytest.approx(-want, rel=1e-6) and ay[0] == 0.0
+    assert ax[1] == pytest.approx(1.0 * 20.0 / 401.0 ** 1.5, rel=1e-6)
+    # the tree walk with theta -> 0 is the same direct sum (eps squared)
+    bb = O.bbox(x, y)
+    nodes = O.build_tree(x, y, m, bb)
+    bx, by, _ = O.bh_accel_periodic(x, y, m, nodes, 1.0, 0.01, 1.0, W, H)
+    assert bx[0] == pytest.approx(-3.0 * 20.0 / 401.0 ** 1.5, rel=1e-6)
+
+
+def test_periodic_positions_wrap_and_ground_walls():
+    W, H = 1000.0, 600.0
+    st = {"x": np.array([999.0, 1.0, 500.0], np.float32), "y": np.array([300.0, 300.0, 300.0], np.float32),
+          "vx": np.array([200.0, -200.0, 0.0], np.float32), "vy": np.zeros(3, np.float32),
+          "mass": np.ones(3, np.float32), "flags": np.zeros(3, np.uint32), "id": np.arange(3, dtype=np.uint32)}
+    s = O.State(st)
+    s.step(dict(G=0.0, dt=0.01, dt_start=0.01, periodic=1, domain_w=W, domain_h=H), nsteps=1)
+    o = np.argsort(s["id"])
+    x = s["x"][o]
+    assert x[0] == pytest.approx(1.0, abs=1e-3) and x[1] == pytest.approx(999.0, abs=1e-3)
+    # groundModeBoundary (SPH.cpp:283-322): a particle below the floor is put
+    # at y = h with vy damped by -0.1 and vx scaled by 1 - friction
+    n = 2
+    g = {"x": np.array([50.0, 100.0], np.float32), "y": np.array([1.0, 50.0], np.float32),
+         "vx": np.array([2.0, 0.0], np.float32), "vy": np.array([-5.0, 0.0], np.float32),
+         "mass": np.ones(n, np.float32), "flags": np.full(n, 2, np.uint32), "id": np.arange(n, dtype=np.uint32),
+         "sph_mass": np.ones(n, np.float32), "rest_dens": np.full(n, 0.008, np.float32),
+         "stiff": np.ones(n, np.float32), "visc": np.ones(n, np.float32), "cohesion": np.ones(n, np.float32),
+         "dens": np.zeros(n, np.float32)}
+    s = O.State(g, sph=True)
+    s.step(dict(G=0.0, dt=0.01, dt_start=0.01, sph=1, sph_h=3.0, sph_cell=3.0, sph_ground=1, domain_w=W,
+                domain_h=H, vertical_gravity=3.0, boundary_friction=0.25), nsteps=1)
+    o = np.argsort(s["id"])
+    gx, gy, gvx, gvy, gay = (s[k][o] for k in ("x", "y", "vx", "vy", "ay"))
+    # particle 0: drift to y = 0.95, wall: y = h, vy = -5 * -0.1, vx = 2 * 0.75; kick: vy += 3 * dt / 2
+    assert gy[0] == np.float32(3.0) and gvx[0] == np.float32(1.5)
+    assert gvy[0] == np.float32(np.float32(0.5) + np.float32(3.0) * np.float32(0.005))
+    assert gay[0] == np.float32(3.0) and gay[1] == np.float32(3.0)
+    assert gy[1] == np.float32(50.0) and gvy[1] == np.float32(np.float32(3.0) * np.float32(0.005))
