@@ -51,6 +51,7 @@ enum CounterSlot : int {
     kCntLevel0 = 8,         // kCntLevel0 + L: nodes allocated on level L
     kCntLevelBase0 = 8 + 64,  // first top index of level L
     kCntBlistAlloc = 8 + 128, // entries of the per-subtree internal-node lists
+    kCntBlistOverflow = 8 + 129, // a workgroup subtree's list did not fit
     kCntSlots = 8 + 136,
 };
 

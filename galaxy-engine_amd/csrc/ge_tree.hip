@@ -567,6 +567,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
             const bool fits = nl <= kBlockListMax && bbase + nl <= T.cap;
             T.blist_off[r] = fits ? bbase : 0u;
             T.blist_cnt[r] = fits ? nl : kNone;
+            if (!fits) counters[kCntBlistOverflow] = 1u;
         }
         __syncthreads();
         if (nl <= kBlockListMax && bbase + nl <= T.cap)
@@ -780,7 +781,10 @@ __global__ __launch_bounds__(kMassThreads) void k_tree_mass(TopNodes T, uint32_t
     __shared__ uint32_t hist[kMaxTopDepth + 1];
     __shared__ uint32_t cursor[kMaxTopDepth];
     __shared__ uint32_t maxd;
-    const uint32_t total = counters[kCntTopTotal];
+    // without an overflowed list, every node left lies above the workgroup
+    // subtrees, i.e. among the nodes the level phase allocated
+    const uint32_t total = counters[kCntBlistOverflow] ? counters[kCntTopTotal]
+                                                       : min(counters[kCntLevelBase0 + kTopLevels], counters[kCntTopTotal]);
     const uint32_t stamp = counters[kCntStamp];
     for (uint32_t d = threadIdx.x; d <= kMaxTopDepth; d += kMassThreads) hist[d] = 0;
     if (threadIdx.x == 0) maxd = 0;

@@ -338,3 +338,20 @@ def test_accuracy_mode_small_theta_is_direct():
     t = np.arange(len(xs), dtype=np.uint32)
     rms_rel, _ = _fp64_errors(d, xs, ys, ms, t)
     assert rms_rel < 2e-5
+
+
+def test_tree_bitexact_at_full_size_after_evolution():
+    # the 1M two-disk collision after 12 frames: every node field of the
+    # device tree equals the oracle's build on the device's particle order
+    st, _ = configs.config1(n=1_000_000)
+    p = _params(theta=0.7, dt=0.005, dt_start=0.005, direct_below=0)
+    with Context(1_000_000) as c:
+        c.upload(st)
+        c.step(p, 12)
+        c.sync()
+        d = c.download(["x", "y", "mass"])
+        got = c.nodes()
+    bb = O.bbox(d["x"], d["y"])
+    ref = O.build_tree(d["x"], d["y"], d["mass"], bb)
+    assert len(got) == len(ref)
+    _assert_nodes_equal(got, ref)
