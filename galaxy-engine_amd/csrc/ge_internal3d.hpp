@@ -1,0 +1,106 @@
+// ge_internal3d.hpp -- device-side layout of the 3D gravity context
+// (ge3_ctx) shared by ge_step3d.hip, ge_octree.hip and ge_abi3d.hip.  Not
+// part of the public ABI.
+#pragma once
+
+#include "ge_internal.hpp"
+#include "../../include/ge_physics3d.h"
+
+namespace ge {
+
+constexpr int kMortonBits3 = 63;  // 21 bits per axis (morton.cpp:67-92)
+
+// Periodic cube of the reference's 3D wrap, centred on the origin (w = 0: none).
+struct Domain3 {
+    float w = 0.0f, h = 0.0f, d = 0.0f;
+};
+
+struct State3 {
+    float *x = nullptr, *y = nullptr, *z = nullptr;
+    float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    float *ax = nullptr, *ay = nullptr, *az = nullptr;
+    float *pvx = nullptr, *pvy = nullptr, *pvz = nullptr;
+    float *m = nullptr;
+    uint32_t *flags = nullptr, *id = nullptr;
+};
+
+// Octree node record read by the walk: 32 bytes, two 16-byte scalar loads.
+struct alignas(16) NodeRec3 {
+    float comx, comy, comz, mass;
+    float size_sq;
+    uint32_t next, start, end;
+};
+
+// Top part of the octree, built level by level (structure of arrays).
+struct TopNodes3 {
+    float *px = nullptr, *py = nullptr, *pz = nullptr, *size = nullptr;
+    uint32_t *start = nullptr, *end = nullptr;
+    uint32_t *child = nullptr;     // 8 per node
+    uint32_t *kind = nullptr;      // the kinds of the quadtree build
+    uint32_t *pending2 = nullptr;  // build stamp of nodes whose mass the block pass computed
+    uint32_t *offset = nullptr;    // pre-order index in the final array
+    uint32_t *broot = nullptr;     // roots handed to the workgroup-subtree phase
+    uint32_t *blist = nullptr;     // internal nodes of each workgroup subtree, level by level
+    uint32_t *blist_off = nullptr;
+    uint32_t *blist_cnt = nullptr;
+    uint32_t *depth = nullptr;
+    uint32_t cap = 0;
+};
+
+struct Buffers3 {
+    State3 st[2];
+    int cur = 0;
+    float4 *pxyzm = nullptr;       // sorted (x, y, z, m) for the walk
+    uint64_t *keys = nullptr;      // 63-bit key per slot (pre-sort order)
+    uint64_t *sk_a = nullptr, *sk_b = nullptr;  // sort keys
+    uint32_t *sv_a = nullptr, *sv_b = nullptr;  // sort values (slots)
+    uint32_t *perm = nullptr;      // last permutation
+    uint32_t *hist = nullptr;
+    float *partials = nullptr;     // bbox partial min/max, 8 per block
+    float *bb = nullptr;           // device {x, y, z, size, minx, miny, minz, 0, maxx, maxy, maxz, 0}
+    uint32_t *counters = nullptr;  // CounterSlot of ge_internal.hpp
+    unsigned long long *inter = nullptr;  // interactions, accepted nodes
+    TopNodes3 top;
+    uint32_t *start_count = nullptr;
+    uint32_t *start_level = nullptr;
+    uint32_t *scan_part = nullptr;
+    NodeRec3 *nodes = nullptr;
+    float4 *node_geom = nullptr;   // (posx, posy, posz, size) per final node
+    uint32_t *node_mask = nullptr; // occupied octants per final node
+    uint32_t node_cap = 0;
+    double *diag = nullptr;        // FP64 direct-sum scratch (on first use)
+    size_t diag_cap = 0;
+};
+
+}  // namespace ge
+
+struct ge3_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cap = 0;
+    uint32_t n = 0;
+    uint32_t nblocks_bb = 1;       // workgroups of the bbox reduction
+    uint32_t leaf_cap = 0;
+    bool have_tree = false;
+    ge::Buffers3 b;
+    std::string err;
+};
+
+namespace ge {
+
+// ge_step3d.hip
+void launch3_integrate_bbox(ge3_ctx *c, float dt, int do_integrate, Domain3 dom);
+void launch3_morton_keys(ge3_ctx *c, const float *bb_device);
+void launch3_sort_gather(ge3_ctx *c);
+void launch3_zero_acc(ge3_ctx *c);
+void launch3_bh(ge3_ctx *c, const ge3_step_params *p, float end_kick_dt, bool count);
+void launch3_direct(ge3_ctx *c, const ge3_step_params *p, float end_kick_dt);
+void launch3_integrate_end(ge3_ctx *c, float dt);
+void launch3_direct_fp64(ge3_ctx *c, double G, double eps, uint32_t nt, const uint32_t *targets, double *out);
+void launch3_pack(ge3_ctx *c);
+// ge_octree.hip
+void launch_octree_build(ge3_ctx *c, const float *bb_device);
+
+constexpr uint32_t kBBoxChunk3 = 8192;  // particles per bbox workgroup
+
+}  // namespace ge

@@ -1,6 +1,7 @@
 // ge_sort.hip -- stable LSD radix sort of packed 64-bit sort words
 // (key << 28 | slot), used for the Morton order (morton.cpp:40-63) and the
-// SPH cell order (neighborSearch.h:636-720).  Three kernels per 8-bit digit:
+// SPH cell order (neighborSearch.h:636-720), and of (63-bit key, slot) pairs
+// for the 3D Morton order (morton.cpp:115-137).  Three kernels per 8-bit digit:
 // per-tile digit counts, one-workgroup scan of the (tile, digit) table, and a
 // stable scatter ranked with wave ballots.  Because the slot sits in the low
 // bits and LSD passes are stable, ties keep ascending slot order.
@@ -93,10 +94,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_table(uint32_t *__restric
     }
 }
 
+// KV: a 32-bit value travels with each key (the 3D sort, whose 63-bit keys
+// leave no room for the slot in the word).
+template <bool KV>
 __global__ __launch_bounds__(kSortThreads) void k_scatter(const uint64_t *__restrict__ in,
                                                           uint64_t *__restrict__ out, uint32_t n,
                                                           int shift, int bits,
-                                                          const uint32_t *__restrict__ offs) {
+                                                          const uint32_t *__restrict__ offs,
+                                                          const uint32_t *__restrict__ vin,
+                                                          uint32_t *__restrict__ vout) {
     __shared__ uint32_t wave_cnt[kSortWaves][256];
     __shared__ uint32_t blk_off[256];
     const uint32_t t = threadIdx.x;
@@ -113,11 +119,13 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const uint64_t *__rest
     uint64_t key[kItems];
     uint32_t rank[kItems];
     uint32_t dig[kItems];
+    uint32_t val[kItems];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         uint64_t i = wbase + (uint64_t)k * 64 + lane;
         bool valid = i < n;
         key[k] = valid ? in[i] : 0ull;
+        if (KV) val[k] = valid ? vin[i] : 0u;
         uint32_t d = (uint32_t)(key[k] >> shift) & dmask;
         dig[k] = d;
         uint64_t peers = __ballot(valid);
@@ -150,6 +158,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const uint64_t *__rest
         if (d != 0xFFFFFFFFu) {
             uint32_t pos = blk_off[d] + wave_cnt[w][d] + rank[k];
             out[pos] = key[k];
+            if (KV) vout[pos] = val[k];
         }
     }
 }
@@ -168,11 +177,34 @@ void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n, i
         hipLaunchKernelGGL(k_digit_count, dim3(nb), dim3(kSortThreads), 0, c->stream, data, n, shift,
                            dmask, c->b.hist);
         hipLaunchKernelGGL(k_scan_table, dim3(1), dim3(kScanThreads), 0, c->stream, c->b.hist, nb);
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kSortThreads), 0, c->stream, data, alt, n, shift,
-                           bits, c->b.hist);
+        hipLaunchKernelGGL(k_scatter<false>, dim3(nb), dim3(kSortThreads), 0, c->stream, data, alt, n, shift,
+                           bits, c->b.hist, nullptr, nullptr);
         uint64_t *t = data;
         data = alt;
         alt = t;
+    }
+}
+
+// Sorts n (key, value) pairs by key bits [lo_bit, hi_bit), stably; on return
+// `keys` / `vals` point to the sorted arrays.  `hist` holds
+// radix_hist_entries(n) entries.
+void radix_sort_kv(hipStream_t stream, uint32_t *hist, uint64_t *&keys, uint64_t *&keys_alt, uint32_t *&vals,
+                   uint32_t *&vals_alt, uint32_t n, int lo_bit, int hi_bit) {
+    if (n <= 1) return;
+    const uint32_t nb = blocks_for(n, kTile);
+    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+        const int bits = hi_bit - shift < 8 ? hi_bit - shift : 8;
+        const uint32_t dmask = (1u << bits) - 1u;
+        hipLaunchKernelGGL(k_digit_count, dim3(nb), dim3(kSortThreads), 0, stream, keys, n, shift, dmask, hist);
+        hipLaunchKernelGGL(k_scan_table, dim3(1), dim3(kScanThreads), 0, stream, hist, nb);
+        hipLaunchKernelGGL(k_scatter<true>, dim3(nb), dim3(kSortThreads), 0, stream, keys, keys_alt, n, shift, bits,
+                           hist, vals, vals_alt);
+        uint64_t *t = keys;
+        keys = keys_alt;
+        keys_alt = t;
+        uint32_t *tv = vals;
+        vals = vals_alt;
+        vals_alt = tv;
     }
 }
 

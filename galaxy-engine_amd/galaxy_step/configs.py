@@ -152,3 +152,65 @@ def config4_request(n, kind, seed):
         return st, prm
     st = _state(*exp_disk(_rng(seed), int(n) - 1, 100.0, 600.0, 5e4))
     return st, dict(G=1.0, theta=0.7, softening=1.0, dt=0.01)
+
+
+def galaxy3d(rng, n_disk, scale, clip, central_mass, thickness=5.0, bulge=20.0, bulge_size=None, G=1.0,
+             noise=0.02, center=(0.0, 0.0, 0.0), bulk_v=(0.0, 0.0, 0.0), tilt=0.0):
+    """One 3D disk galaxy plus a central body (last particle), following the
+    3D galaxy spawner (GE/src/Particles/particlesSpawning.cpp:642-704):
+    exponential radius truncated at clip, uniform angle, vertical offset
+    N(0, 1) * (thickness + bulge * exp(-r / bulge_size)), tangential speed in
+    the disk plane (circular from the enclosed mass, G = 1, as the 2D
+    workloads), then the disk rotated by `tilt` radians about the x axis."""
+    bulge_size = scale * 0.5 if bulge_size is None else bulge_size
+    u = rng.random(n_disk)
+    r = -scale * np.log1p(-u * (1.0 - math.exp(-clip / scale)))
+    ang = rng.random(n_disk) * (2.0 * math.pi)
+    zoff = rng.normal(0.0, 1.0, n_disk) * (thickness + bulge * np.exp(-r / bulge_size))
+    m = rng.uniform(0.5, 1.5, n_disk)
+    order = np.argsort(r, kind="stable")
+    enclosed = np.empty(n_disk)
+    enclosed[order] = central_mass + np.cumsum(m[order]) - m[order]
+    vc = np.sqrt(G * enclosed / np.maximum(r, 1e-6))
+    c, s = np.cos(ang), np.sin(ang)
+    x, y, z = r * c, r * s, zoff
+    vx = -vc * s + rng.normal(0.0, 1.0, n_disk) * noise * vc
+    vy = vc * c + rng.normal(0.0, 1.0, n_disk) * noise * vc
+    vz = rng.normal(0.0, 1.0, n_disk) * noise * vc
+    ct, st_ = math.cos(tilt), math.sin(tilt)
+    y, z = y * ct - z * st_, y * st_ + z * ct
+    vy, vz = vy * ct - vz * st_, vy * st_ + vz * ct
+    cols = []
+    for a, c0 in ((x, center[0]), (y, center[1]), (z, center[2])):
+        cols.append(np.append(a, 0.0) + c0)
+    vel = []
+    for a, b0 in ((vx, bulk_v[0]), (vy, bulk_v[1]), (vz, bulk_v[2])):
+        vel.append(np.append(a, 0.0) + b0)
+    return cols[0], cols[1], cols[2], vel[0], vel[1], vel[2], np.append(m, central_mass)
+
+
+def _state3(x, y, z, vx, vy, vz, m):
+    n = len(x)
+    f = lambda a: np.asarray(a, np.float32)  # noqa: E731
+    return {"x": f(x), "y": f(y), "z": f(z), "vx": f(vx), "vy": f(vy), "vz": f(vz), "mass": f(m),
+            "flags": np.zeros(n, np.uint32), "id": np.arange(n, dtype=np.uint32)}
+
+
+def config3d(seed=5, n=1_000_000, theta=0.7):
+    """3D two-galaxy collision: 2 x n/2 particles (scale 150, clip 600,
+    central body 5e4) at (-1200, -300, -100) / (+1200, +300, +100) moving
+    (+3, 0, 0) / (-3, 0, 0), the second disk tilted 30 degrees; G = 1,
+    softening 1 (added unsquared, as the reference's 3D step), dt = 0.005."""
+    rng = _rng(seed)
+    h = n // 2
+    a = galaxy3d(rng, h - 1, 150.0, 600.0, 5e4, center=(-1200.0, -300.0, -100.0), bulk_v=(3.0, 0.0, 0.0))
+    b = galaxy3d(rng, n - h - 1, 150.0, 600.0, 5e4, center=(1200.0, 300.0, 100.0), bulk_v=(-3.0, 0.0, 0.0),
+                 tilt=math.radians(30.0))
+    st = _state3(*[np.concatenate([p, q]) for p, q in zip(a, b)])
+    return st, dict(G=1.0, theta=theta, softening=1.0, dt=0.005, steps=200)
+
+
+def galaxy3d_small(seed=6, n=20_000, scale=100.0):
+    """One 3D galaxy of n particles (scale 100, clip 600, central 5e4)."""
+    st = _state3(*galaxy3d(_rng(seed), n - 1, scale, 600.0, 5e4))
+    return st, dict(G=1.0, theta=0.8, softening=1.0, dt=0.01, steps=100)

@@ -135,3 +135,58 @@ def test_cpp_facade_compiles_and_links(tmp_path):
     # device the program must fail cleanly with GE_E_NODEVICE, not crash
     exe = _build_facade_program(tmp_path)
     assert exe.exists()
+
+
+HEADER3 = os.path.join(ROOT, "include", "ge_physics3d.h")
+
+
+def test_library_exports_3d_header():
+    from galaxy_step import lib3d
+    lib = lib3d.load_library3()
+    text = open(HEADER3).read()
+    names = sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(ge3_[a-z_0-9]+)\s*\(", text, re.M)))
+    assert len(names) >= 20
+    assert set(names) == set(lib3d.EXPORTS3)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_3d_struct_layout_matches_header(tmp_path):
+    from galaxy_step import lib3d as L
+    src = tmp_path / "sz3.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ge_physics3d.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(ge3_config), '
+                   'sizeof(ge3_particles), sizeof(ge3_step_params), sizeof(ge3_node), '
+                   'offsetof(ge3_step_params, compensated), offsetof(ge3_node, child));return 0;}\n')
+    exe = tmp_path / "sz3"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    want = [C.sizeof(L.Ge3Config), C.sizeof(L.Ge3Particles), C.sizeof(L.Step3Params), C.sizeof(L.Ge3Node),
+            L.Step3Params.compensated.offset, L.Ge3Node.child.offset]
+    assert [int(v) for v in out] == want
+
+
+def test_3d_default_params_are_reference_defaults():
+    from galaxy_step import default_params3
+    p = default_params3()
+    # GE/include/parameters.h:99,114-117,133; globalLogic.cpp:1678
+    assert p.G == pytest.approx(6.674e-11) and p.theta == pytest.approx(0.8)
+    assert p.softening == pytest.approx(1.5) and p.dt == 1.0 and p.direct_below == 5000
+    assert (p.domain_w, p.domain_h, p.domain_d) == (1300.0, 1300.0, 1300.0)
+    assert p.periodic == 0 and p.sph == 0 and p.temperature == 0 and p.compensated == 1
+
+
+def test_3d_create_reports_status_without_crashing():
+    from galaxy_step import lib3d
+    lib = lib3d.load_library3()
+    cfg = lib3d.Ge3Config()
+    lib.ge3_default_config(C.byref(cfg))
+    h = C.c_void_p()
+    assert lib.ge3_create(C.byref(h), None) == -1
+    cfg.capacity = 0
+    assert lib.ge3_create(C.byref(h), C.byref(cfg)) == -1
+    cfg.capacity = 1000
+    rc = lib.ge3_create(C.byref(h), C.byref(cfg))
+    assert rc in (0, -7)
+    if rc == 0:
+        assert lib.ge3_destroy(h) == 0
