@@ -43,6 +43,7 @@ for _p in (os.path.join(ROOT, "galaxy-engine_amd"), os.path.join(ROOT, "oracle")
 # r^-3, G m r^-3 and two accumulates = 14; a quadrupole node 45 and a node to
 # the hexadecapole 160, counted from the expression the walk evaluates.
 FLOPS_PAIR = 14
+FLOPS_PAIR3 = 20  # the 3D interaction: dx, dy, dz, r^2 + eps, rsqrt, r^-3, G m r^-3, three accumulates
 FLOPS_NODE = {0: 14, 1: 45, 2: 160}
 PEAK_FP32_TFLOPS = 157.3            # MI355X vector FP32
 PEAK_HBM_GBS = 8000.0
@@ -62,6 +63,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-sph", action="store_true")
+    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--only-3d", action="store_true", help="run only the 3D leg (profiling) and print its JSON")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=120)
@@ -227,6 +230,69 @@ def sph_impact(a):
             "phase_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items() if v[1]}}
 
 
+def three_d(a, cpu_seconds):
+    """The 3D gravity frame (mode3D's gravity path): two 500k-particle 3D
+    galaxies, theta 0.7, the reference's monopole walk, octree and leaf rule
+    (softening 1 added unsquared), Kahan-compensated accumulation."""
+    import numpy as np
+    from galaxy_step import configs
+    from galaxy_step.lib3d import Context3D, default_params3
+    st, prm = configs.config3d(n=a.n, theta=a.theta)
+    n = len(st["x"])
+    p = default_params3(G=prm["G"], theta=a.theta, softening=prm["softening"], dt=prm["dt"], dt_start=prm["dt"],
+                        direct_below=0, compensated=1)
+    steps = max(a.steps, 5)
+    with Context3D(n) as c:
+        c.upload(st)
+        c.step(p, a.warmup)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(p, steps)
+        c.sync()
+        t1 = time.perf_counter()
+        # the walk alone (synchronous call around one launch, interactions counted)
+        walk = []
+        for _ in range(5):
+            w0 = time.perf_counter()
+            c.compute_gravity(p)
+            walk.append(time.perf_counter() - w0)
+        inter = c.interaction_counts()
+        d = c.download(["ax", "ay", "az"])
+        t = np.sort(np.random.default_rng(0).choice(n, 4096, replace=False)).astype(np.uint32)
+        rx, ry, rz = c.direct_fp64(t, p.G, p.softening)
+        nodes = c.node_count()
+    e2 = (d["ax"][t] - rx) ** 2 + (d["ay"][t] - ry) ** 2 + (d["az"][t] - rz) ** 2
+    a2 = rx ** 2 + ry ** 2 + rz ** 2
+    el = t1 - t0
+    walk_ms = min(walk) * 1e3
+    flops = FLOPS_PAIR3 * inter[0]
+    out = {"workload": f"3D Barnes-Hut frame, two-galaxy collision ({n} particles), theta {a.theta}, "
+                       "reference octree and walk",
+           "particle_updates_per_s": n * steps / el, "ms_per_step": el * 1e3 / steps, "steps": steps,
+           "walk_ms": walk_ms, "walk_tflops": flops / (walk_ms * 1e-3) / 1e12,
+           "interactions_per_particle": inter[0] / n, "accepted_nodes": inter[1], "node_count": nodes,
+           "force_error_vs_fp64": {"rms_per_particle": float(np.sqrt(np.mean(e2 / a2))),
+                                   "rms_force_normalised": float(np.sqrt(np.mean(e2) / np.mean(a2))),
+                                   "samples": int(len(t))}}
+    if cpu_seconds > 0:
+        import oracle3d as O3
+        s = O3.State3(st)
+        q = dict(prm, theta=a.theta, dt_start=prm["dt"], direct_below=0)
+        c0 = time.perf_counter()
+        k = 0
+        while True:
+            s.step(q, 1)
+            k += 1
+            if time.perf_counter() - c0 >= cpu_seconds or k >= 20:
+                break
+        cel = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": n * k / cel, "unit": "particle-updates/s", "kind": "port",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "sample": f"{k} full 3D frames of the same workload through the OpenMP C "
+                                         f"restatement, {cel:.1f} s"}
+    return out
+
+
 def batched(a, accuracy):
     """cfg3: the batched form against the same systems one by one."""
     import numpy as np
@@ -348,6 +414,9 @@ def main():
         sys.exit(2)
     from galaxy_step import configs
 
+    if a.only_3d:
+        print(json.dumps({"three_d": three_d(a, 0.0 if a.no_cpu else min(a.cpu_seconds, 10.0))}), flush=True)
+        return
     st, prm = configs.config1(n=a.n, theta=a.theta)
     n = len(st["x"])
     log(f"n={n} theta={a.theta} accuracy={a.accuracy} warmup={a.warmup} steps={a.steps}")
@@ -363,6 +432,10 @@ def main():
     sph = None if a.no_sph else sph_impact(a)
     if sph:
         log(f"sph impact: {sph['ms_per_step']:.3f} ms/step, phases {sph['phase_ms']}")
+    d3 = None if a.no_3d else three_d(a, 0.0 if a.no_cpu else min(a.cpu_seconds, 10.0))
+    if d3:
+        log(f"3d: {d3['ms_per_step']:.3f} ms/step, walk {d3['walk_ms']:.3f} ms ({d3['walk_tflops']:.1f} TFLOP/s), "
+            f"errors {d3['force_error_vs_fp64']}, cpu {d3.get('cpu_baseline', {}).get('value')}")
     bat = None if a.no_batch else batched(a, a.accuracy)
     if bat:
         log(f"batched: {bat['batched_solves_per_s']:.1f} solves/s vs plain {bat['plain_solves_per_s']:.1f} "
@@ -407,6 +480,7 @@ def main():
                        "note": "reference monopole walk, opening test and leaf rule (the parity path)"},
         "cpu_baseline": cpu,
         "sph_impact": sph,
+        "three_d": d3,
         "batched": bat,
         "served": srv,
     }

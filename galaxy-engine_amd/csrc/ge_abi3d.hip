@@ -445,7 +445,7 @@ int ge3_step(ge3_ctx *c, const ge3_step_params *p, int32_t frames) {
     if (frames < 0) return fail(c, GE_E_INVALID, "frames < 0");
     if (c->n == 0 || frames == 0) return GE_OK;
     (void)hipSetDevice(c->device);
-    for (int32_t k = 0; k < frames; ++k) frame(c, p, k == 0 ? p->dt_start : p->dt, k == frames - 1);
+    for (int32_t k = 0; k < frames; ++k) frame(c, p, k == 0 ? p->dt_start : p->dt, false);
     GE3_CHECK(c, hipGetLastError());
     return GE_OK;
 }
