@@ -79,13 +79,14 @@ template <bool EMIT>
 __device__ uint32_t wave_subtree3(const float *__restrict__ x, const float *__restrict__ y,
                                   const float *__restrict__ z, const float *__restrict__ m, float px0, float py0,
                                   float pz0, float size0, uint32_t s0, uint32_t cnt, uint32_t depth0,
-                                  const StartTable &st, const TreeOut3 &o, WLevel3 *lev, uint32_t *err) {
+                                  const StartTable &st, const TreeOut3 &o, WLevel3 *lev, uint32_t *err,
+                                  uint32_t off = 0) {
 #pragma clang fp contract(off)
     const uint32_t lane = lane_id();
     const bool have = lane < cnt;
-    const float lx = have ? x[s0 + lane] : 0.0f;
-    const float ly = have ? y[s0 + lane] : 0.0f;
-    const float lz = have ? z[s0 + lane] : 0.0f;
+    const float lx = have ? x[s0 + lane - off] : 0.0f;  // x[i - off] is particle i
+    const float ly = have ? y[s0 + lane - off] : 0.0f;
+    const float lz = have ? z[s0 + lane - off] : 0.0f;
 
     // ---- pass A: top-down, every node of a level at once
     bool act = have;
@@ -259,22 +260,33 @@ __device__ uint32_t wave_subtree3(const float *__restrict__ x, const float *__re
     return total;
 }
 
-// subGridMaker3D's seven searches (quadtree.cpp:146-199) by the whole wave
+// subGridMaker3D's seven searches (quadtree.cpp:146-199): z by the whole
+// wave, the two y searches by half-waves, the four x searches by
+// quarter-waves, each group giving the sequential search's result
 __device__ __forceinline__ void wave_split3(const float *__restrict__ x, const float *__restrict__ y,
                                             const float *__restrict__ z, float px, float py, float pz, float size,
-                                            uint32_t s, uint32_t e, uint32_t B[9]) {
+                                            uint32_t s, uint32_t e, uint32_t B[9], uint32_t off = 0) {
 #pragma clang fp contract(off)
     const float half = size * 0.5f;
     const float midx = px + half, midy = py + half, midz = pz + half;
     B[0] = s;
     B[8] = e;
-    B[4] = wave_part_point(z, s, e, midz);
-    B[2] = wave_part_point(y, s, B[4], midy);
-    B[6] = wave_part_point(y, B[4], e, midy);
-    B[1] = wave_part_point(x, s, B[2], midx);
-    B[3] = wave_part_point(x, B[2], B[4], midx);
-    B[5] = wave_part_point(x, B[4], B[6], midx);
-    B[7] = wave_part_point(x, B[6], e, midx);
+    B[4] = wave_part_point(z, s, e, midz, off);
+    // the y searches of the two z halves by the two half-waves together
+    const uint32_t lane = lane_id();
+    const bool h = lane >= 32;
+    const uint32_t ry = group_part_point<32>(y, h ? B[4] : s, h ? e : B[4], midy, off);
+    B[2] = readlane_u(ry, 0);
+    B[6] = readlane_u(ry, 32);
+    // the x searches of the four quarters by the four quarter-waves together
+    const uint32_t g = lane >> 4;
+    const uint32_t lo = g == 0 ? s : (g == 1 ? B[2] : (g == 2 ? B[4] : B[6]));
+    const uint32_t hi = g == 0 ? B[2] : (g == 1 ? B[4] : (g == 2 ? B[6] : e));
+    const uint32_t rx = group_part_point<16>(x, lo, hi, midx, off);
+    B[1] = readlane_u(rx, 0);
+    B[3] = readlane_u(rx, 16);
+    B[5] = readlane_u(rx, 32);
+    B[7] = readlane_u(rx, 48);
 }
 
 __device__ __forceinline__ uint32_t child_count8(const uint32_t B[9]) {
@@ -313,7 +325,7 @@ __device__ void write_children3(TopNodes3 &T, uint32_t node, float px, float py,
 __device__ bool finish_small3(TopNodes3 &T, const StartTable &st, const float *__restrict__ x,
                               const float *__restrict__ y, const float *__restrict__ z, uint32_t node, float px,
                               float py, float pz, float size, uint32_t s, uint32_t e, uint32_t depth,
-                              uint32_t *err) {
+                              uint32_t *err, uint32_t off = 0) {
     const uint32_t c = e - s;
     if (leaf_rule(c, size, st.leaf_cap)) {
         if (lane_id() == 0) {
@@ -324,7 +336,7 @@ __device__ bool finish_small3(TopNodes3 &T, const StartTable &st, const float *_
     }
     if (c <= kWaveSubMax) {
         const TreeOut3 none{nullptr, nullptr, nullptr, 0};
-        wave_subtree3<false>(x, y, z, nullptr, px, py, pz, size, s, c, depth, st, none, nullptr, err);
+        wave_subtree3<false>(x, y, z, nullptr, px, py, pz, size, s, c, depth, st, none, nullptr, err, off);
         if (lane_id() == 0) T.kind[node] = kKindWaveSub;
         return true;
     }
@@ -416,12 +428,29 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
     __shared__ uint32_t fcount[2];
     __shared__ uint32_t blist[kBlockListMax];
     __shared__ uint32_t bcount, bbase;
+    __shared__ float sx[kBlockSubMax], sy[kBlockSubMax], sz[kBlockSubMax];  // the subtree's coordinates
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t nroots = min(counters[kCntBlockRoots], T.cap);
     for (uint32_t r = blockIdx.x; r < nroots; r += gridDim.x) {
+        // a subtree of at most kBlockSubMax particles is staged in LDS (the
+        // probes of its splits read LDS); larger ones read HBM
+        const uint32_t root = T.broot[r];
+        const uint32_t rs = T.start[root], re = T.end[root];
+        const bool staged = re > rs && re - rs <= kBlockSubMax;
+        if (staged) {
+            for (uint32_t i = threadIdx.x; i < re - rs; i += kTreeBlock) {
+                sx[i] = x[rs + i];
+                sy[i] = y[rs + i];
+                sz[i] = z[rs + i];
+            }
+        }
+        const float *xs = staged ? sx : x;  // xs[i - off] is particle i
+        const float *ys = staged ? sy : y;
+        const float *zs = staged ? sz : z;
+        const uint32_t off = staged ? rs : 0u;
         if (threadIdx.x == 0) {
-            front[0][0] = T.broot[r];
+            front[0][0] = root;
             fcount[0] = 1;
             fcount[1] = 0;
             bcount = 0;
@@ -436,10 +465,11 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
                 const uint32_t node = front[cur][i];
                 const float px = T.px[node], py = T.py[node], pz = T.pz[node], size = T.size[node];
                 const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
-                if (finish_small3(T, st, x, y, z, node, px, py, pz, size, s, e, depth, counters + kCntError))
+                if (finish_small3(T, st, xs, ys, zs, node, px, py, pz, size, s, e, depth, counters + kCntError,
+                                  off))
                     continue;
                 uint32_t B[9];
-                wave_split3(x, y, z, px, py, pz, size, s, e, B);
+                wave_split3(xs, ys, zs, px, py, pz, size, s, e, B, off);
                 if (lane == 0) {
                     const uint32_t k = child_count8(B);
                     const uint32_t first = atomicAdd(&counters[kCntTopAlloc], k);

@@ -100,12 +100,13 @@ template <bool EMIT>
 __device__ uint32_t wave_subtree(const float *__restrict__ x, const float *__restrict__ y,
                                  const float *__restrict__ m, float px0, float py0, float size0, uint32_t s0,
                                  uint32_t cnt, uint32_t depth0, const StartTable &st, const TreeOut &o,
-                                 WLevel *lev, uint32_t *err, float &rgm, float &rcx, float &rcy) {
+                                 WLevel *lev, uint32_t *err, float &rgm, float &rcx, float &rcy,
+                                 uint32_t off = 0) {
 #pragma clang fp contract(off)
     const uint32_t lane = lane_id();
     const bool have = lane < cnt;
-    const float lx = have ? x[s0 + lane] : 0.0f;
-    const float ly = have ? y[s0 + lane] : 0.0f;
+    const float lx = have ? x[s0 + lane - off] : 0.0f;  // x[i - off] is particle i
+    const float ly = have ? y[s0 + lane - off] : 0.0f;
 
     // ---- pass A: top-down, every node of a level at once
     bool act = have;
@@ -260,17 +261,22 @@ __device__ uint32_t wave_subtree(const float *__restrict__ x, const float *__res
 }
 
 // Split a node with the whole wave: subGridMaker's three searches
-// (quadtree.cpp:30-65).
+// (quadtree.cpp:30-65), the y search by the wave, then the two x searches
+// by the two half-waves together.
 __device__ __forceinline__ void wave_split(const float *__restrict__ x, const float *__restrict__ y, float px,
-                                           float py, float size, uint32_t s, uint32_t e, uint32_t b[5]) {
+                                           float py, float size, uint32_t s, uint32_t e, uint32_t b[5],
+                                           uint32_t off = 0) {
 #pragma clang fp contract(off)
     const float half = size * 0.5f;
     const float midx = px + half, midy = py + half;
     b[0] = s;
     b[4] = e;
-    b[2] = wave_part_point(y, s, e, midy);
-    b[1] = wave_part_point(x, s, b[2], midx);
-    b[3] = wave_part_point(x, b[2], e, midx);
+    b[2] = wave_part_point(y, s, e, midy, off);
+    // the two x searches at once, one per half-wave
+    const bool hi = lane_id() >= 32;
+    const uint32_t r = group_part_point<32>(x, hi ? b[2] : s, hi ? e : b[2], midx, off);
+    b[1] = readlane_u(r, 0);
+    b[3] = readlane_u(r, 32);
 }
 
 __device__ __forceinline__ uint32_t child_count(const uint32_t b[5]) {
@@ -308,7 +314,7 @@ __device__ void write_children(TopNodes &T, uint32_t node, float px, float py, f
 // Returns false if the node needs splitting.  Uniform across the wave.
 __device__ bool finish_small(TopNodes &T, const StartTable &st, const float *__restrict__ x,
                              const float *__restrict__ y, uint32_t node, float px, float py, float size,
-                             uint32_t s, uint32_t e, uint32_t depth, uint32_t *err) {
+                             uint32_t s, uint32_t e, uint32_t depth, uint32_t *err, uint32_t off = 0) {
     const uint32_t c = e - s;
     if (leaf_rule(c, size, st.leaf_cap)) {
         if (lane_id() == 0) {
@@ -320,7 +326,7 @@ __device__ bool finish_small(TopNodes &T, const StartTable &st, const float *__r
     if (c <= kWaveSubMax) {
         const TreeOut none{nullptr, nullptr, 0, 0};
         float d0, d1, d2;
-        wave_subtree<false>(x, y, nullptr, px, py, size, s, c, depth, st, none, nullptr, err, d0, d1, d2);
+        wave_subtree<false>(x, y, nullptr, px, py, size, s, c, depth, st, none, nullptr, err, d0, d1, d2, off);
         if (lane_id() == 0) T.kind[node] = kKindWaveSub;
         return true;
     }
@@ -418,12 +424,28 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
     __shared__ uint32_t fcount[2];
     __shared__ uint32_t blist[kBlockListMax];  // internal nodes, level by level
     __shared__ uint32_t bcount, bbase;
+    __shared__ float sx[kBlockSubMax], sy[kBlockSubMax];  // the subtree's coordinates
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t nroots = min(counters[kCntBlockRoots], T.cap);
     for (uint32_t r = blockIdx.x; r < nroots; r += gridDim.x) {
+        // a subtree of at most kBlockSubMax particles is staged in LDS, and its
+        // partition-point probes read LDS; larger ones (handed over at the
+        // last level) read HBM.
+        const uint32_t root = T.broot[r];
+        const uint32_t rs = T.start[root], re = T.end[root];
+        const bool staged = re > rs && re - rs <= kBlockSubMax;
+        if (staged) {
+            for (uint32_t i = threadIdx.x; i < re - rs; i += kTreeBlock) {
+                sx[i] = x[rs + i];
+                sy[i] = y[rs + i];
+            }
+        }
+        const float *xs = staged ? sx : x;  // xs[i - off] is particle i
+        const float *ys = staged ? sy : y;
+        const uint32_t off = staged ? rs : 0u;
         if (threadIdx.x == 0) {
-            front[0][0] = T.broot[r];
+            front[0][0] = root;
             fcount[0] = 1;
             fcount[1] = 0;
             bcount = 0;
@@ -438,9 +460,9 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
                 const uint32_t node = front[cur][i];
                 const float px = T.px[node], py = T.py[node], size = T.size[node];
                 const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
-                if (finish_small(T, st, x, y, node, px, py, size, s, e, depth, counters + kCntError)) continue;
+                if (finish_small(T, st, xs, ys, node, px, py, size, s, e, depth, counters + kCntError, off)) continue;
                 uint32_t b[5];
-                wave_split(x, y, px, py, size, s, e, b);
+                wave_split(xs, ys, px, py, size, s, e, b, off);
                 if (lane == 0) {
                     const uint32_t k = child_count(b);
                     const uint32_t first = atomicAdd(&counters[kCntTopAlloc], k);

@@ -39,6 +39,11 @@ __device__ __forceinline__ float readlane_f(float v, uint32_t lane) {
 // (bit k = predicate of particle k of a wave subtree).
 __device__ __forceinline__ uint32_t mask_part_point(uint64_t m, uint32_t first, uint32_t last) {
     uint32_t len = last - first;
+    if (len == 0) return first;
+    // bits [first, last) already partitioned (ones, then zeros): the halving
+    // search returns the first zero, i.e. first + the number of ones
+    const uint64_t r = (m >> first) & (len >= 64 ? ~0ull : ((1ull << len) - 1ull));
+    if ((r & (r + 1ull)) == 0ull) return first + (uint32_t)__popcll(r);
     while (len > 0) {
         uint32_t half = len >> 1;
         uint32_t middle = first + half;
@@ -56,7 +61,10 @@ __device__ __forceinline__ uint32_t mask_part_point(uint64_t m, uint32_t first, 
 // node j of the next six levels of the halving search's decision tree (heap
 // order), loads its probe, and the ballot lets every lane follow the path the
 // sequential search would take.  Uniform result, identical to part_point.
-__device__ uint32_t wave_part_point(const float *__restrict__ v, uint32_t first, uint32_t last, float mid) {
+// `off`: v holds the elements from index off on (v[i - off] is element i), so
+// a range staged in LDS is read without rebasing the pointer.
+__device__ uint32_t wave_part_point(const float *__restrict__ v, uint32_t first, uint32_t last, float mid,
+                                    uint32_t off = 0) {
     const uint32_t j = lane_id();
     uint32_t f = first, len = last - first;
     while (len > 0) {
@@ -71,7 +79,7 @@ __device__ uint32_t wave_part_point(const float *__restrict__ v, uint32_t first,
                 if ((j >> k) & 1u) { fj = fj + half + 1; lj = lj - half - 1; }
                 else { lj = half; }
             }
-            if (ok && lj > 0) pred = v[fj + (lj >> 1)] < mid;
+            if (ok && lj > 0) pred = v[fj + (lj >> 1) - off] < mid;
         }
         const uint64_t bal = __ballot(pred);
         uint32_t node = 1;
@@ -84,6 +92,49 @@ __device__ uint32_t wave_part_point(const float *__restrict__ v, uint32_t first,
         }
     }
     return f;
+}
+
+// The same search by groups of G lanes (G = 32 or 16), each group over its
+// own range [first, last) (uniform within the group): log2(G) halving levels
+// per round.  Per group, identical to part_point.  Called with all lanes.
+template <int G>
+__device__ uint32_t group_part_point(const float *__restrict__ v, uint32_t first, uint32_t last, float mid,
+                                     uint32_t off = 0) {
+    constexpr int D = G == 32 ? 5 : 4;
+    static_assert(G == 32 || G == 16, "group of 32 or 16 lanes");
+    const uint32_t lane = lane_id();
+    const uint32_t j = lane & (uint32_t)(G - 1);
+    const uint32_t shift = lane & ~(uint32_t)(G - 1);
+    uint32_t f = first, len = last - first;
+    while (__ballot(len > 0)) {
+        bool pred = false;
+        if (j >= 1 && len > 0) {
+            const int d = 31 - __clz(j);
+            uint32_t fj = f, lj = len;
+            bool ok = true;
+            for (int k = d - 1; k >= 0; --k) {
+                if (lj == 0) { ok = false; break; }
+                const uint32_t half = lj >> 1;
+                if ((j >> k) & 1u) { fj = fj + half + 1; lj = lj - half - 1; }
+                else { lj = half; }
+            }
+            if (ok && lj > 0) pred = v[fj + (lj >> 1) - off] < mid;
+        }
+        const uint64_t gb = __ballot(pred) >> shift;
+        uint32_t node = 1;
+        for (int k = 0; k < D && len > 0; ++k) {
+            const uint32_t half = len >> 1;
+            const uint32_t p = (uint32_t)((gb >> node) & 1ull);
+            if (p) { f = f + half + 1; len = len - half - 1; }
+            else { len = half; }
+            node = 2 * node + p;
+        }
+    }
+    return f;
+}
+
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
 
 __device__ __forceinline__ uint64_t rfl64(uint64_t v) {
