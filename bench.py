@@ -39,12 +39,13 @@ for _p in (os.path.join(ROOT, "galaxy-engine_amd"), os.path.join(ROOT, "oracle")
         sys.path.insert(0, _p)
 
 # Algorithmic FP32 flops per interaction of the walk (Kahan and opening tests
-# excluded): a leaf partner or a monopole node is dx, dy, r^2 + eps^2, rsqrt,
-# r^-3, G m r^-3 and two accumulates = 14; a quadrupole node 45 and a node to
-# the hexadecapole 160, counted from the expression the walk evaluates.
+# excluded; a fused multiply-add counts 2): a leaf partner or a monopole node
+# is dx, dy, r^2 + eps^2, rsqrt, r^-3, G m r^-3 and two accumulates = 14; a
+# quadrupole node 32 and a node to the hexadecapole 107, counted from the
+# expression the walk evaluates (moments pre-scaled, Horner form in r^2).
 FLOPS_PAIR = 14
 FLOPS_PAIR3 = 20  # the 3D interaction: dx, dy, dz, r^2 + eps, rsqrt, r^-3, G m r^-3, three accumulates
-FLOPS_NODE = {0: 14, 1: 45, 2: 160}
+FLOPS_NODE = {0: 14, 1: 32, 2: 107}
 PEAK_FP32_TFLOPS = 157.3            # MI355X vector FP32
 PEAK_HBM_GBS = 8000.0
 

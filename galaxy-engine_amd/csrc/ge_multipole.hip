@@ -24,15 +24,18 @@
 // reference's float partition points can leave a particle up to half a node
 // outside its node's square (when its Morton key and its float coordinate
 // disagree about a midline); a truncated series about such a node diverges
-// for targets closer than that particle.  Record per node, 6 float4:
+// for targets closer than that particle.  With A_n = C_n / n! (C_n the
+// central moments), T_l = A3_iil, U_lm = A4_iilm and W = A4_iijj, the record
+// per node is 6 float4, each moment scaled by the constant of the derivative
+// term it feeds:
 //   0: (comx, comy, opening radius, mass)
-//   1: (C2xx/2, C2xy/2, C2yy/2, tr C2/2)
-//   2: C3 / 6: (xxx, xxy, xyy, yyy)
-//   3: (T_x / 6, T_y / 6, W / 24, 0)        T_l = C3_iil,  W = C4_iijj
-//   4: C4 / 24: (xxxx, xxxy, xxyy, xyyy)
-//   5: (C4yyyy / 24, U_xx / 24, U_xy / 24, U_yy / 24)   U_lm = C4_iilm
-// with C_n the central moments.  The walk evaluates the Taylor expansion of
-// the softened kernel (|d|^2 + eps^2)^-1/2 to the order asked for.
+//   1: (-15 A2xx, -15 A2xy, -15 A2yy, 3 tr A2)
+//   2: -105 A3: (xxx, xxy, xyy, yyy)
+//   3: (45 T_x, 45 T_y, -45 W, 0)
+//   4: -945 A4: (xxxx, xxxy, xxyy, xyyy)
+//   5: (-945 A4yyyy, 630 U_xx, 630 U_xy, 630 U_yy)
+// The walk evaluates the Taylor expansion of the softened kernel
+// (|d|^2 + eps^2)^-1/2 to the order asked for from these.
 #include "ge_internal.hpp"
 
 namespace ge {
@@ -127,14 +130,18 @@ __device__ void finish_record(uint32_t k, const NodeRec &r, const float4 g, cons
     const double c4xxxx = C[mono(4, 0)], c4xxxy = C[mono(3, 1)], c4xxyy = C[mono(2, 2)];
     const double c4xyyy = C[mono(1, 3)], c4yyyy = C[mono(0, 4)];
     const double uxx = c4xxxx + c4xxyy, uxy = c4xxxy + c4xyyy, uyy = c4xxyy + c4yyyy;
+    // each moment is stored pre-multiplied by the constant of its leading
+    // derivative term, so the walk evaluates the expansion as two short
+    // polynomials in r^2 = 1 / (|d|^2 + eps^2) with no per-node scaling
     float4 *o = mp + 6ull * k;
     o[0] = make_float4(comx, comy, lim, (float)M);
-    o[1] = make_float4((float)(0.5 * c2xx), (float)(0.5 * c2xy), (float)(0.5 * c2yy), (float)(0.5 * (c2xx + c2yy)));
-    o[2] = make_float4((float)(c3xxx / 6.0), (float)(c3xxy / 6.0), (float)(c3xyy / 6.0), (float)(c3yyy / 6.0));
-    o[3] = make_float4((float)((c3xxx + c3xyy) / 6.0), (float)((c3xxy + c3yyy) / 6.0), (float)((uxx + uyy) / 24.0),
+    o[1] = make_float4((float)(-7.5 * c2xx), (float)(-7.5 * c2xy), (float)(-7.5 * c2yy), (float)(1.5 * (c2xx + c2yy)));
+    o[2] = make_float4((float)(-17.5 * c3xxx), (float)(-17.5 * c3xxy), (float)(-17.5 * c3xyy), (float)(-17.5 * c3yyy));
+    o[3] = make_float4((float)(7.5 * (c3xxx + c3xyy)), (float)(7.5 * (c3xxy + c3yyy)), (float)(-1.875 * (uxx + uyy)),
                        0.0f);
-    o[4] = make_float4((float)(c4xxxx / 24.0), (float)(c4xxxy / 24.0), (float)(c4xxyy / 24.0), (float)(c4xyyy / 24.0));
-    o[5] = make_float4((float)(c4yyyy / 24.0), (float)(uxx / 24.0), (float)(uxy / 24.0), (float)(uyy / 24.0));
+    o[4] = make_float4((float)(-39.375 * c4xxxx), (float)(-39.375 * c4xxxy), (float)(-39.375 * c4xxyy),
+                       (float)(-39.375 * c4xyyy));
+    o[5] = make_float4((float)(-39.375 * c4yyyy), (float)(26.25 * uxx), (float)(26.25 * uxy), (float)(26.25 * uyy));
 }
 
 __global__ void k_mp_reset(uint32_t *__restrict__ cnt) {

@@ -217,12 +217,17 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // ORDER 2 / 4 (accuracy modes, no counterpart in the reference): the node is
 // accepted when |d| exceeds its opening radius (size/theta + COM offset) and
 // contributes the Taylor expansion of the softened kernel about its FP64
-// centre of mass up to the quadrupole (2) or hexadecapole (4) term, with
-// R^2 = |d|^2 + eps^2 and D_k = (-1)^k (2k-1)!! R^-(2k+1):
-//   a = G [M D1 d + D3 d (d.A2.d) + D2 (tr A2 d + 2 A2 d)
-//          - D4 d (A3:ddd) - 3 D3 (A3:dd + (T.d) d) - 3 D2 T
-//          + D5 d (A4:dddd) + D4 (4 A4:ddd + 6 (U:dd) d) + D3 (3 W d + 12 U d)]
-// (A_n = C_n / n!, T, U, W traces of A3, A4; d = target - COM).
+// centre of mass up to the quadrupole (2) or hexadecapole (4) term.  With
+// d = target - COM, R^2 = |d|^2 + eps^2, r = 1/R, A_n = C_n / n! and the
+// traces T, U, W of A3 and A4:
+//   a = G (S d + V)
+//   S = -M r^3 + 3 tr A2 r^5 + (-15 d.A2.d + 45 T.d - 45 W) r^7
+//       + (-105 A3:ddd + 630 U:dd) r^9 - 945 A4:dddd r^11
+//   V = (6 A2 d - 9 T) r^5 + (45 A3:dd - 180 U d) r^7 + 420 A4:ddd r^9
+// (the Cartesian Taylor series, D_k = (-1)^k (2k-1)!! r^(2k+1)).  The node
+// record holds each moment times the constant it meets first in S, so S is
+// a Horner polynomial in r^2 and V is one more, with the ratios of the
+// remaining constants folded into three multipliers.
 //
 // Several systems: system s owns the node range [P[off_s], P[off_s+1]) (P is
 // the node-start prefix of the tree build); each lane is active only inside
@@ -269,6 +274,19 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
     uint32_t idx = __builtin_amdgcn_readfirstlane(wlo);
     while (idx < walk_end) {
         const NodeRec nd = nodes[idx];
+        // ORDER > 0: the whole multipole record is fetched with the node, so
+        // an accepting wave does not wait for a second load
+        float4 h0, h1, h2, h3, h4, h5;
+        if (ORDER > 0) {
+            h0 = mp[6ull * idx];
+            h1 = mp[6ull * idx + 1];
+        }
+        if (ORDER >= 4) {
+            h2 = mp[6ull * idx + 2];
+            h3 = mp[6ull * idx + 3];
+            h4 = mp[6ull * idx + 4];
+            h5 = mp[6ull * idx + 5];
+        }
         const uint32_t next = nd.next;
         if (COUNT) ++visits;
         // index after this subtree, saturated so the walk always moves forward
@@ -280,9 +298,7 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
         const bool active = idx >= skip_until && idx < end;
         float dx, dy, dist_sq;
         bool accept;
-        float4 h0;
         if (ORDER > 0) {
-            h0 = mp[6ull * idx];
             dx = h0.x - p.x;
             dy = h0.y - p.y;
             if (per) {
@@ -305,42 +321,39 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
         }
         if (active && accept) {
             const float inv = __builtin_amdgcn_rsqf(dist_sq);
-            const float inv2 = inv * inv;
-            const float inv3 = inv2 * inv;
             float tx, ty;
             if (ORDER > 0) {
-                // d = target - COM
-                const float qx = -dx, qy = -dy;
-                const float inv5 = inv3 * inv2, inv7 = inv5 * inv2;
-                const float D1 = -inv3, D2 = 3.0f * inv5, D3 = -15.0f * inv7;
-                const float4 h1 = mp[6ull * idx + 1];
-                const float aqx = h1.x * qx + h1.y * qy, aqy = h1.y * qx + h1.z * qy;
-                const float qaq = qx * aqx + qy * aqy;
-                float sx = h0.w * D1 * qx + D3 * qx * qaq + D2 * (h1.w * qx + 2.0f * aqx);
-                float sy = h0.w * D1 * qy + D3 * qy * qaq + D2 * (h1.w * qy + 2.0f * aqy);
+                const float qx = -dx, qy = -dy;  // d = target - COM
+                const float r2 = inv * inv;
+                const float gr3 = G * (inv * r2);
+                const float p2x = h1.x * qx + h1.y * qy, p2y = h1.y * qx + h1.z * qy;
+                const float s2 = qx * p2x + qy * p2y;
                 if (ORDER >= 4) {
-                    const float inv9 = inv7 * inv2, inv11 = inv9 * inv2;
-                    const float D4 = 105.0f * inv9, D5 = -945.0f * inv11;
-                    const float4 h2 = mp[6ull * idx + 2], h3 = mp[6ull * idx + 3];
-                    const float4 h4 = mp[6ull * idx + 4], h5 = mp[6ull * idx + 5];
-                    const float qxx = qx * qx, qxy = qx * qy, qyy = qy * qy;
-                    const float bqx = h2.x * qxx + 2.0f * h2.y * qxy + h2.z * qyy;
-                    const float bqy = h2.y * qxx + 2.0f * h2.z * qxy + h2.w * qyy;
-                    const float bqqq = qx * bqx + qy * bqy;
-                    const float tq = h3.x * qx + h3.y * qy;
-                    sx -= D4 * qx * bqqq + 3.0f * D3 * (bqx + qx * tq) + 3.0f * D2 * h3.x;
-                    sy -= D4 * qy * bqqq + 3.0f * D3 * (bqy + qy * tq) + 3.0f * D2 * h3.y;
-                    const float eqx = h4.x * qxx * qx + 3.0f * h4.y * qxx * qy + 3.0f * h4.z * qx * qyy + h4.w * qyy * qy;
-                    const float eqy = h4.y * qxx * qx + 3.0f * h4.z * qxx * qy + 3.0f * h4.w * qx * qyy + h5.x * qyy * qy;
-                    const float eqqqq = qx * eqx + qy * eqy;
-                    const float uqx = h5.y * qx + h5.z * qy, uqy = h5.z * qx + h5.w * qy;
-                    const float uqq = qx * uqx + qy * uqy;
-                    sx += D5 * qx * eqqqq + D4 * (4.0f * eqx + 6.0f * qx * uqq) + D3 * (3.0f * h3.z * qx + 12.0f * uqx);
-                    sy += D5 * qy * eqqqq + D4 * (4.0f * eqy + 6.0f * qy * uqq) + D3 * (3.0f * h3.z * qy + 12.0f * uqy);
+                    const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
+                    const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
+                    const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
+                    const float pux = h5.y * qx + h5.z * qy, puy = h5.z * qx + h5.w * qy;
+                    const float s3u = qx * (p3x + pux) + qy * (p3y + puy);
+                    const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y) + h3.z;
+                    const float c0 = qxx * qx, c3 = qyy * qy, c1 = (3.0f * qxx) * qy, c2 = (3.0f * qyy) * qx;
+                    const float p4x = h4.x * c0 + h4.y * c1 + h4.z * c2 + h4.w * c3;
+                    const float p4y = h4.y * c0 + h4.z * c1 + h4.w * c2 + h5.x * c3;
+                    const float s4 = qx * p4x + qy * p4y;
+                    const float sc = gr3 * (r2 * (r2 * (r2 * (r2 * s4 + s3u) + c7) + h1.w) - h0.w);
+                    const float e = (28.0f / 9.0f) * r2, f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
+                    const float ox = (2.0f * p2x + h3.x) + f * ((3.0f * p3x + 2.0f * pux) + e * p4x);
+                    const float oy = (2.0f * p2y + h3.y) + f * ((3.0f * p3y + 2.0f * puy) + e * p4y);
+                    tx = qx * sc + g5 * ox;
+                    ty = qy * sc + g5 * oy;
+                } else {
+                    const float sc = gr3 * (r2 * (r2 * s2 + h1.w) - h0.w);
+                    const float g5 = gr3 * (-0.4f * r2);
+                    tx = qx * sc + g5 * p2x;
+                    ty = qy * sc + g5 * p2y;
                 }
-                tx = G * sx;
-                ty = G * sy;
             } else {
+                const float inv2 = inv * inv;
+                const float inv3 = inv2 * inv;
                 const float f = G * nd.mass * inv3;
                 tx = dx * f;
                 ty = dy * f;
