@@ -1,7 +1,7 @@
 // ge_abi3d.hip -- the extern "C" entry points of include/ge_physics3d.h:
 // context lifecycle, device buffers, state transfer, the stage calls and the
 // frame driver that orders the 3D stage kernels as mode3D() does
-// (GE/src/globalLogic.cpp:1580-1719, gravity path).
+// (GE/src/globalLogic.cpp:1580-1719: gravity, then SPH3D).
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -56,6 +56,44 @@ void free_all(ge3_ctx *c) {
     dfree(T.blist_cnt); dfree(T.depth);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
     dfree(b.nodes); dfree(b.node_geom); dfree(b.node_mask); dfree(b.diag);
+    for (State3 &s : b.st) {
+        float **fs[] = {&s.sph_mass, &s.rest_dens, &s.stiff, &s.visc, &s.cohesion, &s.dens, &s.press, &s.press_tmp,
+                        &s.pred_dens, &s.ppx, &s.ppy, &s.ppz, &s.pfx, &s.pfy, &s.pfz};
+        for (float **f : fs) dfree(*f);
+    }
+    for (Grid3 *g : {&b.grid_pos, &b.grid_pred}) {
+        dfree(g->keys); dfree(g->keys_alt); dfree(g->first); dfree(g->ord); dfree(g->cell);
+        dfree(g->pk0); dfree(g->pk1); dfree(g->pk2);
+    }
+    dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
+}
+
+// SPH3D state: 15 fields per state buffer, two slot grids, pair-force sums
+int alloc_sph(ge3_ctx *c) {
+    Buffers3 &b = c->b;
+    const uint32_t cap = c->cap;
+    uint32_t bits = 10;
+    while (bits < 24 && (1ull << bits) < 2ull * cap) ++bits;  // about two slots per particle
+    b.slot_bits = bits;
+    for (State3 &s : b.st) {
+        float **fs[] = {&s.sph_mass, &s.rest_dens, &s.stiff, &s.visc, &s.cohesion, &s.dens, &s.press, &s.press_tmp,
+                        &s.pred_dens, &s.ppx, &s.ppy, &s.ppz, &s.pfx, &s.pfy, &s.pfz};
+        for (float **f : fs) GE3_CHECK(c, dalloc(*f, cap));
+    }
+    for (Grid3 *g : {&b.grid_pos, &b.grid_pred}) {
+        GE3_CHECK(c, dalloc(g->keys, cap));
+        GE3_CHECK(c, dalloc(g->keys_alt, cap));
+        GE3_CHECK(c, dalloc(g->first, (1ull << bits) + 1));
+        GE3_CHECK(c, dalloc(g->ord, cap));
+        GE3_CHECK(c, dalloc(g->cell, cap));
+        GE3_CHECK(c, dalloc(g->pk0, cap));
+        GE3_CHECK(c, dalloc(g->pk1, cap));
+    }
+    GE3_CHECK(c, dalloc(b.grid_pos.pk2, cap));
+    GE3_CHECK(c, dalloc(b.sphf, cap));
+    GE3_CHECK(c, dalloc(b.sph_pp, cap));
+    GE3_CHECK(c, dalloc(b.sph_pd, cap));
+    return GE_OK;
 }
 
 int alloc_all(ge3_ctx *c) {
@@ -106,7 +144,10 @@ int alloc_all(ge3_ctx *c) {
 
 int check_params(ge3_ctx *c, const ge3_step_params *p) {
     if (!p) return fail(c, GE_E_INVALID, "params is NULL");
-    if (p->sph) return fail(c, GE_E_UNSUPPORTED, "SPH3D is not implemented");
+    if (p->sph && !c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    if (p->sph && !(p->sph_h > 0.0f && p->sph_cell > 0.0f)) return fail(c, GE_E_INVALID, "sph_h and sph_cell must be > 0");
+    if (p->sph && p->sph_ground && !(p->domain_w > 0.0f && p->domain_h > 0.0f && p->domain_d > 0.0f))
+        return fail(c, GE_E_INVALID, "ground mode needs domain_w, domain_h, domain_d > 0");
     if (p->temperature) return fail(c, GE_E_UNSUPPORTED, "temperature transfer is not implemented");
     if (p->periodic && !(p->domain_w > 0.0f && p->domain_h > 0.0f && p->domain_d > 0.0f))
         return fail(c, GE_E_INVALID, "periodic mode needs domain_w, domain_h, domain_d > 0");
@@ -150,25 +191,40 @@ bool use_direct(const ge3_ctx *c, const ge3_step_params *p) {
 // One gravity frame of mode3D (globalLogic.cpp:1580-1719).
 void frame(ge3_ctx *c, const ge3_step_params *p, float dt_start, bool count) {
     c->leaf_cap = p->leaf_capacity;
+    const bool sph = p->sph != 0;
     if (dt_start != 0.0f) {
         // integrateStart3D + boundingBox3D, computeMortonKeys3D, the sort, the octree
-        launch3_integrate_bbox(c, dt_start, 1, domain_of(p));
+        launch3_integrate_bbox(c, dt_start, 1, domain_of(p), sph ? 1 : 0, p->sph_max_vel);
         launch3_morton_keys(c, c->b.bb);
         launch3_sort_gather(c);
+        c->have_grid = false;
         if (!p->naive) {
             launch_octree_build(c, c->b.bb);
             c->have_tree = true;
         }
     }
     if (p->dt == 0.0f) return;
-    const float end_kick = p->dt * 0.5f;  // integrateEnd3D fused into the force kernel
+    // newGridAVX2 over the sorted positions (globalLogic.cpp:1647)
+    if (sph) sph3_new_grid(c, p);
+    // integrateEnd3D is fused into the last kernel that writes acc: the force
+    // kernel, or the pressure pass when SPH runs (unless the ground walls
+    // follow it)
+    const float end_kick = p->dt * 0.5f;
+    const float grav_kick = sph ? 0.0f : end_kick;
     if (p->G == 0.0f) {
         launch3_zero_acc(c);
-        launch3_integrate_end(c, p->dt);
+        if (!sph) launch3_integrate_end(c, p->dt);
     } else if (use_direct(c, p) || !c->have_tree) {
-        launch3_direct(c, p, end_kick);
+        launch3_direct(c, p, grav_kick);
     } else {
-        launch3_bh(c, p, end_kick, count);
+        launch3_bh(c, p, grav_kick, count);
+    }
+    if (sph) {  // sph3D.pcisphSolver (globalLogic.cpp:1707-1709)
+        sph3_pcisph(c, p, p->sph_ground ? 0.0f : p->dt);
+        if (p->sph_ground) {
+            sph3_ground(c, p);
+            launch3_integrate_end(c, p->dt);
+        }
     }
 }
 
@@ -191,6 +247,24 @@ int put_fields(ge3_ctx *c, uint32_t n, const ge3_particles *p) {
         ids.resize(n);
         for (uint32_t i = 0; i < n; ++i) ids[i] = i;
         GE3_CHECK(c, hipMemcpyAsync(s.id, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->sph) {
+        const float *ssrc[15] = {p->sph_mass, p->rest_dens, p->stiff, p->visc, p->cohesion, p->dens, p->press,
+                                 p->press_tmp, p->pred_dens, p->ppx, p->ppy, p->ppz, p->pfx, p->pfy, p->pfz};
+        float *sdst[15] = {s.sph_mass, s.rest_dens, s.stiff, s.visc, s.cohesion, s.dens, s.press, s.press_tmp,
+                           s.pred_dens, s.ppx, s.ppy, s.ppz, s.pfx, s.pfy, s.pfz};
+        std::vector<float> ones;
+        for (int k = 0; k < 15; ++k) {
+            if (ssrc[k]) {
+                GE3_CHECK(c, hipMemcpyAsync(sdst[k], ssrc[k], fb, hipMemcpyHostToDevice, c->stream));
+            } else if (k == 0) {  // sph_mass defaults to 1
+                ones.assign(n, 1.0f);
+                GE3_CHECK(c, hipMemcpyAsync(sdst[k], ones.data(), fb, hipMemcpyHostToDevice, c->stream));
+            } else {
+                GE3_CHECK(c, hipMemsetAsync(sdst[k], 0, fb, c->stream));
+            }
+        }
+        GE3_CHECK(c, hipStreamSynchronize(c->stream));
     }
     GE3_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
@@ -220,6 +294,16 @@ void ge3_default_params(ge3_step_params *p) {
     p->domain_h = 1300.0f;
     p->domain_d = 1300.0f;
     p->compensated = 1;
+    p->sph_h = 3.0f;             // SPH3D::radiusMultiplier (SPH3D.h:13)
+    p->sph_cell = 3.0f;          // NeighborSearch3DV2AVX2::cellSize (neighborSearch.h:981)
+    p->sph_unit_mass = 0.03f;    // parameters.h:311
+    p->viscosity = 0.3f;
+    p->cohesion_coef = 1.0f;
+    p->delta = 19000.0f;
+    p->stiff_mult = 1.0f;
+    p->sph_max_vel = 250.0f;     // parameters.h:118
+    p->vertical_gravity = 3.0f;  // parameters.h:316
+    p->boundary_friction = 0.0f; // parameters.h:361
 }
 
 int ge3_create(ge3_ctx **out, const ge3_config *cfg) {
@@ -238,7 +322,10 @@ int ge3_create(ge3_ctx **out, const ge3_config *cfg) {
         delete c;
         return GE_E_HIP;
     }
-    if (int r = alloc_all(c)) {
+    c->sph = cfg->enable_sph != 0;
+    int ra = alloc_all(c);
+    if (ra == GE_OK && c->sph) ra = alloc_sph(c);
+    if (int r = ra) {
         free_all(c);
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -282,6 +369,7 @@ int ge3_upload(ge3_ctx *c, uint32_t n, const ge3_particles *p) {
     c->n = n;
     c->b.cur = 0;
     c->have_tree = false;
+    c->have_grid = false;
     if (n == 0) return GE_OK;
     if (int r = put_fields(c, n, p)) return r;
     launch3_pack(c);
@@ -303,6 +391,14 @@ int ge3_download(ge3_ctx *c, const ge3_particles *p) {
         if (p->flags)
             GE3_CHECK(c, hipMemcpyAsync(p->flags, s.flags, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
         if (p->id) GE3_CHECK(c, hipMemcpyAsync(p->id, s.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+        if (c->sph) {
+            float *sdst[15] = {p->sph_mass, p->rest_dens, p->stiff, p->visc, p->cohesion, p->dens, p->press,
+                               p->press_tmp, p->pred_dens, p->ppx, p->ppy, p->ppz, p->pfx, p->pfy, p->pfz};
+            const float *ssrc[15] = {s.sph_mass, s.rest_dens, s.stiff, s.visc, s.cohesion, s.dens, s.press,
+                                     s.press_tmp, s.pred_dens, s.ppx, s.ppy, s.ppz, s.pfx, s.pfy, s.pfz};
+            for (int k = 0; k < 15; ++k)
+                if (sdst[k]) GE3_CHECK(c, hipMemcpyAsync(sdst[k], ssrc[k], fb, hipMemcpyDeviceToHost, c->stream));
+        }
     }
     GE3_CHECK(c, hipStreamSynchronize(c->stream));
     return check_device_errors(c);
@@ -344,6 +440,7 @@ int ge3_sort_by_morton_key(ge3_ctx *c) {
     (void)hipSetDevice(c->device);
     launch3_sort_gather(c);
     c->have_tree = false;
+    c->have_grid = false;
     GE3_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }
@@ -424,9 +521,10 @@ int ge3_integrate_start(ge3_ctx *c, const ge3_step_params *p) {
     if (int r = check_params(c, p)) return r;
     if (c->n == 0) return GE_OK;
     (void)hipSetDevice(c->device);
-    launch3_integrate_bbox(c, p->dt_start, 1, domain_of(p));
+    launch3_integrate_bbox(c, p->dt_start, 1, domain_of(p), p->sph ? 1 : 0, p->sph_max_vel);
     launch3_pack(c);
     c->have_tree = false;
+    c->have_grid = false;
     return ge3_sync(c);
 }
 
@@ -448,6 +546,51 @@ int ge3_step(ge3_ctx *c, const ge3_step_params *p, int32_t frames) {
     for (int32_t k = 0; k < frames; ++k) frame(c, p, k == 0 ? p->dt_start : p->dt, false);
     GE3_CHECK(c, hipGetLastError());
     return GE_OK;
+}
+
+int ge3_sph_new_grid(ge3_ctx *c, const ge3_step_params *p) {
+    if (!c) return GE_E_INVALID;
+    if (int r = check_params(c, p)) return r;
+    if (!c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    if (!(p->sph_cell > 0.0f)) return fail(c, GE_E_INVALID, "sph_cell must be > 0");
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    sph3_new_grid(c, p);
+    return ge3_sync(c);
+}
+
+int ge3_sph_pcisph(ge3_ctx *c, const ge3_step_params *p) {
+    if (!c) return GE_E_INVALID;
+    if (int r = check_params(c, p)) return r;
+    if (!c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    if (!(p->sph_h > 0.0f)) return fail(c, GE_E_INVALID, "sph_h must be > 0");
+    if (c->n == 0) return GE_OK;
+    if (!c->have_grid || c->grid_cell != p->sph_cell)
+        return fail(c, GE_E_INVALID, "no neighbour grid for the current positions and sph_cell (ge3_sph_new_grid)");
+    (void)hipSetDevice(c->device);
+    sph3_pcisph(c, p, 0.0f);
+    if (p->sph_ground) sph3_ground(c, p);
+    return ge3_sync(c);
+}
+
+int ge3_get_neighbors(ge3_ctx *c, int32_t which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total) {
+    if (!c || !off || !total || (which != 0 && which != 1)) return GE_E_INVALID;
+    if (!c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
+    *total = 0;
+    off[0] = 0;
+    if (c->n == 0) return GE_OK;
+    if (!c->have_grid) return fail(c, GE_E_INVALID, "no neighbour grid (ge3_sph_new_grid)");
+    (void)hipSetDevice(c->device);
+    if (int r = sph3_neighbors(c, which, off, idx, cap, total)) return r;
+    return check_device_errors(c);
+}
+
+int ge3_neighbor_counts(ge3_ctx *c, const ge3_step_params *p, uint32_t *counts) {
+    if (!c || !counts) return GE_E_INVALID;
+    if (int r = ge3_sph_new_grid(c, p)) return r;
+    if (c->n == 0) return GE_OK;
+    if (int r = sph3_neighbor_counts(c, counts)) return r;
+    return check_device_errors(c);
 }
 
 int ge3_direct_fp64(ge3_ctx *c, double G, double eps, uint32_t nt, const uint32_t *targets, double *ax, double *ay,

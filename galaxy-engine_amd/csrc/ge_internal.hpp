@@ -200,6 +200,8 @@ namespace ge {
 
 // ---- launchers (each defined next to its kernels) ----
 // sort.hip
+void radix_sort_words(hipStream_t stream, uint32_t *hist, uint64_t *&data, uint64_t *&alt, uint32_t n, int lo_bit,
+                      int hi_bit);
 void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n,
                        int lo_bit, int hi_bit);
 void radix_sort_kv(hipStream_t stream, uint32_t *hist, uint64_t *&keys, uint64_t *&keys_alt, uint32_t *&vals,

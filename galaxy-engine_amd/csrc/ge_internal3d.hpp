@@ -1,6 +1,6 @@
-// ge_internal3d.hpp -- device-side layout of the 3D gravity context
-// (ge3_ctx) shared by ge_step3d.hip, ge_octree.hip and ge_abi3d.hip.  Not
-// part of the public ABI.
+// ge_internal3d.hpp -- device-side layout of the 3D context (ge3_ctx)
+// shared by ge_step3d.hip, ge_octree.hip, ge_sph3d.hip and ge_abi3d.hip.
+// Not part of the public ABI.
 #pragma once
 
 #include "ge_internal.hpp"
@@ -22,6 +22,25 @@ struct State3 {
     float *pvx = nullptr, *pvy = nullptr, *pvz = nullptr;
     float *m = nullptr;
     uint32_t *flags = nullptr, *id = nullptr;
+    // SPH3D fields (allocated with enable_sph)
+    float *sph_mass = nullptr, *rest_dens = nullptr, *stiff = nullptr, *visc = nullptr;
+    float *cohesion = nullptr, *dens = nullptr, *press = nullptr, *press_tmp = nullptr;
+    float *pred_dens = nullptr, *ppx = nullptr, *ppy = nullptr, *ppz = nullptr;
+    float *pfx = nullptr, *pfy = nullptr, *pfz = nullptr;
+};
+
+// SPH3D neighbour grid: particles sorted by a hash slot of their cell,
+// first[s] the first sorted position of slot s (s in [0, slots]); every
+// query compares exact cell coordinates, so the candidate sets are the
+// reference's whatever the hash.
+struct Grid3 {
+    uint64_t *keys = nullptr, *keys_alt = nullptr;  // (slot << kIdxBits | particle), sorted
+    uint32_t *first = nullptr;
+    uint32_t *ord = nullptr;   // particle of sorted position k
+    int4 *cell = nullptr;      // (cx, cy, cz, flags) per sorted position
+    float4 *pk0 = nullptr;     // grid A: (x, y, z, sphMass); grid B: (ppx, ppy, ppz, press)
+    float4 *pk1 = nullptr;     // grid A: (vx, vy, vz, visc); grid B: (predDens, sphMass, -, -)
+    float2 *pk2 = nullptr;     // grid A: (cohesion, dens)
 };
 
 // Octree node record read by the walk: 32 bytes, two 16-byte scalar loads.
@@ -70,6 +89,12 @@ struct Buffers3 {
     uint32_t node_cap = 0;
     double *diag = nullptr;        // FP64 direct-sum scratch (on first use)
     size_t diag_cap = 0;
+    // SPH3D (enable_sph)
+    Grid3 grid_pos, grid_pred;     // over pos (A) and over predPos of visiting particles (B)
+    float4 *sphf = nullptr;        // pair-force sums per particle
+    float4 *sph_pp = nullptr;      // grid-A order: (ppx, ppy, ppz, restDens)
+    float2 *sph_pd = nullptr;      // grid-A order: (press, predDens)
+    uint32_t slot_bits = 0;        // hash slots = 1 << slot_bits
 };
 
 }  // namespace ge
@@ -82,6 +107,9 @@ struct ge3_ctx {
     uint32_t nblocks_bb = 1;       // workgroups of the bbox reduction
     uint32_t leaf_cap = 0;
     bool have_tree = false;
+    bool sph = false;              // SPH3D state allocated
+    bool have_grid = false;        // grid A matches the current positions
+    float grid_cell = 3.0f;        // cell size grid A was built with
     ge::Buffers3 b;
     std::string err;
 };
@@ -89,7 +117,8 @@ struct ge3_ctx {
 namespace ge {
 
 // ge_step3d.hip
-void launch3_integrate_bbox(ge3_ctx *c, float dt, int do_integrate, Domain3 dom);
+void launch3_integrate_bbox(ge3_ctx *c, float dt, int do_integrate, Domain3 dom, int sph_clamp = 0,
+                           float sph_max_vel = 0.0f);
 void launch3_morton_keys(ge3_ctx *c, const float *bb_device);
 void launch3_sort_gather(ge3_ctx *c);
 void launch3_zero_acc(ge3_ctx *c);
@@ -100,6 +129,12 @@ void launch3_direct_fp64(ge3_ctx *c, double G, double eps, uint32_t nt, const ui
 void launch3_pack(ge3_ctx *c);
 // ge_octree.hip
 void launch_octree_build(ge3_ctx *c, const float *bb_device);
+// ge_sph3d.hip
+void sph3_new_grid(ge3_ctx *c, const ge3_step_params *p);
+void sph3_pcisph(ge3_ctx *c, const ge3_step_params *p, float end_kick_dt);
+void sph3_ground(ge3_ctx *c, const ge3_step_params *p);
+int sph3_neighbors(ge3_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total);
+int sph3_neighbor_counts(ge3_ctx *c, uint32_t *counts);
 
 constexpr uint32_t kBBoxChunk3 = 8192;  // particles per bbox workgroup
 

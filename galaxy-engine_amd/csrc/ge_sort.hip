@@ -167,22 +167,26 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(const uint64_t *__rest
 
 // Sorts n packed words by bits [lo_bit, hi_bit).  On return `data` points to
 // the sorted array (the two buffers may have been swapped).
-void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n, int lo_bit,
-                       int hi_bit) {
+void radix_sort_words(hipStream_t stream, uint32_t *hist, uint64_t *&data, uint64_t *&alt, uint32_t n, int lo_bit,
+                      int hi_bit) {
     if (n <= 1) return;
     const uint32_t nb = blocks_for(n, kTile);
     for (int shift = lo_bit; shift < hi_bit; shift += 8) {
         int bits = hi_bit - shift < 8 ? hi_bit - shift : 8;
         uint32_t dmask = (1u << bits) - 1u;
-        hipLaunchKernelGGL(k_digit_count, dim3(nb), dim3(kSortThreads), 0, c->stream, data, n, shift,
-                           dmask, c->b.hist);
-        hipLaunchKernelGGL(k_scan_table, dim3(1), dim3(kScanThreads), 0, c->stream, c->b.hist, nb);
-        hipLaunchKernelGGL(k_scatter<false>, dim3(nb), dim3(kSortThreads), 0, c->stream, data, alt, n, shift,
-                           bits, c->b.hist, nullptr, nullptr);
+        hipLaunchKernelGGL(k_digit_count, dim3(nb), dim3(kSortThreads), 0, stream, data, n, shift, dmask, hist);
+        hipLaunchKernelGGL(k_scan_table, dim3(1), dim3(kScanThreads), 0, stream, hist, nb);
+        hipLaunchKernelGGL(k_scatter<false>, dim3(nb), dim3(kSortThreads), 0, stream, data, alt, n, shift, bits, hist,
+                           nullptr, nullptr);
         uint64_t *t = data;
         data = alt;
         alt = t;
     }
+}
+
+void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n, int lo_bit,
+                       int hi_bit) {
+    radix_sort_words(c->stream, c->b.hist, data, alt, n, lo_bit, hi_bit);
 }
 
 // Sorts n (key, value) pairs by key bits [lo_bit, hi_bit), stably; on return

@@ -1,5 +1,6 @@
-// ge_step3d.hip -- per-frame streaming kernels of the 3D gravity step:
-// integrateStart3D fused with the boundingBox3D reduction, Morton3D keys, the
+// ge_step3d.hip -- per-frame streaming kernels of the 3D step:
+// integrateStart3D (with the SPH speed clamp) fused with the boundingBox3D
+// reduction, Morton3D keys, the
 // key-value sort and Morton-order gather, the wave-coherent octree walk of
 // calculateForceFromGrid3D (with integrateEnd3D's kick fused in), the direct
 // sum of naiveGravity3D, integrateEnd3D, and an FP64 direct sum for error
@@ -55,9 +56,11 @@ __device__ __forceinline__ void block_minmax3(float (&mn)[3], float (&mx)[3]) {
 // Physics3D::integrateStart3D (physics3D.cpp:740-804), then the per-workgroup
 // part of boundingBox3D (globalLogic.cpp:1465-1477) over the drifted positions.
 __global__ __launch_bounds__(kBlock) void k3_integrate_bbox(State3 s, uint32_t n, float dt, int do_integrate,
-                                                            Domain3 dom, float *__restrict__ partials) {
+                                                            Domain3 dom, int sph_clamp, float max_vel,
+                                                            float *__restrict__ partials) {
 #pragma clang fp contract(off)
     const float half_dt = dt * 0.5f;
+    const float max_vsq = max_vel * max_vel;
     const bool per = dom.w > 0.0f;
     const float hw = dom.w * 0.5f, hh = dom.h * 0.5f, hd = dom.d * 0.5f;
     float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
@@ -66,10 +69,22 @@ __global__ __launch_bounds__(kBlock) void k3_integrate_bbox(State3 s, uint32_t n
     for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         float x = s.x[i], y = s.y[i], z = s.z[i];
         if (do_integrate && !(s.flags[i] & GE_FLAG_PINNED)) {
-            const float pvx = s.vx[i], pvy = s.vy[i], pvz = s.vz[i];
-            const float vx = pvx + s.ax[i] * half_dt;
-            const float vy = pvy + s.ay[i] * half_dt;
-            const float vz = pvz + s.az[i] * half_dt;
+            float pvx = s.vx[i], pvy = s.vy[i], pvz = s.vz[i];
+            float vx = pvx + s.ax[i] * half_dt;
+            float vy = pvy + s.ay[i] * half_dt;
+            float vz = pvz + s.az[i] * half_dt;
+            if (sph_clamp) {  // SPH speed clamp, physics3D.cpp:761-781
+                const float vsq = vx * vx + vy * vy + vz * vz;
+                if (vsq > max_vsq) {
+                    const float psq = pvx * pvx + pvy * pvy + pvz * pvz;
+                    if (psq > 0.00001f) {
+                        const float inv = max_vel / sqrtf(psq);
+                        pvx = pvx * inv; pvy = pvy * inv; pvz = pvz * inv;
+                    }
+                    const float inv = max_vel / sqrtf(vsq);
+                    vx = vx * inv; vy = vy * inv; vz = vz * inv;
+                }
+            }
             s.pvx[i] = pvx; s.pvy[i] = pvy; s.pvz[i] = pvz;
             s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
             x = x + vx * dt;
@@ -165,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void k3_morton_keys(const float *__restrict
 
 // sortParticlesByMortonKey3D's gather (morton.cpp:129-136) of every field,
 // plus the (x, y, z, m) stream the walk and the tree read
-__global__ __launch_bounds__(kBlock) void k3_gather(State3 src, State3 dst, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k3_gather(State3 src, State3 dst, uint32_t n, int sph,
                                                     const uint32_t *__restrict__ sorted_slots,
                                                     uint32_t *__restrict__ perm, float4 *__restrict__ pxyzm) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -180,6 +195,15 @@ __global__ __launch_bounds__(kBlock) void k3_gather(State3 src, State3 dst, uint
     dst.flags[i] = src.flags[j];
     dst.id[i] = src.id[j];
     pxyzm[i] = make_float4(x, y, z, m);
+    if (sph) {
+        dst.sph_mass[i] = src.sph_mass[j]; dst.rest_dens[i] = src.rest_dens[j];
+        dst.stiff[i] = src.stiff[j]; dst.visc[i] = src.visc[j];
+        dst.cohesion[i] = src.cohesion[j]; dst.dens[i] = src.dens[j];
+        dst.press[i] = src.press[j]; dst.press_tmp[i] = src.press_tmp[j];
+        dst.pred_dens[i] = src.pred_dens[j];
+        dst.ppx[i] = src.ppx[j]; dst.ppy[i] = src.ppy[j]; dst.ppz[i] = src.ppz[j];
+        dst.pfx[i] = src.pfx[j]; dst.pfy[i] = src.pfy[j]; dst.pfz[i] = src.pfz[j];
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k3_pack(State3 s, uint32_t n, float4 *__restrict__ pxyzm) {
@@ -454,10 +478,10 @@ Domain3 domain3_of(const ge3_step_params *p) {
 
 }  // namespace
 
-void launch3_integrate_bbox(ge3_ctx *c, float dt, int do_integrate, Domain3 dom) {
+void launch3_integrate_bbox(ge3_ctx *c, float dt, int do_integrate, Domain3 dom, int sph_clamp, float sph_max_vel) {
     const uint32_t nb = blocks_for(c->n, kBBoxChunk3);
     hipLaunchKernelGGL(k3_integrate_bbox, dim3(nb), dim3(kBlock), 0, c->stream, c->b.st[c->b.cur], c->n, dt,
-                       do_integrate, dom, c->b.partials);
+                       do_integrate, dom, sph_clamp, sph_max_vel, c->b.partials);
     hipLaunchKernelGGL(k3_bbox_final, dim3(1), dim3(kBlock), 0, c->stream, c->b.partials, nb, c->b.bb);
 }
 
@@ -471,7 +495,7 @@ void launch3_sort_gather(ge3_ctx *c) {
     radix_sort_kv(c->stream, c->b.hist, c->b.sk_a, c->b.sk_b, c->b.sv_a, c->b.sv_b, c->n, 0, kMortonBits3);
     const int cur = c->b.cur;
     hipLaunchKernelGGL(k3_gather, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.st[cur],
-                       c->b.st[1 - cur], c->n, c->b.sv_a, c->b.perm, c->b.pxyzm);
+                       c->b.st[1 - cur], c->n, c->sph ? 1 : 0, c->b.sv_a, c->b.perm, c->b.pxyzm);
     c->b.cur = 1 - cur;
 }
 

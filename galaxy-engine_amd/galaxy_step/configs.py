@@ -214,3 +214,45 @@ def galaxy3d_small(seed=6, n=20_000, scale=100.0):
     """One 3D galaxy of n particles (scale 100, clip 600, central 5e4)."""
     st = _state3(*galaxy3d(_rng(seed), n - 1, scale, 600.0, 5e4))
     return st, dict(G=1.0, theta=0.8, softening=1.0, dt=0.01, steps=100)
+
+
+def uniform_ball(rng, n, radius):
+    """n points uniform in a ball: r = R u^(1/3), isotropic direction."""
+    r = radius * np.cbrt(rng.uniform(0.0, 1.0, n))
+    cz = rng.uniform(-1.0, 1.0, n)
+    ph = rng.uniform(0.0, 2.0 * math.pi, n)
+    s = np.sqrt(1.0 - cz * cz)
+    return r * s * np.cos(ph), r * s * np.sin(ph), r * cz
+
+
+def config3d_sph(seed=7, n_target=200_000, n_impactor=50_000, density=0.3, speed=8.0):
+    """3D gravity + SPH impact: uniform balls of n_target (at rest at the
+    origin) and n_impactor particles at `density` particles per unit volume,
+    the impactor moving +x at `speed` with impact parameter half the target
+    radius.  m = 1, sphMass = 1, stiff / visc / cohesion 1; restDens is set
+    so the reference's predicted density (normalised by restDens) starts at
+    restDens: restDens^2 = 2 * mass * density.  h = cell = 3 (SPH3D's
+    radiusMultiplier and the AVX2 grid's cellSize), dt 0.002."""
+    rng = _rng(seed)
+    rt = (n_target / density / (4.0 / 3.0 * math.pi)) ** (1.0 / 3.0)
+    ri = (n_impactor / density / (4.0 / 3.0 * math.pi)) ** (1.0 / 3.0)
+    tx, ty, tz = uniform_ball(rng, n_target, rt)
+    ix, iy, iz = uniform_ball(rng, n_impactor, ri)
+    x = np.concatenate([tx, ix - (rt + ri + 6.0)])
+    y = np.concatenate([ty, iy + 0.5 * rt])
+    z = np.concatenate([tz, iz])
+    n = n_target + n_impactor
+    vx = np.concatenate([np.zeros(n_target), np.full(n_impactor, speed)])
+    st = _state3(x, y, z, vx, np.zeros(n), np.zeros(n), np.ones(n))
+    st["flags"] = np.full(n, GE_FLAG_SPH, np.uint32)
+    unit = 0.03
+    rest = math.sqrt(2.0 * unit * density)
+    st.update({
+        "sph_mass": np.ones(n, np.float32), "rest_dens": np.full(n, rest, np.float32),
+        "stiff": np.ones(n, np.float32), "visc": np.ones(n, np.float32),
+        "cohesion": np.ones(n, np.float32), "dens": np.zeros(n, np.float32),
+    })
+    prm = dict(G=1.0, theta=0.7, softening=1.0, dt=0.002, steps=500, sph=1, sph_h=3.0, sph_cell=3.0,
+               sph_unit_mass=unit, viscosity=0.3, cohesion_coef=1.0, delta=19000.0, stiff_mult=1.0,
+               sph_max_vel=250.0)
+    return st, prm

@@ -2,8 +2,10 @@
 
 Context3D wraps one ge3_ctx: upload/download of SoA numpy arrays, ge3_step
 and the stage entry points that replace the reference's Morton3D / Octree /
-Physics3D member functions (GE/src/Physics/morton.cpp:64-138,
-quadtree.cpp:109-292, physics3D.cpp:131-470, 740-1168).
+Physics3D / NeighborSearch3DV2AVX2 / SPH3D member functions
+(GE/src/Physics/morton.cpp:64-138, quadtree.cpp:109-292,
+physics3D.cpp:131-470, 740-1168, GE/include/Particles/neighborSearch.h:
+978-1193, GE/src/Physics/SPH3D.cpp:98-366).
 """
 import ctypes as C
 
@@ -16,16 +18,21 @@ EXPORTS3 = [
     "ge3_count", "ge3_upload", "ge3_download", "ge3_bounding_box", "ge3_compute_morton_keys",
     "ge3_get_morton_keys", "ge3_sort_by_morton_key", "ge3_get_permutation", "ge3_build_octree",
     "ge3_get_node_count", "ge3_get_nodes", "ge3_compute_gravity", "ge3_integrate_start", "ge3_integrate_end",
-    "ge3_step", "ge3_direct_fp64", "ge3_get_interaction_counts",
+    "ge3_step", "ge3_direct_fp64", "ge3_get_interaction_counts", "ge3_sph_new_grid", "ge3_sph_pcisph",
+    "ge3_get_neighbors", "ge3_neighbor_counts",
 ]
 
 FIELDS3_F = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "pvx", "pvy", "pvz", "mass"]
 FIELDS3_U = ["flags", "id"]
-PARTICLE3_ORDER = FIELDS3_F + FIELDS3_U
+FIELDS3_SPH = ["sph_mass", "rest_dens", "stiff", "visc", "cohesion", "dens", "press", "press_tmp", "pred_dens",
+               "ppx", "ppy", "ppz", "pfx", "pfy", "pfz"]
+PARTICLE3_ORDER = FIELDS3_F + FIELDS3_U + FIELDS3_SPH
+GRAVITY3_FIELDS = FIELDS3_F + FIELDS3_U
 
 
 class Ge3Config(C.Structure):
-    _fields_ = [("capacity", C.c_uint32), ("device", C.c_int32), ("reserved", C.c_uint32 * 6)]
+    _fields_ = [("capacity", C.c_uint32), ("device", C.c_int32), ("enable_sph", C.c_int32),
+                ("reserved", C.c_uint32 * 5)]
 
 
 class Ge3Particles(C.Structure):
@@ -37,7 +44,11 @@ class Step3Params(C.Structure):
                 ("dt", C.c_float), ("naive", C.c_int32), ("direct_below", C.c_int32), ("periodic", C.c_int32),
                 ("domain_w", C.c_float), ("domain_h", C.c_float), ("domain_d", C.c_float), ("sph", C.c_int32),
                 ("temperature", C.c_int32), ("compensated", C.c_int32), ("leaf_capacity", C.c_uint32),
-                ("walk_block", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("walk_block", C.c_uint32), ("sph_h", C.c_float), ("sph_cell", C.c_float),
+                ("sph_unit_mass", C.c_float), ("viscosity", C.c_float), ("cohesion_coef", C.c_float),
+                ("delta", C.c_float), ("stiff_mult", C.c_float), ("sph_max_vel", C.c_float),
+                ("sph_ground", C.c_int32), ("vertical_gravity", C.c_float), ("boundary_friction", C.c_float),
+                ("reserved", C.c_uint32 * 4)]
 
 
 class Ge3Node(C.Structure):
@@ -87,6 +98,10 @@ def load_library3():
         "ge3_direct_fp64": (C.c_int, [vp, C.c_double, C.c_double, C.c_uint32, P(C.c_uint32), P(C.c_double),
                                       P(C.c_double), P(C.c_double)]),
         "ge3_get_interaction_counts": (C.c_int, [vp, P(C.c_uint64)]),
+        "ge3_sph_new_grid": (C.c_int, [vp, P(Step3Params)]),
+        "ge3_sph_pcisph": (C.c_int, [vp, P(Step3Params)]),
+        "ge3_get_neighbors": (C.c_int, [vp, C.c_int32, P(C.c_int64), P(C.c_uint32), C.c_int64, P(C.c_int64)]),
+        "ge3_neighbor_counts": (C.c_int, [vp, P(Step3Params), P(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -118,12 +133,14 @@ def _uptr(a):
 class Context3D:
     """One 3D device context (ge3_ctx)."""
 
-    def __init__(self, capacity, device=0):
+    def __init__(self, capacity, device=0, sph=False):
         self.lib = load_library3()
         cfg = Ge3Config()
         self.lib.ge3_default_config(C.byref(cfg))
         cfg.capacity = int(capacity)
         cfg.device = int(device)
+        cfg.enable_sph = 1 if sph else 0
+        self.sph = bool(sph)
         h = C.c_void_p()
         rc = self.lib.ge3_create(C.byref(h), C.byref(cfg))
         if rc != GE_OK:
@@ -171,7 +188,7 @@ class Context3D:
 
     def download(self, fields=None):
         n = self.count()
-        fields = fields or PARTICLE3_ORDER
+        fields = fields or (PARTICLE3_ORDER if self.sph else GRAVITY3_FIELDS)
         out = {}
         gp = Ge3Particles()
         for f in fields:
@@ -243,3 +260,25 @@ class Context3D:
         a = (C.c_uint64 * 2)()
         self._check(self.lib.ge3_get_interaction_counts(self.h, a))
         return a[0], a[1]
+
+    def sph_new_grid(self, params):
+        self._check(self.lib.ge3_sph_new_grid(self.h, C.byref(params)))
+
+    def sph_pcisph(self, params):
+        self._check(self.lib.ge3_sph_pcisph(self.h, C.byref(params)))
+
+    def neighbors(self, which=0):
+        """Candidate sets of the current grid at pos (0) or predPos (1): CSR (off, idx)."""
+        n = self.count()
+        off = np.zeros(n + 1, np.int64)
+        total = C.c_int64()
+        op = off.ctypes.data_as(C.POINTER(C.c_int64))
+        self._check(self.lib.ge3_get_neighbors(self.h, int(which), op, None, 0, C.byref(total)))
+        idx = np.zeros(max(total.value, 1), np.uint32)
+        self._check(self.lib.ge3_get_neighbors(self.h, int(which), op, _uptr(idx), total.value, C.byref(total)))
+        return off, idx[:total.value]
+
+    def neighbor_counts(self, params):
+        a = np.zeros(self.count(), np.uint32)
+        self._check(self.lib.ge3_neighbor_counts(self.h, C.byref(params), _uptr(a)))
+        return a

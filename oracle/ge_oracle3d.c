@@ -370,16 +370,284 @@ static void permute3(geo3_state *s, const uint32_t *perm) {
     const int32_t n = s->n;
     uint32_t *t = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
     void *fs[] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az,
-                  s->pvx, s->pvy, s->pvz, s->m, s->flags, s->id};
+                  s->pvx, s->pvy, s->pvz, s->m, s->flags, s->id,
+                  s->sph_mass, s->rest_dens, s->stiff, s->visc, s->cohesion, s->dens,
+                  s->press, s->press_tmp, s->pred_dens, s->ppx, s->ppy, s->ppz, s->pfx, s->pfy, s->pfz};
     for (size_t k = 0; k < sizeof(fs) / sizeof(fs[0]); ++k) {
         uint32_t *a = (uint32_t *)fs[k];
+        if (!a) continue;
         for (int32_t i = 0; i < n; ++i) t[i] = a[perm[i]];
         memcpy(a, t, sizeof(uint32_t) * (size_t)n);
     }
     free(t);
 }
 
-/* mode3D's gravity frame, GE/src/globalLogic.cpp:1580-1719 */
+/* -------------------------------------------------------------- SPH3D --- */
+#define PI_F3 3.14159265358979323846f /* raylib PI (a float) */
+
+/* SPH3D.h:27-53.  pow with float arguments resolves to std::pow(float, float)
+ * once <math.h> is included (raymath.h does), so the 3D kernels compute in
+ * float: powf. */
+void geo3_sph_constants(float h, float out[3]) {
+    out[0] = (2.0f * PI_F3 * powf(h, 5.0f)) / 15.0f; /* smoothingKernel volume */
+    out[1] = -45.0f / (PI_F3 * powf(h, 6.0f));        /* spikyKernelDerivative scale */
+    out[2] = 45.0f / (PI_F3 * powf(h, 6.0f));         /* smoothingKernelLaplacian scale */
+}
+
+/* SPH3D::smoothingKernelCohesion, SPH3D.h:48-53 */
+static inline float coh3(float r, float h) {
+    if (r >= h) return 0.0f;
+    float q = r / h;
+    return (1.0f - q) * (0.5f - q) * (0.5f - q) * 30.0f / (PI_F3 * h * h * h);
+}
+
+typedef struct { int32_t cx, cy, cz; uint32_t i; } cell3_entry;
+
+static int cell3_cmp(const void *a, const void *b) {
+    const cell3_entry *p = (const cell3_entry *)a, *q = (const cell3_entry *)b;
+    if (p->cz != q->cz) return p->cz < q->cz ? -1 : 1;
+    if (p->cy != q->cy) return p->cy < q->cy ? -1 : 1;
+    if (p->cx != q->cx) return p->cx < q->cx ? -1 : 1;
+    return p->i < q->i ? -1 : (p->i > q->i);
+}
+
+/* NeighborSearch3DV2AVX2::posToCellCoord, neighborSearch.h:1005-1010 */
+static inline int32_t cell3(float p, float cell) { return (int32_t)floorf(p / cell); }
+
+typedef struct { cell3_entry *e; int32_t n; float cell; } cell3_grid;
+
+/* newGridAVX2 (neighborSearch.h:1017-1107) restated as a (cz, cy, cx, index)
+ * ordered table; its candidate SET equals the hashed table's, whose queries
+ * compare exact cell coordinates. */
+static cell3_grid grid3_build(int32_t n, const float *x, const float *y, const float *z, float cell) {
+    cell3_grid g;
+    g.n = n;
+    g.cell = cell;
+    g.e = (cell3_entry *)malloc(sizeof(cell3_entry) * (size_t)(n > 0 ? n : 1));
+    for (int32_t i = 0; i < n; ++i) {
+        g.e[i].cx = cell3(x[i], cell);
+        g.e[i].cy = cell3(y[i], cell);
+        g.e[i].cz = cell3(z[i], cell);
+        g.e[i].i = (uint32_t)i;
+    }
+    qsort(g.e, (size_t)n, sizeof(cell3_entry), cell3_cmp);
+    return g;
+}
+
+static int32_t grid3_lower(const cell3_grid *g, int32_t cx, int32_t cy, int32_t cz) {
+    int32_t lo = 0, hi = g->n;
+    while (lo < hi) {
+        int32_t mid = lo + (hi - lo) / 2;
+        const cell3_entry *e = &g->e[mid];
+        int less = e->cz < cz || (e->cz == cz && (e->cy < cy || (e->cy == cy && e->cx < cx)));
+        if (less) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* queryNeighborsAVX2 (neighborSearch.h:1109-1173): dz, dy, dx from -1 to 1,
+ * every particle whose cell matches; no distance filter, self included */
+static int32_t grid3_query(const cell3_grid *g, float qx, float qy, float qz, uint32_t *buf, int32_t cap) {
+    const int32_t cx = cell3(qx, g->cell), cy = cell3(qy, g->cell), cz = cell3(qz, g->cell);
+    int32_t k = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int32_t nx = cx + dx, ny = cy + dy, nz = cz + dz;
+                int32_t p = grid3_lower(g, nx, ny, nz);
+                while (p < g->n && g->e[p].cx == nx && g->e[p].cy == ny && g->e[p].cz == nz) {
+                    if (k < cap) buf[k] = g->e[p].i;
+                    ++k;
+                    ++p;
+                }
+            }
+    return k;
+}
+
+static int u32_cmp3(const void *a, const void *b) {
+    uint32_t p = *(const uint32_t *)a, q = *(const uint32_t *)b;
+    return p < q ? -1 : (p > q);
+}
+
+int64_t geo3_sph_candidates(int32_t n, const float *x, const float *y, const float *z, int32_t nq,
+                            const float *qx, const float *qy, const float *qz, float cell,
+                            int64_t *off, uint32_t *idx, int64_t cap) {
+    cell3_grid g = grid3_build(n, x, y, z, cell);
+    uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+    int64_t total = 0;
+    for (int32_t q = 0; q < nq; ++q) {
+        off[q] = total;
+        int32_t k = grid3_query(&g, qx[q], qy[q], qz[q], buf, n);
+        qsort(buf, (size_t)k, sizeof(uint32_t), u32_cmp3);
+        if (idx && total + k <= cap) memcpy(idx + total, buf, sizeof(uint32_t) * (size_t)k);
+        total += k;
+    }
+    off[nq] = total;
+    free(buf);
+    free(g.e);
+    return total <= cap ? total : -1;
+}
+
+/* SPH3D::computeViscCohesionForces (SPH3D.cpp:98-170) and SPH3D::PCISPH
+ * (SPH3D.cpp:172-308) with maxIter = 1.  The pair sums are applied in a fixed
+ * order (query order, particle by particle) where the reference uses OpenMP
+ * atomics. */
+int32_t geo3_sph_pcisph(geo3_state *s, const geo3_params *p) {
+    const int32_t n = s->n;
+    const float h = p->sph_h, h2 = h * h;
+    float kc[3];
+    geo3_sph_constants(h, kc);
+    const float volume = kc[0], spiky = kc[1], lapS = kc[2];
+    const float coeff = 0.5f * p->dt * p->dt; /* predictionCoeff, SPH3D.cpp:176 */
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    float *fx = (float *)calloc(nn, sizeof(float));
+    float *fy = (float *)calloc(nn, sizeof(float));
+    float *fz = (float *)calloc(nn, sizeof(float));
+    uint32_t *buf = (uint32_t *)malloc(sizeof(uint32_t) * nn);
+    cell3_grid g = grid3_build(n, s->x, s->y, s->z, p->sph_cell);
+    const uint32_t *fl = s->flags;
+#define IS_SPH(k) ((fl[k] & GEO_SPH) != 0)
+#define DRAWN(k) ((fl[k] & GEO_DRAWN) != 0)
+#define PINNED(k) ((fl[k] & GEO_PINNED) != 0)
+    /* viscosity + cohesion, SPH3D.cpp:98-170 */
+    for (int32_t i = 0; i < n; ++i) {
+        if (!IS_SPH(i) || PINNED(i) || DRAWN(i)) continue;
+        int32_t k = grid3_query(&g, s->x[i], s->y[i], s->z[i], buf, n);
+        for (int32_t t = 0; t < k; ++t) {
+            const uint32_t j = buf[t];
+            if (!IS_SPH(j) || DRAWN(j)) continue;
+            if ((int32_t)j == i) continue;
+            const float dx = s->x[j] - s->x[i], dy = s->y[j] - s->y[i], dz = s->z[j] - s->z[i];
+            const float rSq = dx * dx + dy * dy + dz * dz;
+            if (rSq >= h2) continue;
+            const float r = sqrtf(rSq < 1e-6f ? 1e-6f : rSq); /* std::max(rSq, 1e-6f) */
+            const float nrx = dx / r, nry = dy / r, nrz = dz / r;
+            const float mJ = s->sph_mass[j] * p->sph_unit_mass;
+            const float lapW = r >= h ? 0.0f : lapS * (h - r); /* smoothingKernelLaplacian */
+            const float dj = s->dens[j] < 0.001f ? 0.001f : s->dens[j];
+            const float vfx = p->viscosity * s->visc[j] * mJ / dj * lapW * (s->vx[j] - s->vx[i]);
+            const float vfy = p->viscosity * s->visc[j] * mJ / dj * lapW * (s->vy[j] - s->vy[i]);
+            const float vfz = p->viscosity * s->visc[j] * mJ / dj * lapW * (s->vz[j] - s->vz[i]);
+            const float cohCoef = p->cohesion_coef * s->cohesion[i];
+            const float cf = coh3(r, h);
+            const float cfx = cohCoef * mJ * cf * nrx, cfy = cohCoef * mJ * cf * nry, cfz = cohCoef * mJ * cf * nrz;
+            fx[i] = fx[i] + (vfx + cfx);
+            fy[i] = fy[i] + (vfy + cfy);
+            fz[i] = fz[i] + (vfz + cfz);
+            fx[j] = fx[j] - (vfx + cfx);
+            fy[j] = fy[j] - (vfy + cfy);
+            fz[j] = fz[j] - (vfz + cfz);
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) { /* SPH3D.cpp:182-185 */
+        s->press[i] = 0.0f;
+        s->pfx[i] = 0.0f; s->pfy[i] = 0.0f; s->pfz[i] = 0.0f;
+    }
+    /* predicted positions, SPH3D.cpp:190-198 */
+    for (int32_t i = 0; i < n; ++i) {
+        if (!IS_SPH(i) || DRAWN(i)) continue;
+        s->ppx[i] = s->x[i] + (fx[i] / s->sph_mass[i]) * coeff;
+        s->ppy[i] = s->y[i] + (fy[i] / s->sph_mass[i]) * coeff;
+        s->ppz[i] = s->z[i] + (fz[i] / s->sph_mass[i]) * coeff;
+    }
+    const float m2 = p->sph_unit_mass * 2.0f; /* myVar.mass * 2.0f, SPH3D.cpp:225, 262 */
+    /* predicted density and pressure, SPH3D.cpp:200-236 */
+    for (int32_t i = 0; i < n; ++i) {
+        if (!IS_SPH(i) || DRAWN(i)) continue;
+        float pd = 0.0f;
+        int32_t k = grid3_query(&g, s->ppx[i], s->ppy[i], s->ppz[i], buf, n);
+        for (int32_t t = 0; t < k; ++t) {
+            const uint32_t j = buf[t];
+            if (!IS_SPH(j) || DRAWN(j)) continue;
+            const float dx = s->ppx[i] - s->ppx[j], dy = s->ppy[i] - s->ppy[j], dz = s->ppz[i] - s->ppz[j];
+            const float rrSq = dx * dx + dy * dy + dz * dz;
+            if (rrSq >= h * h) continue;
+            const float rr = sqrtf(rrSq);
+            const float mJ = s->sph_mass[j] * m2;
+            const float rho0 = 0.5f * (s->rest_dens[i] + s->rest_dens[j]);
+            const float w = rr >= h ? 0.0f : powf(h - rr, 2.0f) / volume; /* smoothingKernel */
+            pd = pd + mJ * w / rho0;
+        }
+        s->pred_dens[i] = pd;
+        const float err = pd - s->rest_dens[i];
+        float pt = p->delta * err;
+        if (pt < 0.0f) pt = 0.0f;
+        s->press_tmp[i] = pt;
+        s->press[i] = s->press[i] + pt * s->stiff[i] * p->stiff_mult;
+    }
+    /* pressure force, SPH3D.cpp:238-291 */
+    for (int32_t i = 0; i < n; ++i) {
+        if (!IS_SPH(i) || DRAWN(i)) continue;
+        int32_t k = grid3_query(&g, s->ppx[i], s->ppy[i], s->ppz[i], buf, n);
+        for (int32_t t = 0; t < k; ++t) {
+            const uint32_t j = buf[t];
+            if ((int32_t)j == i) continue;
+            if (!IS_SPH(j) || DRAWN(j)) continue;
+            const float dx = s->ppx[i] - s->ppx[j], dy = s->ppy[i] - s->ppy[j], dz = s->ppz[i] - s->ppz[j];
+            const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+            if (rr < 1e-5f || rr >= h) continue;
+            const float gradW = spiky * powf(h - rr, 2.0f); /* spikyKernelDerivative */
+            const float nx = dx / rr, ny = dy / rr, nz = dz / rr;
+            const float avgP = 0.5f * (s->press[i] + s->press[j]);
+            const float avgD = 0.5f * (s->pred_dens[i] + s->pred_dens[j]);
+            float mag = -(s->sph_mass[i] * m2 + s->sph_mass[j] * m2) * avgP / (avgD < 0.01f ? 0.01f : avgD);
+            const float smax = s->sph_mass[i] < s->sph_mass[j] ? s->sph_mass[j] : s->sph_mass[i];
+            const float smin = s->sph_mass[j] < s->sph_mass[i] ? s->sph_mass[j] : s->sph_mass[i];
+            const float ratio = smax / smin;
+            float sc = 8.0f / ratio;
+            if (1.0f < sc) sc = 1.0f;
+            mag = mag * sc;
+            const float pfx = mag * gradW * nx, pfy = mag * gradW * ny, pfz = mag * gradW * nz;
+            fx[i] = fx[i] + pfx; fy[i] = fy[i] + pfy; fz[i] = fz[i] + pfz;
+            fx[j] = fx[j] - pfx; fy[j] = fy[j] - pfy; fz[j] = fz[j] - pfz;
+        }
+    }
+    /* pressF and acceleration, SPH3D.cpp:296-307 */
+    for (int32_t i = 0; i < n; ++i) {
+        s->pfx[i] = fx[i] / s->sph_mass[i];
+        s->pfy[i] = fy[i] / s->sph_mass[i];
+        s->pfz[i] = fz[i] / s->sph_mass[i];
+        if (!PINNED(i)) {
+            s->ax[i] = s->ax[i] + s->pfx[i];
+            s->ay[i] = s->ay[i] + s->pfy[i];
+            s->az[i] = s->az[i] + s->pfz[i];
+        }
+    }
+#undef IS_SPH
+#undef DRAWN
+#undef PINNED
+    free(g.e);
+    free(buf);
+    free(fx);
+    free(fy);
+    free(fz);
+    return 0;
+}
+
+/* SPH3D::groundModeBoundary, SPH3D.cpp:310-366: walls at +-domain/2 */
+void geo3_sph_ground(geo3_state *s, float radius, float w, float h, float d, float vertical_gravity,
+                     float friction) {
+    const float damp = -0.1f; /* SPH3D::boundDamping (SPH3D.h:14) */
+    for (int32_t i = 0; i < s->n; ++i) {
+        if (s->flags[i] & GEO_PINNED) continue;
+        float x = s->x[i], y = s->y[i], z = s->z[i], vx = s->vx[i], vy = s->vy[i], vz = s->vz[i];
+        s->ay[i] = s->ay[i] - vertical_gravity;
+        int hx = 0, hy = 0, hz = 0;
+        if (x - radius < -w / 2.0f) { vx *= damp; x = -w / 2.0f + radius; hx = 1; }
+        if (x + radius > w / 2.0f) { vx *= damp; x = w / 2.0f - radius; hx = 1; }
+        if (y - radius < -h / 2.0f) { vy *= damp; y = -h / 2.0f + radius; hy = 1; }
+        if (y + radius > h / 2.0f) { vy *= damp; y = h / 2.0f - radius; hy = 1; }
+        if (z - radius < -d / 2.0f) { vz *= damp; z = -d / 2.0f + radius; hz = 1; }
+        if (z + radius > d / 2.0f) { vz *= damp; z = d / 2.0f - radius; hz = 1; }
+        if (hx) { vy *= 1.0f - friction; vz *= 1.0f - friction; }
+        if (hy) { vx *= 1.0f - friction; vz *= 1.0f - friction; }
+        if (hz) { vx *= 1.0f - friction; vy *= 1.0f - friction; }
+        s->x[i] = x; s->y[i] = y; s->z[i] = z;
+        s->vx[i] = vx; s->vy[i] = vy; s->vz[i] = vz;
+    }
+}
+
+/* mode3D's physics frame, GE/src/globalLogic.cpp:1580-1719 */
 int32_t geo3_step(geo3_state *s, const geo3_params *p, int32_t nsteps) {
     const int32_t n = s->n;
     if (n <= 0) return 0;
@@ -392,7 +660,7 @@ int32_t geo3_step(geo3_state *s, const geo3_params *p, int32_t nsteps) {
         geo3_node *nodes = NULL;
         int64_t nn = 0;
         if (dt0 != 0.0f) {
-            geo3_integrate_start(s, dt0, p->sph_clamp, p->sph_max_vel, &dom);
+            geo3_integrate_start(s, dt0, p->sph_clamp || p->sph, p->sph_max_vel, &dom);
             float bb[4];
             geo3_bbox(n, s->x, s->y, s->z, bb);
             geo3_morton_keys(n, s->x, s->y, s->z, bb, keys);
@@ -411,6 +679,12 @@ int32_t geo3_step(geo3_state *s, const geo3_params *p, int32_t nsteps) {
                 else
                     geo3_bh_accel(n, s->x, s->y, s->z, s->m, nodes, (int32_t)nn, p->G, p->theta, p->softening,
                                   &dom, s->ax, s->ay, s->az, NULL);
+            }
+            if (p->sph) { /* sph3D.pcisphSolver, globalLogic.cpp:1707-1709 */
+                geo3_sph_pcisph(s, p);
+                if (p->sph_ground)
+                    geo3_sph_ground(s, p->sph_h, p->domain_w, p->domain_h, p->domain_d, p->vertical_gravity,
+                                    p->boundary_friction);
             }
             geo3_integrate_end(s, p->dt);
         }

@@ -15,6 +15,8 @@ NODE3_DTYPE = np.dtype([("posx", "<f4"), ("posy", "<f4"), ("posz", "<f4"),
                         ("next", "<u4"), ("child", "<u4", (8,))])
 
 STATE3_F = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "pvx", "pvy", "pvz", "mass"]
+SPH3_F = ["sph_mass", "rest_dens", "stiff", "visc", "cohesion", "dens", "press", "press_tmp", "pred_dens",
+          "ppx", "ppy", "ppz", "pfx", "pfy", "pfz"]
 
 f32p = C.POINTER(C.c_float)
 u32p = C.POINTER(C.c_uint32)
@@ -25,7 +27,7 @@ f64p = C.POINTER(C.c_double)
 class _State3(C.Structure):
     _fields_ = ([("n", C.c_int32)] + [(f, f32p) for f in
                                       ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "pvx", "pvy", "pvz", "m"]]
-                + [("flags", u32p), ("id", u32p)])
+                + [("flags", u32p), ("id", u32p)] + [(f, f32p) for f in SPH3_F])
 
 
 class _Domain3(C.Structure):
@@ -37,7 +39,11 @@ class _Params3(C.Structure):
                 ("dt_start", C.c_float), ("dt", C.c_float), ("naive", C.c_int32),
                 ("direct_below", C.c_int32), ("sph_clamp", C.c_int32), ("sph_max_vel", C.c_float),
                 ("periodic", C.c_int32), ("domain_w", C.c_float), ("domain_h", C.c_float),
-                ("domain_d", C.c_float), ("leaf_cap", C.c_uint32)]
+                ("domain_d", C.c_float), ("leaf_cap", C.c_uint32), ("sph", C.c_int32),
+                ("sph_h", C.c_float), ("sph_cell", C.c_float), ("sph_unit_mass", C.c_float),
+                ("viscosity", C.c_float), ("cohesion_coef", C.c_float), ("delta", C.c_float),
+                ("stiff_mult", C.c_float), ("sph_ground", C.c_int32), ("vertical_gravity", C.c_float),
+                ("boundary_friction", C.c_float)]
 
 
 _ready = False
@@ -69,6 +75,14 @@ def load():
     lib.geo3_integrate_end.argtypes = [C.POINTER(_State3), C.c_float]
     lib.geo3_step.restype = C.c_int32
     lib.geo3_step.argtypes = [C.POINTER(_State3), C.POINTER(_Params3), C.c_int32]
+    lib.geo3_sph_constants.argtypes = [C.c_float, f32p]
+    lib.geo3_sph_candidates.restype = C.c_int64
+    lib.geo3_sph_candidates.argtypes = [C.c_int32, f32p, f32p, f32p, C.c_int32, f32p, f32p, f32p, C.c_float,
+                                        C.POINTER(C.c_int64), u32p, C.c_int64]
+    lib.geo3_sph_pcisph.restype = C.c_int32
+    lib.geo3_sph_pcisph.argtypes = [C.POINTER(_State3), C.POINTER(_Params3)]
+    lib.geo3_sph_ground.argtypes = [C.POINTER(_State3), C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_float]
     _ready = True
     return lib
 
@@ -162,9 +176,42 @@ def direct_fp64(x, y, z, m, G, eps, targets=None):
     return a[0], a[1], a[2]
 
 
+def sph_constants(h):
+    out = np.zeros(3, np.float32)
+    load().geo3_sph_constants(float(np.float32(h)), _f(out))
+    return out
+
+
+def sph_candidates(x, y, z, qx, qy, qz, cell):
+    """queryNeighborsAVX2 candidate sets: CSR (off, idx), idx ascending per query."""
+    x, y, z, qx, qy, qz = (f32(v) for v in (x, y, z, qx, qy, qz))
+    lib = load()
+    off = np.zeros(len(qx) + 1, np.int64)
+    offp = off.ctypes.data_as(C.POINTER(C.c_int64))
+    total = lib.geo3_sph_candidates(len(x), _f(x), _f(y), _f(z), len(qx), _f(qx), _f(qy), _f(qz),
+                                    float(np.float32(cell)), offp, None, 0)
+    total = int(off[-1])
+    idx = np.zeros(max(total, 1), np.uint32)
+    lib.geo3_sph_candidates(len(x), _f(x), _f(y), _f(z), len(qx), _f(qx), _f(qy), _f(qz),
+                            float(np.float32(cell)), offp, idx.ctypes.data_as(u32p), total)
+    return off, idx[:total]
+
+
 DEFAULTS3 = dict(G=1.0, theta=0.8, softening=1.0, dt=0.01, naive=0, direct_below=5000, sph_clamp=0,
                  sph_max_vel=250.0, periodic=0, domain_w=1300.0, domain_h=1300.0, domain_d=1300.0,
-                 leaf_cap=0)
+                 leaf_cap=0, sph=0, sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3,
+                 cohesion_coef=1.0, delta=19000.0, stiff_mult=1.0, sph_ground=0, vertical_gravity=0.0,
+                 boundary_friction=0.0)
+
+
+def _params3(prm):
+    p = _Params3()
+    d = dict(DEFAULTS3)
+    d.update({k: v for k, v in prm.items() if k in d or k == "dt_start"})
+    d.setdefault("dt_start", d["dt"])
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
 
 
 class State3:
@@ -184,17 +231,25 @@ class State3:
             setattr(self._s, "m" if f == "mass" else f, _f(self.a[f]))
         self._s.flags = self.a["flags"].ctypes.data_as(u32p)
         self._s.id = self.a["id"].ctypes.data_as(u32p)
+        # SPH fields: uploaded values or the device defaults (sph_mass 1, else 0)
+        for f in SPH3_F:
+            v = st.get(f)
+            self.a[f] = f32(v).copy() if v is not None else (np.ones(n, np.float32) if f == "sph_mass"
+                                                             else np.zeros(n, np.float32))
+            setattr(self._s, f, _f(self.a[f]))
 
     def step(self, prm, nsteps=1):
-        p = _Params3()
-        d = dict(DEFAULTS3)
-        d.update({k: v for k, v in prm.items() if k in d or k == "dt_start"})
-        d.setdefault("dt_start", d["dt"])
-        for k, v in d.items():
-            setattr(p, k, v)
+        p = _params3(prm)
         rc = load().geo3_step(C.byref(self._s), C.byref(p), int(nsteps))
         if rc != 0:
             raise RuntimeError(f"oracle 3D step failed {rc}")
+
+    def sph_pcisph(self, prm):
+        p = _params3(prm)
+        load().geo3_sph_pcisph(C.byref(self._s), C.byref(p))
+
+    def sph_ground(self, radius, w, h, d, vertical_gravity, friction):
+        load().geo3_sph_ground(C.byref(self._s), radius, w, h, d, vertical_gravity, friction)
 
     def integrate_start(self, dt, sph_clamp=0, sph_max_vel=250.0, domain=None):
         d = _dom(domain)

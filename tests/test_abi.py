@@ -155,14 +155,17 @@ def test_3d_struct_layout_matches_header(tmp_path):
     from galaxy_step import lib3d as L
     src = tmp_path / "sz3.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ge_physics3d.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(ge3_config), '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ge3_config), '
                    'sizeof(ge3_particles), sizeof(ge3_step_params), sizeof(ge3_node), '
-                   'offsetof(ge3_step_params, compensated), offsetof(ge3_node, child));return 0;}\n')
+                   'offsetof(ge3_step_params, compensated), offsetof(ge3_node, child), '
+                   'offsetof(ge3_config, enable_sph), offsetof(ge3_particles, pfz), '
+                   'offsetof(ge3_step_params, sph_h), offsetof(ge3_step_params, boundary_friction));return 0;}\n')
     exe = tmp_path / "sz3"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
     want = [C.sizeof(L.Ge3Config), C.sizeof(L.Ge3Particles), C.sizeof(L.Step3Params), C.sizeof(L.Ge3Node),
-            L.Step3Params.compensated.offset, L.Ge3Node.child.offset]
+            L.Step3Params.compensated.offset, L.Ge3Node.child.offset, L.Ge3Config.enable_sph.offset,
+            L.Ge3Particles.pfz.offset, L.Step3Params.sph_h.offset, L.Step3Params.boundary_friction.offset]
     assert [int(v) for v in out] == want
 
 
@@ -174,6 +177,9 @@ def test_3d_default_params_are_reference_defaults():
     assert p.softening == pytest.approx(1.5) and p.dt == 1.0 and p.direct_below == 5000
     assert (p.domain_w, p.domain_h, p.domain_d) == (1300.0, 1300.0, 1300.0)
     assert p.periodic == 0 and p.sph == 0 and p.temperature == 0 and p.compensated == 1
+    # SPH3D.h:13, neighborSearch.h:981, parameters.h:118, 311, 316
+    assert p.sph_h == 3.0 and p.sph_cell == 3.0 and p.sph_unit_mass == pytest.approx(0.03)
+    assert p.sph_max_vel == 250.0 and p.vertical_gravity == 3.0 and p.sph_ground == 0
 
 
 def test_3d_create_reports_status_without_crashing():

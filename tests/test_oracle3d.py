@@ -21,6 +21,8 @@ Each function is checked against something independent of its own code:
   geo3_integrate_* -> the closed-form kick-drift(-kick) update and the wrap
       into [-W/2, W/2)
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -330,3 +332,130 @@ def test_step_direct_conserves_momentum():
         assert abs(float(np.sum(s[v].astype(np.float64)))) < 1e-3
     # ids stay a permutation
     assert sorted(s["id"].tolist()) == list(range(600))
+
+
+# ---------------------------------------------------------------- SPH3D ---
+
+def _sph_cloud(n, seed, spread=12.0, offset=-3.0):
+    rng = np.random.default_rng(seed)
+    # negative coordinates too, so floor and truncation would differ
+    return [(rng.uniform(0.0, spread, n) + offset).astype(np.float32) for _ in range(3)]
+
+
+def test_sph3_kernel_constants_closed_form_and_normalisation():
+    from scipy.integrate import quad
+    for h in (3.0, 4.0, 2.5):
+        vol, spiky, lap = (float(v) for v in O3.sph_constants(h))
+        # SPH3D.h:27-46: 2 pi h^5 / 15, -45 / (pi h^6), 45 / (pi h^6)
+        assert vol == pytest.approx(2.0 * math.pi * h ** 5 / 15.0, rel=1e-6)
+        assert spiky == pytest.approx(-45.0 / (math.pi * h ** 6), rel=1e-6)
+        assert lap == pytest.approx(45.0 / (math.pi * h ** 6), rel=1e-6)
+        # the density kernel (h - r)^2 / vol integrates to 1 over the ball
+        integral, _ = quad(lambda r: 4.0 * math.pi * r * r * (h - r) ** 2 / vol, 0.0, h)
+        assert integral == pytest.approx(1.0, rel=1e-6)
+
+
+def test_sph3_candidates_match_brute_force_cells():
+    x, y, z = _sph_cloud(1500, 40)
+    rng = np.random.default_rng(41)
+    qx, qy, qz = [(rng.uniform(-4.0, 10.0, 300)).astype(np.float32) for _ in range(3)]
+    cell = 3.0
+    off, idx = O3.sph_candidates(x, y, z, qx, qy, qz, cell)
+    c = np.stack([np.floor(x / np.float32(cell)), np.floor(y / np.float32(cell)),
+                  np.floor(z / np.float32(cell))], 1).astype(np.int64)
+    for q in range(len(qx)):
+        cq = np.floor(np.array([qx[q], qy[q], qz[q]]) / np.float32(cell)).astype(np.int64)
+        want = np.nonzero(np.all(np.abs(c - cq) <= 1, axis=1))[0]
+        np.testing.assert_array_equal(idx[off[q]:off[q + 1]], want)
+    # a particle is its own candidate (no distance filter, self included)
+    off2, idx2 = O3.sph_candidates(x, y, z, x, y, z, cell)
+    for i in range(0, 1500, 97):
+        assert i in idx2[off2[i]:off2[i + 1]]
+
+
+def _sph3_state(n, seed, spread=10.0, moving=True, pinned=()):
+    x, y, z = _sph_cloud(n, seed, spread, offset=-spread / 2)
+    rng = np.random.default_rng(seed + 1)
+    st = dict(x=x, y=y, z=z, mass=np.ones(n, np.float32),
+              vx=(rng.normal(0, 1, n) if moving else np.zeros(n)).astype(np.float32),
+              vy=(rng.normal(0, 1, n) if moving else np.zeros(n)).astype(np.float32),
+              vz=(rng.normal(0, 1, n) if moving else np.zeros(n)).astype(np.float32),
+              flags=np.full(n, 2, np.uint32), sph_mass=rng.uniform(0.8, 1.2, n).astype(np.float32),
+              rest_dens=np.full(n, 0.2, np.float32), stiff=np.ones(n, np.float32),
+              visc=np.ones(n, np.float32), cohesion=np.ones(n, np.float32))
+    for i in pinned:
+        st["flags"][i] |= 1
+    return st
+
+
+SPH3_PRM = dict(dt=0.002, sph=1, sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3,
+                cohesion_coef=1.0, delta=19000.0, stiff_mult=1.0)
+
+
+def test_sph3_pcisph_conserves_momentum():
+    # every pair force is applied to both particles with opposite signs, so
+    # the sum of sphMass * pressF vanishes when nothing is pinned
+    s = O3.State3(_sph3_state(800, 42))
+    s.sph_pcisph(SPH3_PRM)
+    f = [s["sph_mass"].astype(np.float64) * s[k].astype(np.float64) for k in ("pfx", "pfy", "pfz")]
+    scale = sum(np.abs(v).sum() for v in f)
+    assert scale > 0
+    for v in f:
+        assert abs(v.sum()) < 1e-5 * scale
+    assert np.all(s["pred_dens"] > 0) and np.all(s["press"] >= 0)
+
+
+def test_sph3_two_particles_closed_form():
+    # two SPH particles at rest, r < h: viscosity vanishes, cohesion pulls
+    # them together, pressure pushes them apart; the pressure pair force is
+    # applied by both visits (SPH3D.cpp:238-291), so each feels it twice
+    h, r = 3.0, 1.7
+    st = dict(x=np.array([0.2, 0.2 + r], np.float32), y=np.full(2, 0.4, np.float32),
+              z=np.full(2, 0.4, np.float32), mass=np.ones(2, np.float32), flags=np.full(2, 2, np.uint32),
+              sph_mass=np.ones(2, np.float32), rest_dens=np.full(2, 0.05, np.float32),
+              stiff=np.ones(2, np.float32), visc=np.ones(2, np.float32), cohesion=np.ones(2, np.float32))
+    prm = dict(SPH3_PRM, delta=10.0)
+    s = O3.State3(st)
+    s.sph_pcisph(prm)
+    vol = 2.0 * math.pi * h ** 5 / 15.0
+    unit = prm["sph_unit_mass"]
+    q = r / h
+    wcoh = (1 - q) * (0.5 - q) ** 2 * 30.0 / (math.pi * h ** 3)
+    coh = prm["cohesion_coef"] * 1.0 * unit * wcoh   # on particle 0, along +x, from both visits
+    f_coh = 2.0 * coh
+    dt = prm["dt"]
+    # predicted positions move by the cohesion displacement
+    disp = f_coh / 1.0 * 0.5 * dt * dt
+    rp = r - 2.0 * disp
+    m2 = 2.0 * unit
+    dens = m2 * (h ** 2 / vol) / 0.05 + m2 * ((h - rp) ** 2 / vol) / 0.05
+    press = max(prm["delta"] * (dens - 0.05), 0.0)
+    grad = -45.0 / (math.pi * h ** 6) * (h - rp) ** 2
+    mag = -(m2 + m2) * press / max(dens, 0.01)
+    fp = mag * grad * (-1.0)          # along x for particle 0 (d = x0 - x1 < 0)
+    want0 = f_coh + 2.0 * fp
+    assert s["pred_dens"][0] == pytest.approx(dens, rel=1e-5)
+    assert s["press"][0] == pytest.approx(press, rel=1e-4)
+    assert s["pfx"][0] == pytest.approx(want0, rel=1e-4)
+    assert s["pfx"][1] == pytest.approx(-want0, rel=1e-4)
+    assert abs(s["pfy"][0]) < 1e-6 * abs(want0)
+
+
+def test_sph3_ground_walls():
+    st = dict(x=np.array([-49.5, 49.0, 0.0, 0.0], np.float32), y=np.array([0.0, -49.9, 0.0, 0.0], np.float32),
+              z=np.array([0.0, 0.0, 49.8, 0.0], np.float32), vx=np.array([-2.0, 1.0, 1.0, 1.0], np.float32),
+              vy=np.array([1.0, -3.0, 1.0, 1.0], np.float32), vz=np.array([1.0, 1.0, 4.0, 1.0], np.float32),
+              mass=np.ones(4, np.float32), flags=np.array([2, 2, 2, 3], np.uint32))
+    s = O3.State3(st)
+    s.sph_ground(3.0, 100.0, 100.0, 100.0, 3.0, 0.25)
+    # x wall: position clamped, vx damped by -0.1, friction on vy, vz
+    assert s["x"][0] == np.float32(-50.0 + 3.0) and s["vx"][0] == np.float32(-2.0 * -0.1)
+    assert s["vy"][0] == np.float32(0.75) and s["vz"][0] == np.float32(0.75)
+    assert s["ay"][0] == np.float32(-3.0)
+    # particle 1 hits the +x and the -y walls
+    assert s["x"][1] == np.float32(47.0) and s["y"][1] == np.float32(-47.0)
+    assert s["vx"][1] == np.float32(np.float32(1.0 * -0.1) * 0.75)
+    assert s["vy"][1] == np.float32(np.float32(-3.0 * -0.1) * 0.75)
+    assert s["z"][2] == np.float32(47.0) and s["vz"][2] == np.float32(np.float32(4.0 * -0.1))
+    # pinned: untouched
+    assert s["ay"][3] == 0.0 and s["vx"][3] == 1.0
