@@ -294,6 +294,58 @@ def three_d(a, cpu_seconds):
     return out
 
 
+def sph3d(a, cpu_seconds):
+    """3D gravity + SPH3D impact (mode3D with isSPHEnabled): balls of 200k and
+    50k particles at 0.3 per unit volume, h = cell = 3, theta 0.7, dt 0.002,
+    the reference's octree walk and SPH3D passes."""
+    from galaxy_step import configs
+    from galaxy_step.lib3d import Context3D, default_params3
+    st, prm = configs.config3d_sph()
+    n = len(st["x"])
+    p = default_params3(direct_below=0, compensated=1)
+    for k in ("G", "theta", "softening", "dt", "sph", "sph_h", "sph_cell", "sph_unit_mass", "viscosity",
+              "cohesion_coef", "delta", "stiff_mult", "sph_max_vel"):
+        setattr(p, k, prm[k])
+    p.dt_start = p.dt
+    steps = max(a.steps, 10)
+    with Context3D(n, sph=True) as c:
+        c.upload(st)
+        c.step(p, a.warmup)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(p, steps)
+        c.sync()
+        t1 = time.perf_counter()
+        cnt = c.neighbor_counts(p)
+    el = t1 - t0
+    out = {"workload": f"3D gravity + SPH3D impact, {n} particles (200k + 50k balls), h = cell = 3, "
+                       "theta 0.7, dt 0.002",
+           "particle_updates_per_s": n * steps / el, "ms_per_step": el * 1e3 / steps, "steps": steps,
+           "candidates_per_particle": float(cnt.mean())}
+    if cpu_seconds > 0:
+        import oracle3d as O3
+        s = O3.State3(st)
+        q = {k: getattr(p, k) for k in ("G", "theta", "softening", "dt", "sph", "sph_h", "sph_cell",
+                                         "sph_unit_mass", "viscosity", "cohesion_coef", "delta", "stiff_mult",
+                                         "sph_max_vel")}
+        q.update(dt_start=p.dt, direct_below=0)
+        c0 = time.perf_counter()
+        k = 0
+        while True:
+            s.step(q, 1)
+            k += 1
+            if time.perf_counter() - c0 >= cpu_seconds or k >= 5:
+                break
+        cel = time.perf_counter() - c0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": n * k / cel, "unit": "particle-updates/s", "kind": "port",
+                               "cores": threads,
+                               "sample": f"{k} full 3D gravity + SPH3D frames of the same workload through the C "
+                                         f"restatement ({threads} OpenMP threads for gravity, the SPH passes on "
+                                         f"one thread), {cel:.1f} s"}
+    return out
+
+
 def batched(a, accuracy):
     """cfg3: the batched form against the same systems one by one."""
     import numpy as np
@@ -437,6 +489,9 @@ def main():
     if d3:
         log(f"3d: {d3['ms_per_step']:.3f} ms/step, walk {d3['walk_ms']:.3f} ms ({d3['walk_tflops']:.1f} TFLOP/s), "
             f"errors {d3['force_error_vs_fp64']}, cpu {d3.get('cpu_baseline', {}).get('value')}")
+    s3 = None if (a.no_sph or a.no_3d) else sph3d(a, 0.0 if a.no_cpu else min(a.cpu_seconds, 8.0))
+    if s3:
+        log(f"sph3d: {s3['ms_per_step']:.3f} ms/step, cpu {s3.get('cpu_baseline', {}).get('value')}")
     bat = None if a.no_batch else batched(a, a.accuracy)
     if bat:
         log(f"batched: {bat['batched_solves_per_s']:.1f} solves/s vs plain {bat['plain_solves_per_s']:.1f} "
@@ -482,6 +537,7 @@ def main():
         "cpu_baseline": cpu,
         "sph_impact": sph,
         "three_d": d3,
+        "sph3d_impact": s3,
         "batched": bat,
         "served": srv,
     }

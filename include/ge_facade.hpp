@@ -31,10 +31,13 @@
 
 namespace ge_gpu {
 
+#ifndef GE_GPU_ERROR_DEFINED
+#define GE_GPU_ERROR_DEFINED
 struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
+#endif
 
 // Host-side structure of arrays of the ParticlePhysics fields on the path
 // (GE/include/Particles/particle.h:5-73); empty vectors mean "not given".

@@ -1,5 +1,5 @@
 /*
- * ge_oracle3d.c -- CPU restatement of Galaxy Engine's 3D gravity step.
+ * ge_oracle3d.c -- CPU restatement of Galaxy Engine's 3D physics step (gravity and SPH3D).
  *
  * TEST INFRASTRUCTURE ONLY (the checker of the HIP 3D path); the product
  * library never links it.  Built into libge_oracle.so with the flags of

@@ -121,10 +121,10 @@ def test_config_generators_shapes():
     assert all(2000 <= r[1] <= 500000 and 10 <= r[2] <= 200 for r in reqs)
 
 
-def _build_facade_program(tmp_path):
-    exe = tmp_path / "facade_frame"
+def _build_facade_program(tmp_path, name="facade_frame"):
+    exe = tmp_path / name
     subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-                    os.path.join(ROOT, "tests", "cpp", "facade_frame.cpp"),
+                    os.path.join(ROOT, "tests", "cpp", name + ".cpp"),
                     "-L", os.path.join(ROOT, "galaxy-engine_amd"), "-lge_physics",
                     "-Wl,-rpath," + os.path.join(ROOT, "galaxy-engine_amd"), "-o", str(exe)], check=True)
     return exe
@@ -135,6 +135,7 @@ def test_cpp_facade_compiles_and_links(tmp_path):
     # device the program must fail cleanly with GE_E_NODEVICE, not crash
     exe = _build_facade_program(tmp_path)
     assert exe.exists()
+    assert _build_facade_program(tmp_path, "facade_frame3d").exists()
 
 
 HEADER3 = os.path.join(ROOT, "include", "ge_physics3d.h")
