@@ -31,6 +31,18 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(galaxy_step.lib.EXPORTS)
 
 
+SCENE_HEADER = os.path.join(ROOT, "include", "ge_scene.h")
+
+
+def test_library_exports_scene_header():
+    from galaxy_step import scene
+    lib = scene.load_scene_library()
+    names = declared_functions(SCENE_HEADER)
+    assert set(names) == set(scene.SCENE_EXPORTS)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
 def test_library_exports_serve_header():
     from galaxy_step import serve
     lib = serve._lib()
