@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--no-sph", action="store_true")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--only-3d", action="store_true", help="run only the 3D leg (profiling) and print its JSON")
+    ap.add_argument("--only-sph", action="store_true", help="run only the 2D SPH impact leg (profiling)")
+    ap.add_argument("--only-sph3d", action="store_true", help="run only the 3D SPH impact leg (profiling)")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=120)
@@ -469,6 +471,12 @@ def main():
 
     if a.only_3d:
         print(json.dumps({"three_d": three_d(a, 0.0 if a.no_cpu else min(a.cpu_seconds, 10.0))}), flush=True)
+        return
+    if a.only_sph:
+        print(json.dumps({"sph_impact": sph_impact(a)}), flush=True)
+        return
+    if a.only_sph3d:
+        print(json.dumps({"sph3d_impact": sph3d(a, 0.0 if a.no_cpu else min(a.cpu_seconds, 8.0))}), flush=True)
         return
     st, prm = configs.config1(n=a.n, theta=a.theta)
     n = len(st["x"])
