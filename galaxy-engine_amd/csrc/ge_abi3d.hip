@@ -66,6 +66,10 @@ void free_all(ge3_ctx *c) {
         dfree(g->pk0); dfree(g->pk1); dfree(g->pk2);
     }
     dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
+    for (int k = 0; k < 3; ++k) {
+        dfree(b.nbr_idx[k]);
+        dfree(b.nbr_cnt[k]);
+    }
 }
 
 // SPH3D state: 15 fields per state buffer, two slot grids, pair-force sums
@@ -93,6 +97,13 @@ int alloc_sph(ge3_ctx *c) {
     GE3_CHECK(c, dalloc(b.sphf, cap));
     GE3_CHECK(c, dalloc(b.sph_pp, cap));
     GE3_CHECK(c, dalloc(b.sph_pd, cap));
+    // about 34 kept candidates per particle at the SPH3D test densities; a
+    // denser query falls back to rescanning its candidates
+    b.nbr_cap = 96;
+    for (int k = 0; k < 3; ++k) {
+        GE3_CHECK(c, dalloc(b.nbr_idx[k], (size_t)cap * b.nbr_cap));
+        GE3_CHECK(c, dalloc(b.nbr_cnt[k], cap));
+    }
     return GE_OK;
 }
 

@@ -95,6 +95,12 @@ struct Buffers3 {
     float4 *sph_pp = nullptr;      // grid-A order: (ppx, ppy, ppz, restDens)
     float2 *sph_pd = nullptr;      // grid-A order: (press, predDens)
     uint32_t slot_bits = 0;        // hash slots = 1 << slot_bits
+    // neighbour lists of the SPH passes (viscosity at pos, density and own
+    // pressure visits at predPos, reverse pressure visits): nbr_cap entries
+    // per particle, column-major, and a count per particle
+    uint32_t *nbr_idx[3] = {nullptr, nullptr, nullptr};
+    uint32_t *nbr_cnt[3] = {nullptr, nullptr, nullptr};
+    uint32_t nbr_cap = 0;
 };
 
 }  // namespace ge

@@ -139,20 +139,26 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ x, 
     }
 }
 
-// Grid frame from the extremes: the occupied cell range.
-__global__ void k_frame(const float4 *__restrict__ partials, uint32_t nparts, const float *__restrict__ ext_in,
-                        Search S, uint32_t cells_cap, int32_t *__restrict__ frame, uint32_t *__restrict__ counters) {
-    if (threadIdx.x != 0) return;
+// Grid frame from the extremes: the occupied cell range.  One wave: the
+// lanes stride the per-workgroup partials, then a wave reduction.
+__global__ __launch_bounds__(64) void k_frame(const float4 *__restrict__ partials, uint32_t nparts,
+                                              const float *__restrict__ ext_in, Search S, uint32_t cells_cap,
+                                              int32_t *__restrict__ frame, uint32_t *__restrict__ counters) {
     float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
     if (ext_in) {
         mnx = ext_in[0]; mny = ext_in[1]; mxx = ext_in[2]; mxy = ext_in[3];
     } else {
-        for (uint32_t k = 0; k < nparts; ++k) {
+        for (uint32_t k = threadIdx.x; k < nparts; k += 64) {
             const float4 v = partials[k];
             mnx = glm_min(mnx, v.x); mny = glm_min(mny, v.y);
             mxx = glm_max(mxx, v.z); mxy = glm_max(mxy, v.w);
         }
+        for (int o = 32; o > 0; o >>= 1) {
+            mnx = glm_min(mnx, __shfl_xor(mnx, o, 64)); mny = glm_min(mny, __shfl_xor(mny, o, 64));
+            mxx = glm_max(mxx, __shfl_xor(mxx, o, 64)); mxy = glm_max(mxy, __shfl_xor(mxy, o, 64));
+        }
     }
+    if (threadIdx.x != 0) return;
     if (!(mnx <= mxx) || !(mny <= mxy)) {  // no particle in the grid
         frame[0] = 0; frame[1] = 0; frame[2] = 0; frame[3] = 0; frame[4] = 1;
         return;

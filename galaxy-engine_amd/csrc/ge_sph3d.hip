@@ -141,9 +141,106 @@ __global__ __launch_bounds__(kBlock) void k3_pack_a(State3 s, uint32_t n, const 
     pk2[k] = make_float2(s.cohesion[i], s.dens[i]);
 }
 
+// Neighbour lists: a scan kernel walks a query's candidates once and keeps
+// the positions that pass the pass's distance test (about 15 % of the 27
+// cells' candidates lie within h), so the pair kernels that follow loop over
+// dense lists and a wave's lanes do the expensive pair work together.
+// Column-major: entry t of sorted position k at idx[t * n + k].  A query
+// with more than `cap` kept candidates scans its candidates again instead;
+// both paths visit the same positions in the same (reference) order.
+struct NbrList {
+    uint32_t *idx;
+    uint32_t *cnt;
+    uint32_t cap;
+};
+
+template <class Keep>
+__device__ __forceinline__ void store_kept(const NbrList &L, uint32_t k, uint32_t n, const Grid3View &G, int cx,
+                                           int cy, int cz, uint32_t mask, Keep &&keep) {
+    uint32_t m = 0;
+    for_candidates(G, cx, cy, cz, mask, [&](uint32_t kk, uint32_t fj) {
+        if (!keep(kk, fj)) return;
+        if (m < L.cap) L.idx[(size_t)m * n + k] = kk;
+        ++m;
+    });
+    L.cnt[k] = m;
+}
+
+template <class Keep, class Body>
+__device__ __forceinline__ void for_kept(const NbrList &L, uint32_t k, uint32_t n, const Grid3View &G, int cx, int cy,
+                                         int cz, uint32_t mask, Keep &&keep, Body &&body) {
+    const uint32_t m = L.cnt[k];
+    if (m <= L.cap) {
+        for (uint32_t t = 0; t < m; ++t) body(L.idx[(size_t)t * n + k]);
+    } else {
+        for_candidates(G, cx, cy, cz, mask, [&](uint32_t kk, uint32_t fj) {
+            if (keep(kk, fj)) body(kk);
+        });
+    }
+}
+
+// the viscosity pass's test (SPH3D.cpp:108-126): another visiting particle
+// within h of pos
+struct ViscKeep {
+    const Grid3View &A;
+    uint32_t k;
+    float4 o0;
+    float h2;
+    __device__ __forceinline__ bool operator()(uint32_t kk, uint32_t fj) const {
+#pragma clang fp contract(off)
+        if (kk == k || !visitor3(fj)) return false;
+        const float4 p0 = A.pk0[kk];
+        const float dx = p0.x - o0.x, dy = p0.y - o0.y, dz = p0.z - o0.z;
+        return dx * dx + dy * dy + dz * dz < h2;
+    }
+};
+
+// the density pass's test (SPH3D.cpp:208-222): a visiting particle whose
+// predicted position is within h of the query's (self included)
+struct PredKeep {
+    const float4 *pp;
+    float4 me;
+    float h;
+    __device__ __forceinline__ bool operator()(uint32_t kk, uint32_t fj) const {
+#pragma clang fp contract(off)
+        if (!visitor3(fj)) return false;
+        const float4 pj = pp[kk];
+        const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+        return dx * dx + dy * dy + dz * dz < h * h;
+    }
+};
+
+// the reverse pressure visits (SPH3D.cpp:244-256 from the visitor's side):
+// another visitor whose predicted position is in [1e-5, h) of i's
+struct RevKeep {
+    const Grid3View &B;
+    uint32_t i;
+    float4 me;
+    float h;
+    __device__ __forceinline__ bool operator()(uint32_t kb, uint32_t) const {
+#pragma clang fp contract(off)
+        if (B.ord[kb] == i) return false;
+        const float4 pj = B.pk0[kb];
+        const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
+        const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+        return !(rr < 1e-5f || rr >= h);
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void k3_visc_scan(uint32_t n, Grid3View A, Sph3Const K, NbrList L) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const int4 ci = A.cell[k];
+    if (!visitor3((uint32_t)ci.w)) {
+        L.cnt[k] = 0;
+        return;
+    }
+    store_kept(L, k, n, A, ci.x, ci.y, ci.z, K.mask, ViscKeep{A, k, A.pk0[k], K.h2});
+}
+
 // SPH3D::computeViscCohesionForces (SPH3D.cpp:98-170) in gather form, then
 // the predicted position (SPH3D.cpp:190-198).  pp[k] = (ppx, ppy, ppz, restDens).
-__global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3View A, Sph3Const K,
+__global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3View A, Sph3Const K, NbrList L,
                                                   float4 *__restrict__ sphf, float4 *__restrict__ pp) {
 #pragma clang fp contract(off)
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
@@ -162,12 +259,11 @@ __global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3Vie
     const float mI = o0.w * K.unit;
     const float di = o2.y < 0.001f ? 0.001f : o2.y;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_candidates(A, ci.x, ci.y, ci.z, K.mask, [&](uint32_t kk, uint32_t fj) {
-        if (kk == k || !visitor3(fj)) return;
+    for_kept(L, k, n, A, ci.x, ci.y, ci.z, K.mask, ViscKeep{A, k, o0, K.h2}, [&](uint32_t kk) {
+        const uint32_t fj = (uint32_t)A.cell[kk].w;
         const float4 p0 = A.pk0[kk];
         const float dx = p0.x - o0.x, dy = p0.y - o0.y, dz = p0.z - o0.z;
         const float rsq = dx * dx + dy * dy + dz * dz;
-        if (rsq >= K.h2) return;
         const float r = sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
         const float lapw = r >= K.h ? 0.0f : K.lap * (K.h - r);  // smoothingKernelLaplacian
         const float wc = coh_kernel3(r, K.h);
@@ -207,9 +303,22 @@ __global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3Vie
     pp[k] = make_float4(px, py, pz, s.rest_dens[i]);
 }
 
+__global__ __launch_bounds__(kBlock) void k3_pred_scan(uint32_t n, Grid3View A, const float4 *__restrict__ pp,
+                                                       Sph3Const K, NbrList L) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    if (!visitor3((uint32_t)A.cell[k].w)) {
+        L.cnt[k] = 0;
+        return;
+    }
+    const float4 me = pp[k];
+    store_kept(L, k, n, A, cell_of(me.x, K.cell), cell_of(me.y, K.cell), cell_of(me.z, K.cell), K.mask,
+               PredKeep{pp, me, K.h});
+}
+
 // predicted density and pressure (SPH3D.cpp:200-236); pd[k] = (press, predDens)
 __global__ __launch_bounds__(kBlock) void k3_density(State3 s, uint32_t n, Grid3View A, const float4 *__restrict__ pp,
-                                                     Sph3Const K, float2 *__restrict__ pd) {
+                                                     Sph3Const K, NbrList L, float2 *__restrict__ pd) {
 #pragma clang fp contract(off)
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= n) return;
@@ -222,19 +331,16 @@ __global__ __launch_bounds__(kBlock) void k3_density(State3 s, uint32_t n, Grid3
     }
     const float4 me = pp[k];
     float dens = 0.0f;
-    for_candidates(A, cell_of(me.x, K.cell), cell_of(me.y, K.cell), cell_of(me.z, K.cell), K.mask,
-                   [&](uint32_t kk, uint32_t fj) {
-                       if (!visitor3(fj)) return;
-                       const float4 pj = pp[kk];
-                       const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
-                       const float rrsq = dx * dx + dy * dy + dz * dz;
-                       if (rrsq >= K.h * K.h) return;
-                       const float rr = sqrtf(rrsq);
-                       const float mJ = A.pk0[kk].w * K.unit2;
-                       const float rho0 = 0.5f * (me.w + pj.w);
-                       const float w = rr >= K.h ? 0.0f : (K.h - rr) * (K.h - rr) / K.vol;  // smoothingKernel
-                       dens = dens + mJ * w / rho0;
-                   });
+    for_kept(L, k, n, A, cell_of(me.x, K.cell), cell_of(me.y, K.cell), cell_of(me.z, K.cell), K.mask,
+             PredKeep{pp, me, K.h}, [&](uint32_t kk) {
+                 const float4 pj = pp[kk];
+                 const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+                 const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+                 const float mJ = A.pk0[kk].w * K.unit2;
+                 const float rho0 = 0.5f * (me.w + pj.w);
+                 const float w = rr >= K.h ? 0.0f : (K.h - rr) * (K.h - rr) / K.vol;  // smoothingKernel
+                 dens = dens + mJ * w / rho0;
+             });
     float pt = K.delta * (dens - me.w);
     if (pt < 0.0f) pt = 0.0f;
     const float pr = 0.0f + pt * s.stiff[i] * K.stiff_mult;
@@ -260,6 +366,18 @@ __global__ __launch_bounds__(kBlock) void k3_pack_b(State3 s, uint32_t n, const 
     pk1[k] = make_float4(s.pred_dens[i], s.sph_mass[i], 0.0f, 0.0f);
 }
 
+__global__ __launch_bounds__(kBlock) void k3_rev_scan(uint32_t n, Grid3View A, const float4 *__restrict__ pp,
+                                                      Grid3View B, Sph3Const K, NbrList L) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const int4 ci = A.cell[k];
+    if (!visitor3((uint32_t)ci.w)) {
+        L.cnt[k] = 0;
+        return;
+    }
+    store_kept(L, k, n, B, ci.x, ci.y, ci.z, K.mask, RevKeep{B, A.ord[k], pp[k], K.h});
+}
+
 // pressure force of one visit (SPH3D.cpp:244-281); d = predPos_visitor - predPos_other
 __device__ __forceinline__ void pressure_pair3(float dx, float dy, float dz, float rr, float mi, float mj, float pi_,
                                                float pj_, float di, float dj, const Sph3Const &K, float &fx, float &fy,
@@ -283,41 +401,40 @@ __device__ __forceinline__ void pressure_pair3(float dx, float dy, float dz, flo
 // then the integrateEnd3D kick (physics3D.cpp:1158-1168) when asked
 __global__ __launch_bounds__(kBlock) void k3_pressure(State3 s, uint32_t n, Grid3View A, const float4 *__restrict__ pp,
                                                       const float2 *__restrict__ pd, Grid3View B, Sph3Const K,
-                                                      const float4 *__restrict__ sphf) {
+                                                      NbrList Lp, NbrList Lr, const float4 *__restrict__ sphf) {
 #pragma clang fp contract(off)
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= n) return;
     const uint32_t i = A.ord[k];
-    const uint32_t fi = (uint32_t)A.cell[k].w;
+    const int4 ci = A.cell[k];
+    const uint32_t fi = (uint32_t)ci.w;
     const float4 f0 = sphf[i];
     float fx = f0.x, fy = f0.y, fz = f0.z;
     const float smi = A.pk0[k].w;
     if (visitor3(fi)) {
         const float4 me = pp[k];
         const float2 mp = pd[k];
-        // i's visits: candidates by pos cell around cell(predPos_i)
-        for_candidates(A, cell_of(me.x, K.cell), cell_of(me.y, K.cell), cell_of(me.z, K.cell), K.mask,
-                       [&](uint32_t kk, uint32_t fj) {
-                           if (kk == k || !visitor3(fj)) return;
-                           const float4 pj = pp[kk];
-                           const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
-                           const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
-                           if (rr < 1e-5f || rr >= K.h) return;
-                           const float2 q = pd[kk];
-                           float px, py, pz;
-                           pressure_pair3(dx, dy, dz, rr, smi, A.pk0[kk].w, mp.x, q.x, mp.y, q.y, K, px, py, pz);
-                           fx = fx + px;
-                           fy = fy + py;
-                           fz = fz + pz;
-                       });
+        // i's visits: the density pass's list (candidates by pos cell around
+        // cell(predPos_i) within h), without i itself and closer than 1e-5
+        for_kept(Lp, k, n, A, cell_of(me.x, K.cell), cell_of(me.y, K.cell), cell_of(me.z, K.cell), K.mask,
+                 PredKeep{pp, me, K.h}, [&](uint32_t kk) {
+                     if (kk == k) return;
+                     const float4 pj = pp[kk];
+                     const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+                     const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+                     if (rr < 1e-5f || rr >= K.h) return;
+                     const float2 q = pd[kk];
+                     float px, py, pz;
+                     pressure_pair3(dx, dy, dz, rr, smi, A.pk0[kk].w, mp.x, q.x, mp.y, q.y, K, px, py, pz);
+                     fx = fx + px;
+                     fy = fy + py;
+                     fz = fz + pz;
+                 });
         // visits of i by others: predPos cells around cell(pos_i)
-        const int4 ci = A.cell[k];
-        for_candidates(B, ci.x, ci.y, ci.z, K.mask, [&](uint32_t kb, uint32_t) {
-            if (B.ord[kb] == i) return;
+        for_kept(Lr, k, n, B, ci.x, ci.y, ci.z, K.mask, RevKeep{B, i, me, K.h}, [&](uint32_t kb) {
             const float4 pj = B.pk0[kb];
             const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
             const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
-            if (rr < 1e-5f || rr >= K.h) return;
             const float4 q = B.pk1[kb];
             float px, py, pz;
             pressure_pair3(dx, dy, dz, rr, q.y, smi, pj.w, mp.x, q.x, mp.y, K, px, py, pz);
@@ -453,16 +570,23 @@ void sph3_pcisph(ge3_ctx *c, const ge3_step_params *p, float end_kick_dt) {
     const uint32_t n = c->n;
     const Sph3Const K = make_const3(c, p, end_kick_dt * 0.5f);
     const uint32_t nb = blocks_for(n, kBlock);
-    hipLaunchKernelGGL(k3_visc, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view3(A), K, c->b.sphf, c->b.sph_pp);
+    const NbrList Lv{c->b.nbr_idx[0], c->b.nbr_cnt[0], c->b.nbr_cap};
+    const NbrList Lp{c->b.nbr_idx[1], c->b.nbr_cnt[1], c->b.nbr_cap};
+    const NbrList Lr{c->b.nbr_idx[2], c->b.nbr_cnt[2], c->b.nbr_cap};
+    hipLaunchKernelGGL(k3_visc_scan, dim3(nb), dim3(kBlock), 0, c->stream, n, view3(A), K, Lv);
+    hipLaunchKernelGGL(k3_visc, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view3(A), K, Lv, c->b.sphf, c->b.sph_pp);
     // grid B over the predicted positions of visiting particles
     hipLaunchKernelGGL(k3_slot_keys, dim3(nb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.ppz, s.flags, n, 1,
                        p->sph_cell, K.mask, B.keys);
     build_slots(c, B);
-    hipLaunchKernelGGL(k3_density, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view3(A), c->b.sph_pp, K, c->b.sph_pd);
+    hipLaunchKernelGGL(k3_pred_scan, dim3(nb), dim3(kBlock), 0, c->stream, n, view3(A), c->b.sph_pp, K, Lp);
+    hipLaunchKernelGGL(k3_density, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view3(A), c->b.sph_pp, K, Lp,
+                       c->b.sph_pd);
     hipLaunchKernelGGL(k3_pack_b, dim3(nb), dim3(kBlock), 0, c->stream, s, n, B.keys, p->sph_cell, B.ord, B.cell,
                        B.pk0, B.pk1);
+    hipLaunchKernelGGL(k3_rev_scan, dim3(nb), dim3(kBlock), 0, c->stream, n, view3(A), c->b.sph_pp, view3(B), K, Lr);
     hipLaunchKernelGGL(k3_pressure, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view3(A), c->b.sph_pp, c->b.sph_pd,
-                       view3(B), K, c->b.sphf);
+                       view3(B), K, Lp, Lr, c->b.sphf);
 }
 
 void sph3_ground(ge3_ctx *c, const ge3_step_params *p) {
