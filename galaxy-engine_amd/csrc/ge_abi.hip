@@ -1,6 +1,7 @@
 // ge_abi.hip -- the extern "C" entry points of include/ge_physics.h:
 // context lifecycle, device buffers, state transfer and the frame driver that
 // orders the stage kernels as updateScene() does (globalLogic.cpp:1155-1372).
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -42,6 +43,13 @@ int alloc_state(ge_ctx *c, State &s, uint32_t cap, bool sph) {
                         &s.press_tmp, &s.pred_dens, &s.ppx, &s.ppy, &s.pfx, &s.pfy};
         for (float **f : gs) GE_HIP_CHECK(c, dalloc(*f, cap));
     }
+    if (c->render) {
+        GE_HIP_CHECK(c, dalloc(s.color, cap));
+        GE_HIP_CHECK(c, dalloc(s.pcolor, cap));
+        GE_HIP_CHECK(c, dalloc(s.scolor, cap));
+        GE_HIP_CHECK(c, dalloc(s.size, cap));
+        GE_HIP_CHECK(c, dalloc(s.nbrs, cap));
+    }
     return GE_OK;
 }
 
@@ -52,6 +60,7 @@ void free_state(State &s) {
     for (float **f : fs) dfree(*f);
     dfree(s.flags);
     dfree(s.id);
+    dfree(s.color); dfree(s.pcolor); dfree(s.scolor); dfree(s.size); dfree(s.nbrs);
 }
 
 void free_all(ge_ctx *c) {
@@ -240,6 +249,8 @@ int put_fields(ge_ctx *c, State &s, uint32_t at, uint32_t cnt, const ge_particle
             (r = put(s.ppy, p->ppy, 0.f)) || (r = put(s.pfx, p->pfx, 0.f)) || (r = put(s.pfy, p->pfy, 0.f)))
             return r;
     }
+    // ParticleRendering() defaults: white, size 1, no neighbours (particle.h:101-108)
+    if (c->render) launch_render_fill(c->stream, s.color, s.pcolor, s.scolor, s.size, s.nbrs, at, cnt, 0xFFFFFFFFu, 1.0f);
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }
@@ -499,6 +510,7 @@ int ge_create(ge_ctx **out, const ge_config *cfg) {
     c->device = cfg->device;
     c->cap = cfg->capacity;
     c->sph = cfg->enable_sph != 0;
+    c->render = cfg->enable_render != 0;
     c->max_sys = cfg->max_systems ? cfg->max_systems : 1;
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -932,6 +944,46 @@ int ge_neighbor_counts(ge_ctx *c, const ge_step_params *p, uint32_t *counts) {
     (void)hipSetDevice(c->device);
     launch_integrate_start_bbox(c, 0.0f, 0, 0.0f, 0);  // the grid frame uses the current extremes
     return sph_neighbor_counts(c, p, counts);
+}
+
+int ge_spawn_galaxy(ge_ctx *c, const ge_galaxy_params *g, uint64_t seed, const float *uniforms, uint32_t *first) {
+    if (!c || !g) return GE_E_INVALID;
+    if (c->n && c->nsys > 1) return fail(c, GE_E_UNSUPPORTED, "the spawner appends to single-system contexts");
+    if (!(g->scale_length > 0.0f && g->radius_core_dm > 0.0f && g->outer_radius_dm >= 0.0f &&
+          std::isfinite(g->outer_radius) && std::isfinite(g->center_x) && std::isfinite(g->center_y)))
+        return fail(c, GE_E_INVALID, "scale_length and radius_core_dm must be > 0, radii and centre finite");
+    const uint64_t add = (uint64_t)g->visible + g->dark;
+    if ((uint64_t)c->n + add > c->cap) return fail(c, GE_E_CAPACITY, "particle count above capacity");
+    if (uniforms) {
+        for (uint64_t k = 0; k < 3ull * g->visible + 5ull * g->dark; ++k)
+            if (!(uniforms[k] >= 0.0f && uniforms[k] <= 1.0f)) return fail(c, GE_E_INVALID, "uniforms must lie in [0, 1]");
+    }
+    const uint32_t at = c->n;
+    if (first) *first = at;
+    if (add == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    float *du = nullptr;
+    if (uniforms) {
+        const size_t bytes = sizeof(float) * (3ull * g->visible + 5ull * g->dark);
+        GE_HIP_CHECK(c, hipMalloc(&du, bytes));
+        if (hipMemcpy(du, uniforms, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(du);
+            return fail(c, GE_E_HIP, "uniform copy failed");
+        }
+    }
+    const uint32_t total = (uint32_t)(at + add);
+    if (int r = set_systems(c, 1, &total)) {
+        if (du) (void)hipFree(du);
+        return fail(c, r, "HIP error");
+    }
+    launch_spawn_galaxy(c, c->b.st[c->b.cur], at, g, seed, du);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (du) (void)hipFree(du);
+    if (e != hipSuccess) return fail(c, GE_E_HIP, "galaxy spawn failed");
+    c->layout_version += 1;
+    c->have_tree = false;
+    c->have_grid = false;
+    return GE_OK;
 }
 
 int ge_get_neighbors(ge_ctx *c, int32_t which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total) {

@@ -47,6 +47,7 @@ void free_all(ge3_ctx *c) {
         for (float **f : fs) dfree(*f);
         dfree(s.flags);
         dfree(s.id);
+        dfree(s.color); dfree(s.pcolor); dfree(s.scolor); dfree(s.size); dfree(s.nbrs);
     }
     dfree(b.pxyzm); dfree(b.keys); dfree(b.sk_a); dfree(b.sk_b); dfree(b.sv_a); dfree(b.sv_b); dfree(b.perm);
     dfree(b.hist); dfree(b.partials); dfree(b.bb); dfree(b.counters); dfree(b.inter);
@@ -66,7 +67,7 @@ void free_all(ge3_ctx *c) {
         dfree(g->pk0); dfree(g->pk1); dfree(g->pk2);
     }
     dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
         dfree(b.nbr_idx[k]);
         dfree(b.nbr_cnt[k]);
     }
@@ -100,7 +101,7 @@ int alloc_sph(ge3_ctx *c) {
     // about 34 kept candidates per particle at the SPH3D test densities; a
     // denser query falls back to rescanning its candidates
     b.nbr_cap = 96;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
         GE3_CHECK(c, dalloc(b.nbr_idx[k], (size_t)cap * b.nbr_cap));
         GE3_CHECK(c, dalloc(b.nbr_cnt[k], cap));
     }
@@ -115,6 +116,13 @@ int alloc_all(ge3_ctx *c) {
         for (float **f : fs) GE3_CHECK(c, dalloc(*f, cap));
         GE3_CHECK(c, dalloc(s.flags, cap));
         GE3_CHECK(c, dalloc(s.id, cap));
+        if (c->render) {
+            GE3_CHECK(c, dalloc(s.color, cap));
+            GE3_CHECK(c, dalloc(s.pcolor, cap));
+            GE3_CHECK(c, dalloc(s.scolor, cap));
+            GE3_CHECK(c, dalloc(s.size, cap));
+            GE3_CHECK(c, dalloc(s.nbrs, cap));
+        }
     }
     GE3_CHECK(c, dalloc(b.pxyzm, cap));
     GE3_CHECK(c, dalloc(b.keys, cap));
@@ -277,6 +285,8 @@ int put_fields(ge3_ctx *c, uint32_t n, const ge3_particles *p) {
         }
         GE3_CHECK(c, hipStreamSynchronize(c->stream));
     }
+    // ParticleRendering3D() defaults: white, size 1, no neighbours
+    if (c->render) launch_render_fill(c->stream, s.color, s.pcolor, s.scolor, s.size, s.nbrs, 0, n, 0xFFFFFFFFu, 1.0f);
     GE3_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }
@@ -334,6 +344,7 @@ int ge3_create(ge3_ctx **out, const ge3_config *cfg) {
         return GE_E_HIP;
     }
     c->sph = cfg->enable_sph != 0;
+    c->render = cfg->enable_render != 0;
     int ra = alloc_all(c);
     if (ra == GE_OK && c->sph) ra = alloc_sph(c);
     if (int r = ra) {

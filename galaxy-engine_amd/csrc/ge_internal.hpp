@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/ge_physics.h"
+#include "../../include/ge_render.h"
 
 namespace ge {
 
@@ -74,6 +75,10 @@ struct State {
     float *sph_mass = nullptr, *rest_dens = nullptr, *stiff = nullptr, *visc = nullptr;
     float *cohesion = nullptr, *dens = nullptr, *press = nullptr, *press_tmp = nullptr;
     float *pred_dens = nullptr, *ppx = nullptr, *ppy = nullptr, *pfx = nullptr, *pfy = nullptr;
+    // render fields (allocated with enable_render): RGBA8 colours, size, neighbour count
+    uint32_t *color = nullptr, *pcolor = nullptr, *scolor = nullptr;
+    float *size = nullptr;
+    int32_t *nbrs = nullptr;
 };
 
 // Node record read by the traversal: 32 bytes, one scalar load per visit.
@@ -168,6 +173,7 @@ struct ge_ctx {
     int sys_bits = 0;              // high bits: system index
     std::vector<uint32_t> host_off;  // system offsets (host copy)
     bool sph = false;
+    bool render = false;           // render fields allocated
     bool debug = true;             // keep node geometry for ge_get_nodes
     bool have_tree = false;
     bool have_grid = false;
@@ -230,6 +236,12 @@ int sph_new_grid(ge_ctx *c, const ge_step_params *p);
 int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
 int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total);
 int sph_neighbor_counts(ge_ctx *c, const ge_step_params *p, uint32_t *counts);
+int sph_count_into(ge_ctx *c, int search, float cell, float radius, int32_t *out);
+// render.hip
+void launch_render_fill(hipStream_t st, uint32_t *color, uint32_t *pcolor, uint32_t *scolor, float *size,
+                        int32_t *nbrs, uint32_t at, uint32_t cnt, uint32_t rgba, float sz);
+void launch_spawn_galaxy(ge_ctx *c, const State &s, uint32_t at, const ge_galaxy_params *g, uint64_t seed,
+                         const float *d_uniforms);
 
 inline uint32_t blocks_for(uint64_t n, int per_block) {
     uint64_t b = (n + per_block - 1) / per_block;

@@ -27,6 +27,10 @@ struct State3 {
     float *cohesion = nullptr, *dens = nullptr, *press = nullptr, *press_tmp = nullptr;
     float *pred_dens = nullptr, *ppx = nullptr, *ppy = nullptr, *ppz = nullptr;
     float *pfx = nullptr, *pfy = nullptr, *pfz = nullptr;
+    // render fields (allocated with enable_render): RGBA8 colours, size, neighbour count
+    uint32_t *color = nullptr, *pcolor = nullptr, *scolor = nullptr;
+    float *size = nullptr;
+    int32_t *nbrs = nullptr;
 };
 
 // SPH3D neighbour grid: particles sorted by a hash slot of their cell,
@@ -95,11 +99,11 @@ struct Buffers3 {
     float4 *sph_pp = nullptr;      // grid-A order: (ppx, ppy, ppz, restDens)
     float2 *sph_pd = nullptr;      // grid-A order: (press, predDens)
     uint32_t slot_bits = 0;        // hash slots = 1 << slot_bits
-    // neighbour lists of the SPH passes (viscosity at pos, density and own
-    // pressure visits at predPos, reverse pressure visits): nbr_cap entries
-    // per particle, column-major, and a count per particle
-    uint32_t *nbr_idx[3] = {nullptr, nullptr, nullptr};
-    uint32_t *nbr_cnt[3] = {nullptr, nullptr, nullptr};
+    // neighbour lists of the SPH passes (density and own pressure visits at
+    // predPos, reverse pressure visits): nbr_cap entries per particle,
+    // column-major, and a count per particle
+    uint32_t *nbr_idx[2] = {nullptr, nullptr};
+    uint32_t *nbr_cnt[2] = {nullptr, nullptr};
     uint32_t nbr_cap = 0;
 };
 
@@ -114,6 +118,7 @@ struct ge3_ctx {
     uint32_t leaf_cap = 0;
     bool have_tree = false;
     bool sph = false;              // SPH3D state allocated
+    bool render = false;           // render fields allocated
     bool have_grid = false;        // grid A matches the current positions
     float grid_cell = 3.0f;        // cell size grid A was built with
     ge::Buffers3 b;
@@ -141,6 +146,7 @@ void sph3_pcisph(ge3_ctx *c, const ge3_step_params *p, float end_kick_dt);
 void sph3_ground(ge3_ctx *c, const ge3_step_params *p);
 int sph3_neighbors(ge3_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total);
 int sph3_neighbor_counts(ge3_ctx *c, uint32_t *counts);
+void sph3_count_into(ge3_ctx *c, float cell, int32_t *out);
 
 constexpr uint32_t kBBoxChunk3 = 8192;  // particles per bbox workgroup
 

@@ -437,17 +437,18 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
 }
 
 // ---- candidate lists for the parity hook; counts (cnt) and lists (out),
-// optionally skipping particles whose flags match skip_mask (count 0)
+// optionally skipping particles whose flags match skip_mask (count 0, or
+// left as it was with keep_skipped)
 __global__ __launch_bounds__(kBlock) void k_cand_count(const float *__restrict__ qx, const float *__restrict__ qy,
                                                        uint32_t n, Search S, const int32_t *__restrict__ frame,
                                                        const uint32_t *__restrict__ first,
                                                        const float4 *__restrict__ pk0,
                                                        const uint32_t *__restrict__ flags, uint32_t skip_mask,
-                                                       uint32_t *__restrict__ cnt) {
+                                                       int keep_skipped, uint32_t *__restrict__ cnt) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     if (flags && (flags[i] & skip_mask)) {
-        cnt[i] = 0;
+        if (!keep_skipped) cnt[i] = 0;
         return;
     }
     const float x = qx[i], y = qy[i];
@@ -598,7 +599,7 @@ int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap
     const Search S = make_search(c->sph_search, c->sph_cell, c->sph_radius);
     GE_HIP_CHECK(c, hipMalloc(&dcnt, sizeof(uint32_t) * (n + 1)));
     hipLaunchKernelGGL(k_cand_count, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, qx, qy, n, S, g.frame,
-                       g.first, g.pk0, nullptr, 0u, dcnt);
+                       g.first, g.pk0, nullptr, 0u, 0, dcnt);
     std::vector<uint32_t> hc(n);
     GE_HIP_CHECK(c, hipMemcpyAsync(hc.data(), dcnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -639,9 +640,29 @@ int sph_neighbor_counts(ge_ctx *c, const ge_step_params *p, uint32_t *counts) {
     const Search S = make_search(1, q.sph_cell, q.sph_search_radius);
     uint32_t *dcnt = reinterpret_cast<uint32_t *>(c->b.sphf);  // SPH force scratch: 2 floats per particle
     hipLaunchKernelGGL(k_cand_count, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, n, S, g.frame,
-                       g.first, g.pk0, s.flags, GE_FLAG_PINNED | GE_FLAG_DRAWN | GE_FLAG_DARK, dcnt);
+                       g.first, g.pk0, s.flags, GE_FLAG_PINNED | GE_FLAG_DRAWN | GE_FLAG_DARK, 0, dcnt);
     GE_HIP_CHECK(c, hipMemcpyAsync(counts, dcnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return GE_OK;
+}
+
+// neighborAmount of NeighborSearchV2AVX2 (search 0, neighborSearch.h:779-796)
+// or NeighborSearchV2 (search 1, neighborSearch.h:585-598) into out[] (device),
+// leaving the counts of pinned, drawn and dark-matter particles as they were.
+// Rebuilds grid A with these search settings; a later PCISPH needs a new grid.
+int sph_count_into(ge_ctx *c, int search, float cell, float radius, int32_t *out) {
+    ge_step_params q{};
+    q.sph_search = search;
+    q.sph_cell = cell;
+    q.sph_search_radius = radius;
+    if (int r = sph_new_grid(c, &q)) return r;
+    const Grid &g = c->b.grid_pos;
+    const State &s = c->b.st[c->b.cur];
+    const Search S = make_search(search, cell, radius);
+    hipLaunchKernelGGL(k_cand_count, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, c->n, S,
+                       g.frame, g.first, g.pk0, s.flags, GE_FLAG_PINNED | GE_FLAG_DRAWN | GE_FLAG_DARK, 1,
+                       reinterpret_cast<uint32_t *>(out));
+    c->have_grid = false;
     return GE_OK;
 }
 

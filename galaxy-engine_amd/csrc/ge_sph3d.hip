@@ -141,10 +141,12 @@ __global__ __launch_bounds__(kBlock) void k3_pack_a(State3 s, uint32_t n, const 
     pk2[k] = make_float2(s.cohesion[i], s.dens[i]);
 }
 
-// Neighbour lists: a scan kernel walks a query's candidates once and keeps
-// the positions that pass the pass's distance test (about 15 % of the 27
-// cells' candidates lie within h), so the pair kernels that follow loop over
-// dense lists and a wave's lanes do the expensive pair work together.
+// Neighbour lists of the density and pressure passes: a scan kernel walks a
+// query's candidates once and keeps the positions that pass the pass's
+// distance test (about 15 % of the 27 cells' candidates lie within h), so the
+// pair kernels that follow loop over dense lists and a wave's lanes do the
+// pair work together.  (The viscosity pass reads each candidate set once and
+// gains nothing from a list.)
 // Column-major: entry t of sorted position k at idx[t * n + k].  A query
 // with more than `cap` kept candidates scans its candidates again instead;
 // both paths visit the same positions in the same (reference) order.
@@ -179,22 +181,6 @@ __device__ __forceinline__ void for_kept(const NbrList &L, uint32_t k, uint32_t 
     }
 }
 
-// the viscosity pass's test (SPH3D.cpp:108-126): another visiting particle
-// within h of pos
-struct ViscKeep {
-    const Grid3View &A;
-    uint32_t k;
-    float4 o0;
-    float h2;
-    __device__ __forceinline__ bool operator()(uint32_t kk, uint32_t fj) const {
-#pragma clang fp contract(off)
-        if (kk == k || !visitor3(fj)) return false;
-        const float4 p0 = A.pk0[kk];
-        const float dx = p0.x - o0.x, dy = p0.y - o0.y, dz = p0.z - o0.z;
-        return dx * dx + dy * dy + dz * dz < h2;
-    }
-};
-
 // the density pass's test (SPH3D.cpp:208-222): a visiting particle whose
 // predicted position is within h of the query's (self included)
 struct PredKeep {
@@ -227,20 +213,9 @@ struct RevKeep {
     }
 };
 
-__global__ __launch_bounds__(kBlock) void k3_visc_scan(uint32_t n, Grid3View A, Sph3Const K, NbrList L) {
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    const int4 ci = A.cell[k];
-    if (!visitor3((uint32_t)ci.w)) {
-        L.cnt[k] = 0;
-        return;
-    }
-    store_kept(L, k, n, A, ci.x, ci.y, ci.z, K.mask, ViscKeep{A, k, A.pk0[k], K.h2});
-}
-
 // SPH3D::computeViscCohesionForces (SPH3D.cpp:98-170) in gather form, then
 // the predicted position (SPH3D.cpp:190-198).  pp[k] = (ppx, ppy, ppz, restDens).
-__global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3View A, Sph3Const K, NbrList L,
+__global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3View A, Sph3Const K,
                                                   float4 *__restrict__ sphf, float4 *__restrict__ pp) {
 #pragma clang fp contract(off)
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
@@ -259,11 +234,12 @@ __global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3Vie
     const float mI = o0.w * K.unit;
     const float di = o2.y < 0.001f ? 0.001f : o2.y;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_kept(L, k, n, A, ci.x, ci.y, ci.z, K.mask, ViscKeep{A, k, o0, K.h2}, [&](uint32_t kk) {
-        const uint32_t fj = (uint32_t)A.cell[kk].w;
+    for_candidates(A, ci.x, ci.y, ci.z, K.mask, [&](uint32_t kk, uint32_t fj) {
+        if (kk == k || !visitor3(fj)) return;
         const float4 p0 = A.pk0[kk];
         const float dx = p0.x - o0.x, dy = p0.y - o0.y, dz = p0.z - o0.z;
         const float rsq = dx * dx + dy * dy + dz * dz;
+        if (rsq >= K.h2) return;
         const float r = sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
         const float lapw = r >= K.h ? 0.0f : K.lap * (K.h - r);  // smoothingKernelLaplacian
         const float wc = coh_kernel3(r, K.h);
@@ -489,11 +465,12 @@ __global__ __launch_bounds__(kBlock) void k3_ground(State3 s, uint32_t n, float 
 __global__ __launch_bounds__(kBlock) void k3_cand_count(const float *__restrict__ qx, const float *__restrict__ qy,
                                                         const float *__restrict__ qz, uint32_t n, Grid3View A,
                                                         float cell, uint32_t mask, const uint32_t *__restrict__ flags,
-                                                        uint32_t skip_mask, uint32_t *__restrict__ cnt) {
+                                                        uint32_t skip_mask, int keep_skipped,
+                                                        uint32_t *__restrict__ cnt) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     if (flags && (flags[i] & skip_mask)) {
-        cnt[i] = 0;
+        if (!keep_skipped) cnt[i] = 0;
         return;
     }
     uint32_t c = 0;
@@ -570,11 +547,9 @@ void sph3_pcisph(ge3_ctx *c, const ge3_step_params *p, float end_kick_dt) {
     const uint32_t n = c->n;
     const Sph3Const K = make_const3(c, p, end_kick_dt * 0.5f);
     const uint32_t nb = blocks_for(n, kBlock);
-    const NbrList Lv{c->b.nbr_idx[0], c->b.nbr_cnt[0], c->b.nbr_cap};
-    const NbrList Lp{c->b.nbr_idx[1], c->b.nbr_cnt[1], c->b.nbr_cap};
-    const NbrList Lr{c->b.nbr_idx[2], c->b.nbr_cnt[2], c->b.nbr_cap};
-    hipLaunchKernelGGL(k3_visc_scan, dim3(nb), dim3(kBlock), 0, c->stream, n, view3(A), K, Lv);
-    hipLaunchKernelGGL(k3_visc, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view3(A), K, Lv, c->b.sphf, c->b.sph_pp);
+    const NbrList Lp{c->b.nbr_idx[0], c->b.nbr_cnt[0], c->b.nbr_cap};
+    const NbrList Lr{c->b.nbr_idx[1], c->b.nbr_cnt[1], c->b.nbr_cap};
+    hipLaunchKernelGGL(k3_visc, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view3(A), K, c->b.sphf, c->b.sph_pp);
     // grid B over the predicted positions of visiting particles
     hipLaunchKernelGGL(k3_slot_keys, dim3(nb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.ppz, s.flags, n, 1,
                        p->sph_cell, K.mask, B.keys);
@@ -603,7 +578,7 @@ int sph3_neighbors(ge3_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t c
     const uint32_t mask = (1u << c->b.slot_bits) - 1u;
     uint32_t *dcnt = reinterpret_cast<uint32_t *>(c->b.sphf);  // pair-force scratch: 4 floats per particle
     hipLaunchKernelGGL(k3_cand_count, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, qx, qy, qz, n, view3(g),
-                       c->grid_cell, mask, nullptr, 0u, dcnt);
+                       c->grid_cell, mask, nullptr, 0u, 0, dcnt);
     std::vector<uint32_t> hc(n);
     if (hipMemcpyAsync(hc.data(), dcnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -653,13 +628,29 @@ int sph3_neighbor_counts(ge3_ctx *c, uint32_t *counts) {
     const uint32_t mask = (1u << c->b.slot_bits) - 1u;
     uint32_t *dcnt = reinterpret_cast<uint32_t *>(c->b.sphf);
     hipLaunchKernelGGL(k3_cand_count, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, s.z, n,
-                       view3(g), c->grid_cell, mask, s.flags, GE_FLAG_PINNED | GE_FLAG_DRAWN | GE_FLAG_DARK, dcnt);
+                       view3(g), c->grid_cell, mask, s.flags, GE_FLAG_PINNED | GE_FLAG_DRAWN | GE_FLAG_DARK, 0, dcnt);
     if (hipMemcpyAsync(counts, dcnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
         c->err = "neighbour count copy failed";
         return GE_E_HIP;
     }
     return GE_OK;
+}
+
+// NeighborSearch3DV2AVX2::neighborAmount (neighborSearch.h:1175-1193) into
+// out[] (device), leaving the counts of pinned, drawn and dark-matter
+// particles as they were.  Rebuilds grid A with this cell size.
+void sph3_count_into(ge3_ctx *c, float cell, int32_t *out) {
+    ge3_step_params q{};
+    q.sph_cell = cell;
+    sph3_new_grid(c, &q);
+    const Grid3 &g = c->b.grid_pos;
+    const State3 &s = c->b.st[c->b.cur];
+    const uint32_t mask = (1u << c->b.slot_bits) - 1u;
+    hipLaunchKernelGGL(k3_cand_count, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, s.x, s.y, s.z, c->n,
+                       view3(g), cell, mask, s.flags, GE_FLAG_PINNED | GE_FLAG_DRAWN | GE_FLAG_DARK, 1,
+                       reinterpret_cast<uint32_t *>(out));
+    c->have_grid = false;
 }
 
 }  // namespace ge

@@ -197,6 +197,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(State src, State dst, uint32_
         dst.ppx[i] = src.ppx[j]; dst.ppy[i] = src.ppy[j];
         dst.pfx[i] = src.pfx[j]; dst.pfy[i] = src.pfy[j];
     }
+    if (src.color) {  // render fields
+        dst.color[i] = src.color[j]; dst.pcolor[i] = src.pcolor[j]; dst.scolor[i] = src.scolor[j];
+        dst.size[i] = src.size[j]; dst.nbrs[i] = src.nbrs[j];
+    }
 }
 
 __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
@@ -512,6 +516,10 @@ __global__ __launch_bounds__(kBlock) void k_compact(State src, State dst, uint32
         dst.pred_dens[i] = src.pred_dens[j];
         dst.ppx[i] = src.ppx[j]; dst.ppy[i] = src.ppy[j];
         dst.pfx[i] = src.pfx[j]; dst.pfy[i] = src.pfy[j];
+    }
+    if (src.color) {  // render fields
+        dst.color[i] = src.color[j]; dst.pcolor[i] = src.pcolor[j]; dst.scolor[i] = src.scolor[j];
+        dst.size[i] = src.size[j]; dst.nbrs[i] = src.nbrs[j];
     }
 }
 

@@ -204,6 +204,10 @@ __global__ __launch_bounds__(kBlock) void k3_gather(State3 src, State3 dst, uint
         dst.ppx[i] = src.ppx[j]; dst.ppy[i] = src.ppy[j]; dst.ppz[i] = src.ppz[j];
         dst.pfx[i] = src.pfx[j]; dst.pfy[i] = src.pfy[j]; dst.pfz[i] = src.pfz[j];
     }
+    if (src.color) {  // render fields
+        dst.color[i] = src.color[j]; dst.pcolor[i] = src.pcolor[j]; dst.scolor[i] = src.scolor[j];
+        dst.size[i] = src.size[j]; dst.nbrs[i] = src.nbrs[j];
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k3_pack(State3 s, uint32_t n, float4 *__restrict__ pxyzm) {

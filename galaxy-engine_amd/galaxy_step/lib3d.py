@@ -32,7 +32,7 @@ GRAVITY3_FIELDS = FIELDS3_F + FIELDS3_U
 
 class Ge3Config(C.Structure):
     _fields_ = [("capacity", C.c_uint32), ("device", C.c_int32), ("enable_sph", C.c_int32),
-                ("reserved", C.c_uint32 * 5)]
+                ("enable_render", C.c_int32), ("reserved", C.c_uint32 * 4)]
 
 
 class Ge3Particles(C.Structure):
@@ -133,14 +133,16 @@ def _uptr(a):
 class Context3D:
     """One 3D device context (ge3_ctx)."""
 
-    def __init__(self, capacity, device=0, sph=False):
+    def __init__(self, capacity, device=0, sph=False, render=False):
         self.lib = load_library3()
         cfg = Ge3Config()
         self.lib.ge3_default_config(C.byref(cfg))
         cfg.capacity = int(capacity)
         cfg.device = int(device)
         cfg.enable_sph = 1 if sph else 0
+        cfg.enable_render = 1 if render else 0
         self.sph = bool(sph)
+        self.render = bool(render)
         h = C.c_void_p()
         rc = self.lib.ge3_create(C.byref(h), C.byref(cfg))
         if rc != GE_OK:

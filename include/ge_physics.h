@@ -42,6 +42,8 @@ extern "C" {
 #define GE_FLAG_SPH 2u
 #define GE_FLAG_DRAWN 4u
 #define GE_FLAG_DARK 8u
+#define GE_FLAG_UNIQUE_COLOR 16u /* uniqueColor (read by the colour consumers, ge_render.h) */
+#define GE_FLAG_SOLID 32u        /* isSolid (read by the size consumers, ge_render.h) */
 
 typedef struct ge_ctx ge_ctx;
 
@@ -50,7 +52,8 @@ typedef struct {
     int32_t device;       /* HIP device ordinal */
     int32_t enable_sph;   /* allocate SPH state (needed for ge_step with sph=1) */
     uint32_t max_systems; /* independent systems one context can hold (>= 1) */
-    uint32_t reserved[4];
+    int32_t enable_render; /* keep the render fields of ge_render.h on the device */
+    uint32_t reserved[3];
 } ge_config;
 
 /* Structure-of-arrays view of the ParticlePhysics fields on this path

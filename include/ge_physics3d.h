@@ -30,7 +30,8 @@ typedef struct {
     uint32_t capacity;  /* maximum particle count */
     int32_t device;     /* HIP device ordinal */
     int32_t enable_sph; /* allocate SPH3D state (needed for sph = 1) */
-    uint32_t reserved[5];
+    int32_t enable_render; /* keep the render fields of ge_render.h on the device */
+    uint32_t reserved[4];
 } ge3_config;
 
 /* Structure-of-arrays view of ParticlePhysics3D on this path

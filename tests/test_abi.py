@@ -209,3 +209,55 @@ def test_3d_create_reports_status_without_crashing():
     assert rc in (0, -7)
     if rc == 0:
         assert lib.ge3_destroy(h) == 0
+
+
+RENDER_HEADER = os.path.join(ROOT, "include", "ge_render.h")
+
+
+def test_library_exports_render_header():
+    from galaxy_step import render
+    lib = render.load_render_library()
+    text = open(RENDER_HEADER).read()
+    names = sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(ge3?_[a-z_0-9]+)\s*\(", text, re.M)))
+    assert set(names) == set(render.RENDER_EXPORTS)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_render_struct_layout_matches_header(tmp_path):
+    from galaxy_step import render as R
+    from galaxy_step import lib as L
+    from galaxy_step import lib3d as L3
+    src = tmp_path / "szr.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ge_render.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", '
+                   'sizeof(ge_render_fields), sizeof(ge_visual_params), sizeof(ge_ply_vertex), '
+                   'sizeof(ge_galaxy_params), offsetof(ge_visual_params, p_color), '
+                   'offsetof(ge_visual_params, max_color_acc), offsetof(ge_ply_vertex, radius), '
+                   'offsetof(ge_galaxy_params, first_id), offsetof(ge_config, enable_render), '
+                   'offsetof(ge3_config, enable_render), sizeof(ge_config) + sizeof(ge3_config));return 0;}\n')
+    exe = tmp_path / "szr"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    want = [C.sizeof(R.RenderFields), C.sizeof(R.VisualParams), C.sizeof(R.PlyVertex), C.sizeof(R.GalaxyParams),
+            R.VisualParams.p_color.offset, R.VisualParams.max_color_acc.offset, R.PlyVertex.radius.offset,
+            R.GalaxyParams.first_id.offset, L.GeConfig.enable_render.offset, L3.Ge3Config.enable_render.offset,
+            C.sizeof(L.GeConfig) + C.sizeof(L3.Ge3Config)]
+    assert [int(v) for v in out] == want
+    assert R.PLY_DTYPE.itemsize == C.sizeof(R.PlyVertex) == 20
+
+
+def test_render_defaults_are_reference_defaults():
+    from galaxy_step import render as R
+    v = R.default_visual_params()
+    # densitySize.h:7-12; particleColorVisuals.h:9, 24-25, 33-36; neighborSearch.h:441, 604
+    assert (v.min_size, v.max_size, v.size_acc, v.size_max_neighbors) == (pytest.approx(0.17), pytest.approx(0.62),
+                                                                         22.0, 200)
+    assert v.density_color == 1 and v.density_size == 0 and v.force_size == 0 and v.search == 0
+    assert v.p_color == R.rgba(0, 40, 68, 100) and v.s_color == R.rgba(155, 80, 40, 75)
+    assert v.color_max_neighbors == 200 and v.max_color_acc == 40.0 and v.cell == 3.0 and v.radius == 3.5
+    g = R.default_galaxy_params()
+    # particlesSpawning.h:28-32, particlesSpawning.cpp:141, 165, 200, 222; parameters.h:297
+    assert (g.visible, g.dark, g.outer_radius, g.scale_length) == (40000, 12000, 200.0, 90.0)
+    assert (g.outer_radius_dm, g.radius_core_dm, g.mass_scatter) == (2000.0, 3.5, pytest.approx(0.75))
+    assert g.mass_visible == np.float32(8.5e9) and g.mass_dark == np.float32(1.416e11)
