@@ -579,6 +579,18 @@ void ge_default_visual_params(ge_visual_params *v) {
     v->max_color_acc = 40.0f;
 }
 
+int ge_render_enabled(const ge_ctx *c, int32_t *on) {
+    if (!c || !on) return GE_E_INVALID;
+    *on = c->render ? 1 : 0;
+    return GE_OK;
+}
+
+int ge3_render_enabled(const ge3_ctx *c, int32_t *on) {
+    if (!c || !on) return GE_E_INVALID;
+    *on = c->render ? 1 : 0;
+    return GE_OK;
+}
+
 int ge_render_upload(ge_ctx *c, const ge_render_fields *f) {
     if (!c || !f) return GE_E_INVALID;
     if (!c->render) return fail(c->err, GE_E_INVALID, "context created without enable_render");

@@ -20,6 +20,7 @@
 #include <utility>
 #include <vector>
 
+#include "../../include/ge_render.h"
 #include "../../include/ge_scene.h"
 
 namespace {
@@ -207,6 +208,9 @@ struct Soa {
     std::vector<uint32_t> flags, id;
     std::vector<float> sph_mass, rest_dens, stiff, visc, cohesion, dens, press, press_tmp, pred_dens;
     std::vector<float> ppx, ppy, ppz, pfx, pfy, pfz;
+    std::vector<uint32_t> color, pcolor, scolor;  // render fields (RGBA8)
+    std::vector<float> size;
+    std::vector<int32_t> nbrs;
     std::vector<std::vector<float> *> floats() {
         return {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az, &pvx, &pvy, &pvz, &m, &sph_mass, &rest_dens, &stiff,
                 &visc, &cohesion, &dens, &press, &press_tmp, &pred_dens, &ppx, &ppy, &ppz, &pfx, &pfy, &pfz};
@@ -215,6 +219,20 @@ struct Soa {
         for (auto *v : floats()) v->assign(n, 0.0f);
         flags.assign(n, 0u);
         id.assign(n, 0u);
+        color.assign(n, 0u);
+        pcolor.assign(n, 0u);
+        scolor.assign(n, 0u);
+        size.assign(n, 0.0f);
+        nbrs.assign(n, 0);
+    }
+    ge_render_fields render() {
+        ge_render_fields f{};
+        f.color = color.data();
+        f.pcolor = pcolor.data();
+        f.scolor = scolor.data();
+        f.size = size.data();
+        f.neighbors = nbrs.data();
+        return f;
     }
 };
 
@@ -260,7 +278,14 @@ void decode(const ge_scene *s, Soa &o) {
         if (rd<uint8_t>(r, L.off[F_ISSPH])) f |= GE_FLAG_SPH;
         if (rd<uint8_t>(r, L.off[F_DRAWN])) f |= GE_FLAG_DRAWN;
         if (rd<uint8_t>(r, L.off[F_DARK])) f |= GE_FLAG_DARK;
+        if (rd<uint8_t>(r, L.off[F_UNIQUECOLOR])) f |= GE_FLAG_UNIQUE_COLOR;
+        if (rd<uint8_t>(r, L.off[F_SOLID])) f |= GE_FLAG_SOLID;
         o.flags[i] = f;
+        o.color[i] = rd<uint32_t>(r, L.off[F_COLOR]);  // Color {r, g, b, a}: r in the low byte
+        o.pcolor[i] = rd<uint32_t>(r, L.off[F_PCOLOR]);
+        o.scolor[i] = rd<uint32_t>(r, L.off[F_SCOLOR]);
+        o.size[i] = rd<float>(r, L.off[F_SIZE]);
+        o.nbrs[i] = rd<int32_t>(r, L.off[F_NEIGHBORS]);
     }
 }
 
@@ -276,7 +301,7 @@ bool untouched(float v) {
 
 // structure of arrays (device order) -> records, carrying every field the
 // device does not hold over from the record of the same id
-void encode(ge_scene *s, const Soa &o, uint32_t n) {
+void encode(ge_scene *s, const Soa &o, uint32_t n, bool render) {
     const Layout L = s->layout();
     const int d = L.dim;
     std::unordered_map<uint32_t, uint32_t> by_id;
@@ -320,6 +345,15 @@ void encode(ge_scene *s, const Soa &o, uint32_t n) {
         wr<uint8_t>(r, L.off[F_ISSPH], (o.flags[k] & GE_FLAG_SPH) ? 1 : 0);
         wr<uint8_t>(r, L.off[F_DRAWN], (o.flags[k] & GE_FLAG_DRAWN) ? 1 : 0);
         wr<uint8_t>(r, L.off[F_DARK], (o.flags[k] & GE_FLAG_DARK) ? 1 : 0);
+        wr<uint8_t>(r, L.off[F_UNIQUECOLOR], (o.flags[k] & GE_FLAG_UNIQUE_COLOR) ? 1 : 0);
+        wr<uint8_t>(r, L.off[F_SOLID], (o.flags[k] & GE_FLAG_SOLID) ? 1 : 0);
+        if (render) {
+            wr<uint32_t>(r, L.off[F_COLOR], o.color[k]);
+            wr<uint32_t>(r, L.off[F_PCOLOR], o.pcolor[k]);
+            wr<uint32_t>(r, L.off[F_SCOLOR], o.scolor[k]);
+            wr<float>(r, L.off[F_SIZE], o.size[k]);
+            wr<int32_t>(r, L.off[F_NEIGHBORS], o.nbrs[k]);
+        }
         if (o.id[k] > max_id) max_id = o.id[k];
     }
     s->recs.swap(out);
@@ -581,7 +615,12 @@ int ge_scene_upload(const ge_scene *s, ge_ctx *ctx) {
     Soa o;
     decode(s, o);
     ge_particles p = view_of<ge_particles>(o);
-    const int r = ge_upload(ctx, s->count, &p);
+    int r = ge_upload(ctx, s->count, &p);
+    int32_t render = 0;
+    if (r == GE_OK && (r = ge_render_enabled(ctx, &render)) == GE_OK && render) {
+        ge_render_fields f = o.render();
+        r = ge_render_upload(ctx, &f);
+    }
     if (r != GE_OK) s->err = ge_last_error(ctx);
     return r;
 }
@@ -597,7 +636,12 @@ int ge_scene_upload3(const ge_scene *s, ge3_ctx *ctx) {
     ge3_particles p = view_of<ge3_particles>(o);
     p.z = o.z.data(); p.vz = o.vz.data(); p.az = o.az.data(); p.pvz = o.pvz.data();
     p.ppz = o.ppz.data(); p.pfz = o.pfz.data();
-    const int r = ge3_upload(ctx, s->count, &p);
+    int r = ge3_upload(ctx, s->count, &p);
+    int32_t render = 0;
+    if (r == GE_OK && (r = ge3_render_enabled(ctx, &render)) == GE_OK && render) {
+        ge_render_fields f = o.render();
+        r = ge3_render_upload(ctx, &f);
+    }
     if (r != GE_OK) s->err = ge3_last_error(ctx);
     return r;
 }
@@ -612,7 +656,13 @@ int ge_scene_download(ge_scene *s, ge_ctx *ctx) {
     prefill_untouched(o);
     ge_particles p = view_of<ge_particles>(o);
     if (int r = ge_download(ctx, &p)) return fail(s, r, ge_last_error(ctx));
-    encode(s, o, n);
+    int32_t render = 0;
+    if (int r = ge_render_enabled(ctx, &render)) return fail(s, r, ge_last_error(ctx));
+    if (render) {
+        ge_render_fields f = o.render();
+        if (int r = ge_render_download(ctx, &f)) return fail(s, r, ge_last_error(ctx));
+    }
+    encode(s, o, n, render != 0);
     return GE_OK;
 }
 
@@ -628,7 +678,13 @@ int ge_scene_download3(ge_scene *s, ge3_ctx *ctx) {
     p.z = o.z.data(); p.vz = o.vz.data(); p.az = o.az.data(); p.pvz = o.pvz.data();
     p.ppz = o.ppz.data(); p.pfz = o.pfz.data();
     if (int r = ge3_download(ctx, &p)) return fail(s, r, ge3_last_error(ctx));
-    encode(s, o, n);
+    int32_t render = 0;
+    if (int r = ge3_render_enabled(ctx, &render)) return fail(s, r, ge3_last_error(ctx));
+    if (render) {
+        ge_render_fields f = o.render();
+        if (int r = ge3_render_download(ctx, &f)) return fail(s, r, ge3_last_error(ctx));
+    }
+    encode(s, o, n, render != 0);
     return GE_OK;
 }
 

@@ -9,7 +9,7 @@ from .lib import GE_OK, GeError, load_library
 from .lib3d import load_library3
 
 RENDER_EXPORTS = [
-    "ge_render_upload", "ge_render_download", "ge3_render_upload", "ge3_render_download",
+    "ge_render_enabled", "ge3_render_enabled", "ge_render_upload", "ge_render_download", "ge3_render_upload", "ge3_render_download",
     "ge_default_visual_params", "ge_visuals", "ge3_visuals", "ge_ply_vertices", "ge3_ply_vertices",
     "ge_ply_write", "ge3_ply_write", "ge_ply_format", "ge_default_galaxy_params", "ge_spawn_galaxy",
 ]
@@ -68,6 +68,8 @@ def load_render_library():
     P = C.POINTER
     vp = C.c_void_p
     sig = {
+        "ge_render_enabled": (C.c_int, [vp, P(C.c_int32)]),
+        "ge3_render_enabled": (C.c_int, [vp, P(C.c_int32)]),
         "ge_render_upload": (C.c_int, [vp, P(RenderFields)]),
         "ge_render_download": (C.c_int, [vp, P(RenderFields)]),
         "ge3_render_upload": (C.c_int, [vp, P(RenderFields)]),

@@ -35,6 +35,9 @@ typedef struct {
     int32_t *neighbors;                /* neighbors */
 } ge_render_fields;
 
+/* *on = 1 when the context keeps render fields (created with enable_render) */
+int ge_render_enabled(const ge_ctx *ctx, int32_t *on);
+int ge3_render_enabled(const ge3_ctx *ctx, int32_t *on);
 /* the fields of the current particles, in the context's current order */
 int ge_render_upload(ge_ctx *ctx, const ge_render_fields *f);
 int ge_render_download(ge_ctx *ctx, const ge_render_fields *f);

@@ -27,6 +27,7 @@
 
 #include "ge_physics.h"
 #include "ge_physics3d.h"
+#include "ge_render.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -73,14 +74,16 @@ int ge_scene_set_params(ge_scene *s, const ge_step_params *p);
 int ge_scene_set_params3(ge_scene *s, const ge3_step_params *p);
 
 /* particles into a context (ge_upload / ge3_upload): pos, vel, acc, prevVel,
- * mass, id, the SPH fields when the context has them, and the flags
- * isPinned, isSPH, isBeingDrawn, isDarkMatter (GE_FLAG_*) */
+ * mass, id, the SPH fields when the context has them, the flags isPinned,
+ * isSPH, isBeingDrawn, isDarkMatter, uniqueColor, isSolid (GE_FLAG_*), and
+ * color, pColor, sColor, size, neighbors when it keeps render fields
+ * (ge_render.h) */
 int ge_scene_upload(const ge_scene *s, ge_ctx *ctx);
 int ge_scene_upload3(const ge_scene *s, ge3_ctx *ctx);
 /* the context's particles back into records, in the device order: the
- * device fields from the context, every other field from the record of the
- * same id (a new id gets the reference's defaults for a spawned particle,
- * particle.h:35-63, 100-130) */
+ * device fields from the context (the render fields too when it keeps them),
+ * every other field from the record of the same id (a new id gets the
+ * reference's defaults for a spawned particle, particle.h:35-63, 100-130) */
 int ge_scene_download(ge_scene *s, ge_ctx *ctx);
 int ge_scene_download3(ge_scene *s, ge3_ctx *ctx);
 /* SaveSystem::saveSystem's file layout (saveSystem.cpp:285-611) */

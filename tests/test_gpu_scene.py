@@ -150,3 +150,58 @@ def test_scene_3d_sph_round_trip_through_the_device(tmp_path):
     o = np.argsort(ref["id"])
     ext = float(np.ptp(ref["x"]))
     assert np.abs(g["pos"][:, 0] - ref["x"][o]).max() < 1e-4 * ext
+
+
+def _rgba(c):
+    """raylib Color {r, g, b, a} bytes as the packed RGBA8 word (r in the low byte)."""
+    return np.ascontiguousarray(c).view("<u4").reshape(-1)
+
+
+def test_scene_render_fields_through_the_device(tmp_path):
+    # a context with render fields takes color, pColor, sColor, size,
+    # neighbors, uniqueColor and isSolid from the file, the consumers update
+    # them on the device, and the file gets the device values back
+    from galaxy_step import render as GR
+    import render as R
+    st, _ = configs.config0(seed=13, n=5000)
+    recs = _records_2d(st, 8)
+    recs["isDarkMatter"][::11] = 1
+    yaml = "Theta: 0.800000012\n---\nSoftening: 1\n---\nis3DMode: false\n"
+    src = tmp_path / "Save_r.bin"
+    _file(src, recs, yaml)
+    v = GR.default_visual_params(density_size=1, force_color=0, count_neighbors=1)
+    with Scene.load(src) as sc:
+        n = sc.info()["count"]
+        with Context(n, sph=True, render=True) as c:
+            sc.upload(c)
+            before = GR.render_download(c)
+            fl = c.download(["flags", "x", "y", "ax", "ay"])
+            GR.visuals(c, v)
+            after = GR.render_download(c)
+            sc.download(c)
+        out = tmp_path / "Save_r_out.bin"
+        sc.save(out)
+    # upload: the record fields, colours as Color {r, g, b, a} bytes
+    np.testing.assert_array_equal(before["color"], _rgba(recs["color"]))
+    np.testing.assert_array_equal(before["pcolor"], _rgba(recs["pColor"]))
+    np.testing.assert_array_equal(before["size"], recs["size"])
+    np.testing.assert_array_equal(before["neighbors"], recs["neighbors"])
+    uniq = (fl["flags"] & R.FLAG_UNIQUE) != 0
+    np.testing.assert_array_equal(uniq, recs["uniqueColor"] == 1)
+    np.testing.assert_array_equal((fl["flags"] & R.FLAG_SOLID) != 0, recs["isSolid"] == 1)
+    # the consumers on the device, against the oracle
+    off, _ = O.sph_candidates(fl["x"], fl["y"], fl["x"], fl["y"], 3.0)
+    nb = R.neighbor_counts(off, fl["flags"], before["neighbors"])
+    col, pc, sc_, size = R.visuals(fl["flags"], fl["ax"], fl["ay"], None, before["color"], before["pcolor"],
+                                   before["scolor"], before["size"], nb, v.as_dict())
+    np.testing.assert_array_equal(after["color"], col)
+    np.testing.assert_array_equal(after["size"], size)
+    # and back into the file (same order: no step ran)
+    with Scene.load(out) as back:
+        got = np.frombuffer(back.records(), record_dtype(2))
+    np.testing.assert_array_equal(_rgba(got["color"]), after["color"])
+    np.testing.assert_array_equal(_rgba(got["sColor"]), after["scolor"])
+    np.testing.assert_array_equal(got["size"], after["size"])
+    np.testing.assert_array_equal(got["neighbors"], after["neighbors"])
+    np.testing.assert_array_equal(got["uniqueColor"], recs["uniqueColor"])
+    np.testing.assert_array_equal(got["isSolid"], recs["isSolid"])
