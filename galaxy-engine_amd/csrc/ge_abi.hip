@@ -43,6 +43,12 @@ int alloc_state(ge_ctx *c, State &s, uint32_t cap, bool sph) {
                         &s.press_tmp, &s.pred_dens, &s.ppx, &s.ppy, &s.pfx, &s.pfy};
         for (float **f : gs) GE_HIP_CHECK(c, dalloc(*f, cap));
     }
+    if (c->thermal) {
+        GE_HIP_CHECK(c, dalloc(s.temp, cap));
+        GE_HIP_CHECK(c, dalloc(s.ke, cap));
+        GE_HIP_CHECK(c, dalloc(s.prev_ke, cap));
+        GE_HIP_CHECK(c, dalloc(s.label, cap));
+    }
     if (c->render) {
         GE_HIP_CHECK(c, dalloc(s.color, cap));
         GE_HIP_CHECK(c, dalloc(s.pcolor, cap));
@@ -61,6 +67,7 @@ void free_state(State &s) {
     dfree(s.flags);
     dfree(s.id);
     dfree(s.color); dfree(s.pcolor); dfree(s.scolor); dfree(s.size); dfree(s.nbrs);
+    dfree(s.temp); dfree(s.ke); dfree(s.prev_ke); dfree(s.label);
 }
 
 void free_all(ge_ctx *c) {
@@ -79,6 +86,7 @@ void free_all(ge_ctx *c) {
     dfree(T.kind); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
     dfree(T.blist); dfree(T.blist_off); dfree(T.blist_cnt);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
+    dfree(b.node_temp); dfree(b.node_parent); dfree(b.node_arrive); dfree(b.materials);
     dfree(b.nodes); dfree(b.node_geom); dfree(b.node_mp); dfree(b.mp_cnt); dfree(b.mp_wave_list); dfree(b.mp_big_list); dfree(b.mp_piece_owner); dfree(b.mp_piece_sum); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
         dfree(g->keys); dfree(g->keys_alt); dfree(g->first); dfree(g->ord); dfree(g->frame);
@@ -136,6 +144,12 @@ int alloc_all(ge_ctx *c) {
     b.node_cap = 4u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(b.nodes, b.node_cap));
     GE_HIP_CHECK(c, dalloc(b.node_geom, b.node_cap));
+    if (c->thermal) {
+        GE_HIP_CHECK(c, dalloc(b.node_temp, b.node_cap));
+        GE_HIP_CHECK(c, dalloc(b.node_parent, b.node_cap));
+        GE_HIP_CHECK(c, dalloc(b.node_arrive, b.node_cap));
+        GE_HIP_CHECK(c, dalloc(b.materials, GE_MAX_MATERIALS));
+    }
     if (c->sph) {
         GE_HIP_CHECK(c, dalloc(b.sphf, 2ull * cap));
         GE_HIP_CHECK(c, dalloc(b.sph_pp, 4ull * cap));
@@ -251,6 +265,7 @@ int put_fields(ge_ctx *c, State &s, uint32_t at, uint32_t cnt, const ge_particle
     }
     // ParticleRendering() defaults: white, size 1, no neighbours (particle.h:101-108)
     if (c->render) launch_render_fill(c->stream, s.color, s.pcolor, s.scolor, s.size, s.nbrs, at, cnt, 0xFFFFFFFFu, 1.0f);
+    if (c->thermal) launch_thermal_fill(c, s, at, cnt);  // temp 288, no material
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }
@@ -290,7 +305,9 @@ int fail(ge_ctx *c, int code, const char *msg) {
 
 int check_params(ge_ctx *c, const ge_step_params *p) {
     if (!p) return fail(c, GE_E_INVALID, "params is NULL");
-    if (p->temperature) return fail(c, GE_E_UNSUPPORTED, "temperature transfer is not implemented");
+    if (p->temperature && !c->thermal) return fail(c, GE_E_INVALID, "context created without enable_temperature");
+    if (p->temperature && p->accuracy)
+        return fail(c, GE_E_UNSUPPORTED, "the heat transfer follows the reference walk (accuracy 0)");
     if ((p->periodic || p->sph_ground) && !(p->domain_w > 0.0f && p->domain_h > 0.0f))
         return fail(c, GE_E_INVALID, "periodic and ground modes need domain_w, domain_h > 0");
     if (p->sph && !c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
@@ -375,17 +392,20 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
         PhaseScope s(c, GE_PHASE_SPH_GRID);
         if (int r = sph_new_grid(c, p)) return r;
     }
-    const float end_kick = p->sph ? 0.0f : p->dt * 0.5f;  // fused integrateEnd when no SPH follows
+    const bool temp = p->temperature != 0;  // temperatureCalculation runs before integrateEnd
+    const float end_kick = (p->sph || temp) ? 0.0f : p->dt * 0.5f;  // fused integrateEnd when nothing follows
     const bool ground = p->sph && p->sph_ground;           // groundModeBoundary runs before integrateEnd
-    if (p->G != 0.0f) {
+    // the walk runs for gravity or for the heat transfer (globalLogic.cpp:1316)
+    if (p->G != 0.0f || temp) {
         if (use_direct(c, p) || !c->have_tree) {
             PhaseScope s(c, GE_PHASE_GRAVITY);
-            launch_direct(c, p, end_kick);
+            launch_direct(c, p, end_kick);  // naiveGravityAVX2 moves no heat
         } else {
             if (p->accuracy) {
                 PhaseScope s(c, GE_PHASE_MULTIPOLE);
                 launch_multipoles(c, p->theta);
             }
+            if (temp) launch_node_temps(c);
             PhaseScope s(c, GE_PHASE_GRAVITY);
             launch_bh(c, p, end_kick, count_inter);
         }
@@ -398,11 +418,13 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
     }
     if (p->sph) {
         PhaseScope s(c, GE_PHASE_SPH);
-        if (int r = sph_pcisph(c, p, ground ? 0.0f : p->dt)) return r;
-        if (ground) {
-            launch_sph_ground(c, p);
-            launch_integrate_end(c, p->dt);
-        }
+        if (int r = sph_pcisph(c, p, (ground || temp) ? 0.0f : p->dt)) return r;
+        if (ground) launch_sph_ground(c, p);
+    }
+    if (temp) launch_temperature(c, p);
+    if (ground || temp) {
+        PhaseScope s(c, GE_PHASE_INTEGRATE_END);
+        launch_integrate_end(c, p->dt);
     }
     return GE_OK;
 }
@@ -511,14 +533,27 @@ int ge_create(ge_ctx **out, const ge_config *cfg) {
     c->cap = cfg->capacity;
     c->sph = cfg->enable_sph != 0;
     c->render = cfg->enable_render != 0;
+    c->thermal = cfg->enable_temperature != 0;
+    if (c->thermal && !c->sph) {  // the materials write the SPH fields
+        delete c;
+        return GE_E_INVALID;
+    }
     c->max_sys = cfg->max_systems ? cfg->max_systems : 1;
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return GE_E_HIP;
     }
     t_alloc_bytes = 0;
-    const int ra = alloc_all(c);
+    int ra = alloc_all(c);
     c->device_bytes = t_alloc_bytes;
+    if (ra == GE_OK && c->thermal) {  // the reference's constants and materials until ge_set_thermal
+        ge_thermal_params tp;
+        ge_default_thermal_params(&tp);
+        ge_material mats[GE_MAX_MATERIALS];
+        uint32_t nm = 0;
+        ge_default_materials(mats, GE_MAX_MATERIALS, &nm);
+        ra = ge_set_thermal(c, &tp, mats, nm);
+    }
     if (int r = ra) {
         free_all(c);
         (void)hipStreamDestroy(c->stream);

@@ -10,6 +10,7 @@
 
 #include "../../include/ge_physics.h"
 #include "../../include/ge_render.h"
+#include "../../include/ge_thermal.h"
 
 namespace ge {
 
@@ -79,6 +80,9 @@ struct State {
     uint32_t *color = nullptr, *pcolor = nullptr, *scolor = nullptr;
     float *size = nullptr;
     int32_t *nbrs = nullptr;
+    // temperature fields (allocated with enable_temperature)
+    float *temp = nullptr, *ke = nullptr, *prev_ke = nullptr;
+    uint32_t *label = nullptr;
 };
 
 // Node record read by the traversal: 32 bytes, one scalar load per visit.
@@ -157,6 +161,12 @@ struct Buffers {
     float *sph_pp = nullptr;       // grid-A order (predX, predY, restDens, sphMass)
     float *sph_pd = nullptr;       // grid-A order (press, predDens)
     Grid grid_pos, grid_pred;
+    // temperature coupling (enable_temperature): Node::gridTemp per final
+    // node, the bottom-up pass's parent links and arrival counts, the
+    // material table
+    float *node_temp = nullptr;
+    uint32_t *node_parent = nullptr, *node_arrive = nullptr;
+    ge_material *materials = nullptr;
 };
 
 }  // namespace ge
@@ -174,6 +184,9 @@ struct ge_ctx {
     std::vector<uint32_t> host_off;  // system offsets (host copy)
     bool sph = false;
     bool render = false;           // render fields allocated
+    bool thermal = false;          // temperature fields allocated
+    ge_thermal_params thermal_p{};  // ge_set_thermal
+    uint32_t nmat = 0;             // materials in b.materials
     bool debug = true;             // keep node geometry for ge_get_nodes
     bool have_tree = false;
     bool have_grid = false;
@@ -237,6 +250,10 @@ int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
 int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total);
 int sph_neighbor_counts(ge_ctx *c, const ge_step_params *p, uint32_t *counts);
 int sph_count_into(ge_ctx *c, int search, float cell, float radius, int32_t *out);
+// thermal.hip
+void launch_thermal_fill(ge_ctx *c, const State &s, uint32_t at, uint32_t cnt);
+void launch_node_temps(ge_ctx *c);
+void launch_temperature(ge_ctx *c, const ge_step_params *p);
 // render.hip
 void launch_render_fill(hipStream_t st, uint32_t *color, uint32_t *pcolor, uint32_t *scolor, float *size,
                         int32_t *nbrs, uint32_t at, uint32_t cnt, uint32_t rgba, float sz);

@@ -201,6 +201,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(State src, State dst, uint32_
         dst.color[i] = src.color[j]; dst.pcolor[i] = src.pcolor[j]; dst.scolor[i] = src.scolor[j];
         dst.size[i] = src.size[j]; dst.nbrs[i] = src.nbrs[j];
     }
+    if (src.temp) {  // temperature fields
+        dst.temp[i] = src.temp[j]; dst.label[i] = src.label[j];
+        dst.ke[i] = src.ke[j]; dst.prev_ke[i] = src.prev_ke[j];
+    }
 }
 
 __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
@@ -237,7 +241,7 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // the node-start prefix of the tree build); each lane is active only inside
 // its own system's range, and the wave walks from its lanes' first root to
 // their last end.
-template <int BS, bool KAHAN, bool COUNT, int ORDER>
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
 __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                                            const NodeRec *__restrict__ nodes,
                                            const float4 *__restrict__ mp,
@@ -249,13 +253,16 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                                            float *__restrict__ ax, float *__restrict__ ay,
                                            float *__restrict__ vx, float *__restrict__ vy,
                                            const uint32_t *__restrict__ flags, float half_dt_end,
-                                           Domain dom, unsigned long long *__restrict__ inter) {
+                                           Domain dom, unsigned long long *__restrict__ inter,
+                                           const float *__restrict__ gtemp, float *__restrict__ temp,
+                                           float heat_k, float heat_dt) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * BS + threadIdx.x;
     const bool per = dom.w > 0.0f;
     const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
     const bool live = i < n;
     const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float ti = (TEMP && live) ? temp[i] : 0.0f;  // temp[i], moved by every accepted node
     // a tree that raised an error flag may hold unwritten nodes: do not walk it
     const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | (ORDER > 0 ? kErrMultipole : 0u));
     const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
@@ -373,6 +380,21 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                 ++cnt;
                 ++cnt_nodes;
             }
+            if (TEMP) {  // heat transfer toward the node's average (physics.cpp:261-276)
+#pragma clang fp contract(off)
+                const uint32_t cn = nd.end - nd.start;
+                if (cn > 0u) {
+                    const float avg = gtemp[idx] / (float)cn;
+                    const float diff = avg - ti;
+                    const float invd = 1.0f / sqrtf(dist_sq);
+                    float dist = 0.0f;
+                    if (dist_sq > 1e-16f) dist = 1.0f / invd;
+                    if (dist > 1e-8f) {
+                        const float heat = heat_k * diff / dist;
+                        ti = ti + heat * heat_dt;
+                    }
+                }
+            }
             skip_until = after;
         }
     const bool leaf = next == 0;
@@ -421,6 +443,7 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
         }
     }
     if (!live) return;
+    if (TEMP) temp[i] = ti;
     const float accx = KAHAN ? fx - cx : fx;
     const float accy = KAHAN ? fy - cy : fy;
     ax[i] = accx;
@@ -521,6 +544,10 @@ __global__ __launch_bounds__(kBlock) void k_compact(State src, State dst, uint32
         dst.color[i] = src.color[j]; dst.pcolor[i] = src.pcolor[j]; dst.scolor[i] = src.scolor[j];
         dst.size[i] = src.size[j]; dst.nbrs[i] = src.nbrs[j];
     }
+    if (src.temp) {  // temperature fields
+        dst.temp[i] = src.temp[j]; dst.label[i] = src.label[j];
+        dst.ke[i] = src.ke[j]; dst.prev_ke[i] = src.prev_ke[j];
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero2(float *__restrict__ a, float *__restrict__ b, uint32_t n) {
@@ -594,32 +621,34 @@ void launch_gather(ge_ctx *c) {
     c->b.cur = 1 - cur;
 }
 
-template <int BS, bool KAHAN, bool COUNT, int ORDER>
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
 void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const State &s = c->b.st[c->b.cur];
     const float theta_sq = p->theta * p->theta;
     const float soft_sq = p->softening * p->softening;
     const uint32_t nb = blocks_for(c->n, BS);
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
-    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym, c->b.nodes,
-                       c->b.node_mp, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n, p->G,
-                       theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p), inter);
+    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER, TEMP>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
+                       c->b.nodes, c->b.node_mp, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n,
+                       p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p),
+                       inter, c->b.node_temp, s.temp, c->thermal_p.heat_conductivity, p->dt);
 }
 
-template <int BS, int ORDER>
+template <int BS, int ORDER, bool TEMP>
 void launch_bh_o(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
     const bool k = p->compensated != 0;
-    if (k && count) launch_bh_t<BS, true, true, ORDER>(c, p, end_kick_half_dt);
-    else if (k) launch_bh_t<BS, true, false, ORDER>(c, p, end_kick_half_dt);
-    else if (count) launch_bh_t<BS, false, true, ORDER>(c, p, end_kick_half_dt);
-    else launch_bh_t<BS, false, false, ORDER>(c, p, end_kick_half_dt);
+    if (k && count) launch_bh_t<BS, true, true, ORDER, TEMP>(c, p, end_kick_half_dt);
+    else if (k) launch_bh_t<BS, true, false, ORDER, TEMP>(c, p, end_kick_half_dt);
+    else if (count) launch_bh_t<BS, false, true, ORDER, TEMP>(c, p, end_kick_half_dt);
+    else launch_bh_t<BS, false, false, ORDER, TEMP>(c, p, end_kick_half_dt);
 }
 
 template <int BS>
 void launch_bh_bs(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
-    if (p->accuracy == 2) launch_bh_o<BS, 4>(c, p, end_kick_half_dt, count);
-    else if (p->accuracy == 1) launch_bh_o<BS, 2>(c, p, end_kick_half_dt, count);
-    else launch_bh_o<BS, 0>(c, p, end_kick_half_dt, count);
+    if (p->accuracy == 2) launch_bh_o<BS, 4, false>(c, p, end_kick_half_dt, count);
+    else if (p->accuracy == 1) launch_bh_o<BS, 2, false>(c, p, end_kick_half_dt, count);
+    else if (p->temperature) launch_bh_o<BS, 0, true>(c, p, end_kick_half_dt, count);  // the reference walk only
+    else launch_bh_o<BS, 0, false>(c, p, end_kick_half_dt, count);
 }
 
 void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {

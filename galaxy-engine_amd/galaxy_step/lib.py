@@ -19,6 +19,8 @@ GE_FLAG_DRAWN = 4
 GE_FLAG_DARK = 8
 GE_FLAG_UNIQUE_COLOR = 16
 GE_FLAG_SOLID = 32
+GE_FLAG_HOT_POINT = 64
+GE_FLAG_SOLIDIFIED = 128
 
 # every symbol include/ge_physics.h declares
 EXPORTS = [
@@ -44,7 +46,8 @@ PARTICLE_ORDER = ["x", "y", "vx", "vy", "ax", "ay", "pvx", "pvy", "mass", "flags
 
 class GeConfig(C.Structure):
     _fields_ = [("capacity", C.c_uint32), ("device", C.c_int32), ("enable_sph", C.c_int32),
-                ("max_systems", C.c_uint32), ("enable_render", C.c_int32), ("reserved", C.c_uint32 * 3)]
+                ("max_systems", C.c_uint32), ("enable_render", C.c_int32), ("enable_temperature", C.c_int32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class GeParticles(C.Structure):
@@ -215,7 +218,7 @@ def _uptr(a):
 class Context:
     """One device context (ge_ctx)."""
 
-    def __init__(self, capacity, sph=False, device=0, max_systems=1, render=False):
+    def __init__(self, capacity, sph=False, device=0, max_systems=1, render=False, temperature=False):
         self.lib = load_library()
         cfg = GeConfig()
         self.lib.ge_default_config(C.byref(cfg))
@@ -224,7 +227,9 @@ class Context:
         cfg.enable_sph = 1 if sph else 0
         cfg.max_systems = int(max_systems)
         cfg.enable_render = 1 if render else 0
+        cfg.enable_temperature = 1 if temperature else 0
         self.render = bool(render)
+        self.temperature = bool(temperature)
         h = C.c_void_p()
         rc = self.lib.ge_create(C.byref(h), C.byref(cfg))
         if rc != GE_OK:

@@ -44,6 +44,8 @@ extern "C" {
 #define GE_FLAG_DARK 8u
 #define GE_FLAG_UNIQUE_COLOR 16u /* uniqueColor (read by the colour consumers, ge_render.h) */
 #define GE_FLAG_SOLID 32u        /* isSolid (read by the size consumers, ge_render.h) */
+#define GE_FLAG_HOT_POINT 64u    /* ParticlePhysics::isHotPoint (ge_thermal.h) */
+#define GE_FLAG_SOLIDIFIED 128u  /* ParticlePhysics::hasSolidified (ge_thermal.h) */
 
 typedef struct ge_ctx ge_ctx;
 
@@ -53,7 +55,8 @@ typedef struct {
     int32_t enable_sph;   /* allocate SPH state (needed for ge_step with sph=1) */
     uint32_t max_systems; /* independent systems one context can hold (>= 1) */
     int32_t enable_render; /* keep the render fields of ge_render.h on the device */
-    uint32_t reserved[3];
+    int32_t enable_temperature; /* keep the temperature fields of ge_thermal.h (needs enable_sph) */
+    uint32_t reserved[2];
 } ge_config;
 
 /* Structure-of-arrays view of the ParticlePhysics fields on this path
@@ -96,7 +99,8 @@ typedef struct {
                              into the domain (physics.cpp:1104-1110) and every
                              separation takes the nearest image (physics.cpp:117-120,
                              169-172, 417-420) */
-    int32_t temperature;  /* isTempEnabled: unsupported */
+    int32_t temperature;  /* isTempEnabled: the coupling of ge_thermal.h (a context
+                             created with enable_temperature) */
     int32_t sph_ground;   /* sphGround: SPH::groundModeBoundary after PCISPH
                              (SPH.cpp:283-322) */
     int32_t compensated;  /* Kahan-compensated force accumulation (1 = on) */

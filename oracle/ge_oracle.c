@@ -760,10 +760,12 @@ static void permute_state(geo_state *s, const uint32_t *perm) {
     float *t = (float *)malloc(sizeof(float) * (size_t)n);
     float *fs[] = {s->x, s->y, s->vx, s->vy, s->ax, s->ay, s->pvx, s->pvy, s->m,
                    s->sph_mass, s->rest_dens, s->stiff, s->visc, s->cohesion, s->dens,
-                   s->press, s->press_tmp, s->pred_dens, s->ppx, s->ppy, s->pfx, s->pfy};
+                   s->press, s->press_tmp, s->pred_dens, s->ppx, s->ppy, s->pfx, s->pfy,
+                   s->temp, s->ke, s->prev_ke};
     for (size_t k = 0; k < sizeof(fs) / sizeof(fs[0]); ++k) permute_f(fs[k], perm, n, t);
     permute_u(s->flags, perm, n, (uint32_t *)t);
     permute_u(s->id, perm, n, (uint32_t *)t);
+    permute_u(s->label, perm, n, (uint32_t *)t);
     free(t);
 }
 
@@ -793,12 +795,23 @@ int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps) {
         }
         if (p->dt != 0.0f) {
             for (int32_t i = 0; i < n; ++i) { s->ax[i] = 0.0f; s->ay[i] = 0.0f; }
-            if (p->G != 0.0f) {
-                if (p->naive || n < p->direct_below || !nodes)
+            const int temp_on = p->temperature && s->temp;
+            /* the walk runs for gravity or for the heat transfer
+             * (globalLogic.cpp:1316); the direct sum moves no heat */
+            if (p->G != 0.0f || temp_on) {
+                if (p->naive || n < p->direct_below || !nodes) {
                     geo_direct_accel_dom(n, s->x, s->y, s->m, p->G, p->softening, &dom, s->ax, s->ay);
-                else
+                } else {
                     geo_bh_accel_dom(n, s->x, s->y, s->m, nodes, (int32_t)nn, p->G, p->theta,
                                      p->softening, &dom, s->ax, s->ay, NULL);
+                    if (temp_on) {
+                        float *gt = (float *)malloc(sizeof(float) * (size_t)nn);
+                        geo_node_temps(nodes, (int32_t)nn, s->temp, gt);
+                        geo_walk_heat(n, s->x, s->y, nodes, (int32_t)nn, gt, p->theta, p->softening,
+                                      p->heat_conductivity, p->dt, &dom, s->temp);
+                        free(gt);
+                    }
+                }
             }
             if (p->sph) {
                 geo_sph_pcisph(s, p);
@@ -806,6 +819,8 @@ int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps) {
                     geo_sph_ground(s, p->sph_h, p->domain_w, p->domain_h, p->vertical_gravity,
                                    p->boundary_friction);
             }
+            if (temp_on)
+                geo_temperature(s, p->materials, p->nmaterials, p->ambient_temp, p->ambient_heat_rate, p->dt);
             geo_integrate_end(s, p->dt);
         }
         free(nodes);
@@ -813,4 +828,134 @@ int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps) {
     free(keys);
     free(perm);
     return 0;
+}
+
+/* --------------------------------------------------------- temperature --- */
+/* Node::computeLeafMass / computeInternalMass's gridTemp
+ * (GE/include/Physics/quadtree.h:53-100).  Children are allocated after their
+ * parent, so a reverse sweep meets every child first. */
+void geo_node_temps(const geo_node *nodes, int32_t nn, const float *temp, float *gtemp) {
+    static const int order[4] = {0, 2, 1, 3};
+    for (int32_t k = nn - 1; k >= 0; --k) {
+        const geo_node *g = &nodes[k];
+        float t = 0.0f;
+        if (g->child[0] == U32MAX && g->child[1] == U32MAX && g->child[2] == U32MAX && g->child[3] == U32MAX) {
+            for (uint32_t i = g->start; i < g->end; ++i) t = t + temp[i];
+        } else {
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t c = g->child[order[q]];
+                if (c != U32MAX) t = t + gtemp[c];
+            }
+        }
+        gtemp[k] = t;
+    }
+}
+
+/* the isTempEnabled branch of calculateForceFromGridAVX2
+ * (GE/src/Physics/physics.cpp:261-276) over the same traversal */
+void geo_walk_heat(int32_t n, const float *x, const float *y, const geo_node *nodes, int32_t nn,
+                   const float *gtemp, float theta, float eps, float conductivity, float dt,
+                   const geo_domain *dom, float *temp) {
+    const int per = dom->w > 0.0f;
+    const float hw = dom->w * 0.5f, hh = dom->h * 0.5f;
+    const float thetaSq = theta * theta;
+    const float softSq = eps * eps;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int32_t i = 0; i < n; ++i) {
+        float ti = temp[i];
+        const float xi = x[i], yi = y[i];
+        uint32_t idx = 0;
+        while (idx < (uint32_t)nn) {
+            const geo_node *g = &nodes[idx];
+            if (g->mass <= 0.0f) { idx += g->next + 1; continue; }
+            float dx = g->comx - xi;
+            float dy = g->comy - yi;
+            if (per) { dx = wrap(dx, dom->w, hw); dy = wrap(dy, dom->h, hh); }
+            const float distSq = dx * dx + dy * dy + softSq;
+            const int leaf = g->next == 0;
+            const float sizeSq = g->size * g->size;
+            if (sizeSq < thetaSq * distSq) {
+                const uint32_t count = g->end - g->start;
+                if (count > 0) {
+                    const float avg = gtemp[idx] / (float)count;
+                    const float diff = avg - ti;
+                    const float inv = 1.0f / sqrtf(distSq);
+                    float dist = 0.0f;
+                    if (distSq > 1e-16f) dist = 1.0f / inv;
+                    if (dist > 1e-8f) {
+                        const float heat = conductivity * diff / dist;
+                        ti = ti + heat * dt;
+                    }
+                }
+                idx += g->next + 1;
+            } else if (leaf) {
+                idx += g->next + 1;
+            } else {
+                ++idx;
+            }
+        }
+        temp[i] = ti;
+    }
+}
+
+static const geo_material *find_material(const geo_material *mats, int32_t nmat, uint32_t id) {
+    for (int32_t k = 0; k < nmat; ++k)
+        if (mats[k].id == id) return &mats[k];
+    return NULL;
+}
+
+/* Physics::temperatureCalculation (GE/src/Physics/physics.cpp:567-652);
+ * UpdateVariables::particleBaseMass = 8.5e9 (GE/src/parameters.cpp:44) */
+void geo_temperature(geo_state *s, const geo_material *mats, int32_t nmat, float ambient, float rate, float dt) {
+    const float base_mass = 8500000000.0f;
+    for (int32_t i = 0; i < s->n; ++i) {
+        const geo_material *mt = find_material(mats, nmat, s->label ? s->label[i] : 0u);
+        const float K = mt ? mt->heat_conductivity : 0.05f;
+        const float v = sqrtf(s->vx[i] * s->vx[i] + s->vy[i] * s->vy[i]);
+        const float pv = sqrtf(s->pvx[i] * s->pvx[i] + s->pvy[i] * s->pvy[i]);
+        const float sm = s->sph_mass[i];
+        const float ke = 0.5f * sm * v * v;
+        const float pke = 0.5f * sm * pv * pv;
+        s->ke[i] = ke;
+        s->prev_ke[i] = pke;
+        const float q = fabsf(ke - pke);
+        const float dT = q / (2.0f * K * sm + 1.0f);
+        float t = s->temp[i] + dT;
+        const float diff = t - ambient;
+        const float cool = -(K * rate) * diff * dt;
+        t = t + cool;
+        s->temp[i] = t;
+        if (!mt) continue;
+        uint32_t f = s->flags[i];
+        int hot = (f & GEO_HOT_POINT) != 0, solid = (f & GEO_SOLIDIFIED) != 0;
+        if (t >= mt->hot_point) {
+            s->sph_mass[i] = mt->hot_mass_mult;
+            s->rest_dens[i] = mt->hot_rest_dens;
+            s->stiff[i] = mt->hot_stiff;
+            s->visc[i] = mt->hot_visc;
+            s->cohesion[i] = mt->hot_cohesion;
+            if (mt->cold_point == 0.0f) hot = 1;
+        } else if (t <= mt->cold_point) {
+            s->sph_mass[i] = mt->cold_mass_mult;
+            s->rest_dens[i] = mt->cold_rest_dens;
+            s->stiff[i] = mt->cold_stiff;
+            s->visc[i] = mt->cold_visc;
+            s->cohesion[i] = mt->cold_cohesion;
+        } else {
+            s->sph_mass[i] = mt->mass_mult;
+            s->rest_dens[i] = mt->rest_dens;
+            s->stiff[i] = mt->stiff;
+            s->visc[i] = mt->visc;
+            s->cohesion[i] = mt->cohesion;
+            if (mt->cold_point != 0.0f) hot = 1;
+        }
+        s->m[i] = base_mass * s->sph_mass[i];
+        if (mt->cold_point == 0.0f) {
+            if (t <= mt->hot_point && hot) { solid = 1; hot = 0; }
+        } else {
+            if (t <= mt->cold_point && hot) { solid = 1; hot = 0; }
+        }
+        f &= ~(GEO_HOT_POINT | GEO_SOLIDIFIED);
+        s->flags[i] = f | (hot ? GEO_HOT_POINT : 0u) | (solid ? GEO_SOLIDIFIED : 0u);
+    }
 }

@@ -68,7 +68,27 @@ typedef struct {
     /* SPH fields (may be NULL when SPH is off) */
     float *sph_mass, *rest_dens, *stiff, *visc, *cohesion, *dens;
     float *press, *press_tmp, *pred_dens, *ppx, *ppy, *pfx, *pfy;
+    /* temperature fields (may be NULL when temperature is off): temp,
+     * sphLabel, ke, prevKe; isHotPoint / hasSolidified ride in flags */
+    float *temp;
+    uint32_t *label;
+    float *ke, *prev_ke;
 } geo_state;
+
+/* SPHMaterial (GE/include/Physics/materialsSPH.h:3-47), the fields the
+ * physics reads */
+typedef struct {
+    uint32_t id;
+    float mass_mult, rest_dens, stiff, visc, cohesion;
+    float hot_point, hot_rest_dens, hot_mass_mult, hot_stiff, hot_visc, hot_cohesion;
+    float cold_point, cold_rest_dens, cold_mass_mult, cold_stiff, cold_visc, cold_cohesion;
+    float heat_conductivity;
+    float constraint_resistance, constraint_plastic_point, constraint_plastic_point_mult, constraint_stiffness;
+    int32_t is_plastic;
+} geo_material;
+
+#define GEO_HOT_POINT 64u
+#define GEO_SOLIDIFIED 128u
 
 typedef struct {
     float G, theta, softening;
@@ -88,6 +108,12 @@ typedef struct {
     int32_t sph_ground;       /* myVar.sphGround: SPH::groundModeBoundary after PCISPH */
     float vertical_gravity;   /* myVar.verticalGravity */
     float boundary_friction;  /* myVar.boundaryFriction */
+    int32_t temperature;      /* myVar.isTempEnabled */
+    float heat_conductivity;  /* myVar.globalHeatConductivity (0.045) */
+    float ambient_temp;       /* myVar.ambientTemp (274) */
+    float ambient_heat_rate;  /* myVar.globalAmbientHeatRate (1) */
+    const geo_material *materials; /* SPHMaterials::idToMaterial */
+    int32_t nmaterials;
 } geo_params;
 
 /* domain of the periodic wrap: width and height, or 0 for no wrap */
@@ -161,6 +187,20 @@ void geo_sph_constants(float h, float out[3]); /* {W volume, spiky scale, laplac
 /* one full frame: GE/src/globalLogic.cpp:1155-1372 for the 2D path.
  * Permutes every state array into Morton order.  Returns 0 or negative. */
 int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps);
+
+/* ---- temperature (isTempEnabled) ---- */
+/* Node::gridTemp (GE/include/Physics/quadtree.h:53-100): leaves sum their
+ * particles' temp in order, internal nodes their children's in the order
+ * q = 0, 2, 1, 3 */
+void geo_node_temps(const geo_node *nodes, int32_t nn, const float *temp, float *gtemp);
+/* the heat transfer of calculateForceFromGridAVX2 (GE/src/Physics/physics.cpp:
+ * 206-360; the x86-64 build's walk): every accepted node moves temp[i] toward
+ * its average temperature, in walk order; opened leaves transfer nothing */
+void geo_walk_heat(int32_t n, const float *x, const float *y, const geo_node *nodes, int32_t nn,
+                   const float *gtemp, float theta, float eps, float conductivity, float dt,
+                   const geo_domain *dom, float *temp);
+/* Physics::temperatureCalculation (GE/src/Physics/physics.cpp:567-652) */
+void geo_temperature(geo_state *s, const geo_material *mats, int32_t nmat, float ambient, float rate, float dt);
 
 #ifdef __cplusplus
 }

@@ -23,11 +23,64 @@ STATE_SPH = ["sph_mass", "rest_dens", "stiff", "visc", "cohesion", "dens", "pres
              "pred_dens", "ppx", "ppy", "pfx", "pfy"]
 
 
+STATE_TEMP = ["temp", "label", "ke", "prev_ke"]
+GEO_HOT_POINT, GEO_SOLIDIFIED = 64, 128
+
+
 class _State(C.Structure):
     _fields_ = ([("n", C.c_int32)] + [(f, C.POINTER(C.c_float)) for f in
                                       ["x", "y", "vx", "vy", "ax", "ay", "pvx", "pvy", "m"]]
                 + [("flags", C.POINTER(C.c_uint32)), ("id", C.POINTER(C.c_uint32))]
-                + [(f, C.POINTER(C.c_float)) for f in STATE_SPH])
+                + [(f, C.POINTER(C.c_float)) for f in STATE_SPH]
+                + [("temp", C.POINTER(C.c_float)), ("label", C.POINTER(C.c_uint32)),
+                   ("ke", C.POINTER(C.c_float)), ("prev_ke", C.POINTER(C.c_float))])
+
+
+class Material(C.Structure):
+    """SPHMaterial (GE/include/Physics/materialsSPH.h:3-47), physics fields."""
+    _fields_ = ([("id", C.c_uint32)]
+                + [(f, C.c_float) for f in (
+                    "mass_mult", "rest_dens", "stiff", "visc", "cohesion",
+                    "hot_point", "hot_rest_dens", "hot_mass_mult", "hot_stiff", "hot_visc", "hot_cohesion",
+                    "cold_point", "cold_rest_dens", "cold_mass_mult", "cold_stiff", "cold_visc", "cold_cohesion",
+                    "heat_conductivity", "constraint_resistance", "constraint_plastic_point",
+                    "constraint_plastic_point_mult", "constraint_stiffness")]
+                + [("is_plastic", C.c_int32)])
+
+
+def _mat(id_, mass, rest, stiff, visc, coh, hot, hrest, hmass, hstiff, hvisc, hcoh, cold, crest, cmass,
+         cstiff, cvisc, ccoh, k, res, pp_frac, ppm, cst, plastic):
+    m = Material()
+    vals = dict(id=id_, mass_mult=mass, rest_dens=rest, stiff=stiff, visc=visc, cohesion=coh, hot_point=hot,
+                hot_rest_dens=hrest, hot_mass_mult=hmass, hot_stiff=hstiff, hot_visc=hvisc, hot_cohesion=hcoh,
+                cold_point=cold, cold_rest_dens=crest, cold_mass_mult=cmass, cold_stiff=cstiff, cold_visc=cvisc,
+                cold_cohesion=ccoh, heat_conductivity=k, constraint_resistance=res,
+                constraint_plastic_point=float(np.float32(res) * np.float32(pp_frac)),
+                constraint_plastic_point_mult=ppm, constraint_stiffness=cst, is_plastic=plastic)
+    for key, v in vals.items():
+        setattr(m, key, v)
+    return m
+
+
+def reference_materials():
+    """The seven materials SPHMaterials::Init registers (materialsSPH.h:49-300); a
+    material whose cold point is 0 takes its base values as cold values."""
+    return [
+        _mat(1, 0.6, 0.095, 1.0, 0.075, 0.05, 373.2, 0.065, 0.25, 1.0, 0.055, 0.0, 273.2, 0.046, 0.5, 0.6, 2.2,
+             2500.0, 0.15, 0.06, 0.5, 0.0, 55.0, 0),
+        _mat(2, 3.3, 0.008, 1.4, 3.0, 1750.0, 1370.0, 0.005, 2.8, 1.0, 0.6, 300.0, 0.0, 0.008, 3.3, 1.4, 3.0,
+             1750.0, 0.02, 0.211, 0.5, 0.0, 60.0, 0),
+        _mat(3, 4.0, 0.008, 1.4, 3.0, 1750.0, 1740.0, 0.005, 2.8, 1.0, 0.6, 300.0, 0.0, 0.008, 4.0, 1.4, 3.0,
+             1750.0, 0.02, 0.24, 0.45, 1.8, 60.0, 1),
+        _mat(4, 2.1, 0.008, 1.255, 0.74, 1.0, 1200.0, 0.011, 1.9, 1.12, 0.6, 1.0, 0.0, 0.008, 2.1, 1.255, 0.74,
+             1.0, 0.01, 0.08, 0.5, 0.0, 55.0, 0),
+        _mat(5, 1.9, 0.008, 1.0, 2.23, 3000.0, 950.0, 0.013, 1.8, 0.9, 1.8, 600.0, 0.0, 0.008, 1.9, 1.0, 2.23,
+             3000.0, 0.02, 0.09, 0.5, 2.0, 30.0, 1),
+        _mat(6, 2.3, 0.0095, 1.0, 0.6, 100.0, 1000.0, 0.011, 2.1, 1.0, 0.5, 40.0, 0.0, 0.0095, 2.3, 1.0, 0.6,
+             100.0, 0.1, 0.03, 0.5, 2.0, 20.0, 1),
+        _mat(7, 1.7, 0.0095, 1.0, 0.6, 100.0, 453.0, 0.011, 1.6, 1.0, 0.5, 40.0, 0.0, 0.0095, 1.7, 1.0, 0.6,
+             100.0, 1.1, 5.5, 0.5, 2.0, 6.0, 1),
+    ]
 
 
 class _Params(C.Structure):
@@ -38,7 +91,9 @@ class _Params(C.Structure):
                 ("cohesion_coef", C.c_float), ("delta", C.c_float), ("stiff_mult", C.c_float),
                 ("sph_max_vel", C.c_float), ("sph_search", C.c_int32), ("sph_search_radius", C.c_float),
                 ("periodic", C.c_int32), ("domain_w", C.c_float), ("domain_h", C.c_float),
-                ("sph_ground", C.c_int32), ("vertical_gravity", C.c_float), ("boundary_friction", C.c_float)]
+                ("sph_ground", C.c_int32), ("vertical_gravity", C.c_float), ("boundary_friction", C.c_float),
+                ("temperature", C.c_int32), ("heat_conductivity", C.c_float), ("ambient_temp", C.c_float),
+                ("ambient_heat_rate", C.c_float), ("materials", C.c_void_p), ("nmaterials", C.c_int32)]
 
 
 _lib = None
@@ -92,6 +147,13 @@ def load():
     lib.geo_step.argtypes = [C.POINTER(_State), C.POINTER(_Params), C.c_int32]
     lib.geo_sph_pcisph.restype = C.c_int32
     lib.geo_sph_pcisph.argtypes = [C.POINTER(_State), C.POINTER(_Params)]
+    lib.geo_node_temps.restype = None
+    lib.geo_node_temps.argtypes = [C.c_void_p, C.c_int32, f32p, f32p]
+    lib.geo_walk_heat.restype = None
+    lib.geo_walk_heat.argtypes = [C.c_int32, f32p, f32p, C.c_void_p, C.c_int32, f32p, C.c_float, C.c_float,
+                                  C.c_float, C.c_float, C.c_void_p, f32p]
+    lib.geo_temperature.restype = None
+    lib.geo_temperature.argtypes = [C.POINTER(_State), C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_float]
     _lib = lib
     return lib
 
@@ -210,6 +272,34 @@ def neighbor_amount(x, y, flags, cell=3.0, radius=3.5):
     return out
 
 
+def node_temps(nodes, temp):
+    """Node::gridTemp of every node (NODE_DTYPE array) from the particles' temp."""
+    nd = np.ascontiguousarray(nodes)
+    t = f32(temp)
+    out = np.zeros(len(nd), np.float32)
+    load().geo_node_temps(nd.ctypes.data, len(nd), _f(t), _f(out))
+    return out
+
+
+def walk_heat(x, y, nodes, gtemp, theta, eps, conductivity, dt, temp):
+    x, y, gt = f32(x), f32(y), f32(gtemp)
+    nd = np.ascontiguousarray(nodes)
+    t = f32(temp).copy()
+    dom = _Domain(0.0, 0.0)
+    load().geo_walk_heat(len(x), _f(x), _f(y), nd.ctypes.data, len(nd), _f(gt), theta, eps, conductivity, dt,
+                         C.byref(dom), _f(t))
+    return t
+
+
+def temperature(st, materials, ambient=274.0, rate=1.0, dt=0.045):
+    """Physics::temperatureCalculation on a copy of st (dict of arrays); returns the new dict."""
+    s = State(st, sph=True, temperature=True)
+    arr = (Material * max(len(materials), 1))(*materials)
+    load().geo_temperature(C.byref(s._s), C.addressof(arr) if materials else None, len(materials), ambient, rate,
+                           dt)
+    return s.a
+
+
 def sph_constants(h):
     out = np.zeros(3, np.float32)
     load().geo_sph_constants(h, _f(out))
@@ -219,7 +309,7 @@ def sph_constants(h):
 class State:
     """Host particle state for the oracle step; arrays are permuted in place."""
 
-    def __init__(self, st, sph=False):
+    def __init__(self, st, sph=False, temperature=False):
         n = len(st["x"])
         self.n = n
         self.a = {}
@@ -241,6 +331,16 @@ class State:
         if sph:
             for f in STATE_SPH:
                 setattr(self._s, f, _f(self.a[f]))
+        self.temperature = temperature
+        if temperature:  # ParticlePhysics' temp starts at 288 (particle.h:61)
+            self.a["temp"] = f32(st["temp"]).copy() if st.get("temp") is not None else np.full(n, 288.0, np.float32)
+            self.a["label"] = np.ascontiguousarray(st.get("label", np.zeros(n)), dtype=np.uint32).copy()
+            self.a["ke"] = f32(st["ke"]).copy() if st.get("ke") is not None else np.zeros(n, np.float32)
+            self.a["prev_ke"] = f32(st["prev_ke"]).copy() if st.get("prev_ke") is not None else np.zeros(n, np.float32)
+            self._s.temp = _f(self.a["temp"])
+            self._s.label = _u(self.a["label"])
+            self._s.ke = _f(self.a["ke"])
+            self._s.prev_ke = _f(self.a["prev_ke"])
 
     def step(self, prm, nsteps=1):
         p = _Params()
@@ -248,11 +348,17 @@ class State:
                  sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3, cohesion_coef=1.0,
                  delta=19000.0, stiff_mult=1.0, sph_max_vel=250.0, sph_search=0, sph_search_radius=3.5,
                  periodic=0, domain_w=3840.0, domain_h=2160.0, sph_ground=0, vertical_gravity=3.0,
-                 boundary_friction=0.0)
+                 boundary_friction=0.0, temperature=0, heat_conductivity=0.045, ambient_temp=274.0,
+                 ambient_heat_rate=1.0)
         d.update({k: v for k, v in prm.items() if k in d or k == "dt_start"})
         d.setdefault("dt_start", d["dt"])
         for k, v in d.items():
             setattr(p, k, v)
+        mats = prm.get("materials")
+        if mats:
+            arr = (Material * len(mats))(*mats)
+            p.materials = C.addressof(arr)
+            p.nmaterials = len(mats)
         rc = load().geo_step(C.byref(self._s), C.byref(p), int(nsteps))
         if rc != 0:
             raise RuntimeError(f"oracle step failed {rc}")
