@@ -211,6 +211,8 @@ struct Soa {
     std::vector<uint32_t> color, pcolor, scolor;  // render fields (RGBA8)
     std::vector<float> size;
     std::vector<int32_t> nbrs;
+    std::vector<float> temp, ke, prev_ke;          // temperature fields
+    std::vector<uint32_t> label;
     std::vector<std::vector<float> *> floats() {
         return {&x, &y, &z, &vx, &vy, &vz, &ax, &ay, &az, &pvx, &pvy, &pvz, &m, &sph_mass, &rest_dens, &stiff,
                 &visc, &cohesion, &dens, &press, &press_tmp, &pred_dens, &ppx, &ppy, &ppz, &pfx, &pfy, &pfz};
@@ -224,6 +226,18 @@ struct Soa {
         scolor.assign(n, 0u);
         size.assign(n, 0.0f);
         nbrs.assign(n, 0);
+        temp.assign(n, 0.0f);
+        ke.assign(n, 0.0f);
+        prev_ke.assign(n, 0.0f);
+        label.assign(n, 0u);
+    }
+    ge_thermal_fields thermal() {
+        ge_thermal_fields f{};
+        f.temp = temp.data();
+        f.label = label.data();
+        f.ke = ke.data();
+        f.prev_ke = prev_ke.data();
+        return f;
     }
     ge_render_fields render() {
         ge_render_fields f{};
@@ -280,7 +294,13 @@ void decode(const ge_scene *s, Soa &o) {
         if (rd<uint8_t>(r, L.off[F_DARK])) f |= GE_FLAG_DARK;
         if (rd<uint8_t>(r, L.off[F_UNIQUECOLOR])) f |= GE_FLAG_UNIQUE_COLOR;
         if (rd<uint8_t>(r, L.off[F_SOLID])) f |= GE_FLAG_SOLID;
+        if (rd<uint8_t>(r, L.off[F_HOTPOINT])) f |= GE_FLAG_HOT_POINT;
+        if (rd<uint8_t>(r, L.off[F_SOLIDIFIED])) f |= GE_FLAG_SOLIDIFIED;
         o.flags[i] = f;
+        o.temp[i] = rd<float>(r, L.off[F_TEMP]);
+        o.ke[i] = rd<float>(r, L.off[F_KE]);
+        o.prev_ke[i] = rd<float>(r, L.off[F_PREVKE]);
+        o.label[i] = rd<uint32_t>(r, L.off[F_SPHLABEL]);
         o.color[i] = rd<uint32_t>(r, L.off[F_COLOR]);  // Color {r, g, b, a}: r in the low byte
         o.pcolor[i] = rd<uint32_t>(r, L.off[F_PCOLOR]);
         o.scolor[i] = rd<uint32_t>(r, L.off[F_SCOLOR]);
@@ -301,7 +321,7 @@ bool untouched(float v) {
 
 // structure of arrays (device order) -> records, carrying every field the
 // device does not hold over from the record of the same id
-void encode(ge_scene *s, const Soa &o, uint32_t n, bool render) {
+void encode(ge_scene *s, const Soa &o, uint32_t n, bool render, bool thermal) {
     const Layout L = s->layout();
     const int d = L.dim;
     std::unordered_map<uint32_t, uint32_t> by_id;
@@ -347,6 +367,14 @@ void encode(ge_scene *s, const Soa &o, uint32_t n, bool render) {
         wr<uint8_t>(r, L.off[F_DARK], (o.flags[k] & GE_FLAG_DARK) ? 1 : 0);
         wr<uint8_t>(r, L.off[F_UNIQUECOLOR], (o.flags[k] & GE_FLAG_UNIQUE_COLOR) ? 1 : 0);
         wr<uint8_t>(r, L.off[F_SOLID], (o.flags[k] & GE_FLAG_SOLID) ? 1 : 0);
+        wr<uint8_t>(r, L.off[F_HOTPOINT], (o.flags[k] & GE_FLAG_HOT_POINT) ? 1 : 0);
+        wr<uint8_t>(r, L.off[F_SOLIDIFIED], (o.flags[k] & GE_FLAG_SOLIDIFIED) ? 1 : 0);
+        if (thermal) {
+            wr<float>(r, L.off[F_TEMP], o.temp[k]);
+            wr<float>(r, L.off[F_KE], o.ke[k]);
+            wr<float>(r, L.off[F_PREVKE], o.prev_ke[k]);
+            wr<uint32_t>(r, L.off[F_SPHLABEL], o.label[k]);
+        }
         if (render) {
             wr<uint32_t>(r, L.off[F_COLOR], o.color[k]);
             wr<uint32_t>(r, L.off[F_PCOLOR], o.pcolor[k]);
@@ -606,6 +634,23 @@ int ge_scene_set_params3(ge_scene *s, const ge3_step_params *p) {
     return GE_OK;
 }
 
+int ge_scene_thermal_params(const ge_scene *s, ge_thermal_params *t) {
+    if (!s || !t) return GE_E_INVALID;
+    ge_default_thermal_params(t);
+    get_float(s, "AmbientTemperature", t->ambient_temp);
+    get_float(s, "AmbientHeatRate", t->ambient_heat_rate);
+    get_float(s, "HeatConductivityMultiplier", t->heat_conductivity);
+    return GE_OK;
+}
+
+int ge_scene_set_thermal_params(ge_scene *s, const ge_thermal_params *t) {
+    if (!s || !t) return GE_E_INVALID;
+    set_kv(s, "AmbientTemperature", fmt_float(t->ambient_temp));
+    set_kv(s, "AmbientHeatRate", fmt_float(t->ambient_heat_rate));
+    set_kv(s, "HeatConductivityMultiplier", fmt_float(t->heat_conductivity));
+    return GE_OK;
+}
+
 int ge_scene_upload(const ge_scene *s, ge_ctx *ctx) {
     if (!s || !ctx) return GE_E_INVALID;
     if (s->is3d) {
@@ -620,6 +665,11 @@ int ge_scene_upload(const ge_scene *s, ge_ctx *ctx) {
     if (r == GE_OK && (r = ge_render_enabled(ctx, &render)) == GE_OK && render) {
         ge_render_fields f = o.render();
         r = ge_render_upload(ctx, &f);
+    }
+    int32_t thermal = 0;
+    if (r == GE_OK && (r = ge_thermal_enabled(ctx, &thermal)) == GE_OK && thermal) {
+        ge_thermal_fields f = o.thermal();
+        r = ge_thermal_upload(ctx, &f);
     }
     if (r != GE_OK) s->err = ge_last_error(ctx);
     return r;
@@ -662,7 +712,13 @@ int ge_scene_download(ge_scene *s, ge_ctx *ctx) {
         ge_render_fields f = o.render();
         if (int r = ge_render_download(ctx, &f)) return fail(s, r, ge_last_error(ctx));
     }
-    encode(s, o, n, render != 0);
+    int32_t thermal = 0;
+    if (int r = ge_thermal_enabled(ctx, &thermal)) return fail(s, r, ge_last_error(ctx));
+    if (thermal) {
+        ge_thermal_fields f = o.thermal();
+        if (int r = ge_thermal_download(ctx, &f)) return fail(s, r, ge_last_error(ctx));
+    }
+    encode(s, o, n, render != 0, thermal != 0);
     return GE_OK;
 }
 
@@ -684,7 +740,7 @@ int ge_scene_download3(ge_scene *s, ge3_ctx *ctx) {
         ge_render_fields f = o.render();
         if (int r = ge3_render_download(ctx, &f)) return fail(s, r, ge3_last_error(ctx));
     }
-    encode(s, o, n, render != 0);
+    encode(s, o, n, render != 0, false);
     return GE_OK;
 }
 

@@ -244,6 +244,12 @@ void ge_default_thermal_params(ge_thermal_params *t) {
     t->base_mass = 8500000000.0f;   // parameters.cpp:44
 }
 
+int ge_thermal_enabled(const ge_ctx *c, int32_t *on) {
+    if (!c || !on) return GE_E_INVALID;
+    *on = c->thermal ? 1 : 0;
+    return GE_OK;
+}
+
 int ge_default_materials(ge_material *out, uint32_t cap, uint32_t *count) {
     if (!count || (cap && !out)) return GE_E_INVALID;
     // water, rock, iron, sand, soil, mud, rubber; a material with cold point 0

@@ -5,12 +5,13 @@ import ctypes as C
 
 from .lib import GE_OK, GeError, StepParams, load_library
 from .lib3d import Step3Params, load_library3
+from .thermal import ThermalParams, load_thermal_library
 
 SCENE_EXPORTS = [
     "ge_scene_load", "ge_scene_new", "ge_scene_free", "ge_scene_last_error", "ge_scene_get_info",
     "ge_scene_value", "ge_scene_set_value", "ge_scene_records", "ge_scene_params", "ge_scene_params3",
     "ge_scene_set_params", "ge_scene_set_params3", "ge_scene_upload", "ge_scene_upload3", "ge_scene_download",
-    "ge_scene_download3", "ge_scene_save",
+    "ge_scene_download3", "ge_scene_save", "ge_scene_thermal_params", "ge_scene_set_thermal_params",
 ]
 
 
@@ -27,6 +28,7 @@ def load_scene_library():
     global _ready
     lib = load_library()
     load_library3()
+    load_thermal_library()
     if _ready:
         return lib
     P = C.POINTER
@@ -49,6 +51,8 @@ def load_scene_library():
         "ge_scene_download": (C.c_int, [vp, vp]),
         "ge_scene_download3": (C.c_int, [vp, vp]),
         "ge_scene_save": (C.c_int, [vp, C.c_char_p]),
+        "ge_scene_thermal_params": (C.c_int, [vp, P(ThermalParams)]),
+        "ge_scene_set_thermal_params": (C.c_int, [vp, P(ThermalParams)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -132,6 +136,14 @@ class Scene:
         p = Step3Params()
         self._check(self.lib.ge_scene_params3(self.h, C.byref(p)))
         return p
+
+    def thermal_params(self):
+        t = ThermalParams()
+        self._check(self.lib.ge_scene_thermal_params(self.h, C.byref(t)))
+        return t
+
+    def set_thermal_params(self, t):
+        self._check(self.lib.ge_scene_set_thermal_params(self.h, C.byref(t)))
 
     def set_params(self, p):
         fn = self.lib.ge_scene_set_params3 if isinstance(p, Step3Params) else self.lib.ge_scene_set_params

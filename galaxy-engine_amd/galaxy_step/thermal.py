@@ -8,7 +8,7 @@ import numpy as np
 from .lib import GE_OK, GeError, load_library
 
 THERMAL_EXPORTS = [
-    "ge_default_thermal_params", "ge_default_materials", "ge_set_thermal", "ge_thermal_upload",
+    "ge_default_thermal_params", "ge_thermal_enabled", "ge_default_materials", "ge_set_thermal", "ge_thermal_upload",
     "ge_thermal_download", "ge_temperature_calculation", "ge_get_node_temps",
 ]
 THERMAL_FIELDS = ["temp", "label", "ke", "prev_ke"]
@@ -46,6 +46,7 @@ def load_thermal_library():
     sig = {
         "ge_default_thermal_params": (None, [P(ThermalParams)]),
         "ge_default_materials": (C.c_int, [P(Material), C.c_uint32, P(C.c_uint32)]),
+        "ge_thermal_enabled": (C.c_int, [vp, P(C.c_int32)]),
         "ge_set_thermal": (C.c_int, [vp, P(ThermalParams), P(Material), C.c_uint32]),
         "ge_thermal_upload": (C.c_int, [vp, P(ThermalFields)]),
         "ge_thermal_download": (C.c_int, [vp, P(ThermalFields)]),

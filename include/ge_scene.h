@@ -28,6 +28,7 @@
 #include "ge_physics.h"
 #include "ge_physics3d.h"
 #include "ge_render.h"
+#include "ge_thermal.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -73,15 +74,23 @@ int ge_scene_params3(const ge_scene *s, ge3_step_params *p);
 int ge_scene_set_params(ge_scene *s, const ge_step_params *p);
 int ge_scene_set_params3(ge_scene *s, const ge3_step_params *p);
 
+/* the temperature constants from AmbientTemperature, AmbientHeatRate and
+ * HeatConductivityMultiplier (saveSystem.cpp:103-106) over the defaults of
+ * ge_default_thermal_params, and back */
+int ge_scene_thermal_params(const ge_scene *s, ge_thermal_params *t);
+int ge_scene_set_thermal_params(ge_scene *s, const ge_thermal_params *t);
+
 /* particles into a context (ge_upload / ge3_upload): pos, vel, acc, prevVel,
  * mass, id, the SPH fields when the context has them, the flags isPinned,
- * isSPH, isBeingDrawn, isDarkMatter, uniqueColor, isSolid (GE_FLAG_*), and
- * color, pColor, sColor, size, neighbors when it keeps render fields
- * (ge_render.h) */
+ * isSPH, isBeingDrawn, isDarkMatter, uniqueColor, isSolid, isHotPoint,
+ * hasSolidified (GE_FLAG_*), color, pColor, sColor, size, neighbors when it
+ * keeps render fields (ge_render.h) and temp, sphLabel, ke, prevKe when it
+ * keeps temperature fields (ge_thermal.h) */
 int ge_scene_upload(const ge_scene *s, ge_ctx *ctx);
 int ge_scene_upload3(const ge_scene *s, ge3_ctx *ctx);
 /* the context's particles back into records, in the device order: the
- * device fields from the context (the render fields too when it keeps them),
+ * device fields from the context (the render and temperature fields too
+ * when it keeps them),
  * every other field from the record of the same id (a new id gets the
  * reference's defaults for a spawned particle, particle.h:35-63, 100-130) */
 int ge_scene_download(ge_scene *s, ge_ctx *ctx);

@@ -59,6 +59,8 @@ typedef struct {
 } ge_thermal_fields;
 
 void ge_default_thermal_params(ge_thermal_params *t);
+/* *on = 1 when the context keeps temperature fields (enable_temperature) */
+int ge_thermal_enabled(const ge_ctx *ctx, int32_t *on);
 /* the seven materials SPHMaterials::Init registers (materialsSPH.h:49-300):
  * water, rock, iron, sand, soil, mud, rubber (ids 1-7); *count gets 7 */
 int ge_default_materials(ge_material *out, uint32_t cap, uint32_t *count);

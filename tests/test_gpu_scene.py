@@ -205,3 +205,59 @@ def test_scene_render_fields_through_the_device(tmp_path):
     np.testing.assert_array_equal(got["neighbors"], after["neighbors"])
     np.testing.assert_array_equal(got["uniqueColor"], recs["uniqueColor"])
     np.testing.assert_array_equal(got["isSolid"], recs["isSolid"])
+
+
+def test_scene_temperature_through_the_device(tmp_path):
+    # a temperature scene: temp, sphLabel, ke, prevKe, isHotPoint and
+    # hasSolidified go to a context with temperature fields, the frame runs
+    # the coupling with the file's AmbientTemperature / HeatConductivity, and
+    # the records get the device values back; one frame matches the oracle
+    from galaxy_step import thermal as T
+    import oracle as O
+    st, _ = configs.config0(seed=14, n=6000)
+    recs = _records_2d(st, 9)
+    rng = np.random.default_rng(10)
+    n = len(recs)
+    recs["temp"] = rng.uniform(200.0, 1500.0, n).astype(np.float32)
+    recs["sphLabel"] = rng.integers(0, 8, n).astype(np.uint32)
+    recs["sphMass"] = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    recs["isPinned"] = 0
+    gm = np.float32(1.0 / 6.674e-11)
+    yaml = (f"Theta: 0.699999988\n---\nSoftening: 1\n---\nGravityMultiplier: {gm:.9g}\n---\n"
+            f"TimeMult: {np.float32(0.01 / 0.045):.9g}\n---\nTemperatureSimulation: true\n---\n"
+            f"AmbientTemperature: 300\n---\nHeatConductivityMultiplier: 0.0900000036\n---\nis3DMode: false\n")
+    src = tmp_path / "Save_t.bin"
+    _file(src, recs, yaml)
+    with Scene.load(src) as sc:
+        p = sc.params()
+        tp = sc.thermal_params()
+        assert p.temperature == 1 and tp.ambient_temp == 300.0 and tp.heat_conductivity == np.float32(0.09)
+        p.direct_below = 0
+        with Context(n, sph=True, temperature=True) as c:
+            sc.upload(c)
+            T.set_thermal(c, tp)
+            c.step(p, 1)
+            c.sync()
+            dev = c.download(["id", "flags"])
+            dev.update(T.thermal_download(c))
+            sc.download(c)
+        out = tmp_path / "Save_t_out.bin"
+        sc.save(out)
+    with Scene.load(out) as back:
+        got = np.frombuffer(back.records(), record_dtype(2))
+    np.testing.assert_array_equal(got["id"], dev["id"])
+    np.testing.assert_array_equal(got["temp"], dev["temp"])
+    np.testing.assert_array_equal(got["ke"], dev["ke"])
+    np.testing.assert_array_equal(got["sphLabel"], dev["label"])
+    np.testing.assert_array_equal(got["isHotPoint"], (dev["flags"] & 64) != 0)
+    ref = O.State({"x": recs["pos"][:, 0], "y": recs["pos"][:, 1], "vx": recs["vel"][:, 0],
+                   "vy": recs["vel"][:, 1], "mass": recs["mass"], "id": recs["id"],
+                   "sph_mass": recs["sphMass"], "temp": recs["temp"], "label": recs["sphLabel"],
+                   "flags": (recs["isHotPoint"].astype(np.uint32) * 64 + recs["hasSolidified"].astype(np.uint32) * 128),
+                   "pvx": recs["prevVel"][:, 0], "pvy": recs["prevVel"][:, 1]}, sph=True, temperature=True)
+    ref.step(dict(G=float(p.G), theta=float(p.theta), softening=float(p.softening), dt=float(p.dt),
+                  dt_start=float(p.dt), direct_below=0, temperature=1, ambient_temp=300.0,
+                  heat_conductivity=float(np.float32(0.09)), materials=O.reference_materials()), nsteps=1)
+    o = np.argsort(ref["id"])
+    g = _by_id(got)
+    np.testing.assert_array_equal(g["temp"], ref["temp"][o])
