@@ -1,6 +1,7 @@
 // ge_abi.hip -- the extern "C" entry points of include/ge_physics.h:
 // context lifecycle, device buffers, state transfer and the frame driver that
 // orders the stage kernels as updateScene() does (globalLogic.cpp:1155-1372).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -87,6 +88,10 @@ void free_all(ge_ctx *c) {
     dfree(T.blist); dfree(T.blist_off); dfree(T.blist_cnt);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
     dfree(b.node_temp); dfree(b.node_parent); dfree(b.node_arrive); dfree(b.materials);
+    dfree(b.cons); dfree(b.cons_tmp); dfree(b.cidx); dfree(b.cidx_tmp); dfree(b.ccolor); dfree(b.ccolor_tmp);
+    dfree(b.ckeep); dfree(b.cblock); dfree(b.ccount); dfree(b.corder); dfree(b.cstart); dfree(b.cw_a);
+    dfree(b.cw_b); dfree(b.chist);
+    b.cframe = b.cleft = nullptr;
     dfree(b.nodes); dfree(b.node_geom); dfree(b.node_mp); dfree(b.mp_cnt); dfree(b.mp_wave_list); dfree(b.mp_big_list); dfree(b.mp_piece_owner); dfree(b.mp_piece_sum); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
         dfree(g->keys); dfree(g->keys_alt); dfree(g->first); dfree(g->ord); dfree(g->frame);
@@ -144,6 +149,27 @@ int alloc_all(ge_ctx *c) {
     b.node_cap = 4u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(b.nodes, b.node_cap));
     GE_HIP_CHECK(c, dalloc(b.node_geom, b.node_cap));
+    if (c->cons_cap) {
+        const uint32_t mc = c->cons_cap;
+        GE_HIP_CHECK(c, dalloc(b.cons, mc));
+        GE_HIP_CHECK(c, dalloc(b.cons_tmp, mc));
+        GE_HIP_CHECK(c, dalloc(b.cidx, mc));
+        GE_HIP_CHECK(c, dalloc(b.cidx_tmp, mc));
+        GE_HIP_CHECK(c, dalloc(b.ccolor, mc));
+        GE_HIP_CHECK(c, dalloc(b.ccolor_tmp, mc));
+        GE_HIP_CHECK(c, dalloc(b.ckeep, mc));
+        GE_HIP_CHECK(c, dalloc(b.cblock, (size_t)mc / 256 + 2));
+        GE_HIP_CHECK(c, dalloc(b.ccount, 4));
+        GE_HIP_CHECK(c, hipMemset(b.ccount, 0, 4 * sizeof(uint32_t)));
+        b.cframe = b.ccount + 1;
+        b.cleft = b.ccount + 2;
+        GE_HIP_CHECK(c, dalloc(b.corder, mc));
+        GE_HIP_CHECK(c, dalloc(b.cstart, 72));
+        const uint64_t words = std::max<uint64_t>(2ull * mc, cap);
+        GE_HIP_CHECK(c, dalloc(b.cw_a, words));
+        GE_HIP_CHECK(c, dalloc(b.cw_b, words));
+        GE_HIP_CHECK(c, dalloc(b.chist, radix_hist_entries((uint32_t)words)));
+    }
     if (c->thermal) {
         GE_HIP_CHECK(c, dalloc(b.node_temp, b.node_cap));
         GE_HIP_CHECK(c, dalloc(b.node_parent, b.node_cap));
@@ -306,6 +332,7 @@ int fail(ge_ctx *c, int code, const char *msg) {
 int check_params(ge_ctx *c, const ge_step_params *p) {
     if (!p) return fail(c, GE_E_INVALID, "params is NULL");
     if (p->temperature && !c->thermal) return fail(c, GE_E_INVALID, "context created without enable_temperature");
+    if (p->constraints && c->nsys > 1) return fail(c, GE_E_UNSUPPORTED, "constraints run on single-system contexts");
     if (p->temperature && p->accuracy)
         return fail(c, GE_E_UNSUPPORTED, "the heat transfer follows the reference walk (accuracy 0)");
     if ((p->periodic || p->sph_ground) && !(p->domain_w > 0.0f && p->domain_h > 0.0f))
@@ -393,7 +420,9 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
         if (int r = sph_new_grid(c, p)) return r;
     }
     const bool temp = p->temperature != 0;  // temperatureCalculation runs before integrateEnd
-    const float end_kick = (p->sph || temp) ? 0.0f : p->dt * 0.5f;  // fused integrateEnd when nothing follows
+    const bool cons = p->constraints && c->cons_bound > 0;  // Physics::constraints after PCISPH
+    const bool late = temp || cons;  // work between the forces and integrateEnd
+    const float end_kick = (p->sph || late) ? 0.0f : p->dt * 0.5f;  // fused integrateEnd when nothing follows
     const bool ground = p->sph && p->sph_ground;           // groundModeBoundary runs before integrateEnd
     // the walk runs for gravity or for the heat transfer (globalLogic.cpp:1316)
     if (p->G != 0.0f || temp) {
@@ -411,18 +440,21 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
         }
     } else {
         launch_zero_acc(c);
-        if (!p->sph) {
+        if (!p->sph && !late) {
             PhaseScope s(c, GE_PHASE_INTEGRATE_END);
             launch_integrate_end(c, p->dt);
         }
     }
     if (p->sph) {
         PhaseScope s(c, GE_PHASE_SPH);
-        if (int r = sph_pcisph(c, p, (ground || temp) ? 0.0f : p->dt)) return r;
+        if (int r = sph_pcisph(c, p, (ground || late) ? 0.0f : p->dt)) return r;
         if (ground) launch_sph_ground(c, p);
     }
+    if (cons) {
+        if (int r = cons_solve(c, p, false)) return r;
+    }
     if (temp) launch_temperature(c, p);
-    if (ground || temp) {
+    if (ground || late) {
         PhaseScope s(c, GE_PHASE_INTEGRATE_END);
         launch_integrate_end(c, p->dt);
     }
@@ -534,6 +566,8 @@ int ge_create(ge_ctx **out, const ge_config *cfg) {
     c->sph = cfg->enable_sph != 0;
     c->render = cfg->enable_render != 0;
     c->thermal = cfg->enable_temperature != 0;
+    c->cons_cap = cfg->max_constraints;
+    ge_default_constraint_params(&c->cons_p);
     if (c->thermal && !c->sph) {  // the materials write the SPH fields
         delete c;
         return GE_E_INVALID;
@@ -719,6 +753,9 @@ int ge_step(ge_ctx *c, const ge_step_params *p, int32_t nsteps) {
     (void)hipSetDevice(c->device);
     if (p->accuracy) {
         if (int r = ensure_multipole_buffers(c)) return r;
+    }
+    if (p->constraints) {  // colouring (host round trips) before any frame or capture
+        if (int r = cons_prepare(c)) return r;
     }
     int32_t k = 0;
     if (p->use_graph && !c->prof && nsteps >= 3) {

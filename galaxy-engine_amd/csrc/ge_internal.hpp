@@ -11,6 +11,7 @@
 #include "../../include/ge_physics.h"
 #include "../../include/ge_render.h"
 #include "../../include/ge_thermal.h"
+#include "../../include/ge_constraints.h"
 
 namespace ge {
 
@@ -167,6 +168,17 @@ struct Buffers {
     float *node_temp = nullptr;
     uint32_t *node_parent = nullptr, *node_arrive = nullptr;
     ge_material *materials = nullptr;
+    // particle constraints (max_constraints): the constraints and their
+    // end indices (double-buffered for the compaction), colours, keep flags,
+    // block counts, device count / frame counter / colouring counter, the
+    // colour order and colour starts, sort words and their histogram
+    ge_constraint *cons = nullptr, *cons_tmp = nullptr;
+    uint2 *cidx = nullptr, *cidx_tmp = nullptr;
+    uint32_t *ccolor = nullptr, *ccolor_tmp = nullptr, *ckeep = nullptr, *cblock = nullptr;
+    uint32_t *ccount = nullptr, *cframe = nullptr, *cleft = nullptr;
+    uint32_t *corder = nullptr, *cstart = nullptr;
+    uint64_t *cw_a = nullptr, *cw_b = nullptr;
+    uint32_t *chist = nullptr;
 };
 
 }  // namespace ge
@@ -187,6 +199,11 @@ struct ge_ctx {
     bool thermal = false;          // temperature fields allocated
     ge_thermal_params thermal_p{};  // ge_set_thermal
     uint32_t nmat = 0;             // materials in b.materials
+    uint32_t cons_cap = 0;         // max_constraints
+    uint32_t cons_bound = 0;       // upper bound of the device constraint count
+    bool cons_colored = true;      // colours match the constraints
+    uint32_t ncolors = 0;
+    ge_constraint_params cons_p{};
     bool debug = true;             // keep node geometry for ge_get_nodes
     bool have_tree = false;
     bool have_grid = false;
@@ -254,6 +271,9 @@ int sph_count_into(ge_ctx *c, int search, float cell, float radius, int32_t *out
 void launch_thermal_fill(ge_ctx *c, const State &s, uint32_t at, uint32_t cnt);
 void launch_node_temps(ge_ctx *c);
 void launch_temperature(ge_ctx *c, const ge_step_params *p);
+// constraints.hip
+int cons_prepare(ge_ctx *c);
+int cons_solve(ge_ctx *c, const ge_step_params *p, bool map_done);
 // render.hip
 void launch_render_fill(hipStream_t st, uint32_t *color, uint32_t *pcolor, uint32_t *scolor, float *size,
                         int32_t *nbrs, uint32_t at, uint32_t cnt, uint32_t rgba, float sz);

@@ -47,7 +47,7 @@ PARTICLE_ORDER = ["x", "y", "vx", "vy", "ax", "ay", "pvx", "pvy", "mass", "flags
 class GeConfig(C.Structure):
     _fields_ = [("capacity", C.c_uint32), ("device", C.c_int32), ("enable_sph", C.c_int32),
                 ("max_systems", C.c_uint32), ("enable_render", C.c_int32), ("enable_temperature", C.c_int32),
-                ("reserved", C.c_uint32 * 2)]
+                ("max_constraints", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 class GeParticles(C.Structure):
@@ -86,7 +86,7 @@ class StepParams(C.Structure):
                 ("leaf_capacity", C.c_uint32), ("walk_block", C.c_uint32), ("use_graph", C.c_int32),
                 ("sph_search", C.c_int32), ("sph_search_radius", C.c_float),
                 ("domain_w", C.c_float), ("domain_h", C.c_float), ("vertical_gravity", C.c_float),
-                ("boundary_friction", C.c_float), ("reserved", C.c_uint32 * 2)]
+                ("boundary_friction", C.c_float), ("constraints", C.c_int32), ("reserved", C.c_uint32 * 1)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -218,7 +218,8 @@ def _uptr(a):
 class Context:
     """One device context (ge_ctx)."""
 
-    def __init__(self, capacity, sph=False, device=0, max_systems=1, render=False, temperature=False):
+    def __init__(self, capacity, sph=False, device=0, max_systems=1, render=False, temperature=False,
+                 max_constraints=0):
         self.lib = load_library()
         cfg = GeConfig()
         self.lib.ge_default_config(C.byref(cfg))
@@ -228,6 +229,7 @@ class Context:
         cfg.max_systems = int(max_systems)
         cfg.enable_render = 1 if render else 0
         cfg.enable_temperature = 1 if temperature else 0
+        cfg.max_constraints = int(max_constraints)
         self.render = bool(render)
         self.temperature = bool(temperature)
         h = C.c_void_p()

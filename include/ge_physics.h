@@ -56,7 +56,8 @@ typedef struct {
     uint32_t max_systems; /* independent systems one context can hold (>= 1) */
     int32_t enable_render; /* keep the render fields of ge_render.h on the device */
     int32_t enable_temperature; /* keep the temperature fields of ge_thermal.h (needs enable_sph) */
-    uint32_t reserved[2];
+    uint32_t max_constraints;   /* particle constraints the context can hold (ge_constraints.h) */
+    uint32_t reserved[1];
 } ge_config;
 
 /* Structure-of-arrays view of the ParticlePhysics fields on this path
@@ -127,7 +128,8 @@ typedef struct {
     float domain_w, domain_h; /* myVar.domainSize (parameters.h:97) */
     float vertical_gravity;   /* myVar.verticalGravity (ground mode) */
     float boundary_friction;  /* myVar.boundaryFriction (ground mode) */
-    uint32_t reserved[2];
+    int32_t constraints;      /* run Physics::constraints after PCISPH (ge_constraints.h) */
+    uint32_t reserved[1];
 } ge_step_params;
 
 /* Quadtree node as laid out by the reference (GE/include/Physics/quadtree.h:

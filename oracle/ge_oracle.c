@@ -959,3 +959,129 @@ void geo_temperature(geo_state *s, const geo_material *mats, int32_t nmat, float
         s->flags[i] = f | (hot ? GEO_HOT_POINT : 0u) | (solid ? GEO_SOLIDIFIED : 0u);
     }
 }
+
+/* --------------------------------------------------------- constraints --- */
+static int64_t index_of_id(const geo_state *s, uint32_t id) {
+    for (int32_t i = 0; i < s->n; ++i)
+        if (s->id[i] == id) return i;
+    return -1;
+}
+
+typedef struct { uint32_t color, index; } order_item;
+static int order_cmp(const void *a, const void *b) {
+    const order_item *x = (const order_item *)a, *y = (const order_item *)b;
+    if (x->color != y->color) return (x->color > y->color) - (x->color < y->color);
+    return (x->index > y->index) - (x->index < y->index);
+}
+
+int32_t geo_constraints(geo_state *s, geo_constraint *cons, uint32_t *color, int32_t m, const geo_cparams *p,
+                        const geo_material *mats, int32_t nmat, uint32_t *frame_count) {
+    /* idToIndexTable and the removal of constraints with a missing end or
+     * already broken (physics.cpp:777-817) */
+    int64_t *ia = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+    int64_t *ib = (int64_t *)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+    int32_t k = 0;
+    for (int32_t c = 0; c < m; ++c) {
+        const int64_t a = index_of_id(s, cons[c].id1), b = index_of_id(s, cons[c].id2);
+        if (a < 0 || b < 0 || cons[c].is_broken) continue;
+        cons[k] = cons[c];
+        color[k] = color[c];
+        ia[k] = a;
+        ib[k] = b;
+        ++k;
+    }
+    m = k;
+    if (m == 0) { free(ia); free(ib); return 0; }
+    *frame_count += 1;
+    /* the triangle rule (physics.cpp:823-862): adjacency of unbroken constraints */
+    if (*frame_count % 5 == 0) {
+        uint32_t **adj = (uint32_t **)calloc((size_t)s->n, sizeof(uint32_t *));
+        uint32_t *deg = (uint32_t *)calloc((size_t)s->n, sizeof(uint32_t));
+        uint32_t *cap = (uint32_t *)calloc((size_t)s->n, sizeof(uint32_t));
+        for (int32_t c = 0; c < m; ++c) {
+            if (cons[c].is_broken) continue;
+            const int64_t e[2] = {ia[c], ib[c]};
+            for (int q = 0; q < 2; ++q) {
+                const int64_t v = e[q], o = e[1 - q];
+                if (deg[v] == cap[v]) {
+                    cap[v] = cap[v] ? 2 * cap[v] : 4;
+                    adj[v] = (uint32_t *)realloc(adj[v], sizeof(uint32_t) * cap[v]);
+                }
+                adj[v][deg[v]++] = (uint32_t)o;
+            }
+        }
+        for (int32_t c = 0; c < m; ++c) {
+            if (cons[c].is_broken) continue;
+            const int64_t A = ia[c], B = ib[c];
+            int tri = 0;
+            for (uint32_t x = 0; x < deg[A] && !tri; ++x)
+                for (uint32_t y = 0; y < deg[B]; ++y)
+                    if (adj[A][x] == adj[B][y]) { tri = 1; break; }
+            if (!tri) cons[c].is_broken = 1;
+        }
+        for (int32_t v = 0; v < s->n; ++v) free(adj[v]);
+        free(adj); free(deg); free(cap);
+    }
+    order_item *ord = (order_item *)malloc(sizeof(order_item) * (size_t)m);
+    for (int32_t c = 0; c < m; ++c) { ord[c].color = color[c]; ord[c].index = (uint32_t)c; }
+    qsort(ord, (size_t)m, sizeof(order_item), order_cmp);
+    for (int step = 0; step < 15; ++step) {
+        for (int32_t t = 0; t < m; ++t) {
+            const int32_t c = (int32_t)ord[t].index;
+            geo_constraint *K = &cons[c];
+            if (K->is_broken) continue;
+            const int64_t i = ia[c], j = ib[c];
+            const geo_material *mi = s->label ? find_material(mats, nmat, s->label[i]) : NULL;
+            const geo_material *mj = s->label ? find_material(mats, nmat, s->label[j]) : NULL;
+            float dx = s->x[j] - s->x[i], dy = s->y[j] - s->y[i];
+            if (p->periodic) {
+                dx = fmodf(dx + p->domain_w * 1.5f, p->domain_w) - p->domain_w * 0.5f;
+                dy = fmodf(dy + p->domain_h * 1.5f, p->domain_h) - p->domain_h * 0.5f;
+            }
+            const float len = sqrtf(dx * dx + dy * dy);  /* glm::length */
+            if (len < 0.0001f) continue;
+            const float dirx = dx / len, diry = dy / len;
+            K->displacement = len - K->rest_length;
+            if (!p->unbreakable && mi && mj) {
+                if (!mi->is_plastic || !mj->is_plastic) {
+                    K->is_plastic = 0;
+                    const float limit = (K->resistance * p->resistance_mult) * K->rest_length;
+                    if (fabsf(K->displacement) >= limit) K->is_broken = 1;
+                } else {
+                    K->is_plastic = 1;
+                    if (fabsf(K->displacement) >= K->plasticity_point * K->original_length)
+                        K->rest_length += K->displacement;
+                    const float brk = (K->original_length + K->original_length * (K->resistance * p->resistance_mult)) *
+                                      (mi->constraint_plastic_point_mult + mj->constraint_plastic_point_mult) * 0.5f;
+                    if (fabsf(K->rest_length) >= brk) K->is_broken = 1;
+                }
+                if ((s->flags[i] | s->flags[j]) & GEO_HOT_POINT) K->is_broken = 1;
+            }
+            if (p->forces && !K->is_broken) {
+                const float m1 = s->m[i], m2 = s->m[j];
+                const float sd = K->stiffness * K->displacement;
+                const float sfx = ((sd * dirx) * m1) * p->stiffness_mult, sfy = ((sd * diry) * m1) * p->stiffness_mult;
+                const float rvx = s->vx[j] - s->vx[i], rvy = s->vy[j] - s->vy[i];
+                const float dd = -p->damping * (rvx * dirx + rvy * diry);
+                const float dfx = (dd * dirx) * m1, dfy = (dd * diry) * m1;
+                const float tx = sfx + dfx, ty = sfy + dfy;
+                s->ax[i] = s->ax[i] + tx / m1;
+                s->ay[i] = s->ay[i] + ty / m1;
+                s->ax[j] = s->ax[j] - tx / m2;
+                s->ay[j] = s->ay[j] - ty / m2;
+                const float cf = K->stiffness * p->correction_ratio * p->stiffness_mult;
+                const float cx = (dirx * K->displacement) * cf, cy = (diry * K->displacement) * cf;
+                const float ms = m1 + m2;
+                const float wi = m2 / ms, wj = m1 / ms;
+                s->x[i] = s->x[i] + cx * wi;
+                s->y[i] = s->y[i] + cy * wi;
+                s->x[j] = s->x[j] - cx * wj;
+                s->y[j] = s->y[j] - cy * wj;
+            }
+        }
+    }
+    free(ord);
+    free(ia);
+    free(ib);
+    return m;
+}

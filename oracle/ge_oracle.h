@@ -188,6 +188,32 @@ void geo_sph_constants(float h, float out[3]); /* {W volume, spiky scale, laplac
  * Permutes every state array into Morton order.  Returns 0 or negative. */
 int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps);
 
+/* ---- constraints ---- */
+/* ParticleConstraint (GE/include/Physics/constraint.h:3-14) */
+typedef struct {
+    uint32_t id1, id2;
+    float rest_length, original_length, stiffness, resistance, displacement, plasticity_point;
+    uint8_t is_broken, is_plastic, pad[2];
+} geo_constraint;
+
+typedef struct {
+    float stiffness_mult, resistance_mult; /* globalConstraintStiffnessMult, globalConstraintResistance */
+    int32_t unbreakable;                   /* unbreakableConstraints */
+    int32_t forces;                        /* timeFactor > 0 && gridExists */
+    float damping, correction_ratio;       /* globalConstraintDamping, stiffCorrectionRatio */
+    int32_t periodic;
+    float domain_w, domain_h;
+} geo_cparams;
+
+/* Physics::constraints (GE/src/Physics/physics.cpp:768-950) run serially:
+ * removal (order kept), *frame_count += 1 when any remain, the triangle rule
+ * when it is a multiple of 5, then 15 substeps visiting the constraints in
+ * the order of ascending (color, index).  The serial loop is one of the
+ * outcomes of the reference's parallel loop.  `color` travels with the
+ * constraints through the removal.  Returns the remaining count. */
+int32_t geo_constraints(geo_state *s, geo_constraint *cons, uint32_t *color, int32_t m, const geo_cparams *p,
+                        const geo_material *mats, int32_t nmat, uint32_t *frame_count);
+
 /* ---- temperature (isTempEnabled) ---- */
 /* Node::gridTemp (GE/include/Physics/quadtree.h:53-100): leaves sum their
  * particles' temp in order, internal nodes their children's in the order

@@ -152,6 +152,9 @@ def load():
     lib.geo_walk_heat.restype = None
     lib.geo_walk_heat.argtypes = [C.c_int32, f32p, f32p, C.c_void_p, C.c_int32, f32p, C.c_float, C.c_float,
                                   C.c_float, C.c_float, C.c_void_p, f32p]
+    lib.geo_constraints.restype = C.c_int32
+    lib.geo_constraints.argtypes = [C.POINTER(_State), C.c_void_p, C.POINTER(C.c_uint32), C.c_int32,
+                                    C.POINTER(_CParams), C.c_void_p, C.c_int32, C.POINTER(C.c_uint32)]
     lib.geo_temperature.restype = None
     lib.geo_temperature.argtypes = [C.POINTER(_State), C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_float]
     _lib = lib
@@ -270,6 +273,40 @@ def neighbor_amount(x, y, flags, cell=3.0, radius=3.5):
     out = np.zeros(len(x), np.uint32)
     load().geo_neighbor_amount(len(x), _f(x), _f(y), _u(fl), cell, radius, _u(out))
     return out
+
+
+CONSTRAINT_DTYPE = np.dtype([("id1", "<u4"), ("id2", "<u4"), ("rest_length", "<f4"), ("original_length", "<f4"),
+                             ("stiffness", "<f4"), ("resistance", "<f4"), ("displacement", "<f4"),
+                             ("plasticity_point", "<f4"), ("is_broken", "u1"), ("is_plastic", "u1"),
+                             ("pad", "u1", (2,))])
+
+
+class _CParams(C.Structure):
+    _fields_ = [("stiffness_mult", C.c_float), ("resistance_mult", C.c_float), ("unbreakable", C.c_int32),
+                ("forces", C.c_int32), ("damping", C.c_float), ("correction_ratio", C.c_float),
+                ("periodic", C.c_int32), ("domain_w", C.c_float), ("domain_h", C.c_float)]
+
+
+def constraints(st, cons, colors, materials=(), frame_count=0, **prm):
+    """Physics::constraints once, serially in (colour, index) order.  st: dict of
+    arrays (x, y, vx, vy, ax, ay, mass, flags, id, optional label); returns
+    (state dict, remaining constraints, their colours, frame_count)."""
+    temperature = st.get("label") is not None
+    s = State(st, sph=False, temperature=temperature)
+    cs = np.ascontiguousarray(cons, dtype=CONSTRAINT_DTYPE).copy()
+    col = np.ascontiguousarray(colors, dtype=np.uint32).copy()
+    p = _CParams()
+    d = dict(stiffness_mult=1.0, resistance_mult=1.0, unbreakable=0, forces=1, damping=0.001,
+             correction_ratio=0.013333, periodic=0, domain_w=0.0, domain_h=0.0)
+    d.update(prm)
+    for k, v in d.items():
+        setattr(p, k, v)
+    mats = list(materials)
+    arr = (Material * max(len(mats), 1))(*mats)
+    fc = C.c_uint32(frame_count)
+    m = load().geo_constraints(C.byref(s._s), cs.ctypes.data, _u(col), len(cs), C.byref(p),
+                               C.addressof(arr) if mats else None, len(mats), C.byref(fc))
+    return s.a, cs[:m].copy(), col[:m].copy(), fc.value
 
 
 def node_temps(nodes, temp):
