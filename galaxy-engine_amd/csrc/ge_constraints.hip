@@ -517,6 +517,12 @@ void ge_default_constraint_params(ge_constraint_params *p) {
     p->correction_ratio = 0.013333f;  // physics.h:27
 }
 
+int ge_constraints_capacity(const ge_ctx *c, uint32_t *cap) {
+    if (!c || !cap) return GE_E_INVALID;
+    *cap = c->cons_cap;
+    return GE_OK;
+}
+
 int ge_set_constraint_params(ge_ctx *c, const ge_constraint_params *p) {
     if (!c || !p) return GE_E_INVALID;
     c->cons_p = *p;

@@ -389,6 +389,33 @@ void encode(ge_scene *s, const Soa &o, uint32_t n, bool render, bool thermal) {
     if (n && max_id + 1 > s->gid) s->gid = max_id + 1;  // globalId = maxId + 1 (saveSystem.h:191-196)
 }
 
+// the constraints at the start of the tail: uint32 count, then count raw
+// ParticleConstraint records (saveSystem.cpp:421-429)
+bool tail_constraints(const ge_scene *s, std::vector<ge_constraint> &out) {
+    out.clear();
+    if (s->tail.size() < 4) return true;  // an empty tail: no constraints
+    const uint32_t m = rd<uint32_t>(s->tail.data(), 0);
+    if (s->tail.size() < 4 + (uint64_t)m * sizeof(ge_constraint)) return false;
+    out.resize(m);
+    if (m) std::memcpy(out.data(), s->tail.data() + 4, (size_t)m * sizeof(ge_constraint));
+    return true;
+}
+
+bool set_tail_constraints(ge_scene *s, const std::vector<ge_constraint> &cons) {
+    uint32_t old = 0;
+    if (s->tail.size() >= 4) {
+        old = rd<uint32_t>(s->tail.data(), 0);
+        if (s->tail.size() < 4 + (uint64_t)old * sizeof(ge_constraint)) return false;
+    }
+    std::vector<uint8_t> t(4 + cons.size() * sizeof(ge_constraint));
+    wr<uint32_t>(t.data(), 0, (uint32_t)cons.size());
+    if (!cons.empty()) std::memcpy(t.data() + 4, cons.data(), cons.size() * sizeof(ge_constraint));
+    const size_t rest = s->tail.size() >= 4 ? 4 + (size_t)old * sizeof(ge_constraint) : 0;
+    t.insert(t.end(), s->tail.begin() + (std::ptrdiff_t)rest, s->tail.end());
+    s->tail.swap(t);
+    return true;
+}
+
 template <typename P>
 P view_of(Soa &o) {
     P p{};
@@ -634,6 +661,26 @@ int ge_scene_set_params3(ge_scene *s, const ge3_step_params *p) {
     return GE_OK;
 }
 
+int ge_scene_constraint_params(const ge_scene *s, ge_constraint_params *c, int32_t *enabled) {
+    if (!s || !c) return GE_E_INVALID;
+    ge_default_constraint_params(c);
+    bool b = false;
+    if (enabled) *enabled = get_bool(s, "ParticleConstraints", b) && b ? 1 : 0;
+    if (get_bool(s, "UnbreakableConstraints", b)) c->unbreakable = b ? 1 : 0;
+    get_float(s, "ConstraintsStiffMultiplier", c->stiffness_mult);
+    get_float(s, "ConstraintsResistMultiplier", c->resistance_mult);
+    return GE_OK;
+}
+
+int ge_scene_set_constraint_params(ge_scene *s, const ge_constraint_params *c, int32_t enabled) {
+    if (!s || !c) return GE_E_INVALID;
+    set_kv(s, "ParticleConstraints", enabled ? "true" : "false");
+    set_kv(s, "UnbreakableConstraints", c->unbreakable ? "true" : "false");
+    set_kv(s, "ConstraintsStiffMultiplier", fmt_float(c->stiffness_mult));
+    set_kv(s, "ConstraintsResistMultiplier", fmt_float(c->resistance_mult));
+    return GE_OK;
+}
+
 int ge_scene_thermal_params(const ge_scene *s, ge_thermal_params *t) {
     if (!s || !t) return GE_E_INVALID;
     ge_default_thermal_params(t);
@@ -670,6 +717,15 @@ int ge_scene_upload(const ge_scene *s, ge_ctx *ctx) {
     if (r == GE_OK && (r = ge_thermal_enabled(ctx, &thermal)) == GE_OK && thermal) {
         ge_thermal_fields f = o.thermal();
         r = ge_thermal_upload(ctx, &f);
+    }
+    uint32_t ccap = 0;
+    if (r == GE_OK && (r = ge_constraints_capacity(ctx, &ccap)) == GE_OK && ccap) {
+        std::vector<ge_constraint> cons;
+        if (!tail_constraints(s, cons)) {
+            s->err = "constraint section of the tail truncated";
+            return GE_E_INVALID;
+        }
+        r = ge_constraints_upload(ctx, cons.data(), (uint32_t)cons.size());
     }
     if (r != GE_OK) s->err = ge_last_error(ctx);
     return r;
@@ -717,6 +773,17 @@ int ge_scene_download(ge_scene *s, ge_ctx *ctx) {
     if (thermal) {
         ge_thermal_fields f = o.thermal();
         if (int r = ge_thermal_download(ctx, &f)) return fail(s, r, ge_last_error(ctx));
+    }
+    uint32_t ccap = 0;
+    if (int r = ge_constraints_capacity(ctx, &ccap)) return fail(s, r, ge_last_error(ctx));
+    if (ccap) {
+        uint32_t m = 0;
+        if (int r = ge_constraints_download(ctx, nullptr, 0, &m)) return fail(s, r, ge_last_error(ctx));
+        std::vector<ge_constraint> cons(m);
+        if (m) {
+            if (int r = ge_constraints_download(ctx, cons.data(), m, &m)) return fail(s, r, ge_last_error(ctx));
+        }
+        if (!set_tail_constraints(s, cons)) return fail(s, GE_E_INVALID, "constraint section of the tail truncated");
     }
     encode(s, o, n, render != 0, thermal != 0);
     return GE_OK;

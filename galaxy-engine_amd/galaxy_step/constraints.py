@@ -7,7 +7,7 @@ import numpy as np
 from .lib import GE_OK, GeError, load_library
 
 CONSTRAINT_EXPORTS = [
-    "ge_default_constraint_params", "ge_set_constraint_params", "ge_constraints_upload", "ge_constraints_download",
+    "ge_default_constraint_params", "ge_constraints_capacity", "ge_set_constraint_params", "ge_constraints_upload", "ge_constraints_download",
     "ge_constraints_clear", "ge_constraints_solve", "ge_constraints_order",
 ]
 # ParticleConstraint, the 36-byte record of the save files
@@ -46,6 +46,7 @@ def load_constraint_library():
     u32p = P(C.c_uint32)
     sig = {
         "ge_default_constraint_params": (None, [P(ConstraintParams)]),
+        "ge_constraints_capacity": (C.c_int, [vp, u32p]),
         "ge_set_constraint_params": (C.c_int, [vp, P(ConstraintParams)]),
         "ge_constraints_upload": (C.c_int, [vp, vp, C.c_uint32]),
         "ge_constraints_download": (C.c_int, [vp, vp, C.c_uint32, u32p]),

@@ -5,6 +5,7 @@ import ctypes as C
 
 from .lib import GE_OK, GeError, StepParams, load_library
 from .lib3d import Step3Params, load_library3
+from .constraints import ConstraintParams, load_constraint_library
 from .thermal import ThermalParams, load_thermal_library
 
 SCENE_EXPORTS = [
@@ -12,6 +13,7 @@ SCENE_EXPORTS = [
     "ge_scene_value", "ge_scene_set_value", "ge_scene_records", "ge_scene_params", "ge_scene_params3",
     "ge_scene_set_params", "ge_scene_set_params3", "ge_scene_upload", "ge_scene_upload3", "ge_scene_download",
     "ge_scene_download3", "ge_scene_save", "ge_scene_thermal_params", "ge_scene_set_thermal_params",
+    "ge_scene_constraint_params", "ge_scene_set_constraint_params",
 ]
 
 
@@ -29,6 +31,7 @@ def load_scene_library():
     lib = load_library()
     load_library3()
     load_thermal_library()
+    load_constraint_library()
     if _ready:
         return lib
     P = C.POINTER
@@ -53,6 +56,8 @@ def load_scene_library():
         "ge_scene_save": (C.c_int, [vp, C.c_char_p]),
         "ge_scene_thermal_params": (C.c_int, [vp, P(ThermalParams)]),
         "ge_scene_set_thermal_params": (C.c_int, [vp, P(ThermalParams)]),
+        "ge_scene_constraint_params": (C.c_int, [vp, P(ConstraintParams), P(C.c_int32)]),
+        "ge_scene_set_constraint_params": (C.c_int, [vp, P(ConstraintParams), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -141,6 +146,16 @@ class Scene:
         t = ThermalParams()
         self._check(self.lib.ge_scene_thermal_params(self.h, C.byref(t)))
         return t
+
+    def constraint_params(self):
+        """(ConstraintParams, ParticleConstraints enabled)."""
+        c = ConstraintParams()
+        on = C.c_int32()
+        self._check(self.lib.ge_scene_constraint_params(self.h, C.byref(c), C.byref(on)))
+        return c, bool(on.value)
+
+    def set_constraint_params(self, c, enabled):
+        self._check(self.lib.ge_scene_set_constraint_params(self.h, C.byref(c), 1 if enabled else 0))
 
     def set_thermal_params(self, t):
         self._check(self.lib.ge_scene_set_thermal_params(self.h, C.byref(t)))

@@ -55,6 +55,8 @@ typedef struct {
 } ge_constraint_params;
 
 void ge_default_constraint_params(ge_constraint_params *c);
+/* *cap = max_constraints of the context */
+int ge_constraints_capacity(const ge_ctx *ctx, uint32_t *cap);
 int ge_set_constraint_params(ge_ctx *ctx, const ge_constraint_params *c);
 /* replaces the context's constraints (particleConstraints) */
 int ge_constraints_upload(ge_ctx *ctx, const ge_constraint *c, uint32_t count);

@@ -29,6 +29,7 @@
 #include "ge_physics3d.h"
 #include "ge_render.h"
 #include "ge_thermal.h"
+#include "ge_constraints.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -80,17 +81,25 @@ int ge_scene_set_params3(ge_scene *s, const ge3_step_params *p);
 int ge_scene_thermal_params(const ge_scene *s, ge_thermal_params *t);
 int ge_scene_set_thermal_params(ge_scene *s, const ge_thermal_params *t);
 
+/* the constraint solver's variables from ParticleConstraints,
+ * UnbreakableConstraints, ConstraintsStiffMultiplier and
+ * ConstraintsResistMultiplier (saveSystem.cpp:198-207) over the defaults of
+ * ge_default_constraint_params (*enabled gets ParticleConstraints), and back */
+int ge_scene_constraint_params(const ge_scene *s, ge_constraint_params *c, int32_t *enabled);
+int ge_scene_set_constraint_params(ge_scene *s, const ge_constraint_params *c, int32_t enabled);
+
 /* particles into a context (ge_upload / ge3_upload): pos, vel, acc, prevVel,
  * mass, id, the SPH fields when the context has them, the flags isPinned,
  * isSPH, isBeingDrawn, isDarkMatter, uniqueColor, isSolid, isHotPoint,
  * hasSolidified (GE_FLAG_*), color, pColor, sColor, size, neighbors when it
- * keeps render fields (ge_render.h) and temp, sphLabel, ke, prevKe when it
- * keeps temperature fields (ge_thermal.h) */
+ * keeps render fields (ge_render.h), temp, sphLabel, ke, prevKe when it
+ * keeps temperature fields (ge_thermal.h), and the 2D file's constraints
+ * (the start of the tail) when it holds constraints (ge_constraints.h) */
 int ge_scene_upload(const ge_scene *s, ge_ctx *ctx);
 int ge_scene_upload3(const ge_scene *s, ge3_ctx *ctx);
 /* the context's particles back into records, in the device order: the
- * device fields from the context (the render and temperature fields too
- * when it keeps them),
+ * device fields from the context (the render and temperature fields and the
+ * constraints too when it keeps them),
  * every other field from the record of the same id (a new id gets the
  * reference's defaults for a spawned particle, particle.h:35-63, 100-130) */
 int ge_scene_download(ge_scene *s, ge_ctx *ctx);

@@ -261,3 +261,55 @@ def test_scene_temperature_through_the_device(tmp_path):
     o = np.argsort(ref["id"])
     g = _by_id(got)
     np.testing.assert_array_equal(g["temp"], ref["temp"][o])
+
+
+def test_scene_constraints_through_the_device(tmp_path):
+    # the 2D tail starts with the constraints (count + raw 36-byte records);
+    # a context that holds constraints takes them, the frame solves them with
+    # the file's constraint variables, and the tail gets the device's
+    # constraints back with the rest of the tail (walls, shapes, lights) intact
+    from galaxy_step import constraints as GC
+    from test_gpu_constraints import _lattice
+    st, cons = _lattice(20, 4)
+    n = len(st["x"])
+    recs = random_records(n, 2, 11)
+    recs["pos"] = np.stack([st["x"], st["y"]], 1)
+    recs["vel"] = np.stack([st["vx"], st["vy"]], 1)
+    for f in ("acc", "prevVel", "predPos", "pressF"):
+        recs[f] = 0.0
+    recs["mass"] = st["mass"]
+    recs["id"] = st["id"]
+    for f in ("isPinned", "isSPH", "isBeingDrawn", "isDarkMatter", "isHotPoint"):
+        recs[f] = 0
+    rest_of_tail = bytes(range(40))
+    tail = np.array([len(cons)], np.uint32).tobytes() + cons.tobytes() + rest_of_tail
+    yaml = ("Theta: 0.699999988\n---\nSoftening: 1\n---\nGravityMultiplier: 0\n---\nParticleConstraints: true\n---\n"
+            "ConstraintsStiffMultiplier: 0.5\n---\nis3DMode: false\n")
+    src = tmp_path / "Save_c.bin"
+    head = np.array([190, 1, 2, 3, n], np.uint32).tobytes()
+    src.write_bytes(yaml.encode() + SEP + head + recs.tobytes() + tail)
+    with Scene.load(src) as sc:
+        cp, on = sc.constraint_params()
+        assert on and cp.stiffness_mult == 0.5
+        p = sc.params()
+        p.constraints = 1
+        p.direct_below = 0
+        with Context(n, max_constraints=len(cons) + 8) as c:
+            sc.upload(c)
+            np.testing.assert_array_equal(GC.download(c), cons)
+            GC.set_params(c, cp)
+            c.step(p, 3)
+            c.sync()
+            dev = GC.download(c)
+            sc.download(c)
+        out = tmp_path / "Save_c_out.bin"
+        sc.save(out)
+    raw = out.read_bytes()
+    body = raw[raw.index(SEP) + len(SEP):]
+    rec_end = 20 + n * record_dtype(2).itemsize
+    m = int(np.frombuffer(body[rec_end:rec_end + 4], np.uint32)[0])
+    assert m == len(dev)
+    got = np.frombuffer(body[rec_end + 4:rec_end + 4 + 36 * m], GC.CONSTRAINT_DTYPE)
+    np.testing.assert_array_equal(got, dev)
+    assert body[rec_end + 4 + 36 * m:] == rest_of_tail
+    assert not np.array_equal(dev["displacement"], cons["displacement"])  # the solver ran
