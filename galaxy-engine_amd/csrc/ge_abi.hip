@@ -56,6 +56,7 @@ int alloc_state(ge_ctx *c, State &s, uint32_t cap, bool sph) {
         GE_HIP_CHECK(c, dalloc(s.scolor, cap));
         GE_HIP_CHECK(c, dalloc(s.size, cap));
         GE_HIP_CHECK(c, dalloc(s.nbrs, cap));
+        GE_HIP_CHECK(c, dalloc(s.prev_size, cap));
     }
     return GE_OK;
 }
@@ -67,7 +68,7 @@ void free_state(State &s) {
     for (float **f : fs) dfree(*f);
     dfree(s.flags);
     dfree(s.id);
-    dfree(s.color); dfree(s.pcolor); dfree(s.scolor); dfree(s.size); dfree(s.nbrs);
+    dfree(s.color); dfree(s.pcolor); dfree(s.scolor); dfree(s.size); dfree(s.nbrs); dfree(s.prev_size);
     dfree(s.temp); dfree(s.ke); dfree(s.prev_ke); dfree(s.label);
 }
 
@@ -87,6 +88,7 @@ void free_all(ge_ctx *c) {
     dfree(T.kind); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
     dfree(T.blist); dfree(T.blist_off); dfree(T.blist_cnt);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
+    merger_free(c);
     dfree(b.node_temp); dfree(b.node_parent); dfree(b.node_arrive); dfree(b.materials);
     dfree(b.cons); dfree(b.cons_tmp); dfree(b.cidx); dfree(b.cidx_tmp); dfree(b.ccolor); dfree(b.ccolor_tmp);
     dfree(b.ckeep); dfree(b.cblock); dfree(b.ccount); dfree(b.corder); dfree(b.cstart); dfree(b.cw_a);
@@ -290,7 +292,8 @@ int put_fields(ge_ctx *c, State &s, uint32_t at, uint32_t cnt, const ge_particle
             return r;
     }
     // ParticleRendering() defaults: white, size 1, no neighbours (particle.h:101-108)
-    if (c->render) launch_render_fill(c->stream, s.color, s.pcolor, s.scolor, s.size, s.nbrs, at, cnt, 0xFFFFFFFFu, 1.0f);
+    if (c->render) launch_render_fill(c->stream, s.color, s.pcolor, s.scolor, s.size, s.prev_size, s.nbrs, at, cnt,
+                                      0xFFFFFFFFu, 1.0f);
     if (c->thermal) launch_thermal_fill(c, s, at, cnt);  // temp 288, no material
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
@@ -1055,6 +1058,45 @@ int ge_spawn_galaxy(ge_ctx *c, const ge_galaxy_params *g, uint64_t seed, const f
     c->layout_version += 1;
     c->have_tree = false;
     c->have_grid = false;
+    return GE_OK;
+}
+
+void ge_default_merger_params(ge_merger_params *m) {
+    if (!m) return;
+    std::memset(m, 0, sizeof *m);
+    m->density_radius = 10.0f;     // globalLogic.cpp:1253-1254 once the merger runs
+    m->cell_size = 3.0f;           // neighborSearch.h:11
+    m->texture_half_size = 16.0f;  // parameters.h:161
+    m->size_multiplier = 0.6f;     // parameters.h:202
+}
+
+int ge_merger_cell_size(ge_ctx *c, const ge_merger_params *m, float *cell) {
+    if (!c || !m || !cell) return GE_E_INVALID;
+    if (!c->render) return fail(c, GE_E_INVALID, "the merger reads the render fields: create with enable_render");
+    (void)hipSetDevice(c->device);
+    return merger_cell_size(c, m, cell);
+}
+
+int ge_merge(ge_ctx *c, const ge_merger_params *m, uint32_t *removed) {
+    if (!c || !m) return GE_E_INVALID;
+    if (removed) *removed = 0;
+    if (!c->render) return fail(c, GE_E_INVALID, "the merger reads the render fields: create with enable_render");
+    if (c->nsys > 1) return fail(c, GE_E_UNSUPPORTED, "the merger runs on single-system contexts");
+    if (!(m->cell_size > 0.0f && m->density_radius >= 0.0f))
+        return fail(c, GE_E_INVALID, "cell_size must be > 0 and density_radius >= 0");
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    const uint32_t before = c->n;
+    std::vector<uint32_t> dels;
+    if (int r = merger_run(c, m, dels)) return r;
+    if (dels.empty()) return GE_OK;
+    uint32_t new_n = 0;
+    if (int r = merger_apply(c, dels, new_n)) return r;
+    if (int r = set_systems(c, 1, &new_n)) return fail(c, r, "HIP error");
+    c->layout_version += 1;
+    c->have_tree = false;
+    c->have_grid = false;
+    if (removed) *removed = before - new_n;
     return GE_OK;
 }
 

@@ -47,7 +47,7 @@ void free_all(ge3_ctx *c) {
         for (float **f : fs) dfree(*f);
         dfree(s.flags);
         dfree(s.id);
-        dfree(s.color); dfree(s.pcolor); dfree(s.scolor); dfree(s.size); dfree(s.nbrs);
+        dfree(s.color); dfree(s.pcolor); dfree(s.scolor); dfree(s.size); dfree(s.nbrs); dfree(s.prev_size);
     }
     dfree(b.pxyzm); dfree(b.keys); dfree(b.sk_a); dfree(b.sk_b); dfree(b.sv_a); dfree(b.sv_b); dfree(b.perm);
     dfree(b.hist); dfree(b.partials); dfree(b.bb); dfree(b.counters); dfree(b.inter);
@@ -122,6 +122,7 @@ int alloc_all(ge3_ctx *c) {
             GE3_CHECK(c, dalloc(s.scolor, cap));
             GE3_CHECK(c, dalloc(s.size, cap));
             GE3_CHECK(c, dalloc(s.nbrs, cap));
+            GE3_CHECK(c, dalloc(s.prev_size, cap));
         }
     }
     GE3_CHECK(c, dalloc(b.pxyzm, cap));
@@ -286,7 +287,8 @@ int put_fields(ge3_ctx *c, uint32_t n, const ge3_particles *p) {
         GE3_CHECK(c, hipStreamSynchronize(c->stream));
     }
     // ParticleRendering3D() defaults: white, size 1, no neighbours
-    if (c->render) launch_render_fill(c->stream, s.color, s.pcolor, s.scolor, s.size, s.nbrs, 0, n, 0xFFFFFFFFu, 1.0f);
+    if (c->render) launch_render_fill(c->stream, s.color, s.pcolor, s.scolor, s.size, s.prev_size, s.nbrs, 0, n,
+                                      0xFFFFFFFFu, 1.0f);
     GE3_CHECK(c, hipStreamSynchronize(c->stream));
     return GE_OK;
 }

@@ -12,6 +12,7 @@
 #include "../../include/ge_render.h"
 #include "../../include/ge_thermal.h"
 #include "../../include/ge_constraints.h"
+#include "../../include/ge_merger.h"
 
 namespace ge {
 
@@ -79,7 +80,7 @@ struct State {
     float *pred_dens = nullptr, *ppx = nullptr, *ppy = nullptr, *pfx = nullptr, *pfy = nullptr;
     // render fields (allocated with enable_render): RGBA8 colours, size, neighbour count
     uint32_t *color = nullptr, *pcolor = nullptr, *scolor = nullptr;
-    float *size = nullptr;
+    float *size = nullptr, *prev_size = nullptr;
     int32_t *nbrs = nullptr;
     // temperature fields (allocated with enable_temperature)
     float *temp = nullptr, *ke = nullptr, *prev_ke = nullptr;
@@ -181,6 +182,18 @@ struct Buffers {
     uint32_t *chist = nullptr;
 };
 
+// merger work buffers, allocated on the first ge_merge
+struct MergerBuf {
+    uint64_t *wa = nullptr, *wb = nullptr;
+    uint32_t *hist = nullptr;
+    float *rad = nullptr;
+    uint32_t *del = nullptr, *ecnt = nullptr, *eoff = nullptr, *edges = nullptr, *parent = nullptr;
+    float4 *part = nullptr;
+    uint32_t *grid = nullptr, *flag = nullptr, *dlist = nullptr, *bc = nullptr, *perm = nullptr;
+    size_t edge_cap = 0;
+    uint32_t cap = 0;
+};
+
 }  // namespace ge
 
 struct ge_ctx {
@@ -225,6 +238,7 @@ struct ge_ctx {
         const void *sort_a = nullptr, *grid_a = nullptr, *grid_b = nullptr;
     } graph;
     ge::Buffers b;
+    ge::MergerBuf merger;
     std::string err;
     // phase timing (ge_profile_enable)
     bool prof = false;
@@ -271,12 +285,17 @@ int sph_count_into(ge_ctx *c, int search, float cell, float radius, int32_t *out
 void launch_thermal_fill(ge_ctx *c, const State &s, uint32_t at, uint32_t cnt);
 void launch_node_temps(ge_ctx *c);
 void launch_temperature(ge_ctx *c, const ge_step_params *p);
+// merger.hip
+void merger_free(ge_ctx *c);
+int merger_cell_size(ge_ctx *c, const ge_merger_params *mp, float *cell);
+int merger_run(ge_ctx *c, const ge_merger_params *mp, std::vector<uint32_t> &dels);
+int merger_apply(ge_ctx *c, const std::vector<uint32_t> &dels, uint32_t &new_n);
 // constraints.hip
 int cons_prepare(ge_ctx *c);
 int cons_solve(ge_ctx *c, const ge_step_params *p, bool map_done);
 // render.hip
 void launch_render_fill(hipStream_t st, uint32_t *color, uint32_t *pcolor, uint32_t *scolor, float *size,
-                        int32_t *nbrs, uint32_t at, uint32_t cnt, uint32_t rgba, float sz);
+                        float *prev_size, int32_t *nbrs, uint32_t at, uint32_t cnt, uint32_t rgba, float sz);
 void launch_spawn_galaxy(ge_ctx *c, const State &s, uint32_t at, const ge_galaxy_params *g, uint64_t seed,
                          const float *d_uniforms);
 

@@ -29,7 +29,7 @@ struct State3 {
     float *pfx = nullptr, *pfy = nullptr, *pfz = nullptr;
     // render fields (allocated with enable_render): RGBA8 colours, size, neighbour count
     uint32_t *color = nullptr, *pcolor = nullptr, *scolor = nullptr;
-    float *size = nullptr;
+    float *size = nullptr, *prev_size = nullptr;
     int32_t *nbrs = nullptr;
 };
 

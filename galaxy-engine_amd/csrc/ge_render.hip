@@ -30,8 +30,8 @@ constexpr uint32_t kWhite = 0xFFFFFFFFu;
 
 __global__ __launch_bounds__(kBlock) void k_render_fill(uint32_t *__restrict__ color, uint32_t *__restrict__ pcolor,
                                                         uint32_t *__restrict__ scolor, float *__restrict__ size,
-                                                        int32_t *__restrict__ nbrs, uint32_t at, uint32_t cnt,
-                                                        uint32_t rgba, float sz) {
+                                                        float *__restrict__ prev_size, int32_t *__restrict__ nbrs,
+                                                        uint32_t at, uint32_t cnt, uint32_t rgba, float sz) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= cnt) return;
     const uint32_t i = at + k;
@@ -39,6 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_render_fill(uint32_t *__restrict__ c
     pcolor[i] = kWhite;
     scolor[i] = kWhite;
     size[i] = sz;
+    prev_size[i] = sz;  // previousSize = size (particle.h:121)
     nbrs[i] = 0;
 }
 
@@ -454,6 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_spawn_galaxy(State s, uint32_t at, G
         s.pcolor[i] = kWhite;
         s.scolor[i] = kWhite;
         s.size[i] = 0.125f;
+        s.prev_size[i] = 0.125f;
         s.nbrs[i] = 0;
     }
 }
@@ -496,7 +498,7 @@ int check_visuals(const ge_visual_params *v, std::string &err, bool is3d) {
 
 template <class Ctx>
 int render_copy(Ctx *c, const ge_render_fields *f, bool up, uint32_t *color, uint32_t *pcolor, uint32_t *scolor,
-                float *size, int32_t *nbrs) {
+                float *size, int32_t *nbrs, float *prev_size) {
     const size_t n = c->n;
     if (n == 0) return GE_OK;
     auto mv = [&](void *dev, void *host, size_t bytes) -> int {
@@ -508,7 +510,8 @@ int render_copy(Ctx *c, const ge_render_fields *f, bool up, uint32_t *color, uin
     };
     int r = 0;
     if ((r = mv(color, f->color, 4 * n)) || (r = mv(pcolor, f->pcolor, 4 * n)) || (r = mv(scolor, f->scolor, 4 * n)) ||
-        (r = mv(size, f->size, 4 * n)) || (r = mv(nbrs, f->neighbors, 4 * n)))
+        (r = mv(size, f->size, 4 * n)) || (r = mv(nbrs, f->neighbors, 4 * n)) ||
+        (r = mv(prev_size, f->previous_size, 4 * n)))
         return r;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c->err, GE_E_HIP, "render field copy failed");
     return GE_OK;
@@ -517,10 +520,10 @@ int render_copy(Ctx *c, const ge_render_fields *f, bool up, uint32_t *color, uin
 }  // namespace
 
 void launch_render_fill(hipStream_t st, uint32_t *color, uint32_t *pcolor, uint32_t *scolor, float *size,
-                        int32_t *nbrs, uint32_t at, uint32_t cnt, uint32_t rgba, float sz) {
+                        float *prev_size, int32_t *nbrs, uint32_t at, uint32_t cnt, uint32_t rgba, float sz) {
     if (cnt == 0) return;
     hipLaunchKernelGGL(k_render_fill, dim3(blocks_for(cnt, kBlock)), dim3(kBlock), 0, st, color, pcolor, scolor, size,
-                       nbrs, at, cnt, rgba, sz);
+                       prev_size, nbrs, at, cnt, rgba, sz);
 }
 
 void launch_spawn_galaxy(ge_ctx *c, const State &s, uint32_t at, const ge_galaxy_params *g, uint64_t seed,
@@ -596,7 +599,7 @@ int ge_render_upload(ge_ctx *c, const ge_render_fields *f) {
     if (!c->render) return fail(c->err, GE_E_INVALID, "context created without enable_render");
     (void)hipSetDevice(c->device);
     const State &s = c->b.st[c->b.cur];
-    return render_copy(c, f, true, s.color, s.pcolor, s.scolor, s.size, s.nbrs);
+    return render_copy(c, f, true, s.color, s.pcolor, s.scolor, s.size, s.nbrs, s.prev_size);
 }
 
 int ge_render_download(ge_ctx *c, const ge_render_fields *f) {
@@ -604,7 +607,7 @@ int ge_render_download(ge_ctx *c, const ge_render_fields *f) {
     if (!c->render) return fail(c->err, GE_E_INVALID, "context created without enable_render");
     (void)hipSetDevice(c->device);
     const State &s = c->b.st[c->b.cur];
-    return render_copy(c, f, false, s.color, s.pcolor, s.scolor, s.size, s.nbrs);
+    return render_copy(c, f, false, s.color, s.pcolor, s.scolor, s.size, s.nbrs, s.prev_size);
 }
 
 int ge3_render_upload(ge3_ctx *c, const ge_render_fields *f) {
@@ -612,7 +615,7 @@ int ge3_render_upload(ge3_ctx *c, const ge_render_fields *f) {
     if (!c->render) return fail(c->err, GE_E_INVALID, "context created without enable_render");
     (void)hipSetDevice(c->device);
     const State3 &s = c->b.st[c->b.cur];
-    return render_copy(c, f, true, s.color, s.pcolor, s.scolor, s.size, s.nbrs);
+    return render_copy(c, f, true, s.color, s.pcolor, s.scolor, s.size, s.nbrs, s.prev_size);
 }
 
 int ge3_render_download(ge3_ctx *c, const ge_render_fields *f) {
@@ -620,7 +623,7 @@ int ge3_render_download(ge3_ctx *c, const ge_render_fields *f) {
     if (!c->render) return fail(c->err, GE_E_INVALID, "context created without enable_render");
     (void)hipSetDevice(c->device);
     const State3 &s = c->b.st[c->b.cur];
-    return render_copy(c, f, false, s.color, s.pcolor, s.scolor, s.size, s.nbrs);
+    return render_copy(c, f, false, s.color, s.pcolor, s.scolor, s.size, s.nbrs, s.prev_size);
 }
 
 int ge_visuals(ge_ctx *c, const ge_visual_params *v) {

@@ -209,7 +209,7 @@ struct Soa {
     std::vector<float> sph_mass, rest_dens, stiff, visc, cohesion, dens, press, press_tmp, pred_dens;
     std::vector<float> ppx, ppy, ppz, pfx, pfy, pfz;
     std::vector<uint32_t> color, pcolor, scolor;  // render fields (RGBA8)
-    std::vector<float> size;
+    std::vector<float> size, prev_size;
     std::vector<int32_t> nbrs;
     std::vector<float> temp, ke, prev_ke;          // temperature fields
     std::vector<uint32_t> label;
@@ -225,6 +225,7 @@ struct Soa {
         pcolor.assign(n, 0u);
         scolor.assign(n, 0u);
         size.assign(n, 0.0f);
+        prev_size.assign(n, 0.0f);
         nbrs.assign(n, 0);
         temp.assign(n, 0.0f);
         ke.assign(n, 0.0f);
@@ -246,6 +247,7 @@ struct Soa {
         f.scolor = scolor.data();
         f.size = size.data();
         f.neighbors = nbrs.data();
+        f.previous_size = prev_size.data();
         return f;
     }
 };
@@ -306,6 +308,7 @@ void decode(const ge_scene *s, Soa &o) {
         o.scolor[i] = rd<uint32_t>(r, L.off[F_SCOLOR]);
         o.size[i] = rd<float>(r, L.off[F_SIZE]);
         o.nbrs[i] = rd<int32_t>(r, L.off[F_NEIGHBORS]);
+        o.prev_size[i] = rd<float>(r, L.off[F_PREVSIZE]);
     }
 }
 
@@ -381,6 +384,7 @@ void encode(ge_scene *s, const Soa &o, uint32_t n, bool render, bool thermal) {
             wr<uint32_t>(r, L.off[F_SCOLOR], o.scolor[k]);
             wr<float>(r, L.off[F_SIZE], o.size[k]);
             wr<int32_t>(r, L.off[F_NEIGHBORS], o.nbrs[k]);
+            wr<float>(r, L.off[F_PREVSIZE], o.prev_size[k]);
         }
         if (o.id[k] > max_id) max_id = o.id[k];
     }

@@ -200,6 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(State src, State dst, uint32_
     if (src.color) {  // render fields
         dst.color[i] = src.color[j]; dst.pcolor[i] = src.pcolor[j]; dst.scolor[i] = src.scolor[j];
         dst.size[i] = src.size[j]; dst.nbrs[i] = src.nbrs[j];
+        dst.prev_size[i] = src.prev_size[j];
     }
     if (src.temp) {  // temperature fields
         dst.temp[i] = src.temp[j]; dst.label[i] = src.label[j];
@@ -543,6 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(State src, State dst, uint32
     if (src.color) {  // render fields
         dst.color[i] = src.color[j]; dst.pcolor[i] = src.pcolor[j]; dst.scolor[i] = src.scolor[j];
         dst.size[i] = src.size[j]; dst.nbrs[i] = src.nbrs[j];
+        dst.prev_size[i] = src.prev_size[j];
     }
     if (src.temp) {  // temperature fields
         dst.temp[i] = src.temp[j]; dst.label[i] = src.label[j];

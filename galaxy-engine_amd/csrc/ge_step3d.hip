@@ -207,6 +207,7 @@ __global__ __launch_bounds__(kBlock) void k3_gather(State3 src, State3 dst, uint
     if (src.color) {  // render fields
         dst.color[i] = src.color[j]; dst.pcolor[i] = src.pcolor[j]; dst.scolor[i] = src.scolor[j];
         dst.size[i] = src.size[j]; dst.nbrs[i] = src.nbrs[j];
+        dst.prev_size[i] = src.prev_size[j];
     }
 }
 

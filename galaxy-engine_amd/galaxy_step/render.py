@@ -13,14 +13,14 @@ RENDER_EXPORTS = [
     "ge_default_visual_params", "ge_visuals", "ge3_visuals", "ge_ply_vertices", "ge3_ply_vertices",
     "ge_ply_write", "ge3_ply_write", "ge_ply_format", "ge_default_galaxy_params", "ge_spawn_galaxy",
 ]
-RENDER_FIELDS = ["color", "pcolor", "scolor", "size", "neighbors"]
+RENDER_FIELDS = ["color", "pcolor", "scolor", "size", "neighbors", "previous_size"]
 _RDT = {"color": np.uint32, "pcolor": np.uint32, "scolor": np.uint32, "size": np.float32,
-        "neighbors": np.int32}
+        "neighbors": np.int32, "previous_size": np.float32}
 
 
 class RenderFields(C.Structure):
     _fields_ = [("color", C.c_void_p), ("pcolor", C.c_void_p), ("scolor", C.c_void_p), ("size", C.c_void_p),
-                ("neighbors", C.c_void_p)]
+                ("neighbors", C.c_void_p), ("previous_size", C.c_void_p)]
 
 
 class VisualParams(C.Structure):

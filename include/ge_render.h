@@ -9,10 +9,11 @@
  * ge3_config) keeps, per particle and in the same (Morton) order as the
  * physics fields, the ParticleRendering fields these steps read and write
  * (GE/include/Particles/particle.h:75-140): color, pColor, sColor (RGBA8,
- * red in the low byte: r | g << 8 | b << 16 | a << 24), size and
- * neighbors.  Every sort and system compaction moves them with the particle.
+ * red in the low byte: r | g << 8 | b << 16 | a << 24), size, previousSize
+ * and neighbors.  Every sort and system compaction moves them with the particle.
  * Particles that enter through ge_upload get ParticleRendering()'s defaults
- * (white colours, size 1, neighbors 0); ge_render_upload overrides them.
+ * (white colours, size and previousSize 1, neighbors 0); ge_render_upload
+ * overrides them.
  * uniqueColor and isSolid ride in the flags (GE_FLAG_UNIQUE_COLOR,
  * GE_FLAG_SOLID).
  */
@@ -33,6 +34,7 @@ typedef struct {
     uint32_t *color, *pcolor, *scolor; /* color, pColor, sColor (RGBA8) */
     float *size;                       /* size */
     int32_t *neighbors;                /* neighbors */
+    float *previous_size;              /* previousSize (grown by the merger) */
 } ge_render_fields;
 
 /* *on = 1 when the context keeps render fields (created with enable_render) */

@@ -1085,3 +1085,150 @@ int32_t geo_constraints(geo_state *s, geo_constraint *cons, uint32_t *color, int
     free(ib);
     return m;
 }
+
+/* -------------------------------------------------------------- merger --- */
+static void swap_f(float *a, int64_t i, int64_t j) { if (a) { float t = a[i]; a[i] = a[j]; a[j] = t; } }
+static void swap_u(uint32_t *a, int64_t i, int64_t j) { if (a) { uint32_t t = a[i]; a[i] = a[j]; a[j] = t; } }
+static void swap_i(int32_t *a, int64_t i, int64_t j) { if (a) { int32_t t = a[i]; a[i] = a[j]; a[j] = t; } }
+
+static void swap_particles(geo_state *s, float *size, float *prev_size, int32_t *nb, int64_t i, int64_t j) {
+    float *fs[] = {s->x, s->y, s->vx, s->vy, s->ax, s->ay, s->pvx, s->pvy, s->m,
+                   s->sph_mass, s->rest_dens, s->stiff, s->visc, s->cohesion, s->dens,
+                   s->press, s->press_tmp, s->pred_dens, s->ppx, s->ppy, s->pfx, s->pfy,
+                   s->temp, s->ke, s->prev_ke, size, prev_size};
+    for (size_t k = 0; k < sizeof(fs) / sizeof(fs[0]); ++k) swap_f(fs[k], i, j);
+    swap_u(s->flags, i, j);
+    swap_u(s->id, i, j);
+    swap_u(s->label, i, j);
+    swap_i(nb, i, j);
+}
+
+static int cmp_i64(const void *a, const void *b) {
+    const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return (x > y) - (x < y);
+}
+
+int32_t geo_merge(geo_state *s, float *size, float *prev_size, int32_t *neighbors, const geo_mparams *p) {
+    const int32_t n = s->n;
+    if (n == 0) return 0;
+    const float cell = p->cell_size;
+    float *rad = (float *)malloc(sizeof(float) * (size_t)n);
+    for (int32_t i = 0; i < n; ++i) rad[i] = size[i] * p->texture_half_size * p->size_multiplier;
+    /* UpdateNeighbors */
+    int32_t *act = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t na = 0;
+    for (int32_t i = 0; i < n; ++i)
+        if (!(s->flags[i] & GEO_DARK)) act[na++] = i;
+    uint32_t *nb_off = (uint32_t *)calloc((size_t)n + 1, sizeof(uint32_t));
+    uint32_t *list = NULL;
+    if (na > 0) {
+        float minX = FLT_MAX, minY = FLT_MAX, maxX = -FLT_MAX, maxY = -FLT_MAX;
+        for (int32_t k = 0; k < na; ++k) {
+            const float x = s->x[act[k]], y = s->y[act[k]];
+            if (x < minX) minX = x;
+            if (y < minY) minY = y;
+            if (x > maxX) maxX = x;
+            if (y > maxY) maxY = y;
+        }
+        minX -= cell; minY -= cell; maxX += cell; maxY += cell;
+        int gw = (int)((maxX - minX) / cell) + 1, gh = (int)((maxY - minY) / cell) + 1;
+        if (gw < 1) gw = 1;
+        if (gh < 1) gh = 1;
+        const int64_t ncell = (int64_t)gw * gh;
+        int32_t *cx = (int32_t *)malloc(sizeof(int32_t) * (size_t)na), *cy = (int32_t *)malloc(sizeof(int32_t) * (size_t)na);
+        int64_t *cstart = (int64_t *)calloc((size_t)ncell + 1, sizeof(int64_t));
+        for (int32_t k = 0; k < na; ++k) {
+            int a = (int)((s->x[act[k]] - minX) / cell), b = (int)((s->y[act[k]] - minY) / cell);
+            a = a < 0 ? 0 : (a > gw - 1 ? gw - 1 : a);
+            b = b < 0 ? 0 : (b > gh - 1 ? gh - 1 : b);
+            cx[k] = a; cy[k] = b;
+            cstart[(int64_t)b * gw + a + 1] += 1;
+        }
+        for (int64_t c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
+        int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)ncell);
+        memcpy(cur, cstart, sizeof(int64_t) * (size_t)ncell);
+        int32_t *cpart = (int32_t *)malloc(sizeof(int32_t) * (size_t)na);
+        for (int32_t k = 0; k < na; ++k) cpart[cur[(int64_t)cy[k] * gw + cx[k]]++] = act[k];
+        const float dr2 = p->density_radius * p->density_radius;
+        /* two passes: counts (neighbors), then the lists of ids in scan order */
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int32_t k = 0; k < na; ++k) {
+                const int32_t i = act[k];
+                uint32_t cnt = 0;
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int a = cx[k] + dx, b = cy[k] + dy;
+                        if (a < 0 || a >= gw || b < 0 || b >= gh) continue;
+                        const int64_t c = (int64_t)b * gw + a;
+                        for (int64_t q = cstart[c]; q < cstart[c + 1]; ++q) {
+                            const int32_t j = cpart[q];
+                            if (j == i) continue;
+                            const float ex = s->x[i] - s->x[j], ey = s->y[i] - s->y[j];
+                            if (ex * ex + ey * ey < dr2) {
+                                if (pass == 1) list[nb_off[i] + cnt] = (uint32_t)j;  /* indices stand in for ids */
+                                ++cnt;
+                            }
+                        }
+                    }
+                if (pass == 0) neighbors[i] = (int32_t)cnt;
+            }
+            if (pass == 0) {
+                for (int32_t i = 0; i < n; ++i) nb_off[i + 1] = nb_off[i] + (uint32_t)(neighbors[i] > 0 ? neighbors[i] : 0);
+                list = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)nb_off[n] + 1));
+            }
+        }
+        free(cx); free(cy); free(cstart); free(cur); free(cpart);
+    }
+    /* mergerSolver */
+    uint8_t *del = (uint8_t *)calloc((size_t)n, 1);
+    int64_t *dl = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t nd = 0;
+    for (int32_t i = n - 1; i >= 0 && na > 0; --i) {
+        if ((s->flags[i] & GEO_DARK) || del[i]) continue;
+        for (int32_t t = neighbors[i] - 1; t >= 0; --t) {
+            const int32_t j = (int32_t)list[nb_off[i] + (uint32_t)t];
+            if (j == i) continue;
+            if ((s->flags[j] & GEO_DARK) || del[j]) continue;
+            const float dx = s->x[j] - s->x[i], dy = s->y[j] - s->y[i];
+            const float d2 = dx * dx + dy * dy;
+            const float cr = rad[i] + rad[j];
+            if (d2 <= cr * cr) {
+                const float mP = s->m[i], mN = s->m[j];
+                const float a1 = prev_size[i] * prev_size[i], a2 = prev_size[j] * prev_size[j];
+                const float full = sqrtf(a1 + a2);
+                const float mx = prev_size[i] < prev_size[j] ? prev_size[j] : prev_size[i];
+                if (mP >= mN) {
+                    s->m[i] = mP + mN;
+                    s->vx[i] = (s->vx[i] * mP + s->vx[j] * mN) / s->m[i];
+                    s->vy[i] = (s->vy[i] * mP + s->vy[j] * mN) / s->m[i];
+                    prev_size[i] = mx + (full - mx) * 0.25f;
+                    del[j] = 1;
+                    dl[nd++] = j;
+                } else {
+                    s->m[j] = mP + mN;
+                    s->vx[j] = (s->vx[j] * mN + s->vx[i] * mP) / s->m[j];
+                    s->vy[j] = (s->vy[j] * mN + s->vy[i] * mP) / s->m[j];
+                    prev_size[j] = mx + (full - mx) * 0.25f;
+                    del[i] = 1;
+                    dl[nd++] = i;
+                }
+                break;
+            }
+        }
+    }
+    /* sort, unique, then swap with the back and pop, last index first */
+    qsort(dl, (size_t)nd, sizeof(int64_t), cmp_i64);
+    int64_t u = 0;
+    for (int64_t k = 0; k < nd; ++k)
+        if (u == 0 || dl[k] != dl[u - 1]) dl[u++] = dl[k];
+    int64_t size_now = n;
+    for (int64_t k = u - 1; k >= 0; --k) {
+        const int64_t idx = dl[k];
+        if (idx >= size_now) continue;
+        swap_particles(s, size, prev_size, neighbors, idx, size_now - 1);
+        --size_now;
+    }
+    s->n = (int32_t)size_now;
+    free(rad); free(act); free(nb_off); free(list); free(del); free(dl);
+    return (int32_t)size_now;
+}

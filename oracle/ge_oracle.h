@@ -214,6 +214,23 @@ typedef struct {
 int32_t geo_constraints(geo_state *s, geo_constraint *cons, uint32_t *color, int32_t m, const geo_cparams *p,
                         const geo_material *mats, int32_t nmat, uint32_t *frame_count);
 
+/* ---- merger ---- */
+typedef struct {
+    float density_radius;    /* NeighborSearch::densityRadius */
+    float cell_size;         /* NeighborSearch::cellSize */
+    float texture_half_size; /* particleTextureHalfSize (16) */
+    float size_multiplier;   /* particleSizeMultiplier (0.6) */
+} geo_mparams;
+
+/* NeighborSearch::UpdateNeighbors (GE/include/Particles/neighborSearch.h:48-200;
+ * cells filled in index order, as a serial run fills them) and
+ * Physics::mergerSolver (GE/src/Physics/physics.cpp:954-1067) with
+ * totalRadius = size * halfSize * sizeMultiplier (globalLogic.cpp:1225-1227).
+ * size, prev_size, neighbors are the render fields and move with the
+ * particles through the swap-and-pop deletions, as every state field does.
+ * Returns the particle count left. */
+int32_t geo_merge(geo_state *s, float *size, float *prev_size, int32_t *neighbors, const geo_mparams *p);
+
 /* ---- temperature (isTempEnabled) ---- */
 /* Node::gridTemp (GE/include/Physics/quadtree.h:53-100): leaves sum their
  * particles' temp in order, internal nodes their children's in the order

@@ -155,6 +155,8 @@ def load():
     lib.geo_constraints.restype = C.c_int32
     lib.geo_constraints.argtypes = [C.POINTER(_State), C.c_void_p, C.POINTER(C.c_uint32), C.c_int32,
                                     C.POINTER(_CParams), C.c_void_p, C.c_int32, C.POINTER(C.c_uint32)]
+    lib.geo_merge.restype = C.c_int32
+    lib.geo_merge.argtypes = [C.POINTER(_State), f32p, f32p, C.POINTER(C.c_int32), C.c_void_p]
     lib.geo_temperature.restype = None
     lib.geo_temperature.argtypes = [C.POINTER(_State), C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_float]
     _lib = lib
@@ -307,6 +309,26 @@ def constraints(st, cons, colors, materials=(), frame_count=0, **prm):
     m = load().geo_constraints(C.byref(s._s), cs.ctypes.data, _u(col), len(cs), C.byref(p),
                                C.addressof(arr) if mats else None, len(mats), C.byref(fc))
     return s.a, cs[:m].copy(), col[:m].copy(), fc.value
+
+
+class _MParams(C.Structure):
+    _fields_ = [("density_radius", C.c_float), ("cell_size", C.c_float), ("texture_half_size", C.c_float),
+                ("size_multiplier", C.c_float)]
+
+
+def merge(st, size, prev_size, neighbors, density_radius=10.0, cell_size=3.0, texture_half_size=16.0,
+          size_multiplier=0.6):
+    """UpdateNeighbors + mergerSolver serially on a copy of st (dict of arrays
+    with x, y, vx, vy, mass, flags, id, ...); returns (state dict, size,
+    prev_size, neighbors), all cut to the particles left."""
+    s = State(st, sph=st.get("sph_mass") is not None, temperature=st.get("label") is not None)
+    sz = f32(size).copy()
+    ps = f32(prev_size).copy()
+    nb = np.ascontiguousarray(neighbors, dtype=np.int32).copy()
+    p = _MParams(density_radius, cell_size, texture_half_size, size_multiplier)
+    m = load().geo_merge(C.byref(s._s), _f(sz), _f(ps), nb.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(p))
+    out = {k: v[:m].copy() for k, v in s.a.items()}
+    return out, sz[:m], ps[:m], nb[:m]
 
 
 def node_temps(nodes, temp):

@@ -44,7 +44,8 @@ def _render_fields(n, seed):
     rng = np.random.default_rng(seed)
     r32 = lambda: rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)  # noqa: E731
     return {"color": r32(), "pcolor": r32(), "scolor": r32(), "size": rng.uniform(0.1, 2.0, n).astype(F),
-            "neighbors": rng.integers(0, 300, n).astype(np.int32)}
+            "neighbors": rng.integers(0, 300, n).astype(np.int32),
+            "previous_size": rng.uniform(0.1, 2.0, n).astype(F)}
 
 
 ALL_MODES = dict(density_size=1, force_size=1, solid_color=1, density_color=1, force_color=1)
@@ -73,7 +74,7 @@ def test_upload_gives_render_defaults():
         c.upload(st)
         got = GR.render_download(c)
     assert np.all(got["color"] == R.WHITE) and np.all(got["pcolor"] == R.WHITE) and np.all(got["scolor"] == R.WHITE)
-    assert np.all(got["size"] == 1.0) and np.all(got["neighbors"] == 0)
+    assert np.all(got["size"] == 1.0) and np.all(got["neighbors"] == 0) and np.all(got["previous_size"] == 1.0)
 
 
 @pytest.mark.parametrize("search", [0, 1])
