@@ -429,6 +429,10 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
     __shared__ uint32_t blist[kBlockListMax];
     __shared__ uint32_t bcount, bbase;
     __shared__ float sx[kBlockSubMax], sy[kBlockSubMax], sz[kBlockSubMax];  // the subtree's coordinates
+    // per frontier node of a level: the offset of its children among the
+    // level's children (kNone: not split)
+    __shared__ uint32_t soff[kFrontMax];
+    __shared__ uint32_t lcount, lbase;
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t nroots = min(counters[kCntBlockRoots], T.cap);
@@ -454,6 +458,7 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
             fcount[0] = 1;
             fcount[1] = 0;
             bcount = 0;
+            lcount = 0;
         }
         __syncthreads();
         int cur = 0;
@@ -461,33 +466,61 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
         for (; lvl < kBlockLevelsMax; ++lvl) {
             const uint32_t nf = min(fcount[cur], kFrontMax);
             if (nf == 0) break;
+            // A: every node of the level finished or split (one wave per
+            // node); a split node parks its seven inner partition points in
+            // its own child slots and takes its children's offset in LDS
             for (uint32_t i = w; i < nf; i += kTreeWaves) {
                 const uint32_t node = front[cur][i];
                 const float px = T.px[node], py = T.py[node], pz = T.pz[node], size = T.size[node];
                 const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
                 if (finish_small3(T, st, xs, ys, zs, node, px, py, pz, size, s, e, depth, counters + kCntError,
-                                  off))
+                                  off)) {
+                    if (lane == 0) soff[i] = kNone;
                     continue;
+                }
                 uint32_t B[9];
                 wave_split3(xs, ys, zs, px, py, pz, size, s, e, B, off);
                 if (lane == 0) {
-                    const uint32_t k = child_count8(B);
-                    const uint32_t first = atomicAdd(&counters[kCntTopAlloc], k);
-                    const uint32_t slot = atomicAdd(&fcount[cur ^ 1], k);
-                    if (first + k > T.cap || slot + k > kFrontMax) {
-                        atomicOr(&counters[kCntError], kErrFrontier);
-                    } else {
-                        register_start(st, s, depth);
-                        write_children3(T, node, px, py, pz, size, depth, B, first, &front[cur ^ 1][slot]);
-                        const uint32_t li = atomicAdd(&bcount, 1u);
-                        if (li < kBlockListMax) blist[li] = node;
-                    }
+#pragma unroll
+                    for (int q = 1; q < 8; ++q) T.child[8 * node + q - 1] = B[q];
+                    soff[i] = atomicAdd(&lcount, child_count8(B));
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) fcount[cur] = 0;
-            cur ^= 1;
+            // one global allocation for all children of the level: slots in
+            // the node table are only names, so their order is free
+            if (threadIdx.x == 0) {
+                const uint32_t k = lcount;
+                const uint32_t first = k ? atomicAdd(&counters[kCntTopAlloc], k) : 0u;
+                const bool ok = k <= kFrontMax && first + k <= T.cap;
+                if (!ok) atomicOr(&counters[kCntError], kErrFrontier);
+                lbase = ok ? first : kNone;
+                fcount[cur ^ 1] = ok ? k : 0u;
+                lcount = 0;
+            }
             __syncthreads();
+            // B: the children records, one thread per split node
+            const uint32_t base = lbase;
+            if (base != kNone) {
+                for (uint32_t i = threadIdx.x; i < nf; i += kTreeBlock) {
+                    const uint32_t o = soff[i];
+                    if (o == kNone) continue;
+                    const uint32_t node = front[cur][i];
+                    const float px = T.px[node], py = T.py[node], pz = T.pz[node], size = T.size[node];
+                    const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
+                    uint32_t B[9];
+                    B[0] = s;
+                    B[8] = e;
+#pragma unroll
+                    for (int q = 1; q < 8; ++q) B[q] = T.child[8 * node + q - 1];
+                    register_start(st, s, depth);
+                    write_children3(T, node, px, py, pz, size, depth, B, base + o, &front[cur ^ 1][o]);
+                    const uint32_t li = atomicAdd(&bcount, 1u);
+                    if (li < kBlockListMax) blist[li] = node;
+                }
+            }
+            __syncthreads();
+            cur ^= 1;
         }
         if (lvl == kBlockLevelsMax && threadIdx.x == 0) atomicOr(&counters[kCntError], kErrFrontier);
         const uint32_t nl = bcount;

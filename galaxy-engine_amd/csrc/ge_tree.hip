@@ -424,6 +424,11 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
     __shared__ uint32_t fcount[2];
     __shared__ uint32_t blist[kBlockListMax];  // internal nodes, level by level
     __shared__ uint32_t bcount, bbase;
+    // per frontier node of a level: the three inner partition points and the
+    // offset of its children among the level's children (kNone: not split)
+    __shared__ uint32_t sb[kFrontMax][3];
+    __shared__ uint32_t soff[kFrontMax];
+    __shared__ uint32_t lcount, lbase;
     __shared__ float sx[kBlockSubMax], sy[kBlockSubMax];  // the subtree's coordinates
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
@@ -449,6 +454,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
             fcount[0] = 1;
             fcount[1] = 0;
             bcount = 0;
+            lcount = 0;
         }
         __syncthreads();
         int cur = 0;
@@ -456,31 +462,54 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
         for (; lvl < kBlockLevelsMax; ++lvl) {
             const uint32_t nf = min(fcount[cur], kFrontMax);
             if (nf == 0) break;
+            // A: every node of the level finished or split (one wave per
+            // node); split nodes take their children's offsets in LDS
             for (uint32_t i = w; i < nf; i += kTreeWaves) {
                 const uint32_t node = front[cur][i];
                 const float px = T.px[node], py = T.py[node], size = T.size[node];
                 const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
-                if (finish_small(T, st, xs, ys, node, px, py, size, s, e, depth, counters + kCntError, off)) continue;
+                if (finish_small(T, st, xs, ys, node, px, py, size, s, e, depth, counters + kCntError, off)) {
+                    if (lane == 0) soff[i] = kNone;
+                    continue;
+                }
                 uint32_t b[5];
                 wave_split(xs, ys, px, py, size, s, e, b, off);
                 if (lane == 0) {
-                    const uint32_t k = child_count(b);
-                    const uint32_t first = atomicAdd(&counters[kCntTopAlloc], k);
-                    const uint32_t slot = atomicAdd(&fcount[cur ^ 1], k);
-                    if (first + k > T.cap || slot + k > kFrontMax) {
-                        atomicOr(&counters[kCntError], kErrFrontier);
-                    } else {
-                        register_start(st, s, depth);
-                        write_children(T, node, px, py, size, depth, b, first, &front[cur ^ 1][slot]);
-                        const uint32_t li = atomicAdd(&bcount, 1u);
-                        if (li < kBlockListMax) blist[li] = node;
-                    }
+                    sb[i][0] = b[1]; sb[i][1] = b[2]; sb[i][2] = b[3];
+                    soff[i] = atomicAdd(&lcount, child_count(b));
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) fcount[cur] = 0;
-            cur ^= 1;
+            // one global allocation for all children of the level: slots in
+            // the node table are only names, so their order is free
+            if (threadIdx.x == 0) {
+                const uint32_t k = lcount;
+                const uint32_t first = k ? atomicAdd(&counters[kCntTopAlloc], k) : 0u;
+                const bool ok = k <= kFrontMax && first + k <= T.cap;
+                if (!ok) atomicOr(&counters[kCntError], kErrFrontier);
+                lbase = ok ? first : kNone;
+                fcount[cur ^ 1] = ok ? k : 0u;
+                lcount = 0;
+            }
             __syncthreads();
+            // B: the children records, one thread per split node
+            const uint32_t base = lbase;
+            if (base != kNone) {
+                for (uint32_t i = threadIdx.x; i < nf; i += kTreeBlock) {
+                    const uint32_t o = soff[i];
+                    if (o == kNone) continue;
+                    const uint32_t node = front[cur][i];
+                    const float px = T.px[node], py = T.py[node], size = T.size[node];
+                    const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
+                    const uint32_t b[5] = {s, sb[i][0], sb[i][1], sb[i][2], e};
+                    register_start(st, s, depth);
+                    write_children(T, node, px, py, size, depth, b, base + o, &front[cur ^ 1][o]);
+                    const uint32_t li = atomicAdd(&bcount, 1u);
+                    if (li < kBlockListMax) blist[li] = node;
+                }
+            }
+            __syncthreads();
+            cur ^= 1;
         }
         if (lvl == kBlockLevelsMax && threadIdx.x == 0) atomicOr(&counters[kCntError], kErrFrontier);
         // hand the internal-node list to the block mass pass (kNone: too long,
