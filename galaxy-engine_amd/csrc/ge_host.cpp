@@ -385,16 +385,26 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
             if (slot.req >= 0) return true;
         return false;
     };
+    // The batch round and the SPH chunks run asynchronously; the loop wakes
+    // on whichever finishes first, so an SPH slot is refilled as soon as its
+    // chunk ends rather than when the (longer) batch round does.  The batch
+    // changes membership only between rounds.
+    int32_t batch_R = 0;  // frames of the batch round in flight (0: the batch is idle)
     while (done < nreq) {
         const double now = seconds_since(t0);
         while (next < nreq && reqs[order[next]].arrival_s <= now) waiting.push_back(order[next++]);
+        const bool batch_idle = batch_R == 0;
 
         // ---- admission: first come, first served per kind, within capacity
         std::vector<uint32_t> keep;
-        for (uint32_t t = 0; t < resident.size(); ++t)
-            if (resident[t].remaining > 0) keep.push_back(t);
         std::vector<Resident> next_res;
-        for (uint32_t t : keep) next_res.push_back(resident[t]);
+        if (batch_idle) {
+            for (uint32_t t = 0; t < resident.size(); ++t)
+                if (resident[t].remaining > 0) keep.push_back(t);
+            for (uint32_t t : keep) next_res.push_back(resident[t]);
+        } else {
+            next_res = resident;
+        }
         uint64_t used = 0;
         uint32_t maxn = 0;
         for (auto &r : next_res) {
@@ -403,8 +413,8 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
         }
         Staging stage;
         std::vector<uint32_t> add_counts;
-        const bool idle = next_res.empty() && !sph_busy();
-        bool grav_blocked = false, sph_blocked = false, took = false;
+        const bool idle = batch_idle && next_res.empty() && !sph_busy();
+        bool grav_blocked = !batch_idle, sph_blocked = false, took = false;
         for (auto it = waiting.begin(); it != waiting.end();) {
             const uint32_t q = *it;
             const ge_request &r = reqs[q];
@@ -463,7 +473,7 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
             }
             if (grav_blocked && sph_blocked) break;
         }
-        if (keep.size() != resident.size() || !add_counts.empty()) {
+        if (batch_idle && (keep.size() != resident.size() || !add_counts.empty())) {
             ge_particles pv = stage.view();
             const int rc = ge_systems_update(s->batch, (uint32_t)keep.size(), keep.data(),
                                              (uint32_t)add_counts.size(), add_counts.data(), &pv);
@@ -474,12 +484,12 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
         for (auto &slot : s->sph) on_device += slot.req >= 0;
         max_resident = std::max(max_resident, on_device);
 
-        // ---- launch: one batch round, and the next chunk of every idle SPH slot
-        int32_t R = 0;
-        if (!resident.empty()) {
-            R = (int32_t)cfg.round_frames;
+        // ---- launch: a batch round if the batch is idle, the next chunk of every idle SPH slot
+        if (batch_idle && !resident.empty()) {
+            int32_t R = (int32_t)cfg.round_frames;
             for (auto &r : resident) R = std::min(R, r.remaining);
             if (int rc = ge_step(s->batch, &s->pg, R)) return server_fail(s, rc, ge_last_error(s->batch));
+            batch_R = R;
             ++rounds;
         }
         for (auto &slot : s->sph) {
@@ -494,15 +504,29 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
             slot.inflight = rs;
         }
 
-        // ---- wait
-        if (!resident.empty()) {
-            if (int rc = ge_sync(s->batch)) return server_fail(s, rc, ge_last_error(s->batch));
-            for (auto &r : resident) r.remaining -= R;
-        } else if (sph_busy()) {
+        // ---- wait for the first completion (or an arrival while work runs)
+        uint32_t inflight = batch_R > 0 ? 1u : 0u;
+        ge_server::SphSlot *only = nullptr;
+        for (auto &slot : s->sph)
+            if (slot.req >= 0 && slot.inflight) {
+                ++inflight;
+                only = &slot;
+            }
+        if (inflight == 1 && (cfg.batching == 0 || waiting.empty())) {
+            // one context on the device and nothing that could start beside
+            // it: block on it
+            if (batch_R > 0) {
+                if (int rc = ge_sync(s->batch)) return server_fail(s, rc, ge_last_error(s->batch));
+            } else {
+                (void)ge_sync(only->ctx);  // its status is read with the completion below
+            }
+        } else if (inflight > 0) {
             bool any = false;
             while (!any) {
+                int32_t d = 0;
+                if (batch_R > 0 && ge_poll(s->batch, &d) == GE_OK && d) any = true;
                 for (auto &slot : s->sph) {
-                    int32_t d = 0;
+                    d = 0;
                     if (slot.req >= 0 && slot.inflight && ge_poll(slot.ctx, &d) == GE_OK && d) any = true;
                 }
                 if (!any) {
@@ -516,6 +540,19 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
         }
 
         // ---- completions
+        if (batch_R > 0) {
+            int32_t d = 0;
+            if (ge_poll(s->batch, &d) == GE_OK && d) {
+                if (int rc = ge_sync(s->batch)) return server_fail(s, rc, ge_last_error(s->batch));
+                for (auto &r : resident) r.remaining -= batch_R;
+                batch_R = 0;
+                for (uint32_t t = 0; t < resident.size(); ++t) {
+                    if (resident[t].remaining > 0) continue;
+                    const uint32_t q = resident[t].req;
+                    finish(q, ge_download_system(s->batch, t, &reqs[q].out));
+                }
+            }
+        }
         for (auto &slot : s->sph) {
             if (slot.req < 0 || !slot.inflight) continue;
             int32_t d = 0;
@@ -528,11 +565,6 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
             if (rc == GE_OK) rc = ge_download(slot.ctx, &reqs[q].out);
             finish(q, rc);
             slot.req = -1;
-        }
-        for (uint32_t t = 0; t < resident.size(); ++t) {
-            if (resident[t].remaining > 0) continue;
-            const uint32_t q = resident[t].req;
-            finish(q, ge_download_system(s->batch, t, &reqs[q].out));
         }
     }
     // leave the batch empty for the next run
