@@ -360,10 +360,11 @@ int check_device_errors(ge_ctx *c) {
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     if (e) {
         char buf[200];
-        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s%s%s%s)", e,
+        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s%s%s%s%s)", e,
                       (e & kErrNodeCap) ? "node capacity " : "", (e & kErrDepth) ? "tree depth " : "",
                       (e & kErrGridRange) ? "SPH grid range " : "", (e & kErrNeighborCap) ? "neighbour cap " : "",
-                      (e & kErrFrontier) ? "tree frontier " : "", (e & kErrMultipole) ? "multipole lists" : "");
+                      (e & kErrFrontier) ? "tree frontier " : "", (e & kErrMultipole) ? "multipole lists " : "",
+                      (e & kErrSort) ? "sort wait" : "");
         c->err = buf;
         uint32_t z = 0;
         (void)hipMemcpy(c->b.counters + kCntError, &z, sizeof z, hipMemcpyHostToDevice);

@@ -40,6 +40,7 @@ enum : uint32_t {
     kErrNeighborCap = 8u,
     kErrFrontier = 16u,  // workgroup subtree frontier or level bound exceeded
     kErrMultipole = 32u, // multipole work lists overflowed
+    kErrSort = 64u,      // a radix-sort look-back wait passed its spin limit
 };
 
 // counters block (device, uint32 each)
@@ -250,12 +251,14 @@ namespace ge {
 
 // ---- launchers (each defined next to its kernels) ----
 // sort.hip
+// err: where a look-back wait that passed its spin limit raises kErrSort
+// (null: a word at the end of hist)
 void radix_sort_words(hipStream_t stream, uint32_t *hist, uint64_t *&data, uint64_t *&alt, uint32_t n, int lo_bit,
-                      int hi_bit);
+                      int hi_bit, uint32_t *err = nullptr);
 void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n,
                        int lo_bit, int hi_bit);
 void radix_sort_kv(hipStream_t stream, uint32_t *hist, uint64_t *&keys, uint64_t *&keys_alt, uint32_t *&vals,
-                   uint32_t *&vals_alt, uint32_t n, int lo_bit, int hi_bit);
+                   uint32_t *&vals_alt, uint32_t n, int lo_bit, int hi_bit, uint32_t *err = nullptr);
 uint32_t radix_hist_entries(uint32_t cap);
 // step.hip
 void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate,
