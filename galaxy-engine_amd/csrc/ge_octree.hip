@@ -565,23 +565,35 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_emit(TopNodes3 T, StartTab
         if (kind == kKindWaveSub) {
             wave_subtree3<true>(x, y, z, m, T.px[t], T.py[t], T.pz[t], T.size[t], s, e - s, depth, st, o, lev,
                                 counters + kCntError);
-        } else if (lane == 0) {
+        } else {
             float gm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
             uint32_t mask = 0;
             if (kind == kKindLeaf) {
-                for (uint32_t i = s; i < e; ++i) {
-                    const float mi = m[i];
-                    gm = gm + mi;
-                    cx = cx + x[i] * mi;
-                    cy = cy + y[i] * mi;
-                    cz = cz + z[i] * mi;
+                // the wave loads up to 64 particles at once, the sums run in
+                // particle order
+                for (uint32_t c0 = s; c0 < e; c0 += 64u) {
+                    const uint32_t cnt = min(64u, e - c0);
+                    const bool have = lane < cnt;
+                    const float lm = have ? m[c0 + lane] : 0.0f;
+                    const float lx = have ? x[c0 + lane] : 0.0f;
+                    const float ly = have ? y[c0 + lane] : 0.0f;
+                    const float lz = have ? z[c0 + lane] : 0.0f;
+                    for (uint32_t j = 0; j < cnt; ++j) {
+                        const float mj = readlane_f(lm, j);
+                        gm = gm + mj;
+                        cx = cx + readlane_f(lx, j) * mj;
+                        cy = cy + readlane_f(ly, j) * mj;
+                        cz = cz + readlane_f(lz, j) * mj;
+                    }
                 }
                 if (gm > 0) { cx = cx / gm; cy = cy / gm; cz = cz / gm; }
             } else {
                 for (int q = 0; q < 8; ++q) mask |= (T.child[8 * t + q] != kNone) ? (1u << q) : 0u;
             }
-            write_node3(o, idx, cx, cy, cz, gm, T.size[t], next, s, e, mask);
-            if (idx < o.cap) o.geom[idx] = make_float4(T.px[t], T.py[t], T.pz[t], T.size[t]);
+            if (lane == 0) {
+                write_node3(o, idx, cx, cy, cz, gm, T.size[t], next, s, e, mask);
+                if (idx < o.cap) o.geom[idx] = make_float4(T.px[t], T.py[t], T.pz[t], T.size[t]);
+            }
         }
     }
 }

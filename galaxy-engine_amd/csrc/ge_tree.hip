@@ -560,22 +560,30 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_emit(TopNodes T, StartTable
             float gm, cx, cy;
             wave_subtree<true>(x, y, m, T.px[t], T.py[t], T.size[t], s, e - s, depth, st, o, lev,
                                counters + kCntError, gm, cx, cy);
-        } else if (lane == 0) {
+        } else {
             float gm = 0.0f, cx = 0.0f, cy = 0.0f;
             uint32_t mask = 0;
             if (kind == kKindLeaf) {
-                // computeLeafMass (quadtree.h:53-67)
-                for (uint32_t i = s; i < e; ++i) {
-                    const float mi = m[i];
-                    gm = gm + mi;
-                    cx = cx + x[i] * mi;
-                    cy = cy + y[i] * mi;
+                // computeLeafMass (quadtree.h:53-67): the wave loads up to 64
+                // particles at once, the sums run in particle order
+                for (uint32_t c0 = s; c0 < e; c0 += 64u) {
+                    const uint32_t cnt = min(64u, e - c0);
+                    const bool have = lane < cnt;
+                    const float lm = have ? m[c0 + lane] : 0.0f;
+                    const float lx = have ? x[c0 + lane] : 0.0f;
+                    const float ly = have ? y[c0 + lane] : 0.0f;
+                    for (uint32_t j = 0; j < cnt; ++j) {
+                        const float mj = readlane_f(lm, j);
+                        gm = gm + mj;
+                        cx = cx + readlane_f(lx, j) * mj;
+                        cy = cy + readlane_f(ly, j) * mj;
+                    }
                 }
                 if (gm > 0) { cx = cx / gm; cy = cy / gm; }
             } else {
                 for (int q = 0; q < 4; ++q) mask |= (T.child[4 * t + q] != kNone) ? (1u << q) : 0u;
             }
-            write_node(o, idx, cx, cy, gm, T.px[t], T.py[t], T.size[t], next, s, e, mask);
+            if (lane == 0) write_node(o, idx, cx, cy, gm, T.px[t], T.py[t], T.size[t], next, s, e, mask);
         }
     }
 }
