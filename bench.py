@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--only-3d", action="store_true", help="run only the 3D leg (profiling) and print its JSON")
     ap.add_argument("--only-sph", action="store_true", help="run only the 2D SPH impact leg (profiling)")
     ap.add_argument("--only-sph3d", action="store_true", help="run only the 3D SPH impact leg (profiling)")
+    ap.add_argument("--only-couplings", action="store_true",
+                    help="run only the leg of the steps either side and the couplings")
+    ap.add_argument("--no-couplings", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=120)
@@ -423,6 +426,183 @@ def batched(a, accuracy):
             "params": {"accuracy": accuracy, "leaf_capacity": 16, "graphs": True}}
 
 
+def _lattice(side, seed):
+    """A jittered square lattice with right, up and up-right springs (random
+    ids, shuffled constraint order), as the constraint parity tests build it."""
+    import numpy as np
+    from galaxy_step import constraints as GC
+    F = np.float32
+    rng = np.random.default_rng(seed)
+    n = side * side
+    gx, gy = np.meshgrid(np.arange(side), np.arange(side), indexing="xy")
+    x = (gx.ravel() + rng.normal(0, 0.08, n)).astype(F)
+    y = (gy.ravel() + rng.normal(0, 0.08, n)).astype(F)
+    ids = rng.permutation(np.arange(100, 100 + 3 * n, 3)).astype(np.uint32)
+    a = np.arange(n).reshape(side, side)
+    rows = np.concatenate([np.stack([a[:, :-1].ravel(), a[:, 1:].ravel()], 1),
+                           np.stack([a[:-1, :].ravel(), a[1:, :].ravel()], 1),
+                           np.stack([a[:-1, :-1].ravel(), a[1:, 1:].ravel()], 1)])
+    rng.shuffle(rows)
+    cons = np.zeros(len(rows), GC.CONSTRAINT_DTYPE)
+    cons["id1"], cons["id2"] = ids[rows[:, 0]], ids[rows[:, 1]]
+    d = np.hypot(x[rows[:, 1]] - x[rows[:, 0]], y[rows[:, 1]] - y[rows[:, 0]]).astype(F)
+    cons["rest_length"] = (d * rng.uniform(0.9, 1.1, len(rows))).astype(F)
+    cons["original_length"] = d
+    cons["stiffness"] = rng.uniform(20, 60, len(rows)).astype(F)
+    cons["resistance"] = rng.uniform(0.05, 0.3, len(rows)).astype(F)
+    cons["plasticity_point"] = (cons["resistance"] * F(0.5)).astype(F)
+    st = {"x": x, "y": y, "vx": rng.normal(0, 0.5, n).astype(F), "vy": rng.normal(0, 0.5, n).astype(F),
+          "ax": np.zeros(n, F), "ay": np.zeros(n, F),
+          "mass": rng.uniform(0.5, 2.0, n).astype(F), "id": ids, "flags": np.zeros(n, np.uint32),
+          "label": rng.choice([0, 1, 3, 5], n).astype(np.uint32), "temp": np.full(n, 300.0, F)}
+    return st, cons
+
+
+def _crowd(n, seed, box):
+    """Particles crowded enough to overlap (render sizes 0.05-0.3), 10 % dark."""
+    import numpy as np
+    from galaxy_step.lib import GE_FLAG_DARK
+    F = np.float32
+    rng = np.random.default_rng(seed)
+    st = {"x": rng.uniform(-box, box, n).astype(F), "y": rng.uniform(-box, box, n).astype(F),
+          "vx": rng.normal(0, 1, n).astype(F), "vy": rng.normal(0, 1, n).astype(F),
+          "mass": rng.choice([1.0, 2.0, 3.0, 5.0], n).astype(F),
+          "flags": np.where(rng.random(n) < 0.1, GE_FLAG_DARK, 0).astype(np.uint32),
+          "id": rng.permutation(np.arange(7, 7 + 2 * n, 2)).astype(np.uint32)}
+    rf = {"size": rng.uniform(0.05, 0.3, n).astype(F), "previous_size": rng.uniform(0.05, 0.3, n).astype(F),
+          "neighbors": np.zeros(n, np.int32)}
+    return st, rf
+
+
+def couplings(a, cpu_seconds):
+    """The steps either side of the frame and the couplings, each timed on the
+    device (wall time around synchronised calls, best of several) and, where
+    the oracle restates them in C, on the host on a bounded sample."""
+    import numpy as np
+    from galaxy_step import Context, configs, default_params
+    from galaxy_step import constraints as GC
+    from galaxy_step import merger as GM
+    from galaxy_step import render as GR
+    from galaxy_step import thermal as T
+
+    def best(fn, reps, sync):
+        fn()
+        sync()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            sync()
+            ts.append(time.perf_counter() - t0)
+        return min(ts) * 1e3
+
+    out = {}
+    # render fields: neighbour counts, size and colour modes; PLY vertices
+    st, prm = configs.config1(n=a.n, theta=a.theta)
+    n = len(st["x"])
+    p = default_params(G=prm["G"], theta=a.theta, softening=prm["softening"], dt=prm["dt"], dt_start=prm["dt"],
+                       direct_below=0)
+    with Context(n, sph=True, render=True) as c:  # neighbour counts run on the SPH grid
+        c.upload(st)
+        c.step(p, 1)
+        c.sync()
+        v = GR.default_visual_params(count_neighbors=1, density_size=1, density_color=1)
+        ms = best(lambda: GR.visuals(c, v), 5, c.sync)
+        out["visuals"] = {"workload": f"{n} particles of the two-disk frame: neighbour counts (3x3 cells of 3, "
+                                      "radius 3.5), size by density, density colours",
+                          "ms": ms, "particles_per_s": n / (ms * 1e-3)}
+        ms = best(lambda: GR.ply_vertices(c), 3, c.sync)
+        out["ply_vertices"] = {"workload": f"{n} particles: order-keeping compaction of the visible ones into "
+                                           "20-byte PLY vertices on the device, copied to the host",
+                               "ms": ms, "vertices_per_s": n / (ms * 1e-3)}
+    # the galaxy spawner: 800k visible + 200k dark particles appended per call
+    g = GR.default_galaxy_params(visible=800_000, dark=200_000)
+    spawn = []
+    for k in range(3):
+        with Context(1_000_000, render=True) as c:
+            GR.spawn_galaxy(c, GR.default_galaxy_params(visible=1000, dark=100), seed=99)
+            c.sync()
+            with Context(1_000_000, render=True) as c2:
+                t0 = time.perf_counter()
+                GR.spawn_galaxy(c2, g, seed=k)
+                c2.sync()
+                spawn.append(time.perf_counter() - t0)
+    ms = min(spawn) * 1e3
+    out["spawn_galaxy"] = {"workload": "1,000,000 particles (800k visible, 200k dark) appended with counter-based "
+                                       "draws", "ms": ms, "particles_per_s": 1e6 / (ms * 1e-3)}
+    # temperature coupling on the SPH impact: frames with and without it
+    st, prm = configs.config2()
+    n = len(st["x"])
+    st["temp"] = np.random.default_rng(8).uniform(250.0, 1500.0, n).astype(np.float32)
+    st["label"] = np.zeros(n, np.uint32)
+    frames = {}
+    for temp in (0, 1):
+        pt = default_params(direct_below=0, compensated=1, temperature=temp)
+        for k in ("G", "theta", "softening", "dt", "sph", "sph_h", "sph_cell", "sph_unit_mass", "viscosity",
+                  "cohesion_coef", "delta", "stiff_mult", "sph_max_vel"):
+            setattr(pt, k, prm[k])
+        pt.dt_start = pt.dt
+        with Context(n, sph=True, temperature=True) as c:
+            c.upload(st)
+            T.thermal_upload(c, st)
+            frames[temp] = best(lambda: c.step(pt, 5), 3, c.sync) / 5
+    out["temperature"] = {"workload": f"gravity + SPH impact, {n} particles: node temperatures, heat transfer "
+                                      "in the walk and temperatureCalculation each frame",
+                          "ms_per_frame": frames[1], "ms_per_frame_without": frames[0],
+                          "added_ms": frames[1] - frames[0]}
+    # constraints: Physics::constraints on a 400 x 400 spring lattice
+    st, cons = _lattice(400, 1)
+    n = len(st["x"])
+    pc = default_params(G=0.0, theta=0.7, softening=1.0, dt=0.01, dt_start=0.01, direct_below=0, constraints=1)
+    with Context(n, sph=True, temperature=True, max_constraints=len(cons)) as c:
+        c.upload(st)
+        T.thermal_upload(c, st)
+        GC.upload(c, cons)
+        ms = best(lambda: GC.solve(c, pc), 5, c.sync)
+        left = len(GC.download(c))
+    out["constraints"] = {"workload": f"{n}-particle lattice, {len(cons)} springs: removal, 15 coloured substeps, "
+                                      "breaking and plastic rules", "ms": ms,
+                          "constraint_substeps_per_s": 15 * len(cons) / (ms * 1e-3), "constraints_left": left}
+    # the merger on a crowded million
+    # density 0.055 per unit area: well merged, but below percolation of the
+    # overlap graph (its largest component has about a hundred particles)
+    st, rf = _crowd(1_000_000, 3, 2130.0)
+    n = len(st["x"])
+    m = GM.default_merger_params()
+    ts, removed = [], 0
+    for _ in range(3):
+        with Context(n, render=True) as c:
+            c.upload(st)
+            GR.render_upload(c, rf)
+            c.sync()
+            t0 = time.perf_counter()
+            removed = GM.merge(c, m)
+            c.sync()
+            ts.append(time.perf_counter() - t0)
+    ms = min(ts) * 1e3
+    out["merger"] = {"workload": f"{n} crowded particles (10 % dark): UpdateNeighbors grid and the greedy "
+                                 "merge loop", "ms": ms, "particles_per_s": n / (ms * 1e-3), "merged": removed}
+    if cpu_seconds > 0:
+        import oracle as O
+        ss, srf = _crowd(100_000, 3, 2130.0 * (0.1 ** 0.5))
+        t0 = time.perf_counter()
+        O.merge(ss, srf["size"], srf["previous_size"], srf["neighbors"], **m.as_dict())
+        el = time.perf_counter() - t0
+        out["merger"]["cpu_baseline"] = {"particles_per_s": 1e5 / el, "cores": 1, "kind": "port",
+                                         "sample": "100,000 particles at the same density through the serial C "
+                                                   f"restatement, {el:.2f} s"}
+        st2, cons2 = _lattice(100, 1)
+        colors = np.arange(len(cons2), dtype=np.uint32)  # any order: a serial loop
+        t0 = time.perf_counter()
+        O.constraints(st2, cons2, colors, materials=O.reference_materials())
+        el = time.perf_counter() - t0
+        out["constraints"]["cpu_baseline"] = {"constraint_substeps_per_s": 15 * len(cons2) / el, "cores": 1,
+                                              "kind": "port",
+                                              "sample": f"100 x 100 lattice ({len(cons2)} springs), one solve through "
+                                                        f"the serial C restatement, {el:.2f} s"}
+    return out
+
+
 def served(a, accuracy):
     """cfg4: Poisson request streams through the server, batched and plain."""
     import numpy as np
@@ -490,6 +670,9 @@ def main():
     if a.only_sph3d:
         print(json.dumps({"sph3d_impact": sph3d(a, 0.0 if a.no_cpu else min(a.cpu_seconds, 8.0))}), flush=True)
         return
+    if a.only_couplings:
+        print(json.dumps({"couplings": couplings(a, 0.0 if a.no_cpu else a.cpu_seconds)}), flush=True)
+        return
     st, prm = configs.config1(n=a.n, theta=a.theta)
     n = len(st["x"])
     log(f"n={n} theta={a.theta} accuracy={a.accuracy} warmup={a.warmup} steps={a.steps}")
@@ -517,6 +700,15 @@ def main():
         log(f"batched: {bat['batched_solves_per_s']:.1f} solves/s vs plain {bat['plain_solves_per_s']:.1f} "
             f"({bat['speedup']:.2f}x)")
     srv = None if a.no_serve else served(a, a.accuracy)
+    cpl = None
+    if not a.no_couplings:
+        try:
+            cpl = couplings(a, 0.0 if a.no_cpu else a.cpu_seconds)
+            log("couplings: " + ", ".join(f"{k} {v.get('ms', v.get('ms_per_frame', 0.0)):.2f} ms"
+                                          for k, v in cpl.items()))
+        except Exception as e:  # an auxiliary leg: report it and keep the headline line
+            cpl = {"error": f"{type(e).__name__}: {e}"}
+            log(f"couplings leg failed: {cpl['error']}")
     phase_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in head["phases"].items() if v[1]}
     achieved = head["walk_tflops"]
     out = {
@@ -558,6 +750,7 @@ def main():
         "sph_impact": sph,
         "three_d": d3,
         "sph3d_impact": s3,
+        "couplings": cpl,
         "batched": bat,
         "served": srv,
     }
