@@ -148,6 +148,21 @@ __global__ void k_mp_reset(uint32_t *__restrict__ cnt) {
     if (threadIdx.x < kMpSlots) cnt[threadIdx.x] = 0u;
 }
 
+// mono_add over particles j, j + stride, ... < end, in that order, with four
+// loads in flight (the sums keep the order, so the result is the plain loop's)
+__device__ __forceinline__ void mono_range(const float4 *__restrict__ pxym, uint32_t j, uint32_t end,
+                                           uint32_t stride, double cx, double cy, double *__restrict__ S,
+                                           double &r2) {
+    for (; j < end && end - j > 3 * stride; j += 4 * stride) {
+        const float4 p0 = pxym[j], p1 = pxym[j + stride], p2 = pxym[j + 2 * stride], p3 = pxym[j + 3 * stride];
+        mono_add(p0, cx, cy, S, r2);
+        mono_add(p1, cx, cy, S, r2);
+        mono_add(p2, cx, cy, S, r2);
+        mono_add(p3, cx, cy, S, r2);
+    }
+    for (; j < end; j += stride) mono_add(pxym[j], cx, cy, S, r2);
+}
+
 // Thread per node: small nodes are finished here; the others are listed.
 __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict__ nodes,
                                                        const float4 *__restrict__ geom,
@@ -168,13 +183,25 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict
             for (int c = 0; c < kMono; ++c) S[c] = 0.0;
             const double cx = r.comx, cy = r.comy;
             double r2 = 0.0;
-            for (uint32_t j = r.start; j < r.end; ++j) mono_add(pxym[j], cx, cy, S, r2);
+            mono_range(pxym, r.start, r.end, 1u, cx, cy, S, r2);
             finish_record(k, r, geom[k], S, r2, inv_theta, mp);
-        } else if (count <= kMpWaveMax) {
-            const uint32_t w = atomicAdd(&cnt[kMpWaveCount], 1u);
-            if (w < wave_cap) wave_list[w] = k;
-            else atomicOr(err, kErrMultipole);
-        } else {
+        }
+        // wave-list entries: one atomic per wave (the list order is free)
+        const bool to_wave = count > kMpThreadMax && count <= kMpWaveMax;
+        const uint64_t wm = __ballot(to_wave);
+        if (wm) {
+            const uint32_t lane = threadIdx.x & 63u;
+            const uint32_t leader = (uint32_t)__builtin_ctzll(wm);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&cnt[kMpWaveCount], (uint32_t)__popcll(wm));
+            base = (uint32_t)__shfl((int)base, (int)leader, 64);
+            if (to_wave) {
+                const uint32_t w = base + (uint32_t)__popcll(wm & ((1ull << lane) - 1ull));
+                if (w < wave_cap) wave_list[w] = k;
+                else atomicOr(err, kErrMultipole);
+            }
+        }
+        if (count > kMpWaveMax) {
             const uint32_t pieces = (count + kMpWaveMax - 1) / kMpWaveMax;
             const uint32_t b = atomicAdd(&cnt[kMpBigCount], 1u);
             const uint32_t f = atomicAdd(&cnt[kMpPieceCount], pieces);
@@ -207,7 +234,7 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_wave(const NodeRec *__restrict_
         for (int c = 0; c < kMono; ++c) S[c] = 0.0;
         const double cx = r.comx, cy = r.comy;
         double r2 = 0.0;
-        for (uint32_t j = r.start + lane; j < r.end; j += 64) mono_add(pxym[j], cx, cy, S, r2);
+        mono_range(pxym, r.start + lane, r.end, 64u, cx, cy, S, r2);
         wave_sum(S, r2);
         if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp);
     }
@@ -237,7 +264,7 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_pieces(const NodeRec *__restric
         for (int c = 0; c < kMono; ++c) S[c] = 0.0;
         const double cx = r.comx, cy = r.comy;
         double r2 = 0.0;
-        for (uint32_t j = lo + lane; j < hi; j += 64) mono_add(pxym[j], cx, cy, S, r2);
+        mono_range(pxym, lo + lane, hi, 64u, cx, cy, S, r2);
         wave_sum(S, r2);
         if (lane <= kMono) {
             double v = r2;  // slot kMono: largest squared offset
