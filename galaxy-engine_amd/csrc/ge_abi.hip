@@ -89,7 +89,7 @@ void free_all(ge_ctx *c) {
     dfree(T.blist); dfree(T.blist_off); dfree(T.blist_cnt);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
     merger_free(c);
-    dfree(b.node_temp); dfree(b.node_parent); dfree(b.node_arrive); dfree(b.materials);
+    dfree(b.node_temp); dfree(b.node_tavg); dfree(b.node_parent); dfree(b.node_arrive); dfree(b.materials);
     dfree(b.cons); dfree(b.cons_tmp); dfree(b.cidx); dfree(b.cidx_tmp); dfree(b.ccolor); dfree(b.ccolor_tmp);
     dfree(b.ckeep); dfree(b.cblock); dfree(b.ccount); dfree(b.corder); dfree(b.cstart); dfree(b.cw_a);
     dfree(b.cw_b); dfree(b.chist);
@@ -174,6 +174,7 @@ int alloc_all(ge_ctx *c) {
     }
     if (c->thermal) {
         GE_HIP_CHECK(c, dalloc(b.node_temp, b.node_cap));
+        GE_HIP_CHECK(c, dalloc(b.node_tavg, b.node_cap));
         GE_HIP_CHECK(c, dalloc(b.node_parent, b.node_cap));
         GE_HIP_CHECK(c, dalloc(b.node_arrive, b.node_cap));
         GE_HIP_CHECK(c, dalloc(b.materials, GE_MAX_MATERIALS));

@@ -168,6 +168,7 @@ struct Buffers {
     // node, the bottom-up pass's parent links and arrival counts, the
     // material table
     float *node_temp = nullptr;
+    float *node_tavg = nullptr;  // gridTemp / particle count of each node (what the walk reads)
     uint32_t *node_parent = nullptr, *node_arrive = nullptr;
     ge_material *materials = nullptr;
     // particle constraints (max_constraints): the constraints and their

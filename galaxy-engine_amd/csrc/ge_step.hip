@@ -255,7 +255,7 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                                            float *__restrict__ vx, float *__restrict__ vy,
                                            const uint32_t *__restrict__ flags, float half_dt_end,
                                            Domain dom, unsigned long long *__restrict__ inter,
-                                           const float *__restrict__ gtemp, float *__restrict__ temp,
+                                           const float *__restrict__ gtavg, float *__restrict__ temp,
                                            float heat_k, float heat_dt) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * BS + threadIdx.x;
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
 #pragma clang fp contract(off)
                 const uint32_t cn = nd.end - nd.start;
                 if (cn > 0u) {
-                    const float avg = gtemp[idx] / (float)cn;
+                    const float avg = gtavg[idx];  // gridTemp / cn, divided once per node
                     const float diff = avg - ti;
                     const float invd = 1.0f / sqrtf(dist_sq);
                     float dist = 0.0f;
@@ -633,7 +633,7 @@ void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER, TEMP>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
                        c->b.nodes, c->b.node_mp, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n,
                        p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p),
-                       inter, c->b.node_temp, s.temp, c->thermal_p.heat_conductivity, p->dt);
+                       inter, c->b.node_tavg, s.temp, c->thermal_p.heat_conductivity, p->dt);
 }
 
 template <int BS, int ORDER, bool TEMP>

@@ -185,6 +185,20 @@ void launch_thermal_fill(ge_ctx *c, const State &s, uint32_t at, uint32_t cnt) {
                        s.ke, s.prev_ke, at, cnt);
 }
 
+// The average temperature of each node, gridTemp / particle count, as the
+// walk's heat transfer divides it (physics.cpp:261-276): once per node here
+// instead of once per accepting wave and node.
+__global__ __launch_bounds__(kBlock) void k_node_tavg(const NodeRec *__restrict__ nodes,
+                                                      const uint32_t *__restrict__ counters,
+                                                      const float *__restrict__ node_temp,
+                                                      float *__restrict__ tavg) {
+    const uint32_t nn = node_total(counters);
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nn) return;
+    const uint32_t cn = nodes[k].end - nodes[k].start;
+    tavg[k] = cn > 0u ? node_temp[k] / (float)cn : 0.0f;
+}
+
 void launch_node_temps(ge_ctx *c) {
     const State &s = c->b.st[c->b.cur];
     const uint32_t cap = c->b.node_cap;
@@ -195,6 +209,8 @@ void launch_node_temps(ge_ctx *c) {
                        c->b.node_parent);
     hipLaunchKernelGGL(k_node_temps_up, dim3(nb), dim3(kBlock), 0, c->stream, c->b.nodes, c->b.counters,
                        c->b.node_parent, c->b.node_arrive, s.temp, c->n, c->b.node_temp);
+    hipLaunchKernelGGL(k_node_tavg, dim3(nb), dim3(kBlock), 0, c->stream, c->b.nodes, c->b.counters,
+                       c->b.node_temp, c->b.node_tavg);
 }
 
 void launch_temperature(ge_ctx *c, const ge_step_params *p) {
