@@ -42,7 +42,11 @@ int ge_autotune(ge_ctx *ctx, const ge_step_params *base, int32_t frames, ge_tune
  * read back and dropped and waiting requests join, while the others stay
  * resident in HBM.  Gravity + SPH requests each take a single-system SPH
  * context of their own (one stream each) and run concurrently with the
- * batch.  Admission is first come, first served, within an HBM budget: the
+ * batch, round_frames frames per chunk.  Batch rounds and SPH chunks run
+ * asynchronously: the loop wakes on whichever finishes first, so an SPH
+ * context gets its next chunk as soon as its last one ends (the batch
+ * changes membership only between its rounds).  Admission is first come,
+ * first served, within an HBM budget: the
  * contexts are sized from the budget at creation, and a request is admitted
  * when its particles fit the free capacity (and, for SPH, a free context).
  * With batching = 0 the server is the plain form: one request on the device
