@@ -202,11 +202,19 @@ __device__ uint32_t wave_subtree(const float *__restrict__ x, const float *__res
     // ---- leaf masses: computeLeafMass order, every leaf at once
     const float lm = have ? m[s0 + lane] : 0.0f;
     float lgm = 0.0f, lcx = 0.0f, lcy = 0.0f;
-    for (uint32_t j = 0; j < cnt; ++j) {
-        const float mj = readlane_f(lm, j);
-        const float xj = readlane_f(lx, j);
-        const float yj = readlane_f(ly, j);
-        if (j >= lane && j < leaf_end) {
+    // a leaf's start lane sums its particles lane, lane + 1, ... in order,
+    // shuffled in; the loop runs for the longest leaf of the wave
+    const uint32_t span = leaf_end > lane ? leaf_end - lane : 0u;
+    uint32_t maxspan = span;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) maxspan = max(maxspan, (uint32_t)__shfl_xor((int)maxspan, o, 64));
+    maxspan = __builtin_amdgcn_readfirstlane(maxspan);
+    for (uint32_t t = 0; t < maxspan; ++t) {
+        const uint32_t src = min(lane + t, 63u);  // every lane takes part in the shuffles
+        const float mj = shfl_f(lm, src);
+        const float xj = shfl_f(lx, src);
+        const float yj = shfl_f(ly, src);
+        if (t < span) {
             lgm = lgm + mj;
             lcx = lcx + xj * mj;
             lcy = lcy + yj * mj;
