@@ -428,7 +428,12 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_level(TopNodes3 T, StartTa
     }
 }
 
-__global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartTable st, const float *__restrict__ x,
+// Workgroup size of the subtree phase: more waves take a level's nodes in
+// parallel (its LDS use does not depend on it)
+constexpr int kSubBlock3 = 1024;
+constexpr int kSubWaves3 = kSubBlock3 / 64;
+
+__global__ __launch_bounds__(kSubBlock3) void k3_tree_blocks(TopNodes3 T, StartTable st, const float *__restrict__ x,
                                                              const float *__restrict__ y,
                                                              const float *__restrict__ z,
                                                              uint32_t *__restrict__ counters) {
@@ -451,7 +456,7 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
         const uint32_t rs = T.start[root], re = T.end[root];
         const bool staged = re > rs && re - rs <= kBlockSubMax;
         if (staged) {
-            for (uint32_t i = threadIdx.x; i < re - rs; i += kTreeBlock) {
+            for (uint32_t i = threadIdx.x; i < re - rs; i += kSubBlock3) {
                 sx[i] = x[rs + i];
                 sy[i] = y[rs + i];
                 sz[i] = z[rs + i];
@@ -477,7 +482,7 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
             // A: every node of the level finished or split (one wave per
             // node); a split node parks its seven inner partition points in
             // its own child slots and takes its children's offset in LDS
-            for (uint32_t i = w; i < nf; i += kTreeWaves) {
+            for (uint32_t i = w; i < nf; i += kSubWaves3) {
                 const uint32_t node = front[cur][i];
                 const float px = T.px[node], py = T.py[node], pz = T.pz[node], size = T.size[node];
                 const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
@@ -510,7 +515,7 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
             // B: the children records, one thread per split node
             const uint32_t base = lbase;
             if (base != kNone) {
-                for (uint32_t i = threadIdx.x; i < nf; i += kTreeBlock) {
+                for (uint32_t i = threadIdx.x; i < nf; i += kSubBlock3) {
                     const uint32_t o = soff[i];
                     if (o == kNone) continue;
                     const uint32_t node = front[cur][i];
@@ -541,7 +546,7 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_blocks(TopNodes3 T, StartT
         }
         __syncthreads();
         if (nl <= kBlockListMax && bbase + nl <= T.cap)
-            for (uint32_t i = threadIdx.x; i < nl; i += kTreeBlock) T.blist[bbase + i] = blist[i];
+            for (uint32_t i = threadIdx.x; i < nl; i += kSubBlock3) T.blist[bbase + i] = blist[i];
         __syncthreads();
     }
 }
@@ -734,7 +739,7 @@ void launch_octree_build(ge3_ctx *c, const float *bb_device) {
     }
     uint64_t roots = n / (kWaveSubMax + 1) + 1;
     if (roots > 4096) roots = 4096;
-    hipLaunchKernelGGL(k3_tree_blocks, dim3((uint32_t)roots), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y, s.z,
+    hipLaunchKernelGGL(k3_tree_blocks, dim3((uint32_t)roots), dim3(kSubBlock3), 0, c->stream, T, st, s.x, s.y, s.z,
                        c->b.counters);
     const uint32_t len = c->n + 1;
     const uint32_t parts = (uint32_t)((len + kScanTile - 1) / kScanTile);

@@ -423,9 +423,14 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTabl
     }
 }
 
+// Workgroup size of the subtree phase: more waves take a level's nodes in
+// parallel (its LDS use does not depend on it)
+constexpr int kSubBlock = 512;
+constexpr int kSubWaves = kSubBlock / 64;
+
 // One workgroup per subtree root handed over by the level phase: the level
 // loop of that subtree with its frontier in LDS.
-__global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTable st, const float *__restrict__ x,
+__global__ __launch_bounds__(kSubBlock) void k_tree_blocks(TopNodes T, StartTable st, const float *__restrict__ x,
                                                             const float *__restrict__ y,
                                                             uint32_t *__restrict__ counters) {
     __shared__ uint32_t front[2][kFrontMax];
@@ -449,7 +454,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
         const uint32_t rs = T.start[root], re = T.end[root];
         const bool staged = re > rs && re - rs <= kBlockSubMax;
         if (staged) {
-            for (uint32_t i = threadIdx.x; i < re - rs; i += kTreeBlock) {
+            for (uint32_t i = threadIdx.x; i < re - rs; i += kSubBlock) {
                 sx[i] = x[rs + i];
                 sy[i] = y[rs + i];
             }
@@ -472,7 +477,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
             if (nf == 0) break;
             // A: every node of the level finished or split (one wave per
             // node); split nodes take their children's offsets in LDS
-            for (uint32_t i = w; i < nf; i += kTreeWaves) {
+            for (uint32_t i = w; i < nf; i += kSubWaves) {
                 const uint32_t node = front[cur][i];
                 const float px = T.px[node], py = T.py[node], size = T.size[node];
                 const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
@@ -503,7 +508,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
             // B: the children records, one thread per split node
             const uint32_t base = lbase;
             if (base != kNone) {
-                for (uint32_t i = threadIdx.x; i < nf; i += kTreeBlock) {
+                for (uint32_t i = threadIdx.x; i < nf; i += kSubBlock) {
                     const uint32_t o = soff[i];
                     if (o == kNone) continue;
                     const uint32_t node = front[cur][i];
@@ -532,7 +537,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_blocks(TopNodes T, StartTab
         }
         __syncthreads();
         if (nl <= kBlockListMax && bbase + nl <= T.cap)
-            for (uint32_t i = threadIdx.x; i < nl; i += kTreeBlock) T.blist[bbase + i] = blist[i];
+            for (uint32_t i = threadIdx.x; i < nl; i += kSubBlock) T.blist[bbase + i] = blist[i];
         __syncthreads();
     }
 }
@@ -734,7 +739,7 @@ void launch_tree_build(ge_ctx *c) {
     // roots of workgroup subtrees each hold more than 64 particles
     uint64_t roots = n / (kWaveSubMax + 1) + c->nsys;
     if (roots > 4096) roots = 4096;
-    hipLaunchKernelGGL(k_tree_blocks, dim3((uint32_t)roots), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y,
+    hipLaunchKernelGGL(k_tree_blocks, dim3((uint32_t)roots), dim3(kSubBlock), 0, c->stream, T, st, s.x, s.y,
                        c->b.counters);
     const uint32_t len = c->n + 1;
     const uint32_t parts = (uint32_t)((len + kScanTile - 1) / kScanTile);
