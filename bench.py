@@ -634,7 +634,7 @@ def served(a, accuracy):
                "sph_requests": sum(r["sph"] for r in reqs),
                "particle_updates": int(sum(len(r["state"]["x"]) * r["steps"] for r in reqs))}
         for form, batching in (("served", 1), ("plain", 0)):
-            cfg = S.default_server_config(batching=batching, sph_slots=4, sph_capacity=1 << 19,
+            cfg = S.default_server_config(batching=batching, sph_slots=6, sph_capacity=1 << 19,
                                           batch_capacity=1 << 25, max_systems=512, round_frames=8, use_graph=1)
             with S.Server(cfg, pg, ps) as srv:
                 _, res, stats = srv.run(reqs, ("x",))
