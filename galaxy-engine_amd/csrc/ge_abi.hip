@@ -361,11 +361,7 @@ int check_device_errors(ge_ctx *c) {
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     if (e) {
         char buf[200];
-        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s%s%s%s%s)", e,
-                      (e & kErrNodeCap) ? "node capacity " : "", (e & kErrDepth) ? "tree depth " : "",
-                      (e & kErrGridRange) ? "SPH grid range " : "", (e & kErrNeighborCap) ? "neighbour cap " : "",
-                      (e & kErrFrontier) ? "tree frontier " : "", (e & kErrMultipole) ? "multipole lists " : "",
-                      (e & kErrSort) ? "sort wait" : "");
+        describe_device_errors(e, buf, sizeof buf);
         c->err = buf;
         uint32_t z = 0;
         (void)hipMemcpy(c->b.counters + kCntError, &z, sizeof z, hipMemcpyHostToDevice);

@@ -182,9 +182,8 @@ int check_device_errors(ge3_ctx *c) {
     GE3_CHECK(c, hipMemcpyAsync(&e, c->b.counters + kCntError, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     GE3_CHECK(c, hipStreamSynchronize(c->stream));
     if (e) {
-        char buf[160];
-        std::snprintf(buf, sizeof buf, "device error flags 0x%x (%s%s%s)", e, (e & kErrNodeCap) ? "node capacity " : "",
-                      (e & kErrDepth) ? "tree depth " : "", (e & kErrFrontier) ? "tree frontier" : "");
+        char buf[200];
+        describe_device_errors(e, buf, sizeof buf);
         c->err = buf;
         uint32_t z = 0;
         (void)hipMemcpy(c->b.counters + kCntError, &z, sizeof z, hipMemcpyHostToDevice);

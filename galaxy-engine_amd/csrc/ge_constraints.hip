@@ -390,7 +390,7 @@ void incidence(ge_ctx *c, uint32_t m) {
     Buffers &b = c->b;
     hipLaunchKernelGGL(k_inc_words, dim3(blocks_for(m, kCBlock)), dim3(kCBlock), 0, c->stream, b.cidx, m, b.cw_a,
                        b.ccolor, b.ckeep);  // the keep flags are free until the next solve maps ids again
-    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, 2 * m, 32, 64);
+    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, 2 * m, 32, 64, b.counters + kCntError);
 }
 
 }  // namespace
@@ -443,7 +443,7 @@ int cons_solve(ge_ctx *c, const ge_step_params *p, bool map_done) {
     // 1. ids -> indices, keep flags
     if (!map_done) {
         hipLaunchKernelGGL(k_id_words, dim3(blocks_for(n, kCBlock)), dim3(kCBlock), 0, c->stream, s.id, n, b.cw_a);
-        radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, n, 32, 64);
+        radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, n, 32, 64, b.counters + kCntError);
         hipLaunchKernelGGL(k_map_ids, dim3(nbc), dim3(kCBlock), 0, c->stream, b.cons, b.ccount, b.cw_a, n, b.cidx,
                            b.ckeep);
     }
@@ -458,12 +458,12 @@ int cons_solve(ge_ctx *c, const ge_step_params *p, bool map_done) {
     hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, c->stream, b.ccount, b.cblock + nbc, b.cframe);
     // 3. triangle rule on every fifth counted frame
     hipLaunchKernelGGL(k_adj_words, dim3(nbc), dim3(kCBlock), 0, c->stream, b.cidx, b.ccount, bound, b.cw_a);
-    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, 2 * bound, 0, 64);
+    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, 2 * bound, 0, 64, b.counters + kCntError);
     hipLaunchKernelGGL(k_triangles, dim3(nbc), dim3(kCBlock), 0, c->stream, b.cons, b.cidx, b.ccount, b.cframe, b.cw_a,
                        2 * bound);
     // colour order of the remaining constraints
     hipLaunchKernelGGL(k_color_words, dim3(nbc), dim3(kCBlock), 0, c->stream, b.ccolor, b.ccount, bound, b.cw_a);
-    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, bound, 0, 40);
+    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, bound, 0, 40, b.counters + kCntError);
     hipLaunchKernelGGL(k_color_table, dim3(blocks_for((uint64_t)bound + kMaxColors + 1, kCBlock)), dim3(kCBlock), 0,
                        c->stream, b.cw_a, b.ccount, bound, c->ncolors, b.corder, b.cstart);
     // 4. substeps
@@ -494,7 +494,7 @@ int cons_prepare(ge_ctx *c) {
     const State &s = c->b.st[c->b.cur];
     const uint32_t n = c->n, bound = c->cons_bound;
     hipLaunchKernelGGL(k_id_words, dim3(blocks_for(n, kCBlock)), dim3(kCBlock), 0, c->stream, s.id, n, b.cw_a);
-    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, n, 32, 64);
+    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, n, 32, 64, b.counters + kCntError);
     hipLaunchKernelGGL(k_map_ids, dim3(blocks_for(bound, kCBlock)), dim3(kCBlock), 0, c->stream, b.cons, b.ccount,
                        b.cw_a, n, b.cidx, b.ckeep);
     return cons_color(c);
@@ -588,7 +588,7 @@ int ge_constraints_order(ge_ctx *c, uint32_t *order, uint32_t cap, uint32_t *col
     Buffers &b = c->b;
     const uint32_t nbc = blocks_for(bound, kCBlock);
     hipLaunchKernelGGL(k_color_words, dim3(nbc), dim3(kCBlock), 0, c->stream, b.ccolor, b.ccount, bound, b.cw_a);
-    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, bound, 0, 40);
+    radix_sort_words(c->stream, b.chist, b.cw_a, b.cw_b, bound, 0, 40, b.counters + kCntError);
     hipLaunchKernelGGL(k_color_table, dim3(blocks_for((uint64_t)bound + kMaxColors + 1, kCBlock)), dim3(kCBlock), 0,
                        c->stream, b.cw_a, b.ccount, bound, c->ncolors, b.corder, b.cstart);
     const uint32_t w = bound < cap ? bound : cap;

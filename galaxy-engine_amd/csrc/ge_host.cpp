@@ -189,6 +189,7 @@ struct ge_server {
     };
     std::vector<SphSlot> sph;
     uint64_t device_bytes = 0;
+    uint64_t round_limit = 0;  // ge_server_set_round_limit (0: none)
     std::string err;
 };
 
@@ -197,6 +198,29 @@ namespace {
 int server_fail(ge_server *s, int code, const std::string &msg) {
     if (s) s->err = msg;
     return code;
+}
+
+// Returns every context to the idle state a run starts from: waits for the
+// chunks and rounds still queued, forgets which request each SPH context held,
+// and empties the batch.  Runs at the start of every ge_server_run and on each
+// of its error returns, so a run that ended early leaves no slot that a later
+// run would take for busy (or finish with a request index of the old run).
+void server_reset(ge_server *s) {
+    for (auto &slot : s->sph) {
+        if (slot.ctx) (void)ge_sync(slot.ctx);
+        slot.req = -1;
+        slot.remaining = 0;
+        slot.inflight = 0;
+    }
+    if (s->batch) {
+        (void)ge_sync(s->batch);
+        (void)ge_systems_update(s->batch, 0, nullptr, 0, nullptr, nullptr);
+    }
+}
+
+int server_abort(ge_server *s, int code, const std::string &msg) {
+    server_reset(s);
+    return server_fail(s, code, msg);
 }
 
 // Particles of a gravity request staged for one ge_systems_update call.
@@ -350,6 +374,12 @@ int ge_server_destroy(ge_server *s) {
 
 const char *ge_server_last_error(const ge_server *s) { return s ? s->err.c_str() : "null server"; }
 
+int ge_server_set_round_limit(ge_server *s, uint64_t max_rounds) {
+    if (!s) return GE_E_INVALID;
+    s->round_limit = max_rounds;
+    return GE_OK;
+}
+
 int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_request_result *res,
                   ge_server_stats *stats) {
     if (!s || (nreq && (!reqs || !res))) return GE_E_INVALID;
@@ -371,9 +401,13 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
     std::deque<uint32_t> waiting;
     uint32_t next = 0, done = 0, max_resident = 0;
     uint64_t rounds = 0;
-    // start from an empty batch
-    if (int r = ge_systems_update(s->batch, 0, nullptr, 0, nullptr, nullptr))
-        return server_fail(s, r, ge_last_error(s->batch));
+    // start from idle contexts and an empty batch, whatever the last run left
+    server_reset(s);
+    if (s->batch) {
+        uint32_t left = 1;
+        if (ge_count(s->batch, &left) != GE_OK || left != 0)
+            return server_abort(s, GE_E_HIP, "the batch context could not be emptied");
+    }
     const Clock::time_point t0 = Clock::now();
     auto finish = [&](uint32_t q, int status) {
         res[q].done_s = seconds_since(t0);
@@ -391,6 +425,8 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
     // changes membership only between rounds.
     int32_t batch_R = 0;  // frames of the batch round in flight (0: the batch is idle)
     while (done < nreq) {
+        if (s->round_limit && rounds >= s->round_limit)
+            return server_abort(s, GE_E_CAPACITY, "round limit reached");
         const double now = seconds_since(t0);
         while (next < nreq && reqs[order[next]].arrival_s <= now) waiting.push_back(order[next++]);
         const bool batch_idle = batch_R == 0;
@@ -477,7 +513,7 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
             ge_particles pv = stage.view();
             const int rc = ge_systems_update(s->batch, (uint32_t)keep.size(), keep.data(),
                                              (uint32_t)add_counts.size(), add_counts.data(), &pv);
-            if (rc != GE_OK) return server_fail(s, rc, ge_last_error(s->batch));
+            if (rc != GE_OK) return server_abort(s, rc, ge_last_error(s->batch));
             resident.swap(next_res);
         }
         uint32_t on_device = (uint32_t)resident.size();
@@ -488,7 +524,7 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
         if (batch_idle && !resident.empty()) {
             int32_t R = (int32_t)cfg.round_frames;
             for (auto &r : resident) R = std::min(R, r.remaining);
-            if (int rc = ge_step(s->batch, &s->pg, R)) return server_fail(s, rc, ge_last_error(s->batch));
+            if (int rc = ge_step(s->batch, &s->pg, R)) return server_abort(s, rc, ge_last_error(s->batch));
             batch_R = R;
             ++rounds;
         }
@@ -516,7 +552,7 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
             // one context on the device and nothing that could start beside
             // it: block on it
             if (batch_R > 0) {
-                if (int rc = ge_sync(s->batch)) return server_fail(s, rc, ge_last_error(s->batch));
+                if (int rc = ge_sync(s->batch)) return server_abort(s, rc, ge_last_error(s->batch));
             } else {
                 (void)ge_sync(only->ctx);  // its status is read with the completion below
             }
@@ -543,7 +579,7 @@ int ge_server_run(ge_server *s, uint32_t nreq, const ge_request *reqs, ge_reques
         if (batch_R > 0) {
             int32_t d = 0;
             if (ge_poll(s->batch, &d) == GE_OK && d) {
-                if (int rc = ge_sync(s->batch)) return server_fail(s, rc, ge_last_error(s->batch));
+                if (int rc = ge_sync(s->batch)) return server_abort(s, rc, ge_last_error(s->batch));
                 for (auto &r : resident) r.remaining -= batch_R;
                 batch_R = 0;
                 for (uint32_t t = 0; t < resident.size(); ++t) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,15 @@ enum : uint32_t {
     kErrMultipole = 32u, // multipole work lists overflowed
     kErrSort = 64u,      // a radix-sort look-back wait passed its spin limit
 };
+
+// Text for a set of device error flags ("node capacity tree depth ..."), used
+// by the 2D and 3D contexts' reports.
+inline void describe_device_errors(uint32_t e, char *buf, size_t len) {
+    std::snprintf(buf, len, "device error flags 0x%x (%s%s%s%s%s%s%s)", e, (e & kErrNodeCap) ? "node capacity " : "",
+                  (e & kErrDepth) ? "tree depth " : "", (e & kErrGridRange) ? "SPH grid range " : "",
+                  (e & kErrNeighborCap) ? "neighbour cap " : "", (e & kErrFrontier) ? "tree frontier " : "",
+                  (e & kErrMultipole) ? "multipole lists " : "", (e & kErrSort) ? "sort wait" : "");
+}
 
 // counters block (device, uint32 each)
 enum CounterSlot : int {
@@ -252,14 +262,15 @@ namespace ge {
 
 // ---- launchers (each defined next to its kernels) ----
 // sort.hip
-// err: where a look-back wait that passed its spin limit raises kErrSort
-// (null: a word at the end of hist)
+// err: the context's error word (counters + kCntError), where a look-back
+// wait that passed its spin limit raises kErrSort; every caller passes it so
+// that the next ge_sync reports a sort that gave up
 void radix_sort_words(hipStream_t stream, uint32_t *hist, uint64_t *&data, uint64_t *&alt, uint32_t n, int lo_bit,
-                      int hi_bit, uint32_t *err = nullptr);
+                      int hi_bit, uint32_t *err);
 void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n,
                        int lo_bit, int hi_bit);
 void radix_sort_kv(hipStream_t stream, uint32_t *hist, uint64_t *&keys, uint64_t *&keys_alt, uint32_t *&vals,
-                   uint32_t *&vals_alt, uint32_t n, int lo_bit, int hi_bit, uint32_t *err = nullptr);
+                   uint32_t *&vals_alt, uint32_t n, int lo_bit, int hi_bit, uint32_t *err);
 uint32_t radix_hist_entries(uint32_t cap);
 // step.hip
 void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate,

@@ -488,7 +488,7 @@ int merger_run(ge_ctx *c, const ge_merger_params *mp, std::vector<uint32_t> &del
                        reinterpret_cast<MGrid *>(M.grid), M.flag);
     hipLaunchKernelGGL(k_m_words, dim3(nbm), dim3(kMBlock), 0, c->stream, s.x, s.y, s.flags, n,
                        reinterpret_cast<const MGrid *>(M.grid), M.wa);
-    radix_sort_words(c->stream, M.hist, M.wa, M.wb, n, kIdxBits, kIdxBits + kCellKeyBits);
+    radix_sort_words(c->stream, M.hist, M.wa, M.wb, n, kIdxBits, kIdxBits + kCellKeyBits, c->b.counters + kCntError);
     const float dr2 = mp->density_radius * mp->density_radius;
     const MGrid *gp = reinterpret_cast<const MGrid *>(M.grid);
     hipLaunchKernelGGL(k_m_edges, dim3(nbm), dim3(kMBlock), 0, c->stream, s.x, s.y, s.flags, M.rad, n, gp, M.wa, dr2,
@@ -516,7 +516,7 @@ int merger_run(ge_ctx *c, const ge_merger_params *mp, std::vector<uint32_t> &del
     hipLaunchKernelGGL(k_m_union, dim3(nbm), dim3(kMBlock), 0, c->stream, M.eoff, M.edges, n, M.parent);
     hipLaunchKernelGGL(k_m_comp_words, dim3(nbm), dim3(kMBlock), 0, c->stream, M.eoff, M.parent, n, M.wb);
     uint64_t *cw = M.wb, *ca = M.wa;  // the cell words are no longer needed
-    radix_sort_words(c->stream, M.hist, cw, ca, n, 0, 64);
+    radix_sort_words(c->stream, M.hist, cw, ca, n, 0, 64, c->b.counters + kCntError);
     hipLaunchKernelGGL(k_m_merge, dim3(nbm), dim3(kMBlock), 0, c->stream, cw, n, M.eoff, M.edges, s.m, s.vx, s.vy,
                        prev, M.del);
     hipLaunchKernelGGL(k_m_del_count, dim3(nbm), dim3(kMBlock), 0, c->stream, M.del, n, M.bc);

@@ -517,7 +517,8 @@ Grid3View view3(const Grid3 &g) { return Grid3View{g.first, g.ord, g.cell, g.pk0
 void build_slots(ge3_ctx *c, Grid3 &g) {
     const uint32_t n = c->n;
     const uint32_t nslots = 1u << c->b.slot_bits;
-    radix_sort_words(c->stream, c->b.hist, g.keys, g.keys_alt, n, kIdxBits, kIdxBits + (int)c->b.slot_bits + 1);
+    radix_sort_words(c->stream, c->b.hist, g.keys, g.keys_alt, n, kIdxBits, kIdxBits + (int)c->b.slot_bits + 1,
+                     c->b.counters + kCntError);
     hipLaunchKernelGGL(k3_slot_table, dim3(blocks_for((uint64_t)nslots + 1, kBlock)), dim3(kBlock), 0, c->stream,
                        g.keys, n, nslots, g.first);
 }

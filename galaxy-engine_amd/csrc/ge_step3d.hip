@@ -497,7 +497,8 @@ void launch3_morton_keys(ge3_ctx *c, const float *bb_device) {
 }
 
 void launch3_sort_gather(ge3_ctx *c) {
-    radix_sort_kv(c->stream, c->b.hist, c->b.sk_a, c->b.sk_b, c->b.sv_a, c->b.sv_b, c->n, 0, kMortonBits3);
+    radix_sort_kv(c->stream, c->b.hist, c->b.sk_a, c->b.sk_b, c->b.sv_a, c->b.sv_b, c->n, 0, kMortonBits3,
+                  c->b.counters + kCntError);
     const int cur = c->b.cur;
     hipLaunchKernelGGL(k3_gather, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.st[cur],
                        c->b.st[1 - cur], c->n, c->sph ? 1 : 0, c->b.sv_a, c->b.perm, c->b.pxyzm);

@@ -8,7 +8,7 @@ from .lib import (FIELDS_SPH, FIELDS_U, GE_OK, Context, GeError, GeParticles, PA
                   StepParams, load_library)
 
 SERVE_EXPORTS = ["ge_autotune", "ge_server_default_config", "ge_server_create", "ge_server_destroy",
-                 "ge_server_last_error", "ge_server_run"]
+                 "ge_server_last_error", "ge_server_run", "ge_server_set_round_limit"]
 
 
 class TuneResult(C.Structure):
@@ -58,6 +58,7 @@ def _lib():
             "ge_server_destroy": (C.c_int, [vp]),
             "ge_server_last_error": (C.c_char_p, [vp]),
             "ge_server_run": (C.c_int, [vp, C.c_uint32, P(Request), P(RequestResult), P(ServerStats)]),
+            "ge_server_set_round_limit": (C.c_int, [vp, C.c_uint64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -125,6 +126,13 @@ class Server:
             self.close()
         except Exception:
             pass
+
+    def set_round_limit(self, max_rounds):
+        """ge_server_set_round_limit: later runs stop after max_rounds batch
+        rounds (0: no limit)."""
+        rc = self.lib.ge_server_set_round_limit(self.h, int(max_rounds))
+        if rc != GE_OK:
+            raise GeError(rc, "ge_server_set_round_limit failed")
 
     def run(self, requests, out_fields=("x", "y", "vx", "vy", "ax", "ay", "id")):
         nreq = len(requests)

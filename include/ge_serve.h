@@ -105,9 +105,18 @@ int ge_server_create(ge_server **out, const ge_server_config *cfg, const ge_step
                      const ge_step_params *sph);
 int ge_server_destroy(ge_server *srv);
 const char *ge_server_last_error(const ge_server *srv);
-/* runs nreq requests (any arrival order) to completion in real time */
+/* runs nreq requests (any arrival order) to completion in real time.  Every
+ * run starts from idle contexts and an empty batch, whatever the previous run
+ * left: a run that returned an error (its unfinished requests keep done_s =
+ * -1) does not affect the next one. */
 int ge_server_run(ge_server *srv, uint32_t nreq, const ge_request *reqs, ge_request_result *res,
                   ge_server_stats *stats);
+/* A bounded run: ge_server_run stops with GE_E_CAPACITY ("round limit
+ * reached") once it has launched max_rounds batch rounds, after waiting for
+ * the work still queued; 0 (the default) removes the limit.  Lets a caller
+ * drain a server on a deadline; requests not finished by then keep
+ * done_s = -1. */
+int ge_server_set_round_limit(ge_server *srv, uint64_t max_rounds);
 
 #ifdef __cplusplus
 }

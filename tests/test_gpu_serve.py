@@ -118,3 +118,24 @@ def test_autotune_choice_and_state_restored():
         da, db = a.download(list(OUT)), b.download(list(OUT))
     for f in OUT:
         np.testing.assert_array_equal(da[f], db[f], err_msg=f)
+
+
+def test_run_after_an_aborted_run_is_unaffected():
+    """A run stopped by the round limit while batch rounds and SPH chunks are
+    in flight leaves nothing behind: the next run on the same server gives
+    every request the result it has alone."""
+    reqs = _requests(seed=23)
+    pg, ps = _gravity_params(), _sph_params()
+    with S.Server(_cfg(round_frames=1), pg, ps) as srv:
+        srv.set_round_limit(2)
+        with pytest.raises(S.GeError) as e:
+            srv.run(reqs, OUT)
+        assert e.value.code == -5  # GE_E_CAPACITY: round limit reached
+        srv.set_round_limit(0)
+        outs, results, stats = srv.run(reqs, OUT)
+    assert stats["completed"] == len(reqs)
+    assert all(r["status"] == 0 for r in results)
+    for k, rq in enumerate(reqs):
+        ref = _alone(rq, pg, ps)
+        for f in OUT:
+            np.testing.assert_array_equal(outs[k][f], ref[f], err_msg=f"request {k} field {f}")
