@@ -578,6 +578,12 @@ int ge_create(ge_ctx **out, const ge_config *cfg) {
         delete c;
         return GE_E_HIP;
     }
+    // the fixed scratch of the entry point that allocates nothing, once per device
+    if (int r = noalloc_setup(c->device)) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return r;
+    }
     t_alloc_bytes = 0;
     int ra = alloc_all(c);
     c->device_bytes = t_alloc_bytes;

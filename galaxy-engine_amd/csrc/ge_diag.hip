@@ -3,6 +3,9 @@
 // these (its UI reports particle counts and FPS only, GE/src/UI/UI.cpp);
 // they are the measurement side of the accuracy form: FP64 is used here and
 // nowhere on the step itself.
+#include <atomic>
+#include <mutex>
+
 #include "ge_internal.hpp"
 
 namespace ge {
@@ -113,6 +116,33 @@ __global__ __launch_bounds__(kBlock) void k_pack_pxym(State s, uint32_t n, float
     if (i < n) pxym[i] = make_float4(s.x[i], s.y[i], s.m[i], 0.0f);
 }
 
+// Final sums of the block partials on the device: thread k < 7 adds moment k
+// of blocks 0, 1, ..., nb-1 in that order, thread 7 the nbp potential
+// partials, each from 0.0, which is the order and rounding of the host loop
+// in ge_diagnostics, so both entry points return the same bits.
+__global__ __launch_bounds__(64) void k_diag_final(const double *__restrict__ part, uint32_t nb, uint32_t nbp,
+                                                    double *__restrict__ sums) {
+    const uint32_t k = threadIdx.x;
+    if (k < 7) {
+        double v = 0.0;
+        for (uint32_t b = 0; b < nb; ++b) v += part[7ull * b + k];
+        sums[k] = v;
+    } else if (k == 7) {
+        double v = 0.0;
+        for (uint32_t b = 0; b < nbp; ++b) v += part[7ull * nb + b];
+        sums[7] = v;
+    }
+}
+
+// Scratch of ge_diagnostics_noalloc: moment partials of at most
+// kNoAllocMomentBlocks blocks, potential partials of at most
+// kNoAllocPotentialBlocks blocks, then the 8 final sums.  188 KiB per device,
+// held until the process ends.
+constexpr uint32_t kNoAllocMomentBlocks = 1024;
+constexpr uint32_t kNoAllocPotentialBlocks = 16384;  // potential over at most 4,194,304 particles
+constexpr size_t kNoAllocDoubles = 7ull * kNoAllocMomentBlocks + kNoAllocPotentialBlocks + 8;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
 }  // namespace
 
 }  // namespace ge
@@ -125,6 +155,33 @@ int diag_fail(ge_ctx *c, int code, const char *msg) {
     c->err = msg;
     return code;
 }
+
+// Device address of `scratch` per device.  In host code the array's name is
+// only a stand-in for the symbol, so kernels and copies get this address,
+// looked up once per device by ge_noalloc_setup with that device current.
+constexpr int kMaxDevices = 64;
+std::atomic<double *> g_scratch[kMaxDevices];
+std::mutex g_scratch_mu;
+
+}  // namespace
+
+namespace ge {
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return GE_E_NODEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device].load()) return GE_OK;
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != device) return GE_E_HIP;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return GE_E_HIP;
+    g_scratch[device].store(static_cast<double *>(ptr));
+    return GE_OK;
+}
+
+}  // namespace ge
+
+namespace {
 
 // device scratch for partial sums; allocated on first use, kept by the context
 int diag_scratch(ge_ctx *c, size_t doubles, double *&out) {
@@ -179,6 +236,46 @@ int ge_diagnostics(ge_ctx *c, uint32_t sys, const ge_step_params *p, int32_t wit
     out->comx = v[0] > 0 ? v[5] / v[0] : 0.0;
     out->comy = v[0] > 0 ? v[6] / v[0] : 0.0;
     out->potential = with_potential ? -0.5 * (double)p->G * pot : 0.0;
+    return GE_OK;
+}
+
+int ge_diagnostics_noalloc(ge_ctx *c, uint32_t sys, const ge_step_params *p, int32_t with_potential,
+                           ge_diag *out) {
+    if (!c || !out || (with_potential && !p)) return GE_E_INVALID;
+    if (c->n == 0 || sys >= c->nsys) return diag_fail(c, GE_E_INVALID, "no such system");
+    double *part = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch[c->device].load() : nullptr;
+    if (!part) return diag_fail(c, GE_E_HIP, "no scratch address for this device");
+    const uint32_t lo = c->host_off[sys], hi = c->host_off[sys + 1];
+    const uint32_t cnt = hi - lo;
+    uint32_t nb = blocks_for(cnt, kDiagThreads);
+    if (nb > kNoAllocMomentBlocks) nb = kNoAllocMomentBlocks;
+    const uint32_t nbp = with_potential ? blocks_for(cnt, kDiagThreads) : 0u;
+    if (nbp > kNoAllocPotentialBlocks)
+        return diag_fail(c, GE_E_CAPACITY, "with_potential: the system has more than 4194304 particles "
+                                           "(the fixed scratch of ge_diagnostics_noalloc)");
+    (void)hipSetDevice(c->device);
+    double *sums = part + 7ull * kNoAllocMomentBlocks + kNoAllocPotentialBlocks;
+    const State &s = c->b.st[c->b.cur];
+    hipLaunchKernelGGL(k_moments, dim3(nb), dim3(kDiagThreads), 0, c->stream, s, lo, hi, part);
+    if (with_potential) {
+        hipLaunchKernelGGL(k_pack_pxym, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, s, c->n,
+                           c->b.pxym);
+        const double eps2 = (double)p->softening * (double)p->softening;
+        hipLaunchKernelGGL(k_potential, dim3(nbp), dim3(kDiagThreads), 0, c->stream, c->b.pxym, lo, hi, eps2,
+                           part + 7ull * nb);
+    }
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, c->stream, part, nb, nbp, sums);
+    double v[8];
+    GE_HIP_CHECK(c, hipMemcpyAsync(v, sums, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    out->mass = v[0];
+    out->px = v[1];
+    out->py = v[2];
+    out->lz = v[3];
+    out->kinetic = v[4];
+    out->comx = v[0] > 0 ? v[5] / v[0] : 0.0;
+    out->comy = v[0] > 0 ? v[6] / v[0] : 0.0;
+    out->potential = with_potential ? -0.5 * (double)p->G * v[7] : 0.0;
     return GE_OK;
 }
 

@@ -272,6 +272,9 @@ void radix_sort_packed(ge_ctx *c, uint64_t *&data, uint64_t *&alt, uint32_t n,
 void radix_sort_kv(hipStream_t stream, uint32_t *hist, uint64_t *&keys, uint64_t *&keys_alt, uint32_t *&vals,
                    uint32_t *&vals_alt, uint32_t n, int lo_bit, int hi_bit, uint32_t *err);
 uint32_t radix_hist_entries(uint32_t cap);
+// diag.hip: looks up the device address of the fixed scratch of
+// ge_diagnostics_noalloc for `device` (current), once per device
+int noalloc_setup(int device);
 // step.hip
 void launch_integrate_start_bbox(ge_ctx *c, float dt, int sph_clamp, float max_vel, int do_integrate,
                                  Domain dom = Domain{});

@@ -30,7 +30,7 @@ EXPORTS = [
     "ge_compute_gravity", "ge_integrate_start", "ge_integrate_end", "ge_sph_new_grid",
     "ge_sph_pcisph", "ge_get_keys", "ge_get_perm", "ge_get_node_count", "ge_get_nodes",
     "ge_get_interactions", "ge_get_neighbors", "ge_profile_enable", "ge_profile_read",
-    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes", "ge_diagnostics", "ge_direct_fp64", "ge_get_interaction_counts", "ge_get_walk_stats", "ge_upload_aos", "ge_download_aos", "ge_neighbor_counts",
+    "ge_upload_systems", "ge_system_offsets", "ge_systems_update", "ge_download_system", "ge_poll", "ge_memory_bytes", "ge_diagnostics", "ge_diagnostics_noalloc", "ge_direct_fp64", "ge_get_interaction_counts", "ge_get_walk_stats", "ge_upload_aos", "ge_download_aos", "ge_neighbor_counts",
 ]
 PHASES = ["integrate_bbox", "keys", "sort", "gather", "tree", "gravity", "sph_grid", "sph",
           "integrate_end", "multipole"]
@@ -172,6 +172,7 @@ def load_library():
         "ge_poll": (C.c_int, [vp, P(C.c_int32)]),
         "ge_memory_bytes": (C.c_int, [vp, P(C.c_uint64)]),
         "ge_diagnostics": (C.c_int, [vp, C.c_uint32, P(StepParams), C.c_int32, P(GeDiag)]),
+        "ge_diagnostics_noalloc": (C.c_int, [vp, C.c_uint32, P(StepParams), C.c_int32, P(GeDiag)]),
         "ge_direct_fp64": (C.c_int, [vp, C.c_uint32, P(C.c_uint32), P(StepParams), P(C.c_double),
                                      P(C.c_double)]),
     }
@@ -320,10 +321,13 @@ class Context:
         self._check(self.lib.ge_poll(self.h, C.byref(d)))
         return bool(d.value)
 
-    def diagnostics(self, sys=0, params=None, potential=False):
+    def diagnostics(self, sys=0, params=None, potential=False, noalloc=False):
+        """ge_diagnostics, or ge_diagnostics_noalloc (the entry point that
+        allocates nothing) with noalloc=True."""
         d = GeDiag()
-        self._check(self.lib.ge_diagnostics(self.h, int(sys), C.byref(params) if params is not None else None,
-                                            1 if potential else 0, C.byref(d)))
+        fn = self.lib.ge_diagnostics_noalloc if noalloc else self.lib.ge_diagnostics
+        self._check(fn(self.h, int(sys), C.byref(params) if params is not None else None,
+                       1 if potential else 0, C.byref(d)))
         return {k: getattr(d, k) for k, _ in d._fields_}
 
     def direct_fp64(self, targets, params):

@@ -276,6 +276,18 @@ typedef struct {
  * O(N^2) FP64 direct sum with the walk's Plummer softening */
 int ge_diagnostics(ge_ctx *ctx, uint32_t sys, const ge_step_params *p, int32_t with_potential,
                    ge_diag *out);
+/* ge_diagnostics for callers that cannot allocate memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc); it has
+ * no counterpart in the reference.  Same arguments and the same bits as
+ * ge_diagnostics, but it calls no hipMalloc, hipFree or allocating library
+ * routine: its partial sums live in a fixed device array of 188 KiB per GPU
+ * (looked up once per device by ge_create, held until the process ends).
+ * Limit: with_potential needs a system of at most 4,194,304 particles
+ * (GE_E_CAPACITY beyond it); the moments have no limit.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (contexts used from
+ * several host threads at once) race; serialise them. */
+int ge_diagnostics_noalloc(ge_ctx *ctx, uint32_t sys, const ge_step_params *p, int32_t with_potential,
+                           ge_diag *out);
 /* FP64 direct-sum accelerations of the particles in slots targets[k] from
  * every other particle of their system: G m_j d / (|d|^2 + eps^2)^1.5, the
  * force the tree walk approximates (the reference to measure its error) */
