@@ -104,11 +104,14 @@ __device__ __forceinline__ float coh_kernel(float r, float h) {
     return (1.0f - q) * (0.5f - q) * (0.5f - q) * 30.0f / (kPiF * h * h);
 }
 
-// SPH::spikyKernelDerivative (SPH.h:45-50): float scale times ::pow(double)
+// SPH::spikyKernelDerivative (SPH.h:45-50): float scale times pow(h - r, 2.0f),
+// the float overload of pow in the reference's build; (h - r)^2 is formed as
+// one correctly rounded float product
 __device__ __forceinline__ float spiky_grad(float r, float h, float scale) {
+#pragma clang fp contract(off)
     if (r >= h) return 0.0f;
-    const double d = (double)(h - r);
-    return (float)((double)scale * (d * d));
+    const float d = h - r;
+    return scale * (d * d);
 }
 
 // min/max of (x, y) over the included particles, per workgroup
@@ -510,10 +513,11 @@ SphConst make_const(const ge_step_params *p, float half_dt) {
     K.delta = p->delta;
     K.stiff_mult = p->stiff_mult;
     K.coeff = 0.5f * p->dt * p->dt;  // predictionCoeff (SPH.cpp:156)
-    // SPH.h:38-56: ::pow in double with the float PI, rounded to float
-    K.vol = (float)(((double)kPiF * std::pow((double)K.h, 4.0)) / 6.0);
-    K.spiky = (float)(-45.0 / ((double)kPiF * std::pow((double)K.h, 6.0)));
-    K.lap = (float)(45.0 / ((double)kPiF * std::pow((double)K.h, 6.0)));
+    // SPH.h:38-56 in float: pow(float, float) is std::pow's float overload in
+    // the reference's build, and PI is raylib's float constant
+    K.vol = (kPiF * std::pow(K.h, 4.0f)) / 6.0f;
+    K.spiky = -45.0f / (kPiF * std::pow(K.h, 6.0f));
+    K.lap = 45.0f / (kPiF * std::pow(K.h, 6.0f));
     K.half_dt = half_dt;
     K.S = make_search(p->sph_search, p->sph_cell, p->sph_search_radius);
     return K;

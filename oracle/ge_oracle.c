@@ -567,18 +567,21 @@ int64_t geo_sph_candidates_mode(int32_t n, const float *x, const float *y,
 
 #define PI_F 3.14159265358979323846f /* raylib PI (a float) */
 
-/* SPH.h:38-56: constants computed through ::pow(double) with float PI, then
- * rounded to float at the assignment. */
+/* SPH.h:38-56 in float arithmetic.  The calls are pow(float, float) with the
+ * float raylib PI: the reference's precompiled header includes raymath.h, whose
+ * <math.h> is the C++ wrapper that brings std::pow's float overload into the
+ * global namespace, so pow(h, 4.0f) is powf(h, 4.0f) and every product and
+ * quotient below is a float operation. */
 void geo_sph_constants(float h, float out[3]) {
-    out[0] = (float)(((double)PI_F * pow((double)h, 4.0)) / 6.0);  /* W volume */
-    out[1] = (float)(-45.0 / ((double)PI_F * pow((double)h, 6.0))); /* spiky scale */
-    out[2] = (float)(45.0 / ((double)PI_F * pow((double)h, 6.0)));  /* laplacian */
+    out[0] = (PI_F * powf(h, 4.0f)) / 6.0f;  /* smoothingKernel volume */
+    out[1] = -45.0f / (PI_F * powf(h, 6.0f)); /* spikyKernelDerivative scale */
+    out[2] = 45.0f / (PI_F * powf(h, 6.0f));  /* smoothingKernelLaplacian */
 }
 
-/* SPH::spikyKernelDerivative, SPH.h:45-50: float scale times double pow */
+/* SPH::spikyKernelDerivative, SPH.h:45-50: float scale times powf(h - r, 2) */
 static inline float spiky_grad(float r, float h, float scale) {
     if (r >= h) return 0.0f;
-    return (float)((double)scale * pow((double)(h - r), 2.0));
+    return scale * powf(h - r, 2.0f);
 }
 
 /* SPH::smoothingKernelCohesion, SPH.h:58-64 */

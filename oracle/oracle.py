@@ -147,6 +147,8 @@ def load():
     lib.geo_step.argtypes = [C.POINTER(_State), C.POINTER(_Params), C.c_int32]
     lib.geo_sph_pcisph.restype = C.c_int32
     lib.geo_sph_pcisph.argtypes = [C.POINTER(_State), C.POINTER(_Params)]
+    lib.geo_integrate_start.restype = None
+    lib.geo_integrate_start.argtypes = [C.POINTER(_State), C.c_float, C.c_int32, C.c_float]
     lib.geo_node_temps.restype = None
     lib.geo_node_temps.argtypes = [C.c_void_p, C.c_int32, f32p, f32p]
     lib.geo_walk_heat.restype = None
@@ -402,28 +404,51 @@ class State:
             self._s.prev_ke = _f(self.a["prev_ke"])
 
     def step(self, prm, nsteps=1):
-        p = _Params()
-        d = dict(G=1.0, theta=0.8, softening=1.0, dt=0.01, naive=0, direct_below=5000, sph=0,
-                 sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3, cohesion_coef=1.0,
-                 delta=19000.0, stiff_mult=1.0, sph_max_vel=250.0, sph_search=0, sph_search_radius=3.5,
-                 periodic=0, domain_w=3840.0, domain_h=2160.0, sph_ground=0, vertical_gravity=3.0,
-                 boundary_friction=0.0, temperature=0, heat_conductivity=0.045, ambient_temp=274.0,
-                 ambient_heat_rate=1.0)
-        d.update({k: v for k, v in prm.items() if k in d or k == "dt_start"})
-        d.setdefault("dt_start", d["dt"])
-        for k, v in d.items():
-            setattr(p, k, v)
-        mats = prm.get("materials")
-        if mats:
-            arr = (Material * len(mats))(*mats)
-            p.materials = C.addressof(arr)
-            p.nmaterials = len(mats)
+        p, keep = _make_params(prm)
         rc = load().geo_step(C.byref(self._s), C.byref(p), int(nsteps))
+        del keep
         if rc != 0:
             raise RuntimeError(f"oracle step failed {rc}")
 
+    def pcisph(self, prm):
+        """geo_sph_pcisph alone (SPH::PCISPH, SPH.cpp:152-281) on the current
+        state, in the current order: no integration, no sort."""
+        p, keep = _make_params(dict(prm, sph=1))
+        rc = load().geo_sph_pcisph(C.byref(self._s), C.byref(p))
+        del keep
+        if rc != 0:
+            raise RuntimeError(f"oracle pcisph failed {rc}")
+
+    def integrate_start(self, dt, sph, sph_max_vel):
+        """geo_integrate_start alone (Physics::integrateStart,
+        physics.cpp:1069-1112)."""
+        load().geo_integrate_start(C.byref(self._s), float(dt), int(sph), float(sph_max_vel))
+
     def __getitem__(self, k):
         return self.a[k]
+
+
+def _make_params(prm):
+    """geo_params from a dict of step parameters (defaults: the reference's
+    UpdateVariables values); returns (params, objects to keep alive)."""
+    p = _Params()
+    d = dict(G=1.0, theta=0.8, softening=1.0, dt=0.01, naive=0, direct_below=5000, sph=0,
+             sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3, cohesion_coef=1.0,
+             delta=19000.0, stiff_mult=1.0, sph_max_vel=250.0, sph_search=0, sph_search_radius=3.5,
+             periodic=0, domain_w=3840.0, domain_h=2160.0, sph_ground=0, vertical_gravity=3.0,
+             boundary_friction=0.0, temperature=0, heat_conductivity=0.045, ambient_temp=274.0,
+             ambient_heat_rate=1.0)
+    d.update({k: v for k, v in prm.items() if k in d or k == "dt_start"})
+    d.setdefault("dt_start", d["dt"])
+    for k, v in d.items():
+        setattr(p, k, v)
+    arr = None
+    mats = prm.get("materials")
+    if mats:
+        arr = (Material * len(mats))(*mats)
+        p.materials = C.addressof(arr)
+        p.nmaterials = len(mats)
+    return p, arr
 
 
 class _Domain(C.Structure):

@@ -69,13 +69,19 @@ def main():
         x = rng.uniform(-500, 500, n).astype(np.float32)
         y = rng.uniform(-50, 300, n).astype(np.float32)
         boxes.append({"x": x.tolist(), "y": y.tolist(), "bb": bbox(x, y)})
-    # SPH constants: W volume pi h^4 / 6, spiky -45 / (pi h^6), laplacian 45 / (pi h^6)
-    pi_f = float(np.float32(math.pi))
+    # SPH constants (SPH.h:38-56): W volume pi h^4 / 6, spiky -45 / (pi h^6),
+    # laplacian 45 / (pi h^6), each operation in float with the float pi (the
+    # reference's pow(float, float) is the float overload; h^4 and h^6 are
+    # exact floats for these h).  At h = 2.5, 3 and 3.5 the float form differs
+    # in the last bit from the same expression evaluated in double and rounded.
+    F = np.float32
+    pi_f = F(math.pi)
     sph = []
-    for h in (1.0, 3.0, 4.0, 2.5):
-        sph.append({"h": h, "volume": float(np.float32(pi_f * h ** 4 / 6.0)),
-                    "spiky": float(np.float32(-45.0 / (pi_f * h ** 6))),
-                    "lap": float(np.float32(45.0 / (pi_f * h ** 6)))})
+    for h in (1.0, 3.0, 4.0, 2.5, 3.5):
+        h4, h6 = F(h ** 4), F(h ** 6)
+        sph.append({"h": h, "volume": float((pi_f * h4) / F(6.0)),
+                    "spiky": float(F(-45.0) / (pi_f * h6)),
+                    "lap": float(F(45.0) / (pi_f * h6))})
     # Hand-checked tree: bbox (0,0,size 8); 8 particles placed so that the
     # root splits into four quadrants, the lower-left quadrant (size 4, 5
     # particles) splits again and the others are leaves by the count <= 4 &&

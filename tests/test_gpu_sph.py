@@ -137,6 +137,40 @@ def test_pcisph_one_frame_matches_oracle(mixed, dense):
         _close(got[f], ref[f], f)
 
 
+def test_speed_clamp_frame_matches_oracle():
+    """integrateStart's SPH speed clamp (physics.cpp:1086-1100): particles
+    whose half-kicked speed passes sphMaxVel (250) are scaled back to it, and
+    their prevVel too; the drift that follows is bit-exact."""
+    n = 4000
+    st = _mixed_flags(_sph_block(n, 9), 10)
+    rng = np.random.default_rng(11)
+    fast = rng.random(n) < 0.08
+    ang = rng.uniform(0, 2 * np.pi, n)
+    st["vx"][fast] = (400.0 * np.cos(ang[fast])).astype(np.float32)
+    st["vy"][fast] = (400.0 * np.sin(ang[fast])).astype(np.float32)
+    kick = (~fast) & (rng.random(n) < 0.05)  # the half kick alone passes the limit
+    st["ax"] = np.zeros(n, np.float32)
+    st["ax"][kick] = np.float32(3.0e5)
+    st["vx"][kick & (rng.random(n) < 0.5)] = 0.0  # prevVel below 1e-5: not rescaled
+    st["vy"][kick] = np.where(st["vx"][kick] == 0.0, np.float32(0.001), st["vy"][kick])
+    p = _params()
+    s = O.State(st, sph=True)
+    s.step(_oracle_prm(p), nsteps=1)
+    fields = ("x", "y", "vx", "vy", "pvx", "pvy", "ax", "ay", "pfx", "pfy", "id")
+    ref = _by_id({k: s[k].copy() for k in fields})
+    with Context(n, sph=True) as c:
+        c.upload(st)
+        c.step(p, 1)
+        c.sync()
+        got = _by_id(c.download(list(fields)))
+    clamped = np.hypot(ref["pvx"], ref["pvy"]) < np.hypot(st["vx"], st["vy"])[np.argsort(st["id"])] * 0.99
+    assert clamped.sum() > 0.05 * n  # the clamp fired
+    for f in ("x", "y", "pvx", "pvy"):
+        np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+    for f in ("vx", "vy", "pfx", "pfy"):
+        _close(got[f], ref[f], f)
+
+
 def test_pcisph_momentum_conserved_gpu():
     n = 5000
     st = _sph_block(n, 5)

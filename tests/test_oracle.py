@@ -18,9 +18,21 @@ by something independent of its own code:
   geo_integrate_* -> kick-drift-kick on a harmonic force: closed-form one-step
       update
   geo_sph_candidates -> O(N^2) brute force over cell coordinates
-  geo_sph_constants  -> closed forms in kat.json; W integrates to 1
-  geo_sph_pcisph  -> symmetry: pair forces cancel (momentum conserved) for
-      equal masses; density of an isolated particle equals its self term.
+  geo_sph_constants  -> the float-pow values in kat.json (which differ in the
+      last bit from the double form at h = 2.5, 3 and 3.5); W integrates to 1
+  geo_sph_pcisph  -> two-particle closed forms, one per pair term, written from
+      the physics of SPH.cpp:98-281 in FP64: viscosity (visc and density of
+      the other particle, 0.001 floor, both visits), cohesion (own cohesion,
+      both visits), self-inclusive predicted density over the averaged rest
+      density, the doubled pressure pair force with the min(1, 8 / massRatio)
+      clamp, and a pinned particle's one-sided visit; plus symmetry (momentum
+      conserved) and the isolated particle's self term
+  computeInternalMass order -> a known answer where only the fold q = 0, 2,
+      1, 3 gives the root mass 2^24
+  MAC softening placement   -> a target whose acceptance of one node flips
+      with the placement of eps^2 (checked through its acceleration)
+  SPH speed clamp (integrateStart) -> closed form of the clamp of vel and
+      prevVel.
 """
 import json
 import math
@@ -290,7 +302,11 @@ def test_sph_candidates_brute_force():
 def test_sph_constants_closed_form():
     for c in KAT["sph_constants"]:
         k = O.sph_constants(c["h"])
-        np.testing.assert_allclose(k, [c["volume"], c["spiky"], c["lap"]], rtol=2e-7)
+        # exact: the float evaluation order of SPH.h:38-56
+        np.testing.assert_array_equal(k, np.array([c["volume"], c["spiky"], c["lap"]], np.float32))
+        h = c["h"]
+        np.testing.assert_allclose(k, [math.pi * h ** 4 / 6, -45 / (math.pi * h ** 6), 45 / (math.pi * h ** 6)],
+                                   rtol=3e-7)
     # W = (h - r)^2 / volume integrates to 1 over the disk of radius h
     h = 3.0
     vol = O.sph_constants(h)[0]
@@ -466,3 +482,235 @@ def test_periodic_positions_wrap_and_ground_walls():
     assert gvy[0] == np.float32(np.float32(0.5) + np.float32(3.0) * np.float32(0.005))
     assert gay[0] == np.float32(3.0) and gay[1] == np.float32(3.0)
     assert gy[1] == np.float32(50.0) and gvy[1] == np.float32(np.float32(3.0) * np.float32(0.005))
+
+
+# ---- two-particle closed forms of the 2D PCISPH pair terms -----------------
+# SPH.cpp:98-281 with i = particle 0, j = particle 1, written in FP64 from the
+# physics, independent of the oracle's code:
+#   pass 1 (viscosity + cohesion, at pos): each particle visits the other, so
+#     F_i = mu lap (v_j - v_i) (visc_j m_j / max(dens_j, 0.001)
+#                               + visc_i m_i / max(dens_i, 0.001))
+#         + c W_coh(r) n_ij (coh_i m_j + coh_j m_i),   F_j = -F_i
+#     (a pinned or drawn particle does not visit, so only the other's term)
+#   predPos = pos + F / sphMass * dt^2 / 2
+#   predDens_i = m_i W(0) / rho0_i + m_j W(rr) / ((rho0_i + rho0_j) / 2)
+#   press_i = max(0, delta (predDens_i - rho0_i)) stiff_i stiffMultiplier
+#   pass 2 (pressure): both visits add the same pair force, so
+#     F_i += 2 mag gradW(rr) (pp_i - pp_j) / rr,
+#     mag = -(m_i + m_j) (P_i + P_j) / 2 / max((d_i + d_j) / 2, 0.01)
+#           * min(1, 8 / massRatio)
+#   pressF = F / sphMass; acc += pressF unless pinned
+# with m = sphMass * myVar.mass, W(r) = (h - r)^2 / (pi h^4 / 6),
+# gradW(r) = -45 / (pi h^6) (h - r)^2, lap = 45 / (pi h^6),
+# W_coh(r) = (1 - q)(1/2 - q)^2 30 / (pi h^2), q = r / h.
+
+_PAIR_PRM = dict(sph_h=3.0, sph_cell=3.0, sph_unit_mass=0.03, viscosity=0.3, cohesion_coef=1.0,
+                 delta=19000.0, stiff_mult=1.0, dt=0.002)
+
+
+def _pair(**kw):
+    st = {"x": [0.4, 2.1], "y": [0.5, 1.1], "vx": [0.0, 0.0], "vy": [0.0, 0.0], "mass": [1.0, 1.0],
+          "flags": [O.GEO_SPH, O.GEO_SPH], "sph_mass": [1.0, 1.0], "rest_dens": [0.008, 0.008],
+          "stiff": [1.0, 1.0], "visc": [1.0, 1.0], "cohesion": [0.0, 0.0], "dens": [0.0, 0.0]}
+    st.update(kw)
+    out = {k: (np.asarray(v, np.uint32) if k == "flags" else np.asarray(v, np.float32)) for k, v in st.items()}
+    return out
+
+
+def _pair_closed_form(st, prm):
+    f = {k: np.asarray(v, np.float64) for k, v in st.items() if k != "flags"}
+    h, unit, mu, c = prm["sph_h"], prm["sph_unit_mass"], prm["viscosity"], prm["cohesion_coef"]
+    vol, lap, spiky = math.pi * h ** 4 / 6, 45 / (math.pi * h ** 6), -45 / (math.pi * h ** 6)
+    m = f["sph_mass"] * unit
+    pos = np.stack([f["x"], f["y"]], 1)
+    vel = np.stack([f["vx"], f["vy"]], 1)
+    visits = [(int(st["flags"][k]) & O.GEO_PINNED) == 0 for k in (0, 1)]
+    d = pos[1] - pos[0]
+    r = math.hypot(*d)
+    n = d / r
+    q = r / h
+    wc = (1 - q) * (0.5 - q) ** 2 * 30 / (math.pi * h * h)
+    dens = np.maximum(f["dens"], 0.001)
+    F = np.zeros((2, 2))
+    if visits[0]:  # i visits j (SPH.cpp:110-147)
+        t = mu * f["visc"][1] * m[1] / dens[1] * lap * (vel[1] - vel[0]) + c * f["cohesion"][0] * m[1] * wc * n
+        F[0] += t
+        F[1] -= t
+    if visits[1]:  # j visits i
+        t = mu * f["visc"][0] * m[0] / dens[0] * lap * (vel[0] - vel[1]) + c * f["cohesion"][1] * m[0] * wc * (-n)
+        F[1] += t
+        F[0] -= t
+    pp = pos + F / f["sph_mass"][:, None] * (0.5 * prm["dt"] ** 2)
+    rr = math.hypot(*(pp[0] - pp[1]))
+    rho = f["rest_dens"]
+    w0, wr = h * h / vol, (h - rr) ** 2 / vol
+    pd = np.array([m[0] * w0 / rho[0] + m[1] * wr / (0.5 * (rho[0] + rho[1])),
+                   m[1] * w0 / rho[1] + m[0] * wr / (0.5 * (rho[0] + rho[1]))])
+    press = np.maximum(0.0, prm["delta"] * (pd - rho)) * f["stiff"] * prm["stiff_mult"]
+    ratio = f["sph_mass"].max() / f["sph_mass"].min()
+    mag = -(m[0] + m[1]) * 0.5 * (press[0] + press[1]) / max(0.5 * (pd[0] + pd[1]), 0.01) * min(1.0, 8.0 / ratio)
+    pf = mag * spiky * (h - rr) ** 2 * (pp[0] - pp[1]) / rr
+    F[0] += 2 * pf
+    F[1] -= 2 * pf
+    pressF = F / f["sph_mass"][:, None]
+    return {"pred_dens": pd, "press": press, "pressF": pressF, "ppos": pp}
+
+
+def _check_pair(st, prm, rtol=2e-5):
+    want = _pair_closed_form(st, prm)
+    s = O.State(st, sph=True)
+    s.pcisph(prm)
+    np.testing.assert_allclose(s["pred_dens"], want["pred_dens"], rtol=rtol)
+    np.testing.assert_allclose(s["press"], want["press"], rtol=rtol, atol=1e-9)
+    got = np.stack([s["pfx"], s["pfy"]], 1).astype(np.float64)
+    scale = np.abs(want["pressF"]).max()
+    assert scale > 0
+    np.testing.assert_allclose(got, want["pressF"], rtol=0, atol=rtol * scale)
+    np.testing.assert_allclose(np.stack([s["ppx"], s["ppy"]], 1), want["ppos"], rtol=0, atol=1e-5)
+    pinned = (np.asarray(st["flags"]) & O.GEO_PINNED) != 0
+    np.testing.assert_array_equal(s["ax"][pinned], 0)  # acc untouched for pinned particles
+    np.testing.assert_allclose(s["ax"][~pinned], got[~pinned, 0], rtol=1e-6)
+    return want
+
+
+def test_pcisph_pair_pressure_doubled_and_mass_ratio_clamped():
+    # sphMass ratio 20 > 8: the pair force is scaled by 8 / 20; each pair is
+    # visited from both sides, so it enters twice; low rest densities make
+    # both pressures positive and unequal (stiff 1 and 2)
+    st = _pair(sph_mass=[1.0, 20.0], rest_dens=[0.002, 0.003], stiff=[1.0, 2.0])
+    want = _check_pair(st, _PAIR_PRM)
+    assert want["press"].min() > 0 and want["press"][0] != want["press"][1]
+
+
+def test_pcisph_pair_viscosity_uses_other_particles_visc_and_density():
+    prm = dict(_PAIR_PRM, delta=0.0)  # no pressure
+    vel = dict(vx=[1.0, -2.0], vy=[-0.5, 0.7])
+    # dens = 0 on the live path: the 0.001 floor
+    _check_pair(_pair(visc=[0.5, 2.0], sph_mass=[1.0, 3.0], **vel), prm)
+    # a written density of the other particle replaces the floor (dens_j = 5)
+    _check_pair(_pair(visc=[0.5, 2.0], sph_mass=[1.0, 3.0], dens=[0.0, 5.0], **vel), prm)
+    # a pinned particle does not visit, but is visited (one-sided term)
+    _check_pair(_pair(visc=[0.5, 2.0], sph_mass=[1.0, 3.0], flags=[O.GEO_SPH | O.GEO_PINNED, O.GEO_SPH], **vel), prm)
+
+
+def test_pcisph_pair_cohesion_uses_own_cohesion():
+    prm = dict(_PAIR_PRM, delta=0.0)
+    _check_pair(_pair(cohesion=[0.3, 1.7], sph_mass=[1.0, 4.0]), prm)
+
+
+def test_pcisph_pair_all_terms_together():
+    st = _pair(vx=[1.0, -2.0], vy=[-0.5, 0.7], visc=[0.5, 2.0], cohesion=[0.3, 1.7], sph_mass=[1.0, 12.0],
+               rest_dens=[0.002, 0.004], stiff=[1.5, 0.5])
+    _check_pair(st, _PAIR_PRM)
+
+
+def test_pair_closed_form_separates_the_plausible_slips():
+    """The closed form itself tells the slips apart (so a restatement with one
+    of them fails the tests above): pressure not doubled, clamp dropped, visc_i
+    for visc_j, coh_j for coh_i."""
+    st = _pair(vx=[1.0, -2.0], vy=[-0.5, 0.7], visc=[0.5, 2.0], cohesion=[0.3, 1.7], sph_mass=[1.0, 20.0],
+               rest_dens=[0.002, 0.003], stiff=[1.0, 2.0])
+    base = _pair_closed_form(st, _PAIR_PRM)["pressF"]
+    swapped = dict(st, visc=st["visc"][::-1].copy(), cohesion=st["cohesion"][::-1].copy())
+    alt = _pair_closed_form(swapped, _PAIR_PRM)["pressF"]
+    assert np.abs(alt - base).max() > 1e-3 * np.abs(base).max()
+
+
+# ---- quadtree mass fold order (quadtree.h:70-94) ---------------------------
+
+def test_tree_mass_fold_order_known_answer():
+    """computeInternalMass adds the children in loop order subGrids[0][0],
+    [0][1], [1][0], [1][1], i.e. q = 0, 2, 1, 3 (q & 1 -> +x, q & 2 -> +y).
+    One particle per quadrant, 2^24 in quadrant 2 and 1 in the others: in
+    float, 1 + 2^24 rounds (to even) to 2^24, which absorbs the two later
+    units, so the root mass is exactly 2^24.  The fold q = 0, 1, 2, 3 gives
+    2 + 2^24 = 2^24 + 2, then 2^24 + 4."""
+    x = np.array([1.0, 7.0, 1.0, 7.0], np.float32)  # Morton order: q = 0, 1, 2, 3
+    y = np.array([1.0, 1.0, 7.0, 7.0], np.float32)
+    m = np.array([1.0, 1.0, 2.0 ** 24, 1.0], np.float32)
+    bb = O.bbox(x, y)
+    np.testing.assert_array_equal(bb, [1.0, 1.0, 6.0])
+    nodes = O.build_tree(x, y, m, bb)
+    assert len(nodes) == 5 and nodes[0]["next"] == 4
+    assert nodes[0]["mass"] == np.float32(2.0 ** 24)
+    assert list(nodes[0]["child"]) == [1, 2, 3, 4]
+    np.testing.assert_array_equal(nodes["mass"][1:], m)
+
+
+# ---- MAC softening placement (physics.cpp:239-252) -------------------------
+
+def _mac_scene():
+    # A = (0, 0) and B = (8, 8) set the root square [0, 8]^2.  Quadrant 3
+    # holds the pair P1 = (4.6, 5), P2 = (5.4, 5) in the leaf [4, 6]^2 (size
+    # 2, COM (5, 5)) and B in the leaf [6, 8]^2; quadrant 1 holds the targets
+    # T = (5, 1.3) and T2 = (5, 1.7) in the leaf [4, 6] x [0, 2].
+    pts = {"A": (0.0, 0.0), "B": (8.0, 8.0), "P1": (4.6, 5.0), "P2": (5.4, 5.0), "T": (5.0, 1.3), "T2": (5.0, 1.7)}
+    names = list(pts)
+    x = np.array([pts[k][0] for k in names], np.float32)
+    y = np.array([pts[k][1] for k in names], np.float32)
+    bb = O.bbox(x, y)
+    perm = O.sort_perm(O.morton_keys(x, y, bb))
+    return [names[i] for i in perm], x[perm], y[perm], bb
+
+
+def _soft(src, tgt, msrc, eps2):
+    d = np.asarray(src, np.float64) - np.asarray(tgt, np.float64)
+    return msrc * d / (d @ d + eps2) ** 1.5
+
+
+def test_mac_softening_placement_flips_acceptance():
+    """With theta = 0.5 and eps = 2 the size-2 pair leaf (s^2 = 4) is accepted
+    when 4 < 0.25 (d^2 + 4), i.e. d^2 > 12.  From T, d^2 = 13.69: accepted
+    (dropping eps^2 or using eps unsquared would open it).  From T2,
+    d^2 = 10.89: opened (adding eps^2 outside theta^2 would accept it).
+    Every other node's decision is the same for all these forms."""
+    names, x, y, bb = _mac_scene()
+    m = np.ones(len(x), np.float32)
+    nodes = O.build_tree(x, y, m, bb)
+    ax, ay, _ = O.bh_accel(x, y, m, nodes, 1.0, 0.5, 2.0)
+    P = {k: (float(x[i]), float(y[i])) for i, k in enumerate(names)}
+    eps2 = 4.0
+    com = ((P["P1"][0] + P["P2"][0]) / 2, (P["P1"][1] + P["P2"][1]) / 2)
+    want_T = (_soft(P["A"], P["T"], 1, eps2) + _soft(P["T2"], P["T"], 1, eps2) + _soft(com, P["T"], 2, eps2)
+              + _soft(P["B"], P["T"], 1, eps2))
+    want_T2 = (_soft(P["A"], P["T2"], 1, eps2) + _soft(P["T"], P["T2"], 1, eps2) + _soft(P["P1"], P["T2"], 1, eps2)
+               + _soft(P["P2"], P["T2"], 1, eps2) + _soft(P["B"], P["T2"], 1, eps2))
+    # the flipped decisions, for contrast
+    open_T = want_T - _soft(com, P["T"], 2, eps2) + _soft(P["P1"], P["T"], 1, eps2) + _soft(P["P2"], P["T"], 1, eps2)
+    acc_T2 = (want_T2 - _soft(P["P1"], P["T2"], 1, eps2) - _soft(P["P2"], P["T2"], 1, eps2)
+              + _soft(com, P["T2"], 2, eps2))
+    for k, want, flipped in (("T", want_T, open_T), ("T2", want_T2, acc_T2)):
+        i = names.index(k)
+        got = np.array([ax[i], ay[i]], np.float64)
+        tol = 1e-5 * np.linalg.norm(want)
+        assert np.linalg.norm(got - want) < tol, (k, got, want)
+        assert np.linalg.norm(flipped - want) > 100 * tol, k
+
+
+# ---- SPH speed clamp in integrateStart (physics.cpp:1086-1100) -------------
+
+def test_sph_speed_clamp_closed_form():
+    """With SPH on, a half-kicked speed above sphMaxVel is scaled back to it,
+    and prevVel too (unless its square is at most 1e-5); pinned particles are
+    skipped; with SPH off nothing is clamped."""
+    vmax, dt = 250.0, 0.01
+    st = {"x": np.zeros(4), "y": np.zeros(4), "vx": [300.0, 100.0, 0.0, 300.0], "vy": [400.0, 0.0, 0.001, 400.0],
+          "ax": [0.0, 0.0, 60000.0, 0.0], "ay": [0.0, 0.0, 0.0, 0.0], "mass": np.ones(4),
+          "flags": [O.GEO_SPH, O.GEO_SPH, O.GEO_SPH, O.GEO_SPH | O.GEO_PINNED]}
+    s = O.State(st, sph=True)
+    s.integrate_start(dt, 1, vmax)
+    # particle 0: |v| = 500 -> (150, 200); prevVel (300, 400) -> (150, 200)
+    np.testing.assert_allclose([s["vx"][0], s["vy"][0]], [150.0, 200.0], rtol=1e-6)
+    np.testing.assert_allclose([s["pvx"][0], s["pvy"][0]], [150.0, 200.0], rtol=1e-6)
+    np.testing.assert_allclose([s["x"][0], s["y"][0]], [1.5, 2.0], rtol=1e-6)
+    # particle 1 below the limit: unchanged
+    assert (s["vx"][1], s["pvx"][1]) == (100.0, 100.0)
+    # particle 2: the half kick takes it to (300, 0.001) > 250; prevVel^2 = 1e-6 stays
+    np.testing.assert_allclose([s["vx"][2], s["vy"][2]], [250.0, 0.001 * 250.0 / math.hypot(300.0, 0.001)],
+                               rtol=1e-6)
+    assert s["pvy"][2] == np.float32(0.001) and s["pvx"][2] == 0.0
+    # particle 3 pinned: untouched
+    assert (s["vx"][3], s["vy"][3], s["x"][3]) == (300.0, 400.0, 0.0)
+    s2 = O.State(st, sph=True)
+    s2.integrate_start(dt, 0, vmax)
+    assert (s2["vx"][0], s2["vy"][0]) == (300.0, 400.0)
