@@ -38,16 +38,18 @@ for _p in (os.path.join(ROOT, "galaxy-engine_amd"), os.path.join(ROOT, "oracle")
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
-# Algorithmic FP32 flops per interaction of the walk (Kahan and opening tests
-# excluded; a fused multiply-add counts 2): a leaf partner or a monopole node
-# is dx, dy, r^2 + eps^2, rsqrt, r^-3, G m r^-3 and two accumulates = 14; a
-# quadrupole node 32 and a node to the hexadecapole 107, counted from the
-# expression the walk evaluates (moments pre-scaled, Horner form in r^2).
-FLOPS_PAIR = 14
-FLOPS_PAIR3 = 20  # the 3D interaction: dx, dy, dz, r^2 + eps, rsqrt, r^-3, G m r^-3, three accumulates
-FLOPS_NODE = {0: 14, 1: 32, 2: 107}
-PEAK_FP32_TFLOPS = 157.3            # MI355X vector FP32
-PEAK_HBM_GBS = 8000.0
+# Algorithmic counts, committed once (bench/roofline_constants.json): 14 FP32
+# flops per gravity interaction (accepted node or leaf partner), SPH flops per
+# pair per pass, compulsory HBM bytes per particle and per node of each phase,
+# and the MI355X peaks.  The expression-level flops of the accuracy modes'
+# node terms (32, 107) are reported beside the 14-flop figure, labelled.
+with open(os.path.join(ROOT, "bench", "roofline_constants.json")) as _f:
+    CONST = json.load(_f)
+FLOPS_PAIR = CONST["gravity"]["flops_per_interaction"]
+FLOPS_PAIR3 = CONST["gravity3d"]["flops_per_interaction"]
+FLOPS_NODE = {int(k): v for k, v in CONST["gravity"]["expression_flops_per_node"].items()}
+PEAK_FP32_TFLOPS = CONST["peaks"]["fp32_flops_per_s"] / 1e12
+PEAK_HBM_GBS = CONST["peaks"]["hbm_bytes_per_s"] / 1e9
 
 
 def parse():
@@ -74,7 +76,7 @@ def parse():
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=120)
-    ap.add_argument("--serve-rates", type=str, default="50,200,800")
+    ap.add_argument("--serve-rates", type=str, default="10,50,200,800")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain elementwise copy of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -114,23 +116,90 @@ def traffic_from_profiles(accuracy, block):
     return t.get("bytes_per_launch_by_block", {}).get(str(block or 256))
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads():
+    """The host cores this process may use: OMP_NUM_THREADS when set (the
+    GPU host's share), else the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(st, prm, budget_s):
+    """The oracle's OpenMP restatement of the reference step on this host, at
+    the host's thread count and at half of it (the reference's default,
+    main.cpp:20-22), each for about budget_s / 2 seconds of frames."""
     import oracle as O
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    s = O.State(st)
-    p = dict(prm, dt_start=prm["dt"])
-    t0 = time.perf_counter()
-    steps = 0
-    while True:
-        s.step(p, 1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 50:
-            break
+    full = _host_threads()
     n = len(st["x"])
-    return {"value": n * steps / el, "unit": "particle-updates/s", "cores": cores, "kind": "port",
+    p = dict(prm, dt_start=prm["dt"])
+    runs = {}
+    for threads in (full, max(1, full // 2)):
+        O.set_threads(threads)
+        s = O.State(st)
+        t0 = time.perf_counter()
+        steps = 0
+        while True:
+            s.step(p, 1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s / 2 or steps >= 30:
+                break
+        runs[threads] = (n * steps / el, steps, el)
+    O.set_threads(full)
+    v, steps, el = runs[full]
+    hv, hsteps, hel = runs[max(1, full // 2)]
+    return {"value": v, "unit": "particle-updates/s", "cores": full, "kind": "port",
             "sample": f"{steps} full frames of the same {n}-particle workload (theta={prm['theta']}, "
-                      f"reference walk) through the OpenMP C restatement of the reference step, {el:.1f} s"}
+                      f"reference walk) through the OpenMP C restatement of the reference step, {el:.1f} s",
+            "half_threads": {"value": hv, "cores": max(1, full // 2), "sample": f"{hsteps} frames, {hel:.1f} s",
+                             "note": "the reference's default thread count, half the host's"},
+            "cpu_model": _cpu_model(), "compiler": O.build_info()}
+
+
+def step_roofline(phase_ms, ms_per_step, n, m_nodes, interactions, accuracy, sph_flops=None):
+    """T* = sum over phases of max(B / HBM peak, F / FP32 peak) from the
+    committed constants; achieved = T* / T_measured."""
+    bpp, bpn = CONST["bytes_per_particle"], CONST["bytes_per_node"]
+    bw, fl = CONST["peaks"]["hbm_bytes_per_s"], CONST["peaks"]["fp32_flops_per_s"]
+    rows = {}
+    tstar = 0.0
+    for ph, ms in phase_ms.items():
+        if ph not in bpp and ph != "gravity" and ph != "sph":
+            continue
+        if ph == "gravity":
+            b = bpp["walk"] * n + bpn["walk"] * m_nodes + (bpn["multipole"] * m_nodes if accuracy == 2 else 0)
+            f = FLOPS_PAIR * interactions
+        elif ph == "sph":
+            b = bpp["sph_passes"] * n
+            f = sph_flops or 0
+        else:
+            b = bpp[ph] * n + bpn.get(ph, 0) * m_nodes
+            f = 0
+        if ph == "sph_grid":
+            b *= 2  # two grids per frame
+        t = max(b / bw, f / fl) * 1e3
+        rows[ph] = {"bytes": int(b), "flops": int(f), "t_star_ms": round(t, 5), "measured_ms": ms,
+                    "frac": round(t / ms, 4) if ms > 0 else None}
+        tstar += t
+    return {"t_star_ms": round(tstar, 5), "t_measured_ms": ms_per_step, "achieved": tstar / ms_per_step,
+            "phases": rows,
+            "formula": "T* = sum_p max(B_p / 8.0e12 B/s, F_p / 157.3e12 flop/s), constants in "
+                       "bench/roofline_constants.json"}
 
 
 def force_errors(c, p, n, samples=4096, seed=0):
@@ -198,10 +267,12 @@ def single_system(a, st, prm, accuracy, tune):
     out["phases"] = phases
     grav_ms, grav_launches = phases["gravity"]
     out["walk_ms"] = grav_ms / max(grav_launches, 1)
-    flops = FLOPS_NODE[accuracy] * nodes_acc + FLOPS_PAIR * pairs
-    out["walk_tflops"] = flops / (out["walk_ms"] * 1e-3) / 1e12
-    out["interactions"] = {"nodes": nodes_acc, "pairs": pairs, "flops": flops,
-                           "per_particle": (nodes_acc + pairs) / n}
+    alg = FLOPS_PAIR * (nodes_acc + pairs)
+    expr = FLOPS_NODE[accuracy] * nodes_acc + FLOPS_PAIR * pairs
+    out["walk_tflops"] = alg / (out["walk_ms"] * 1e-3) / 1e12
+    out["walk_tflops_expression"] = expr / (out["walk_ms"] * 1e-3) / 1e12
+    out["interactions"] = {"nodes": nodes_acc, "pairs": pairs, "per_particle": (nodes_acc + pairs) / n,
+                           "flops": alg, "expression_flops": expr}
     out["params"] = {"accuracy": accuracy, "leaf_capacity": p.leaf_capacity, "walk_block": p.walk_block}
     return out
 
@@ -230,10 +301,45 @@ def sph_impact(a):
         t1 = time.perf_counter()
         ph = c.profile_read()
         c.profile_enable(False)
+        d = c.download(["x", "y", "ppx", "ppy", "flags"])
+        nodes_acc, pairs = c.interaction_counts()
+        m_nodes = c.node_count()
     el = t1 - t0
+    pc = sph_pair_counts(d, prm["sph_h"])
+    fpp = CONST["sph_flops_per_pair"]
+    sph_flops = (fpp["viscosity_cohesion"] * pc["pos"] + fpp["density"] * (pc["pred"] + pc["visitors"])
+                 + fpp["pressure"] * pc["pred"])
+    phase_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items() if v[1]}
+    sph_ms = phase_ms.get("sph", 0.0)
+    step = step_roofline(phase_ms, el * 1e3 / steps, n, m_nodes, nodes_acc + pairs, 0, sph_flops=sph_flops)
     return {"workload": f"gravity + SPH impact, {n} particles (200k + 50k), h = cell = 4, theta 0.7, dt 0.002",
             "particle_updates_per_s": n * steps / el, "ms_per_step": el * 1e3 / steps, "steps": steps,
-            "phase_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in ph.items() if v[1]}}
+            "phase_ms": phase_ms,
+            "sph_pairs": pc,
+            "sph_passes": {"flops": int(sph_flops), "ms": sph_ms,
+                           "tflops": sph_flops / (sph_ms * 1e-3) / 1e12 if sph_ms else None,
+                           "frac_fp32": sph_flops / (sph_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS if sph_ms else None,
+                           "gbps_compulsory": CONST["bytes_per_particle"]["sph_passes"] * n / (sph_ms * 1e-3) / 1e9
+                           if sph_ms else None,
+                           "note": "algorithmic flops: 44 per ordered pair within h at pos (viscosity and "
+                                   "cohesion), 16 per pair within h at predPos including self (density), 32 per "
+                                   "ordered pair within h at predPos (pressure), pairs counted on the last "
+                                   "frame's state"},
+            "step_roofline": step}
+
+
+def sph_pair_counts(d, h):
+    """Ordered pairs (i, j), j != i, of SPH visitors (SPH, not being drawn)
+    closer than h at pos and at predPos, counted with a k-d tree."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    vis = ((d["flags"] & 2) != 0) & ((d["flags"] & 4) == 0)
+    out = {"visitors": int(vis.sum())}
+    for key, (fx, fy) in (("pos", ("x", "y")), ("pred", ("ppx", "ppy"))):
+        pts = np.stack([d[fx][vis], d[fy][vis]], 1).astype(np.float64)
+        # query_pairs keeps r <= h; the passes keep r < h (equal distances are measure-zero here)
+        out[key] = 2 * len(cKDTree(pts).query_pairs(float(h), output_type="ndarray"))
+    return out
 
 
 def three_d(a, cpu_seconds):
@@ -364,7 +470,14 @@ def batched(a, accuracy):
                        accuracy=accuracy, direct_below=0, leaf_capacity=16, use_graph=1)
     steps = a.batch_steps
     log(f"batched: {len(sizes)} systems, {total} particles, {steps} steps")
+    from galaxy_step import serve as S
     with Context(total, max_systems=len(sizes)) as c:
+        c.upload(batch, counts=sizes)
+        # leaf capacity in {8, 16, 32} and walk block in {64, 128, 256} picked on the ensemble itself
+        leaf, block, tr = S.autotune(c, p, frames=3)
+        p.leaf_capacity, p.walk_block = leaf, block
+        tune = {"leaf_capacity": leaf, "walk_block": block, "ms_per_frame": tr.ms_per_frame,
+                "table_ms": [[round(v, 4) for v in row] for row in tr.table_ms]}
         c.upload(batch, counts=sizes)
         c.step(p, 4)  # warm the code paths and the graph
         c.sync()
@@ -423,7 +536,9 @@ def batched(a, accuracy):
                                              "max_system_particles": int(sizes[worst])},
             "force_error_rms_normalised": {"median_system": float(np.median(norm_sys)),
                                            "max_system": float(norm_sys.max())},
-            "params": {"accuracy": accuracy, "leaf_capacity": 16, "graphs": True}}
+            "params": {"accuracy": accuracy, "leaf_capacity": p.leaf_capacity, "walk_block": p.walk_block,
+                       "graphs": True},
+            "autotune": tune}
 
 
 def _lattice(side, seed):
@@ -711,6 +826,9 @@ def main():
             log(f"couplings leg failed: {cpl['error']}")
     phase_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in head["phases"].items() if v[1]}
     achieved = head["walk_tflops"]
+    inter = head["interactions"]
+    step = step_roofline(phase_ms, head["ms_per_step"], n, head["node_count"], inter["nodes"] + inter["pairs"],
+                         head["params"]["accuracy"])
     out = {
         "metric": "particle-updates/s, 1M-body BH step theta=0.7; batched solves/s + p99 latency",
         "value": head["value"],
@@ -732,9 +850,18 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profiles(head["params"]["accuracy"], head["params"]["walk_block"]),
+                     "traffic_source": "profiles/bh_walk_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE per walk "
+                                       "launch at this block size, calibrated on a copy kernel (not measured "
+                                       "in this run)",
                      "kernel": "k_bh (tree walk)", "kernel_ms": head["walk_ms"],
-                     "interactions": head["interactions"],
-                     "flops_per_node": FLOPS_NODE[head["params"]["accuracy"]], "flops_per_pair": FLOPS_PAIR},
+                     "interactions": inter, "flops_per_interaction": FLOPS_PAIR,
+                     "expression": {"achieved": head["walk_tflops_expression"],
+                                    "frac": head["walk_tflops_expression"] / PEAK_FP32_TFLOPS,
+                                    "flops_per_node": FLOPS_NODE[head["params"]["accuracy"]],
+                                    "flops_per_pair": FLOPS_PAIR,
+                                    "note": "flops of the expression the walk evaluates (a node to the "
+                                            "hexadecapole is 107); not the roofline's algorithmic count"},
+                     "step": step},
         "force_error_vs_fp64": head["errors"],
         "graph_ms_per_step": head["graph_ms_per_step"],
         "host_transfer": {"upload_ms": head["upload_ms"], "download_ms": head["download_ms"],
@@ -744,7 +871,7 @@ def main():
         "phase_ms": phase_ms,
         "plain_form": {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
                        "force_error_vs_fp64": plain["errors"], "walk_ms": plain["walk_ms"],
-                       "walk_tflops": plain["walk_tflops"],
+                       "walk_tflops": plain["walk_tflops"], "walk_frac": plain["walk_tflops"] / PEAK_FP32_TFLOPS,
                        "note": "reference monopole walk, opening test and leaf rule (the parity path)"},
         "cpu_baseline": cpu,
         "sph_impact": sph,

@@ -82,6 +82,7 @@ void free_all(ge_ctx *c) {
     dfree(b.pxym); dfree(b.sort_a); dfree(b.sort_b); dfree(b.keys); dfree(b.perm); dfree(b.hist);
     dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.interactions);
     dfree(b.sys_off); dfree(b.sys_of); dfree(b.sys_bb); dfree(b.chunk_lo); dfree(b.chunk_first); dfree(b.sys_src);
+    dfree(b.sys_wave_off);
     b.bb = nullptr;
     TopNodes &T = b.top;
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
@@ -124,6 +125,7 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.chunk_lo, (size_t)b.max_chunks + 1));
     GE_HIP_CHECK(c, dalloc(b.chunk_first, (size_t)c->max_sys + 1));
     GE_HIP_CHECK(c, dalloc(b.sys_src, (size_t)c->max_sys + 1));
+    GE_HIP_CHECK(c, dalloc(b.sys_wave_off, (size_t)c->max_sys + 1));
     b.bb = b.sys_bb;  // system 0's box doubles as the single-scene box
     GE_HIP_CHECK(c, dalloc(b.interactions, 4));  // interactions, accepted nodes, wave visits, leaf trips
     TopNodes &T = b.top;
@@ -232,6 +234,10 @@ int set_systems(ge_ctx *c, uint32_t nsys, const uint32_t *counts) {
     }
     chunk_first[nsys] = (uint32_t)chunk_lo.size();
     if (chunk_lo.size() > c->b.max_chunks) return GE_E_CAPACITY;
+    // the walk's waves start at each system's first slot, so a wave holds the
+    // same particles whether its system runs alone or in a batch
+    std::vector<uint32_t> wave_off(nsys + 1, 0);
+    for (uint32_t s = 0; s < nsys; ++s) wave_off[s + 1] = wave_off[s] + (counts[s] + 63u) / 64u;
     GE_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     c->idx_bits = idx_bits;
     c->sys_bits = sys_bits;
@@ -243,6 +249,8 @@ int set_systems(ge_ctx *c, uint32_t nsys, const uint32_t *counts) {
     GE_HIP_CHECK(c, hipMemcpy(c->b.sys_off, off.data(), sizeof(uint32_t) * (nsys + 1), hipMemcpyHostToDevice));
     GE_HIP_CHECK(c, hipMemcpy(c->b.chunk_lo, chunk_lo.data(), sizeof(uint32_t) * chunk_lo.size(), hipMemcpyHostToDevice));
     GE_HIP_CHECK(c, hipMemcpy(c->b.chunk_first, chunk_first.data(), sizeof(uint32_t) * (nsys + 1), hipMemcpyHostToDevice));
+    GE_HIP_CHECK(c, hipMemcpy(c->b.sys_wave_off, wave_off.data(), sizeof(uint32_t) * (nsys + 1), hipMemcpyHostToDevice));
+    c->nwaves = wave_off[nsys];
     if (n > 0) launch_sys_of(c);
     return GE_OK;
 }
@@ -344,7 +352,7 @@ int check_params(ge_ctx *c, const ge_step_params *p) {
     if (p->sph && !c->sph) return fail(c, GE_E_INVALID, "context created without enable_sph");
     if (p->sph && c->nsys > 1) return fail(c, GE_E_UNSUPPORTED, "SPH runs on single-system contexts");
     if (p->sph && !(p->sph_cell > 0.0f && p->sph_h > 0.0f)) return fail(c, GE_E_INVALID, "sph_h and sph_cell must be > 0");
-    if (p->accuracy < 0 || p->accuracy > 2) return fail(c, GE_E_INVALID, "accuracy must be 0, 1 or 2");
+    if (p->accuracy < 0 || p->accuracy > 3) return fail(c, GE_E_INVALID, "accuracy must be 0, 1, 2 or 3");
     if (p->accuracy && !(p->theta > 0.0f)) return fail(c, GE_E_INVALID, "accuracy mode needs theta > 0");
     if (p->leaf_capacity > 64) return fail(c, GE_E_INVALID, "leaf_capacity must be 0 (reference rule) or 1..64");
     if (p->sph_search != 0 && p->sph_search != 1) return fail(c, GE_E_INVALID, "sph_search must be 0 or 1");

@@ -157,6 +157,7 @@ struct Buffers {
     uint32_t *chunk_lo = nullptr;  // reduction chunks: first slot of each chunk (+ end)
     uint32_t *chunk_first = nullptr;  // first chunk of each system (nsys + 1)
     uint32_t *sys_src = nullptr;      // compaction: old first slot of each kept system
+    uint32_t *sys_wave_off = nullptr; // nsys + 1 prefix of walk waves per system (64 particles each)
     uint32_t max_chunks = 0;
     TopNodes top;
     uint32_t *start_count = nullptr;  // per particle: nodes whose range starts there (then its prefix)
@@ -216,6 +217,7 @@ struct ge_ctx {
     uint32_t max_sys = 1;
     uint32_t nsys = 1;
     uint32_t nchunks = 1;
+    uint32_t nwaves = 0;  // walk waves: each system's particles in 64-particle waves from its first slot
     int idx_bits = 28;             // low bits of a packed sort word: slot within the system
     int sys_bits = 0;              // high bits: system index
     std::vector<uint32_t> host_off;  // system offsets (host copy)

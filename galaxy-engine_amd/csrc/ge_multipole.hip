@@ -28,7 +28,7 @@
 // central moments), T_l = A3_iil, U_lm = A4_iilm and W = A4_iijj, the record
 // per node is 6 float4, each moment scaled by the constant of the derivative
 // term it feeds:
-//   0: (comx, comy, opening radius, mass)
+//   0: (comx, comy, opening radius squared, mass)
 //   1: (-15 A2xx, -15 A2xy, -15 A2yy, 3 tr A2)
 //   2: -105 A3: (xxx, xxy, xyy, yyy)
 //   3: (45 T_x, 45 T_y, -45 W, 0)
@@ -134,7 +134,7 @@ __device__ void finish_record(uint32_t k, const NodeRec &r, const float4 g, cons
     // derivative term, so the walk evaluates the expansion as two short
     // polynomials in r^2 = 1 / (|d|^2 + eps^2) with no per-node scaling
     float4 *o = mp + 6ull * k;
-    o[0] = make_float4(comx, comy, lim, (float)M);
+    o[0] = make_float4(comx, comy, lim * lim, (float)M);
     o[1] = make_float4((float)(-7.5 * c2xx), (float)(-7.5 * c2xy), (float)(-7.5 * c2yy), (float)(1.5 * (c2xx + c2yy)));
     o[2] = make_float4((float)(-17.5 * c3xxx), (float)(-17.5 * c3xxy), (float)(-17.5 * c3xyy), (float)(-17.5 * c3yyy));
     o[3] = make_float4((float)(7.5 * (c3xxx + c3xyy)), (float)(7.5 * (c3xxy + c3yyy)), (float)(-1.875 * (uxx + uyy)),

@@ -239,15 +239,17 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // remaining constants folded into three multipliers.
 //
 // Several systems: system s owns the node range [P[off_s], P[off_s+1]) (P is
-// the node-start prefix of the tree build); each lane is active only inside
-// its own system's range, and the wave walks from its lanes' first root to
-// their last end.
+// the node-start prefix of the tree build).  Waves are laid out per system:
+// wave w of system s holds its particles 64 (w - W_s) .. 64 (w - W_s) + 63
+// (W the per-system wave prefix), so a wave holds the same particles whether
+// the system runs alone or in a batch, and rules that look at the whole wave
+// (the leaf rule of the accuracy modes) give every system its alone result.
 template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
 __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                                            const NodeRec *__restrict__ nodes,
                                            const float4 *__restrict__ mp,
                                            const uint32_t *__restrict__ start_prefix,
-                                           const uint32_t *__restrict__ sys_of,
+                                           const uint32_t *__restrict__ sys_wave_off, uint32_t nsys,
                                            const uint32_t *__restrict__ sys_off,
                                            const uint32_t *__restrict__ counters, uint32_t n,
                                            float G, float theta_sq, float soft_sq,
@@ -258,10 +260,19 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                                            const float *__restrict__ gtavg, float *__restrict__ temp,
                                            float heat_k, float heat_dt) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t i = blk * BS + threadIdx.x;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blk * (BS / kWave) + (threadIdx.x / kWave));
+    // the wave's system: the last s with W_s <= w (wave-uniform search)
+    uint32_t slo = 0, shi = nsys;
+    while (shi - slo > 1u) {
+        const uint32_t mid = (slo + shi) >> 1;
+        if (sys_wave_off[mid] <= w) slo = mid;
+        else shi = mid;
+    }
+    const uint32_t sy = slo;
+    const uint32_t i = sys_off[sy] + (w - sys_wave_off[sy]) * kWave + (threadIdx.x & (kWave - 1));
     const bool per = dom.w > 0.0f;
     const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
-    const bool live = i < n;
+    const bool live = w < sys_wave_off[nsys] && i < sys_off[sy + 1];
     const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float ti = (TEMP && live) ? temp[i] : 0.0f;  // temp[i], moved by every accepted node
     // a tree that raised an error flag may hold unwritten nodes: do not walk it
@@ -269,7 +280,6 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
     const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
     uint32_t root = kNone, end = 0u;
     if (live) {
-        const uint32_t sy = sys_of[i];
         root = start_prefix[sys_off[sy]];
         end = min(start_prefix[sys_off[sy + 1]], node_count);
     }
@@ -293,9 +303,11 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
             h0 = mp[6ull * idx];
             h1 = mp[6ull * idx + 1];
         }
-        if (ORDER >= 4) {
+        if (ORDER >= 3) {
             h2 = mp[6ull * idx + 2];
             h3 = mp[6ull * idx + 3];
+        }
+        if (ORDER >= 4) {
             h4 = mp[6ull * idx + 4];
             h5 = mp[6ull * idx + 5];
         }
@@ -319,7 +331,7 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
             }
             const float d2 = dx * dx + dy * dy;
             dist_sq = d2 + soft_sq;
-            accept = d2 > h0.z * h0.z;
+            accept = d2 > h0.z;  // h0.z holds the squared opening radius
         } else {
 #pragma clang fp contract(off)
             dx = nd.comx - p.x;
@@ -331,7 +343,14 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
             dist_sq = dx * dx + dy * dy + soft_sq;
             accept = nd.size_sq < theta_sq * dist_sq;
         }
-        if (active && accept) {
+        const bool leaf = next == 0;
+        // ORDER > 0: a leaf that any active lane opens is summed directly by
+        // every active lane (exact, and the wave runs that loop anyway); those
+        // lanes skip the leaf's expansion.  ORDER 0 keeps the reference's
+        // per-lane choice.
+        bool direct = active && !accept && leaf;
+        if (ORDER > 0 && leaf && __ballot(direct)) direct = active;
+        if (active && accept && !direct) {
             const float inv = __builtin_amdgcn_rsqf(dist_sq);
             float tx, ty;
             if (ORDER > 0) {
@@ -355,6 +374,19 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                     const float e = (28.0f / 9.0f) * r2, f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
                     const float ox = (2.0f * p2x + h3.x) + f * ((3.0f * p3x + 2.0f * pux) + e * p4x);
                     const float oy = (2.0f * p2y + h3.y) + f * ((3.0f * p3y + 2.0f * puy) + e * p4y);
+                    tx = qx * sc + g5 * ox;
+                    ty = qy * sc + g5 * oy;
+                } else if (ORDER == 3) {
+                    // the same series without the fourth-order moments (A4, U, W)
+                    const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
+                    const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
+                    const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
+                    const float s3 = qx * p3x + qy * p3y;
+                    const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y);
+                    const float sc = gr3 * (r2 * (r2 * (r2 * s3 + c7) + h1.w) - h0.w);
+                    const float f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
+                    const float ox = (2.0f * p2x + h3.x) + f * (3.0f * p3x);
+                    const float oy = (2.0f * p2y + h3.y) + f * (3.0f * p3y);
                     tx = qx * sc + g5 * ox;
                     ty = qy * sc + g5 * oy;
                 } else {
@@ -398,9 +430,39 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
             }
             skip_until = after;
         }
-    const bool leaf = next == 0;
-        const bool open_leaf = active && !accept && leaf;
-        if (__ballot(open_leaf)) {
+        const bool open_leaf = direct;
+        if (ORDER > 0 && __ballot(open_leaf)) {
+            // leaf direct sum: the pairs go to a plain per-leaf sum of
+            // m_j d_j r^-3, scaled by G and compensated once per leaf
+            const uint32_t jend = min(nd.end, n);
+            if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
+            float sx = 0.f, sy = 0.f;
+            for (uint32_t j = nd.start; j < jend; ++j) {
+                const float4 q = pxym[j];
+                if (open_leaf && j != i) {
+                    float px = q.x - p.x, py = q.y - p.y;
+                    if (per) {
+                        px = wrap_d(px, dom.w, hw);
+                        py = wrap_d(py, dom.h, hh);
+                    }
+                    const float d2 = px * px + py * py + soft_sq;
+                    const float inv = __builtin_amdgcn_rsqf(d2);
+                    const float f = q.z * (inv * inv * inv);
+                    sx = fmaf(px, f, sx);
+                    sy = fmaf(py, f, sy);
+                    if (COUNT) ++cnt;
+                }
+            }
+            if (open_leaf) {
+                if (KAHAN) {
+                    kahan_add(fx, cx, G * sx);
+                    kahan_add(fy, cy, G * sy);
+                } else {
+                    fx = fmaf(G, sx, fx);
+                    fy = fmaf(G, sy, fy);
+                }
+            }
+        } else if (ORDER == 0 && __ballot(open_leaf)) {
             // leaf direct sum over [start, end), skipping i == j (physics.cpp:163-196)
             const uint32_t jend = min(nd.end, n);
             if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
@@ -628,10 +690,11 @@ void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const State &s = c->b.st[c->b.cur];
     const float theta_sq = p->theta * p->theta;
     const float soft_sq = p->softening * p->softening;
-    const uint32_t nb = blocks_for(c->n, BS);
+    const uint32_t nb = blocks_for((uint64_t)c->nwaves * kWave, BS);
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
     hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER, TEMP>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
-                       c->b.nodes, c->b.node_mp, c->b.start_count, c->b.sys_of, c->b.sys_off, c->b.counters, c->n,
+                       c->b.nodes, c->b.node_mp, c->b.start_count, c->b.sys_wave_off, c->nsys, c->b.sys_off,
+                       c->b.counters, c->n,
                        p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p),
                        inter, c->b.node_tavg, s.temp, c->thermal_p.heat_conductivity, p->dt);
 }
@@ -648,6 +711,7 @@ void launch_bh_o(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, boo
 template <int BS>
 void launch_bh_bs(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool count) {
     if (p->accuracy == 2) launch_bh_o<BS, 4, false>(c, p, end_kick_half_dt, count);
+    else if (p->accuracy == 3) launch_bh_o<BS, 3, false>(c, p, end_kick_half_dt, count);
     else if (p->accuracy == 1) launch_bh_o<BS, 2, false>(c, p, end_kick_half_dt, count);
     else if (p->temperature) launch_bh_o<BS, 0, true>(c, p, end_kick_half_dt, count);  // the reference walk only
     else launch_bh_o<BS, 0, false>(c, p, end_kick_half_dt, count);

@@ -107,9 +107,11 @@ typedef struct {
     int32_t compensated;  /* Kahan-compensated force accumulation (1 = on) */
     int32_t accuracy;     /* 0: the reference's monopole walk and opening test;
                              1: quadrupole, 2: moments up to the hexadecapole,
-                             both about the FP64 centre of mass with the
-                             offset-corrected opening test
-                             |d| > size/theta + |COM - centre|
+                             3: up to the octupole; all about the FP64 centre
+                             of mass with the offset-corrected opening test
+                             |d| > max(size/theta + |COM - centre|, b_max/theta),
+                             and a leaf that any lane of a 64-particle wave
+                             opens is summed directly by the whole wave
                              (no counterpart in the reference) */
     uint32_t leaf_capacity; /* 0: the reference's leaf rule (quadtree.cpp:16-17);
                                1..64: a node of at most this many particles is a

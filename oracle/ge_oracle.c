@@ -565,6 +565,19 @@ int64_t geo_sph_candidates_mode(int32_t n, const float *x, const float *y,
     return total <= cap ? total : -1;
 }
 
+#ifndef GEO_CFLAGS
+#define GEO_CFLAGS "unknown flags"
+#endif
+const char *geo_build_info(void) {
+#if defined(__clang__)
+    return "clang " __clang_version__ " " GEO_CFLAGS;
+#elif defined(__GNUC__)
+    return "gcc " __VERSION__ " " GEO_CFLAGS;
+#else
+    return "unknown compiler " GEO_CFLAGS;
+#endif
+}
+
 #define PI_F 3.14159265358979323846f /* raylib PI (a float) */
 
 /* SPH.h:38-56 in float arithmetic.  The calls are pow(float, float) with the

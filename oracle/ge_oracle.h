@@ -184,6 +184,8 @@ void geo_neighbor_amount(int32_t n, const float *x, const float *y, const uint32
 int32_t geo_sph_pcisph(geo_state *s, const geo_params *p);
 /* SPH kernel constants computed as the reference does (double pow, float PI) */
 void geo_sph_constants(float h, float out[3]); /* {W volume, spiky scale, laplacian scale} */
+/* compiler and flags this library was built with (reported with CPU timings) */
+const char *geo_build_info(void);
 /* one full frame: GE/src/globalLogic.cpp:1155-1372 for the 2D path.
  * Permutes every state array into Morton order.  Returns 0 or negative. */
 int32_t geo_step(geo_state *s, const geo_params *p, int32_t nsteps);

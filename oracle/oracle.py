@@ -143,6 +143,8 @@ def load():
     lib.geo_sph_candidates.argtypes = [C.c_int32, f32p, f32p, C.c_int32, f32p, f32p, C.c_float,
                                        i64p, u32p, C.c_int64]
     lib.geo_sph_constants.argtypes = [C.c_float, f32p]
+    lib.geo_build_info.restype = C.c_char_p
+    lib.geo_build_info.argtypes = []
     lib.geo_step.restype = C.c_int32
     lib.geo_step.argtypes = [C.POINTER(_State), C.POINTER(_Params), C.c_int32]
     lib.geo_sph_pcisph.restype = C.c_int32
@@ -359,6 +361,18 @@ def temperature(st, materials, ambient=274.0, rate=1.0, dt=0.045):
     load().geo_temperature(C.byref(s._s), C.addressof(arr) if materials else None, len(materials), ambient, rate,
                            dt)
     return s.a
+
+
+def build_info():
+    """Compiler and flags of libge_oracle.so."""
+    return load().geo_build_info().decode()
+
+
+def set_threads(k):
+    """OpenMP threads of the oracle's parallel loops (omp_set_num_threads in
+    the libgomp the library loaded)."""
+    load()
+    C.CDLL("libgomp.so.1").omp_set_num_threads(int(k))
 
 
 def sph_constants(h):
