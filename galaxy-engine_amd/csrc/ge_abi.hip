@@ -83,6 +83,7 @@ void free_all(ge_ctx *c) {
     dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.interactions);
     dfree(b.sys_off); dfree(b.sys_of); dfree(b.sys_bb); dfree(b.chunk_lo); dfree(b.chunk_first); dfree(b.sys_src);
     dfree(b.sys_wave_off);
+    dfree(b.ccreate);
     b.bb = nullptr;
     TopNodes &T = b.top;
     dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
@@ -165,6 +166,8 @@ int alloc_all(ge_ctx *c) {
         GE_HIP_CHECK(c, dalloc(b.cblock, (size_t)mc / 256 + 2));
         GE_HIP_CHECK(c, dalloc(b.ccount, 4));
         GE_HIP_CHECK(c, hipMemset(b.ccount, 0, 4 * sizeof(uint32_t)));
+        if (c->sph)  // createConstraints queries the SPH grid
+            GE_HIP_CHECK(c, dalloc(b.ccreate, (size_t)cap + (cap + 1023) / 1024 + 4));
         b.cframe = b.ccount + 1;
         b.cleft = b.ccount + 2;
         GE_HIP_CHECK(c, dalloc(b.corder, mc));
@@ -405,6 +408,13 @@ bool use_direct(const ge_ctx *c, const ge_step_params *p) {
     return p->naive || (int64_t)c->n < (int64_t)p->direct_below;
 }
 
+// createConstraints runs each frame of a context that keeps temperature
+// fields (hasSolidified is set by temperatureCalculation) and constraints;
+// it reads the created count back, so such frames are not captured in graphs.
+bool creates_constraints(const ge_ctx *c, const ge_step_params *p) {
+    return p->constraints && c->thermal && c->cons_cap > 0 && c->b.ccreate;
+}
+
 // One frame of updateScene() (globalLogic.cpp:1155-1372).
 int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) {
     c->leaf_cap = p->leaf_capacity;
@@ -423,8 +433,22 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
             c->have_tree = true;
         }
     }
+    bool grid_done = false;
+    if (creates_constraints(c, p)) {
+        // createConstraints (globalLogic.cpp:1299-1300) queries the grid the
+        // frame built on the sorted positions, before the forces
+        PhaseScope s(c, GE_PHASE_SPH_GRID);
+        if (dt_start == 0.0f) launch_integrate_start_bbox(c, 0.0f, 0, 0.0f, 0);  // the grid frame's extremes
+        if (int r = sph_new_grid(c, p)) return r;
+        grid_done = true;
+        uint32_t created = 0;
+        if (int r = sph_create_constraints(c, p, &created)) return r;
+        if (created) {
+            if (int r = cons_prepare(c)) return r;
+        }
+    }
     if (p->dt == 0.0f) return GE_OK;
-    if (p->sph) {
+    if (p->sph && !grid_done) {
         PhaseScope s(c, GE_PHASE_SPH_GRID);
         if (int r = sph_new_grid(c, p)) return r;
     }
@@ -773,7 +797,7 @@ int ge_step(ge_ctx *c, const ge_step_params *p, int32_t nsteps) {
         if (int r = cons_prepare(c)) return r;
     }
     int32_t k = 0;
-    if (p->use_graph && !c->prof && nsteps >= 3) {
+    if (p->use_graph && !c->prof && nsteps >= 3 && !creates_constraints(c, p)) {
         // the first frame integrates with dt_start, so it runs eagerly
         if (int r = frame(c, p, p->dt_start, false)) return r;
         k = 1;

@@ -577,6 +577,30 @@ int ge_constraints_solve(ge_ctx *c, const ge_step_params *p) {
     return GE_OK;
 }
 
+int ge_create_constraints(ge_ctx *c, const ge_step_params *p, uint32_t *created) {
+    if (!c || !p) return GE_E_INVALID;
+    if (created) *created = 0;
+    if (!c->b.ccreate) return failc(c, GE_E_INVALID, "needs a context with enable_sph and max_constraints > 0");
+    if (c->nsys > 1) return failc(c, GE_E_UNSUPPORTED, "constraints run on single-system contexts");
+    if (!(p->sph_cell > 0.0f && p->sph_search_radius > 0.0f && p->sph_search_radius <= 8.0f * p->sph_cell))
+        return failc(c, GE_E_INVALID, "sph_cell and sph_search_radius must be > 0 (radius at most 8 cells)");
+    if (c->n == 0) return GE_OK;
+    (void)hipSetDevice(c->device);
+    launch_integrate_start_bbox(c, 0.0f, 0, 0.0f, 0);  // the grid frame: the current extremes
+    if (int r = sph_new_grid(c, p)) return r;
+    uint32_t h = 0;
+    if (int r = sph_create_constraints(c, p, &h)) return r;
+    if (created) *created = h;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return failc(c, GE_E_HIP, "constraint creation failed");
+    uint32_t e = 0;
+    if (hipMemcpy(&e, c->b.counters + kCntError, sizeof e, hipMemcpyDeviceToHost) == hipSuccess && (e & kErrConsCap)) {
+        uint32_t z = e & ~kErrConsCap;
+        (void)hipMemcpy(c->b.counters + kCntError, &z, sizeof z, hipMemcpyHostToDevice);
+        return failc(c, GE_E_CAPACITY, "created constraints beyond max_constraints (the excess was dropped)");
+    }
+    return GE_OK;
+}
+
 int ge_constraints_order(ge_ctx *c, uint32_t *order, uint32_t cap, uint32_t *color_start, uint32_t color_cap,
                          uint32_t *ncolors) {
     if (!c || !ncolors || (cap && !order) || (color_cap && !color_start)) return GE_E_INVALID;

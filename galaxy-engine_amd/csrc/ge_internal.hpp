@@ -42,15 +42,17 @@ enum : uint32_t {
     kErrFrontier = 16u,  // workgroup subtree frontier or level bound exceeded
     kErrMultipole = 32u, // multipole work lists overflowed
     kErrSort = 64u,      // a radix-sort look-back wait passed its spin limit
+    kErrConsCap = 128u,  // created constraints beyond max_constraints (the excess is dropped)
 };
 
 // Text for a set of device error flags ("node capacity tree depth ..."), used
 // by the 2D and 3D contexts' reports.
 inline void describe_device_errors(uint32_t e, char *buf, size_t len) {
-    std::snprintf(buf, len, "device error flags 0x%x (%s%s%s%s%s%s%s)", e, (e & kErrNodeCap) ? "node capacity " : "",
+    std::snprintf(buf, len, "device error flags 0x%x (%s%s%s%s%s%s%s%s)", e, (e & kErrNodeCap) ? "node capacity " : "",
                   (e & kErrDepth) ? "tree depth " : "", (e & kErrGridRange) ? "SPH grid range " : "",
                   (e & kErrNeighborCap) ? "neighbour cap " : "", (e & kErrFrontier) ? "tree frontier " : "",
-                  (e & kErrMultipole) ? "multipole lists " : "", (e & kErrSort) ? "sort wait" : "");
+                  (e & kErrMultipole) ? "multipole lists " : "", (e & kErrSort) ? "sort wait " : "",
+                  (e & kErrConsCap) ? "constraint capacity" : "");
 }
 
 // counters block (device, uint32 each)
@@ -158,6 +160,7 @@ struct Buffers {
     uint32_t *chunk_first = nullptr;  // first chunk of each system (nsys + 1)
     uint32_t *sys_src = nullptr;      // compaction: old first slot of each kept system
     uint32_t *sys_wave_off = nullptr; // nsys + 1 prefix of walk waves per system (64 particles each)
+    uint32_t *ccreate = nullptr;      // constraint creation: per-slot counts, block sums, totals
     uint32_t max_chunks = 0;
     TopNodes top;
     uint32_t *start_count = nullptr;  // per particle: nodes whose range starts there (then its prefix)
@@ -301,6 +304,7 @@ int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt);
 int sph_neighbors(ge_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t cap, int64_t *total);
 int sph_neighbor_counts(ge_ctx *c, const ge_step_params *p, uint32_t *counts);
 int sph_count_into(ge_ctx *c, int search, float cell, float radius, int32_t *out);
+int sph_create_constraints(ge_ctx *c, const ge_step_params *p, uint32_t *created);
 // thermal.hip
 void launch_thermal_fill(ge_ctx *c, const State &s, uint32_t at, uint32_t cnt);
 void launch_node_temps(ge_ctx *c);

@@ -492,6 +492,159 @@ __global__ __launch_bounds__(kBlock) void k_cand_fill(const float *__restrict__ 
     }
 }
 
+// ---- Physics::createConstraints' automatic branch (physics.cpp:654-766):
+// particles with hasSolidified link to neighbour candidates of their position
+// ----------------------------------------------------------------------------
+
+constexpr int kCcBlock = 1024;
+
+__device__ __forceinline__ const ge_material *cc_material(const ge_material *mats, uint32_t nmat, uint32_t id) {
+    for (uint32_t k = 0; k < nmat; ++k)
+        if (mats[k].id == id) return &mats[k];
+    return nullptr;
+}
+
+// Calls f(j, dist_sq) for every candidate j of slot i that the automatic
+// branch admits, in the reference's visiting order: cells dx outer, dy inner
+// (neighborSearch.h:730-731, 559-560), ascending slot inside a cell (the grid
+// orders a cell by slot; NeighborSearchV2's counting sort does the same).
+template <class F>
+__device__ __forceinline__ void cc_visit(uint32_t i, const State &s, const GridView &A, const Search &S,
+                                         const ge_material *mats, uint32_t nmat, F &&f) {
+#pragma clang fp contract(off)
+    const float xi = s.x[i], yi = s.y[i];
+    const uint32_t idi = s.id[i];
+    const ge_material *mi = s.label ? cc_material(mats, nmat, s.label[i]) : nullptr;
+    const int qx = cell_coord(xi, S), qy = cell_coord(yi, S);
+    const int32_t *fr = A.frame;
+    if (!fr[4]) return;
+    for (int dx = -S.R; dx <= S.R; ++dx) {
+        const int64_t col = (int64_t)qx + dx - fr[0];
+        if (col < 0 || col >= fr[2]) continue;
+        for (int dy = -S.R; dy <= S.R; ++dy) {
+            const int64_t row = (int64_t)qy + dy - fr[1];
+            if (row < 0 || row >= fr[3]) continue;
+            const uint64_t cidx = (uint64_t)row * (uint64_t)fr[2] + (uint64_t)col;
+            const uint32_t lo = A.first[cidx], hi = A.first[cidx + 1];
+            for (uint32_t kk = lo; kk < hi; ++kk) {
+                const float4 g = A.pk0[kk];
+                if (!within(S, g.x - xi, g.y - yi)) continue;  // NeighborSearchV2's radius filter
+                const uint32_t j = A.ord[kk];
+                if (j == i) continue;
+                const float ex = g.x - xi, ey = g.y - yi;
+                const float dist_sq = ex * ex + ey * ey;  // glm::dot (physics.cpp:706)
+                if (dist_sq > 12.0f) continue;
+                const ge_material *mj = s.label ? cc_material(mats, nmat, s.label[j]) : nullptr;
+                if (mi && mj && mi->cold_point == 0.0f && mj->cold_point != 0.0f) continue;
+                if (!(idi < s.id[j])) continue;
+                f(j, dist_sq, mi, mj);
+            }
+        }
+    }
+}
+
+// per slot: the number of constraints it creates; hasSolidified is consumed
+__global__ __launch_bounds__(kCcBlock) void k_cc_count(State s, uint32_t n, GridView A, Search S,
+                                                       const ge_material *__restrict__ mats, uint32_t nmat,
+                                                       uint32_t *__restrict__ cnt, uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t red[kCcBlock / 64];
+    const uint32_t i = blockIdx.x * kCcBlock + threadIdx.x;
+    uint32_t k = 0;
+    if (i < n && (s.flags[i] & GE_FLAG_SOLIDIFIED)) {
+        cc_visit(i, s, A, S, mats, nmat, [&](uint32_t, float, const ge_material *, const ge_material *) { ++k; });
+        s.flags[i] &= ~GE_FLAG_SOLIDIFIED;  // pi.hasSolidified = false (physics.cpp:693)
+    }
+    if (i < n) cnt[i] = k;
+    uint32_t w = k;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int q = 0; q < kCcBlock / 64; ++q) t += red[q];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// one workgroup: exclusive scan of the nb block sums in place; the created
+// total and the first free constraint index go to tot[0], tot[1], and the
+// count grows by what fits (an overflow raises kErrConsCap)
+__global__ __launch_bounds__(1024) void k_cc_scan(uint32_t *__restrict__ bsum, uint32_t nb,
+                                                  uint32_t *__restrict__ ccount, uint32_t cap,
+                                                  uint32_t *__restrict__ tot, uint32_t *__restrict__ err) {
+    __shared__ uint32_t sh[1024];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t t = base + threadIdx.x;
+        const uint32_t v = t < nb ? bsum[t] : 0u;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+            const uint32_t a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (t < nb) bsum[t] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += sh[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t first = ccount[0];
+        const uint64_t want = (uint64_t)first + carry;
+        tot[0] = carry;
+        tot[1] = first;
+        if (want > cap) atomicOr(err, kErrConsCap);
+        ccount[0] = want > cap ? cap : (uint32_t)want;
+    }
+}
+
+// each creating slot writes its constraints at first + its exclusive offset
+__global__ __launch_bounds__(kCcBlock) void k_cc_write(State s, uint32_t n, GridView A, Search S,
+                                                       const ge_material *__restrict__ mats, uint32_t nmat,
+                                                       const uint32_t *__restrict__ cnt,
+                                                       const uint32_t *__restrict__ bsum,
+                                                       const uint32_t *__restrict__ tot, uint32_t cap,
+                                                       ge_constraint *__restrict__ cons) {
+#pragma clang fp contract(off)
+    __shared__ uint32_t sh[kCcBlock];
+    const uint32_t i = blockIdx.x * kCcBlock + threadIdx.x;
+    const uint32_t k = i < n ? cnt[i] : 0u;
+    sh[threadIdx.x] = k;
+    __syncthreads();
+    for (uint32_t o = 1; o < kCcBlock; o <<= 1) {
+        const uint32_t a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += a;
+        __syncthreads();
+    }
+    if (k == 0) return;
+    uint64_t at = (uint64_t)tot[1] + bsum[blockIdx.x] + sh[threadIdx.x] - k;
+    const uint32_t idi = s.id[i];
+    cc_visit(i, s, A, S, mats, nmat, [&](uint32_t j, float dist_sq, const ge_material *mi, const ge_material *mj) {
+        if (at < cap) {
+            ge_constraint r;
+            r.id1 = idi;
+            r.id2 = s.id[j];
+            r.rest_length = r.original_length = sqrtf(dist_sq);  // glm::distance
+            const bool both = mi && mj;
+            r.stiffness = both ? mi->constraint_stiffness : 60.0f;
+            r.resistance = both ? (mi->constraint_resistance + mj->constraint_resistance) * 0.5f : 0.6f;
+            r.plasticity_point = both ? (mi->constraint_plastic_point + mj->constraint_plastic_point) * 0.5f : 0.6f;
+            r.displacement = 0.0f;
+            r.is_broken = 0;
+            r.is_plastic = 0;
+            r.pad[0] = r.pad[1] = 0;
+            cons[at] = r;
+        }
+        ++at;
+    });
+}
+
 Search make_search(int mode, float cell, float radius) {
     Search S;
     S.mode = mode;
@@ -667,6 +820,45 @@ int sph_count_into(ge_ctx *c, int search, float cell, float radius, int32_t *out
                        g.frame, g.first, g.pk0, s.flags, GE_FLAG_PINNED | GE_FLAG_DRAWN | GE_FLAG_DARK, 1,
                        reinterpret_cast<uint32_t *>(out));
     c->have_grid = false;
+    return GE_OK;
+}
+
+}  // namespace ge
+
+namespace ge {
+
+// Physics::createConstraints' automatic branch on the current state, over
+// grid A (built by the caller on the current positions).  *created gets the
+// number of constraints appended (a host read of one word).
+int sph_create_constraints(ge_ctx *c, const ge_step_params *p, uint32_t *created) {
+    *created = 0;
+    const uint32_t n = c->n;
+    if (n == 0 || c->cons_cap == 0) return GE_OK;
+    Buffers &b = c->b;
+    const State &s = b.st[b.cur];
+    const Search S = make_search(p->sph_search, p->sph_cell, p->sph_search_radius);
+    const GridView A = view(b.grid_pos);
+    const ge_material *mats = c->thermal ? b.materials : nullptr;
+    const uint32_t nmat = c->thermal ? c->nmat : 0u;
+    const uint32_t nb = blocks_for(n, kCcBlock);
+    uint32_t *cnt = b.ccreate, *bsum = b.ccreate + c->cap, *tot = bsum + blocks_for(c->cap, kCcBlock);
+    hipLaunchKernelGGL(k_cc_count, dim3(nb), dim3(kCcBlock), 0, c->stream, s, n, A, S, mats, nmat, cnt, bsum);
+    hipLaunchKernelGGL(k_cc_scan, dim3(1), dim3(1024), 0, c->stream, bsum, nb, b.ccount, c->cons_cap, tot,
+                       b.counters + kCntError);
+    hipLaunchKernelGGL(k_cc_write, dim3(nb), dim3(kCcBlock), 0, c->stream, s, n, A, S, mats, nmat, cnt, bsum, tot,
+                       c->cons_cap, b.cons);
+    uint32_t h = 0;
+    if (hipMemcpyAsync(&h, tot, sizeof h, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        c->err = "created constraint count copy failed";
+        return GE_E_HIP;
+    }
+    *created = h;
+    if (h) {
+        const uint64_t bound = (uint64_t)c->cons_bound + h;
+        c->cons_bound = bound > c->cons_cap ? c->cons_cap : (uint32_t)bound;
+        c->cons_colored = false;  // new constraints: index map and colours again
+    }
     return GE_OK;
 }
 

@@ -8,7 +8,7 @@ from .lib import GE_OK, GeError, load_library
 
 CONSTRAINT_EXPORTS = [
     "ge_default_constraint_params", "ge_constraints_capacity", "ge_set_constraint_params", "ge_constraints_upload", "ge_constraints_download",
-    "ge_constraints_clear", "ge_constraints_solve", "ge_constraints_order",
+    "ge_constraints_clear", "ge_constraints_solve", "ge_constraints_order", "ge_create_constraints",
 ]
 # ParticleConstraint, the 36-byte record of the save files
 CONSTRAINT_DTYPE = np.dtype([("id1", "<u4"), ("id2", "<u4"), ("rest_length", "<f4"), ("original_length", "<f4"),
@@ -53,6 +53,7 @@ def load_constraint_library():
         "ge_constraints_clear": (C.c_int, [vp]),
         "ge_constraints_solve": (C.c_int, [vp, vp]),
         "ge_constraints_order": (C.c_int, [vp, u32p, C.c_uint32, u32p, C.c_uint32, u32p]),
+        "ge_create_constraints": (C.c_int, [vp, vp, u32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -105,6 +106,14 @@ def clear(ctx):
 
 def solve(ctx, p):
     _check(ctx, load_constraint_library().ge_constraints_solve(ctx.h, C.byref(p)))
+
+
+def create(ctx, params):
+    """ge_create_constraints: the solidified particles' constraints on the
+    current state; returns the number appended."""
+    n = C.c_uint32()
+    _check(ctx, load_constraint_library().ge_create_constraints(ctx.h, C.byref(params), C.byref(n)))
+    return n.value
 
 
 def colors(ctx, count):

@@ -68,6 +68,22 @@ int ge_constraints_clear(ge_ctx *ctx);
 /* Physics::constraints once, on the current state (ge_step with
  * ge_step_params.constraints = 1 runs it after PCISPH, as updateScene does) */
 int ge_constraints_solve(ge_ctx *ctx, const ge_step_params *p);
+/* Physics::createConstraints (GE/src/Physics/physics.cpp:654-766), the
+ * branch that runs without user input: every particle with hasSolidified
+ * (GE_FLAG_SOLIDIFIED, set by temperatureCalculation) consumes the flag and
+ * links to each neighbour candidate j of its position (the grid and search
+ * of p: sph_cell, sph_search, sph_search_radius) with |p_j - p_i|^2 <= 12,
+ * id_i < id_j and not (both materials known, cold point of i zero and of j
+ * non-zero): rest and original length the distance, the stiffness of i's
+ * material and the mean resistance and plastic point of the two (60, 0.6,
+ * 0.6 without materials).  The constraints are appended in slot order of i,
+ * then the reference's cell order (dx outer, dy inner), ascending slot
+ * within a cell.  ge_step with constraints = 1 on a context with
+ * temperature fields runs this every frame after the tree, as updateScene
+ * does (globalLogic.cpp:1299-1300).  Needs enable_sph and max_constraints;
+ * *created gets the number appended.  GE_E_CAPACITY when they did not all
+ * fit (the excess is dropped). */
+int ge_create_constraints(ge_ctx *ctx, const ge_step_params *p, uint32_t *created);
 /* the order the substeps visit constraints (indices into the current
  * constraints, colour by colour) and the colour boundaries; *ncolors gets
  * their number (for checking the solve against a serial loop) */

@@ -920,6 +920,62 @@ static const geo_material *find_material(const geo_material *mats, int32_t nmat,
     return NULL;
 }
 
+/* Physics::createConstraints' automatic branch (physics.cpp:654-766).  The
+ * constraintMap check never fires here: the map is cleared for every
+ * solidified particle (physics.cpp:692) and each candidate appears once in
+ * its query, so no pair is skipped as already present. */
+int32_t geo_create_constraints(geo_state *s, const geo_material *mats, int32_t nmat, int32_t mode, float cell,
+                               float radius, geo_constraint *out, int32_t cap) {
+    const int32_t n = s->n;
+    const search_t S = make_search(mode, cell, radius);
+    int32_t *cx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t *cy = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int32_t j = 0; j < n; ++j) {
+        cx[j] = cell_coord(s->x[j], &S);
+        cy[j] = cell_coord(s->y[j], &S);
+    }
+    int32_t m = 0, over = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!(s->flags[i] & GEO_SOLIDIFIED)) continue;
+        s->flags[i] &= ~GEO_SOLIDIFIED; /* pi.hasSolidified = false (physics.cpp:693) */
+        const geo_material *mi = s->label ? find_material(mats, nmat, s->label[i]) : NULL;
+        const int32_t qx = cell_coord(s->x[i], &S), qy = cell_coord(s->y[i], &S);
+        for (int32_t dx = -S.R; dx <= S.R; ++dx) {
+            for (int32_t dy = -S.R; dy <= S.R; ++dy) {
+                for (int32_t j = 0; j < n; ++j) {
+                    if (cx[j] != qx + dx || cy[j] != qy + dy) continue;
+                    if (S.mode) { /* NeighborSearchV2's radius filter on the candidate */
+                        const float fx = s->x[j] - s->x[i], fy = s->y[j] - s->y[i];
+                        if (!(fx * fx + fy * fy <= S.r2)) continue;
+                    }
+                    if (j == i) continue;
+                    const float ex = s->x[j] - s->x[i], ey = s->y[j] - s->y[i];
+                    const float dist_sq = ex * ex + ey * ey; /* glm::dot */
+                    if (dist_sq > 12.0f) continue;
+                    const geo_material *mj = s->label ? find_material(mats, nmat, s->label[j]) : NULL;
+                    if (mi && mj && mi->cold_point == 0.0f && mj->cold_point != 0.0f) continue;
+                    if (!(s->id[i] < s->id[j])) continue;
+                    geo_constraint k;
+                    memset(&k, 0, sizeof k);
+                    k.id1 = s->id[i];
+                    k.id2 = s->id[j];
+                    k.rest_length = k.original_length = sqrtf(dist_sq); /* glm::distance */
+                    k.stiffness = (mi && mj) ? mi->constraint_stiffness : 60.0f;
+                    k.resistance = (mi && mj) ? (mi->constraint_resistance + mj->constraint_resistance) * 0.5f : 0.6f;
+                    k.plasticity_point =
+                        (mi && mj) ? (mi->constraint_plastic_point + mj->constraint_plastic_point) * 0.5f : 0.6f;
+                    if (m < cap) out[m] = k;
+                    else over = 1;
+                    ++m;
+                }
+            }
+        }
+    }
+    free(cx);
+    free(cy);
+    return over ? -1 : m;
+}
+
 /* Physics::temperatureCalculation (GE/src/Physics/physics.cpp:567-652);
  * UpdateVariables::particleBaseMass = 8.5e9 (GE/src/parameters.cpp:44) */
 void geo_temperature(geo_state *s, const geo_material *mats, int32_t nmat, float ambient, float rate, float dt) {

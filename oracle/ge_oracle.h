@@ -216,6 +216,21 @@ typedef struct {
 int32_t geo_constraints(geo_state *s, geo_constraint *cons, uint32_t *color, int32_t m, const geo_cparams *p,
                         const geo_material *mats, int32_t nmat, uint32_t *frame_count);
 
+/* Physics::createConstraints' automatic branch (GE/src/Physics/physics.cpp:
+ * 654-766 with no key, selection or drawing flag): every particle i with
+ * hasSolidified (GEO_SOLIDIFIED), in index order, clears the flag and, for
+ * each neighbour candidate j != i of its position (the search of `mode`,
+ * `cell`, `radius`, as QueryNeighbors::queryNeighbors returns them) with
+ * |p_j - p_i|^2 <= 12, id_i < id_j and not (both materials known, cold
+ * point of i zero and of j non-zero), appends {id_i, id_j, d, d,
+ * stiffness, resistance, 0, plasticity} (materials of i and j: the
+ * stiffness of i, the means of resistance and plastic point; otherwise 60,
+ * 0.6, 0.6).  Candidates are visited cell by cell, dx outer and dy inner as
+ * the reference loops, ascending index within a cell.  At most cap are
+ * written; returns the number appended, or -1 when cap was too small. */
+int32_t geo_create_constraints(geo_state *s, const geo_material *mats, int32_t nmat, int32_t mode, float cell,
+                               float radius, geo_constraint *out, int32_t cap);
+
 /* ---- merger ---- */
 typedef struct {
     float density_radius;    /* NeighborSearch::densityRadius */

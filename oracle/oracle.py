@@ -159,6 +159,9 @@ def load():
     lib.geo_constraints.restype = C.c_int32
     lib.geo_constraints.argtypes = [C.POINTER(_State), C.c_void_p, C.POINTER(C.c_uint32), C.c_int32,
                                     C.POINTER(_CParams), C.c_void_p, C.c_int32, C.POINTER(C.c_uint32)]
+    lib.geo_create_constraints.restype = C.c_int32
+    lib.geo_create_constraints.argtypes = [C.POINTER(_State), C.c_void_p, C.c_int32, C.c_int32, C.c_float,
+                                           C.c_float, C.c_void_p, C.c_int32]
     lib.geo_merge.restype = C.c_int32
     lib.geo_merge.argtypes = [C.POINTER(_State), f32p, f32p, C.POINTER(C.c_int32), C.c_void_p]
     lib.geo_temperature.restype = None
@@ -313,6 +316,26 @@ def constraints(st, cons, colors, materials=(), frame_count=0, **prm):
     m = load().geo_constraints(C.byref(s._s), cs.ctypes.data, _u(col), len(cs), C.byref(p),
                                C.addressof(arr) if mats else None, len(mats), C.byref(fc))
     return s.a, cs[:m].copy(), col[:m].copy(), fc.value
+
+
+def create_constraints(st, materials=(), mode=0, cell=3.0, radius=3.5):
+    """Physics::createConstraints' automatic branch (solidified particles) on
+    the state dict st (x, y, flags, id, optional label).  Returns (flags
+    after the call, the appended constraints in creation order)."""
+    temperature = st.get("label") is not None
+    s = State(st, sph=False, temperature=temperature)
+    mats = list(materials)
+    arr = (Material * max(len(mats), 1))(*mats)
+    cap = 64
+    while True:
+        out = np.zeros(cap, CONSTRAINT_DTYPE)
+        flags = s.a["flags"].copy()
+        m = load().geo_create_constraints(C.byref(s._s), C.addressof(arr) if mats else None, len(mats), int(mode),
+                                          float(cell), float(radius), out.ctypes.data, cap)
+        if m >= 0:
+            return s.a["flags"].copy(), out[:m].copy()
+        s.a["flags"][:] = flags  # restore and retry with room for every constraint
+        cap *= 4
 
 
 class _MParams(C.Structure):

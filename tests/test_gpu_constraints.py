@@ -11,7 +11,7 @@ import oracle as O
 from galaxy_step import constraints as GC
 from galaxy_step import default_params
 from galaxy_step import thermal as T
-from galaxy_step.lib import GE_FLAG_HOT_POINT, Context
+from galaxy_step.lib import GE_FLAG_HOT_POINT, GE_FLAG_SOLIDIFIED, Context
 
 pytestmark = pytest.mark.gpu
 
@@ -140,3 +140,88 @@ def test_constraint_rejections():
         GC.upload(c, cons[:10])
         GC.clear(c)
         assert len(GC.download(c)) == 0
+
+
+# ---- createConstraints' automatic branch (physics.cpp:654-766) -------------
+
+def _solid_lattice(side, seed, frac=0.35):
+    rng = np.random.default_rng(seed)
+    n = side * side
+    gx, gy = np.meshgrid(np.arange(side), np.arange(side))
+    x = (gx.ravel() * 1.6 + (gy.ravel() % 2) * 0.8 + rng.normal(0, 0.12, n) - 30).astype(F)
+    y = (gy.ravel() * 1.39 + rng.normal(0, 0.12, n) - 20).astype(F)
+    ids = rng.permutation(n).astype(np.uint32) * 2 + 11
+    flags = (np.uint32(2) | np.where(rng.random(n) < frac, np.uint32(GE_FLAG_SOLIDIFIED), np.uint32(0))).astype(np.uint32)
+    st = {"x": x, "y": y, "vx": rng.normal(0, 0.2, n).astype(F), "vy": rng.normal(0, 0.2, n).astype(F),
+          "mass": np.ones(n, F), "id": ids, "flags": flags, "sph_mass": np.ones(n, F),
+          "rest_dens": np.full(n, 0.008, F), "stiff": np.ones(n, F), "visc": np.ones(n, F),
+          "cohesion": np.ones(n, F), "label": rng.integers(0, 8, n).astype(np.uint32),
+          "temp": np.full(n, 250.0, F)}
+    return st
+
+
+CREATE_FIELDS = ("id1", "id2", "rest_length", "original_length", "stiffness", "resistance", "displacement",
+                 "plasticity_point", "is_broken", "is_plastic")
+
+
+@pytest.mark.parametrize("search", [0, 1])
+def test_create_constraints_stage_matches_oracle(search):
+    st = _solid_lattice(56, 5)
+    n = len(st["x"])
+    _, pre = _lattice(6, 9)  # constraints already present: creation appends after them
+    mats = O.reference_materials()
+    p = _params(sph_search=search, sph_cell=3.0, sph_search_radius=3.5)
+    with Context(n, sph=True, temperature=True, max_constraints=len(pre) + 20 * n) as c:
+        c.upload(st)
+        T.thermal_upload(c, st)
+        GC.upload(c, pre)
+        created = GC.create(c, p)
+        got = GC.download(c)
+        flags = c.download(["flags"])["flags"]
+    want_flags, want = O.create_constraints(st, mats, mode=search, cell=3.0, radius=3.5)
+    assert created == len(want) > 1000
+    assert len(got) == len(pre) + created
+    _cmp(got[:len(pre)], pre, CREATE_FIELDS)
+    _cmp(got[len(pre):], want, CREATE_FIELDS)
+    np.testing.assert_array_equal(flags, want_flags)
+
+
+def test_create_constraints_capacity_error():
+    st = _solid_lattice(20, 6, frac=1.0)
+    n = len(st["x"])
+    with Context(n, sph=True, temperature=True, max_constraints=50) as c:
+        c.upload(st)
+        T.thermal_upload(c, st)
+        with pytest.raises(Exception):
+            GC.create(c, _params())
+        assert len(GC.download(c)) == 50  # what fitted stays
+
+
+def test_frame_creates_constraints_at_the_reference_point():
+    """One ge_step frame with temperature and constraints: the constraints it
+    creates are the oracle's creation on the state at createConstraints'
+    point of updateScene (after integrateStart and the Morton sort), found
+    with the stage calls on a second context."""
+    st = _solid_lattice(48, 7)
+    n = len(st["x"])
+    mats = O.reference_materials()
+    p = _params(G=1.0, sph=1, sph_h=3.0, sph_cell=3.0, temperature=1, dt=0.004, dt_start=0.004)
+    with Context(n, sph=True, temperature=True, max_constraints=20 * n) as c:
+        c.upload(st)
+        T.thermal_upload(c, st)
+        c.step(p, 1)
+        c.sync()
+        got = GC.download(c)
+    with Context(n, sph=True, temperature=True, max_constraints=8) as c:
+        c.upload(st)
+        T.thermal_upload(c, st)
+        c.integrate_start(p)
+        bb = c.bounding_box()
+        c.compute_morton_keys(bb)
+        c.sort_by_morton_key()
+        at = c.download(["x", "y", "id", "flags"])
+        at.update(T.thermal_download(c))
+    _, want = O.create_constraints({k: at[k] for k in ("x", "y", "id", "flags", "label")}, mats)
+    assert len(want) > 500 and len(got) == len(want)
+    # the frame's solve moved no end of a fresh constraint past breaking
+    _cmp(got, want, ("id1", "id2", "original_length", "stiffness", "resistance", "plasticity_point"))

@@ -138,3 +138,114 @@ def test_library_exports_and_layout(tmp_path):
     p = GC.default_constraint_params()
     assert (p.stiffness_mult, p.resistance_mult, p.unbreakable, p.forces) == (1.0, 1.0, 0, 1)
     assert p.damping == F(0.001) and p.correction_ratio == F(0.013333)
+
+
+# ---- createConstraints' automatic branch (physics.cpp:654-766) -------------
+
+SOL = O.GEO_SOLIDIFIED
+
+
+def _cstate(x, y, ids, flags, labels=None):
+    n = len(x)
+    st = {"x": np.asarray(x, F), "y": np.asarray(y, F), "mass": np.ones(n, F), "id": np.asarray(ids, np.uint32),
+          "flags": np.asarray(flags, np.uint32)}
+    if labels is not None:
+        st["label"] = np.asarray(labels, np.uint32)
+    return st
+
+
+def test_create_constraints_hand_placed():
+    """A (id 5) and B (id 7) 2 apart; C (id 3) 1.5 beyond B, 3.5 from A
+    (3.5^2 = 12.25 > 12); D (id 1) 3.3 from A (inside sqrt 12) but two cells of
+    3 away, so never a candidate.  Only A and C are solidified."""
+    st = _cstate([0.5, 2.5, 4.0, -2.95], [0.5, 0.5, 0.5, 0.5], [5, 7, 3, 1], [SOL, 0, SOL, 0])
+    flags, cons = O.create_constraints(st)
+    np.testing.assert_array_equal(flags, [0, 0, 0, 0])  # hasSolidified consumed
+    assert [(int(c["id1"]), int(c["id2"])) for c in cons] == [(5, 7), (3, 7)]
+    np.testing.assert_array_equal(cons["rest_length"], F([2.0, 1.5]))
+    np.testing.assert_array_equal(cons["original_length"], cons["rest_length"])
+    # no materials: stiffness 60, resistance and plastic point 0.6
+    np.testing.assert_array_equal(cons["stiffness"], F([60, 60]))
+    np.testing.assert_array_equal(cons["resistance"], F([0.6, 0.6]))
+    np.testing.assert_array_equal(cons["plasticity_point"], F([0.6, 0.6]))
+    assert not cons["is_broken"].any() and not cons["is_plastic"].any() and not cons["displacement"].any()
+    # a lower id on the other side: only the particle with the lower id creates the pair
+    st2 = _cstate([0.5, 2.5], [0.5, 0.5], [9, 4], [SOL, 0])
+    _, cons2 = O.create_constraints(st2)
+    assert len(cons2) == 0
+
+
+def test_create_constraints_materials():
+    """Rock (2, cold point 0) next to water (1, cold point 273.2) is skipped
+    from the rock's side; rock-rock takes rock's stiffness (60) and the mean
+    resistance (0.211); iron-rock takes iron's stiffness and the means; water
+    creating toward rock is allowed (the rule looks at i's cold point)."""
+    mats = O.reference_materials()
+    by_id = {m.id: m for m in mats}
+    x = [0.5, 1.5, 2.5, 3.5]
+    st = _cstate(x, [0.5] * 4, [1, 2, 3, 4], [SOL, SOL, SOL, 0], labels=[2, 1, 3, 2])
+    _, cons = O.create_constraints(st, mats)
+    got = [(int(c["id1"]), int(c["id2"])) for c in cons]
+    # i = id1 (rock): j = id2 water skipped, id3 iron ok, id4 rock ok
+    # i = id2 (water): j = id3 iron, id4 rock (water's cold point is not 0)
+    # i = id3 (iron): j = id4 rock
+    assert got == [(1, 3), (1, 4), (2, 3), (2, 4), (3, 4)]
+    k = {p: c for p, c in zip(got, cons)}
+    rock, water, iron = by_id[2], by_id[1], by_id[3]
+    assert k[(1, 4)]["stiffness"] == F(rock.constraint_stiffness)
+    assert k[(1, 4)]["resistance"] == F((F(rock.constraint_resistance) + F(rock.constraint_resistance)) * F(0.5))
+    assert k[(1, 3)]["resistance"] == F((F(rock.constraint_resistance) + F(iron.constraint_resistance)) * F(0.5))
+    assert k[(1, 3)]["plasticity_point"] == F(
+        (F(rock.constraint_plastic_point) + F(iron.constraint_plastic_point)) * F(0.5))
+    assert k[(2, 3)]["stiffness"] == F(water.constraint_stiffness)
+    assert k[(3, 4)]["stiffness"] == F(iron.constraint_stiffness)
+
+
+def _brute_create(st, mats, cell=3.0):
+    """O(N^2) restatement written independently: every ordered pair that the
+    rules admit, ordered by (i, cell dx, cell dy, j)."""
+    x, y, ids, fl = st["x"], st["y"], st["id"], st["flags"]
+    lab = st.get("label")
+    mat = {m.id: m for m in mats}
+    cx = np.floor(x / F(cell)).astype(np.int64)
+    cy = np.floor(y / F(cell)).astype(np.int64)
+    rows = []
+    for i in np.nonzero(fl & SOL)[0]:
+        dx, dy = x - x[i], y - y[i]
+        d2 = dx * dx + dy * dy
+        ok = (np.abs(cx - cx[i]) <= 1) & (np.abs(cy - cy[i]) <= 1) & (np.arange(len(x)) != i) & (d2 <= F(12))
+        ok &= ids[i] < ids
+        mi = mat.get(int(lab[i])) if lab is not None else None
+        for j in np.nonzero(ok)[0]:
+            mj = mat.get(int(lab[j])) if lab is not None else None
+            if mi is not None and mj is not None and mi.cold_point == 0.0 and mj.cold_point != 0.0:
+                continue
+            both = mi is not None and mj is not None
+            rows.append(((int(i), int(cx[j] - cx[i]), int(cy[j] - cy[i]), int(j)),
+                         (int(ids[i]), int(ids[j]), F(np.sqrt(d2[j])),
+                          F(mi.constraint_stiffness) if both else F(60),
+                          (F(mi.constraint_resistance) + F(mj.constraint_resistance)) * F(0.5) if both else F(0.6),
+                          (F(mi.constraint_plastic_point) + F(mj.constraint_plastic_point)) * F(0.5) if both else F(0.6))))
+    rows.sort(key=lambda r: r[0])
+    return [r[1] for r in rows]
+
+
+def test_create_constraints_lattice_against_brute_force():
+    rng = np.random.default_rng(3)
+    side = 40
+    gx, gy = np.meshgrid(np.arange(side), np.arange(side))
+    x = (gx.ravel() * 1.7 + (gy.ravel() % 2) * 0.85 + rng.normal(0, 0.15, side * side) - 20).astype(F)
+    y = (gy.ravel() * 1.47 + rng.normal(0, 0.15, side * side) - 15).astype(F)
+    n = len(x)
+    ids = rng.permutation(n).astype(np.uint32) + 100
+    flags = np.where(rng.random(n) < 0.3, SOL, 0).astype(np.uint32) | np.where(rng.random(n) < 0.2, 2, 0).astype(np.uint32)
+    labels = rng.integers(0, 8, n).astype(np.uint32)
+    st = _cstate(x, y, ids, flags, labels=labels)
+    mats = O.reference_materials()
+    out_flags, cons = O.create_constraints(st, mats)
+    want = _brute_create(st, mats)
+    assert len(want) > 500
+    got = [(int(c["id1"]), int(c["id2"]), c["rest_length"], c["stiffness"], c["resistance"], c["plasticity_point"])
+           for c in cons]
+    assert got == want
+    np.testing.assert_array_equal(out_flags, flags & ~np.uint32(SOL))
