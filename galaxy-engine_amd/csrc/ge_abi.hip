@@ -96,10 +96,10 @@ void free_all(ge_ctx *c) {
     dfree(b.ckeep); dfree(b.cblock); dfree(b.ccount); dfree(b.corder); dfree(b.cstart); dfree(b.cw_a);
     dfree(b.cw_b); dfree(b.chist);
     b.cframe = b.cleft = nullptr;
-    dfree(b.nodes); dfree(b.node_geom); dfree(b.node_mp); dfree(b.mp_cnt); dfree(b.mp_wave_list); dfree(b.mp_big_list); dfree(b.mp_piece_owner); dfree(b.mp_piece_sum); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd);
+    dfree(b.nodes); dfree(b.node_geom); dfree(b.node_mp); dfree(b.mp_cnt); dfree(b.mp_wave_list); dfree(b.mp_big_list); dfree(b.mp_piece_owner); dfree(b.mp_piece_sum); dfree(b.sphf); dfree(b.sph_pp); dfree(b.sph_pd); dfree(b.sph_ppxy);
     for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
         dfree(g->keys); dfree(g->keys_alt); dfree(g->first); dfree(g->ord); dfree(g->frame);
-        dfree(g->pk0); dfree(g->pk1); dfree(g->pkf);
+        dfree(g->pk0); dfree(g->pk1); dfree(g->pkf); dfree(g->xy);
     }
 }
 
@@ -188,6 +188,7 @@ int alloc_all(ge_ctx *c) {
         GE_HIP_CHECK(c, dalloc(b.sphf, 2ull * cap));
         GE_HIP_CHECK(c, dalloc(b.sph_pp, 4ull * cap));
         GE_HIP_CHECK(c, dalloc(b.sph_pd, 2ull * cap));
+        GE_HIP_CHECK(c, dalloc(b.sph_ppxy, 2ull * cap));
         for (Grid *g : {&b.grid_pos, &b.grid_pred}) {
             // dense cell table: at least 4 cells per particle of capacity, at most 2^24
             uint64_t cells = 4ull * cap + 4096;
@@ -202,6 +203,7 @@ int alloc_all(ge_ctx *c) {
             GE_HIP_CHECK(c, dalloc(g->pk0, cap));
             GE_HIP_CHECK(c, dalloc(g->pk1, cap));
             GE_HIP_CHECK(c, dalloc(g->pkf, cap));
+            GE_HIP_CHECK(c, dalloc(g->xy, cap));
         }
     }
     return GE_OK;

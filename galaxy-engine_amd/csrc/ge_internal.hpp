@@ -134,6 +134,7 @@ struct Grid {
     float4 *pk0 = nullptr;         // per cell-order position: packed fields
     float4 *pk1 = nullptr;
     uint32_t *pkf = nullptr;       // flags in cell order
+    float2 *xy = nullptr;          // the position the grid was built on, NaN for particles no pass visits
     uint32_t cells_cap = 0;
 };
 constexpr int kCellBits = 24;      // cell keys below 2^24
@@ -177,6 +178,7 @@ struct Buffers {
     float *sphf = nullptr;         // SPH force scratch (x, y interleaved), 2 per particle
     float *sph_pp = nullptr;       // grid-A order (predX, predY, restDens, sphMass)
     float *sph_pd = nullptr;       // grid-A order (press, predDens)
+    float *sph_ppxy = nullptr;     // grid-A order (predX, predY) of visitors, NaN otherwise
     Grid grid_pos, grid_pred;
     // temperature coupling (enable_temperature): Node::gridTemp per final
     // node, the bottom-up pass's parent links and arrival counts, the

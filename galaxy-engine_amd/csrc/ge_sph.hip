@@ -60,7 +60,14 @@ struct GridView {
     const int32_t *frame;  // cx0, cy0, W, H, valid
     const float4 *pk0, *pk1;
     const uint32_t *pkf;
+    const float2 *xy;      // position of the candidate, NaN when no pass visits it
 };
+
+// a NaN position fails every distance test, so a candidate that takes no part
+// costs one 8-byte load
+__device__ __forceinline__ float2 lean_pos(bool visits, float x, float y) {
+    return visits ? make_float2(x, y) : make_float2(__builtin_nanf(""), __builtin_nanf(""));
+}
 
 __device__ __forceinline__ bool is_visitor(uint32_t f) { return (f & GE_FLAG_SPH) && !(f & GE_FLAG_DRAWN); }
 
@@ -218,29 +225,50 @@ __global__ __launch_bounds__(kBlock) void k_cell_table(const uint64_t *__restric
 // cell-order packs of grid A: (x, y, vx, vy), (sphMass, visc, cohesion, dens), flags
 __global__ __launch_bounds__(kBlock) void k_pack_a(State s, uint32_t n, const uint32_t *__restrict__ ord,
                                                    float4 *__restrict__ pk0, float4 *__restrict__ pk1,
-                                                   uint32_t *__restrict__ pkf) {
+                                                   uint32_t *__restrict__ pkf, float2 *__restrict__ xy) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= n) return;
     const uint32_t i = ord[k];
-    pk0[k] = make_float4(s.x[i], s.y[i], s.vx[i], s.vy[i]);
+    const float x = s.x[i], y = s.y[i];
+    const uint32_t f = s.flags[i];
+    pk0[k] = make_float4(x, y, s.vx[i], s.vy[i]);
     pk1[k] = make_float4(s.sph_mass[i], s.visc[i], s.cohesion[i], s.dens[i]);
-    pkf[k] = s.flags[i];
+    pkf[k] = f;
+    xy[k] = lean_pos(is_visitor(f), x, y);
+}
+
+// The pair passes give each particle kSphLanes lanes: lane q of the group
+// takes the candidates lo + q, lo + q + kSphLanes, ... of every row, and the
+// group adds its partial sums in a fixed butterfly order, so the result is
+// deterministic.  At the particle counts of an SPH scene this quadruples the
+// waves in flight, which hides the latency of the candidate reads.
+constexpr uint32_t kSphLanes = 8;
+
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (uint32_t o = 1; o < kSphLanes; o <<= 1) v += __shfl_xor(v, (int)o, 64);
+    return v;
 }
 
 // SPH::computeViscCohesionForces (SPH.cpp:98-150) in gather form, then the
 // predicted position (SPH.cpp:173-181).  pp[k] = (predX, predY, restDens, sphMass).
 __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A, SphConst K,
-                                                 float2 *__restrict__ sphf, float4 *__restrict__ pp) {
+                                                 float2 *__restrict__ sphf, float4 *__restrict__ pp,
+                                                 float2 *__restrict__ ppxy) {
 #pragma clang fp contract(off)
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t k = t / kSphLanes, q0 = t % kSphLanes;
     if (k >= n) return;
     const uint32_t i = A.ord[k];
     const uint32_t fi = A.pkf[k];
     const float4 own = A.pk0[k];
     const float4 oq = A.pk1[k];
     if (!is_visitor(fi)) {
-        sphf[i] = make_float2(0.0f, 0.0f);
-        pp[k] = make_float4(s.ppx[i], s.ppy[i], s.rest_dens[i], oq.x);
+        if (q0 == 0) {
+            sphf[i] = make_float2(0.0f, 0.0f);
+            pp[k] = make_float4(s.ppx[i], s.ppy[i], s.rest_dens[i], oq.x);
+            ppxy[k] = lean_pos(false, 0.0f, 0.0f);
+        }
         return;
     }
     const bool fwd = !(fi & GE_FLAG_PINNED);
@@ -251,15 +279,15 @@ __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A
     for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
         uint32_t lo, hi;
         if (!row_range(A.frame, A.first, cx, cy + dr, K.S.R, lo, hi)) continue;
-        for (uint32_t kk = lo; kk < hi; ++kk) {
-            if (kk == k) continue;
-            const uint32_t fj = A.pkf[kk];
-            if (!is_visitor(fj)) continue;
-            const float4 pj = A.pk0[kk];
-            const float dx = pj.x - own.x, dy = pj.y - own.y;
+        for (uint32_t kk = lo + q0; kk < hi; kk += kSphLanes) {
+            // non-visitors carry NaN and fail both tests
+            const float2 q = A.xy[kk];
+            const float dx = q.x - own.x, dy = q.y - own.y;
             if (!within(K.S, dx, dy)) continue;
             const float rsq = dx * dx + dy * dy;
-            if (rsq >= K.h2) continue;
+            if (!(rsq < K.h2) || kk == k) continue;
+            const float4 pj = A.pk0[kk];
+            const uint32_t fj = A.pkf[kk];
             const float r = sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
             const float4 qj = A.pk1[kk];
             const float wc = coh_kernel(r, K.h);
@@ -286,25 +314,33 @@ __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A
             }
         }
     }
+    sx = group_sum(sx);
+    sy = group_sum(sy);
+    if (q0 != 0) return;
     sphf[i] = make_float2(sx, sy);
     const float px = own.x + (sx / oq.x) * K.coeff;
     const float py = own.y + (sy / oq.x) * K.coeff;
     s.ppx[i] = px;
     s.ppy[i] = py;
     pp[k] = make_float4(px, py, s.rest_dens[i], oq.x);
+    ppxy[k] = make_float2(px, py);
 }
 
 // predicted density and pressure (SPH.cpp:183-219); pd[k] = (press, predDens)
 __global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridView A, const float4 *__restrict__ pp,
-                                                    SphConst K, float2 *__restrict__ pd) {
+                                                    const float2 *__restrict__ ppxy, SphConst K,
+                                                    float2 *__restrict__ pd) {
 #pragma clang fp contract(off)
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t k = t / kSphLanes, q0 = t % kSphLanes;
     if (k >= n) return;
     const uint32_t i = A.ord[k];
     const uint32_t fi = A.pkf[k];
     if (!is_visitor(fi)) {
-        s.press[i] = 0.0f;  // press is cleared for every particle (SPH.cpp:162-165)
-        pd[k] = make_float2(0.0f, s.pred_dens[i]);
+        if (q0 == 0) {
+            s.press[i] = 0.0f;  // press is cleared for every particle (SPH.cpp:162-165)
+            pd[k] = make_float2(0.0f, s.pred_dens[i]);
+        }
         return;
     }
     const float4 me = pp[k];
@@ -313,16 +349,17 @@ __global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridVie
     for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
         uint32_t lo, hi;
         if (!row_range(A.frame, A.first, cx, cy + dr, K.S.R, lo, hi)) continue;
-        for (uint32_t kk = lo; kk < hi; ++kk) {
-            if (!is_visitor(A.pkf[kk])) continue;
+        for (uint32_t kk = lo + q0; kk < hi; kk += kSphLanes) {
+            // predicted positions, NaN for non-visitors (they fail the test)
+            const float2 q = ppxy[kk];
+            const float dx = me.x - q.x, dy = me.y - q.y;
+            const float rrsq = dx * dx + dy * dy;
+            if (!(rrsq < K.h * K.h)) continue;
             if (K.S.mode) {  // filter on the candidate's grid position (pos)
-                const float4 gj = A.pk0[kk];
+                const float2 gj = A.xy[kk];
                 if (!within(K.S, gj.x - me.x, gj.y - me.y)) continue;
             }
             const float4 pj = pp[kk];
-            const float dx = me.x - pj.x, dy = me.y - pj.y;
-            const float rrsq = dx * dx + dy * dy;
-            if (rrsq >= K.h * K.h) continue;
             const float rr = sqrtf(rrsq);
             const float mJ = pj.w * K.unit;
             const float rho0 = 0.5f * (me.z + pj.z);
@@ -330,6 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridVie
             dens = dens + mJ * w / rho0;
         }
     }
+    dens = group_sum(dens);
+    if (q0 != 0) return;
     float pt = K.delta * (dens - me.z);
     if (pt < 0.0f) pt = 0.0f;
     const float pr = 0.0f + pt * s.stiff[i] * K.stiff_mult;
@@ -341,12 +380,15 @@ __global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridVie
 
 // grid B packs: (predX, predY, press, predDens), (sphMass, -, -, -)
 __global__ __launch_bounds__(kBlock) void k_pack_b(State s, uint32_t n, const uint32_t *__restrict__ ord,
-                                                   float4 *__restrict__ pk0, float4 *__restrict__ pk1) {
+                                                   float4 *__restrict__ pk0, float4 *__restrict__ pk1,
+                                                   float2 *__restrict__ xy) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= n) return;
     const uint32_t i = ord[k];
-    pk0[k] = make_float4(s.ppx[i], s.ppy[i], s.press[i], s.pred_dens[i]);
+    const float x = s.ppx[i], y = s.ppy[i];
+    pk0[k] = make_float4(x, y, s.press[i], s.pred_dens[i]);
     pk1[k] = make_float4(s.sph_mass[i], 0.0f, 0.0f, 0.0f);
+    xy[k] = make_float2(x, y);  // grid B holds visitors only
 }
 
 // pressure force of one visit (SPH.cpp:233-256); dx, dy = predPos_visitor - predPos_other
@@ -369,14 +411,16 @@ __device__ __forceinline__ void pressure_pair(float dx, float dy, float rr, floa
 // pressure forces (SPH.cpp:221-264), pressF and acc (SPH.cpp:271-280), then
 // the integrateEnd kick (physics.cpp:1114-1125) when the frame asks for it
 __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridView A, const float4 *__restrict__ pp,
-                                                     const float2 *__restrict__ pd, GridView B, SphConst K,
-                                                     const float2 *__restrict__ sphf) {
+                                                     const float2 *__restrict__ ppxy, const float2 *__restrict__ pd,
+                                                     GridView B, SphConst K, const float2 *__restrict__ sphf) {
 #pragma clang fp contract(off)
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t k = t / kSphLanes, q0 = t % kSphLanes;
     if (k >= n) return;
     const uint32_t i = A.ord[k];
     const uint32_t fi = A.pkf[k];
     float2 f = sphf[i];
+    float gx = 0.0f, gy = 0.0f;  // this lane's share of the pair forces
     if (is_visitor(fi)) {
         const float4 me = pp[k];
         const float2 mp = pd[k];
@@ -385,21 +429,21 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
         for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
             uint32_t lo, hi;
             if (!row_range(A.frame, A.first, qx, qy + dr, K.S.R, lo, hi)) continue;
-            for (uint32_t kk = lo; kk < hi; ++kk) {
-                if (kk == k || !is_visitor(A.pkf[kk])) continue;
+            for (uint32_t kk = lo + q0; kk < hi; kk += kSphLanes) {
+                const float2 qp = ppxy[kk];  // NaN for non-visitors: fails the test
+                const float dx = me.x - qp.x, dy = me.y - qp.y;
+                const float rr = sqrtf(dx * dx + dy * dy);
+                if (!(rr >= 1e-5f && rr < K.h) || kk == k) continue;
                 if (K.S.mode) {
-                    const float4 gj = A.pk0[kk];
+                    const float2 gj = A.xy[kk];
                     if (!within(K.S, gj.x - me.x, gj.y - me.y)) continue;
                 }
                 const float4 pj = pp[kk];
-                const float dx = me.x - pj.x, dy = me.y - pj.y;
-                const float rr = sqrtf(dx * dx + dy * dy);
-                if (rr < 1e-5f || rr >= K.h) continue;
                 const float2 q = pd[kk];
                 float fx, fy;
                 pressure_pair(dx, dy, rr, me.w, pj.w, mp.x, q.x, mp.y, q.y, K, fx, fy);
-                f.x = f.x + fx;
-                f.y = f.y + fy;
+                gx = gx + fx;
+                gy = gy + fy;
             }
         }
         // visits of i by others: predPos cells around cell(pos_i)
@@ -408,22 +452,28 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
         for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
             uint32_t lo, hi;
             if (!row_range(B.frame, B.first, px, py + dr, K.S.R, lo, hi)) continue;
-            for (uint32_t kb = lo; kb < hi; ++kb) {
+            for (uint32_t kb = lo + q0; kb < hi; kb += kSphLanes) {
+                const float2 qp = B.xy[kb];
+                const float dx = qp.x - me.x, dy = qp.y - me.y;
+                const float rr = sqrtf(dx * dx + dy * dy);
+                if (!(rr >= 1e-5f && rr < K.h)) continue;
+                // j's query at predPos_j meets i at pos_i
+                if (!within(K.S, mypos.x - qp.x, mypos.y - qp.y)) continue;
                 if (B.ord[kb] == i) continue;
                 const float4 pj = B.pk0[kb];
-                // j's query at predPos_j meets i at pos_i
-                if (!within(K.S, mypos.x - pj.x, mypos.y - pj.y)) continue;
-                const float dx = pj.x - me.x, dy = pj.y - me.y;
-                const float rr = sqrtf(dx * dx + dy * dy);
-                if (rr < 1e-5f || rr >= K.h) continue;
                 const float mj = B.pk1[kb].x;
                 float fx, fy;
                 pressure_pair(dx, dy, rr, mj, me.w, pj.z, mp.x, pj.w, mp.y, K, fx, fy);
-                f.x = f.x - fx;
-                f.y = f.y - fy;
+                gx = gx - fx;
+                gy = gy - fy;
             }
         }
     }
+    gx = group_sum(gx);
+    gy = group_sum(gy);
+    if (q0 != 0) return;
+    f.x = f.x + gx;
+    f.y = f.y + gy;
     const float sm = s.sph_mass[i];
     const float pfx = f.x / sm, pfy = f.y / sm;
     s.pfx[i] = pfx;
@@ -676,7 +726,7 @@ SphConst make_const(const ge_step_params *p, float half_dt) {
     return K;
 }
 
-GridView view(const Grid &g) { return GridView{g.first, g.ord, g.frame, g.pk0, g.pk1, g.pkf}; }
+GridView view(const Grid &g) { return GridView{g.first, g.ord, g.frame, g.pk0, g.pk1, g.pkf, g.xy}; }
 
 uint32_t grid_blocks(uint32_t n) {
     uint32_t b = blocks_for(n, kBlock);
@@ -706,7 +756,7 @@ int sph_new_grid(ge_ctx *c, const ge_step_params *p) {
                        g.frame, g.keys);
     build_grid(c, g);
     hipLaunchKernelGGL(k_pack_a, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, s, n, g.ord, g.pk0, g.pk1,
-                       g.pkf);
+                       g.pkf, g.xy);
     c->have_grid = true;
     c->sph_cell = p->sph_cell;
     c->sph_search = p->sph_search;
@@ -722,11 +772,13 @@ int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt) {
     const uint32_t n = c->n;
     const SphConst K = make_const(p, end_kick_dt * 0.5f);
     const uint32_t nb = blocks_for(n, kBlock);
+    const uint32_t nbl = blocks_for((uint64_t)n * kSphLanes, kBlock);  // the pair passes: kSphLanes per particle
     float2 *sphf = reinterpret_cast<float2 *>(c->b.sphf);
     // pp (predX, predY, restDens, sphMass) and pd (press, predDens) in grid-A order
     float4 *ppA = reinterpret_cast<float4 *>(c->b.sph_pp);
     float2 *pdA = reinterpret_cast<float2 *>(c->b.sph_pd);
-    hipLaunchKernelGGL(k_visc, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view(A), K, sphf, ppA);
+    float2 *ppxy = reinterpret_cast<float2 *>(c->b.sph_ppxy);
+    hipLaunchKernelGGL(k_visc, dim3(nbl), dim3(kBlock), 0, c->stream, s, n, view(A), K, sphf, ppA, ppxy);
     // grid B over the predicted positions of visiting particles
     const uint32_t mb = grid_blocks(n);
     hipLaunchKernelGGL(k_minmax, dim3(mb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.flags, n, 1,
@@ -736,9 +788,10 @@ int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt) {
     hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.flags, n, 1, K.S, B.frame,
                        B.keys);
     build_grid(c, B);
-    hipLaunchKernelGGL(k_density, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view(A), ppA, K, pdA);
-    hipLaunchKernelGGL(k_pack_b, dim3(nb), dim3(kBlock), 0, c->stream, s, n, B.ord, B.pk0, B.pk1);
-    hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view(A), ppA, pdA, view(B), K, sphf);
+    hipLaunchKernelGGL(k_density, dim3(nbl), dim3(kBlock), 0, c->stream, s, n, view(A), ppA, ppxy, K, pdA);
+    hipLaunchKernelGGL(k_pack_b, dim3(nb), dim3(kBlock), 0, c->stream, s, n, B.ord, B.pk0, B.pk1, B.xy);
+    hipLaunchKernelGGL(k_pressure, dim3(nbl), dim3(kBlock), 0, c->stream, s, n, view(A), ppA, ppxy, pdA, view(B), K,
+                       sphf);
     return GE_OK;
 }
 
