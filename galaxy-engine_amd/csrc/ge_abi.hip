@@ -115,6 +115,7 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.perm, cap));
     b.hist_cap = radix_hist_entries(cap);
     GE_HIP_CHECK(c, dalloc(b.hist, b.hist_cap));
+    GE_HIP_CHECK(c, hipMemset(b.hist, 0, sizeof(uint32_t) * b.hist_cap));
     b.max_chunks = cap / kChunk + c->max_sys + 1;
     GE_HIP_CHECK(c, dalloc(b.partials, 4ull * b.max_chunks));
     GE_HIP_CHECK(c, dalloc(b.partials2, 4 * 4096));
@@ -176,6 +177,7 @@ int alloc_all(ge_ctx *c) {
         GE_HIP_CHECK(c, dalloc(b.cw_a, words));
         GE_HIP_CHECK(c, dalloc(b.cw_b, words));
         GE_HIP_CHECK(c, dalloc(b.chist, radix_hist_entries((uint32_t)words)));
+        GE_HIP_CHECK(c, hipMemset(b.chist, 0, sizeof(uint32_t) * radix_hist_entries((uint32_t)words)));
     }
     if (c->thermal) {
         GE_HIP_CHECK(c, dalloc(b.node_temp, b.node_cap));

@@ -133,6 +133,7 @@ int alloc_all(ge3_ctx *c) {
     GE3_CHECK(c, dalloc(b.sv_b, cap));
     GE3_CHECK(c, dalloc(b.perm, cap));
     GE3_CHECK(c, dalloc(b.hist, radix_hist_entries(cap)));
+    GE3_CHECK(c, hipMemset(b.hist, 0, sizeof(uint32_t) * radix_hist_entries(cap)));
     GE3_CHECK(c, dalloc(b.partials, 8ull * blocks_for(cap, kBBoxChunk3)));
     GE3_CHECK(c, dalloc(b.bb, 12));
     GE3_CHECK(c, dalloc(b.counters, kCntSlots));

@@ -462,6 +462,7 @@ int merger_run(ge_ctx *c, const ge_merger_params *mp, std::vector<uint32_t> &del
         const size_t cap = M.cap;
         if (!(need(hipMalloc(&M.wa, sizeof(uint64_t) * cap)) && need(hipMalloc(&M.wb, sizeof(uint64_t) * cap)) &&
               need(hipMalloc(&M.hist, sizeof(uint32_t) * radix_hist_entries((uint32_t)cap))) &&
+              need(hipMemset(M.hist, 0, sizeof(uint32_t) * radix_hist_entries((uint32_t)cap))) &&
               need(hipMalloc(&M.rad, sizeof(float) * cap)) && need(hipMalloc(&M.del, sizeof(uint32_t) * cap)) &&
               need(hipMalloc(&M.ecnt, sizeof(uint32_t) * cap)) &&
               need(hipMalloc(&M.eoff, sizeof(uint32_t) * (cap + 1))) &&
