@@ -620,6 +620,9 @@ int ge_create(ge_ctx **out, const ge_config *cfg) {
         delete c;
         return r;
     }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && cus > 0)
+        c->cus = (uint32_t)cus;
     t_alloc_bytes = 0;
     int ra = alloc_all(c);
     c->device_bytes = t_alloc_bytes;

@@ -69,8 +69,12 @@ enum CounterSlot : int {
     kCntLevelBase0 = 8 + 64,  // first top index of level L
     kCntBlistAlloc = 8 + 128, // entries of the per-subtree internal-node lists
     kCntBlistOverflow = 8 + 129, // a workgroup subtree's list did not fit
-    kCntSlots = 8 + 136,
+    kCntWalkQ0 = 8 + 136,   // kWalkQueues tickets of the persistent walk, then its finished-wave count
+    kCntSlots = 8 + 146,
 };
+
+// ticket ranges of a persistent walk launch (one per XCD)
+constexpr uint32_t kWalkQueues = 8;
 
 // Periodic domain of the reference's isPeriodicBoundaryEnabled && !infiniteDomain
 // (w = 0: no wrap).
@@ -223,6 +227,7 @@ struct ge_ctx {
     uint32_t nsys = 1;
     uint32_t nchunks = 1;
     uint32_t nwaves = 0;  // walk waves: each system's particles in 64-particle waves from its first slot
+    uint32_t cus = 256;   // compute units of the device (sizes persistent launches)
     int idx_bits = 28;             // low bits of a packed sort word: slot within the system
     int sys_bits = 0;              // high bits: system index
     std::vector<uint32_t> host_off;  // system offsets (host copy)

@@ -244,8 +244,8 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // (W the per-system wave prefix), so a wave holds the same particles whether
 // the system runs alone or in a batch, and rules that look at the whole wave
 // (the leaf rule of the accuracy modes) give every system its alone result.
-template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
-__global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST>
+__global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                                            const NodeRec *__restrict__ nodes,
                                            const float4 *__restrict__ mp,
                                            const uint32_t *__restrict__ start_prefix,
@@ -258,263 +258,306 @@ __global__ __launch_bounds__(BS) void k_bh(const float4 *__restrict__ pxym,
                                            const uint32_t *__restrict__ flags, float half_dt_end,
                                            Domain dom, unsigned long long *__restrict__ inter,
                                            const float *__restrict__ gtavg, float *__restrict__ temp,
-                                           float heat_k, float heat_dt) {
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t w = __builtin_amdgcn_readfirstlane(blk * (BS / kWave) + (threadIdx.x / kWave));
-    // the wave's system: the last s with W_s <= w (wave-uniform search)
-    uint32_t slo = 0, shi = nsys;
-    while (shi - slo > 1u) {
-        const uint32_t mid = (slo + shi) >> 1;
-        if (sys_wave_off[mid] <= w) slo = mid;
-        else shi = mid;
-    }
-    const uint32_t sy = slo;
-    const uint32_t i = sys_off[sy] + (w - sys_wave_off[sy]) * kWave + (threadIdx.x & (kWave - 1));
-    const bool per = dom.w > 0.0f;
-    const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
-    const bool live = w < sys_wave_off[nsys] && i < sys_off[sy + 1];
-    const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float ti = (TEMP && live) ? temp[i] : 0.0f;  // temp[i], moved by every accepted node
-    // a tree that raised an error flag may hold unwritten nodes: do not walk it
-    const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | (ORDER > 0 ? kErrMultipole : 0u));
-    const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
-    uint32_t root = kNone, end = 0u;
-    if (live) {
-        root = start_prefix[sys_off[sy]];
-        end = min(start_prefix[sys_off[sy + 1]], node_count);
-    }
-    uint32_t wlo = root, whi = end;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        wlo = min(wlo, (uint32_t)__shfl_xor((int)wlo, o, kWave));
-        whi = max(whi, (uint32_t)__shfl_xor((int)whi, o, kWave));
-    }
-    const uint32_t walk_end = __builtin_amdgcn_readfirstlane(whi);
-    uint32_t skip_until = root;
-    float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
-    uint32_t cnt = 0, cnt_nodes = 0, visits = 0, leaf_trips = 0;  // visits and trips: per wave
-    uint32_t idx = __builtin_amdgcn_readfirstlane(wlo);
-    while (idx < walk_end) {
-        const NodeRec nd = nodes[idx];
-        // ORDER > 0: the whole multipole record is fetched with the node, so
-        // an accepting wave does not wait for a second load
-        float4 h0, h1, h2, h3, h4, h5;
-        if (ORDER > 0) {
-            h0 = mp[6ull * idx];
-            h1 = mp[6ull * idx + 1];
-        }
-        if (ORDER >= 3) {
-            h2 = mp[6ull * idx + 2];
-            h3 = mp[6ull * idx + 3];
-        }
-        if (ORDER >= 4) {
-            h4 = mp[6ull * idx + 4];
-            h5 = mp[6ull * idx + 5];
-        }
-        const uint32_t next = nd.next;
-        if (COUNT) ++visits;
-        // index after this subtree, saturated so the walk always moves forward
-        const uint32_t after = (next >= node_count - idx) ? node_count : idx + next + 1;
-        if (nd.mass <= 0.0f) {  // empty node: skip the subtree (physics.cpp:107-110)
-            idx = after;
-            continue;
-        }
-        const bool active = idx >= skip_until && idx < end;
-        float dx, dy, dist_sq;
-        bool accept;
-        if (ORDER > 0) {
-            dx = h0.x - p.x;
-            dy = h0.y - p.y;
-            if (per) {
-                dx = wrap_d(dx, dom.w, hw);
-                dy = wrap_d(dy, dom.h, hh);
-            }
-            const float d2 = dx * dx + dy * dy;
-            dist_sq = d2 + soft_sq;
-            accept = d2 > h0.z;  // h0.z holds the squared opening radius
+                                           float heat_k, float heat_dt, uint32_t *__restrict__ walk_q) {
+    // !PERSIST: wave w of the launch walks wave w of the layout.
+    // PERSIST: the launch is persistent.  The layout's workgroup tiles (BS/64
+    // consecutive waves, the tile a one-shot launch gives a workgroup) are cut
+    // into kWalkQueues contiguous ranges; workgroup b starts on range
+    // b mod kWalkQueues (the XCD that hardware dispatch gives it, so
+    // neighbouring tiles share an L2) and, when its range runs dry, takes
+    // tiles from the next ranges.  walk_q[0..kWalkQueues) are the ranges'
+    // tickets and walk_q[kWalkQueues] counts the waves that have finished;
+    // the last wave out clears them all for the next launch.
+    __shared__ uint32_t tile_s;
+    const uint32_t total_tiles = (sys_wave_off[nsys] + (BS / kWave) - 1) / (BS / kWave);
+    uint32_t q_at = blockIdx.x % kWalkQueues, q_seen = 0;
+    bool first = true;
+    for (;;) {
+        uint32_t blk;
+        if (!PERSIST) {
+            if (!first) break;
+            first = false;
+            blk = xcd_block(blockIdx.x, gridDim.x);
         } else {
-#pragma clang fp contract(off)
-            dx = nd.comx - p.x;
-            dy = nd.comy - p.y;
-            if (per) {  // scalar variant (physics.cpp:117-120)
-                dx = wrap_d(dx, dom.w, hw);
-                dy = wrap_d(dy, dom.h, hh);
+            blk = kNone;
+            while (q_seen < kWalkQueues) {
+                const uint32_t lo = (uint32_t)((uint64_t)total_tiles * q_at / kWalkQueues);
+                const uint32_t hi = (uint32_t)((uint64_t)total_tiles * (q_at + 1) / kWalkQueues);
+                __syncthreads();  // every wave has read the previous ticket
+                if (threadIdx.x == 0) tile_s = atomicAdd(&walk_q[q_at], 1u);
+                __syncthreads();
+                const uint32_t t = tile_s;
+                if (t < hi - lo) {
+                    blk = lo + t;
+                    break;
+                }
+                q_at = (q_at + 1) % kWalkQueues;
+                ++q_seen;
             }
-            dist_sq = dx * dx + dy * dy + soft_sq;
-            accept = nd.size_sq < theta_sq * dist_sq;
+            if (blk == kNone) break;
         }
-        const bool leaf = next == 0;
-        // ORDER > 0: a leaf that any active lane opens is summed directly by
-        // every active lane (exact, and the wave runs that loop anyway); those
-        // lanes skip the leaf's expansion.  ORDER 0 keeps the reference's
-        // per-lane choice.
-        bool direct = active && !accept && leaf;
-        if (ORDER > 0 && leaf && __ballot(direct)) direct = active;
-        if (active && accept && !direct) {
-            const float inv = __builtin_amdgcn_rsqf(dist_sq);
-            float tx, ty;
-            if (ORDER > 0) {
-                const float qx = -dx, qy = -dy;  // d = target - COM
-                const float r2 = inv * inv;
-                const float gr3 = G * (inv * r2);
-                const float p2x = h1.x * qx + h1.y * qy, p2y = h1.y * qx + h1.z * qy;
-                const float s2 = qx * p2x + qy * p2y;
-                if (ORDER >= 4) {
-                    const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
-                    const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
-                    const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
-                    const float pux = h5.y * qx + h5.z * qy, puy = h5.z * qx + h5.w * qy;
-                    const float s3u = qx * (p3x + pux) + qy * (p3y + puy);
-                    const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y) + h3.z;
-                    const float c0 = qxx * qx, c3 = qyy * qy, c1 = (3.0f * qxx) * qy, c2 = (3.0f * qyy) * qx;
-                    const float p4x = h4.x * c0 + h4.y * c1 + h4.z * c2 + h4.w * c3;
-                    const float p4y = h4.y * c0 + h4.z * c1 + h4.w * c2 + h5.x * c3;
-                    const float s4 = qx * p4x + qy * p4y;
-                    const float sc = gr3 * (r2 * (r2 * (r2 * (r2 * s4 + s3u) + c7) + h1.w) - h0.w);
-                    const float e = (28.0f / 9.0f) * r2, f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
-                    const float ox = (2.0f * p2x + h3.x) + f * ((3.0f * p3x + 2.0f * pux) + e * p4x);
-                    const float oy = (2.0f * p2y + h3.y) + f * ((3.0f * p3y + 2.0f * puy) + e * p4y);
-                    tx = qx * sc + g5 * ox;
-                    ty = qy * sc + g5 * oy;
-                } else if (ORDER == 3) {
-                    // the same series without the fourth-order moments (A4, U, W)
-                    const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
-                    const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
-                    const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
-                    const float s3 = qx * p3x + qy * p3y;
-                    const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y);
-                    const float sc = gr3 * (r2 * (r2 * (r2 * s3 + c7) + h1.w) - h0.w);
-                    const float f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
-                    const float ox = (2.0f * p2x + h3.x) + f * (3.0f * p3x);
-                    const float oy = (2.0f * p2y + h3.y) + f * (3.0f * p3y);
-                    tx = qx * sc + g5 * ox;
-                    ty = qy * sc + g5 * oy;
-                } else {
-                    const float sc = gr3 * (r2 * (r2 * s2 + h1.w) - h0.w);
-                    const float g5 = gr3 * (-0.4f * r2);
-                    tx = qx * sc + g5 * p2x;
-                    ty = qy * sc + g5 * p2y;
-                }
-            } else {
-                const float inv2 = inv * inv;
-                const float inv3 = inv2 * inv;
-                const float f = G * nd.mass * inv3;
-                tx = dx * f;
-                ty = dy * f;
-            }
-            if (KAHAN) {
-                kahan_add(fx, cx, tx);
-                kahan_add(fy, cy, ty);
-            } else {
-                fx += tx;
-                fy += ty;
-            }
-            if (COUNT) {
-                ++cnt;
-                ++cnt_nodes;
-            }
-            if (TEMP) {  // heat transfer toward the node's average (physics.cpp:261-276)
-#pragma clang fp contract(off)
-                const uint32_t cn = nd.end - nd.start;
-                if (cn > 0u) {
-                    const float avg = gtavg[idx];  // gridTemp / cn, divided once per node
-                    const float diff = avg - ti;
-                    const float invd = 1.0f / sqrtf(dist_sq);
-                    float dist = 0.0f;
-                    if (dist_sq > 1e-16f) dist = 1.0f / invd;
-                    if (dist > 1e-8f) {
-                        const float heat = heat_k * diff / dist;
-                        ti = ti + heat * heat_dt;
-                    }
-                }
-            }
-            skip_until = after;
+        const uint32_t w = __builtin_amdgcn_readfirstlane(blk * (BS / kWave) + (threadIdx.x / kWave));
+        // the wave's system: the last s with W_s <= w (wave-uniform search)
+        uint32_t slo = 0, shi = nsys;
+        while (shi - slo > 1u) {
+            const uint32_t mid = (slo + shi) >> 1;
+            if (sys_wave_off[mid] <= w) slo = mid;
+            else shi = mid;
         }
-        const bool open_leaf = direct;
-        if (ORDER > 0 && __ballot(open_leaf)) {
-            // leaf direct sum: the pairs go to a plain per-leaf sum of
-            // m_j d_j r^-3, scaled by G and compensated once per leaf
-            const uint32_t jend = min(nd.end, n);
-            if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
-            float sx = 0.f, sy = 0.f;
-            for (uint32_t j = nd.start; j < jend; ++j) {
-                const float4 q = pxym[j];
-                if (open_leaf && j != i) {
-                    float px = q.x - p.x, py = q.y - p.y;
-                    if (per) {
-                        px = wrap_d(px, dom.w, hw);
-                        py = wrap_d(py, dom.h, hh);
-                    }
-                    const float d2 = px * px + py * py + soft_sq;
-                    const float inv = __builtin_amdgcn_rsqf(d2);
-                    const float f = q.z * (inv * inv * inv);
-                    sx = fmaf(px, f, sx);
-                    sy = fmaf(py, f, sy);
-                    if (COUNT) ++cnt;
-                }
-            }
-            if (open_leaf) {
-                if (KAHAN) {
-                    kahan_add(fx, cx, G * sx);
-                    kahan_add(fy, cy, G * sy);
-                } else {
-                    fx = fmaf(G, sx, fx);
-                    fy = fmaf(G, sy, fy);
-                }
-            }
-        } else if (ORDER == 0 && __ballot(open_leaf)) {
-            // leaf direct sum over [start, end), skipping i == j (physics.cpp:163-196)
-            const uint32_t jend = min(nd.end, n);
-            if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
-            for (uint32_t j = nd.start; j < jend; ++j) {
-                const float4 q = pxym[j];
-                if (open_leaf && j != i) {
-                    float px = q.x - p.x, py = q.y - p.y;
-                    if (per) {  // physics.cpp:169-172
-                        px = wrap_d(px, dom.w, hw);
-                        py = wrap_d(py, dom.h, hh);
-                    }
-                    const float d2 = px * px + py * py + soft_sq;
-                    const float inv = __builtin_amdgcn_rsqf(d2);
-                    const float f = G * q.z * (inv * inv * inv);
-                    if (KAHAN) {
-                        kahan_add(fx, cx, px * f);
-                        kahan_add(fy, cy, py * f);
-                    } else {
-                        fx = fmaf(px, f, fx);
-                        fy = fmaf(py, f, fy);
-                    }
-                    if (COUNT) ++cnt;
-                }
-            }
+        const uint32_t sy = slo;
+        const uint32_t i = sys_off[sy] + (w - sys_wave_off[sy]) * kWave + (threadIdx.x & (kWave - 1));
+        const bool per = dom.w > 0.0f;
+        const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
+        const bool live = w < sys_wave_off[nsys] && i < sys_off[sy + 1];
+        const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float ti = (TEMP && live) ? temp[i] : 0.0f;  // temp[i], moved by every accepted node
+        // a tree that raised an error flag may hold unwritten nodes: do not walk it
+        const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | (ORDER > 0 ? kErrMultipole : 0u));
+        const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
+        uint32_t root = kNone, end = 0u;
+        if (live) {
+            root = start_prefix[sys_off[sy]];
+            end = min(start_prefix[sys_off[sy + 1]], node_count);
         }
-        const bool descend = active && !accept && !leaf;
-        idx = __ballot(descend) ? idx + 1u : after;
-    }
-    if (COUNT) {
-        uint64_t c = cnt, cn = cnt_nodes;
+        uint32_t wlo = root, whi = end;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
-            c += __shfl_xor(c, o, kWave);
-            cn += __shfl_xor(cn, o, kWave);
+            wlo = min(wlo, (uint32_t)__shfl_xor((int)wlo, o, kWave));
+            whi = max(whi, (uint32_t)__shfl_xor((int)whi, o, kWave));
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(inter, (unsigned long long)c);
-            atomicAdd(inter + 1, (unsigned long long)cn);
-            atomicAdd(inter + 2, (unsigned long long)visits);
-            atomicAdd(inter + 3, (unsigned long long)leaf_trips);
+        const uint32_t walk_end = __builtin_amdgcn_readfirstlane(whi);
+        uint32_t skip_until = root;
+        float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
+        uint32_t cnt = 0, cnt_nodes = 0, visits = 0, leaf_trips = 0;  // visits and trips: per wave
+        uint32_t idx = __builtin_amdgcn_readfirstlane(wlo);
+        while (idx < walk_end) {
+            const NodeRec nd = nodes[idx];
+            // ORDER > 0: the whole multipole record is fetched with the node, so
+            // an accepting wave does not wait for a second load
+            float4 h0, h1, h2, h3, h4, h5;
+            if (ORDER > 0) {
+                h0 = mp[6ull * idx];
+                h1 = mp[6ull * idx + 1];
+            }
+            if (ORDER >= 3) {
+                h2 = mp[6ull * idx + 2];
+                h3 = mp[6ull * idx + 3];
+            }
+            if (ORDER >= 4) {
+                h4 = mp[6ull * idx + 4];
+                h5 = mp[6ull * idx + 5];
+            }
+            const uint32_t next = nd.next;
+            if (COUNT) ++visits;
+            // index after this subtree, saturated so the walk always moves forward
+            const uint32_t after = (next >= node_count - idx) ? node_count : idx + next + 1;
+            if (nd.mass <= 0.0f) {  // empty node: skip the subtree (physics.cpp:107-110)
+                idx = after;
+                continue;
+            }
+            const bool active = idx >= skip_until && idx < end;
+            float dx, dy, dist_sq;
+            bool accept;
+            if (ORDER > 0) {
+                dx = h0.x - p.x;
+                dy = h0.y - p.y;
+                if (per) {
+                    dx = wrap_d(dx, dom.w, hw);
+                    dy = wrap_d(dy, dom.h, hh);
+                }
+                const float d2 = dx * dx + dy * dy;
+                dist_sq = d2 + soft_sq;
+                accept = d2 > h0.z;  // h0.z holds the squared opening radius
+            } else {
+#pragma clang fp contract(off)
+                dx = nd.comx - p.x;
+                dy = nd.comy - p.y;
+                if (per) {  // scalar variant (physics.cpp:117-120)
+                    dx = wrap_d(dx, dom.w, hw);
+                    dy = wrap_d(dy, dom.h, hh);
+                }
+                dist_sq = dx * dx + dy * dy + soft_sq;
+                accept = nd.size_sq < theta_sq * dist_sq;
+            }
+            const bool leaf = next == 0;
+            // ORDER > 0: a leaf that any active lane opens is summed directly by
+            // every active lane (exact, and the wave runs that loop anyway); those
+            // lanes skip the leaf's expansion.  ORDER 0 keeps the reference's
+            // per-lane choice.
+            bool direct = active && !accept && leaf;
+            if (ORDER > 0 && leaf && __ballot(direct)) direct = active;
+            if (active && accept && !direct) {
+                const float inv = __builtin_amdgcn_rsqf(dist_sq);
+                float tx, ty;
+                if (ORDER > 0) {
+                    const float qx = -dx, qy = -dy;  // d = target - COM
+                    const float r2 = inv * inv;
+                    const float gr3 = G * (inv * r2);
+                    const float p2x = h1.x * qx + h1.y * qy, p2y = h1.y * qx + h1.z * qy;
+                    const float s2 = qx * p2x + qy * p2y;
+                    if (ORDER >= 4) {
+                        const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
+                        const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
+                        const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
+                        const float pux = h5.y * qx + h5.z * qy, puy = h5.z * qx + h5.w * qy;
+                        const float s3u = qx * (p3x + pux) + qy * (p3y + puy);
+                        const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y) + h3.z;
+                        const float c0 = qxx * qx, c3 = qyy * qy, c1 = (3.0f * qxx) * qy, c2 = (3.0f * qyy) * qx;
+                        const float p4x = h4.x * c0 + h4.y * c1 + h4.z * c2 + h4.w * c3;
+                        const float p4y = h4.y * c0 + h4.z * c1 + h4.w * c2 + h5.x * c3;
+                        const float s4 = qx * p4x + qy * p4y;
+                        const float sc = gr3 * (r2 * (r2 * (r2 * (r2 * s4 + s3u) + c7) + h1.w) - h0.w);
+                        const float e = (28.0f / 9.0f) * r2, f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
+                        const float ox = (2.0f * p2x + h3.x) + f * ((3.0f * p3x + 2.0f * pux) + e * p4x);
+                        const float oy = (2.0f * p2y + h3.y) + f * ((3.0f * p3y + 2.0f * puy) + e * p4y);
+                        tx = qx * sc + g5 * ox;
+                        ty = qy * sc + g5 * oy;
+                    } else if (ORDER == 3) {
+                        // the same series without the fourth-order moments (A4, U, W)
+                        const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
+                        const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
+                        const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
+                        const float s3 = qx * p3x + qy * p3y;
+                        const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y);
+                        const float sc = gr3 * (r2 * (r2 * (r2 * s3 + c7) + h1.w) - h0.w);
+                        const float f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
+                        const float ox = (2.0f * p2x + h3.x) + f * (3.0f * p3x);
+                        const float oy = (2.0f * p2y + h3.y) + f * (3.0f * p3y);
+                        tx = qx * sc + g5 * ox;
+                        ty = qy * sc + g5 * oy;
+                    } else {
+                        const float sc = gr3 * (r2 * (r2 * s2 + h1.w) - h0.w);
+                        const float g5 = gr3 * (-0.4f * r2);
+                        tx = qx * sc + g5 * p2x;
+                        ty = qy * sc + g5 * p2y;
+                    }
+                } else {
+                    const float inv2 = inv * inv;
+                    const float inv3 = inv2 * inv;
+                    const float f = G * nd.mass * inv3;
+                    tx = dx * f;
+                    ty = dy * f;
+                }
+                if (KAHAN) {
+                    kahan_add(fx, cx, tx);
+                    kahan_add(fy, cy, ty);
+                } else {
+                    fx += tx;
+                    fy += ty;
+                }
+                if (COUNT) {
+                    ++cnt;
+                    ++cnt_nodes;
+                }
+                if (TEMP) {  // heat transfer toward the node's average (physics.cpp:261-276)
+#pragma clang fp contract(off)
+                    const uint32_t cn = nd.end - nd.start;
+                    if (cn > 0u) {
+                        const float avg = gtavg[idx];  // gridTemp / cn, divided once per node
+                        const float diff = avg - ti;
+                        const float invd = 1.0f / sqrtf(dist_sq);
+                        float dist = 0.0f;
+                        if (dist_sq > 1e-16f) dist = 1.0f / invd;
+                        if (dist > 1e-8f) {
+                            const float heat = heat_k * diff / dist;
+                            ti = ti + heat * heat_dt;
+                        }
+                    }
+                }
+                skip_until = after;
+            }
+            const bool open_leaf = direct;
+            if (ORDER > 0 && __ballot(open_leaf)) {
+                // leaf direct sum: the pairs go to a plain per-leaf sum of
+                // m_j d_j r^-3, scaled by G and compensated once per leaf
+                const uint32_t jend = min(nd.end, n);
+                if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
+                float sx = 0.f, sy = 0.f;
+                for (uint32_t j = nd.start; j < jend; ++j) {
+                    const float4 q = pxym[j];
+                    if (open_leaf && j != i) {
+                        float px = q.x - p.x, py = q.y - p.y;
+                        if (per) {
+                            px = wrap_d(px, dom.w, hw);
+                            py = wrap_d(py, dom.h, hh);
+                        }
+                        const float d2 = px * px + py * py + soft_sq;
+                        const float inv = __builtin_amdgcn_rsqf(d2);
+                        const float f = q.z * (inv * inv * inv);
+                        sx = fmaf(px, f, sx);
+                        sy = fmaf(py, f, sy);
+                        if (COUNT) ++cnt;
+                    }
+                }
+                if (open_leaf) {
+                    if (KAHAN) {
+                        kahan_add(fx, cx, G * sx);
+                        kahan_add(fy, cy, G * sy);
+                    } else {
+                        fx = fmaf(G, sx, fx);
+                        fy = fmaf(G, sy, fy);
+                    }
+                }
+            } else if (ORDER == 0 && __ballot(open_leaf)) {
+                // leaf direct sum over [start, end), skipping i == j (physics.cpp:163-196)
+                const uint32_t jend = min(nd.end, n);
+                if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
+                for (uint32_t j = nd.start; j < jend; ++j) {
+                    const float4 q = pxym[j];
+                    if (open_leaf && j != i) {
+                        float px = q.x - p.x, py = q.y - p.y;
+                        if (per) {  // physics.cpp:169-172
+                            px = wrap_d(px, dom.w, hw);
+                            py = wrap_d(py, dom.h, hh);
+                        }
+                        const float d2 = px * px + py * py + soft_sq;
+                        const float inv = __builtin_amdgcn_rsqf(d2);
+                        const float f = G * q.z * (inv * inv * inv);
+                        if (KAHAN) {
+                            kahan_add(fx, cx, px * f);
+                            kahan_add(fy, cy, py * f);
+                        } else {
+                            fx = fmaf(px, f, fx);
+                            fy = fmaf(py, f, fy);
+                        }
+                        if (COUNT) ++cnt;
+                    }
+                }
+            }
+            const bool descend = active && !accept && !leaf;
+            idx = __ballot(descend) ? idx + 1u : after;
+        }
+        if (COUNT) {
+            uint64_t c = cnt, cn = cnt_nodes;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                c += __shfl_xor(c, o, kWave);
+                cn += __shfl_xor(cn, o, kWave);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                atomicAdd(inter, (unsigned long long)c);
+                atomicAdd(inter + 1, (unsigned long long)cn);
+                atomicAdd(inter + 2, (unsigned long long)visits);
+                atomicAdd(inter + 3, (unsigned long long)leaf_trips);
+            }
+        }
+        if (live) {
+            if (TEMP) temp[i] = ti;
+            const float accx = KAHAN ? fx - cx : fx;
+            const float accy = KAHAN ? fy - cy : fy;
+            ax[i] = accx;
+            ay[i] = accy;
+            if (half_dt_end != 0.0f && !(flags[i] & GE_FLAG_PINNED)) {
+#pragma clang fp contract(off)
+                vx[i] = vx[i] + accx * half_dt_end;
+                vy[i] = vy[i] + accy * half_dt_end;
+            }
         }
     }
-    if (!live) return;
-    if (TEMP) temp[i] = ti;
-    const float accx = KAHAN ? fx - cx : fx;
-    const float accy = KAHAN ? fy - cy : fy;
-    ax[i] = accx;
-    ay[i] = accy;
-    if (half_dt_end != 0.0f && !(flags[i] & GE_FLAG_PINNED)) {
-#pragma clang fp contract(off)
-        vx[i] = vx[i] + accx * half_dt_end;
-        vy[i] = vy[i] + accy * half_dt_end;
+    if (PERSIST && (threadIdx.x & (kWave - 1)) == 0) {
+        const uint32_t waves = gridDim.x * (BS / kWave);
+        if (atomicAdd(&walk_q[kWalkQueues], 1u) == waves - 1u)
+            for (uint32_t k = 0; k <= kWalkQueues; ++k) walk_q[k] = 0u;
     }
 }
 
@@ -685,18 +728,39 @@ void launch_gather(ge_ctx *c) {
     c->b.cur = 1 - cur;
 }
 
-template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
-void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST>
+void launch_bh_k(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, uint32_t nb, uint32_t *walk_q) {
     const State &s = c->b.st[c->b.cur];
     const float theta_sq = p->theta * p->theta;
     const float soft_sq = p->softening * p->softening;
-    const uint32_t nb = blocks_for((uint64_t)c->nwaves * kWave, BS);
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
-    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER, TEMP>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
+    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER, TEMP, PERSIST>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
                        c->b.nodes, c->b.node_mp, c->b.start_count, c->b.sys_wave_off, c->nsys, c->b.sys_off,
                        c->b.counters, c->n,
                        p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p),
-                       inter, c->b.node_tavg, s.temp, c->thermal_p.heat_conductivity, p->dt);
+                       inter, c->b.node_tavg, s.temp, c->thermal_p.heat_conductivity, p->dt, walk_q);
+}
+
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
+void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
+    const uint32_t nb = blocks_for((uint64_t)c->nwaves * kWave, BS);
+    if (c->nsys <= 1) {
+        launch_bh_k<BS, KAHAN, COUNT, ORDER, TEMP, false>(c, p, end_kick_half_dt, nb, nullptr);
+        return;
+    }
+    // a batch of systems runs one persistent launch: as many workgroups as
+    // the device holds at once, each pulling waves from the ticket ranges
+    static int per_cu = 0;  // one value per kernel instance
+    if (per_cu == 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_bh<BS, KAHAN, COUNT, ORDER, TEMP, true>, BS, 0) !=
+                hipSuccess || v <= 0)
+            v = 1;
+        per_cu = v;
+    }
+    const uint64_t resident = (uint64_t)per_cu * c->cus;
+    launch_bh_k<BS, KAHAN, COUNT, ORDER, TEMP, true>(c, p, end_kick_half_dt, resident < nb ? (uint32_t)resident : nb,
+                                                     c->b.counters + kCntWalkQ0);
 }
 
 template <int BS, int ORDER, bool TEMP>
