@@ -50,6 +50,8 @@ struct Search {
 struct SphConst {
     float h, h2, cell, unit, visc, coh, delta, stiff_mult, coeff;
     float vol, spiky, lap;  // kernel constants (SPH.h:38-56)
+    float h2hi;             // a bound with sqrt(r2) < h  =>  r2 < h2hi (cheap pre-test)
+    float inv_h, inv_vol, coh_norm;  // 1/h, 1/vol, 30/(pi h^2)
     float half_dt;          // fused integrateEnd (0 = none)
     Search S;
 };
@@ -103,12 +105,19 @@ __device__ __forceinline__ bool row_range(const int32_t *fr, const uint32_t *fir
     return lo < hi;
 }
 
+// The pair passes divide with the hardware reciprocal (v_rcp_f32, within
+// 1 ulp) where the reference divides: the pair terms then differ from the
+// reference's by a few ulp, far inside the passes' tolerance, at a fraction
+// of the issue cost of a correctly rounded division.  Tests that decide
+// which pairs interact keep the reference's exact arithmetic.
+__device__ __forceinline__ float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+
 // SPH::smoothingKernelCohesion (SPH.h:58-64)
-__device__ __forceinline__ float coh_kernel(float r, float h) {
+__device__ __forceinline__ float coh_kernel(float r, const SphConst &K) {
 #pragma clang fp contract(off)
-    if (r >= h) return 0.0f;
-    const float q = r / h;
-    return (1.0f - q) * (0.5f - q) * (0.5f - q) * 30.0f / (kPiF * h * h);
+    if (r >= K.h) return 0.0f;
+    const float q = r * K.inv_h;
+    return (1.0f - q) * (0.5f - q) * (0.5f - q) * K.coh_norm;
 }
 
 // SPH::spikyKernelDerivative (SPH.h:45-50): float scale times pow(h - r, 2.0f),
@@ -274,6 +283,8 @@ __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A
     const bool fwd = !(fi & GE_FLAG_PINNED);
     const float mI = oq.x * K.unit;
     const float di = oq.w < 0.001f ? 0.001f : oq.w;
+    const float aI = K.visc * oq.y * mI * rcp(di) * K.lap;  // j's visit of i: viscosity factor
+    const float cI = K.coh * oq.z;
     const int cx = cell_coord(own.x, K.S), cy = cell_coord(own.y, K.S);
     float sx = 0.0f, sy = 0.0f;
     for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
@@ -288,27 +299,29 @@ __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A
             if (!(rsq < K.h2) || kk == k) continue;
             const float4 pj = A.pk0[kk];
             const uint32_t fj = A.pkf[kk];
-            const float r = sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
+            const float r = __builtin_amdgcn_sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
+            const float inv_r = rcp(r);
             const float4 qj = A.pk1[kk];
-            const float wc = coh_kernel(r, K.h);
+            const float wc = coh_kernel(r, K);
             if (fwd) {  // i's visit of j
                 const float mJ = qj.x * K.unit;
                 const float dj = qj.w < 0.001f ? 0.001f : qj.w;
-                const float vfx = K.visc * qj.y * mJ / dj * K.lap * (pj.z - own.z);
-                const float vfy = K.visc * qj.y * mJ / dj * K.lap * (pj.w - own.w);
-                const float cc = K.coh * oq.z;
-                const float cfx = cc * mJ * wc * (dx / r);
-                const float cfy = cc * mJ * wc * (dy / r);
+                const float a = K.visc * qj.y * mJ * rcp(dj) * K.lap;
+                const float vfx = a * (pj.z - own.z);
+                const float vfy = a * (pj.w - own.w);
+                const float cw = cI * mJ * wc;
+                const float cfx = cw * (dx * inv_r);
+                const float cfy = cw * (dy * inv_r);
                 sx = sx + (vfx + cfx);
                 sy = sy + (vfy + cfy);
             }
             if (!(fj & GE_FLAG_PINNED)) {  // j's visit of i, scattered back to i
                 const float ex = own.x - pj.x, ey = own.y - pj.y;
-                const float vfx = K.visc * oq.y * mI / di * K.lap * (own.z - pj.z);
-                const float vfy = K.visc * oq.y * mI / di * K.lap * (own.w - pj.w);
-                const float cc = K.coh * qj.z;
-                const float cfx = cc * mI * wc * (ex / r);
-                const float cfy = cc * mI * wc * (ey / r);
+                const float vfx = aI * (own.z - pj.z);
+                const float vfy = aI * (own.w - pj.w);
+                const float cw = K.coh * qj.z * mI * wc;
+                const float cfx = cw * (ex * inv_r);
+                const float cfy = cw * (ey * inv_r);
                 sx = sx - (vfx + cfx);
                 sy = sy - (vfy + cfy);
             }
@@ -360,11 +373,11 @@ __global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridVie
                 if (!within(K.S, gj.x - me.x, gj.y - me.y)) continue;
             }
             const float4 pj = pp[kk];
-            const float rr = sqrtf(rrsq);
+            const float rr = __builtin_amdgcn_sqrtf(rrsq);
             const float mJ = pj.w * K.unit;
             const float rho0 = 0.5f * (me.z + pj.z);
-            const float w = (K.h - rr) * (K.h - rr) / K.vol;
-            dens = dens + mJ * w / rho0;
+            const float w = (K.h - rr) * (K.h - rr) * K.inv_vol;
+            dens = dens + mJ * w * rcp(rho0);
         }
     }
     dens = group_sum(dens);
@@ -396,13 +409,14 @@ __device__ __forceinline__ void pressure_pair(float dx, float dy, float rr, floa
                                               float di, float dj, const SphConst &K, float &fx, float &fy) {
 #pragma clang fp contract(off)
     const float gw = spiky_grad(rr, K.h, K.spiky);
-    const float nx = dx / rr, ny = dy / rr;
+    const float inv_rr = rcp(rr);
+    const float nx = dx * inv_rr, ny = dy * inv_rr;
     const float avgP = 0.5f * (pi_ + pj_);
     const float avgD = 0.5f * (di + dj);
-    float mag = -(mi * K.unit + mj * K.unit) * avgP / (avgD < 0.01f ? 0.01f : avgD);
+    float mag = -(mi * K.unit + mj * K.unit) * avgP * rcp(avgD < 0.01f ? 0.01f : avgD);
     const float smax = mi < mj ? mj : mi;
     const float smin = mj < mi ? mj : mi;
-    const float sc8 = 8.0f / (smax / smin);
+    const float sc8 = 8.0f * smin * rcp(smax);  // 8 / (smax / smin)
     mag = mag * (sc8 < 1.0f ? sc8 : 1.0f);
     fx = mag * gw * nx;
     fy = mag * gw * ny;
@@ -432,7 +446,9 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
             for (uint32_t kk = lo + q0; kk < hi; kk += kSphLanes) {
                 const float2 qp = ppxy[kk];  // NaN for non-visitors: fails the test
                 const float dx = me.x - qp.x, dy = me.y - qp.y;
-                const float rr = sqrtf(dx * dx + dy * dy);
+                const float rsq = dx * dx + dy * dy;
+                if (!(rsq < K.h2hi)) continue;  // certainly outside h; the exact test follows
+                const float rr = sqrtf(rsq);
                 if (!(rr >= 1e-5f && rr < K.h) || kk == k) continue;
                 if (K.S.mode) {
                     const float2 gj = A.xy[kk];
@@ -455,7 +471,9 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
             for (uint32_t kb = lo + q0; kb < hi; kb += kSphLanes) {
                 const float2 qp = B.xy[kb];
                 const float dx = qp.x - me.x, dy = qp.y - me.y;
-                const float rr = sqrtf(dx * dx + dy * dy);
+                const float rsq = dx * dx + dy * dy;
+                if (!(rsq < K.h2hi)) continue;
+                const float rr = sqrtf(rsq);
                 if (!(rr >= 1e-5f && rr < K.h)) continue;
                 // j's query at predPos_j meets i at pos_i
                 if (!within(K.S, mypos.x - qp.x, mypos.y - qp.y)) continue;
@@ -722,6 +740,12 @@ SphConst make_const(const ge_step_params *p, float half_dt) {
     K.spiky = -45.0f / (kPiF * std::pow(K.h, 6.0f));
     K.lap = 45.0f / (kPiF * std::pow(K.h, 6.0f));
     K.half_dt = half_dt;
+    // sqrt is correctly rounded and h a float, so sqrt(r2) < h implies r2 < h^2
+    // exactly; the float bound sits a little above h^2
+    K.h2hi = std::nextafter((float)((double)K.h * K.h * (1.0 + 1e-6)), INFINITY);
+    K.inv_h = 1.0f / K.h;
+    K.inv_vol = 1.0f / K.vol;
+    K.coh_norm = 30.0f / (kPiF * K.h * K.h);
     K.S = make_search(p->sph_search, p->sph_cell, p->sph_search_radius);
     return K;
 }
