@@ -105,6 +105,75 @@ __device__ __forceinline__ bool row_range(const int32_t *fr, const uint32_t *fir
     return lo < hi;
 }
 
+// particles of cells (c0..c1, row) as one range of cell-order positions
+__device__ __forceinline__ bool row_span(const int32_t *fr, const uint32_t *first, int c0, int c1, int row,
+                                         uint32_t &lo, uint32_t &hi) {
+    if (!fr[4]) return false;
+    const int64_t r = (int64_t)row - fr[1];
+    if (r < 0 || r >= fr[3]) return false;
+    int64_t a = (int64_t)c0 - fr[0], b = (int64_t)c1 - fr[0];
+    if (b < 0 || a >= fr[2]) return false;
+    if (a < 0) a = 0;
+    if (b >= fr[2]) b = fr[2] - 1;
+    const uint64_t base = (uint64_t)r * (uint64_t)fr[2];
+    lo = first[base + (uint64_t)a];
+    hi = first[base + (uint64_t)b + 1];
+    return lo < hi;
+}
+
+// ---- LDS tiles of candidate rows.  A workgroup's particles are consecutive
+// in cell order, so when the query cells of its visitors share one row y,
+// the candidates of all of them lie in three contiguous ranges of cell-order
+// positions: cells c_min - 1 .. c_max + 1 of rows y - 1, y, y + 1.  Those
+// ranges are staged in LDS once per workgroup and every lane reads its
+// candidates there.  A workgroup whose queries span rows, or whose rows
+// exceed the tile, reads global memory instead; both paths visit the same
+// candidates in the same order.  (NeighborSearchV2's wider search, mode 1,
+// always reads global memory.)
+constexpr uint32_t kTileCand = 384;
+
+struct TileSpan {  // one per workgroup, in LDS
+    int cmin, cmax, rmin, rmax;  // query cells of the workgroup's visitors
+    uint32_t lo[3];              // first grid position of each staged row
+    uint32_t base[4];            // first tile slot of each staged row; base[3] = staged count
+    uint32_t ok;
+};
+
+__device__ __forceinline__ void span_init(TileSpan &T) {
+    T.cmin = 0x7FFFFFFF;
+    T.cmax = -0x7FFFFFFF - 1;
+    T.rmin = 0x7FFFFFFF;
+    T.rmax = -0x7FFFFFFF - 1;
+}
+
+__device__ __forceinline__ void span_add(TileSpan &T, int cx, int cy) {
+    atomicMin(&T.cmin, cx);
+    atomicMax(&T.cmax, cx);
+    atomicMin(&T.rmin, cy);
+    atomicMax(&T.rmax, cy);
+}
+
+// one thread: the staged rows of the span (no visitor: nothing staged)
+__device__ __forceinline__ void span_plan(TileSpan &T, const int32_t *fr, const uint32_t *first) {
+    const bool one_row = T.rmin == T.rmax;
+    uint32_t tot = 0;
+    for (int d = 0; d < 3; ++d) {
+        uint32_t lo = 0, hi = 0;
+        if (!one_row || !row_span(fr, first, T.cmin - 1, T.cmax + 1, T.rmin + d - 1, lo, hi)) lo = hi = 0;
+        T.lo[d] = lo;
+        T.base[d] = tot;
+        tot += hi - lo;
+    }
+    T.base[3] = tot;
+    T.ok = one_row && tot <= kTileCand;
+}
+
+// grid position of tile slot e (e < base[3])
+__device__ __forceinline__ uint32_t span_pos(const TileSpan &T, uint32_t e) {
+    const uint32_t d = e >= T.base[2] ? 2u : (e >= T.base[1] ? 1u : 0u);
+    return T.lo[d] + (e - T.base[d]);
+}
+
 // The pair passes divide with the hardware reciprocal (v_rcp_f32, within
 // 1 ulp) where the reference divides: the pair terms then differ from the
 // reference's by a few ulp, far inside the passes' tolerance, at a fraction
@@ -260,19 +329,47 @@ __device__ __forceinline__ float group_sum(float v) {
 }
 
 // SPH::computeViscCohesionForces (SPH.cpp:98-150) in gather form, then the
-// predicted position (SPH.cpp:173-181).  pp[k] = (predX, predY, restDens, sphMass).
+// predicted position (SPH.cpp:173-181).  pp[k] = (predX, predY, restDens,
+// sphMass).  Candidates come from an LDS tile when the workgroup's query
+// cells share a row.
 __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A, SphConst K,
                                                  float2 *__restrict__ sphf, float4 *__restrict__ pp,
                                                  float2 *__restrict__ ppxy) {
 #pragma clang fp contract(off)
+    __shared__ TileSpan TA;
+    __shared__ float2 t_xy[kTileCand], t_v[kTileCand];
+    __shared__ float4 t_q[kTileCand];
+    __shared__ uint32_t t_f[kTileCand];
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t k = t / kSphLanes, q0 = t % kSphLanes;
-    if (k >= n) return;
-    const uint32_t i = A.ord[k];
-    const uint32_t fi = A.pkf[k];
-    const float4 own = A.pk0[k];
-    const float4 oq = A.pk1[k];
-    if (!is_visitor(fi)) {
+    const bool have = k < n;
+    const uint32_t i = have ? A.ord[k] : 0u;
+    const uint32_t fi = have ? A.pkf[k] : 0u;
+    const bool vis = have && is_visitor(fi);
+    const float4 own = have ? A.pk0[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 oq = have ? A.pk1[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int cx = vis ? cell_coord(own.x, K.S) : 0, cy = vis ? cell_coord(own.y, K.S) : 0;
+    const bool tiles = K.S.mode == 0;  // uniform over the launch
+    if (tiles) {
+        if (threadIdx.x == 0) span_init(TA);
+        __syncthreads();
+        if (vis && q0 == 0) span_add(TA, cx, cy);
+        __syncthreads();
+        if (threadIdx.x == 0) span_plan(TA, A.frame, A.first);
+        __syncthreads();
+        if (TA.ok)
+            for (uint32_t e = threadIdx.x; e < TA.base[3]; e += kBlock) {
+                const uint32_t kk = span_pos(TA, e);
+                const float4 p0 = A.pk0[kk];
+                t_xy[e] = A.xy[kk];
+                t_v[e] = make_float2(p0.z, p0.w);
+                t_q[e] = A.pk1[kk];
+                t_f[e] = A.pkf[kk];
+            }
+        __syncthreads();
+    }
+    if (!have) return;
+    if (!vis) {
         if (q0 == 0) {
             sphf[i] = make_float2(0.0f, 0.0f);
             pp[k] = make_float4(s.ppx[i], s.ppy[i], s.rest_dens[i], oq.x);
@@ -280,35 +377,46 @@ __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A
         }
         return;
     }
+    const bool ta = tiles && TA.ok;
     const bool fwd = !(fi & GE_FLAG_PINNED);
     const float mI = oq.x * K.unit;
     const float di = oq.w < 0.001f ? 0.001f : oq.w;
     const float aI = K.visc * oq.y * mI * rcp(di) * K.lap;  // j's visit of i: viscosity factor
     const float cI = K.coh * oq.z;
-    const int cx = cell_coord(own.x, K.S), cy = cell_coord(own.y, K.S);
     float sx = 0.0f, sy = 0.0f;
     for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
         uint32_t lo, hi;
         if (!row_range(A.frame, A.first, cx, cy + dr, K.S.R, lo, hi)) continue;
+        const uint32_t off = ta ? TA.base[dr + 1] - TA.lo[dr + 1] : 0u;  // tile slot = kk + off
         for (uint32_t kk = lo + q0; kk < hi; kk += kSphLanes) {
             // non-visitors carry NaN and fail both tests
-            const float2 q = A.xy[kk];
+            const float2 q = ta ? t_xy[kk + off] : A.xy[kk];
             const float dx = q.x - own.x, dy = q.y - own.y;
             if (!within(K.S, dx, dy)) continue;
             const float rsq = dx * dx + dy * dy;
             if (!(rsq < K.h2) || kk == k) continue;
-            const float4 pj = A.pk0[kk];
-            const uint32_t fj = A.pkf[kk];
+            float2 vj;
+            float4 qj;
+            uint32_t fj;
+            if (ta) {
+                vj = t_v[kk + off];
+                qj = t_q[kk + off];
+                fj = t_f[kk + off];
+            } else {
+                const float4 pj = A.pk0[kk];
+                vj = make_float2(pj.z, pj.w);
+                qj = A.pk1[kk];
+                fj = A.pkf[kk];
+            }
             const float r = __builtin_amdgcn_sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
             const float inv_r = rcp(r);
-            const float4 qj = A.pk1[kk];
             const float wc = coh_kernel(r, K);
             if (fwd) {  // i's visit of j
                 const float mJ = qj.x * K.unit;
                 const float dj = qj.w < 0.001f ? 0.001f : qj.w;
                 const float a = K.visc * qj.y * mJ * rcp(dj) * K.lap;
-                const float vfx = a * (pj.z - own.z);
-                const float vfy = a * (pj.w - own.w);
+                const float vfx = a * (vj.x - own.z);
+                const float vfy = a * (vj.y - own.w);
                 const float cw = cI * mJ * wc;
                 const float cfx = cw * (dx * inv_r);
                 const float cfy = cw * (dy * inv_r);
@@ -316,9 +424,9 @@ __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A
                 sy = sy + (vfy + cfy);
             }
             if (!(fj & GE_FLAG_PINNED)) {  // j's visit of i, scattered back to i
-                const float ex = own.x - pj.x, ey = own.y - pj.y;
-                const float vfx = aI * (own.z - pj.z);
-                const float vfy = aI * (own.w - pj.w);
+                const float ex = -dx, ey = -dy;  // pos_i - pos_j
+                const float vfx = aI * (own.z - vj.x);
+                const float vfy = aI * (own.w - vj.y);
                 const float cw = K.coh * qj.z * mI * wc;
                 const float cfx = cw * (ex * inv_r);
                 const float cfy = cw * (ey * inv_r);
@@ -339,32 +447,56 @@ __global__ __launch_bounds__(kBlock) void k_visc(State s, uint32_t n, GridView A
     ppxy[k] = make_float2(px, py);
 }
 
-// predicted density and pressure (SPH.cpp:183-219); pd[k] = (press, predDens)
+// predicted density and pressure (SPH.cpp:183-219); pd[k] = (press,
+// predDens).  Candidates from an LDS tile as in k_visc (queries at predPos).
 __global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridView A, const float4 *__restrict__ pp,
                                                     const float2 *__restrict__ ppxy, SphConst K,
                                                     float2 *__restrict__ pd) {
 #pragma clang fp contract(off)
+    __shared__ TileSpan TA;
+    __shared__ float2 t_xy[kTileCand], t_rm[kTileCand];
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t k = t / kSphLanes, q0 = t % kSphLanes;
-    if (k >= n) return;
-    const uint32_t i = A.ord[k];
-    const uint32_t fi = A.pkf[k];
-    if (!is_visitor(fi)) {
+    const bool have = k < n;
+    const uint32_t i = have ? A.ord[k] : 0u;
+    const uint32_t fi = have ? A.pkf[k] : 0u;
+    const bool vis = have && is_visitor(fi);
+    const float4 me = vis ? pp[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int cx = vis ? cell_coord(me.x, K.S) : 0, cy = vis ? cell_coord(me.y, K.S) : 0;
+    const bool tiles = K.S.mode == 0;  // uniform over the launch
+    if (tiles) {
+        if (threadIdx.x == 0) span_init(TA);
+        __syncthreads();
+        if (vis && q0 == 0) span_add(TA, cx, cy);
+        __syncthreads();
+        if (threadIdx.x == 0) span_plan(TA, A.frame, A.first);
+        __syncthreads();
+        if (TA.ok)
+            for (uint32_t e = threadIdx.x; e < TA.base[3]; e += kBlock) {
+                const uint32_t kk = span_pos(TA, e);
+                const float4 pj = pp[kk];
+                t_xy[e] = ppxy[kk];
+                t_rm[e] = make_float2(pj.z, pj.w);
+            }
+        __syncthreads();
+    }
+    if (!have) return;
+    if (!vis) {
         if (q0 == 0) {
             s.press[i] = 0.0f;  // press is cleared for every particle (SPH.cpp:162-165)
             pd[k] = make_float2(0.0f, s.pred_dens[i]);
         }
         return;
     }
-    const float4 me = pp[k];
-    const int cx = cell_coord(me.x, K.S), cy = cell_coord(me.y, K.S);
+    const bool ta = tiles && TA.ok;
     float dens = 0.0f;
     for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
         uint32_t lo, hi;
         if (!row_range(A.frame, A.first, cx, cy + dr, K.S.R, lo, hi)) continue;
+        const uint32_t off = ta ? TA.base[dr + 1] - TA.lo[dr + 1] : 0u;
         for (uint32_t kk = lo + q0; kk < hi; kk += kSphLanes) {
             // predicted positions, NaN for non-visitors (they fail the test)
-            const float2 q = ppxy[kk];
+            const float2 q = ta ? t_xy[kk + off] : ppxy[kk];
             const float dx = me.x - q.x, dy = me.y - q.y;
             const float rrsq = dx * dx + dy * dy;
             if (!(rrsq < K.h * K.h)) continue;
@@ -372,10 +504,16 @@ __global__ __launch_bounds__(kBlock) void k_density(State s, uint32_t n, GridVie
                 const float2 gj = A.xy[kk];
                 if (!within(K.S, gj.x - me.x, gj.y - me.y)) continue;
             }
-            const float4 pj = pp[kk];
+            float2 rm;  // (restDens, sphMass) of the candidate
+            if (ta) {
+                rm = t_rm[kk + off];
+            } else {
+                const float4 pj = pp[kk];
+                rm = make_float2(pj.z, pj.w);
+            }
             const float rr = __builtin_amdgcn_sqrtf(rrsq);
-            const float mJ = pj.w * K.unit;
-            const float rho0 = 0.5f * (me.z + pj.z);
+            const float mJ = rm.y * K.unit;
+            const float rho0 = 0.5f * (me.z + rm.x);
             const float w = (K.h - rr) * (K.h - rr) * K.inv_vol;
             dens = dens + mJ * w * rcp(rho0);
         }
@@ -423,28 +561,82 @@ __device__ __forceinline__ void pressure_pair(float dx, float dy, float rr, floa
 }
 
 // pressure forces (SPH.cpp:221-264), pressF and acc (SPH.cpp:271-280), then
-// the integrateEnd kick (physics.cpp:1114-1125) when the frame asks for it
+// the integrateEnd kick (physics.cpp:1114-1125) when the frame asks for it.
+// Grid-A candidates (queries at predPos) and grid-B candidates (queries at
+// pos) each come from an LDS tile when the workgroup's queries allow.
 __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridView A, const float4 *__restrict__ pp,
                                                      const float2 *__restrict__ ppxy, const float2 *__restrict__ pd,
                                                      GridView B, SphConst K, const float2 *__restrict__ sphf) {
 #pragma clang fp contract(off)
+    __shared__ TileSpan TA, TB;
+    __shared__ float2 a_xy[kTileCand], a_pd[kTileCand];
+    __shared__ float a_m[kTileCand];
+    __shared__ float2 b_xy[kTileCand], b_pq[kTileCand];
+    __shared__ float b_m[kTileCand];
+    __shared__ uint32_t b_ord[kTileCand];
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t k = t / kSphLanes, q0 = t % kSphLanes;
-    if (k >= n) return;
-    const uint32_t i = A.ord[k];
-    const uint32_t fi = A.pkf[k];
+    const bool have = k < n;
+    const uint32_t i = have ? A.ord[k] : 0u;
+    const uint32_t fi = have ? A.pkf[k] : 0u;
+    const bool vis = have && is_visitor(fi);
+    float4 me = make_float4(0.f, 0.f, 0.f, 0.f), mypos = me;
+    float2 mp = make_float2(0.f, 0.f);
+    int qx = 0, qy = 0, px = 0, py = 0;
+    if (vis) {
+        me = pp[k];
+        mp = pd[k];
+        mypos = A.pk0[k];
+        qx = cell_coord(me.x, K.S);
+        qy = cell_coord(me.y, K.S);
+        px = cell_coord(mypos.x, K.S);
+        py = cell_coord(mypos.y, K.S);
+    }
+    const bool tiles = K.S.mode == 0;  // uniform over the launch
+    if (tiles) {
+        if (threadIdx.x == 0) {
+            span_init(TA);
+            span_init(TB);
+        }
+        __syncthreads();
+        if (vis && q0 == 0) {
+            span_add(TA, qx, qy);
+            span_add(TB, px, py);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) span_plan(TA, A.frame, A.first);
+        if (threadIdx.x == 64) span_plan(TB, B.frame, B.first);
+        __syncthreads();
+        if (TA.ok)
+            for (uint32_t e = threadIdx.x; e < TA.base[3]; e += kBlock) {
+                const uint32_t kk = span_pos(TA, e);
+                a_xy[e] = ppxy[kk];
+                a_m[e] = pp[kk].w;
+                a_pd[e] = pd[kk];
+            }
+        if (TB.ok)
+            for (uint32_t e = threadIdx.x; e < TB.base[3]; e += kBlock) {
+                const uint32_t kb = span_pos(TB, e);
+                const float4 q = B.pk0[kb];
+                b_xy[e] = make_float2(q.x, q.y);
+                b_pq[e] = make_float2(q.z, q.w);
+                b_m[e] = B.pk1[kb].x;
+                b_ord[e] = B.ord[kb];
+            }
+        __syncthreads();
+    }
+    if (!have) return;
     float2 f = sphf[i];
     float gx = 0.0f, gy = 0.0f;  // this lane's share of the pair forces
-    if (is_visitor(fi)) {
-        const float4 me = pp[k];
-        const float2 mp = pd[k];
+    if (vis) {
+        const bool ta = tiles && TA.ok, tb = tiles && TB.ok;
         // i's visits: candidates by pos cell around cell(predPos_i)
-        const int qx = cell_coord(me.x, K.S), qy = cell_coord(me.y, K.S);
         for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
             uint32_t lo, hi;
             if (!row_range(A.frame, A.first, qx, qy + dr, K.S.R, lo, hi)) continue;
+            const uint32_t off = ta ? TA.base[dr + 1] - TA.lo[dr + 1] : 0u;  // tile slot = kk + off
             for (uint32_t kk = lo + q0; kk < hi; kk += kSphLanes) {
-                const float2 qp = ppxy[kk];  // NaN for non-visitors: fails the test
+                const float2 qp = ta ? a_xy[kk + off] : ppxy[kk];  // NaN for non-visitors: fails the test
                 const float dx = me.x - qp.x, dy = me.y - qp.y;
                 const float rsq = dx * dx + dy * dy;
                 if (!(rsq < K.h2hi)) continue;  // certainly outside h; the exact test follows
@@ -454,22 +646,21 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
                     const float2 gj = A.xy[kk];
                     if (!within(K.S, gj.x - me.x, gj.y - me.y)) continue;
                 }
-                const float4 pj = pp[kk];
-                const float2 q = pd[kk];
+                const float mj = ta ? a_m[kk + off] : pp[kk].w;
+                const float2 q = ta ? a_pd[kk + off] : pd[kk];
                 float fx, fy;
-                pressure_pair(dx, dy, rr, me.w, pj.w, mp.x, q.x, mp.y, q.y, K, fx, fy);
+                pressure_pair(dx, dy, rr, me.w, mj, mp.x, q.x, mp.y, q.y, K, fx, fy);
                 gx = gx + fx;
                 gy = gy + fy;
             }
         }
         // visits of i by others: predPos cells around cell(pos_i)
-        const float4 mypos = A.pk0[k];
-        const int px = cell_coord(mypos.x, K.S), py = cell_coord(mypos.y, K.S);
         for (int dr = -K.S.R; dr <= K.S.R; ++dr) {
             uint32_t lo, hi;
             if (!row_range(B.frame, B.first, px, py + dr, K.S.R, lo, hi)) continue;
+            const uint32_t off = tb ? TB.base[dr + 1] - TB.lo[dr + 1] : 0u;
             for (uint32_t kb = lo + q0; kb < hi; kb += kSphLanes) {
-                const float2 qp = B.xy[kb];
+                const float2 qp = tb ? b_xy[kb + off] : B.xy[kb];
                 const float dx = qp.x - me.x, dy = qp.y - me.y;
                 const float rsq = dx * dx + dy * dy;
                 if (!(rsq < K.h2hi)) continue;
@@ -477,11 +668,19 @@ __global__ __launch_bounds__(kBlock) void k_pressure(State s, uint32_t n, GridVi
                 if (!(rr >= 1e-5f && rr < K.h)) continue;
                 // j's query at predPos_j meets i at pos_i
                 if (!within(K.S, mypos.x - qp.x, mypos.y - qp.y)) continue;
-                if (B.ord[kb] == i) continue;
-                const float4 pj = B.pk0[kb];
-                const float mj = B.pk1[kb].x;
+                if ((tb ? b_ord[kb + off] : B.ord[kb]) == i) continue;
+                float2 pq;
+                float mj;
+                if (tb) {
+                    pq = b_pq[kb + off];
+                    mj = b_m[kb + off];
+                } else {
+                    const float4 pj = B.pk0[kb];
+                    pq = make_float2(pj.z, pj.w);
+                    mj = B.pk1[kb].x;
+                }
                 float fx, fy;
-                pressure_pair(dx, dy, rr, mj, me.w, pj.z, mp.x, pj.w, mp.y, K, fx, fy);
+                pressure_pair(dx, dy, rr, mj, me.w, pq.x, mp.x, pq.y, mp.y, K, fx, fy);
                 gx = gx - fx;
                 gy = gy - fy;
             }
