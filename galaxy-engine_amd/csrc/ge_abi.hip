@@ -58,6 +58,16 @@ int alloc_state(ge_ctx *c, State &s, uint32_t cap, bool sph) {
         GE_HIP_CHECK(c, dalloc(s.nbrs, cap));
         GE_HIP_CHECK(c, dalloc(s.prev_size, cap));
     }
+    if (c->precise) {  // zero home forms restart from the first positions they meet
+        GE_HIP_CHECK(c, dalloc(s.hx, cap));
+        GE_HIP_CHECK(c, dalloc(s.hy, cap));
+        GE_HIP_CHECK(c, dalloc(s.ox, cap));
+        GE_HIP_CHECK(c, dalloc(s.oy, cap));
+        GE_HIP_CHECK(c, hipMemset(s.hx, 0, sizeof(int32_t) * cap));
+        GE_HIP_CHECK(c, hipMemset(s.hy, 0, sizeof(int32_t) * cap));
+        GE_HIP_CHECK(c, hipMemset(s.ox, 0, sizeof(float) * cap));
+        GE_HIP_CHECK(c, hipMemset(s.oy, 0, sizeof(float) * cap));
+    }
     return GE_OK;
 }
 
@@ -70,6 +80,7 @@ void free_state(State &s) {
     dfree(s.id);
     dfree(s.color); dfree(s.pcolor); dfree(s.scolor); dfree(s.size); dfree(s.nbrs); dfree(s.prev_size);
     dfree(s.temp); dfree(s.ke); dfree(s.prev_ke); dfree(s.label);
+    dfree(s.hx); dfree(s.hy); dfree(s.ox); dfree(s.oy);
 }
 
 void free_all(ge_ctx *c) {
@@ -79,7 +90,7 @@ void free_all(ge_ctx *c) {
     Buffers &b = c->b;
     free_state(b.st[0]);
     free_state(b.st[1]);
-    dfree(b.pxym); dfree(b.sort_a); dfree(b.sort_b); dfree(b.keys); dfree(b.perm); dfree(b.hist);
+    dfree(b.pxym); dfree(b.pho); dfree(b.node_home); dfree(b.sort_a); dfree(b.sort_b); dfree(b.keys); dfree(b.perm); dfree(b.hist);
     dfree(b.partials); dfree(b.partials2); dfree(b.counters); dfree(b.interactions);
     dfree(b.sys_off); dfree(b.sys_of); dfree(b.sys_bb); dfree(b.chunk_lo); dfree(b.chunk_first); dfree(b.sys_src);
     dfree(b.sys_wave_off);
@@ -109,6 +120,7 @@ int alloc_all(ge_ctx *c) {
     if (int r = alloc_state(c, b.st[0], cap, c->sph)) return r;
     if (int r = alloc_state(c, b.st[1], cap, c->sph)) return r;
     GE_HIP_CHECK(c, dalloc(b.pxym, cap));
+    if (c->precise) GE_HIP_CHECK(c, dalloc(b.pho, cap));
     GE_HIP_CHECK(c, dalloc(b.sort_a, cap));
     GE_HIP_CHECK(c, dalloc(b.sort_b, cap));
     GE_HIP_CHECK(c, dalloc(b.keys, cap));
@@ -603,6 +615,7 @@ int ge_create(ge_ctx **out, const ge_config *cfg) {
     c->sph = cfg->enable_sph != 0;
     c->render = cfg->enable_render != 0;
     c->thermal = cfg->enable_temperature != 0;
+    c->precise = cfg->precise_positions != 0;
     c->cons_cap = cfg->max_constraints;
     ge_default_constraint_params(&c->cons_p);
     if (c->thermal && !c->sph) {  // the materials write the SPH fields

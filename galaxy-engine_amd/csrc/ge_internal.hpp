@@ -102,7 +102,65 @@ struct State {
     // temperature fields (allocated with enable_temperature)
     float *temp = nullptr, *ke = nullptr, *prev_ke = nullptr;
     uint32_t *label = nullptr;
+    // precision form (allocated with precise_positions): home cell and offset
+    int32_t *hx = nullptr, *hy = nullptr;
+    float *ox = nullptr, *oy = nullptr;
 };
+
+// ---- precision form: a position is kept as a home cell h (side kHomeCell,
+// centre h * kHomeCell) and a float offset o from that centre, |o| <= cell/2
+// after each drift.  Its float view, the nearest float to h * cell + o, is
+// what every other part of the frame reads as pos.  Separations taken as
+// (h_j - h_i) * cell + (o_j - o_i) are exact up to one rounding of the
+// result, wherever the two positions sit (|pos| < 2^20 keeps h * cell exact).
+constexpr float kHomeCell = 0.0625f;
+constexpr float kHomeInv = 16.0f;
+
+__device__ __forceinline__ float home_view(int32_t h, float o) {
+#pragma clang fp contract(off)
+    return (float)h * kHomeCell + o;  // the product is exact
+}
+
+// A position whose view is not x was written by something other than the
+// integrator (upload, constraints, walls): its home form restarts from x.
+__device__ __forceinline__ void home_sync(float x, int32_t &h, float &o) {
+#pragma clang fp contract(off)
+    if (home_view(h, o) == x) return;
+    const float c = rintf(x * kHomeInv);
+    if (!(fabsf(c) < 2147483520.0f)) {  // NaN or beyond int32 cells: x is the offset
+        h = 0;
+        o = x;
+        return;
+    }
+    h = (int32_t)c;
+    o = x - c * kHomeCell;  // exact: within half a cell of x
+}
+
+// whole cells moved from the offset into the home cell
+__device__ __forceinline__ void home_norm(int32_t &h, float &o) {
+#pragma clang fp contract(off)
+    const float c = rintf(o * kHomeInv);
+    if (c != 0.0f && fabsf(c) < 1073741824.0f) {
+        h += (int32_t)c;
+        o = o - c * kHomeCell;  // exact
+    }
+}
+
+// the position moved by a float distance d (the periodic wrap): whole cells
+// of d go to the home cell, the rest (exact, at most half a cell) to the offset
+__device__ __forceinline__ void home_shift(int32_t &h, float &o, float d) {
+#pragma clang fp contract(off)
+    const float c = rintf(d * kHomeInv);
+    h += (int32_t)c;
+    o = o + (d - c * kHomeCell);
+    home_norm(h, o);
+}
+
+// separation b - a of two home positions
+__device__ __forceinline__ float home_sep(int32_t hb, float ob, int32_t ha, float oa) {
+#pragma clang fp contract(off)
+    return (float)(hb - ha) * kHomeCell + (ob - oa);
+}
 
 // Node record read by the traversal: 32 bytes, one scalar load per visit.
 struct alignas(16) NodeRec {
@@ -147,6 +205,7 @@ struct Buffers {
     State st[2];
     int cur = 0;
     float4 *pxym = nullptr;        // sorted (x, y, m, 0) for the traversal
+    float4 *pho = nullptr;         // precision form: sorted (ox, oy, hx, hy) (cells as int bits)
     uint64_t *sort_a = nullptr, *sort_b = nullptr;  // packed sort words
     uint64_t *keys = nullptr;      // 36-bit Morton key per slot (pre-sort order)
     uint32_t *perm = nullptr;      // last permutation
@@ -174,6 +233,7 @@ struct Buffers {
     NodeRec *nodes = nullptr;
     float4 *node_geom = nullptr;   // (posx, posy, size, 0) per final node
     float4 *node_mp = nullptr;     // accuracy modes: 6 float4 multipole record per node
+    float4 *node_home = nullptr;   // precision form: expansion centre per node as (ox, oy, hx, hy)
     uint32_t *mp_cnt = nullptr;    // accuracy modes: list counters of the moment pass
     uint32_t *mp_wave_list = nullptr, *mp_big_list = nullptr, *mp_piece_owner = nullptr;
     double *mp_piece_sum = nullptr;
@@ -234,6 +294,7 @@ struct ge_ctx {
     bool sph = false;
     bool render = false;           // render fields allocated
     bool thermal = false;          // temperature fields allocated
+    bool precise = false;          // precision form (home cells) allocated
     ge_thermal_params thermal_p{};  // ge_set_thermal
     uint32_t nmat = 0;             // materials in b.materials
     uint32_t cons_cap = 0;         // max_constraints

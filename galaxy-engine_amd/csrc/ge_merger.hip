@@ -386,6 +386,9 @@ __global__ __launch_bounds__(kMBlock) void k_m_take(State src, State dst, const 
     if (src.temp) {
         dst.temp[k] = src.temp[j]; dst.label[k] = src.label[j]; dst.ke[k] = src.ke[j]; dst.prev_ke[k] = src.prev_ke[j];
     }
+    if (src.hx) {
+        dst.hx[k] = src.hx[j]; dst.hy[k] = src.hy[j]; dst.ox[k] = src.ox[j]; dst.oy[k] = src.oy[j];
+    }
 }
 
 // the largest 2 * totalRadius of the non-dark particles, as float bits

@@ -52,9 +52,23 @@ enum : int { kMpWaveCount = 0, kMpBigCount = 1, kMpPieceCount = 2, kMpSlots = 4 
 
 __host__ __device__ constexpr int mono(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
 
+// particle j as (x, y, m) in FP64: its float position, or with the precision
+// form (pho != nullptr) its home form, which FP64 holds exactly
+struct MonoPt {
+    double x, y, m;
+};
+
+__device__ __forceinline__ MonoPt mono_pt(const float4 *__restrict__ pxym, const float4 *__restrict__ pho, uint32_t j) {
+    const float4 p = pxym[j];
+    if (!pho) return MonoPt{(double)p.x, (double)p.y, (double)p.z};
+    const float4 h = pho[j];
+    return MonoPt{(double)__float_as_int(h.z) * (double)kHomeCell + (double)h.x,
+                  (double)__float_as_int(h.w) * (double)kHomeCell + (double)h.y, (double)p.z};
+}
+
 // adds the monomials of particle p about (cx, cy) to S; r2 = largest squared offset
-__device__ __forceinline__ void mono_add(const float4 p, double cx, double cy, double *__restrict__ S, double &r2) {
-    const double m = p.z, x = (double)p.x - cx, y = (double)p.y - cy;
+__device__ __forceinline__ void mono_add(const MonoPt p, double cx, double cy, double *__restrict__ S, double &r2) {
+    const double m = p.m, x = p.x - cx, y = p.y - cy;
     r2 = fmax(r2, x * x + y * y);
     double xp[5], yp[5];
     xp[0] = m;
@@ -107,23 +121,36 @@ __device__ __forceinline__ void mom_shift_add(const double *__restrict__ S, doub
         }
 }
 
-// Central moments about the exact centre of mass from S (about the float
-// centre of mass r.comx, r.comy), packed into the node's record.
+// Central moments about the expansion centre from S (about the float centre
+// of mass r.comx, r.comy), packed into the node's record.  The centre is the
+// exact centre of mass rounded to float, or, with the precision form
+// (home != nullptr), to its home form, which is stored in home[k].
 __device__ void finish_record(uint32_t k, const NodeRec &r, const float4 g, const double *__restrict__ S, double r2,
-                              float inv_theta, float4 *__restrict__ mp) {
+                              float inv_theta, float4 *__restrict__ mp, float4 *__restrict__ home) {
     const double cx0 = r.comx, cy0 = r.comy;
     const double M = S[mono(0, 0)];
     const double dx = M > 0.0 ? S[mono(1, 0)] / M : 0.0, dy = M > 0.0 ? S[mono(0, 1)] / M : 0.0;
-    const float comx = (float)(cx0 + dx), comy = (float)(cy0 + dy);
+    double ex = (double)(float)(cx0 + dx), ey = (double)(float)(cy0 + dy);
+    if (home) {
+        const double cx = cx0 + dx, cy = cy0 + dy;
+        const double hx = rint(cx * (double)kHomeInv), hy = rint(cy * (double)kHomeInv);
+        const bool fits = fabs(hx) < 2147483520.0 && fabs(hy) < 2147483520.0;
+        const float ocx = fits ? (float)(cx - hx * (double)kHomeCell) : (float)cx;
+        const float ocy = fits ? (float)(cy - hy * (double)kHomeCell) : (float)cy;
+        const int32_t ihx = fits ? (int32_t)hx : 0, ihy = fits ? (int32_t)hy : 0;
+        ex = (double)ihx * (double)kHomeCell + (double)ocx;
+        ey = (double)ihy * (double)kHomeCell + (double)ocy;
+        home[k] = make_float4(ocx, ocy, __int_as_float(ihx), __int_as_float(ihy));
+    }
+    const float comx = (float)ex, comy = (float)ey;
     double C[kMono];
 #pragma unroll
     for (int c = 0; c < kMono; ++c) C[c] = 0.0;
-    mom_shift_add(S, (double)comx - cx0, (double)comy - cy0, C);
+    mom_shift_add(S, ex - cx0, ey - cy0, C);
     const float half = 0.5f * g.z;
     const float ox = comx - (g.x + half), oy = comy - (g.y + half);
-    // largest particle offset from the exact COM, bounded through the float one
-    const double bmax = sqrt(r2) + sqrt(((double)comx - cx0) * ((double)comx - cx0) +
-                                        ((double)comy - cy0) * ((double)comy - cy0));
+    // largest particle offset from the exact COM, bounded through the centre
+    const double bmax = sqrt(r2) + sqrt((ex - cx0) * (ex - cx0) + (ey - cy0) * (ey - cy0));
     const float lim = fmaxf(g.z * inv_theta + sqrtf(ox * ox + oy * oy), (float)(bmax * (double)inv_theta) * 1.0000002f);
     const double c2xx = C[mono(2, 0)], c2xy = C[mono(1, 1)], c2yy = C[mono(0, 2)];
     const double c3xxx = C[mono(3, 0)], c3xxy = C[mono(2, 1)], c3xyy = C[mono(1, 2)], c3yyy = C[mono(0, 3)];
@@ -150,29 +177,32 @@ __global__ void k_mp_reset(uint32_t *__restrict__ cnt) {
 
 // mono_add over particles j, j + stride, ... < end, in that order, with four
 // loads in flight (the sums keep the order, so the result is the plain loop's)
-__device__ __forceinline__ void mono_range(const float4 *__restrict__ pxym, uint32_t j, uint32_t end,
-                                           uint32_t stride, double cx, double cy, double *__restrict__ S,
-                                           double &r2) {
+__device__ __forceinline__ void mono_range(const float4 *__restrict__ pxym, const float4 *__restrict__ pho,
+                                           uint32_t j, uint32_t end, uint32_t stride, double cx, double cy,
+                                           double *__restrict__ S, double &r2) {
     for (; j < end && end - j > 3 * stride; j += 4 * stride) {
-        const float4 p0 = pxym[j], p1 = pxym[j + stride], p2 = pxym[j + 2 * stride], p3 = pxym[j + 3 * stride];
+        const MonoPt p0 = mono_pt(pxym, pho, j), p1 = mono_pt(pxym, pho, j + stride);
+        const MonoPt p2 = mono_pt(pxym, pho, j + 2 * stride), p3 = mono_pt(pxym, pho, j + 3 * stride);
         mono_add(p0, cx, cy, S, r2);
         mono_add(p1, cx, cy, S, r2);
         mono_add(p2, cx, cy, S, r2);
         mono_add(p3, cx, cy, S, r2);
     }
-    for (; j < end; j += stride) mono_add(pxym[j], cx, cy, S, r2);
+    for (; j < end; j += stride) mono_add(mono_pt(pxym, pho, j), cx, cy, S, r2);
 }
 
 // Thread per node: small nodes are finished here; the others are listed.
 __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict__ nodes,
                                                        const float4 *__restrict__ geom,
                                                        const uint32_t *__restrict__ counters,
-                                                       const float4 *__restrict__ pxym, float inv_theta,
+                                                       const float4 *__restrict__ pxym,
+                                                       const float4 *__restrict__ pho, float inv_theta,
                                                        uint32_t *__restrict__ cnt, uint32_t *__restrict__ wave_list,
                                                        uint32_t wave_cap, uint32_t *__restrict__ big_list,
                                                        uint32_t *__restrict__ big_first, uint32_t big_cap,
                                                        uint32_t *__restrict__ piece_owner, uint32_t piece_cap,
-                                                       uint32_t *__restrict__ err, float4 *__restrict__ mp) {
+                                                       uint32_t *__restrict__ err, float4 *__restrict__ mp,
+                                                       float4 *__restrict__ home) {
     const uint32_t nn = counters[kCntNodeTotal];
     for (uint32_t k = blockIdx.x * kMpBlock + threadIdx.x; k < nn; k += gridDim.x * kMpBlock) {
         const NodeRec r = nodes[k];
@@ -183,8 +213,8 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict
             for (int c = 0; c < kMono; ++c) S[c] = 0.0;
             const double cx = r.comx, cy = r.comy;
             double r2 = 0.0;
-            mono_range(pxym, r.start, r.end, 1u, cx, cy, S, r2);
-            finish_record(k, r, geom[k], S, r2, inv_theta, mp);
+            mono_range(pxym, pho, r.start, r.end, 1u, cx, cy, S, r2);
+            finish_record(k, r, geom[k], S, r2, inv_theta, mp, home);
         }
         // wave-list entries: one atomic per wave (the list order is free)
         const bool to_wave = count > kMpThreadMax && count <= kMpWaveMax;
@@ -219,10 +249,11 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict
 // Wave per listed node (kMpThreadMax < count <= kMpWaveMax).
 __global__ __launch_bounds__(kMpBlock) void k_mp_wave(const NodeRec *__restrict__ nodes,
                                                       const float4 *__restrict__ geom,
-                                                      const float4 *__restrict__ pxym, float inv_theta,
+                                                      const float4 *__restrict__ pxym,
+                                                      const float4 *__restrict__ pho, float inv_theta,
                                                       const uint32_t *__restrict__ cnt,
                                                       const uint32_t *__restrict__ wave_list, uint32_t wave_cap,
-                                                      float4 *__restrict__ mp) {
+                                                      float4 *__restrict__ mp, float4 *__restrict__ home) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nw = min(cnt[kMpWaveCount], wave_cap);
     const uint32_t waves = gridDim.x * (kMpBlock / 64);
@@ -234,15 +265,16 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_wave(const NodeRec *__restrict_
         for (int c = 0; c < kMono; ++c) S[c] = 0.0;
         const double cx = r.comx, cy = r.comy;
         double r2 = 0.0;
-        mono_range(pxym, r.start + lane, r.end, 64u, cx, cy, S, r2);
+        mono_range(pxym, pho, r.start + lane, r.end, 64u, cx, cy, S, r2);
         wave_sum(S, r2);
-        if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp);
+        if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp, home);
     }
 }
 
 // Wave per piece of a large node: sums about the node's float centre of mass.
 __global__ __launch_bounds__(kMpBlock) void k_mp_pieces(const NodeRec *__restrict__ nodes,
                                                         const float4 *__restrict__ pxym,
+                                                        const float4 *__restrict__ pho,
                                                         const uint32_t *__restrict__ cnt,
                                                         const uint32_t *__restrict__ big_list,
                                                         const uint32_t *__restrict__ big_first, uint32_t big_cap,
@@ -264,7 +296,7 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_pieces(const NodeRec *__restric
         for (int c = 0; c < kMono; ++c) S[c] = 0.0;
         const double cx = r.comx, cy = r.comy;
         double r2 = 0.0;
-        mono_range(pxym, lo + lane, hi, 64u, cx, cy, S, r2);
+        mono_range(pxym, pho, lo + lane, hi, 64u, cx, cy, S, r2);
         wave_sum(S, r2);
         if (lane <= kMono) {
             double v = r2;  // slot kMono: largest squared offset
@@ -284,7 +316,7 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_big(const NodeRec *__restrict__
                                                      const uint32_t *__restrict__ big_list,
                                                      const uint32_t *__restrict__ big_first, uint32_t big_cap,
                                                      uint32_t piece_cap, const double *__restrict__ piece_sum,
-                                                     float4 *__restrict__ mp) {
+                                                     float4 *__restrict__ mp, float4 *__restrict__ home) {
     const uint32_t nb = min(cnt[kMpBigCount], big_cap);
     const uint32_t np = cnt[kMpPieceCount];
     if (np > piece_cap) return;  // the error flag is already raised
@@ -305,7 +337,7 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_big(const NodeRec *__restrict__
             r2 = fmax(r2, ps[kMono]);
         }
         wave_sum(S, r2);
-        if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp);
+        if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp, home);
     }
 }
 
@@ -322,6 +354,8 @@ int ensure_multipole_buffers(ge_ctx *c) {
     b.mp_big_cap = (uint32_t)(((uint64_t)c->cap / kMpWaveMax + 1) * 64 + 1024);
     b.mp_piece_cap = (uint32_t)(((uint64_t)c->cap / kMpWaveMax + 1) * 128 + 1024);
     GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.node_mp), sizeof(float4) * 6ull * b.node_cap));
+    if (c->precise)
+        GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.node_home), sizeof(float4) * b.node_cap));
     GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.mp_cnt), sizeof(uint32_t) * kMpSlots));
     GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.mp_wave_list), sizeof(uint32_t) * b.mp_wave_cap));
     GE_HIP_CHECK(c, hipMalloc(reinterpret_cast<void **>(&b.mp_big_list), sizeof(uint32_t) * 2ull * b.mp_big_cap));
@@ -337,15 +371,17 @@ void launch_multipoles(ge_ctx *c, float theta) {
     hipLaunchKernelGGL(k_mp_reset, dim3(1), dim3(64), 0, c->stream, b.mp_cnt);
     uint32_t nb = blocks_for(b.node_cap, kMpBlock);
     if (nb > 8192) nb = 8192;
+    const float4 *pho = c->precise ? b.pho : nullptr;
+    float4 *home = c->precise ? b.node_home : nullptr;
     hipLaunchKernelGGL(k_mp_small, dim3(nb), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.counters, b.pxym,
-                       inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.mp_big_list, big_first, b.mp_big_cap,
-                       b.mp_piece_owner, b.mp_piece_cap, b.counters + kCntError, b.node_mp);
-    hipLaunchKernelGGL(k_mp_wave, dim3(2048), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.pxym, inv_theta,
-                       b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.node_mp);
-    hipLaunchKernelGGL(k_mp_pieces, dim3(1024), dim3(kMpBlock), 0, c->stream, b.nodes, b.pxym, b.mp_cnt, b.mp_big_list,
-                       big_first, b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.mp_piece_sum);
+                       pho, inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.mp_big_list, big_first,
+                       b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.counters + kCntError, b.node_mp, home);
+    hipLaunchKernelGGL(k_mp_wave, dim3(2048), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.pxym, pho,
+                       inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.node_mp, home);
+    hipLaunchKernelGGL(k_mp_pieces, dim3(1024), dim3(kMpBlock), 0, c->stream, b.nodes, b.pxym, pho, b.mp_cnt,
+                       b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.mp_piece_sum);
     hipLaunchKernelGGL(k_mp_big, dim3(256), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, inv_theta, b.mp_cnt,
-                       b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_cap, b.mp_piece_sum, b.node_mp);
+                       b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_cap, b.mp_piece_sum, b.node_mp, home);
 }
 
 }  // namespace ge

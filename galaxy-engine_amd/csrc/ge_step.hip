@@ -72,13 +72,37 @@ __global__ __launch_bounds__(kBlock) void k_integrate_bbox(State s, const uint32
             }
             s.pvx[i] = pvx; s.pvy[i] = pvy;
             s.vx[i] = vx; s.vy[i] = vy;
-            x = x + vx * dt;
-            y = y + vy * dt;
-            if (dom.w > 0.0f) {  // physics.cpp:1104-1110
-                if (x < 0.0f) x += dom.w;
-                else if (x >= dom.w) x -= dom.w;
-                if (y < 0.0f) y += dom.h;
-                else if (y >= dom.h) y -= dom.h;
+            if (s.hx) {
+                // precision form: the drift moves the offset from the home cell
+                int32_t hx = s.hx[i], hy = s.hy[i];
+                float ox = s.ox[i], oy = s.oy[i];
+                home_sync(x, hx, ox);
+                home_sync(y, hy, oy);
+                ox = ox + vx * dt;
+                oy = oy + vy * dt;
+                home_norm(hx, ox);
+                home_norm(hy, oy);
+                x = home_view(hx, ox);
+                y = home_view(hy, oy);
+                if (dom.w > 0.0f) {  // the wrap of physics.cpp:1104-1110, by whole domains
+                    const float sx = x < 0.0f ? 1.0f : (x >= dom.w ? -1.0f : 0.0f);
+                    const float sy = y < 0.0f ? 1.0f : (y >= dom.h ? -1.0f : 0.0f);
+                    if (sx != 0.0f) home_shift(hx, ox, sx * dom.w);
+                    if (sy != 0.0f) home_shift(hy, oy, sy * dom.h);
+                    x = home_view(hx, ox);
+                    y = home_view(hy, oy);
+                }
+                s.hx[i] = hx; s.hy[i] = hy;
+                s.ox[i] = ox; s.oy[i] = oy;
+            } else {
+                x = x + vx * dt;
+                y = y + vy * dt;
+                if (dom.w > 0.0f) {  // physics.cpp:1104-1110
+                    if (x < 0.0f) x += dom.w;
+                    else if (x >= dom.w) x -= dom.w;
+                    if (y < 0.0f) y += dom.h;
+                    else if (y >= dom.h) y -= dom.h;
+                }
             }
             s.x[i] = x; s.y[i] = y;
         }
@@ -174,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(State src, State dst, uint32_
                                                    const uint64_t *__restrict__ sorted,
                                                    const uint32_t *__restrict__ sys_off, PackFmt fmt,
                                                    uint32_t *__restrict__ perm,
-                                                   float4 *__restrict__ pxym) {
+                                                   float4 *__restrict__ pxym, float4 *__restrict__ pho) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint64_t w = sorted[i];
@@ -205,6 +229,15 @@ __global__ __launch_bounds__(kBlock) void k_gather(State src, State dst, uint32_
     if (src.temp) {  // temperature fields
         dst.temp[i] = src.temp[j]; dst.label[i] = src.label[j];
         dst.ke[i] = src.ke[j]; dst.prev_ke[i] = src.prev_ke[j];
+    }
+    if (src.hx) {  // precision form, made to agree with pos first
+        int32_t hx = src.hx[j], hy = src.hy[j];
+        float ox = src.ox[j], oy = src.oy[j];
+        home_sync(x, hx, ox);
+        home_sync(y, hy, oy);
+        dst.hx[i] = hx; dst.hy[i] = hy;
+        dst.ox[i] = ox; dst.oy[i] = oy;
+        pho[i] = make_float4(ox, oy, __int_as_float(hx), __int_as_float(hy));
     }
 }
 
@@ -244,7 +277,7 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
 // (W the per-system wave prefix), so a wave holds the same particles whether
 // the system runs alone or in a batch, and rules that look at the whole wave
 // (the leaf rule of the accuracy modes) give every system its alone result.
-template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST>
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST, bool PREC>
 __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                                            const NodeRec *__restrict__ nodes,
                                            const float4 *__restrict__ mp,
@@ -258,7 +291,9 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                                            const uint32_t *__restrict__ flags, float half_dt_end,
                                            Domain dom, unsigned long long *__restrict__ inter,
                                            const float *__restrict__ gtavg, float *__restrict__ temp,
-                                           float heat_k, float heat_dt, uint32_t *__restrict__ walk_q) {
+                                           float heat_k, float heat_dt, uint32_t *__restrict__ walk_q,
+                                           const float4 *__restrict__ pho,
+                                           const float4 *__restrict__ node_home) {
     // !PERSIST: wave w of the launch walks wave w of the layout.
     // PERSIST: the launch is persistent.  The layout's workgroup tiles (BS/64
     // consecutive waves, the tile a one-shot launch gives a workgroup) are cut
@@ -310,6 +345,10 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
         const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
         const bool live = w < sys_wave_off[nsys] && i < sys_off[sy + 1];
         const float4 p = live ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // PREC: the home form (ox, oy, hx, hy) of the target; separations come
+        // from home forms (the precision form)
+        const float4 ph = (PREC && live) ? pho[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int32_t phx = __float_as_int(ph.z), phy = __float_as_int(ph.w);
         float ti = (TEMP && live) ? temp[i] : 0.0f;  // temp[i], moved by every accepted node
         // a tree that raised an error flag may hold unwritten nodes: do not walk it
         const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | (ORDER > 0 ? kErrMultipole : 0u));
@@ -347,6 +386,8 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                 h4 = mp[6ull * idx + 4];
                 h5 = mp[6ull * idx + 5];
             }
+            float4 hc;  // PREC: the expansion centre's home form
+            if (PREC && ORDER > 0) hc = node_home[idx];
             const uint32_t next = nd.next;
             if (COUNT) ++visits;
             // index after this subtree, saturated so the walk always moves forward
@@ -359,8 +400,13 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
             float dx, dy, dist_sq;
             bool accept;
             if (ORDER > 0) {
-                dx = h0.x - p.x;
-                dy = h0.y - p.y;
+                if (PREC) {
+                    dx = home_sep(__float_as_int(hc.z), hc.x, phx, ph.x);
+                    dy = home_sep(__float_as_int(hc.w), hc.y, phy, ph.y);
+                } else {
+                    dx = h0.x - p.x;
+                    dy = h0.y - p.y;
+                }
                 if (per) {
                     dx = wrap_d(dx, dom.w, hw);
                     dy = wrap_d(dy, dom.h, hh);
@@ -476,7 +522,15 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                 for (uint32_t j = nd.start; j < jend; ++j) {
                     const float4 q = pxym[j];
                     if (open_leaf && j != i) {
-                        float px = q.x - p.x, py = q.y - p.y;
+                        float px, py;
+                        if (PREC) {
+                            const float4 qh = pho[j];
+                            px = home_sep(__float_as_int(qh.z), qh.x, phx, ph.x);
+                            py = home_sep(__float_as_int(qh.w), qh.y, phy, ph.y);
+                        } else {
+                            px = q.x - p.x;
+                            py = q.y - p.y;
+                        }
                         if (per) {
                             px = wrap_d(px, dom.w, hw);
                             py = wrap_d(py, dom.h, hh);
@@ -505,7 +559,15 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                 for (uint32_t j = nd.start; j < jend; ++j) {
                     const float4 q = pxym[j];
                     if (open_leaf && j != i) {
-                        float px = q.x - p.x, py = q.y - p.y;
+                        float px, py;
+                        if (PREC) {
+                            const float4 qh = pho[j];
+                            px = home_sep(__float_as_int(qh.z), qh.x, phx, ph.x);
+                            py = home_sep(__float_as_int(qh.w), qh.y, phy, ph.y);
+                        } else {
+                            px = q.x - p.x;
+                            py = q.y - p.y;
+                        }
                         if (per) {  // physics.cpp:169-172
                             px = wrap_d(px, dom.w, hw);
                             py = wrap_d(py, dom.h, hh);
@@ -562,9 +624,11 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
 }
 
 // Physics::naiveGravity (physics.cpp:392-434): O(N^2) with the softening
-// added unsquared; sources staged through LDS one tile at a time.
-__global__ __launch_bounds__(kBlock) void k_direct(const float4 *__restrict__ pxym, uint32_t n, float G,
-                                                   float eps, float *__restrict__ ax,
+// added unsquared; sources staged through LDS one tile at a time.  PREC: the
+// separations come from the home forms (the precision form).
+template <bool PREC>
+__global__ __launch_bounds__(kBlock) void k_direct(const float4 *__restrict__ pxym, const float4 *__restrict__ pho,
+                                                   uint32_t n, float G, float eps, float *__restrict__ ax,
                                                    float *__restrict__ ay, float *__restrict__ vx,
                                                    float *__restrict__ vy,
                                                    const uint32_t *__restrict__ flags,
@@ -572,17 +636,28 @@ __global__ __launch_bounds__(kBlock) void k_direct(const float4 *__restrict__ px
     const bool per = dom.w > 0.0f;
     const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
     __shared__ float4 tile[kBlock];
+    __shared__ float4 htile[PREC ? kBlock : 1];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const float4 p = i < n ? pxym[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 ph = (PREC && i < n) ? pho[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float tx = 0.f, ty = 0.f;
     for (uint32_t base = 0; base < n; base += kBlock) {
         const uint32_t j = base + threadIdx.x;
         tile[threadIdx.x] = j < n ? pxym[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (PREC) htile[threadIdx.x] = j < n ? pho[j] : make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
         const uint32_t lim = min((uint32_t)kBlock, n - base);
         for (uint32_t k = 0; k < lim; ++k) {
             const float4 q = tile[k];
-            float dx = q.x - p.x, dy = q.y - p.y;
+            float dx, dy;
+            if (PREC) {
+                const float4 qh = htile[k];
+                dx = home_sep(__float_as_int(qh.z), qh.x, __float_as_int(ph.z), ph.x);
+                dy = home_sep(__float_as_int(qh.w), qh.y, __float_as_int(ph.w), ph.y);
+            } else {
+                dx = q.x - p.x;
+                dy = q.y - p.y;
+            }
             if (per) {  // physics.cpp:417-420
                 dx = wrap_d(dx, dom.w, hw);
                 dy = wrap_d(dy, dom.h, hh);
@@ -655,6 +730,10 @@ __global__ __launch_bounds__(kBlock) void k_compact(State src, State dst, uint32
         dst.temp[i] = src.temp[j]; dst.label[i] = src.label[j];
         dst.ke[i] = src.ke[j]; dst.prev_ke[i] = src.prev_ke[j];
     }
+    if (src.hx) {  // precision form
+        dst.hx[i] = src.hx[j]; dst.hy[i] = src.hy[j];
+        dst.ox[i] = src.ox[j]; dst.oy[i] = src.oy[j];
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_zero2(float *__restrict__ a, float *__restrict__ b, uint32_t n) {
@@ -724,21 +803,28 @@ void launch_gather(ge_ctx *c) {
     const int cur = c->b.cur;
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.st[cur],
                        c->b.st[1 - cur], c->n, (int)c->sph, c->b.sort_a, c->b.sys_off, pack_fmt(c), c->b.perm,
-                       c->b.pxym);
+                       c->b.pxym, c->b.pho);
     c->b.cur = 1 - cur;
 }
 
-template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST>
-void launch_bh_k(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, uint32_t nb, uint32_t *walk_q) {
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST, bool PREC>
+void launch_bh_p(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, uint32_t nb, uint32_t *walk_q) {
     const State &s = c->b.st[c->b.cur];
     const float theta_sq = p->theta * p->theta;
     const float soft_sq = p->softening * p->softening;
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
-    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER, TEMP, PERSIST>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
+    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER, TEMP, PERSIST, PREC>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
                        c->b.nodes, c->b.node_mp, c->b.start_count, c->b.sys_wave_off, c->nsys, c->b.sys_off,
                        c->b.counters, c->n,
                        p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p),
-                       inter, c->b.node_tavg, s.temp, c->thermal_p.heat_conductivity, p->dt, walk_q);
+                       inter, c->b.node_tavg, s.temp, c->thermal_p.heat_conductivity, p->dt, walk_q, c->b.pho,
+                       c->b.node_home);
+}
+
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST>
+void launch_bh_k(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, uint32_t nb, uint32_t *walk_q) {
+    if (c->precise) launch_bh_p<BS, KAHAN, COUNT, ORDER, TEMP, PERSIST, true>(c, p, end_kick_half_dt, nb, walk_q);
+    else launch_bh_p<BS, KAHAN, COUNT, ORDER, TEMP, PERSIST, false>(c, p, end_kick_half_dt, nb, walk_q);
 }
 
 template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
@@ -753,7 +839,7 @@ void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     static int per_cu = 0;  // one value per kernel instance
     if (per_cu == 0) {
         int v = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_bh<BS, KAHAN, COUNT, ORDER, TEMP, true>, BS, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_bh<BS, KAHAN, COUNT, ORDER, TEMP, true, false>, BS, 0) !=
                 hipSuccess || v <= 0)
             v = 1;
         per_cu = v;
@@ -792,8 +878,14 @@ void launch_bh(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, bool 
 
 void launch_direct(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     const State &s = c->b.st[c->b.cur];
-    hipLaunchKernelGGL(k_direct, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.pxym, c->n,
-                       p->G, p->softening, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p));
+    if (c->precise)
+        hipLaunchKernelGGL(k_direct<true>, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.pxym,
+                           c->b.pho, c->n, p->G, p->softening, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt,
+                           domain_of(p));
+    else
+        hipLaunchKernelGGL(k_direct<false>, dim3(blocks_for(c->n, kBlock)), dim3(kBlock), 0, c->stream, c->b.pxym,
+                           c->b.pho, c->n, p->G, p->softening, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt,
+                           domain_of(p));
 }
 
 void launch_sys_of(ge_ctx *c) {

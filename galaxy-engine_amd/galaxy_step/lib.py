@@ -47,7 +47,7 @@ PARTICLE_ORDER = ["x", "y", "vx", "vy", "ax", "ay", "pvx", "pvy", "mass", "flags
 class GeConfig(C.Structure):
     _fields_ = [("capacity", C.c_uint32), ("device", C.c_int32), ("enable_sph", C.c_int32),
                 ("max_systems", C.c_uint32), ("enable_render", C.c_int32), ("enable_temperature", C.c_int32),
-                ("max_constraints", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("max_constraints", C.c_uint32), ("precise_positions", C.c_int32)]
 
 
 class GeParticles(C.Structure):
@@ -220,7 +220,7 @@ class Context:
     """One device context (ge_ctx)."""
 
     def __init__(self, capacity, sph=False, device=0, max_systems=1, render=False, temperature=False,
-                 max_constraints=0):
+                 max_constraints=0, precise_positions=False):
         self.lib = load_library()
         cfg = GeConfig()
         self.lib.ge_default_config(C.byref(cfg))
@@ -231,6 +231,7 @@ class Context:
         cfg.enable_render = 1 if render else 0
         cfg.enable_temperature = 1 if temperature else 0
         cfg.max_constraints = int(max_constraints)
+        cfg.precise_positions = 1 if precise_positions else 0
         self.render = bool(render)
         self.temperature = bool(temperature)
         h = C.c_void_p()

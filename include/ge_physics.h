@@ -57,7 +57,16 @@ typedef struct {
     int32_t enable_render; /* keep the render fields of ge_render.h on the device */
     int32_t enable_temperature; /* keep the temperature fields of ge_thermal.h (needs enable_sph) */
     uint32_t max_constraints;   /* particle constraints the context can hold (ge_constraints.h) */
-    uint32_t reserved[1];
+    int32_t precise_positions;  /* precision form (no counterpart in the reference): every
+                                   position is also kept as a home cell of side 1/16 and a
+                                   float offset from that cell's centre; the integrator
+                                   drifts the offset, and the walk's particle pairs and node
+                                   centres (accuracy >= 1) and the direct sum take separations
+                                   from these, so they keep their precision however far the
+                                   system sits from the origin (|x|, |y| < 2^20).  pos stays
+                                   the nearest float; a position written by anything but the
+                                   integrator (upload, constraints, walls) restarts its home
+                                   form from that float. */
 } ge_config;
 
 /* Structure-of-arrays view of the ParticlePhysics fields on this path
