@@ -53,14 +53,15 @@ enum : int { kMpWaveCount = 0, kMpBigCount = 1, kMpPieceCount = 2, kMpSlots = 4 
 __host__ __device__ constexpr int mono(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
 
 // particle j as (x, y, m) in FP64: its float position, or with the precision
-// form (pho != nullptr) its home form, which FP64 holds exactly
+// form (PREC) its home form, which FP64 holds exactly
 struct MonoPt {
     double x, y, m;
 };
 
+template <bool PREC>
 __device__ __forceinline__ MonoPt mono_pt(const float4 *__restrict__ pxym, const float4 *__restrict__ pho, uint32_t j) {
     const float4 p = pxym[j];
-    if (!pho) return MonoPt{(double)p.x, (double)p.y, (double)p.z};
+    if (!PREC) return MonoPt{(double)p.x, (double)p.y, (double)p.z};
     const float4 h = pho[j];
     return MonoPt{(double)__float_as_int(h.z) * (double)kHomeCell + (double)h.x,
                   (double)__float_as_int(h.w) * (double)kHomeCell + (double)h.y, (double)p.z};
@@ -177,21 +178,23 @@ __global__ void k_mp_reset(uint32_t *__restrict__ cnt) {
 
 // mono_add over particles j, j + stride, ... < end, in that order, with four
 // loads in flight (the sums keep the order, so the result is the plain loop's)
+template <bool PREC>
 __device__ __forceinline__ void mono_range(const float4 *__restrict__ pxym, const float4 *__restrict__ pho,
                                            uint32_t j, uint32_t end, uint32_t stride, double cx, double cy,
                                            double *__restrict__ S, double &r2) {
     for (; j < end && end - j > 3 * stride; j += 4 * stride) {
-        const MonoPt p0 = mono_pt(pxym, pho, j), p1 = mono_pt(pxym, pho, j + stride);
-        const MonoPt p2 = mono_pt(pxym, pho, j + 2 * stride), p3 = mono_pt(pxym, pho, j + 3 * stride);
+        const MonoPt p0 = mono_pt<PREC>(pxym, pho, j), p1 = mono_pt<PREC>(pxym, pho, j + stride);
+        const MonoPt p2 = mono_pt<PREC>(pxym, pho, j + 2 * stride), p3 = mono_pt<PREC>(pxym, pho, j + 3 * stride);
         mono_add(p0, cx, cy, S, r2);
         mono_add(p1, cx, cy, S, r2);
         mono_add(p2, cx, cy, S, r2);
         mono_add(p3, cx, cy, S, r2);
     }
-    for (; j < end; j += stride) mono_add(mono_pt(pxym, pho, j), cx, cy, S, r2);
+    for (; j < end; j += stride) mono_add(mono_pt<PREC>(pxym, pho, j), cx, cy, S, r2);
 }
 
 // Thread per node: small nodes are finished here; the others are listed.
+template <bool PREC>
 __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict__ nodes,
                                                        const float4 *__restrict__ geom,
                                                        const uint32_t *__restrict__ counters,
@@ -213,7 +216,7 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict
             for (int c = 0; c < kMono; ++c) S[c] = 0.0;
             const double cx = r.comx, cy = r.comy;
             double r2 = 0.0;
-            mono_range(pxym, pho, r.start, r.end, 1u, cx, cy, S, r2);
+            mono_range<PREC>(pxym, pho, r.start, r.end, 1u, cx, cy, S, r2);
             finish_record(k, r, geom[k], S, r2, inv_theta, mp, home);
         }
         // wave-list entries: one atomic per wave (the list order is free)
@@ -247,6 +250,7 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict
 }
 
 // Wave per listed node (kMpThreadMax < count <= kMpWaveMax).
+template <bool PREC>
 __global__ __launch_bounds__(kMpBlock) void k_mp_wave(const NodeRec *__restrict__ nodes,
                                                       const float4 *__restrict__ geom,
                                                       const float4 *__restrict__ pxym,
@@ -265,13 +269,14 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_wave(const NodeRec *__restrict_
         for (int c = 0; c < kMono; ++c) S[c] = 0.0;
         const double cx = r.comx, cy = r.comy;
         double r2 = 0.0;
-        mono_range(pxym, pho, r.start + lane, r.end, 64u, cx, cy, S, r2);
+        mono_range<PREC>(pxym, pho, r.start + lane, r.end, 64u, cx, cy, S, r2);
         wave_sum(S, r2);
         if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp, home);
     }
 }
 
 // Wave per piece of a large node: sums about the node's float centre of mass.
+template <bool PREC>
 __global__ __launch_bounds__(kMpBlock) void k_mp_pieces(const NodeRec *__restrict__ nodes,
                                                         const float4 *__restrict__ pxym,
                                                         const float4 *__restrict__ pho,
@@ -296,7 +301,7 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_pieces(const NodeRec *__restric
         for (int c = 0; c < kMono; ++c) S[c] = 0.0;
         const double cx = r.comx, cy = r.comy;
         double r2 = 0.0;
-        mono_range(pxym, pho, lo + lane, hi, 64u, cx, cy, S, r2);
+        mono_range<PREC>(pxym, pho, lo + lane, hi, 64u, cx, cy, S, r2);
         wave_sum(S, r2);
         if (lane <= kMono) {
             double v = r2;  // slot kMono: largest squared offset
@@ -373,13 +378,24 @@ void launch_multipoles(ge_ctx *c, float theta) {
     if (nb > 8192) nb = 8192;
     const float4 *pho = c->precise ? b.pho : nullptr;
     float4 *home = c->precise ? b.node_home : nullptr;
-    hipLaunchKernelGGL(k_mp_small, dim3(nb), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.counters, b.pxym,
-                       pho, inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.mp_big_list, big_first,
-                       b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.counters + kCntError, b.node_mp, home);
-    hipLaunchKernelGGL(k_mp_wave, dim3(2048), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.pxym, pho,
-                       inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.node_mp, home);
-    hipLaunchKernelGGL(k_mp_pieces, dim3(1024), dim3(kMpBlock), 0, c->stream, b.nodes, b.pxym, pho, b.mp_cnt,
-                       b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.mp_piece_sum);
+    if (c->precise) {
+        hipLaunchKernelGGL(k_mp_small<true>, dim3(nb), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.counters,
+                           b.pxym, pho, inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.mp_big_list, big_first,
+                           b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.counters + kCntError, b.node_mp, home);
+        hipLaunchKernelGGL(k_mp_wave<true>, dim3(2048), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.pxym, pho,
+                           inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.node_mp, home);
+        hipLaunchKernelGGL(k_mp_pieces<true>, dim3(1024), dim3(kMpBlock), 0, c->stream, b.nodes, b.pxym, pho, b.mp_cnt,
+                           b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.mp_piece_sum);
+    } else {
+        hipLaunchKernelGGL(k_mp_small<false>, dim3(nb), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.counters,
+                           b.pxym, pho, inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.mp_big_list, big_first,
+                           b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap, b.counters + kCntError, b.node_mp, home);
+        hipLaunchKernelGGL(k_mp_wave<false>, dim3(2048), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, b.pxym,
+                           pho, inv_theta, b.mp_cnt, b.mp_wave_list, b.mp_wave_cap, b.node_mp, home);
+        hipLaunchKernelGGL(k_mp_pieces<false>, dim3(1024), dim3(kMpBlock), 0, c->stream, b.nodes, b.pxym, pho,
+                           b.mp_cnt, b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_owner, b.mp_piece_cap,
+                           b.mp_piece_sum);
+    }
     hipLaunchKernelGGL(k_mp_big, dim3(256), dim3(kMpBlock), 0, c->stream, b.nodes, b.node_geom, inv_theta, b.mp_cnt,
                        b.mp_big_list, big_first, b.mp_big_cap, b.mp_piece_cap, b.mp_piece_sum, b.node_mp, home);
 }

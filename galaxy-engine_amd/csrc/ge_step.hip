@@ -367,6 +367,7 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
         const uint32_t walk_end = __builtin_amdgcn_readfirstlane(whi);
         uint32_t skip_until = root;
         float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
+        float nfx = 0.f, nfy = 0.f;  // KAHAN, ORDER > 0: accepted-node terms since the last leaf
         uint32_t cnt = 0, cnt_nodes = 0, visits = 0, leaf_trips = 0;  // visits and trips: per wave
         uint32_t idx = __builtin_amdgcn_readfirstlane(wlo);
         while (idx < walk_end) {
@@ -484,7 +485,10 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                     tx = dx * f;
                     ty = dy * f;
                 }
-                if (KAHAN) {
+                if (KAHAN && ORDER > 0) {  // plain within a node run, compensated per run
+                    nfx += tx;
+                    nfy += ty;
+                } else if (KAHAN) {
                     kahan_add(fx, cx, tx);
                     kahan_add(fy, cy, ty);
                 } else {
@@ -519,34 +523,45 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                 const uint32_t jend = min(nd.end, n);
                 if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
                 float sx = 0.f, sy = 0.f;
-                for (uint32_t j = nd.start; j < jend; ++j) {
-                    const float4 q = pxym[j];
-                    if (open_leaf && j != i) {
-                        float px, py;
-                        if (PREC) {
-                            const float4 qh = pho[j];
-                            px = home_sep(__float_as_int(qh.z), qh.x, phx, ph.x);
-                            py = home_sep(__float_as_int(qh.w), qh.y, phy, ph.y);
-                        } else {
-                            px = q.x - p.x;
-                            py = q.y - p.y;
-                        }
-                        if (per) {
-                            px = wrap_d(px, dom.w, hw);
-                            py = wrap_d(py, dom.h, hh);
-                        }
-                        const float d2 = px * px + py * py + soft_sq;
-                        const float inv = __builtin_amdgcn_rsqf(d2);
-                        const float f = q.z * (inv * inv * inv);
-                        sx = fmaf(px, f, sx);
-                        sy = fmaf(py, f, sy);
-                        if (COUNT) ++cnt;
+                // every lane evaluates every pair (no branch in the loop); a
+                // lane that does not take the pair selects a zero factor.  Two
+                // pairs per trip, their loads issued together.
+                auto pair = [&](const float4 q, const float4 qh, uint32_t j) {
+                    float px, py;
+                    if (PREC) {
+                        px = home_sep(__float_as_int(qh.z), qh.x, phx, ph.x);
+                        py = home_sep(__float_as_int(qh.w), qh.y, phy, ph.y);
+                    } else {
+                        px = q.x - p.x;
+                        py = q.y - p.y;
                     }
+                    if (per) {
+                        px = wrap_d(px, dom.w, hw);
+                        py = wrap_d(py, dom.h, hh);
+                    }
+                    const float d2 = px * px + py * py + soft_sq;
+                    const float inv = __builtin_amdgcn_rsqf(d2);
+                    const bool take = open_leaf && j != i;
+                    const float f = take ? q.z * (inv * inv * inv) : 0.0f;
+                    sx = fmaf(px, f, sx);
+                    sy = fmaf(py, f, sy);
+                    if (COUNT) cnt += take ? 1u : 0u;
+                };
+                const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                uint32_t j = nd.start;
+                for (; j + 1 < jend; j += 2) {
+                    const float4 q0 = pxym[j], q1 = pxym[j + 1];
+                    const float4 qh0 = PREC ? pho[j] : zero4, qh1 = PREC ? pho[j + 1] : zero4;
+                    pair(q0, qh0, j);
+                    pair(q1, qh1, j + 1);
                 }
+                if (j < jend) pair(pxym[j], PREC ? pho[j] : zero4, j);
                 if (open_leaf) {
-                    if (KAHAN) {
-                        kahan_add(fx, cx, G * sx);
-                        kahan_add(fy, cy, G * sy);
+                    if (KAHAN) {  // the node run since the last leaf and this leaf's sum
+                        kahan_add(fx, cx, fmaf(G, sx, nfx));
+                        kahan_add(fy, cy, fmaf(G, sy, nfy));
+                        nfx = 0.f;
+                        nfy = 0.f;
                     } else {
                         fx = fmaf(G, sx, fx);
                         fy = fmaf(G, sy, fy);
@@ -605,6 +620,10 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
         }
         if (live) {
             if (TEMP) temp[i] = ti;
+            if (KAHAN && ORDER > 0) {
+                kahan_add(fx, cx, nfx);
+                kahan_add(fy, cy, nfy);
+            }
             const float accx = KAHAN ? fx - cx : fx;
             const float accy = KAHAN ? fy - cy : fy;
             ax[i] = accx;
