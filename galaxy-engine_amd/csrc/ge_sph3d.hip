@@ -41,7 +41,15 @@ struct Sph3Const {
     float vol, spiky, lap;  // SPH3D.h:27-46
     float half_dt;          // fused integrateEnd3D (0 = none)
     uint32_t mask;          // hash slots - 1
+    float h2hi;             // a bound with sqrt(r2) < h  =>  r2 < h2hi (cheap pre-test)
+    float inv_h, inv_vol, coh_norm;  // 1/h, 1/vol, 30/(pi h^3)
 };
+
+// The pair passes divide with the hardware reciprocal (v_rcp_f32, within
+// 1 ulp) where the reference divides: the pair terms then differ from the
+// reference's by a few ulp, far inside the passes' tolerance.  Tests that
+// decide which pairs interact keep the reference's exact arithmetic.
+__device__ __forceinline__ float rcp3(float v) { return __builtin_amdgcn_rcpf(v); }
 
 struct Grid3View {
     const uint32_t *first;
@@ -89,11 +97,11 @@ __device__ __forceinline__ void for_candidates(const Grid3View &G, int cx, int c
 }
 
 // SPH3D::smoothingKernelCohesion (SPH3D.h:48-53)
-__device__ __forceinline__ float coh_kernel3(float r, float h) {
+__device__ __forceinline__ float coh_kernel3(float r, const Sph3Const &K) {
 #pragma clang fp contract(off)
-    if (r >= h) return 0.0f;
-    const float q = r / h;
-    return (1.0f - q) * (0.5f - q) * (0.5f - q) * 30.0f / (kPiF * h * h * h);
+    if (r >= K.h) return 0.0f;
+    const float q = r * K.inv_h;
+    return (1.0f - q) * (0.5f - q) * (0.5f - q) * K.coh_norm;
 }
 
 // packed sort words (slot << kIdxBits | particle); particles outside the grid
@@ -213,19 +221,52 @@ struct RevKeep {
     }
 };
 
+// The viscosity pass gives each particle kSph3Lanes lanes: lane q takes the
+// cells q, q + kSph3Lanes, ... of the 27 (numbered in the reference's dz, dy,
+// dx order), so the cells' hashes and slot ranges are read in parallel, and
+// the group adds its partial sums in a fixed butterfly order (deterministic).
+// (The density and pressure passes keep one thread per particle over the
+// neighbour lists below: eight lanes scanning the 27 cells twice per
+// pressure pass measured slower than building the lists once.)
+constexpr uint32_t kSph3Lanes = 8;
+
+__device__ __forceinline__ float group_sum3(float v) {
+#pragma unroll
+    for (uint32_t o = 1; o < kSph3Lanes; o <<= 1) v += __shfl_xor(v, (int)o, 64);
+    return v;
+}
+
+// f(kk, flags) for the candidates of this lane's cells around (cx, cy, cz)
+template <class F>
+__device__ __forceinline__ void for_lane_candidates(const Grid3View &G, int cx, int cy, int cz, uint32_t mask,
+                                                    uint32_t q0, F &&f) {
+    for (uint32_t c = q0; c < 27u; c += kSph3Lanes) {
+        const int nx = cx + (int)(c % 3u) - 1, ny = cy + (int)((c / 3u) % 3u) - 1, nz = cz + (int)(c / 9u) - 1;
+        const uint32_t s = slot_of(nx, ny, nz, mask);
+        const uint32_t lo = G.first[s], hi = G.first[s + 1];
+        for (uint32_t kk = lo; kk < hi; ++kk) {
+            const int4 cc = G.cell[kk];
+            if (cc.x == nx && cc.y == ny && cc.z == nz) f(kk, (uint32_t)cc.w);
+        }
+    }
+}
+
 // SPH3D::computeViscCohesionForces (SPH3D.cpp:98-170) in gather form, then
 // the predicted position (SPH3D.cpp:190-198).  pp[k] = (ppx, ppy, ppz, restDens).
 __global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3View A, Sph3Const K,
                                                   float4 *__restrict__ sphf, float4 *__restrict__ pp) {
 #pragma clang fp contract(off)
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t k = t / kSph3Lanes, q0 = t % kSph3Lanes;
     if (k >= n) return;
     const uint32_t i = A.ord[k];
     const int4 ci = A.cell[k];
     const uint32_t fi = (uint32_t)ci.w;
     if (!visitor3(fi)) {
-        sphf[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pp[k] = make_float4(s.ppx[i], s.ppy[i], s.ppz[i], s.rest_dens[i]);
+        if (q0 == 0) {
+            sphf[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            pp[k] = make_float4(s.ppx[i], s.ppy[i], s.ppz[i], s.rest_dens[i]);
+        }
         return;
     }
     const float4 o0 = A.pk0[k], o1 = A.pk1[k];
@@ -233,42 +274,46 @@ __global__ __launch_bounds__(kBlock) void k3_visc(State3 s, uint32_t n, Grid3Vie
     const bool fwd = !(fi & GE_FLAG_PINNED);
     const float mI = o0.w * K.unit;
     const float di = o2.y < 0.001f ? 0.001f : o2.y;
+    const float aI = K.visc * o1.w * mI * rcp3(di);  // j's visit of i: viscosity factor without the kernel
+    const float cI = K.coh * o2.x;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_candidates(A, ci.x, ci.y, ci.z, K.mask, [&](uint32_t kk, uint32_t fj) {
+    for_lane_candidates(A, ci.x, ci.y, ci.z, K.mask, q0, [&](uint32_t kk, uint32_t fj) {
         if (kk == k || !visitor3(fj)) return;
         const float4 p0 = A.pk0[kk];
         const float dx = p0.x - o0.x, dy = p0.y - o0.y, dz = p0.z - o0.z;
         const float rsq = dx * dx + dy * dy + dz * dz;
         if (rsq >= K.h2) return;
-        const float r = sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
+        const float r = __builtin_amdgcn_sqrtf(rsq < 1e-6f ? 1e-6f : rsq);
+        const float inv_r = rcp3(r);
         const float lapw = r >= K.h ? 0.0f : K.lap * (K.h - r);  // smoothingKernelLaplacian
-        const float wc = coh_kernel3(r, K.h);
+        const float wc = coh_kernel3(r, K);
         const float4 p1 = A.pk1[kk];
         const float2 p2 = A.pk2[kk];
         if (fwd) {  // i's visit of j (SPH3D.cpp:108-158)
             const float mJ = p0.w * K.unit;
             const float dj = p2.y < 0.001f ? 0.001f : p2.y;
-            const float vfx = K.visc * p1.w * mJ / dj * lapw * (p1.x - o1.x);
-            const float vfy = K.visc * p1.w * mJ / dj * lapw * (p1.y - o1.y);
-            const float vfz = K.visc * p1.w * mJ / dj * lapw * (p1.z - o1.z);
-            const float cc = K.coh * o2.x;
-            const float cfx = cc * mJ * wc * (dx / r), cfy = cc * mJ * wc * (dy / r), cfz = cc * mJ * wc * (dz / r);
+            const float a = K.visc * p1.w * mJ * rcp3(dj) * lapw;
+            const float vfx = a * (p1.x - o1.x), vfy = a * (p1.y - o1.y), vfz = a * (p1.z - o1.z);
+            const float cw = cI * mJ * wc * inv_r;
+            const float cfx = cw * dx, cfy = cw * dy, cfz = cw * dz;
             sx = sx + (vfx + cfx);
             sy = sy + (vfy + cfy);
             sz = sz + (vfz + cfz);
         }
         if (!(fj & GE_FLAG_PINNED)) {  // j's visit of i, scattered back to i (SPH3D.cpp:160-168)
-            const float ex = o0.x - p0.x, ey = o0.y - p0.y, ez = o0.z - p0.z;
-            const float vfx = K.visc * o1.w * mI / di * lapw * (o1.x - p1.x);
-            const float vfy = K.visc * o1.w * mI / di * lapw * (o1.y - p1.y);
-            const float vfz = K.visc * o1.w * mI / di * lapw * (o1.z - p1.z);
-            const float cc = K.coh * p2.x;
-            const float cfx = cc * mI * wc * (ex / r), cfy = cc * mI * wc * (ey / r), cfz = cc * mI * wc * (ez / r);
+            const float a = aI * lapw;
+            const float vfx = a * (o1.x - p1.x), vfy = a * (o1.y - p1.y), vfz = a * (o1.z - p1.z);
+            const float cw = K.coh * p2.x * mI * wc * inv_r;
+            const float cfx = cw * -dx, cfy = cw * -dy, cfz = cw * -dz;  // pos_i - pos_j
             sx = sx - (vfx + cfx);
             sy = sy - (vfy + cfy);
             sz = sz - (vfz + cfz);
         }
     });
+    sx = group_sum3(sx);
+    sy = group_sum3(sy);
+    sz = group_sum3(sz);
+    if (q0 != 0) return;
     sphf[i] = make_float4(sx, sy, sz, 0.0f);
     const float px = o0.x + (sx / o0.w) * K.coeff;
     const float py = o0.y + (sy / o0.w) * K.coeff;
@@ -311,11 +356,11 @@ __global__ __launch_bounds__(kBlock) void k3_density(State3 s, uint32_t n, Grid3
              PredKeep{pp, me, K.h}, [&](uint32_t kk) {
                  const float4 pj = pp[kk];
                  const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
-                 const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+                 const float rr = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
                  const float mJ = A.pk0[kk].w * K.unit2;
                  const float rho0 = 0.5f * (me.w + pj.w);
-                 const float w = rr >= K.h ? 0.0f : (K.h - rr) * (K.h - rr) / K.vol;  // smoothingKernel
-                 dens = dens + mJ * w / rho0;
+                 const float w = rr >= K.h ? 0.0f : (K.h - rr) * (K.h - rr) * K.inv_vol;  // smoothingKernel
+                 dens = dens + mJ * w * rcp3(rho0);
              });
     float pt = K.delta * (dens - me.w);
     if (pt < 0.0f) pt = 0.0f;
@@ -360,13 +405,14 @@ __device__ __forceinline__ void pressure_pair3(float dx, float dy, float dz, flo
                                                float &fz) {
 #pragma clang fp contract(off)
     const float gw = rr >= K.h ? 0.0f : K.spiky * ((K.h - rr) * (K.h - rr));  // spikyKernelDerivative
-    const float nx = dx / rr, ny = dy / rr, nz = dz / rr;
+    const float inv_rr = rcp3(rr);
+    const float nx = dx * inv_rr, ny = dy * inv_rr, nz = dz * inv_rr;
     const float avgP = 0.5f * (pi_ + pj_);
     const float avgD = 0.5f * (di + dj);
-    float mag = -(mi * K.unit2 + mj * K.unit2) * avgP / (avgD < 0.01f ? 0.01f : avgD);
+    float mag = -(mi * K.unit2 + mj * K.unit2) * avgP * rcp3(avgD < 0.01f ? 0.01f : avgD);
     const float smax = mi < mj ? mj : mi;
     const float smin = mj < mi ? mj : mi;
-    const float sc8 = 8.0f / (smax / smin);
+    const float sc8 = 8.0f * smin * rcp3(smax);  // 8 / (smax / smin)
     mag = mag * (sc8 < 1.0f ? sc8 : 1.0f);
     fx = mag * gw * nx;
     fy = mag * gw * ny;
@@ -508,6 +554,12 @@ Sph3Const make_const3(const ge3_ctx *c, const ge3_step_params *p, float half_dt)
     K.lap = 45.0f / (kPiF * std::pow(K.h, 6.0f));
     K.half_dt = half_dt;
     K.mask = (1u << c->b.slot_bits) - 1u;
+    // sqrt is correctly rounded and h a float, so sqrt(r2) < h implies r2 < h^2
+    // exactly; the float bound sits a little above h^2
+    K.h2hi = std::nextafter((float)((double)K.h * K.h * (1.0 + 1e-6)), INFINITY);
+    K.inv_h = 1.0f / K.h;
+    K.inv_vol = 1.0f / K.vol;
+    K.coh_norm = 30.0f / (kPiF * K.h * K.h * K.h);
     return K;
 }
 
@@ -550,7 +602,9 @@ void sph3_pcisph(ge3_ctx *c, const ge3_step_params *p, float end_kick_dt) {
     const uint32_t nb = blocks_for(n, kBlock);
     const NbrList Lp{c->b.nbr_idx[0], c->b.nbr_cnt[0], c->b.nbr_cap};
     const NbrList Lr{c->b.nbr_idx[1], c->b.nbr_cnt[1], c->b.nbr_cap};
-    hipLaunchKernelGGL(k3_visc, dim3(nb), dim3(kBlock), 0, c->stream, s, n, view3(A), K, c->b.sphf, c->b.sph_pp);
+    // the viscosity pass: kSph3Lanes per particle
+    hipLaunchKernelGGL(k3_visc, dim3(blocks_for((uint64_t)n * kSph3Lanes, kBlock)), dim3(kBlock), 0, c->stream, s, n,
+                       view3(A), K, c->b.sphf, c->b.sph_pp);
     // grid B over the predicted positions of visiting particles
     hipLaunchKernelGGL(k3_slot_keys, dim3(nb), dim3(kBlock), 0, c->stream, s.ppx, s.ppy, s.ppz, s.flags, n, 1,
                        p->sph_cell, K.mask, B.keys);
