@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--theta", type=float, default=0.7)
-    ap.add_argument("--accuracy", type=int, default=2, help="0 reference walk, 1 quadrupole, 2 hexadecapole")
+    ap.add_argument("--accuracy", type=int, default=2,
+                    help="0 reference walk, 1 quadrupole, 3 octupole, 2 hexadecapole")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bound on the CPU-baseline sample (0 disables it)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -182,7 +183,7 @@ def step_roofline(phase_ms, ms_per_step, n, m_nodes, interactions, accuracy, sph
         if ph not in bpp and ph != "gravity" and ph != "sph":
             continue
         if ph == "gravity":
-            b = bpp["walk"] * n + bpn["walk"] * m_nodes + (bpn["multipole"] * m_nodes if accuracy == 2 else 0)
+            b = bpp["walk"] * n + bpn["walk"] * m_nodes + (bpn["multipole"] * m_nodes if accuracy != 0 else 0)
             f = FLOPS_PAIR * interactions
         elif ph == "sph":
             b = bpp["sph_passes"] * n
