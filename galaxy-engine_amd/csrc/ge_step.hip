@@ -571,12 +571,12 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                 // leaf direct sum over [start, end), skipping i == j (physics.cpp:163-196)
                 const uint32_t jend = min(nd.end, n);
                 if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
-                for (uint32_t j = nd.start; j < jend; ++j) {
-                    const float4 q = pxym[j];
+                // two partners per trip, their loads issued together; each lane
+                // still adds exactly its own partners, in order
+                auto partner = [&](const float4 q, const float4 qh, uint32_t j) {
                     if (open_leaf && j != i) {
                         float px, py;
                         if (PREC) {
-                            const float4 qh = pho[j];
                             px = home_sep(__float_as_int(qh.z), qh.x, phx, ph.x);
                             py = home_sep(__float_as_int(qh.w), qh.y, phy, ph.y);
                         } else {
@@ -599,7 +599,16 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                         }
                         if (COUNT) ++cnt;
                     }
+                };
+                const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                uint32_t j = nd.start;
+                for (; j + 1 < jend; j += 2) {
+                    const float4 q0 = pxym[j], q1 = pxym[j + 1];
+                    const float4 qh0 = PREC ? pho[j] : zero4, qh1 = PREC ? pho[j + 1] : zero4;
+                    partner(q0, qh0, j);
+                    partner(q1, qh1, j + 1);
                 }
+                if (j < jend) partner(pxym[j], PREC ? pho[j] : zero4, j);
             }
             const bool descend = active && !accept && !leaf;
             idx = __ballot(descend) ? idx + 1u : after;

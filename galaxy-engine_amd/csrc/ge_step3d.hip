@@ -296,9 +296,9 @@ __global__ __launch_bounds__(BS) void k3_bh(const float4 *__restrict__ pxyzm, co
         const bool open_leaf = active && !accept && leaf;
         if (__ballot(open_leaf)) {
             // leaf direct sum over [start, end), j != i (physics3D.cpp:200-236)
-            const uint32_t jend = min(h1.w, n);
-            for (uint32_t j = h1.z; j < jend; ++j) {
-                const float4 q = pxyzm[j];
+            // two partners per trip, their loads issued together; each lane
+            // still adds exactly its own partners, in order
+            auto partner = [&](const float4 q, uint32_t j) {
                 if (open_leaf && j != i) {
                     float px = q.x - p.x, py = q.y - p.y, pz = q.z - p.z;
                     if (per) {  // physics3D.cpp:206-210
@@ -320,7 +320,15 @@ __global__ __launch_bounds__(BS) void k3_bh(const float4 *__restrict__ pxyzm, co
                     }
                     if (COUNT) ++cnt;
                 }
+            };
+            const uint32_t jend = min(h1.w, n);
+            uint32_t j = h1.z;
+            for (; j + 1 < jend; j += 2) {
+                const float4 q0 = pxyzm[j], q1 = pxyzm[j + 1];
+                partner(q0, j);
+                partner(q1, j + 1);
             }
+            if (j < jend) partner(pxyzm[j], j);
         }
         const bool descend = active && !accept && !leaf;
         idx = __ballot(descend) ? idx + 1u : after;
