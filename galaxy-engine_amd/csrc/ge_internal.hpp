@@ -20,7 +20,7 @@ namespace ge {
 constexpr int kBlock = 256;            // threads per workgroup for streaming kernels
 constexpr int kIdxBits = 28;           // low bits of a packed sort word hold the slot
 constexpr int kMortonBits = 36;        // 18 bits per axis (morton.cpp:3-25)
-constexpr uint32_t kChunk = 8192;      // particles per bounding-box reduction chunk
+constexpr uint32_t kChunk = 1024;      // particles per bounding-box reduction chunk (one workgroup each)
 constexpr uint64_t kIdxMask = (1ull << kIdxBits) - 1;
 constexpr uint32_t kMaxParticles = 1u << kIdxBits;
 constexpr uint32_t kNone = 0xFFFFFFFFu;

@@ -148,6 +148,6 @@ int sph3_neighbors(ge3_ctx *c, int which, int64_t *off, uint32_t *idx, int64_t c
 int sph3_neighbor_counts(ge3_ctx *c, uint32_t *counts);
 void sph3_count_into(ge3_ctx *c, float cell, int32_t *out);
 
-constexpr uint32_t kBBoxChunk3 = 8192;  // particles per bbox workgroup
+constexpr uint32_t kBBoxChunk3 = 1024;  // particles per bbox workgroup
 
 }  // namespace ge

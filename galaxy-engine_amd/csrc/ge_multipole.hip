@@ -99,6 +99,78 @@ __device__ __forceinline__ void wave_sum(double *__restrict__ S, double &r2) {
     for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
 }
 
+// Half-wave and half-row swaps of a double (v_permlane32_swap /
+// v_permlane16_swap on each 32-bit half): with HALF = 32, lanes 32-63 of x
+// trade places with lanes 0-31 of y; with HALF = 16 the odd 16-lane rows of
+// x trade with the even rows of y.
+template <int HALF>
+__device__ __forceinline__ void swap_halves(double &x, double &y) {
+    const uint64_t xb = (uint64_t)__double_as_longlong(x), yb = (uint64_t)__double_as_longlong(y);
+    const uint32_t xl = (uint32_t)xb, xh = (uint32_t)(xb >> 32), yl = (uint32_t)yb, yh = (uint32_t)(yb >> 32);
+    uint32_t rxl, ryl, rxh, ryh;
+    if (HALF == 32) {
+        const auto lo = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+        rxl = lo[0]; ryl = lo[1]; rxh = hi[0]; ryh = hi[1];
+    } else {
+        const auto lo = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+        rxl = lo[0]; ryl = lo[1]; rxh = hi[0]; ryh = hi[1];
+    }
+    x = __longlong_as_double((long long)(((uint64_t)rxh << 32) | rxl));
+    y = __longlong_as_double((long long)(((uint64_t)ryh << 32) | ryl));
+}
+
+// One transposing step over lane bit BIT with W slots: a lane whose bit is
+// clear keeps slots [0, W) and the one whose bit is set keeps [W, 2W), each
+// adding its partner's copy.  The slot that may hold r2 (the last component)
+// takes the max.
+template <int W, int BIT>
+__device__ __forceinline__ void tstep(double *__restrict__ v, uint32_t lane, uint32_t &cb) {
+    const bool hi = (lane >> BIT) & 1u;
+    cb += hi ? (uint32_t)W : 0u;
+    const bool mx = cb + (uint32_t)W == (uint32_t)(kMono + 1);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        double a = v[i], b = v[W + i];
+        if (BIT == 5) {
+            swap_halves<32>(a, b);
+        } else if (BIT == 4) {
+            swap_halves<16>(a, b);
+        } else {
+            const double keep = hi ? b : a;
+            b = __shfl_xor(hi ? a : b, 1 << BIT, 64);
+            a = keep;
+        }
+        v[i] = (i == W - 1 && mx) ? fmax(a, b) : a + b;
+    }
+}
+
+// The same sums transposed: lanes 4c .. 4c + 3 end with component c (c <
+// kMono: the sum of S[c] over the wave; c == kMono: the wave max of r2).
+// Each of the first four steps halves the components a lane carries, so a
+// wave moves 17 doubles instead of 6 x 16, and the two widest steps are
+// lane swaps.  The order of the sums is fixed.
+__device__ __forceinline__ double wave_sum_t(const double *__restrict__ S, double r2) {
+    const uint32_t lane = threadIdx.x & 63u;
+    double v[kMono + 1];
+#pragma unroll
+    for (int c = 0; c < kMono; ++c) v[c] = S[c];
+    v[kMono] = r2;
+    uint32_t cb = 0;  // first component of the block this lane keeps
+    tstep<8, 5>(v, lane, cb);
+    tstep<4, 4>(v, lane, cb);
+    tstep<2, 3>(v, lane, cb);
+    tstep<1, 2>(v, lane, cb);
+    const bool mx = cb == (uint32_t)kMono;
+#pragma unroll
+    for (int o = 2; o > 0; o >>= 1) {
+        const double recv = __shfl_xor(v[0], o, 64);
+        v[0] = mx ? fmax(v[0], recv) : v[0] + recv;
+    }
+    return v[0];
+}
+
 // T = S shifted by w: T_ab = sum_{i<=a, j<=b} C(a,i) C(b,j) (-w_x)^(a-i) (-w_y)^(b-j) S_ij
 __device__ __forceinline__ void mom_shift_add(const double *__restrict__ S, double wx, double wy, double *__restrict__ T) {
     constexpr double binom[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
@@ -249,29 +321,54 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_small(const NodeRec *__restrict
     }
 }
 
-// Wave per listed node (kMpThreadMax < count <= kMpWaveMax).
+// Wave per listed node (kMpThreadMax < count <= kMpWaveMax).  A wave sums
+// up to kMpBatch consecutive list entries one after another and parks each
+// total in its own LDS slice; lane t then finishes entry t, so the record's
+// arithmetic runs on kMpBatch lanes at once instead of one.
+constexpr uint32_t kMpBatch = 8;
+
 template <bool PREC>
 __global__ __launch_bounds__(kMpBlock) void k_mp_wave(const NodeRec *__restrict__ nodes,
-                                                      const float4 *__restrict__ geom,
-                                                      const float4 *__restrict__ pxym,
-                                                      const float4 *__restrict__ pho, float inv_theta,
-                                                      const uint32_t *__restrict__ cnt,
-                                                      const uint32_t *__restrict__ wave_list, uint32_t wave_cap,
-                                                      float4 *__restrict__ mp, float4 *__restrict__ home) {
+                                                         const float4 *__restrict__ geom,
+                                                         const float4 *__restrict__ pxym,
+                                                         const float4 *__restrict__ pho, float inv_theta,
+                                                         const uint32_t *__restrict__ cnt,
+                                                         const uint32_t *__restrict__ wave_list, uint32_t wave_cap,
+                                                         float4 *__restrict__ mp, float4 *__restrict__ home) {
+    constexpr uint32_t kWaves = kMpBlock / 64;
+    __shared__ double tot[kWaves][kMpBatch][kPiece];
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wid = threadIdx.x >> 6;
     const uint32_t nw = min(cnt[kMpWaveCount], wave_cap);
-    const uint32_t waves = gridDim.x * (kMpBlock / 64);
-    for (uint32_t e = blockIdx.x * (kMpBlock / 64) + (threadIdx.x >> 6); e < nw; e += waves) {
-        const uint32_t k = wave_list[e];
-        const NodeRec r = nodes[k];
-        double S[kMono];
+    const uint32_t waves = gridDim.x * kWaves;
+    for (uint32_t e0 = (blockIdx.x * kWaves + wid) * kMpBatch; e0 < nw; e0 += waves * kMpBatch) {
+        const uint32_t nt = min(kMpBatch, nw - e0);
+        for (uint32_t t = 0; t < nt; ++t) {
+            const NodeRec r = nodes[wave_list[e0 + t]];
+            double S[kMono];
 #pragma unroll
-        for (int c = 0; c < kMono; ++c) S[c] = 0.0;
-        const double cx = r.comx, cy = r.comy;
-        double r2 = 0.0;
-        mono_range<PREC>(pxym, pho, r.start + lane, r.end, 64u, cx, cy, S, r2);
-        wave_sum(S, r2);
-        if (lane == 0) finish_record(k, r, geom[k], S, r2, inv_theta, mp, home);
+            for (int c = 0; c < kMono; ++c) S[c] = 0.0;
+            const double cx = r.comx, cy = r.comy;
+            double r2 = 0.0;
+            mono_range<PREC>(pxym, pho, r.start + lane, r.end, 64u, cx, cy, S, r2);
+            const double v = wave_sum_t(S, r2);
+            if ((lane & 3u) == 0u) tot[wid][t][lane >> 2] = v;
+        }
+        // the slice is this wave's own: a wave-level fence orders its LDS
+        // writes before the reads of other lanes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < nt) {
+            const uint32_t k = wave_list[e0 + lane];
+            double S[kMono];
+#pragma unroll
+            for (int c = 0; c < kMono; ++c) S[c] = tot[wid][lane][c];
+            finish_record(k, nodes[k], geom[k], S, tot[wid][lane][kMono], inv_theta, mp, home);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -302,14 +399,9 @@ __global__ __launch_bounds__(kMpBlock) void k_mp_pieces(const NodeRec *__restric
         const double cx = r.comx, cy = r.comy;
         double r2 = 0.0;
         mono_range<PREC>(pxym, pho, lo + lane, hi, 64u, cx, cy, S, r2);
-        wave_sum(S, r2);
-        if (lane <= kMono) {
-            double v = r2;  // slot kMono: largest squared offset
-#pragma unroll
-            for (int c = 0; c < kMono; ++c)
-                if ((int)lane == c) v = S[c];
-            piece_sum[(size_t)kPiece * e + lane] = v;
-        }
+        // slot kMono: the largest squared offset
+        const double v = wave_sum_t(S, r2);
+        if ((lane & 3u) == 0u) piece_sum[(size_t)kPiece * e + (lane >> 2)] = v;
     }
 }
 
