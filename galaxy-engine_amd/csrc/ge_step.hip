@@ -248,39 +248,75 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float v) {
     sum = t;
 }
 
+// The walks' wave layout.  Several systems: system s owns the node range
+// [P[off_s], P[off_s+1]) (P is the node-start prefix of the tree build), and
+// waves are laid out per system: wave w of system s holds its particles
+// 64 (w - W_s) .. 64 (w - W_s) + 63 (W the per-system wave prefix), so a wave
+// holds the same particles whether the system runs alone or in a batch.
+//
+// !PERSIST: workgroup b of the launch takes tile xcd_block(b) once.
+// PERSIST: the launch is persistent.  The layout's workgroup tiles (BS/64
+// consecutive waves, the tile a one-shot launch gives a workgroup) are cut
+// into kWalkQueues contiguous ranges; workgroup b starts on range
+// b mod kWalkQueues (the XCD that hardware dispatch gives it, so
+// neighbouring tiles share an L2) and, when its range runs dry, takes tiles
+// from the next ranges.  walk_q[0..kWalkQueues) are the ranges' tickets and
+// walk_q[kWalkQueues] counts the waves that have finished; the last wave out
+// clears them all for the next launch (walk_retire).
+template <bool PERSIST>
+__device__ __forceinline__ uint32_t walk_next_tile(uint32_t total_tiles, uint32_t *walk_q, uint32_t &q_at,
+                                                   uint32_t &q_seen, bool &first, uint32_t *tile_s) {
+    if (!PERSIST) {
+        if (!first) return kNone;
+        first = false;
+        return xcd_block(blockIdx.x, gridDim.x);
+    }
+    while (q_seen < kWalkQueues) {
+        const uint32_t lo = (uint32_t)((uint64_t)total_tiles * q_at / kWalkQueues);
+        const uint32_t hi = (uint32_t)((uint64_t)total_tiles * (q_at + 1) / kWalkQueues);
+        __syncthreads();  // every wave has read the previous ticket
+        if (threadIdx.x == 0) *tile_s = atomicAdd(&walk_q[q_at], 1u);
+        __syncthreads();
+        const uint32_t t = *tile_s;
+        if (t < hi - lo) return lo + t;
+        q_at = (q_at + 1) % kWalkQueues;
+        ++q_seen;
+    }
+    return kNone;
+}
+
+template <int BS, bool PERSIST>
+__device__ __forceinline__ void walk_retire(uint32_t *walk_q) {
+    if (PERSIST && (threadIdx.x & (kWave - 1)) == 0) {
+        const uint32_t waves = gridDim.x * (BS / kWave);
+        if (atomicAdd(&walk_q[kWalkQueues], 1u) == waves - 1u)
+            for (uint32_t k = 0; k <= kWalkQueues; ++k) walk_q[k] = 0u;
+    }
+}
+
+// the system of wave w: the last s with W_s <= w (wave-uniform search)
+__device__ __forceinline__ uint32_t wave_system(const uint32_t *__restrict__ sys_wave_off, uint32_t nsys,
+                                                uint32_t w) {
+    uint32_t slo = 0, shi = nsys;
+    while (shi - slo > 1u) {
+        const uint32_t mid = (slo + shi) >> 1;
+        if (sys_wave_off[mid] <= w) slo = mid;
+        else shi = mid;
+    }
+    return slo;
+}
+
 // Barnes-Hut accelerations: the per-particle pre-order walk of
 // Physics::calculateForceFromGrid (physics.cpp:91-205), executed by a whole
 // wave in lockstep.  Every lane walks its own reference path; the wave visits
 // the union of the 64 paths.  A lane that accepted node k stays muted until
 // the walk leaves k's subtree (index k + next + 1).  Node reads are
 // wave-uniform (scalar loads).  The acceptance test uses the reference's
-// unfused arithmetic so the accepted-node sets match the CPU path.
-//
-// ORDER 2 / 4 (accuracy modes, no counterpart in the reference): the node is
-// accepted when |d| exceeds its opening radius (size/theta + COM offset) and
-// contributes the Taylor expansion of the softened kernel about its FP64
-// centre of mass up to the quadrupole (2) or hexadecapole (4) term.  With
-// d = target - COM, R^2 = |d|^2 + eps^2, r = 1/R, A_n = C_n / n! and the
-// traces T, U, W of A3 and A4:
-//   a = G (S d + V)
-//   S = -M r^3 + 3 tr A2 r^5 + (-15 d.A2.d + 45 T.d - 45 W) r^7
-//       + (-105 A3:ddd + 630 U:dd) r^9 - 945 A4:dddd r^11
-//   V = (6 A2 d - 9 T) r^5 + (45 A3:dd - 180 U d) r^7 + 420 A4:ddd r^9
-// (the Cartesian Taylor series, D_k = (-1)^k (2k-1)!! r^(2k+1)).  The node
-// record holds each moment times the constant it meets first in S, so S is
-// a Horner polynomial in r^2 and V is one more, with the ratios of the
-// remaining constants folded into three multipliers.
-//
-// Several systems: system s owns the node range [P[off_s], P[off_s+1]) (P is
-// the node-start prefix of the tree build).  Waves are laid out per system:
-// wave w of system s holds its particles 64 (w - W_s) .. 64 (w - W_s) + 63
-// (W the per-system wave prefix), so a wave holds the same particles whether
-// the system runs alone or in a batch, and rules that look at the whole wave
-// (the leaf rule of the accuracy modes) give every system its alone result.
-template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST, bool PREC>
+// unfused arithmetic so the accepted-node sets match the CPU path.  TEMP adds
+// the walk's node-level heat transfer (physics.cpp:261-276).
+template <int BS, bool KAHAN, bool COUNT, bool TEMP, bool PERSIST, bool PREC>
 __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                                            const NodeRec *__restrict__ nodes,
-                                           const float4 *__restrict__ mp,
                                            const uint32_t *__restrict__ start_prefix,
                                            const uint32_t *__restrict__ sys_wave_off, uint32_t nsys,
                                            const uint32_t *__restrict__ sys_off,
@@ -292,54 +328,16 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                                            Domain dom, unsigned long long *__restrict__ inter,
                                            const float *__restrict__ gtavg, float *__restrict__ temp,
                                            float heat_k, float heat_dt, uint32_t *__restrict__ walk_q,
-                                           const float4 *__restrict__ pho,
-                                           const float4 *__restrict__ node_home) {
-    // !PERSIST: wave w of the launch walks wave w of the layout.
-    // PERSIST: the launch is persistent.  The layout's workgroup tiles (BS/64
-    // consecutive waves, the tile a one-shot launch gives a workgroup) are cut
-    // into kWalkQueues contiguous ranges; workgroup b starts on range
-    // b mod kWalkQueues (the XCD that hardware dispatch gives it, so
-    // neighbouring tiles share an L2) and, when its range runs dry, takes
-    // tiles from the next ranges.  walk_q[0..kWalkQueues) are the ranges'
-    // tickets and walk_q[kWalkQueues] counts the waves that have finished;
-    // the last wave out clears them all for the next launch.
+                                           const float4 *__restrict__ pho) {
     __shared__ uint32_t tile_s;
     const uint32_t total_tiles = (sys_wave_off[nsys] + (BS / kWave) - 1) / (BS / kWave);
     uint32_t q_at = blockIdx.x % kWalkQueues, q_seen = 0;
     bool first = true;
     for (;;) {
-        uint32_t blk;
-        if (!PERSIST) {
-            if (!first) break;
-            first = false;
-            blk = xcd_block(blockIdx.x, gridDim.x);
-        } else {
-            blk = kNone;
-            while (q_seen < kWalkQueues) {
-                const uint32_t lo = (uint32_t)((uint64_t)total_tiles * q_at / kWalkQueues);
-                const uint32_t hi = (uint32_t)((uint64_t)total_tiles * (q_at + 1) / kWalkQueues);
-                __syncthreads();  // every wave has read the previous ticket
-                if (threadIdx.x == 0) tile_s = atomicAdd(&walk_q[q_at], 1u);
-                __syncthreads();
-                const uint32_t t = tile_s;
-                if (t < hi - lo) {
-                    blk = lo + t;
-                    break;
-                }
-                q_at = (q_at + 1) % kWalkQueues;
-                ++q_seen;
-            }
-            if (blk == kNone) break;
-        }
+        const uint32_t blk = walk_next_tile<PERSIST>(total_tiles, walk_q, q_at, q_seen, first, &tile_s);
+        if (blk == kNone) break;
         const uint32_t w = __builtin_amdgcn_readfirstlane(blk * (BS / kWave) + (threadIdx.x / kWave));
-        // the wave's system: the last s with W_s <= w (wave-uniform search)
-        uint32_t slo = 0, shi = nsys;
-        while (shi - slo > 1u) {
-            const uint32_t mid = (slo + shi) >> 1;
-            if (sys_wave_off[mid] <= w) slo = mid;
-            else shi = mid;
-        }
-        const uint32_t sy = slo;
+        const uint32_t sy = wave_system(sys_wave_off, nsys, w);
         const uint32_t i = sys_off[sy] + (w - sys_wave_off[sy]) * kWave + (threadIdx.x & (kWave - 1));
         const bool per = dom.w > 0.0f;
         const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
@@ -351,7 +349,7 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
         const int32_t phx = __float_as_int(ph.z), phy = __float_as_int(ph.w);
         float ti = (TEMP && live) ? temp[i] : 0.0f;  // temp[i], moved by every accepted node
         // a tree that raised an error flag may hold unwritten nodes: do not walk it
-        const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | (ORDER > 0 ? kErrMultipole : 0u));
+        const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth);
         const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
         uint32_t root = kNone, end = 0u;
         if (live) {
@@ -367,28 +365,10 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
         const uint32_t walk_end = __builtin_amdgcn_readfirstlane(whi);
         uint32_t skip_until = root;
         float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
-        float nfx = 0.f, nfy = 0.f;  // KAHAN, ORDER > 0: accepted-node terms since the last leaf
         uint32_t cnt = 0, cnt_nodes = 0, visits = 0, leaf_trips = 0;  // visits and trips: per wave
         uint32_t idx = __builtin_amdgcn_readfirstlane(wlo);
         while (idx < walk_end) {
             const NodeRec nd = nodes[idx];
-            // ORDER > 0: the whole multipole record is fetched with the node, so
-            // an accepting wave does not wait for a second load
-            float4 h0, h1, h2, h3, h4, h5;
-            if (ORDER > 0) {
-                h0 = mp[6ull * idx];
-                h1 = mp[6ull * idx + 1];
-            }
-            if (ORDER >= 3) {
-                h2 = mp[6ull * idx + 2];
-                h3 = mp[6ull * idx + 3];
-            }
-            if (ORDER >= 4) {
-                h4 = mp[6ull * idx + 4];
-                h5 = mp[6ull * idx + 5];
-            }
-            float4 hc;  // PREC: the expansion centre's home form
-            if (PREC && ORDER > 0) hc = node_home[idx];
             const uint32_t next = nd.next;
             if (COUNT) ++visits;
             // index after this subtree, saturated so the walk always moves forward
@@ -400,22 +380,7 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
             const bool active = idx >= skip_until && idx < end;
             float dx, dy, dist_sq;
             bool accept;
-            if (ORDER > 0) {
-                if (PREC) {
-                    dx = home_sep(__float_as_int(hc.z), hc.x, phx, ph.x);
-                    dy = home_sep(__float_as_int(hc.w), hc.y, phy, ph.y);
-                } else {
-                    dx = h0.x - p.x;
-                    dy = h0.y - p.y;
-                }
-                if (per) {
-                    dx = wrap_d(dx, dom.w, hw);
-                    dy = wrap_d(dy, dom.h, hh);
-                }
-                const float d2 = dx * dx + dy * dy;
-                dist_sq = d2 + soft_sq;
-                accept = d2 > h0.z;  // h0.z holds the squared opening radius
-            } else {
+            {
 #pragma clang fp contract(off)
                 dx = nd.comx - p.x;
                 dy = nd.comy - p.y;
@@ -427,73 +392,18 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                 accept = nd.size_sq < theta_sq * dist_sq;
             }
             const bool leaf = next == 0;
-            // ORDER > 0: a leaf that any active lane opens is summed directly by
-            // every active lane (exact, and the wave runs that loop anyway); those
-            // lanes skip the leaf's expansion.  ORDER 0 keeps the reference's
-            // per-lane choice.
-            bool direct = active && !accept && leaf;
-            if (ORDER > 0 && leaf && __ballot(direct)) direct = active;
-            if (active && accept && !direct) {
+            const bool open_leaf = active && !accept && leaf;
+            if (active && accept) {
                 const float inv = __builtin_amdgcn_rsqf(dist_sq);
-                float tx, ty;
-                if (ORDER > 0) {
-                    const float qx = -dx, qy = -dy;  // d = target - COM
-                    const float r2 = inv * inv;
-                    const float gr3 = G * (inv * r2);
-                    const float p2x = h1.x * qx + h1.y * qy, p2y = h1.y * qx + h1.z * qy;
-                    const float s2 = qx * p2x + qy * p2y;
-                    if (ORDER >= 4) {
-                        const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
-                        const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
-                        const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
-                        const float pux = h5.y * qx + h5.z * qy, puy = h5.z * qx + h5.w * qy;
-                        const float s3u = qx * (p3x + pux) + qy * (p3y + puy);
-                        const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y) + h3.z;
-                        const float c0 = qxx * qx, c3 = qyy * qy, c1 = (3.0f * qxx) * qy, c2 = (3.0f * qyy) * qx;
-                        const float p4x = h4.x * c0 + h4.y * c1 + h4.z * c2 + h4.w * c3;
-                        const float p4y = h4.y * c0 + h4.z * c1 + h4.w * c2 + h5.x * c3;
-                        const float s4 = qx * p4x + qy * p4y;
-                        const float sc = gr3 * (r2 * (r2 * (r2 * (r2 * s4 + s3u) + c7) + h1.w) - h0.w);
-                        const float e = (28.0f / 9.0f) * r2, f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
-                        const float ox = (2.0f * p2x + h3.x) + f * ((3.0f * p3x + 2.0f * pux) + e * p4x);
-                        const float oy = (2.0f * p2y + h3.y) + f * ((3.0f * p3y + 2.0f * puy) + e * p4y);
-                        tx = qx * sc + g5 * ox;
-                        ty = qy * sc + g5 * oy;
-                    } else if (ORDER == 3) {
-                        // the same series without the fourth-order moments (A4, U, W)
-                        const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
-                        const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
-                        const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
-                        const float s3 = qx * p3x + qy * p3y;
-                        const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y);
-                        const float sc = gr3 * (r2 * (r2 * (r2 * s3 + c7) + h1.w) - h0.w);
-                        const float f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
-                        const float ox = (2.0f * p2x + h3.x) + f * (3.0f * p3x);
-                        const float oy = (2.0f * p2y + h3.y) + f * (3.0f * p3y);
-                        tx = qx * sc + g5 * ox;
-                        ty = qy * sc + g5 * oy;
-                    } else {
-                        const float sc = gr3 * (r2 * (r2 * s2 + h1.w) - h0.w);
-                        const float g5 = gr3 * (-0.4f * r2);
-                        tx = qx * sc + g5 * p2x;
-                        ty = qy * sc + g5 * p2y;
-                    }
+                const float inv2 = inv * inv;
+                const float inv3 = inv2 * inv;
+                const float f = G * nd.mass * inv3;
+                if (KAHAN) {
+                    kahan_add(fx, cx, dx * f);
+                    kahan_add(fy, cy, dy * f);
                 } else {
-                    const float inv2 = inv * inv;
-                    const float inv3 = inv2 * inv;
-                    const float f = G * nd.mass * inv3;
-                    tx = dx * f;
-                    ty = dy * f;
-                }
-                if (KAHAN && ORDER > 0) {  // plain within a node run, compensated per run
-                    nfx += tx;
-                    nfy += ty;
-                } else if (KAHAN) {
-                    kahan_add(fx, cx, tx);
-                    kahan_add(fy, cy, ty);
-                } else {
-                    fx += tx;
-                    fy += ty;
+                    fx += dx * f;
+                    fy += dy * f;
                 }
                 if (COUNT) {
                     ++cnt;
@@ -516,58 +426,7 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
                 }
                 skip_until = after;
             }
-            const bool open_leaf = direct;
-            if (ORDER > 0 && __ballot(open_leaf)) {
-                // leaf direct sum: the pairs go to a plain per-leaf sum of
-                // m_j d_j r^-3, scaled by G and compensated once per leaf
-                const uint32_t jend = min(nd.end, n);
-                if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
-                float sx = 0.f, sy = 0.f;
-                // every lane evaluates every pair (no branch in the loop); a
-                // lane that does not take the pair selects a zero factor.  Two
-                // pairs per trip, their loads issued together.
-                auto pair = [&](const float4 q, const float4 qh, uint32_t j) {
-                    float px, py;
-                    if (PREC) {
-                        px = home_sep(__float_as_int(qh.z), qh.x, phx, ph.x);
-                        py = home_sep(__float_as_int(qh.w), qh.y, phy, ph.y);
-                    } else {
-                        px = q.x - p.x;
-                        py = q.y - p.y;
-                    }
-                    if (per) {
-                        px = wrap_d(px, dom.w, hw);
-                        py = wrap_d(py, dom.h, hh);
-                    }
-                    const float d2 = px * px + py * py + soft_sq;
-                    const float inv = __builtin_amdgcn_rsqf(d2);
-                    const bool take = open_leaf && j != i;
-                    const float f = take ? q.z * (inv * inv * inv) : 0.0f;
-                    sx = fmaf(px, f, sx);
-                    sy = fmaf(py, f, sy);
-                    if (COUNT) cnt += take ? 1u : 0u;
-                };
-                const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                uint32_t j = nd.start;
-                for (; j + 1 < jend; j += 2) {
-                    const float4 q0 = pxym[j], q1 = pxym[j + 1];
-                    const float4 qh0 = PREC ? pho[j] : zero4, qh1 = PREC ? pho[j + 1] : zero4;
-                    pair(q0, qh0, j);
-                    pair(q1, qh1, j + 1);
-                }
-                if (j < jend) pair(pxym[j], PREC ? pho[j] : zero4, j);
-                if (open_leaf) {
-                    if (KAHAN) {  // the node run since the last leaf and this leaf's sum
-                        kahan_add(fx, cx, fmaf(G, sx, nfx));
-                        kahan_add(fy, cy, fmaf(G, sy, nfy));
-                        nfx = 0.f;
-                        nfy = 0.f;
-                    } else {
-                        fx = fmaf(G, sx, fx);
-                        fy = fmaf(G, sy, fy);
-                    }
-                }
-            } else if (ORDER == 0 && __ballot(open_leaf)) {
+            if (__ballot(open_leaf)) {
                 // leaf direct sum over [start, end), skipping i == j (physics.cpp:163-196)
                 const uint32_t jend = min(nd.end, n);
                 if (COUNT) leaf_trips += jend > nd.start ? jend - nd.start : 0u;
@@ -629,7 +488,277 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
         }
         if (live) {
             if (TEMP) temp[i] = ti;
-            if (KAHAN && ORDER > 0) {
+            const float accx = KAHAN ? fx - cx : fx;
+            const float accy = KAHAN ? fy - cy : fy;
+            ax[i] = accx;
+            ay[i] = accy;
+            if (half_dt_end != 0.0f && !(flags[i] & GE_FLAG_PINNED)) {
+#pragma clang fp contract(off)
+                vx[i] = vx[i] + accx * half_dt_end;
+                vy[i] = vy[i] + accy * half_dt_end;
+            }
+        }
+    }
+    walk_retire<BS, PERSIST>(walk_q);
+}
+
+// Acceleration of one accepted node in the accuracy modes (ORDER 2 / 3 / 4,
+// no counterpart in the reference): the Taylor expansion of the softened
+// kernel about the node's FP64 centre of mass up to the quadrupole (2),
+// octupole (3) or hexadecapole (4) term.  With d = target - COM,
+// R^2 = |d|^2 + eps^2, r = 1/R, A_n = C_n / n! and the traces T, U, W of A3
+// and A4:
+//   a = G (S d + V)
+//   S = -M r^3 + 3 tr A2 r^5 + (-15 d.A2.d + 45 T.d - 45 W) r^7
+//       + (-105 A3:ddd + 630 U:dd) r^9 - 945 A4:dddd r^11
+//   V = (6 A2 d - 9 T) r^5 + (45 A3:dd - 180 U d) r^7 + 420 A4:ddd r^9
+// (the Cartesian Taylor series, D_k = (-1)^k (2k-1)!! r^(2k+1)).  The node
+// record holds each moment times the constant it meets first in S, so S is
+// a Horner polynomial in r^2 and V is one more, with the ratios of the
+// remaining constants folded into three multipliers.  qx, qy = d; inv = r.
+template <int ORDER>
+__device__ __forceinline__ void mp_accel(float4 h0, float4 h1, float4 h2, float4 h3, float4 h4, float4 h5, float qx,
+                                         float qy, float inv, float G, float &tx, float &ty) {
+    const float r2 = inv * inv;
+    const float gr3 = G * (inv * r2);
+    const float p2x = h1.x * qx + h1.y * qy, p2y = h1.y * qx + h1.z * qy;
+    if (ORDER >= 4) {
+        const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
+        const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
+        const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
+        const float pux = h5.y * qx + h5.z * qy, puy = h5.z * qx + h5.w * qy;
+        const float s3u = qx * (p3x + pux) + qy * (p3y + puy);
+        const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y) + h3.z;
+        const float c0 = qxx * qx, c3 = qyy * qy, c1 = (3.0f * qxx) * qy, c2 = (3.0f * qyy) * qx;
+        const float p4x = h4.x * c0 + h4.y * c1 + h4.z * c2 + h4.w * c3;
+        const float p4y = h4.y * c0 + h4.z * c1 + h4.w * c2 + h5.x * c3;
+        const float s4 = qx * p4x + qy * p4y;
+        const float sc = gr3 * (r2 * (r2 * (r2 * (r2 * s4 + s3u) + c7) + h1.w) - h0.w);
+        const float e = (28.0f / 9.0f) * r2, f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
+        const float ox = (2.0f * p2x + h3.x) + f * ((3.0f * p3x + 2.0f * pux) + e * p4x);
+        const float oy = (2.0f * p2y + h3.y) + f * ((3.0f * p3y + 2.0f * puy) + e * p4y);
+        tx = qx * sc + g5 * ox;
+        ty = qy * sc + g5 * oy;
+    } else if (ORDER == 3) {
+        // the same series without the fourth-order moments (A4, U, W)
+        const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
+        const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
+        const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
+        const float s3 = qx * p3x + qy * p3y;
+        const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y);
+        const float sc = gr3 * (r2 * (r2 * (r2 * s3 + c7) + h1.w) - h0.w);
+        const float f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
+        const float ox = (2.0f * p2x + h3.x) + f * (3.0f * p3x);
+        const float oy = (2.0f * p2y + h3.y) + f * (3.0f * p3y);
+        tx = qx * sc + g5 * ox;
+        ty = qy * sc + g5 * oy;
+    } else {
+        const float s2 = qx * p2x + qy * p2y;
+        const float sc = gr3 * (r2 * (r2 * s2 + h1.w) - h0.w);
+        const float g5 = gr3 * (-0.4f * r2);
+        tx = qx * sc + g5 * p2x;
+        ty = qy * sc + g5 * p2y;
+    }
+}
+
+// Leaf direct sum of the accuracy modes' walk: sx, sy += m_j d_j r^-3 over
+// particles [j0, j1), two pairs per trip with their loads issued together.
+// With softening the self pair adds exactly zero (its separation is zero),
+// so SELF (the self test) is only needed without softening.
+template <bool SELF, bool PREC>
+__device__ __forceinline__ void leaf_sum(const float4 *__restrict__ pxym, const float4 *__restrict__ pho, uint32_t j0,
+                                         uint32_t j1, float4 p, float4 ph, uint32_t self, bool per, Domain dom,
+                                         float hw, float hh, float soft_sq, float &sx, float &sy) {
+    const int32_t phx = __float_as_int(ph.z), phy = __float_as_int(ph.w);
+    auto pair = [&](const float4 q, const float4 qh, uint32_t j) {
+        float px, py;
+        if (PREC) {
+            px = home_sep(__float_as_int(qh.z), qh.x, phx, ph.x);
+            py = home_sep(__float_as_int(qh.w), qh.y, phy, ph.y);
+        } else {
+            px = q.x - p.x;
+            py = q.y - p.y;
+        }
+        if (per) {
+            px = wrap_d(px, dom.w, hw);
+            py = wrap_d(py, dom.h, hh);
+        }
+        const float inv = __builtin_amdgcn_rsqf(fmaf(px, px, fmaf(py, py, soft_sq)));
+        float f = q.z * (inv * inv * inv);
+        if (SELF) f = j != self ? f : 0.0f;
+        sx = fmaf(px, f, sx);
+        sy = fmaf(py, f, sy);
+    };
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t j = j0;
+    for (; j + 1 < j1; j += 2) {
+        const float4 q0 = pxym[j], q1 = pxym[j + 1];
+        const float4 qh0 = PREC ? pho[j] : zero4, qh1 = PREC ? pho[j + 1] : zero4;
+        pair(q0, qh0, j);
+        pair(q1, qh1, j + 1);
+    }
+    if (j < j1) pair(pxym[j], PREC ? pho[j] : zero4, j);
+}
+
+// The walk of the accuracy modes (no counterpart in the reference).  A node
+// is accepted when |d| exceeds its opening radius (size/theta + COM offset,
+// or b_max/theta) for every lane of the wave, and then every lane adds its
+// expansion (mp_accel); when some lane fails, the wave descends, and a leaf
+// that any lane opens is summed directly by every lane.  Each lane therefore
+// sums over a partition of its system into nodes whose opening test it passes
+// and single particles, a refinement of the partition its own walk would
+// take; every block runs with all lanes busy, and the walk keeps no per-lane
+// state.  Lanes past the end of the system walk as copies of its last
+// particle and write nothing.  The node record and its moments are read with
+// scalar loads.  With softening the self pair adds exactly zero (its
+// separation is zero), so the leaf loop needs no self test; without
+// softening it keeps one.  KAHAN: the accepted-node terms between two leaves
+// are summed plainly and added, with the leaf's sum, to a compensated sum.
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool PERSIST, bool PREC>
+__global__ __launch_bounds__(BS, 8) void k_bh_mp(const float4 *__restrict__ pxym,
+                                              const NodeRec *__restrict__ nodes,
+                                              const float4 *__restrict__ mp,
+                                              const uint32_t *__restrict__ start_prefix,
+                                              const uint32_t *__restrict__ sys_wave_off, uint32_t nsys,
+                                              const uint32_t *__restrict__ sys_off,
+                                              const uint32_t *__restrict__ counters, uint32_t n, float G,
+                                              float soft_sq, float *__restrict__ ax, float *__restrict__ ay,
+                                              float *__restrict__ vx, float *__restrict__ vy,
+                                              const uint32_t *__restrict__ flags, float half_dt_end, Domain dom,
+                                              unsigned long long *__restrict__ inter,
+                                              uint32_t *__restrict__ walk_q, const float4 *__restrict__ pho,
+                                              const float4 *__restrict__ node_home) {
+    __shared__ uint32_t tile_s;
+    const uint32_t total_waves = sys_wave_off[nsys];
+    const uint32_t total_tiles = (total_waves + (BS / kWave) - 1) / (BS / kWave);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    uint32_t q_at = blockIdx.x % kWalkQueues, q_seen = 0;
+    bool first = true;
+    for (;;) {
+        const uint32_t blk = walk_next_tile<PERSIST>(total_tiles, walk_q, q_at, q_seen, first, &tile_s);
+        if (blk == kNone) break;
+        const uint32_t w = __builtin_amdgcn_readfirstlane(blk * (BS / kWave) + (threadIdx.x / kWave));
+        if (w >= total_waves) continue;
+        const uint32_t sy = wave_system(sys_wave_off, nsys, w);
+        const uint32_t s_lo = sys_off[sy], s_hi = sys_off[sy + 1];
+        const uint32_t i = s_lo + (w - sys_wave_off[sy]) * kWave + lane;
+        const bool live = i < s_hi;
+        const uint32_t ic = live ? i : s_hi - 1u;  // the wave's first lane is always live
+        const bool per = dom.w > 0.0f;
+        const float hw = dom.w * 0.5f, hh = dom.h * 0.5f;
+        const float4 p = pxym[ic];
+        const float4 ph = PREC ? pho[ic] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int32_t phx = __float_as_int(ph.z), phy = __float_as_int(ph.w);
+        // a tree that raised an error flag may hold unwritten nodes: do not walk it
+        const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | kErrMultipole);
+        const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
+        const uint32_t root = __builtin_amdgcn_readfirstlane(start_prefix[s_lo]);
+        const uint32_t end = __builtin_amdgcn_readfirstlane(min(start_prefix[s_hi], node_count));
+        float fx = 0.f, fy = 0.f, cx = 0.f, cy = 0.f;
+        float nfx = 0.f, nfy = 0.f;  // KAHAN: accepted-node terms since the last leaf
+        uint32_t cnt = 0, cnt_nodes = 0, visits = 0, leaf_trips = 0;  // visits and trips: per wave
+        uint32_t idx = root;
+        while (idx < end) {
+            // the node, its whole multipole record and (PREC) its centre's
+            // home form are loaded together, before any branch
+            const NodeRec nd = nodes[idx];
+            const float4 *rec = mp + 6ull * idx;
+            const float4 h0 = rec[0], h1 = rec[1];
+            float4 h2 = make_float4(0.f, 0.f, 0.f, 0.f), h3 = h2, h4 = h2, h5 = h2;
+            if (ORDER >= 3) {
+                h2 = rec[2];
+                h3 = rec[3];
+            }
+            if (ORDER >= 4) {
+                h4 = rec[4];
+                h5 = rec[5];
+                // keeps these loads in the same round trip as the node's
+                // instead of a second one inside the accepting branch
+                asm volatile("" ::"s"(h4.x), "s"(h4.y), "s"(h4.z), "s"(h4.w), "s"(h5.x), "s"(h5.y), "s"(h5.z),
+                             "s"(h5.w));
+            }
+            float4 hc;  // PREC: the expansion centre's home form
+            if (PREC) hc = node_home[idx];
+            const uint32_t next = nd.next;
+            if (COUNT) ++visits;
+            // index after this subtree, saturated so the walk always moves forward
+            const uint32_t after = (next >= node_count - idx) ? node_count : idx + next + 1;
+            // empty node (mass <= 0, NaN excluded), tested on the scalar unit:
+            // its subtree is skipped
+            const uint32_t mb = __float_as_uint(nd.mass);
+            const bool empty = (int32_t)mb <= 0 && (mb & 0x7fffffffu) <= 0x7f800000u;
+            float dx, dy;  // COM - target
+            if (PREC) {
+                dx = home_sep(__float_as_int(hc.z), hc.x, phx, ph.x);
+                dy = home_sep(__float_as_int(hc.w), hc.y, phy, ph.y);
+            } else {
+                dx = h0.x - p.x;
+                dy = h0.y - p.y;
+            }
+            if (per) {
+                dx = wrap_d(dx, dom.w, hw);
+                dy = wrap_d(dy, dom.h, hh);
+            }
+            const float d2 = dx * dx + dy * dy;
+            const bool take = !empty && !__ballot(!(d2 > h0.z));  // h0.z: the squared opening radius
+            const bool open = !empty && !take && next == 0;
+            if (take) {
+                float tx, ty;
+                mp_accel<ORDER>(h0, h1, h2, h3, h4, h5, -dx, -dy, __builtin_amdgcn_rsqf(d2 + soft_sq), G, tx, ty);
+                if (KAHAN) {  // plain within a node run, compensated per run
+                    nfx += tx;
+                    nfy += ty;
+                } else {
+                    fx += tx;
+                    fy += ty;
+                }
+                if (COUNT) {
+                    cnt += live ? 1u : 0u;
+                    cnt_nodes += live ? 1u : 0u;
+                }
+            }
+            if (open) {
+                // leaf direct sum: the pairs go to a plain per-leaf sum of
+                // m_j d_j r^-3, scaled by G and compensated once per leaf
+                const uint32_t jend = min(nd.end, n);
+                if (COUNT) {  // pairs j != i of live lanes
+                    const uint32_t len = jend > nd.start ? jend - nd.start : 0u;
+                    leaf_trips += len;
+                    if (live) cnt += len - ((nd.start <= i && i < jend) ? 1u : 0u);
+                }
+                float sx = 0.f, sy = 0.f;
+                if (soft_sq > 0.0f)
+                    leaf_sum<false, PREC>(pxym, pho, nd.start, jend, p, ph, ic, per, dom, hw, hh, soft_sq, sx, sy);
+                else
+                    leaf_sum<true, PREC>(pxym, pho, nd.start, jend, p, ph, ic, per, dom, hw, hh, soft_sq, sx, sy);
+                if (KAHAN) {  // the node run since the last leaf and this leaf's sum
+                    kahan_add(fx, cx, fmaf(G, sx, nfx));
+                    kahan_add(fy, cy, fmaf(G, sy, nfy));
+                    nfx = 0.f;
+                    nfy = 0.f;
+                } else {
+                    fx = fmaf(G, sx, fx);
+                    fy = fmaf(G, sy, fy);
+                }
+            }
+            idx = (empty || take || open) ? after : idx + 1u;
+        }
+        if (COUNT) {
+            uint64_t c = cnt, cn = cnt_nodes;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                c += __shfl_xor(c, o, kWave);
+                cn += __shfl_xor(cn, o, kWave);
+            }
+            if (lane == 0) {
+                atomicAdd(inter, (unsigned long long)c);
+                atomicAdd(inter + 1, (unsigned long long)cn);
+                atomicAdd(inter + 2, (unsigned long long)visits);
+                atomicAdd(inter + 3, (unsigned long long)leaf_trips);
+            }
+        }
+        if (live) {
+            if (KAHAN) {
                 kahan_add(fx, cx, nfx);
                 kahan_add(fy, cy, nfy);
             }
@@ -644,11 +773,7 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
             }
         }
     }
-    if (PERSIST && (threadIdx.x & (kWave - 1)) == 0) {
-        const uint32_t waves = gridDim.x * (BS / kWave);
-        if (atomicAdd(&walk_q[kWalkQueues], 1u) == waves - 1u)
-            for (uint32_t k = 0; k <= kWalkQueues; ++k) walk_q[k] = 0u;
-    }
+    walk_retire<BS, PERSIST>(walk_q);
 }
 
 // Physics::naiveGravity (physics.cpp:392-434): O(N^2) with the softening
@@ -841,18 +966,30 @@ void launch_bh_p(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, uin
     const float theta_sq = p->theta * p->theta;
     const float soft_sq = p->softening * p->softening;
     auto *inter = reinterpret_cast<unsigned long long *>(c->b.interactions);
-    hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, ORDER, TEMP, PERSIST, PREC>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
-                       c->b.nodes, c->b.node_mp, c->b.start_count, c->b.sys_wave_off, c->nsys, c->b.sys_off,
-                       c->b.counters, c->n,
-                       p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt, domain_of(p),
-                       inter, c->b.node_tavg, s.temp, c->thermal_p.heat_conductivity, p->dt, walk_q, c->b.pho,
-                       c->b.node_home);
+    if constexpr (ORDER == 0)
+        hipLaunchKernelGGL((k_bh<BS, KAHAN, COUNT, TEMP, PERSIST, PREC>), dim3(nb), dim3(BS), 0, c->stream, c->b.pxym,
+                           c->b.nodes, c->b.start_count, c->b.sys_wave_off, c->nsys, c->b.sys_off, c->b.counters,
+                           c->n, p->G, theta_sq, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags, end_kick_half_dt,
+                           domain_of(p), inter, c->b.node_tavg, s.temp, c->thermal_p.heat_conductivity, p->dt, walk_q,
+                           c->b.pho);
+    else
+        hipLaunchKernelGGL((k_bh_mp<BS, KAHAN, COUNT, ORDER, PERSIST, PREC>), dim3(nb), dim3(BS), 0, c->stream,
+                           c->b.pxym, c->b.nodes, c->b.node_mp, c->b.start_count, c->b.sys_wave_off, c->nsys,
+                           c->b.sys_off, c->b.counters, c->n, p->G, soft_sq, s.ax, s.ay, s.vx, s.vy, s.flags,
+                           end_kick_half_dt, domain_of(p), inter, walk_q, c->b.pho, c->b.node_home);
 }
 
 template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP, bool PERSIST>
 void launch_bh_k(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt, uint32_t nb, uint32_t *walk_q) {
     if (c->precise) launch_bh_p<BS, KAHAN, COUNT, ORDER, TEMP, PERSIST, true>(c, p, end_kick_half_dt, nb, walk_q);
     else launch_bh_p<BS, KAHAN, COUNT, ORDER, TEMP, PERSIST, false>(c, p, end_kick_half_dt, nb, walk_q);
+}
+
+// the persistent form of a walk kernel, for the occupancy query
+template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
+const void *persistent_walk() {
+    if constexpr (ORDER == 0) return reinterpret_cast<const void *>(k_bh<BS, KAHAN, COUNT, TEMP, true, false>);
+    else return reinterpret_cast<const void *>(k_bh_mp<BS, KAHAN, COUNT, ORDER, true, false>);
 }
 
 template <int BS, bool KAHAN, bool COUNT, int ORDER, bool TEMP>
@@ -867,7 +1004,7 @@ void launch_bh_t(ge_ctx *c, const ge_step_params *p, float end_kick_half_dt) {
     static int per_cu = 0;  // one value per kernel instance
     if (per_cu == 0) {
         int v = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_bh<BS, KAHAN, COUNT, ORDER, TEMP, true, false>, BS, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, persistent_walk<BS, KAHAN, COUNT, ORDER, TEMP>(), BS, 0) !=
                 hipSuccess || v <= 0)
             v = 1;
         per_cu = v;
