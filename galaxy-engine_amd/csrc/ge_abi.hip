@@ -97,7 +97,7 @@ void free_all(ge_ctx *c) {
     dfree(b.ccreate);
     b.bb = nullptr;
     TopNodes &T = b.top;
-    dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.parent); dfree(T.child);
+    dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.slot); dfree(T.child);
     dfree(T.kind); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
     dfree(T.blist); dfree(T.blist_off); dfree(T.blist_cnt);
     dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
@@ -146,7 +146,7 @@ int alloc_all(ge_ctx *c) {
     T.cap = 2u * cap + 1024u;
     GE_HIP_CHECK(c, dalloc(T.px, T.cap)); GE_HIP_CHECK(c, dalloc(T.py, T.cap));
     GE_HIP_CHECK(c, dalloc(T.size, T.cap)); GE_HIP_CHECK(c, dalloc(T.start, T.cap));
-    GE_HIP_CHECK(c, dalloc(T.end, T.cap)); GE_HIP_CHECK(c, dalloc(T.parent, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.end, T.cap)); GE_HIP_CHECK(c, dalloc(T.slot, T.cap));
     GE_HIP_CHECK(c, dalloc(T.child, 4ull * T.cap)); GE_HIP_CHECK(c, dalloc(T.kind, T.cap));
     GE_HIP_CHECK(c, dalloc(T.pending2, T.cap));
     GE_HIP_CHECK(c, dalloc(T.offset, T.cap));
@@ -159,7 +159,7 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.start_level, (size_t)cap + 1));
     GE_HIP_CHECK(c, dalloc(b.scan_part, (size_t)cap / 1024 + 64));
     // zeroed top nodes read as "internal" (never emitted) until a build writes them
-    for (uint32_t *u : {T.start, T.end, T.parent, T.kind, T.pending2, T.offset, T.broot, T.blist_cnt,
+    for (uint32_t *u : {T.start, T.end, T.slot, T.kind, T.pending2, T.offset, T.broot, T.blist_cnt,
                         T.depth})
         GE_HIP_CHECK(c, hipMemset(u, 0, sizeof(uint32_t) * T.cap));
     GE_HIP_CHECK(c, hipMemset(T.child, 0xFF, sizeof(uint32_t) * 4ull * T.cap));

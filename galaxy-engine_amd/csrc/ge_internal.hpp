@@ -171,7 +171,8 @@ struct alignas(16) NodeRec {
 // Top part of the tree, built level by level (structure of arrays).
 struct TopNodes {
     float *px = nullptr, *py = nullptr, *size = nullptr;
-    uint32_t *start = nullptr, *end = nullptr, *parent = nullptr;
+    uint32_t *start = nullptr, *end = nullptr;
+    uint32_t *slot = nullptr;      // scratch per node: its entry in the top-mass pass's list
     uint32_t *child = nullptr;     // 4 per node
     uint32_t *kind = nullptr;      // 0 internal, 1 leaf, 2 wave subtree, 3 pending, 4 dead
     uint32_t *pending2 = nullptr;  // build stamp of nodes whose mass the block pass computed

@@ -306,7 +306,6 @@ __device__ void write_children(TopNodes &T, uint32_t node, float px, float py, f
             T.size[c] = half;
             T.start[c] = b[q];
             T.end[c] = b[q + 1];
-            T.parent[c] = node;
             T.depth[c] = depth + 1;
             T.kind[c] = kKindPending;
             T.child[4 * node + q] = c;
@@ -359,7 +358,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTabl
         const float *bb = sys_bb + 8ull * i;
         T.px[i] = bb[0]; T.py[i] = bb[1]; T.size[i] = bb[2];
         T.start[i] = sys_off[i]; T.end[i] = sys_off[i + 1];
-        T.parent[i] = kNone; T.kind[i] = kKindPending; T.depth[i] = 0;
+        T.kind[i] = kKindPending; T.depth[i] = 0;
     }
     if (blockIdx.x == 0) {
         const uint32_t t = threadIdx.x;
@@ -665,15 +664,25 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_block_mass(TopNodes T, uint
 
 // Masses of the remaining internal top nodes (above the workgroup subtrees,
 // or in subtrees whose list overflowed), deepest first, by one workgroup:
-// nodes of one depth are independent; barriers order the depths.
+// nodes of one depth are independent; barriers order the depths.  The nodes
+// are listed by depth (T.broot, each node's entry in T.slot).  Up to
+// kMassLds listed nodes: every child value a node needs from below the list
+// (leaves, wave subtrees, workgroup subtrees: final before this pass) is
+// loaded up front into the registers of the thread that owns the node, and
+// the listed nodes' own results stay in LDS, so a depth costs LDS reads and
+// one barrier.  Longer lists take the same sums through global memory.
 constexpr int kMassThreads = 1024;
 constexpr int kMaxTopDepth = 256;
+constexpr int kMassPer = 4;  // list entries per thread
+constexpr uint32_t kMassLds = kMassThreads * kMassPer;
+constexpr uint32_t kMassExt = 0xFFFFFFFEu;  // child value held in registers
 
 __global__ __launch_bounds__(kMassThreads) void k_tree_mass(TopNodes T, uint32_t *__restrict__ counters, TreeOut o) {
 #pragma clang fp contract(off)
     __shared__ uint32_t hist[kMaxTopDepth + 1];
     __shared__ uint32_t cursor[kMaxTopDepth];
     __shared__ uint32_t maxd;
+    __shared__ float lm[kMassLds], lx[kMassLds], ly[kMassLds];
     // without an overflowed list, every node left lies above the workgroup
     // subtrees, i.e. among the nodes the level phase allocated
     const uint32_t total = counters[kCntBlistOverflow] ? counters[kCntTopTotal]
@@ -707,12 +716,84 @@ __global__ __launch_bounds__(kMassThreads) void k_tree_mass(TopNodes T, uint32_t
     for (uint32_t t = threadIdx.x; t < total; t += kMassThreads) {
         if (T.kind[t] != kKindInternal || T.pending2[t] == stamp) continue;
         const uint32_t d = T.depth[t];
-        if (d < kMaxTopDepth) T.broot[atomicAdd(&cursor[d], 1u)] = t;
+        if (d < kMaxTopDepth) {
+            const uint32_t e = atomicAdd(&cursor[d], 1u);
+            T.broot[e] = t;
+            T.slot[t] = e;
+        }
     }
     __syncthreads();
+    const uint32_t len = hist[kMaxTopDepth];
+    if (len > kMassLds) {
+        for (int d = (int)maxd; d >= 0; --d) {
+            const uint32_t lo = hist[d], hi = hist[d + 1];
+            for (uint32_t i = lo + threadIdx.x; i < hi; i += kMassThreads) internal_mass(T, o, T.broot[i]);
+            __syncthreads();
+        }
+        return;
+    }
+    // children in the summation order q = 0, 2, 1, 3 of computeInternalMass:
+    // a listed node's entry, kMassExt (value in registers) or kNone
+    uint32_t ref[kMassPer][4], ot[kMassPer];
+    float em[kMassPer][4], ex[kMassPer][4], ey[kMassPer][4], sz[kMassPer];
+#pragma unroll
+    for (int k = 0; k < kMassPer; ++k) {
+        const uint32_t i = threadIdx.x + (uint32_t)k * kMassThreads;
+        ot[k] = kNone;
+        sz[k] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ref[k][q] = kNone;
+            em[k][q] = ex[k][q] = ey[k][q] = 0.0f;
+        }
+        if (i >= len) continue;
+        const uint32_t t = T.broot[i];
+        ot[k] = T.offset[t];
+        sz[k] = T.size[t];
+        const int order[4] = {0, 2, 1, 3};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t ch = T.child[4 * t + order[q]];
+            if (ch == kNone || ch >= T.cap) continue;
+            const uint32_t oc = T.offset[ch];
+            if (oc >= o.cap) continue;
+            if (T.kind[ch] == kKindInternal && T.pending2[ch] != stamp) {
+                ref[k][q] = T.slot[ch];
+            } else {
+                const NodeRec r = o.nodes[oc];
+                ref[k][q] = kMassExt;
+                em[k][q] = r.mass;
+                ex[k][q] = r.comx;
+                ey[k][q] = r.comy;
+            }
+        }
+    }
     for (int d = (int)maxd; d >= 0; --d) {
         const uint32_t lo = hist[d], hi = hist[d + 1];
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += kMassThreads) internal_mass(T, o, T.broot[i]);
+#pragma unroll
+        for (int k = 0; k < kMassPer; ++k) {
+            const uint32_t i = threadIdx.x + (uint32_t)k * kMassThreads;
+            if (i < lo || i >= hi) continue;
+            float gm = 0.0f, cx = 0.0f, cy = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t rq = ref[k][q];
+                if (rq == kNone) continue;
+                const bool ext = rq == kMassExt;
+                const uint32_t li = ext ? 0u : rq;  // an LDS entry in range either way
+                const float cm = ext ? em[k][q] : lm[li];
+                const float ccx = ext ? ex[k][q] : lx[li];
+                const float ccy = ext ? ey[k][q] : ly[li];
+                gm = gm + cm;
+                cx = cx + ccx * cm;
+                cy = cy + ccy * cm;
+            }
+            if (gm > 0) { cx = cx / gm; cy = cy / gm; }
+            lm[i] = gm;
+            lx[i] = cx;
+            ly[i] = cy;
+            if (ot[k] < o.cap) reinterpret_cast<float4 *>(o.nodes)[2 * ot[k]] = make_float4(cx, cy, gm, sz[k] * sz[k]);
+        }
         __syncthreads();
     }
 }
