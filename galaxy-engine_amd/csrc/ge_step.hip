@@ -349,7 +349,7 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
         const int32_t phx = __float_as_int(ph.z), phy = __float_as_int(ph.w);
         float ti = (TEMP && live) ? temp[i] : 0.0f;  // temp[i], moved by every accepted node
         // a tree that raised an error flag may hold unwritten nodes: do not walk it
-        const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth);
+        const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | kErrFrontier);
         const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
         uint32_t root = kNone, end = 0u;
         if (live) {
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(BS, 8) void k_bh_mp(const float4 *__restrict__ pxym
         const float4 ph = PREC ? pho[ic] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int32_t phx = __float_as_int(ph.z), phy = __float_as_int(ph.w);
         // a tree that raised an error flag may hold unwritten nodes: do not walk it
-        const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | kErrMultipole);
+        const uint32_t tree_err = counters[kCntError] & (kErrNodeCap | kErrDepth | kErrFrontier | kErrMultipole);
         const uint32_t node_count = tree_err ? 0u : __builtin_amdgcn_readfirstlane(counters[kCntNodeTotal]);
         const uint32_t root = __builtin_amdgcn_readfirstlane(start_prefix[s_lo]);
         const uint32_t end = __builtin_amdgcn_readfirstlane(min(start_prefix[s_hi], node_count));

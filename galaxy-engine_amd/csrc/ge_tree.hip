@@ -426,6 +426,12 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTabl
 // parallel (its LDS use does not depend on it)
 constexpr int kSubBlock = 512;
 constexpr int kSubWaves = kSubBlock / 64;
+// Nodes split per level of a workgroup subtree (a staged subtree splits at
+// most 4096 / 65 per level), and its internal nodes listed for the mass
+// pass (a longer list leaves them to the top-node pass).  Both keep the
+// workgroup's LDS under a third of the CU's, so three subtrees run per CU.
+constexpr uint32_t kSplitMax = 384;
+constexpr uint32_t kQuadListMax = 1024;
 
 // One workgroup per subtree root handed over by the level phase: the level
 // loop of that subtree with its frontier in LDS.
@@ -434,13 +440,13 @@ __global__ __launch_bounds__(kSubBlock) void k_tree_blocks(TopNodes T, StartTabl
                                                             uint32_t *__restrict__ counters) {
     __shared__ uint32_t front[2][kFrontMax];
     __shared__ uint32_t fcount[2];
-    __shared__ uint32_t blist[kBlockListMax];  // internal nodes, level by level
+    __shared__ uint32_t blist[kQuadListMax];  // internal nodes, level by level
     __shared__ uint32_t bcount, bbase;
-    // per frontier node of a level: the three inner partition points and the
-    // offset of its children among the level's children (kNone: not split)
-    __shared__ uint32_t sb[kFrontMax][3];
-    __shared__ uint32_t soff[kFrontMax];
-    __shared__ uint32_t lcount, lbase;
+    // per node split on a level: its frontier entry, the three inner
+    // partition points and the offset of its children among the level's
+    // children
+    __shared__ uint32_t sfi[kSplitMax], sb[kSplitMax][3], soff[kSplitMax];
+    __shared__ uint32_t nsplit, scount, lcount, lbase;
     __shared__ float sx[kBlockSubMax], sy[kBlockSubMax];  // the subtree's coordinates
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
@@ -467,6 +473,7 @@ __global__ __launch_bounds__(kSubBlock) void k_tree_blocks(TopNodes T, StartTabl
             fcount[1] = 0;
             bcount = 0;
             lcount = 0;
+            nsplit = 0;
         }
         __syncthreads();
         int cur = 0;
@@ -480,15 +487,19 @@ __global__ __launch_bounds__(kSubBlock) void k_tree_blocks(TopNodes T, StartTabl
                 const uint32_t node = front[cur][i];
                 const float px = T.px[node], py = T.py[node], size = T.size[node];
                 const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
-                if (finish_small(T, st, xs, ys, node, px, py, size, s, e, depth, counters + kCntError, off)) {
-                    if (lane == 0) soff[i] = kNone;
+                if (finish_small(T, st, xs, ys, node, px, py, size, s, e, depth, counters + kCntError, off))
                     continue;
-                }
                 uint32_t b[5];
                 wave_split(xs, ys, px, py, size, s, e, b, off);
                 if (lane == 0) {
-                    sb[i][0] = b[1]; sb[i][1] = b[2]; sb[i][2] = b[3];
-                    soff[i] = atomicAdd(&lcount, child_count(b));
+                    const uint32_t k = atomicAdd(&nsplit, 1u);
+                    if (k < kSplitMax) {
+                        sfi[k] = i;
+                        sb[k][0] = b[1]; sb[k][1] = b[2]; sb[k][2] = b[3];
+                        soff[k] = atomicAdd(&lcount, child_count(b));
+                    } else {
+                        atomicOr(&counters[kCntError], kErrFrontier);
+                    }
                 }
             }
             __syncthreads();
@@ -502,22 +513,24 @@ __global__ __launch_bounds__(kSubBlock) void k_tree_blocks(TopNodes T, StartTabl
                 lbase = ok ? first : kNone;
                 fcount[cur ^ 1] = ok ? k : 0u;
                 lcount = 0;
+                scount = min(nsplit, kSplitMax);
+                nsplit = 0;
             }
             __syncthreads();
             // B: the children records, one thread per split node
             const uint32_t base = lbase;
+            const uint32_t ns = scount;
             if (base != kNone) {
-                for (uint32_t i = threadIdx.x; i < nf; i += kSubBlock) {
-                    const uint32_t o = soff[i];
-                    if (o == kNone) continue;
-                    const uint32_t node = front[cur][i];
+                for (uint32_t k = threadIdx.x; k < ns; k += kSubBlock) {
+                    const uint32_t o = soff[k];
+                    const uint32_t node = front[cur][sfi[k]];
                     const float px = T.px[node], py = T.py[node], size = T.size[node];
                     const uint32_t s = T.start[node], e = T.end[node], depth = T.depth[node];
-                    const uint32_t b[5] = {s, sb[i][0], sb[i][1], sb[i][2], e};
+                    const uint32_t b[5] = {s, sb[k][0], sb[k][1], sb[k][2], e};
                     register_start(st, s, depth);
                     write_children(T, node, px, py, size, depth, b, base + o, &front[cur ^ 1][o]);
                     const uint32_t li = atomicAdd(&bcount, 1u);
-                    if (li < kBlockListMax) blist[li] = node;
+                    if (li < kQuadListMax) blist[li] = node;
                 }
             }
             __syncthreads();
@@ -528,14 +541,14 @@ __global__ __launch_bounds__(kSubBlock) void k_tree_blocks(TopNodes T, StartTabl
         // the last mass pass takes these nodes instead)
         const uint32_t nl = bcount;
         if (threadIdx.x == 0) {
-            bbase = nl <= kBlockListMax ? atomicAdd(&counters[kCntBlistAlloc], nl) : kNone;
-            const bool fits = nl <= kBlockListMax && bbase + nl <= T.cap;
+            bbase = nl <= kQuadListMax ? atomicAdd(&counters[kCntBlistAlloc], nl) : kNone;
+            const bool fits = nl <= kQuadListMax && bbase + nl <= T.cap;
             T.blist_off[r] = fits ? bbase : 0u;
             T.blist_cnt[r] = fits ? nl : kNone;
             if (!fits) counters[kCntBlistOverflow] = 1u;
         }
         __syncthreads();
-        if (nl <= kBlockListMax && bbase + nl <= T.cap)
+        if (nl <= kQuadListMax && bbase + nl <= T.cap)
             for (uint32_t i = threadIdx.x; i < nl; i += kSubBlock) T.blist[bbase + i] = blist[i];
         __syncthreads();
     }
