@@ -97,10 +97,10 @@ void free_all(ge_ctx *c) {
     dfree(b.ccreate);
     b.bb = nullptr;
     TopNodes &T = b.top;
-    dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.slot); dfree(T.child);
+    dfree(T.px); dfree(T.py); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.slot); dfree(T.exact); dfree(T.child);
     dfree(T.kind); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.depth);
     dfree(T.blist); dfree(T.blist_off); dfree(T.blist_cnt);
-    dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
+    dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part); dfree(b.kstart); dfree(b.kdirty);
     merger_free(c);
     dfree(b.node_temp); dfree(b.node_tavg); dfree(b.node_parent); dfree(b.node_arrive); dfree(b.materials);
     dfree(b.cons); dfree(b.cons_tmp); dfree(b.cidx); dfree(b.cidx_tmp); dfree(b.ccolor); dfree(b.ccolor_tmp);
@@ -147,6 +147,7 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(T.px, T.cap)); GE_HIP_CHECK(c, dalloc(T.py, T.cap));
     GE_HIP_CHECK(c, dalloc(T.size, T.cap)); GE_HIP_CHECK(c, dalloc(T.start, T.cap));
     GE_HIP_CHECK(c, dalloc(T.end, T.cap)); GE_HIP_CHECK(c, dalloc(T.slot, T.cap));
+    GE_HIP_CHECK(c, dalloc(T.exact, T.cap));
     GE_HIP_CHECK(c, dalloc(T.child, 4ull * T.cap)); GE_HIP_CHECK(c, dalloc(T.kind, T.cap));
     GE_HIP_CHECK(c, dalloc(T.pending2, T.cap));
     GE_HIP_CHECK(c, dalloc(T.offset, T.cap));
@@ -158,8 +159,10 @@ int alloc_all(ge_ctx *c) {
     GE_HIP_CHECK(c, dalloc(b.start_count, (size_t)cap + 1));
     GE_HIP_CHECK(c, dalloc(b.start_level, (size_t)cap + 1));
     GE_HIP_CHECK(c, dalloc(b.scan_part, (size_t)cap / 1024 + 64));
+    GE_HIP_CHECK(c, dalloc(b.kstart, kKeyStartEntries));
+    GE_HIP_CHECK(c, dalloc(b.kdirty, kKeyDirtyEntries));
     // zeroed top nodes read as "internal" (never emitted) until a build writes them
-    for (uint32_t *u : {T.start, T.end, T.slot, T.kind, T.pending2, T.offset, T.broot, T.blist_cnt,
+    for (uint32_t *u : {T.start, T.end, T.slot, T.exact, T.kind, T.pending2, T.offset, T.broot, T.blist_cnt,
                         T.depth})
         GE_HIP_CHECK(c, hipMemset(u, 0, sizeof(uint32_t) * T.cap));
     GE_HIP_CHECK(c, hipMemset(T.child, 0xFF, sizeof(uint32_t) * 4ull * T.cap));
@@ -445,7 +448,7 @@ int frame(ge_ctx *c, const ge_step_params *p, float dt_start, bool count_inter) 
         { PhaseScope s(c, GE_PHASE_GATHER); launch_gather(c); }
         if (!p->naive) {
             PhaseScope s(c, GE_PHASE_TREE);
-            launch_tree_build(c);
+            launch_tree_build(c, true);
             c->have_tree = true;
         }
     }

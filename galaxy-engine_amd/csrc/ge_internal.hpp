@@ -32,6 +32,15 @@ constexpr int kTopLevels = 16;         // launches of the grid-wide level phase
 constexpr uint32_t kBlockSubMax = 4096;
 constexpr int kMaxDepth = 48;          // wave-subtree recursion depth (LDS stack)
 constexpr int kBlockLevelsMax = 160;   // local levels a workgroup subtree may take
+// Key-run splits of the level phase (one system): for levels L = 1..kKeyLevels
+// the first particle (+1, 0 = none) of each run of equal L-digit Morton
+// prefixes, and for levels L < kKeyLevels a flag per prefix whose particles'
+// float quadrants disagree with their key digits somewhere in it.
+constexpr int kKeyLevels = 10;
+constexpr uint32_t kKeyStartEntries = ((1u << (2 * (kKeyLevels + 1))) - 4u) / 3u;  // sum of 4^L, L = 1..10
+constexpr uint32_t kKeyDirtyEntries = ((1u << (2 * kKeyLevels)) - 1u) / 3u;        // sum of 4^L, L = 0..9
+__host__ __device__ constexpr uint32_t key_start_base(int L) { return ((1u << (2 * L)) - 4u) / 3u; }
+__host__ __device__ constexpr uint32_t key_dirty_base(int L) { return ((1u << (2 * L)) - 1u) / 3u; }
 
 // device error flag bits
 enum : uint32_t {
@@ -173,6 +182,7 @@ struct TopNodes {
     float *px = nullptr, *py = nullptr, *size = nullptr;
     uint32_t *start = nullptr, *end = nullptr;
     uint32_t *slot = nullptr;      // scratch per node: its entry in the top-mass pass's list
+    uint32_t *exact = nullptr;     // level phase: 1 if the node's range is a Morton-prefix run
     uint32_t *child = nullptr;     // 4 per node
     uint32_t *kind = nullptr;      // 0 internal, 1 leaf, 2 wave subtree, 3 pending, 4 dead
     uint32_t *pending2 = nullptr;  // build stamp of nodes whose mass the block pass computed
@@ -229,6 +239,8 @@ struct Buffers {
     uint32_t max_chunks = 0;
     TopNodes top;
     uint32_t *start_count = nullptr;  // per particle: nodes whose range starts there (then its prefix)
+    uint32_t *kstart = nullptr;       // key-run starts of the top levels (kKeyStartEntries)
+    uint8_t *kdirty = nullptr;        // prefixes whose float quadrants disagree (kKeyDirtyEntries)
     uint32_t *start_level = nullptr;  // per particle: shallowest level of such a node
     uint32_t *scan_part = nullptr;    // block sums of the node-start scan
     NodeRec *nodes = nullptr;
@@ -366,7 +378,10 @@ void launch_integrate_end(ge_ctx *c, float dt);
 void launch_sys_of(ge_ctx *c);
 void launch_compact(ge_ctx *c, uint32_t n_kept);
 // tree.hip
-void launch_tree_build(ge_ctx *c);
+// sorted_words: the packed sort words in b.sort_a are the sort of the
+// current particle order (the frame's own sort), so the level phase may split
+// Morton-prefix runs without searching
+void launch_tree_build(ge_ctx *c, bool sorted_words = false);
 // sph.hip
 int sph_new_grid(ge_ctx *c, const ge_step_params *p);
 int sph_pcisph(ge_ctx *c, const ge_step_params *p, float end_kick_dt);

@@ -34,6 +34,8 @@
 //   7. k_tree_mass: one workgroup finishes the rest (the nodes above the
 //      workgroup subtrees), deepest first.  Both sum children in the order
 //      q = 0, 2, 1, 3 of computeInternalMass.
+#include <algorithm>
+
 #include "ge_internal.hpp"
 #include "ge_tree_common.hpp"
 
@@ -294,8 +296,9 @@ __device__ __forceinline__ uint32_t child_count(const uint32_t b[5]) {
 }
 
 // Lane 0: an internal node and its children at indices first.. (q order).
+// exact: the children's ranges are Morton-prefix runs (key-run split).
 __device__ void write_children(TopNodes &T, uint32_t node, float px, float py, float size, uint32_t depth,
-                               const uint32_t b[5], uint32_t first, uint32_t *out_list) {
+                               const uint32_t b[5], uint32_t first, uint32_t *out_list, uint32_t exact = 0u) {
     T.kind[node] = kKindInternal;
     const float half = size * 0.5f;
     uint32_t c = first, j = 0;
@@ -307,6 +310,7 @@ __device__ void write_children(TopNodes &T, uint32_t node, float px, float py, f
             T.start[c] = b[q];
             T.end[c] = b[q + 1];
             T.depth[c] = depth + 1;
+            T.exact[c] = exact;
             T.kind[c] = kKindPending;
             T.child[4 * node + q] = c;
             if (out_list) out_list[j++] = c;
@@ -346,19 +350,38 @@ __device__ __forceinline__ void push_block_root(TopNodes &T, uint32_t *counters,
     else atomicOr(&counters[kCntError], kErrFrontier);
 }
 
+// Key-run splits: the data a node's split needs when its range is a run of
+// equal Morton prefixes (one system)
+struct KeySplit {
+    const uint64_t *words;  // sorted packed words: key << idx_bits | slot
+    uint32_t *start;        // kKeyStartEntries
+    uint8_t *dirty;         // kKeyDirtyEntries
+    int idx_bits;
+    int on;                 // 0: every split searches
+};
+
+__device__ __forceinline__ uint64_t word_key(const KeySplit &k, uint32_t i) {
+    return (k.words[i] >> k.idx_bits) & ((1ull << kMortonBits) - 1ull);
+}
+
 __global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTable st, const float *__restrict__ sys_bb,
                                                            const uint32_t *__restrict__ sys_off, uint32_t nsys,
-                                                           uint32_t n, uint32_t *__restrict__ counters) {
+                                                           uint32_t n, uint32_t *__restrict__ counters, KeySplit ks) {
     const uint32_t i = blockIdx.x * kTreeBlock + threadIdx.x;
     if (i <= n) {
         st.count[i] = 0;
         st.level[i] = 0xFFFFFFFFu;
+    }
+    if (ks.on) {
+        if (i < kKeyStartEntries) ks.start[i] = 0u;
+        if (i < kKeyDirtyEntries) ks.dirty[i] = 0u;
     }
     if (i < nsys) {  // the root of system i: Quadtree::root (quadtree.cpp:90-107)
         const float *bb = sys_bb + 8ull * i;
         T.px[i] = bb[0]; T.py[i] = bb[1]; T.size[i] = bb[2];
         T.start[i] = sys_off[i]; T.end[i] = sys_off[i + 1];
         T.kind[i] = kKindPending; T.depth[i] = 0;
+        T.exact[i] = ks.on ? 1u : 0u;
     }
     if (blockIdx.x == 0) {
         const uint32_t t = threadIdx.x;
@@ -377,10 +400,104 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTabl
     }
 }
 
+// Key runs of the top levels, one thread per sorted particle i (one
+// system).  i starts the run of its L-digit prefix for every level L deeper
+// than the digits it shares with particle i - 1; it records that for
+// L = 1..kKeyLevels.  It also follows its key's path down from the root
+// square with subGridMaker's float arithmetic (mid = pos + size / 2, child
+// pos = pos + size / 2 on the upper side) and flags the prefix of every
+// level whose quadrant test (x < mid.x, y < mid.y) disagrees with its key
+// digit.  A node whose range is a prefix run and whose prefix is not flagged
+// has every particle's quadrant equal to its key digit, so the predicates are
+// partitioned along the sorted range and partition_point returns the
+// children's run starts.
+__global__ __launch_bounds__(kTreeBlock) void k_tree_keys(KeySplit ks, const float *__restrict__ x,
+                                                          const float *__restrict__ y, uint32_t n,
+                                                          const float *__restrict__ sys_bb) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kTreeBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = word_key(ks, i);
+    int shared = -1;  // leading digits equal to particle i - 1's
+    if (i > 0) {
+        const uint64_t d = k ^ word_key(ks, i - 1);
+        shared = d ? (__clzll(d) - (64 - kMortonBits)) / 2 : kMortonBits / 2;
+    }
+    for (int L = shared + 1 < 1 ? 1 : shared + 1; L <= kKeyLevels; ++L)
+        ks.start[key_start_base(L) + (uint32_t)(k >> (kMortonBits - 2 * L))] = i + 1u;
+    const float px0 = x[i], py0 = y[i];
+    float px = sys_bb[0], py = sys_bb[1], size = sys_bb[2];
+    for (int L = 0; L < kKeyLevels; ++L) {
+        const float half = size * 0.5f;
+        const uint32_t dig = (uint32_t)(k >> (kMortonBits - 2 - 2 * L)) & 3u;
+        const uint32_t qx = (px0 < px + half) ? 0u : 1u;
+        const uint32_t qy = (py0 < py + half) ? 0u : 2u;
+        if ((qx | qy) != dig)
+            ks.dirty[key_dirty_base(L) + (uint32_t)(L ? k >> (kMortonBits - 2 * L) : 0ull)] = 1u;
+        px = px + ((dig & 1u) ? half : 0.0f);
+        py = py + ((dig & 2u) ? half : 0.0f);
+        size = half;
+    }
+}
+
+// Probe t (0-based) of partition_point's halving search over [first, last)
+// when every probe below `guess` tests true and every other false; false if
+// that search takes fewer probes.
+__device__ __forceinline__ bool guessed_probe(uint32_t first, uint32_t last, uint32_t guess, uint32_t t,
+                                              uint32_t &pos) {
+    uint32_t f = first, len = last - first;
+    for (uint32_t k = 0; len > 0; ++k) {
+        const uint32_t half = len >> 1, m = f + half;
+        if (k == t) {
+            pos = m;
+            return true;
+        }
+        if (m < guess) {
+            f = m + 1;
+            len = len - half - 1;
+        } else {
+            len = half;
+        }
+    }
+    return false;
+}
+
+// A prefix run holding a particle whose quadrant disagrees with its key
+// digit: the partition points are still the children's run starts b[1..3]
+// whenever every probe the three searches would take on that assumption
+// (the y search over [b0, b4), the x searches over [b0, b2) and [b2, b4))
+// tests as assumed, because the searches then take exactly those paths.
+// The wave checks all probes in one round; false means a probe disagrees and
+// the node must be searched.
+__device__ __forceinline__ bool verify_guess(const float *__restrict__ x, const float *__restrict__ y, float px,
+                                             float py, float size, const uint32_t b[5]) {
+#pragma clang fp contract(off)
+    const float half = size * 0.5f;
+    const float midx = px + half, midy = py + half;
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t c = lane_id() + 64u * (uint32_t)r;  // check c of 96: search c / 32, probe c % 32
+        const uint32_t sidx = c >> 5, t = c & 31u;
+        if (sidx > 2u) continue;
+        // search 0: y over [b0, b4) toward b2; 1: x over [b0, b2) toward b1;
+        // 2: x over [b2, b4) toward b3
+        const uint32_t lo = sidx == 2u ? b[2] : b[0];
+        const uint32_t hi = sidx == 1u ? b[2] : b[4];
+        const uint32_t g = sidx == 0u ? b[2] : (sidx == 1u ? b[1] : b[3]);
+        uint32_t pos;
+        if (!guessed_probe(lo, hi, g, t, pos)) continue;
+        const bool pred = sidx == 0u ? (y[pos] < midy) : (x[pos] < midx);
+        if (pred != (pos < g)) ok = false;
+    }
+    // ranges hold fewer than 2^28 particles, so no search takes more than 32 probes
+    return __ballot(!ok) == 0ull;
+}
+
 __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTable st, const float *__restrict__ x,
                                                            const float *__restrict__ y,
                                                            uint32_t *__restrict__ counters, int L,
-                                                           int last_level) {
+                                                           int last_level, KeySplit ks) {
     const uint32_t lane = lane_id();
     const uint32_t base = counters[kCntLevelBase0 + L];
     uint32_t cnt = counters[kCntLevel0 + L];
@@ -405,7 +522,21 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTabl
             continue;
         }
         uint32_t b[5];
-        wave_split(x, y, px, py, size, s, e, b);
+        // a prefix run whose quadrants all agree with its key digits splits
+        // at its children's run starts; any other node searches
+        bool keyed = false;
+        if (ks.on && L < kKeyLevels && T.exact[node]) {
+            const uint32_t P = L ? (uint32_t)(word_key(ks, s) >> (kMortonBits - 2 * L)) : 0u;
+            const uint32_t *cs = ks.start + key_start_base(L + 1) + 4u * P;
+            const uint32_t c1 = cs[1], c2 = cs[2], c3 = cs[3];
+            b[0] = s;
+            b[4] = e;
+            b[2] = c2 ? c2 - 1u : (c3 ? c3 - 1u : e);
+            b[1] = c1 ? c1 - 1u : b[2];
+            b[3] = c3 ? c3 - 1u : e;
+            keyed = !ks.dirty[key_dirty_base(L) + P] || verify_guess(x, y, px, py, size, b);
+        }
+        if (!keyed) wave_split(x, y, px, py, size, s, e, b);
         if (lane == 0) {
             const uint32_t k = child_count(b);
             const uint32_t off = atomicAdd(&counters[kCntLevel0 + L + 1], k);
@@ -416,7 +547,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTabl
                 push_block_root(T, counters, node);
             } else {
                 register_start(st, s, (uint32_t)L);
-                write_children(T, node, px, py, size, (uint32_t)L, b, first, nullptr);
+                write_children(T, node, px, py, size, (uint32_t)L, b, first, nullptr, keyed ? 1u : 0u);
             }
         }
     }
@@ -813,13 +944,20 @@ __global__ __launch_bounds__(kMassThreads) void k_tree_mass(TopNodes T, uint32_t
 
 }  // namespace
 
-void launch_tree_build(ge_ctx *c) {
+void launch_tree_build(ge_ctx *c, bool sorted_words) {
     const State &s = c->b.st[c->b.cur];
     TopNodes &T = c->b.top;
     const uint64_t n = c->n;
     const StartTable st{c->b.start_count, c->b.start_level, c->leaf_cap};
-    hipLaunchKernelGGL(k_tree_clear, dim3(blocks_for(n + 1, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, T, st,
-                       c->b.sys_bb, c->b.sys_off, c->nsys, c->n, c->b.counters);
+    // key-run splits for a single system (a batch's systems search)
+    const KeySplit ks{c->b.sort_a, c->b.kstart, c->b.kdirty, c->idx_bits,
+                      (sorted_words && c->nsys == 1) ? 1 : 0};
+    const uint64_t clear_n = ks.on ? std::max<uint64_t>(n + 1, kKeyStartEntries) : n + 1;
+    hipLaunchKernelGGL(k_tree_clear, dim3(blocks_for(clear_n, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, T, st,
+                       c->b.sys_bb, c->b.sys_off, c->nsys, c->n, c->b.counters, ks);
+    if (ks.on && n > 0)
+        hipLaunchKernelGGL(k_tree_keys, dim3(blocks_for(n, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, ks, s.x, s.y,
+                           c->n, c->b.sys_bb);
     // level L+1 only holds children of nodes above kBlockSubMax particles
     const uint64_t big_children = 4 * (n / (kBlockSubMax + 1)) + 4;
     for (int L = 0; L < kTopLevels; ++L) {
@@ -828,7 +966,7 @@ void launch_tree_build(ge_ctx *c) {
         if (L > 0 && maxn > big_children) maxn = big_children;
         if (maxn > n) maxn = n;
         hipLaunchKernelGGL(k_tree_level, dim3(wave_grid(maxn)), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y,
-                           c->b.counters, L, kTopLevels - 1);
+                           c->b.counters, L, kTopLevels - 1, ks);
     }
     // roots of workgroup subtrees each hold more than 64 particles
     uint64_t roots = n / (kWaveSubMax + 1) + c->nsys;
