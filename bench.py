@@ -868,7 +868,9 @@ def main():
                      "traffic_source": "profiles/bh_walk_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE per walk "
                                        "launch at this block size, calibrated on a copy kernel (not measured "
                                        "in this run)",
-                     "kernel": "k_bh (tree walk)", "kernel_ms": head["walk_ms"],
+                     "kernel": ("k_bh_mp (accuracy-mode tree walk)" if head["params"]["accuracy"]
+                                else "k_bh (reference tree walk)"),
+                     "kernel_ms": head["walk_ms"],
                      "interactions": inter, "flops_per_interaction": FLOPS_PAIR,
                      "expression": {"achieved": head["walk_tflops_expression"],
                                     "frac": head["walk_tflops_expression"] / PEAK_FP32_TFLOPS,
