@@ -26,16 +26,18 @@
 // disagree about a midline); a truncated series about such a node diverges
 // for targets closer than that particle.  With A_n = C_n / n! (C_n the
 // central moments), T_l = A3_iil, U_lm = A4_iilm and W = A4_iijj, the record
-// per node is 6 float4, each moment scaled by the constant of the derivative
-// term it feeds:
+// per node is 6 float4 holding the coefficients of the gradient polynomials
+// the walk evaluates (g2 = -6 A2 d + 9 T, g3 = -45 A3:dd + 180 U d,
+// g4 = -420 A4:ddd) and the constants of the series:
 //   0: (comx, comy, opening radius squared, mass)
-//   1: (-15 A2xx, -15 A2xy, -15 A2yy, 3 tr A2)
-//   2: -105 A3: (xxx, xxy, xyy, yyy)
-//   3: (45 T_x, 45 T_y, -45 W, 0)
-//   4: -945 A4: (xxxx, xxxy, xxyy, xyyy)
-//   5: (-945 A4yyyy, 630 U_xx, 630 U_xy, 630 U_yy)
+//   1: (-6 A2xx, -6 A2xy, -6 A2yy, 3 tr A2)
+//   2: (9 T_x, 9 T_y, -45 W, 180 U_xy)
+//   3: -45 A3: (xxx, xxy, xyy, yyy)
+//   4: (180 U_xx, 180 U_yy, -420 A4xxxx, -420 A4xxxy)
+//   5: -420 A4: (xxyy, xyyy, yyyy), 0
 // The walk evaluates the Taylor expansion of the softened kernel
-// (|d|^2 + eps^2)^-1/2 to the order asked for from these.
+// (|d|^2 + eps^2)^-1/2 to the order asked for from these; the quadrupole
+// reads records 0-1, the octupole 0-3 (without W and U).
 #include "ge_internal.hpp"
 
 namespace ge {
@@ -230,18 +232,16 @@ __device__ void finish_record(uint32_t k, const NodeRec &r, const float4 g, cons
     const double c4xxxx = C[mono(4, 0)], c4xxxy = C[mono(3, 1)], c4xxyy = C[mono(2, 2)];
     const double c4xyyy = C[mono(1, 3)], c4yyyy = C[mono(0, 4)];
     const double uxx = c4xxxx + c4xxyy, uxy = c4xxxy + c4xyyy, uyy = c4xxyy + c4yyyy;
-    // each moment is stored pre-multiplied by the constant of its leading
-    // derivative term, so the walk evaluates the expansion as two short
-    // polynomials in r^2 = 1 / (|d|^2 + eps^2) with no per-node scaling
+    // each moment is stored as the coefficient of the gradient polynomial it
+    // enters, so the walk applies no per-node scaling
     float4 *o = mp + 6ull * k;
     o[0] = make_float4(comx, comy, lim * lim, (float)M);
-    o[1] = make_float4((float)(-7.5 * c2xx), (float)(-7.5 * c2xy), (float)(-7.5 * c2yy), (float)(1.5 * (c2xx + c2yy)));
-    o[2] = make_float4((float)(-17.5 * c3xxx), (float)(-17.5 * c3xxy), (float)(-17.5 * c3xyy), (float)(-17.5 * c3yyy));
-    o[3] = make_float4((float)(7.5 * (c3xxx + c3xyy)), (float)(7.5 * (c3xxy + c3yyy)), (float)(-1.875 * (uxx + uyy)),
-                       0.0f);
-    o[4] = make_float4((float)(-39.375 * c4xxxx), (float)(-39.375 * c4xxxy), (float)(-39.375 * c4xxyy),
-                       (float)(-39.375 * c4xyyy));
-    o[5] = make_float4((float)(-39.375 * c4yyyy), (float)(26.25 * uxx), (float)(26.25 * uxy), (float)(26.25 * uyy));
+    o[1] = make_float4((float)(-3.0 * c2xx), (float)(-3.0 * c2xy), (float)(-3.0 * c2yy), (float)(1.5 * (c2xx + c2yy)));
+    o[2] = make_float4((float)(1.5 * (c3xxx + c3xyy)), (float)(1.5 * (c3xxy + c3yyy)), (float)(-1.875 * (uxx + uyy)),
+                       (float)(7.5 * uxy));
+    o[3] = make_float4((float)(-7.5 * c3xxx), (float)(-7.5 * c3xxy), (float)(-7.5 * c3xyy), (float)(-7.5 * c3yyy));
+    o[4] = make_float4((float)(7.5 * uxx), (float)(7.5 * uyy), (float)(-17.5 * c4xxxx), (float)(-17.5 * c4xxxy));
+    o[5] = make_float4((float)(-17.5 * c4xxyy), (float)(-17.5 * c4xyyy), (float)(-17.5 * c4yyyy), 0.0f);
 }
 
 __global__ void k_mp_reset(uint32_t *__restrict__ cnt) {

@@ -506,59 +506,66 @@ __global__ __launch_bounds__(BS, 8) void k_bh(const float4 *__restrict__ pxym,
 // no counterpart in the reference): the Taylor expansion of the softened
 // kernel about the node's FP64 centre of mass up to the quadrupole (2),
 // octupole (3) or hexadecapole (4) term.  With d = target - COM,
-// R^2 = |d|^2 + eps^2, r = 1/R, A_n = C_n / n! and the traces T, U, W of A3
-// and A4:
-//   a = G (S d + V)
-//   S = -M r^3 + 3 tr A2 r^5 + (-15 d.A2.d + 45 T.d - 45 W) r^7
-//       + (-105 A3:ddd + 630 U:dd) r^9 - 945 A4:dddd r^11
-//   V = (6 A2 d - 9 T) r^5 + (45 A3:dd - 180 U d) r^7 + 420 A4:ddd r^9
-// (the Cartesian Taylor series, D_k = (-1)^k (2k-1)!! r^(2k+1)).  The node
-// record holds each moment times the constant it meets first in S, so S is
-// a Horner polynomial in r^2 and V is one more, with the ratios of the
-// remaining constants folded into three multipliers.  qx, qy = d; inv = r.
+// R^2 = |d|^2 + eps^2, r = 1/R, k = r^2, A_n = C_n / n! and the traces T, U,
+// W of A3 and A4:
+//   a = G r^3 (S d - k V)
+//   S = -M + k (3 tr A2 + k (F2 + k (F3 + k F4)))
+//   V = g2 + k (g3 + k g4)
+//   g2 = -6 A2 d + 9 T,  g3 = -45 A3:dd + 180 U d,  g4 = -420 A4:ddd
+//   F2 = -15 A2:dd + 45 T.d - 45 W = 2.5 d.(g2 + 9 T) - 45 W
+//   F3 = -105 A3:ddd + 630 U:dd = (7/3) (d.g3 + d.(180 U d) / 2)
+//   F4 = -945 A4:dddd = (9/4) d.g4
+// (the Cartesian Taylor series, D_n = (-1)^n (2n-1)!! r^(2n+1); the scalar
+// parts F_n follow from the gradients g_n by Euler's rule for homogeneous
+// polynomials).  The node record holds the coefficients of g2, g3 and g4
+// and the constants (multipole.hip's finish_record), so the evaluation is
+// three short polynomials and two Horner chains in k.  qx, qy = d; inv = r.
 template <int ORDER>
 __device__ __forceinline__ void mp_accel(float4 h0, float4 h1, float4 h2, float4 h3, float4 h4, float4 h5, float qx,
                                          float qy, float inv, float G, float &tx, float &ty) {
-    const float r2 = inv * inv;
-    const float gr3 = G * (inv * r2);
-    const float p2x = h1.x * qx + h1.y * qy, p2y = h1.y * qx + h1.z * qy;
-    if (ORDER >= 4) {
-        const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
-        const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
-        const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
-        const float pux = h5.y * qx + h5.z * qy, puy = h5.z * qx + h5.w * qy;
-        const float s3u = qx * (p3x + pux) + qy * (p3y + puy);
-        const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y) + h3.z;
-        const float c0 = qxx * qx, c3 = qyy * qy, c1 = (3.0f * qxx) * qy, c2 = (3.0f * qyy) * qx;
-        const float p4x = h4.x * c0 + h4.y * c1 + h4.z * c2 + h4.w * c3;
-        const float p4y = h4.y * c0 + h4.z * c1 + h4.w * c2 + h5.x * c3;
-        const float s4 = qx * p4x + qy * p4y;
-        const float sc = gr3 * (r2 * (r2 * (r2 * (r2 * s4 + s3u) + c7) + h1.w) - h0.w);
-        const float e = (28.0f / 9.0f) * r2, f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
-        const float ox = (2.0f * p2x + h3.x) + f * ((3.0f * p3x + 2.0f * pux) + e * p4x);
-        const float oy = (2.0f * p2y + h3.y) + f * ((3.0f * p3y + 2.0f * puy) + e * p4y);
-        tx = qx * sc + g5 * ox;
-        ty = qy * sc + g5 * oy;
-    } else if (ORDER == 3) {
-        // the same series without the fourth-order moments (A4, U, W)
-        const float qxx = qx * qx, qyy = qy * qy, qxy2 = (qx + qx) * qy;
-        const float p3x = h2.x * qxx + h2.y * qxy2 + h2.z * qyy;
-        const float p3y = h2.y * qxx + h2.z * qxy2 + h2.w * qyy;
-        const float s3 = qx * p3x + qy * p3y;
-        const float c7 = qx * (p2x + h3.x) + qy * (p2y + h3.y);
-        const float sc = gr3 * (r2 * (r2 * (r2 * s3 + c7) + h1.w) - h0.w);
-        const float f = (5.0f / 7.0f) * r2, g5 = gr3 * (-0.2f * r2);
-        const float ox = (2.0f * p2x + h3.x) + f * (3.0f * p3x);
-        const float oy = (2.0f * p2y + h3.y) + f * (3.0f * p3y);
-        tx = qx * sc + g5 * ox;
-        ty = qy * sc + g5 * oy;
+    const float k = inv * inv;
+    const float A = G * (inv * k);
+    float S, vx, vy;
+    if (ORDER == 2) {
+        const float gx = fmaf(h1.x, qx, h1.y * qy), gy = fmaf(h1.y, qx, h1.z * qy);
+        const float f2 = 2.5f * fmaf(qx, gx, qy * gy);
+        S = fmaf(k, fmaf(k, f2, h1.w), -h0.w);
+        vx = gx;
+        vy = gy;
     } else {
-        const float s2 = qx * p2x + qy * p2y;
-        const float sc = gr3 * (r2 * (r2 * s2 + h1.w) - h0.w);
-        const float g5 = gr3 * (-0.4f * r2);
-        tx = qx * sc + g5 * p2x;
-        ty = qy * sc + g5 * p2y;
+        const float xx = qx * qx, yy = qy * qy, xy2 = (qx + qx) * qy;
+        // g2 (with 9 T) and F2
+        const float g2x = fmaf(h1.x, qx, fmaf(h1.y, qy, h2.x));
+        const float g2y = fmaf(h1.y, qx, fmaf(h1.z, qy, h2.y));
+        const float d2t = fmaf(qx, g2x + h2.x, qy * (g2y + h2.y));
+        if (ORDER == 3) {  // the series without the fourth-order moments (A4, U, W)
+            const float g3x = fmaf(h3.x, xx, fmaf(h3.y, xy2, h3.z * yy));
+            const float g3y = fmaf(h3.y, xx, fmaf(h3.z, xy2, h3.w * yy));
+            const float f3 = (7.0f / 3.0f) * fmaf(qx, g3x, qy * g3y);
+            S = fmaf(k, fmaf(k, fmaf(k, f3, 2.5f * d2t), h1.w), -h0.w);
+            vx = fmaf(k, g3x, g2x);
+            vy = fmaf(k, g3y, g2y);
+        } else {
+            const float f2 = fmaf(2.5f, d2t, h2.z);
+            // g3 and the U part of F3
+            const float ux = fmaf(h4.x, qx, h2.w * qy), uy = fmaf(h2.w, qx, h4.y * qy);
+            const float g3x = fmaf(h3.x, xx, fmaf(h3.y, xy2, fmaf(h3.z, yy, ux)));
+            const float g3y = fmaf(h3.y, xx, fmaf(h3.z, xy2, fmaf(h3.w, yy, uy)));
+            const float t3 = fmaf(qx, g3x, fmaf(qy, g3y, 0.5f * fmaf(qx, ux, qy * uy)));  // (3/7) F3
+            // g4
+            const float x3 = xx * qx, y3 = yy * qy, a3 = 3.0f * (xx * qy), b3 = 3.0f * (qx * yy);
+            const float g4x = fmaf(h4.z, x3, fmaf(h4.w, a3, fmaf(h5.x, b3, h5.y * y3)));
+            const float g4y = fmaf(h4.w, x3, fmaf(h5.x, a3, fmaf(h5.y, b3, h5.z * y3)));
+            const float dg4 = fmaf(qx, g4x, qy * g4y);                                    // (4/9) F4
+            const float in3 = fmaf(k * (27.0f / 28.0f), dg4, t3);                          // (3/7) (F3 + k F4)
+            const float in2 = fmaf(k * (7.0f / 3.0f), in3, f2);                            // F2 + k (F3 + k F4)
+            S = fmaf(k, fmaf(k, in2, h1.w), -h0.w);
+            vx = fmaf(k, fmaf(k, g4x, g3x), g2x);
+            vy = fmaf(k, fmaf(k, g4y, g3y), g2y);
+        }
     }
+    tx = A * fmaf(qx, S, -k * vx);
+    ty = A * fmaf(qy, S, -k * vy);
 }
 
 // Leaf direct sum of the accuracy modes' walk: sx, sy += m_j d_j r^-3 over
