@@ -376,3 +376,64 @@ def test_tree_bitexact_at_full_size_after_evolution():
     ref = O.build_tree(d["x"], d["y"], d["mass"], bb)
     assert len(got) == len(ref)
     _assert_nodes_equal(got, ref)
+
+
+def _midline_set(seed=5, ncl=40000, nmid=24000, levels=9):
+    """Particles within three float spacings of the midlines of the node
+    squares of the first `levels` levels (subGridMaker's float arithmetic),
+    plus a dense cluster that keeps the top levels big and two corners that
+    fix the bounding box.  Their float quadrant tests disagree with their
+    Morton digits at every one of those levels (hundreds per level)."""
+    f32 = np.float32
+    rng = np.random.default_rng(seed)
+    corners = np.array([[-1000.3, -800.7], [1500.9, 1200.1]], np.float32)
+    cl = (rng.normal(0, 30, (ncl, 2)) + np.array([300.0, 250.0])).astype(np.float32)
+    pts = np.vstack([corners, cl])
+    bb = O.bbox(pts[:, 0], pts[:, 1])
+    x0, y0, size = f32(bb[0]), f32(bb[1]), f32(bb[2])
+    mids = ([], [])
+    pos, s = [(x0, y0)], size
+    for _ in range(levels):
+        half = f32(s * f32(0.5))
+        mids[0].extend(f32(px + half) for px, _ in pos)
+        mids[1].extend(f32(py + half) for _, py in pos)
+        pos = [(f32(px + (half if q & 1 else f32(0))), f32(py + (half if q & 2 else f32(0))))
+               for px, py in pos for q in range(4)]
+        s = half
+    mx, my = np.unique(np.array(mids[0], np.float32)), np.unique(np.array(mids[1], np.float32))
+    out = np.zeros((nmid, 2), np.float32)
+    for k in range(nmid):
+        ax = int(rng.integers(2))
+        line = mx if ax == 0 else my
+        v = line[rng.integers(len(line))]
+        off = int(rng.integers(-3, 4))
+        for _ in range(abs(off)):
+            v = np.nextafter(v, f32(np.inf) if off > 0 else f32(-np.inf), dtype=np.float32)
+        other = f32(rng.uniform(-900, 1400) if ax == 0 else rng.uniform(-700, 1100))
+        out[k] = (v, other) if ax == 0 else (other, v)
+    pts = np.vstack([pts, out])
+    n = len(pts)
+    z = np.zeros(n, np.float32)
+    return {"x": pts[:, 0].copy(), "y": pts[:, 1].copy(), "vx": z, "vy": z, "mass": np.ones(n, np.float32),
+            "id": np.arange(n, dtype=np.uint32)}
+
+
+@pytest.mark.parametrize("leaf_capacity", [0, 32])
+def test_tree_bitexact_particles_on_midlines(leaf_capacity):
+    # the frame's tree (whose top levels split Morton-prefix runs, checked
+    # against the particles' float quadrant tests) equals the oracle's on
+    # particles sitting on and next to the node midlines of nine levels
+    st = _midline_set()
+    n = len(st["x"])
+    p = _params(theta=0.7, dt=0.001, dt_start=0.001, direct_below=0, leaf_capacity=leaf_capacity)
+    with Context(n) as c:
+        c.upload(st)
+        c.step(p, 1)
+        c.sync()
+        d = c.download(["x", "y", "mass"])
+        got = c.nodes()
+    np.testing.assert_array_equal(np.sort(d["x"]), np.sort(st["x"]))  # zero velocities: nothing moved
+    bb = O.bbox(d["x"], d["y"])
+    ref = O.build_tree(d["x"], d["y"], d["mass"], bb, leaf_capacity)
+    assert len(got) == len(ref)
+    _assert_nodes_equal(got, ref)
