@@ -52,10 +52,10 @@ void free_all(ge3_ctx *c) {
     dfree(b.pxyzm); dfree(b.keys); dfree(b.sk_a); dfree(b.sk_b); dfree(b.sv_a); dfree(b.sv_b); dfree(b.perm);
     dfree(b.hist); dfree(b.partials); dfree(b.bb); dfree(b.counters); dfree(b.inter);
     TopNodes3 &T = b.top;
-    dfree(T.px); dfree(T.py); dfree(T.pz); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.child);
+    dfree(T.px); dfree(T.py); dfree(T.pz); dfree(T.size); dfree(T.start); dfree(T.end); dfree(T.child); dfree(T.exact);
     dfree(T.kind); dfree(T.pending2); dfree(T.offset); dfree(T.broot); dfree(T.blist); dfree(T.blist_off);
     dfree(T.blist_cnt); dfree(T.depth);
-    dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part);
+    dfree(b.start_count); dfree(b.start_level); dfree(b.scan_part); dfree(b.kstart); dfree(b.kdirty);
     dfree(b.nodes); dfree(b.node_geom); dfree(b.node_mask); dfree(b.diag);
     for (State3 &s : b.st) {
         float **fs[] = {&s.sph_mass, &s.rest_dens, &s.stiff, &s.visc, &s.cohesion, &s.dens, &s.press, &s.press_tmp,
@@ -148,12 +148,15 @@ int alloc_all(ge3_ctx *c) {
     GE3_CHECK(c, dalloc(T.broot, T.cap)); GE3_CHECK(c, dalloc(T.blist, T.cap));
     GE3_CHECK(c, dalloc(T.blist_off, T.cap)); GE3_CHECK(c, dalloc(T.blist_cnt, T.cap));
     GE3_CHECK(c, dalloc(T.depth, T.cap));
-    for (uint32_t *u : {T.start, T.end, T.kind, T.pending2, T.offset, T.broot, T.blist_cnt, T.depth})
+    GE3_CHECK(c, dalloc(T.exact, T.cap));
+    for (uint32_t *u : {T.start, T.end, T.kind, T.pending2, T.offset, T.broot, T.blist_cnt, T.depth, T.exact})
         GE3_CHECK(c, hipMemset(u, 0, sizeof(uint32_t) * T.cap));
     GE3_CHECK(c, hipMemset(T.child, 0xFF, sizeof(uint32_t) * 8ull * T.cap));
     GE3_CHECK(c, dalloc(b.start_count, (size_t)cap + 1));
     GE3_CHECK(c, dalloc(b.start_level, (size_t)cap + 1));
     GE3_CHECK(c, dalloc(b.scan_part, (size_t)cap / 1024 + 64));
+    GE3_CHECK(c, dalloc(b.kstart, kKeyStartEntries3));
+    GE3_CHECK(c, dalloc(b.kdirty, kKeyDirtyEntries3));
     // Octree::root reserves 8N (quadtree.cpp:279); measured octrees hold
     // about 1.3N nodes, so 4N with an overflow flag
     b.node_cap = 4u * cap + 1024u;
@@ -219,7 +222,7 @@ void frame(ge3_ctx *c, const ge3_step_params *p, float dt_start, bool count) {
         launch3_sort_gather(c);
         c->have_grid = false;
         if (!p->naive) {
-            launch_octree_build(c, c->b.bb);
+            launch_octree_build(c, c->b.bb, true);
             c->have_tree = true;
         }
     }

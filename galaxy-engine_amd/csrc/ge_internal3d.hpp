@@ -67,8 +67,19 @@ struct TopNodes3 {
     uint32_t *blist_off = nullptr;
     uint32_t *blist_cnt = nullptr;
     uint32_t *depth = nullptr;
+    uint32_t *exact = nullptr;     // level phase: 1 if the node's range is a Morton-prefix run
     uint32_t cap = 0;
 };
+
+// Key-run splits of the octree level phase: for levels L = 1..kKeyLevels3
+// the first particle (+1, 0 = none) of each run of equal L-digit (3-bit
+// digit) Morton prefixes, and for levels L < kKeyLevels3 a flag per prefix
+// whose particles' float octant tests disagree with their key digits.
+constexpr int kKeyLevels3 = 7;
+constexpr uint32_t kKeyStartEntries3 = ((1u << (3 * (kKeyLevels3 + 1))) - 8u) / 7u;  // sum of 8^L, L = 1..7
+constexpr uint32_t kKeyDirtyEntries3 = ((1u << (3 * kKeyLevels3)) - 1u) / 7u;        // sum of 8^L, L = 0..6
+__host__ __device__ constexpr uint32_t key_start_base3(int L) { return ((1u << (3 * L)) - 8u) / 7u; }
+__host__ __device__ constexpr uint32_t key_dirty_base3(int L) { return ((1u << (3 * L)) - 1u) / 7u; }
 
 struct Buffers3 {
     State3 st[2];
@@ -85,6 +96,8 @@ struct Buffers3 {
     unsigned long long *inter = nullptr;  // interactions, accepted nodes
     TopNodes3 top;
     uint32_t *start_count = nullptr;
+    uint32_t *kstart = nullptr;       // key-run starts of the top levels (kKeyStartEntries3)
+    uint8_t *kdirty = nullptr;        // prefixes whose float octant tests disagree (kKeyDirtyEntries3)
     uint32_t *start_level = nullptr;
     uint32_t *scan_part = nullptr;
     NodeRec3 *nodes = nullptr;
@@ -139,7 +152,10 @@ void launch3_integrate_end(ge3_ctx *c, float dt);
 void launch3_direct_fp64(ge3_ctx *c, double G, double eps, uint32_t nt, const uint32_t *targets, double *out);
 void launch3_pack(ge3_ctx *c);
 // ge_octree.hip
-void launch_octree_build(ge3_ctx *c, const float *bb_device);
+// sorted_keys: the sort keys in b.sk_a are the sort of the current particle
+// order (the frame's own sort), so the level phase may split Morton-prefix
+// runs without searching
+void launch_octree_build(ge3_ctx *c, const float *bb_device, bool sorted_keys = false);
 // ge_sph3d.hip
 void sph3_new_grid(ge3_ctx *c, const ge3_step_params *p);
 void sph3_pcisph(ge3_ctx *c, const ge3_step_params *p, float end_kick_dt);

@@ -14,6 +14,8 @@
 // particles, one workgroup per subtree below that, one wave per subtree of at
 // most 64 particles (three ballot masks, seven bit searches per split), then
 // the records, and the internal masses deepest level first.
+#include <algorithm>
+
 #include "ge_internal3d.hpp"
 #include "ge_tree_common.hpp"
 
@@ -305,8 +307,10 @@ __device__ __forceinline__ uint32_t child_count8(const uint32_t B[9]) {
 }
 
 // Lane 0: an internal node and its children at indices first.. (octant order).
+// exact: the children's ranges are Morton-prefix runs (key-run split).
 __device__ void write_children3(TopNodes3 &T, uint32_t node, float px, float py, float pz, float size,
-                                uint32_t depth, const uint32_t B[9], uint32_t first, uint32_t *out_list) {
+                                uint32_t depth, const uint32_t B[9], uint32_t first, uint32_t *out_list,
+                                uint32_t exact = 0u) {
     T.kind[node] = kKindInternal;
     const float half = size * 0.5f;
     uint32_t c = first, j = 0;
@@ -320,6 +324,7 @@ __device__ void write_children3(TopNodes3 &T, uint32_t node, float px, float py,
             T.start[c] = B[q];
             T.end[c] = B[q + 1];
             T.depth[c] = depth + 1;
+            T.exact[c] = exact;
             T.kind[c] = kKindPending;
             T.child[8 * node + q] = c;
             if (out_list) out_list[j++] = c;
@@ -357,17 +362,32 @@ __device__ __forceinline__ void push_block_root3(TopNodes3 &T, uint32_t *counter
     else atomicOr(&counters[kCntError], kErrFrontier);
 }
 
+// Key-run splits: the data a node's split needs when its range is a run of
+// equal Morton prefixes
+struct KeySplit3 {
+    const uint64_t *keys;  // sorted 63-bit keys
+    uint32_t *start;       // kKeyStartEntries3
+    uint8_t *dirty;        // kKeyDirtyEntries3
+    int on;                // 0: every split searches
+};
+
 __global__ __launch_bounds__(kTreeBlock) void k3_tree_clear(TopNodes3 T, StartTable st, const float *__restrict__ bb,
-                                                            uint32_t n, uint32_t *__restrict__ counters) {
+                                                            uint32_t n, uint32_t *__restrict__ counters,
+                                                            KeySplit3 ks) {
     const uint32_t i = blockIdx.x * kTreeBlock + threadIdx.x;
     if (i <= n) {
         st.count[i] = 0;
         st.level[i] = 0xFFFFFFFFu;
     }
+    if (ks.on) {
+        if (i < kKeyStartEntries3) ks.start[i] = 0u;
+        if (i < kKeyDirtyEntries3) ks.dirty[i] = 0u;
+    }
     if (i == 0) {  // Octree::root (quadtree.cpp:275-292)
         T.px[0] = bb[0]; T.py[0] = bb[1]; T.pz[0] = bb[2]; T.size[0] = bb[3];
         T.start[0] = 0; T.end[0] = n;
         T.kind[0] = kKindPending; T.depth[0] = 0;
+        T.exact[0] = ks.on ? 1u : 0u;
     }
     if (blockIdx.x == 0) {
         const uint32_t t = threadIdx.x;
@@ -386,9 +406,109 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_clear(TopNodes3 T, StartTa
     }
 }
 
+// Key runs of the top levels, one thread per sorted particle i: the octree
+// counterpart of the quadtree's k_tree_keys.  i starts the run of its
+// L-digit prefix for every level L deeper than the digits it shares with
+// particle i - 1 (recorded for L = 1..kKeyLevels3), and flags the prefix of
+// every level on its key's path whose octant tests (subGridMaker3D's
+// x < mid.x, y < mid.y, z < mid.z on the node cube, child corner
+// pos + size / 2 on the upper side) disagree with its key digit.
+__global__ __launch_bounds__(kTreeBlock) void k3_tree_keys(KeySplit3 ks, const float *__restrict__ x,
+                                                           const float *__restrict__ y, const float *__restrict__ z,
+                                                           uint32_t n, const float *__restrict__ bb) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * kTreeBlock + threadIdx.x;
+    if (i >= n) return;
+    constexpr int kDigits = kMortonBits3 / 3;
+    const uint64_t k = ks.keys[i];
+    int shared = -1;  // leading digits equal to particle i - 1's
+    if (i > 0) {
+        const uint64_t d = k ^ ks.keys[i - 1];
+        shared = d ? (__clzll(d) - (64 - kMortonBits3)) / 3 : kDigits;
+    }
+    for (int L = shared + 1 < 1 ? 1 : shared + 1; L <= kKeyLevels3; ++L)
+        ks.start[key_start_base3(L) + (uint32_t)(k >> (kMortonBits3 - 3 * L))] = i + 1u;
+    const float qx0 = x[i], qy0 = y[i], qz0 = z[i];
+    float px = bb[0], py = bb[1], pz = bb[2], size = bb[3];
+    for (int L = 0; L < kKeyLevels3; ++L) {
+        const float half = size * 0.5f;
+        const uint32_t dig = (uint32_t)(k >> (kMortonBits3 - 3 - 3 * L)) & 7u;
+        const uint32_t q = ((qx0 < px + half) ? 0u : 1u) | ((qy0 < py + half) ? 0u : 2u) | ((qz0 < pz + half) ? 0u : 4u);
+        if (q != dig)
+            ks.dirty[key_dirty_base3(L) + (uint32_t)(L ? k >> (kMortonBits3 - 3 * L) : 0ull)] = 1u;
+        px = px + ((dig & 1u) ? half : 0.0f);
+        py = py + ((dig & 2u) ? half : 0.0f);
+        pz = pz + ((dig & 4u) ? half : 0.0f);
+        size = half;
+    }
+}
+
+// Probe t (0-based) of partition_point's halving search over [first, last)
+// when every probe below `guess` tests true and every other false; false if
+// that search takes fewer probes.
+__device__ __forceinline__ bool guessed_probe3(uint32_t first, uint32_t last, uint32_t guess, uint32_t t,
+                                               uint32_t &pos) {
+    uint32_t f = first, len = last - first;
+    for (uint32_t k = 0; len > 0; ++k) {
+        const uint32_t half = len >> 1, m = f + half;
+        if (k == t) {
+            pos = m;
+            return true;
+        }
+        if (m < guess) {
+            f = m + 1;
+            len = len - half - 1;
+        } else {
+            len = half;
+        }
+    }
+    return false;
+}
+
+// A prefix run holding a particle whose octant disagrees with its key digit:
+// the partition points are the children's run starts B[1..7] when every
+// probe the seven searches would take on that assumption tests as assumed
+// (z over [B0, B8) toward B4; y over [B0, B4) toward B2 and [B4, B8) toward
+// B6; x over the four quarters toward B1, B3, B5, B7).  The wave checks all
+// of them in one round, up to 32 probes per search.
+__device__ __forceinline__ bool verify_guess3(const float *__restrict__ x, const float *__restrict__ y,
+                                              const float *__restrict__ z, float px, float py, float pz, float size,
+                                              const uint32_t B[9]) {
+#pragma clang fp contract(off)
+    const float half = size * 0.5f;
+    const float midx = px + half, midy = py + half, midz = pz + half;
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t c = lane_id() + 64u * (uint32_t)r;  // check c of 256: search c / 32, probe c % 32
+        const uint32_t sidx = c >> 5, t = c & 31u;
+        if (sidx > 6u) continue;
+        // search 0: z over [B0, B8) toward B4; 1, 2: y over the z halves
+        // toward B2, B6; 3..6: x over the quarters toward B1, B3, B5, B7
+        uint32_t lo, hi, g;
+        const float *v;
+        float mid;
+        if (sidx == 0u) {
+            lo = B[0]; hi = B[8]; g = B[4]; v = z; mid = midz;
+        } else if (sidx <= 2u) {
+            const uint32_t h = sidx - 1u;
+            lo = B[4 * h]; hi = B[4 * h + 4]; g = B[4 * h + 2]; v = y; mid = midy;
+        } else {
+            const uint32_t h = sidx - 3u;
+            lo = B[2 * h]; hi = B[2 * h + 2]; g = B[2 * h + 1]; v = x; mid = midx;
+        }
+        uint32_t pos;
+        if (!guessed_probe3(lo, hi, g, t, pos)) continue;
+        if ((v[pos] < mid) != (pos < g)) ok = false;
+    }
+    // ranges hold fewer than 2^28 particles, so no search takes more than 32 probes
+    return __ballot(!ok) == 0ull;
+}
+
 __global__ __launch_bounds__(kTreeBlock) void k3_tree_level(TopNodes3 T, StartTable st, const float *__restrict__ x,
                                                             const float *__restrict__ y, const float *__restrict__ z,
-                                                            uint32_t *__restrict__ counters, int L, int last_level) {
+                                                            uint32_t *__restrict__ counters, int L, int last_level,
+                                                            KeySplit3 ks) {
     const uint32_t lane = lane_id();
     const uint32_t base = counters[kCntLevelBase0 + L];
     uint32_t cnt = counters[kCntLevel0 + L];
@@ -412,7 +532,23 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_level(TopNodes3 T, StartTa
             continue;
         }
         uint32_t B[9];
-        wave_split3(x, y, z, px, py, pz, size, s, e, B);
+        // a prefix run whose octants all agree with its key digits splits at
+        // its children's run starts; a run holding a disagreeing particle
+        // takes them after one round of probe checks; any other node searches
+        bool keyed = false;
+        if (ks.on && L < kKeyLevels3 && T.exact[node]) {
+            const uint32_t P = L ? (uint32_t)(ks.keys[s] >> (kMortonBits3 - 3 * L)) : 0u;
+            const uint32_t *cs = ks.start + key_start_base3(L + 1) + 8u * P;
+            B[0] = s;
+            B[8] = e;
+#pragma unroll
+            for (int q = 7; q >= 1; --q) {
+                const uint32_t cq = cs[q];
+                B[q] = cq ? cq - 1u : B[q + 1];
+            }
+            keyed = !ks.dirty[key_dirty_base3(L) + P] || verify_guess3(x, y, z, px, py, pz, size, B);
+        }
+        if (!keyed) wave_split3(x, y, z, px, py, pz, size, s, e, B);
         if (lane == 0) {
             const uint32_t k = child_count8(B);
             const uint32_t off = atomicAdd(&counters[kCntLevel0 + L + 1], k);
@@ -422,7 +558,7 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_level(TopNodes3 T, StartTa
                 push_block_root3(T, counters, node);
             } else {
                 register_start(st, s, (uint32_t)L);
-                write_children3(T, node, px, py, pz, size, (uint32_t)L, B, first, nullptr);
+                write_children3(T, node, px, py, pz, size, (uint32_t)L, B, first, nullptr, keyed ? 1u : 0u);
             }
         }
     }
@@ -720,13 +856,18 @@ __global__ __launch_bounds__(kMassThreads) void k3_tree_mass(TopNodes3 T, uint32
 
 }  // namespace
 
-void launch_octree_build(ge3_ctx *c, const float *bb_device) {
+void launch_octree_build(ge3_ctx *c, const float *bb_device, bool sorted_keys) {
     const State3 &s = c->b.st[c->b.cur];
     TopNodes3 &T = c->b.top;
     const uint64_t n = c->n;
     const StartTable st{c->b.start_count, c->b.start_level, c->leaf_cap};
-    hipLaunchKernelGGL(k3_tree_clear, dim3(blocks_for(n + 1, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, T, st,
-                       bb_device, c->n, c->b.counters);
+    const KeySplit3 ks{c->b.sk_a, c->b.kstart, c->b.kdirty, sorted_keys ? 1 : 0};
+    const uint64_t clear_n = ks.on ? std::max<uint64_t>(n + 1, kKeyStartEntries3) : n + 1;
+    hipLaunchKernelGGL(k3_tree_clear, dim3(blocks_for(clear_n, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, T, st,
+                       bb_device, c->n, c->b.counters, ks);
+    if (ks.on && n > 0)
+        hipLaunchKernelGGL(k3_tree_keys, dim3(blocks_for(n, kTreeBlock)), dim3(kTreeBlock), 0, c->stream, ks, s.x, s.y,
+                           s.z, c->n, bb_device);
     // level L+1 only holds children of nodes above kBlockSubMax particles
     const uint64_t big_children = 8 * (n / (kBlockSubMax + 1)) + 8;
     for (int L = 0; L < kTopLevels; ++L) {
@@ -735,7 +876,7 @@ void launch_octree_build(ge3_ctx *c, const float *bb_device) {
         if (L > 0 && maxn > big_children) maxn = big_children;
         if (maxn > n) maxn = n;
         hipLaunchKernelGGL(k3_tree_level, dim3(wave_grid(maxn)), dim3(kTreeBlock), 0, c->stream, T, st, s.x, s.y,
-                           s.z, c->b.counters, L, kTopLevels - 1);
+                           s.z, c->b.counters, L, kTopLevels - 1, ks);
     }
     uint64_t roots = n / (kWaveSubMax + 1) + 1;
     if (roots > 4096) roots = 4096;

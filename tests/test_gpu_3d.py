@@ -292,3 +292,75 @@ def test_edge_cases_and_rejections():
             c.step(_params(temperature=1), 1)
         with pytest.raises(Exception):
             c.step(_params(periodic=1, domain_d=0.0), 1)
+
+
+def _step_nodes(st, leaf_capacity=0):
+    # one frame with zero velocities: the frame's own sort and octree (whose
+    # top levels split Morton-prefix runs) on positions that do not move
+    n = len(st["x"])
+    z = np.zeros(n, np.float32)
+    st = dict(st, vx=z, vy=z, vz=z)
+    with Context3D(n) as c:
+        c.upload(st)
+        c.step(_params(theta=0.7, dt=0.001, dt_start=0.001, direct_below=0, leaf_capacity=leaf_capacity), 1)
+        c.sync()
+        d = c.download(["x", "y", "z", "mass"])
+        got = c.nodes()
+    np.testing.assert_array_equal(np.sort(d["x"]), np.sort(st["x"]))
+    bb = O3.bbox(d["x"], d["y"], d["z"])
+    ref = O3.build_tree(d["x"], d["y"], d["z"], d["mass"], bb, leaf_capacity)
+    return got, ref
+
+
+def _midline_set3(seed=6, ncl=60000, nmid=30000, levels=5):
+    """Particles within three float spacings of the midlines of the node cubes
+    of the first `levels` levels (subGridMaker3D's float arithmetic) on one
+    axis, plus a dense cluster and two corners that fix the bounding cube."""
+    f32 = np.float32
+    rng = np.random.default_rng(seed)
+    corners = np.array([[-1000.3, -800.7, -600.1], [1500.9, 1200.1, 1300.3]], np.float32)
+    cl = (rng.normal(0, 40, (ncl, 3)) + np.array([300.0, 250.0, 100.0])).astype(np.float32)
+    pts = np.vstack([corners, cl])
+    bb = O3.bbox(pts[:, 0], pts[:, 1], pts[:, 2])
+    lo, size = [f32(v) for v in bb[:3]], f32(bb[3])
+    mids = ([], [], [])
+    pos, s = [tuple(lo)], size
+    for _ in range(levels):
+        half = f32(s * f32(0.5))
+        for a in range(3):
+            mids[a].extend(f32(p[a] + half) for p in pos)
+        pos = [tuple(f32(p[a] + (half if (q >> a) & 1 else f32(0))) for a in range(3)) for p in pos for q in range(8)]
+        s = half
+    lines = [np.unique(np.array(m, np.float32)) for m in mids]
+    out = np.zeros((nmid, 3), np.float32)
+    for k in range(nmid):
+        ax = int(rng.integers(3))
+        v = lines[ax][rng.integers(len(lines[ax]))]
+        off = int(rng.integers(-3, 4))
+        for _ in range(abs(off)):
+            v = np.nextafter(v, f32(np.inf) if off > 0 else f32(-np.inf), dtype=np.float32)
+        row = rng.uniform([-900, -700, -500], [1400, 1100, 1200]).astype(np.float32)
+        row[ax] = v
+        out[k] = row
+    pts = np.vstack([pts, out])
+    n = len(pts)
+    return {"x": pts[:, 0].copy(), "y": pts[:, 1].copy(), "z": pts[:, 2].copy(), "mass": np.ones(n, np.float32),
+            "id": np.arange(n, dtype=np.uint32)}
+
+
+@pytest.mark.parametrize("leaf_capacity", [0, 32])
+def test_frame_octree_bitexact_on_midlines(leaf_capacity):
+    got, ref = _step_nodes(_midline_set3(), leaf_capacity)
+    _assert_nodes_equal(got, ref)
+
+
+def test_frame_octree_bitexact_dense_clumps():
+    st, _ = configs.galaxy3d_small(seed=12, n=30000)
+    st = dict(st)
+    rng = np.random.default_rng(1)
+    for f in ("x", "y", "z"):
+        st[f] = st[f].copy()
+        st[f][:5000] = st[f][0]
+        st[f][5000:8000] = st[f][1] + rng.uniform(0, 1e-3, 3000).astype(np.float32)
+    got, ref = _step_nodes(st)
+    _assert_nodes_equal(got, ref)
