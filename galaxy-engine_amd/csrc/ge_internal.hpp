@@ -37,6 +37,7 @@ constexpr int kBlockLevelsMax = 160;   // local levels a workgroup subtree may t
 // prefixes, and for levels L < kKeyLevels a flag per prefix whose particles'
 // float quadrants disagree with their key digits somewhere in it.
 constexpr int kKeyLevels = 10;
+constexpr uint32_t kExactTag = 1u << 31;  // a node's prefix tag: set bit, then the prefix (at most 2 * kKeyLevels bits)
 constexpr uint32_t kKeyStartEntries = ((1u << (2 * (kKeyLevels + 1))) - 4u) / 3u;  // sum of 4^L, L = 1..10
 constexpr uint32_t kKeyDirtyEntries = ((1u << (2 * kKeyLevels)) - 1u) / 3u;        // sum of 4^L, L = 0..9
 __host__ __device__ constexpr uint32_t key_start_base(int L) { return ((1u << (2 * L)) - 4u) / 3u; }
@@ -182,7 +183,7 @@ struct TopNodes {
     float *px = nullptr, *py = nullptr, *size = nullptr;
     uint32_t *start = nullptr, *end = nullptr;
     uint32_t *slot = nullptr;      // scratch per node: its entry in the top-mass pass's list
-    uint32_t *exact = nullptr;     // level phase: 1 if the node's range is a Morton-prefix run
+    uint32_t *exact = nullptr;     // level phase: kExactTag | P if the node's range is the run of key prefix P, else 0
     uint32_t *child = nullptr;     // 4 per node
     uint32_t *kind = nullptr;      // 0 internal, 1 leaf, 2 wave subtree, 3 pending, 4 dead
     uint32_t *pending2 = nullptr;  // build stamp of nodes whose mass the block pass computed

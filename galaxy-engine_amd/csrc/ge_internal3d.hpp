@@ -67,7 +67,7 @@ struct TopNodes3 {
     uint32_t *blist_off = nullptr;
     uint32_t *blist_cnt = nullptr;
     uint32_t *depth = nullptr;
-    uint32_t *exact = nullptr;     // level phase: 1 if the node's range is a Morton-prefix run
+    uint32_t *exact = nullptr;     // level phase: kExactTag | P if the node's range is the run of key prefix P, else 0
     uint32_t cap = 0;
 };
 

@@ -307,7 +307,9 @@ __device__ __forceinline__ uint32_t child_count8(const uint32_t B[9]) {
 }
 
 // Lane 0: an internal node and its children at indices first.. (octant order).
-// exact: the children's ranges are Morton-prefix runs (key-run split).
+// exact: the node's own prefix tag (kExactTag | P) when its children's
+// ranges are Morton-prefix runs (key-run split); child q then gets the tag
+// of prefix 8 P + q.
 __device__ void write_children3(TopNodes3 &T, uint32_t node, float px, float py, float pz, float size,
                                 uint32_t depth, const uint32_t B[9], uint32_t first, uint32_t *out_list,
                                 uint32_t exact = 0u) {
@@ -324,7 +326,7 @@ __device__ void write_children3(TopNodes3 &T, uint32_t node, float px, float py,
             T.start[c] = B[q];
             T.end[c] = B[q + 1];
             T.depth[c] = depth + 1;
-            T.exact[c] = exact;
+            T.exact[c] = exact ? (kExactTag | (((exact & ~kExactTag) << 3) + (uint32_t)q)) : 0u;
             T.kind[c] = kKindPending;
             T.child[8 * node + q] = c;
             if (out_list) out_list[j++] = c;
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_clear(TopNodes3 T, StartTa
         T.px[0] = bb[0]; T.py[0] = bb[1]; T.pz[0] = bb[2]; T.size[0] = bb[3];
         T.start[0] = 0; T.end[0] = n;
         T.kind[0] = kKindPending; T.depth[0] = 0;
-        T.exact[0] = ks.on ? 1u : 0u;
+        T.exact[0] = ks.on ? kExactTag : 0u;  // prefix 0
     }
     if (blockIdx.x == 0) {
         const uint32_t t = threadIdx.x;
@@ -523,9 +525,11 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_level(TopNodes3 T, StartTa
     const uint32_t nwaves = (gridDim.x * kTreeBlock) >> 6;
     for (uint32_t t = wave; t < cnt; t += nwaves) {
         const uint32_t node = base + t;
-        if (T.kind[node] == kKindDead) continue;
+        // every field read at once: the chain of dependent loads sets a level's time
+        const uint32_t kind = T.kind[node], ex = T.exact[node];
         const float px = T.px[node], py = T.py[node], pz = T.pz[node], size = T.size[node];
         const uint32_t s = T.start[node], e = T.end[node];
+        if (kind == kKindDead) continue;
         if (finish_small3(T, st, x, y, z, node, px, py, pz, size, s, e, (uint32_t)L, counters + kCntError)) continue;
         if (e - s <= kBlockSubMax || L == last_level) {
             if (lane == 0) push_block_root3(T, counters, node);
@@ -536,8 +540,8 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_level(TopNodes3 T, StartTa
         // its children's run starts; a run holding a disagreeing particle
         // takes them after one round of probe checks; any other node searches
         bool keyed = false;
-        if (ks.on && L < kKeyLevels3 && T.exact[node]) {
-            const uint32_t P = L ? (uint32_t)(ks.keys[s] >> (kMortonBits3 - 3 * L)) : 0u;
+        if (ks.on && L < kKeyLevels3 && ex) {
+            const uint32_t P = ex & ~kExactTag;  // the run's key prefix, L digits
             const uint32_t *cs = ks.start + key_start_base3(L + 1) + 8u * P;
             B[0] = s;
             B[8] = e;
@@ -558,7 +562,7 @@ __global__ __launch_bounds__(kTreeBlock) void k3_tree_level(TopNodes3 T, StartTa
                 push_block_root3(T, counters, node);
             } else {
                 register_start(st, s, (uint32_t)L);
-                write_children3(T, node, px, py, pz, size, (uint32_t)L, B, first, nullptr, keyed ? 1u : 0u);
+                write_children3(T, node, px, py, pz, size, (uint32_t)L, B, first, nullptr, keyed ? ex : 0u);
             }
         }
     }

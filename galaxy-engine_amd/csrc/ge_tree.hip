@@ -296,7 +296,9 @@ __device__ __forceinline__ uint32_t child_count(const uint32_t b[5]) {
 }
 
 // Lane 0: an internal node and its children at indices first.. (q order).
-// exact: the children's ranges are Morton-prefix runs (key-run split).
+// exact: the node's own prefix tag (kExactTag | P) when its children's
+// ranges are Morton-prefix runs (key-run split); child q then gets the tag
+// of prefix 4 P + q.
 __device__ void write_children(TopNodes &T, uint32_t node, float px, float py, float size, uint32_t depth,
                                const uint32_t b[5], uint32_t first, uint32_t *out_list, uint32_t exact = 0u) {
     T.kind[node] = kKindInternal;
@@ -310,7 +312,7 @@ __device__ void write_children(TopNodes &T, uint32_t node, float px, float py, f
             T.start[c] = b[q];
             T.end[c] = b[q + 1];
             T.depth[c] = depth + 1;
-            T.exact[c] = exact;
+            T.exact[c] = exact ? (kExactTag | (((exact & ~kExactTag) << 2) + (uint32_t)q)) : 0u;
             T.kind[c] = kKindPending;
             T.child[4 * node + q] = c;
             if (out_list) out_list[j++] = c;
@@ -381,7 +383,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_clear(TopNodes T, StartTabl
         T.px[i] = bb[0]; T.py[i] = bb[1]; T.size[i] = bb[2];
         T.start[i] = sys_off[i]; T.end[i] = sys_off[i + 1];
         T.kind[i] = kKindPending; T.depth[i] = 0;
-        T.exact[i] = ks.on ? 1u : 0u;
+        T.exact[i] = ks.on ? kExactTag : 0u;  // prefix 0
     }
     if (blockIdx.x == 0) {
         const uint32_t t = threadIdx.x;
@@ -513,9 +515,11 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTabl
     const uint32_t nwaves = (gridDim.x * kTreeBlock) >> 6;
     for (uint32_t t = wave; t < cnt; t += nwaves) {
         const uint32_t node = base + t;
-        if (T.kind[node] == kKindDead) continue;
+        // every field read at once: the chain of dependent loads sets a level's time
+        const uint32_t kind = T.kind[node], ex = T.exact[node];
         const float px = T.px[node], py = T.py[node], size = T.size[node];
         const uint32_t s = T.start[node], e = T.end[node];
+        if (kind == kKindDead) continue;
         if (finish_small(T, st, x, y, node, px, py, size, s, e, (uint32_t)L, counters + kCntError)) continue;
         if (e - s <= kBlockSubMax || L == last_level) {
             if (lane == 0) push_block_root(T, counters, node);
@@ -525,8 +529,8 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTabl
         // a prefix run whose quadrants all agree with its key digits splits
         // at its children's run starts; any other node searches
         bool keyed = false;
-        if (ks.on && L < kKeyLevels && T.exact[node]) {
-            const uint32_t P = L ? (uint32_t)(word_key(ks, s) >> (kMortonBits - 2 * L)) : 0u;
+        if (ks.on && L < kKeyLevels && ex) {
+            const uint32_t P = ex & ~kExactTag;  // the run's key prefix, L digits
             const uint32_t *cs = ks.start + key_start_base(L + 1) + 4u * P;
             const uint32_t c1 = cs[1], c2 = cs[2], c3 = cs[3];
             b[0] = s;
@@ -547,7 +551,7 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_level(TopNodes T, StartTabl
                 push_block_root(T, counters, node);
             } else {
                 register_start(st, s, (uint32_t)L);
-                write_children(T, node, px, py, size, (uint32_t)L, b, first, nullptr, keyed ? 1u : 0u);
+                write_children(T, node, px, py, size, (uint32_t)L, b, first, nullptr, keyed ? ex : 0u);
             }
         }
     }
