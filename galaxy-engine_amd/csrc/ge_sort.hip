@@ -17,8 +17,10 @@
 // publish before they wait, so every wait ends; each wait is also bounded by
 // a spin limit that raises an error flag), then writes each digit's run of
 // the LDS-sorted tile to its place, so consecutive lanes write consecutive
-// words.  Look-back statuses carry the sort's generation in their high word,
-// so the status tables never need clearing.
+// words.  Look-back statuses are 32-bit words (two flag bits, a 30-bit
+// count), cleared by the counting kernel, which runs first: the look-back
+// is bound by the bytes of its coherent round trips, which a generation tag
+// in each status would double.
 #include "ge_internal.hpp"
 
 namespace ge {
@@ -39,14 +41,13 @@ constexpr int kLookWindow = 8;                     // predecessor tiles read per
 constexpr uint32_t kOneSweepMax = 1u << 30;        // published counts are 30 bits
 
 // hist layout (uint32 units): digit totals [pass][digit], digit bases
-// [pass][digit], tickets [pass] and the generation word at fixed offsets (a
-// buffer serves sorts of any length), then the look-back statuses, 64-bit
-// [pass][tile][digit]
-constexpr uint32_t kSortHeader = (2u * kOsPasses * kRadix + kOsPasses + 1u + 63u) & ~63u;
+// [pass][digit] and tickets [pass] at fixed offsets (a buffer serves sorts
+// of any length), then the look-back statuses [pass][tile][digit]
+constexpr uint32_t kSortHeader = (2u * kOsPasses * kRadix + kOsPasses + 63u) & ~63u;
 
 struct SortMem {
-    uint32_t *totals, *bases, *tickets, *gen;
-    uint64_t *status;
+    uint32_t *totals, *bases, *tickets;
+    uint32_t *status;
 };
 
 __host__ __device__ inline SortMem sort_mem(uint32_t *hist) {
@@ -54,26 +55,28 @@ __host__ __device__ inline SortMem sort_mem(uint32_t *hist) {
     m.totals = hist;
     m.bases = m.totals + kOsPasses * kRadix;
     m.tickets = m.bases + kOsPasses * kRadix;
-    m.gen = m.tickets + kOsPasses;
-    m.status = reinterpret_cast<uint64_t *>(hist + kSortHeader);
+    m.status = hist + kSortHeader;
     return m;
 }
 
-__device__ __forceinline__ uint64_t ld_status(const uint64_t *a) {
+__device__ __forceinline__ uint32_t ld_status(const uint32_t *a) {
     return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_status(uint64_t *a, uint64_t v) {
+__device__ __forceinline__ void st_status(uint32_t *a, uint32_t v) {
     __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Digit counts of every pass: each thread takes kOsItems consecutive keys and
 // adds one LDS count per run of equal digits; the workgroup then adds its
-// non-zero counts to the per-pass totals.
+// non-zero counts to the per-pass totals.  Workgroup b also clears tile b's
+// look-back statuses of every pass.
 __global__ __launch_bounds__(kOsThreads) void k_digit_totals(const uint64_t *__restrict__ in, uint32_t n,
                                                              int lo_bit, int npass, uint32_t last_mask,
-                                                             uint32_t *__restrict__ totals) {
+                                                             uint32_t *__restrict__ totals,
+                                                             uint32_t *__restrict__ status, uint32_t nb) {
     __shared__ uint32_t cnt[kOsPasses][kRadix];
     const uint32_t t = threadIdx.x;
+    for (int p = 0; p < npass; ++p) st_status(status + ((uint64_t)p * nb + blockIdx.x) * kRadix + t, 0u);
     for (uint32_t e = t; e < kOsPasses * kRadix; e += kOsThreads) (&cnt[0][0])[e] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kOsTile + (uint64_t)t * kOsItems;
@@ -124,8 +127,7 @@ __device__ __forceinline__ uint32_t wg_excl_scan(uint32_t v, uint32_t *sh) {
 }
 
 // One workgroup: each pass's digit bases from its totals; the totals are
-// cleared for the next sort, the tickets reset and the generation advanced
-// (its top bit always set, so no stale count can pass for a status).
+// cleared for the next sort and the tickets reset.
 __global__ __launch_bounds__(kOsThreads) void k_digit_bases(uint32_t *__restrict__ hist, int npass) {
     __shared__ uint32_t sh[kOsWaves];
     const SortMem m = sort_mem(hist);
@@ -136,7 +138,6 @@ __global__ __launch_bounds__(kOsThreads) void k_digit_bases(uint32_t *__restrict
         m.totals[p * kRadix + t] = 0u;
     }
     if (t < (uint32_t)kOsPasses) m.tickets[t] = 0u;
-    if (t == 0) m.gen[0] = 0x80000000u | ((m.gen[0] + 1u) & 0x7FFFFFFFu);
 }
 
 // One pass: ticketed tile, ballot ranks, LDS sort of the tile by digit,
@@ -164,7 +165,6 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(const uint64_t *__restr
     for (uint32_t e = t; e < kOsWaves * kRadix; e += kOsThreads) (&wave_cnt[0][0])[e] = 0;
     __syncthreads();
     const uint32_t tile = tile_s;
-    const uint64_t gen = (uint64_t)m.gen[0] << 32;
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t wbase = (uint64_t)tile * kOsTile + (uint64_t)w * 64 * kOsItems;
@@ -205,9 +205,9 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(const uint64_t *__restr
         wave_cnt[k][t] = count;
         count += c;
     }
-    uint64_t *mine = m.status + ((uint64_t)pass * nb + tile) * kRadix + t;
+    uint32_t *mine = m.status + ((uint64_t)pass * nb + tile) * kRadix + t;
     const bool live = t <= dmask;
-    if (live && tile > 0) st_status(mine, gen | kFlagAgg | count);
+    if (live && tile > 0) st_status(mine, kFlagAgg | count);
     // the tile sorted by digit in LDS (while the predecessors publish)
     dstart[t] = wg_excl_scan(count, sh);
     __syncthreads();
@@ -221,22 +221,22 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(const uint64_t *__restr
         }
     }
     // look back kLookWindow tiles per round trip; a tile not yet published
-    // (or published by an earlier sort) ends the round and is read again
+    // (status still cleared) ends the round and is read again
     uint32_t excl = 0;
     if (live && tile > 0) {
         int64_t j = (int64_t)tile - 1;
         uint32_t spins = 0;
-        const uint64_t *col = m.status + (uint64_t)pass * nb * kRadix + t;
+        const uint32_t *col = m.status + (uint64_t)pass * nb * kRadix + t;
         while (j >= 0) {
-            uint64_t v[kLookWindow];
+            uint32_t v[kLookWindow];
 #pragma unroll
             for (int k = 0; k < kLookWindow; ++k)
-                v[k] = (j - k >= 0) ? ld_status(col + (uint64_t)(j - k) * kRadix) : (gen | kFlagInc);
+                v[k] = (j - k >= 0) ? ld_status(col + (uint64_t)(j - k) * kRadix) : kFlagInc;
             int k = 0;
             bool inc = false;
             for (; k < kLookWindow && j - k >= 0; ++k) {
-                const uint32_t lo = (uint32_t)v[k];
-                const uint32_t f = (v[k] & 0xFFFFFFFF00000000ull) == gen ? (lo & ~kValMask) : 0u;
+                const uint32_t lo = v[k];
+                const uint32_t f = lo & ~kValMask;
                 if (f == 0u) break;
                 excl += lo & kValMask;
                 if (f == kFlagInc) {
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kOsThreads) void k_onesweep(const uint64_t *__restr
             }
         }
     }
-    if (live) st_status(mine, gen | kFlagInc | (excl + count));
+    if (live) st_status(mine, kFlagInc | (excl + count));
     blk_off[t] = live ? m.bases[pass * kRadix + t] + excl : 0u;
     __syncthreads();
     const uint64_t t0 = (uint64_t)tile * kOsTile;
@@ -290,7 +290,7 @@ void onesweep(hipStream_t stream, uint32_t *hist, uint64_t *&keys, uint64_t *&ke
     const int last_bits = hi_bit - (lo_bit + kDigitBits * (npass - 1));
     const SortMem m = sort_mem(hist);
     hipLaunchKernelGGL(k_digit_totals, dim3(nb), dim3(kOsThreads), 0, stream, keys, n, lo_bit, npass,
-                       (1u << last_bits) - 1u, m.totals);
+                       (1u << last_bits) - 1u, m.totals, m.status, nb);
     hipLaunchKernelGGL(k_digit_bases, dim3(1), dim3(kOsThreads), 0, stream, hist, npass);
     for (int p = 0; p < npass; ++p) {
         const int shift = lo_bit + kDigitBits * p;
@@ -336,7 +336,7 @@ void radix_sort_kv(hipStream_t stream, uint32_t *hist, uint64_t *&keys, uint64_t
 // must start at zero)
 uint32_t radix_hist_entries(uint32_t cap) {
     const uint64_t nb = blocks_for(cap, kOsTile);
-    return (uint32_t)(kSortHeader + 2ull * kOsPasses * nb * kRadix);
+    return (uint32_t)(kSortHeader + (uint64_t)kOsPasses * nb * kRadix);
 }
 
 }  // namespace ge
