@@ -493,11 +493,16 @@ __device__ __forceinline__ bool verify_guess3(const float *__restrict__ x, const
         if (sidx == 0u) {
             lo = B[0]; hi = B[8]; g = B[4]; v = z; mid = midz;
         } else if (sidx <= 2u) {
-            const uint32_t h = sidx - 1u;
-            lo = B[4 * h]; hi = B[4 * h + 4]; g = B[4 * h + 2]; v = y; mid = midy;
+            // selects, not B[4 h]: an index that varies across lanes would
+            // move B to scratch memory for the whole kernel
+            const bool h = sidx == 2u;
+            lo = h ? B[4] : B[0]; hi = h ? B[8] : B[4]; g = h ? B[6] : B[2]; v = y; mid = midy;
         } else {
             const uint32_t h = sidx - 3u;
-            lo = B[2 * h]; hi = B[2 * h + 2]; g = B[2 * h + 1]; v = x; mid = midx;
+            lo = h == 0u ? B[0] : (h == 1u ? B[2] : (h == 2u ? B[4] : B[6]));
+            hi = h == 0u ? B[2] : (h == 1u ? B[4] : (h == 2u ? B[6] : B[8]));
+            g = h == 0u ? B[1] : (h == 1u ? B[3] : (h == 2u ? B[5] : B[7]));
+            v = x; mid = midx;
         }
         uint32_t pos;
         if (!guessed_probe3(lo, hi, g, t, pos)) continue;
