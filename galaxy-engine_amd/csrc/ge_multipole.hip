@@ -34,7 +34,9 @@
 //   2: (9 T_x, 9 T_y, -45 W, 180 U_xy)
 //   3: -45 A3: (xxx, xxy, xyy, yyy)
 //   4: (180 U_xx, 180 U_yy, -420 A4xxxx, -420 A4xxxy)
-//   5: -420 A4: (xxyy, xyyy, yyyy), 0
+//   5: -420 (3 A4xxyy, A4xyyy, A4yyyy, 3 A4xxxy)
+// (the factors 3 are g4's binomial weights, stored where a term only
+// appears with them)
 // The walk evaluates the Taylor expansion of the softened kernel
 // (|d|^2 + eps^2)^-1/2 to the order asked for from these; the quadrupole
 // reads records 0-1, the octupole 0-3 (without W and U).
@@ -241,7 +243,7 @@ __device__ void finish_record(uint32_t k, const NodeRec &r, const float4 g, cons
                        (float)(7.5 * uxy));
     o[3] = make_float4((float)(-7.5 * c3xxx), (float)(-7.5 * c3xxy), (float)(-7.5 * c3xyy), (float)(-7.5 * c3yyy));
     o[4] = make_float4((float)(7.5 * uxx), (float)(7.5 * uyy), (float)(-17.5 * c4xxxx), (float)(-17.5 * c4xxxy));
-    o[5] = make_float4((float)(-17.5 * c4xxyy), (float)(-17.5 * c4xyyy), (float)(-17.5 * c4yyyy), 0.0f);
+    o[5] = make_float4((float)(-52.5 * c4xxyy), (float)(-17.5 * c4xyyy), (float)(-17.5 * c4yyyy), (float)(-52.5 * c4xxxy));
 }
 
 __global__ void k_mp_reset(uint32_t *__restrict__ cnt) {

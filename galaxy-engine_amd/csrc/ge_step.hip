@@ -551,11 +551,11 @@ __device__ __forceinline__ void mp_accel(float4 h0, float4 h1, float4 h2, float4
             const float ux = fmaf(h4.x, qx, h2.w * qy), uy = fmaf(h2.w, qx, h4.y * qy);
             const float g3x = fmaf(h3.x, xx, fmaf(h3.y, xy2, fmaf(h3.z, yy, ux)));
             const float g3y = fmaf(h3.y, xx, fmaf(h3.z, xy2, fmaf(h3.w, yy, uy)));
-            const float t3 = fmaf(qx, g3x, fmaf(qy, g3y, 0.5f * fmaf(qx, ux, qy * uy)));  // (3/7) F3
-            // g4
-            const float x3 = xx * qx, y3 = yy * qy, a3 = 3.0f * (xx * qy), b3 = 3.0f * (qx * yy);
-            const float g4x = fmaf(h4.z, x3, fmaf(h4.w, a3, fmaf(h5.x, b3, h5.y * y3)));
-            const float g4y = fmaf(h4.w, x3, fmaf(h5.x, a3, fmaf(h5.y, b3, h5.z * y3)));
+            const float t3 = fmaf(qx, fmaf(0.5f, ux, g3x), qy * fmaf(0.5f, uy, g3y));  // (3/7) F3
+            // g4 (h5.x and h5.w carry the binomial weight 3; h5.y's is applied here)
+            const float x3 = xx * qx, y3 = yy * qy, a = xx * qy, b = qx * yy, b3 = 3.0f * b;
+            const float g4x = fmaf(h4.z, x3, fmaf(h5.w, a, fmaf(h5.x, b, h5.y * y3)));
+            const float g4y = fmaf(h4.w, x3, fmaf(h5.x, a, fmaf(h5.y, b3, h5.z * y3)));
             const float dg4 = fmaf(qx, g4x, qy * g4y);                                    // (4/9) F4
             const float in3 = fmaf(k * (27.0f / 28.0f), dg4, t3);                          // (3/7) (F3 + k F4)
             const float in2 = fmaf(k * (7.0f / 3.0f), in3, f2);                            // F2 + k (F3 + k F4)
